@@ -1,0 +1,309 @@
+/*
+ * marmel_hip.h - C ABI of libmarmel_hip, the MI355X (gfx950) path-tracing
+ * library that replaces the CPU renderers of marmel-trace.
+ *
+ * Everything here is plain C: fixed-width integers, doubles, floats and
+ * pointers. No HIP or torch type crosses the boundary. The caller owns every
+ * host buffer it passes in; the library owns all device memory.
+ *
+ * Scene input (mh_scene_soa) is the FLATTENED scene of the C11 host:
+ *   - primitives in file order (the index into `prims` is the prim_index that
+ *     breaks nearest-hit ties, as Geometry.prims does in
+ *     reference src/geometry.h:68-72 / src/bvh.c:641-646),
+ *   - the material table (reference src/material.h Material),
+ *   - the sky (reference src/material.h SkyParams),
+ *   - water metadata (reference src/scene.h:81-106),
+ *   - the host-built binary BVH over the non-plane primitives (node array +
+ *     leaf-order primitive index array) and the plane side list, as
+ *     reference src/bvh.c:66-76 keeps them.
+ * mh_scene_create() converts that into the device layout (fp32 child-box
+ * nodes, leaf-ordered fp32 and fp64 primitive records) and uploads it once.
+ *
+ * Return codes keep the reference meanings (reference src/pathtrace.h:69-71):
+ *   0 OK, 1 NULL pointer, 2 non-positive image dimensions,
+ *   3 spp < 1 or depth < 0.
+ * Library-specific failures use:
+ *   4  scene uses a feature outside the GPU path (text in mh_last_error),
+ *   5  invalid scene data (bad index, malformed BVH),
+ *   100+ HIP runtime error (text in mh_last_error).
+ *
+ * Threading: calls on one mh_scene are synchronous on that scene's own HIP
+ * stream and must not be issued from two threads at once; distinct scenes
+ * may be used concurrently. mh_last_error / mh_last_seconds are per thread.
+ */
+#ifndef MARMEL_HIP_H
+#define MARMEL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_ABI_VERSION 1
+
+/* Opaque device-resident scene handle. */
+struct mh_scene;
+typedef struct mh_scene mh_scene;
+
+/* ------------------------------------------------------------------ */
+/* Scene input types                                                   */
+/* ------------------------------------------------------------------ */
+
+typedef struct mh_vec3 {
+    double x, y, z;
+} mh_vec3;
+
+/* Primitive kinds (reference src/geometry.h:32-40; CSG and SDF are not on
+ * the GPU path and are rejected with code 4). */
+enum {
+    MH_PRIM_SPHERE   = 0,
+    MH_PRIM_PLANE    = 1,
+    MH_PRIM_BOX      = 2,
+    MH_PRIM_TRIANGLE = 3,
+    MH_PRIM_CYLINDER = 4
+};
+
+/* mh_prim.flags */
+enum {
+    MH_PRIM_SMOOTH = 1, /* triangle: interpolate attrs[i].n0..n2 (Primitive.radius > 0.5) */
+    MH_PRIM_UV     = 2  /* triangle: interpolate attrs[i].uv0..uv2 (Primitive.has_uv)     */
+};
+
+/*
+ * One primitive (reference Primitive, src/geometry.h:42-66):
+ *   SPHERE    p0 = center, r0 = radius
+ *   PLANE     p0 = point on the plane, p1 = unit normal
+ *   BOX       p0 = center, p1 = half extents
+ *   TRIANGLE  p0, p1, p2 = vertices a, b, c
+ *   CYLINDER  p0 = base centre, p1 = top centre, r0 = bottom radius, r1 = top radius
+ */
+typedef struct mh_prim {
+    int32_t kind;
+    int32_t material;
+    int32_t flags;
+    int32_t reserved;
+    mh_vec3 p0, p1, p2;
+    double  r0, r1;
+} mh_prim;
+
+/* Optional per-primitive vertex attributes (smooth normals, uvs). */
+typedef struct mh_prim_attr {
+    mh_vec3 n0, n1, n2;
+    mh_vec3 uv0, uv1, uv2;
+} mh_prim_attr;
+
+/* Texture kinds on the GPU path (reference src/material.h TextureKind). */
+enum {
+    MH_TEXTURE_NONE       = 0,
+    MH_TEXTURE_CHECKER    = 1,
+    MH_TEXTURE_UV_CHECKER = 7
+};
+
+/* Surface material (reference Material, src/material.h). */
+typedef struct mh_material {
+    mh_vec3 albedo;
+    mh_vec3 specular;
+    double  shininess;
+    double  reflectivity;
+    double  transparency;
+    double  ior;
+    int32_t is_water;
+    int32_t beer_lambert;
+    int32_t pbr;
+    int32_t texture_kind;
+    mh_vec3 absorption;
+    mh_vec3 deep_color;
+    double  metallic;
+    double  roughness;
+    mh_vec3 emissive;
+    double  texture_scale;
+    mh_vec3 texture_color_a;
+    mh_vec3 texture_color_b;
+    mh_vec3 atmosphere_glow;
+    double  bump_strength;   /* must be <= 1e-6 on the GPU path */
+} mh_material;
+
+/* Sky and sun (reference SkyParams, src/material.h). The opt-in volumetric
+ * clouds, stars, nebula, galaxy and fog must be off (0) on the GPU path. */
+typedef struct mh_sky {
+    mh_vec3  sun_dir;
+    mh_vec3  sun_color;
+    mh_vec3  horizon_color;
+    mh_vec3  zenith_color;
+    double   gradient_gamma;
+    double   sun_glow_exponent;
+    double   sun_glow_strength;
+    double   cloud_height;
+    double   cloud_scale;
+    double   cloud_coverage;
+    double   cloud_softness;
+    double   cloud_sharpness;
+    int32_t  cloud_octaves;
+    uint32_t seed;
+    double   sun_radius;        /* degrees; 0 = point sun */
+    double   cloud_thickness;   /* must be 0 */
+    double   star_intensity;    /* must be 0 */
+    double   nebula_intensity;  /* must be 0 */
+    double   galaxy_intensity;  /* must be 0 */
+    double   fog_density;       /* must be 0 */
+} mh_sky;
+
+/* Pinhole / thin-lens camera (reference Camera, src/camera.h:17-48). */
+typedef struct mh_camera {
+    mh_vec3 position;
+    mh_vec3 lower_left;
+    mh_vec3 horizontal;
+    mh_vec3 vertical;
+    double  aperture;
+    double  focus_distance;
+    double  dome_radius;
+    double  anamorphic_squeeze;
+} mh_camera;
+
+/* Host BVH node (binary; reference BvhNode src/bvh.c:50-57 with fp32 boxes
+ * rounded OUTWARD from the double bounds). Leaf: count > 0, primitives
+ * leaf_prims[first .. first+count). Interior: count == 0, children left/right. */
+typedef struct mh_bvh_node {
+    float   bmin[3];
+    float   bmax[3];
+    int32_t left;
+    int32_t right;
+    int32_t first;
+    int32_t count;
+} mh_bvh_node;
+
+typedef struct mh_scene_soa {
+    int32_t             abi_version;     /* MH_ABI_VERSION */
+    int32_t             n_prims;
+    const mh_prim      *prims;           /* file order                       */
+    const mh_prim_attr *attrs;           /* NULL, or n_prims entries         */
+    int32_t             n_materials;
+    const mh_material  *materials;
+    mh_sky              sky;
+    double              water_level;
+    int32_t             water_material;  /* -1 = none                        */
+    int32_t             n_nodes;         /* 0 when every primitive is a plane */
+    const mh_bvh_node  *nodes;           /* node 0 is the root               */
+    int32_t             n_leaf_prims;
+    const int32_t      *leaf_prims;      /* prim indices in leaf order       */
+    int32_t             n_planes;
+    const int32_t      *planes;          /* plane prim indices, file order   */
+} mh_scene_soa;
+
+/* ------------------------------------------------------------------ */
+/* Ray query types                                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct mh_ray {
+    double ox, oy, oz;
+    double dx, dy, dz;
+} mh_ray;
+
+/* Nearest hit (reference Hit, src/geometry.h:74-83). prim_index -1 = miss. */
+typedef struct mh_hit {
+    double  t;
+    int32_t prim_index;
+    int32_t material;
+    int32_t front_face;
+    int32_t reserved;
+    double  nx, ny, nz;
+    double  u, v;
+} mh_hit;
+
+/* ------------------------------------------------------------------ */
+/* Render parameters                                                   */
+/* ------------------------------------------------------------------ */
+
+enum {
+    MH_MODE_PATHTRACE = 0,  /* reference pathtrace_render      */
+    MH_MODE_WHITTED   = 1   /* reference render_image (fixed)  */
+};
+
+enum {
+    MH_PREC_FP32 = 0,       /* fp32 geometry and shading (throughput path) */
+    MH_PREC_FP64 = 1        /* fp64 throughout (parity path)               */
+};
+
+typedef struct mh_render_params {
+    int32_t width;
+    int32_t height;
+    int32_t spp;
+    int32_t max_depth;
+    int32_t mode;           /* MH_MODE_*  */
+    int32_t precision;      /* MH_PREC_*  */
+    int32_t sample_offset;  /* first sample index (progressive chunks) */
+    int32_t block_size;     /* 0 = library default; otherwise a multiple of 64 */
+} mh_render_params;
+
+/* Per-call timing of the last render on the calling thread, measured with
+ * HIP events recorded on the scene's stream around each kernel. */
+typedef struct mh_timing {
+    double total_ms;        /* first event to last event               */
+    double kernel_ms;       /* the render kernel(s) only               */
+    double tonemap_ms;      /* mean -> 8-bit conversion                */
+    int64_t samples;        /* width * height * spp                    */
+    int64_t rays;           /* extension + shadow queries (counted)    */
+} mh_timing;
+
+/* ------------------------------------------------------------------ */
+/* Entry points                                                        */
+/* ------------------------------------------------------------------ */
+
+/* Library / device info. Returns 0 and fills `name` (NUL-terminated). */
+int mh_device_info(char *name, size_t name_len, int *cu_count, size_t *hbm_bytes);
+
+/* Upload a flattened scene (one host->device copy of every array).
+ * Replaces the scene handed to reference pathtrace_render (src/pathtrace.h:78). */
+int mh_scene_create(const mh_scene_soa *soa, struct mh_scene **out);
+void mh_scene_destroy(struct mh_scene *s);
+
+/* Device bytes held by the scene (nodes + primitive records + tables). */
+size_t mh_scene_device_bytes(const struct mh_scene *s);
+
+/* Nearest-hit queries for a ray batch in [tmin, tmax]
+ * (reference scene_intersect, src/scene.h:162 / src/scene.c:767-775). */
+int mh_trace_closest(struct mh_scene *s, const mh_ray *rays, int n,
+                     double tmin, double tmax, int precision, mh_hit *out);
+
+/* Any-hit occlusion for a ray batch in [tmin, tmax]
+ * (reference scene_occluded, src/scene.h:169 / src/bvh.c:714-813). */
+int mh_trace_any(struct mh_scene *s, const mh_ray *rays, int n,
+                 double tmin, double tmax, int precision, uint8_t *occluded);
+
+/* Drop-in for reference pathtrace_render (src/pathtrace.h:78-80): path-trace
+ * a width x height image with `samples_per_pixel` and `max_depth` into the
+ * TOP-DOWN RGB8 buffer rgb_out[(y*width+x)*3+c]. fp32 precision. */
+int mh_pathtrace_render(struct mh_scene *s, const mh_camera *cam, int width,
+                        int height, int samples_per_pixel, int max_depth,
+                        unsigned char *rgb_out);
+
+/* Drop-in for reference render_image (src/render.h:29-31; render_image_ex
+ * with adaptive off, src/render.c:1627-1651): fixed-spp Whitted render. */
+int mh_render_image(struct mh_scene *s, const mh_camera *cam, int width,
+                    int height, int samples_per_pixel, int max_depth,
+                    unsigned char *rgb_out);
+
+/* General render. rgb_out (w*h*3 bytes) and linear_out (w*h*3 floats, the
+ * per-pixel mean radiance) are host buffers and may each be NULL: the result
+ * then stays in the scene's device framebuffer (fetch it with
+ * mh_framebuffer_download). */
+int mh_render(struct mh_scene *s, const mh_camera *cam,
+              const mh_render_params *p, unsigned char *rgb_out,
+              float *linear_out);
+
+/* Copy the device framebuffer of the last mh_render on `s` to the host. */
+int mh_framebuffer_download(struct mh_scene *s, unsigned char *rgb_out,
+                            float *linear_out);
+
+/* Timing and error state of the last call on the calling thread. */
+int mh_last_timing(mh_timing *out);
+double mh_last_seconds(void);   /* like reference render_last_seconds() */
+const char *mh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MARMEL_HIP_H */

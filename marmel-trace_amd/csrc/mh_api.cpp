@@ -1,0 +1,677 @@
+// mh_api.cpp - the C ABI of libmarmel_hip (declared in include/marmel_hip.h).
+//
+// mh_scene_create converts the host's flattened scene into the device layout
+// (mh_scene_dev.h) and uploads it once; the render / query entry points launch
+// the kernels on the scene's own stream, time them with HIP events recorded
+// on that stream, and copy results back only when the caller asks.
+#include "marmel_hip.h"
+#include "mh_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace mh;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local mh_timing g_timing;
+thread_local double g_seconds = 0.0;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return fail(100 + (int)e_, "%s failed: %s", #expr, hipGetErrorString(e_));               \
+    } while (0)
+
+template <typename R> V3<R> cv(const mh_vec3& v) { return V3<R>{(R)v.x, (R)v.y, (R)v.z}; }
+
+template <typename R> DMat<R> conv_mat(const mh_material& a)
+{
+    DMat<R> m;
+    memset(&m, 0, sizeof m);
+    m.albedo = cv<R>(a.albedo);
+    m.specular = cv<R>(a.specular);
+    m.shininess = (R)a.shininess;
+    m.reflectivity = (R)a.reflectivity;
+    m.transparency = (R)a.transparency;
+    m.ior = (R)a.ior;
+    m.is_water = a.is_water;
+    m.beer_lambert = a.beer_lambert;
+    m.pbr = a.pbr;
+    m.texture_kind = a.texture_kind;
+    m.absorption = cv<R>(a.absorption);
+    m.deep_color = cv<R>(a.deep_color);
+    m.metallic = (R)a.metallic;
+    m.roughness = (R)a.roughness;
+    m.emissive = cv<R>(a.emissive);
+    m.texture_scale = (R)a.texture_scale;
+    m.tex_a = cv<R>(a.texture_color_a);
+    m.tex_b = cv<R>(a.texture_color_b);
+    m.glow = cv<R>(a.atmosphere_glow);
+    m.has_emission = (a.emissive.x != 0.0 || a.emissive.y != 0.0 || a.emissive.z != 0.0) ? 1 : 0;
+    double g2 = a.atmosphere_glow.x * a.atmosphere_glow.x + a.atmosphere_glow.y * a.atmosphere_glow.y +
+                a.atmosphere_glow.z * a.atmosphere_glow.z;
+    m.has_glow = g2 > 1e-6 ? 1 : 0;
+    return m;
+}
+
+template <typename R> DSky<R> conv_sky(const mh_sky& k)
+{
+    DSky<R> s;
+    memset(&s, 0, sizeof s);
+    s.sun_dir = cv<R>(k.sun_dir);
+    s.sun_color = cv<R>(k.sun_color);
+    s.horizon = cv<R>(k.horizon_color);
+    s.zenith = cv<R>(k.zenith_color);
+    s.gamma = (R)k.gradient_gamma;
+    s.glow_exp = (R)k.sun_glow_exponent;
+    s.glow_strength = (R)k.sun_glow_strength;
+    s.cloud_height = (R)k.cloud_height;
+    s.cloud_scale = (R)k.cloud_scale;
+    s.coverage = (R)k.cloud_coverage;
+    s.softness = (R)k.cloud_softness;
+    s.sharpness = (R)k.cloud_sharpness;
+    s.octaves = k.cloud_octaves < 1 ? 1 : k.cloud_octaves;
+    s.seed = k.seed;
+    s.sun_radius = (R)k.sun_radius;
+    // solid angle of the sun disk as the reference NEE uses it (1 for a point sun)
+    double rad = k.sun_radius * 3.14159265358979323846 / 180.0;
+    s.sun_omega = (R)(k.sun_radius > 0.0 ? 2.0 * 3.14159265358979323846 * (1.0 - cos(rad)) : 1.0);
+    return s;
+}
+
+P32 pack32(const mh_prim& q)
+{
+    P32 r;
+    memset(&r, 0, sizeof r);
+    float kf;
+    int kv = q.kind | (q.flags << 8);
+    memcpy(&kf, &kv, 4);
+    float pf, mf;
+    memcpy(&mf, &q.material, 4);
+    r.a = make_float4((float)q.p0.x, (float)q.p0.y, (float)q.p0.z, kf);
+    r.c.w = mf;
+    switch (q.kind) {
+    case MH_PRIM_TRIANGLE:
+        // edges from the double vertices, then rounded
+        r.b = make_float4((float)(q.p1.x - q.p0.x), (float)(q.p1.y - q.p0.y), (float)(q.p1.z - q.p0.z), 0.f);
+        r.c = make_float4((float)(q.p2.x - q.p0.x), (float)(q.p2.y - q.p0.y), (float)(q.p2.z - q.p0.z), mf);
+        break;
+    case MH_PRIM_SPHERE:
+        r.b = make_float4((float)q.r0, 0.f, 0.f, 0.f);
+        break;
+    case MH_PRIM_CYLINDER:
+        r.b = make_float4((float)q.p1.x, (float)q.p1.y, (float)q.p1.z, 0.f);
+        r.c = make_float4((float)q.r0, (float)q.r1, 0.f, mf);
+        break;
+    default: // box, plane
+        r.b = make_float4((float)q.p1.x, (float)q.p1.y, (float)q.p1.z, 0.f);
+        break;
+    }
+    (void)pf;
+    return r;
+}
+
+void set_prim_index32(P32& r, int idx) { memcpy(&r.b.w, &idx, 4); }
+
+P64 pack64(const mh_prim& q, int idx)
+{
+    P64 r;
+    r.p0[0] = q.p0.x; r.p0[1] = q.p0.y; r.p0[2] = q.p0.z;
+    r.p1[0] = q.p1.x; r.p1[1] = q.p1.y; r.p1[2] = q.p1.z;
+    r.p2[0] = q.p2.x; r.p2[1] = q.p2.y; r.p2[2] = q.p2.z;
+    r.r0 = q.r0;
+    r.r1 = q.r1;
+    r.kind = q.kind;
+    r.prim = idx;
+    r.material = q.material;
+    r.flags = q.flags;
+    return r;
+}
+
+template <typename R> DAttr<R> conv_attr(const mh_prim_attr& a)
+{
+    DAttr<R> d;
+    d.n0 = cv<R>(a.n0); d.n1 = cv<R>(a.n1); d.n2 = cv<R>(a.n2);
+    d.uv0 = cv<R>(a.uv0); d.uv1 = cv<R>(a.uv1); d.uv2 = cv<R>(a.uv2);
+    return d;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+} // namespace
+
+struct mh_scene {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<DevBuf> bufs;
+    size_t device_bytes = 0;
+    SceneView<float> v32;
+    SceneView<double> v64;
+    // framebuffer / scratch (grown on demand)
+    float* fb_linear = nullptr;
+    unsigned char* fb_rgb = nullptr;
+    size_t fb_pixels = 0;
+    int fb_w = 0, fb_h = 0;
+    unsigned long long* d_rays = nullptr;
+    int* d_overflow = nullptr;
+    void* wstack = nullptr;
+    size_t wstack_bytes = 0;
+    void* qbuf = nullptr;
+    size_t qbuf_bytes = 0;
+};
+
+namespace {
+
+int upload(mh_scene* s, const void* src, size_t bytes, void** dst)
+{
+    *dst = nullptr;
+    if (bytes == 0) return 0;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, bytes));
+    s->bufs.push_back({p, bytes});
+    s->device_bytes += bytes;
+    HIP_TRY(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s->stream));
+    *dst = p;
+    return 0;
+}
+
+int validate(const mh_scene_soa* soa)
+{
+    if (soa->abi_version != MH_ABI_VERSION) return fail(5, "abi_version %d != %d", soa->abi_version, MH_ABI_VERSION);
+    if (soa->n_prims < 0 || soa->n_materials < 0 || soa->n_nodes < 0 || soa->n_leaf_prims < 0 || soa->n_planes < 0)
+        return fail(5, "negative count in scene");
+    if ((soa->n_prims && !soa->prims) || (soa->n_materials && !soa->materials) || (soa->n_nodes && !soa->nodes) ||
+        (soa->n_leaf_prims && !soa->leaf_prims) || (soa->n_planes && !soa->planes))
+        return fail(1, "NULL array with non-zero count");
+    const mh_sky& k = soa->sky;
+    if (k.cloud_thickness > 1e-4) return fail(4, "volumetric clouds (cloud_thickness %g) are outside the GPU path", k.cloud_thickness);
+    if (k.star_intensity > 0.0 || k.nebula_intensity > 0.0 || k.galaxy_intensity > 0.0)
+        return fail(4, "stars / nebula / galaxy sky layers are outside the GPU path");
+    if (k.fog_density > 0.0) return fail(4, "fog is outside the GPU path");
+    for (int i = 0; i < soa->n_materials; ++i) {
+        const mh_material& m = soa->materials[i];
+        if (m.bump_strength > 1e-6) return fail(4, "material %d: bump mapping is outside the GPU path", i);
+        if (m.texture_kind != MH_TEXTURE_NONE && m.texture_kind != MH_TEXTURE_CHECKER &&
+            m.texture_kind != MH_TEXTURE_UV_CHECKER)
+            return fail(4, "material %d: texture kind %d is outside the GPU path", i, m.texture_kind);
+    }
+    int n_planes = 0;
+    for (int i = 0; i < soa->n_prims; ++i) {
+        const mh_prim& p = soa->prims[i];
+        if (p.kind < MH_PRIM_SPHERE || p.kind > MH_PRIM_CYLINDER)
+            return fail(4, "prim %d: kind %d is outside the GPU path", i, p.kind);
+        if (p.kind == MH_PRIM_PLANE) ++n_planes;
+        if ((p.flags & ~(MH_PRIM_SMOOTH | MH_PRIM_UV)) != 0) return fail(5, "prim %d: bad flags", i);
+    }
+    if (n_planes != soa->n_planes) return fail(5, "plane list has %d entries, scene has %d planes", soa->n_planes, n_planes);
+    for (int i = 0; i < soa->n_planes; ++i) {
+        int pi = soa->planes[i];
+        if (pi < 0 || pi >= soa->n_prims || soa->prims[pi].kind != MH_PRIM_PLANE)
+            return fail(5, "plane list entry %d is not a plane", i);
+    }
+    if (soa->n_leaf_prims != soa->n_prims - n_planes)
+        return fail(5, "leaf list has %d entries, expected %d", soa->n_leaf_prims, soa->n_prims - n_planes);
+    if (soa->n_leaf_prims > (1 << 27)) return fail(5, "too many primitives (%d)", soa->n_leaf_prims);
+    if ((soa->n_leaf_prims > 0) != (soa->n_nodes > 0)) return fail(5, "BVH / primitive count mismatch");
+    for (int i = 0; i < soa->n_leaf_prims; ++i) {
+        int pi = soa->leaf_prims[i];
+        if (pi < 0 || pi >= soa->n_prims || soa->prims[pi].kind == MH_PRIM_PLANE)
+            return fail(5, "leaf list entry %d is invalid", i);
+    }
+    return 0;
+}
+
+// Walk the host tree from the root: checks child indices, leaf ranges, that
+// every leaf slot is covered once, and the depth bound of the GPU stack.
+int check_tree(const mh_scene_soa* soa, std::vector<int>* order)
+{
+    if (soa->n_nodes == 0) return 0;
+    std::vector<char> seen(soa->n_nodes, 0);
+    std::vector<char> slot(soa->n_leaf_prims, 0);
+    std::vector<std::pair<int, int>> st;
+    st.push_back({0, 1});
+    int covered = 0;
+    while (!st.empty()) {
+        auto [n, depth] = st.back();
+        st.pop_back();
+        if (n < 0 || n >= soa->n_nodes) return fail(5, "BVH child index %d out of range", n);
+        if (seen[n]) return fail(5, "BVH node %d reached twice", n);
+        seen[n] = 1;
+        if (depth > 60) return fail(5, "BVH deeper than 60 levels");
+        const mh_bvh_node& nd = soa->nodes[n];
+        if (nd.count > 0) {
+            if (nd.count > 8) return fail(5, "BVH leaf %d holds %d > 8 primitives", n, nd.count);
+            if (nd.first < 0 || nd.first + nd.count > soa->n_leaf_prims) return fail(5, "BVH leaf %d range invalid", n);
+            for (int i = 0; i < nd.count; ++i) {
+                if (slot[nd.first + i]) return fail(5, "leaf slot %d referenced twice", nd.first + i);
+                slot[nd.first + i] = 1;
+            }
+            covered += nd.count;
+        } else {
+            order->push_back(n);
+            st.push_back({nd.right, depth + 1});
+            st.push_back({nd.left, depth + 1});
+        }
+    }
+    if (covered != soa->n_leaf_prims) return fail(5, "BVH leaves cover %d of %d primitives", covered, soa->n_leaf_prims);
+    return 0;
+}
+
+template <typename R> CamView<R> conv_cam(const mh_camera& c)
+{
+    CamView<R> v;
+    v.pos = cv<R>(c.position);
+    v.ll = cv<R>(c.lower_left);
+    v.hor = cv<R>(c.horizontal);
+    v.ver = cv<R>(c.vertical);
+    v.aperture = (R)c.aperture;
+    v.focus = (R)c.focus_distance;
+    v.dome = (R)c.dome_radius;
+    v.squeeze = (R)c.anamorphic_squeeze;
+    // normalised viewport axes, as the thin-lens sampler forms them (in double)
+    auto nrm = [](mh_vec3 a) {
+        double l2 = a.x * a.x + a.y * a.y + a.z * a.z;
+        if (l2 <= 1e-12) return mh_vec3{0, 0, 0};
+        double k = 1.0 / sqrt(l2);
+        return mh_vec3{a.x * k, a.y * k, a.z * k};
+    };
+    v.right_n = cv<R>(nrm(c.horizontal));
+    v.up_n = cv<R>(nrm(c.vertical));
+    return v;
+}
+
+int ensure_fb(mh_scene* s, int w, int h)
+{
+    size_t px = (size_t)w * (size_t)h;
+    if (px > s->fb_pixels) {
+        if (s->fb_linear) (void)hipFree(s->fb_linear);
+        if (s->fb_rgb) (void)hipFree(s->fb_rgb);
+        s->fb_linear = nullptr;
+        s->fb_rgb = nullptr;
+        s->fb_pixels = 0;
+        HIP_TRY(hipMalloc(&s->fb_linear, px * 3 * sizeof(float)));
+        HIP_TRY(hipMalloc(&s->fb_rgb, px * 3));
+        s->fb_pixels = px;
+    }
+    s->fb_w = w;
+    s->fb_h = h;
+    return 0;
+}
+
+int ensure_query(mh_scene* s, size_t bytes)
+{
+    if (bytes > s->qbuf_bytes) {
+        if (s->qbuf) (void)hipFree(s->qbuf);
+        s->qbuf = nullptr;
+        s->qbuf_bytes = 0;
+        HIP_TRY(hipMalloc(&s->qbuf, bytes));
+        s->qbuf_bytes = bytes;
+    }
+    return 0;
+}
+
+float ms_between(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 0.f;
+    return ms;
+}
+
+} // namespace
+
+extern "C" {
+
+int mh_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes)
+{
+    hipDeviceProp_t p;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipGetDeviceProperties(&p, dev));
+    if (name && name_len) {
+        snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+    return 0;
+}
+
+int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
+{
+    if (!soa || !out) return fail(1, "NULL argument");
+    *out = nullptr;
+    int rc = validate(soa);
+    if (rc) return rc;
+    std::vector<int> inner;
+    rc = check_tree(soa, &inner);
+    if (rc) return rc;
+
+    mh_scene* s = new mh_scene();
+    auto bail = [&](int code) {
+        mh_scene_destroy(s);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(101, "hipStreamCreate failed"));
+    for (auto& e : s->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(101, "hipEventCreate failed"));
+
+    // device node numbering: interior nodes in DFS order
+    std::vector<int> dev_index(soa->n_nodes, -1);
+    for (size_t i = 0; i < inner.size(); ++i) dev_index[inner[i]] = (int)i;
+    auto ref_of = [&](int n) {
+        const mh_bvh_node& nd = soa->nodes[n];
+        if (nd.count > 0) return ~((nd.first << 3) | (nd.count - 1));
+        return dev_index[n];
+    };
+    std::vector<DNode> nodes(inner.size());
+    for (size_t i = 0; i < inner.size(); ++i) {
+        const mh_bvh_node& nd = soa->nodes[inner[i]];
+        const mh_bvh_node& L = soa->nodes[nd.left];
+        const mh_bvh_node& Rn = soa->nodes[nd.right];
+        DNode d;
+        d.c0 = make_float4(L.bmin[0], L.bmax[0], L.bmin[1], L.bmax[1]);
+        d.c1 = make_float4(L.bmin[2], L.bmax[2], Rn.bmin[0], Rn.bmax[0]);
+        d.c2 = make_float4(Rn.bmin[1], Rn.bmax[1], Rn.bmin[2], Rn.bmax[2]);
+        d.c3 = make_int4(ref_of(nd.left), ref_of(nd.right), 0, 0);
+        nodes[i] = d;
+    }
+    int root_ref = soa->n_nodes > 0 ? ref_of(0) : INT_MIN;
+
+    bool any_attr = false;
+    if (soa->attrs)
+        for (int i = 0; i < soa->n_prims && !any_attr; ++i) any_attr = soa->prims[i].flags != 0;
+
+    std::vector<P32> p32(soa->n_leaf_prims);
+    std::vector<P64> p64(soa->n_leaf_prims);
+    std::vector<DAttr<float>> a32;
+    std::vector<DAttr<double>> a64;
+    if (any_attr) {
+        a32.resize(soa->n_leaf_prims);
+        a64.resize(soa->n_leaf_prims);
+    }
+    for (int i = 0; i < soa->n_leaf_prims; ++i) {
+        int pi = soa->leaf_prims[i];
+        mh_prim q = soa->prims[pi];
+        if (!soa->attrs) q.flags = 0;
+        p32[i] = pack32(q);
+        set_prim_index32(p32[i], pi);
+        p64[i] = pack64(q, pi);
+        if (any_attr) {
+            a32[i] = conv_attr<float>(soa->attrs[pi]);
+            a64[i] = conv_attr<double>(soa->attrs[pi]);
+        }
+    }
+    std::vector<P32> pl32(soa->n_planes);
+    std::vector<P64> pl64(soa->n_planes);
+    for (int i = 0; i < soa->n_planes; ++i) {
+        int pi = soa->planes[i];
+        pl32[i] = pack32(soa->prims[pi]);
+        set_prim_index32(pl32[i], pi);
+        pl64[i] = pack64(soa->prims[pi], pi);
+    }
+    std::vector<DMat<float>> m32(soa->n_materials);
+    std::vector<DMat<double>> m64(soa->n_materials);
+    for (int i = 0; i < soa->n_materials; ++i) {
+        m32[i] = conv_mat<float>(soa->materials[i]);
+        m64[i] = conv_mat<double>(soa->materials[i]);
+    }
+
+    void *d_nodes, *d_p32, *d_p64, *d_pl32, *d_pl64, *d_plidx, *d_a32, *d_a64, *d_m32, *d_m64;
+    if ((rc = upload(s, nodes.data(), nodes.size() * sizeof(DNode), &d_nodes)) ||
+        (rc = upload(s, p32.data(), p32.size() * sizeof(P32), &d_p32)) ||
+        (rc = upload(s, p64.data(), p64.size() * sizeof(P64), &d_p64)) ||
+        (rc = upload(s, pl32.data(), pl32.size() * sizeof(P32), &d_pl32)) ||
+        (rc = upload(s, pl64.data(), pl64.size() * sizeof(P64), &d_pl64)) ||
+        (rc = upload(s, soa->planes, (size_t)soa->n_planes * sizeof(int), &d_plidx)) ||
+        (rc = upload(s, a32.data(), a32.size() * sizeof(DAttr<float>), &d_a32)) ||
+        (rc = upload(s, a64.data(), a64.size() * sizeof(DAttr<double>), &d_a64)) ||
+        (rc = upload(s, m32.data(), m32.size() * sizeof(DMat<float>), &d_m32)) ||
+        (rc = upload(s, m64.data(), m64.size() * sizeof(DMat<double>), &d_m64)))
+        return bail(rc);
+    if (hipMalloc(&s->d_rays, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&s->d_overflow, sizeof(int)) != hipSuccess)
+        return bail(fail(102, "hipMalloc failed for counters"));
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(103, "upload failed"));
+
+    // emissive area lights: first 8 PBR spheres with emission, file order
+    int nl = 0;
+    DLight<float> l32[8];
+    DLight<double> l64[8];
+    for (int i = 0; i < soa->n_prims && nl < 8; ++i) {
+        const mh_prim& p = soa->prims[i];
+        if (p.kind != MH_PRIM_SPHERE || p.material < 0 || p.material >= soa->n_materials) continue;
+        const mh_material& m = soa->materials[p.material];
+        if (!m.pbr) continue;
+        if (m.emissive.x == 0.0 && m.emissive.y == 0.0 && m.emissive.z == 0.0) continue;
+        if (!(p.r0 > 0.0)) continue;
+        l32[nl] = {i, cv<float>(p.p0), (float)p.r0, cv<float>(m.emissive)};
+        l64[nl] = {i, cv<double>(p.p0), p.r0, cv<double>(m.emissive)};
+        ++nl;
+    }
+
+    auto fill = [&](auto& v, const void* pr, const void* pl, const void* at, const void* mt, auto* lights, auto tag) {
+        typedef decltype(tag) R;
+        v.nodes = (const DNode*)d_nodes;
+        v.n_nodes = (int)nodes.size();
+        v.root_ref = root_ref;
+        v.prims = (const typename PrimT<R>::type*)pr;
+        v.planes = (const typename PrimT<R>::type*)pl;
+        v.n_planes = soa->n_planes;
+        v.plane_index = (const int*)d_plidx;
+        v.attrs = (const DAttr<R>*)at;
+        v.mats = (const DMat<R>*)mt;
+        v.n_mats = soa->n_materials;
+        v.sky = conv_sky<R>(soa->sky);
+        v.water_level = (R)soa->water_level;
+        v.water_material = soa->water_material;
+        v.n_lights = nl;
+        for (int i = 0; i < 8; ++i) v.lights[i] = lights[i];
+    };
+    fill(s->v32, d_p32, d_pl32, d_a32, d_m32, l32, 0.0f);
+    fill(s->v64, d_p64, d_pl64, d_a64, d_m64, l64, 0.0);
+    *out = s;
+    return 0;
+}
+
+void mh_scene_destroy(mh_scene* s)
+{
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto& b : s->bufs) (void)hipFree(b.p);
+    if (s->fb_linear) (void)hipFree(s->fb_linear);
+    if (s->fb_rgb) (void)hipFree(s->fb_rgb);
+    if (s->d_rays) (void)hipFree(s->d_rays);
+    if (s->d_overflow) (void)hipFree(s->d_overflow);
+    if (s->wstack) (void)hipFree(s->wstack);
+    if (s->qbuf) (void)hipFree(s->qbuf);
+    for (auto& e : s->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+size_t mh_scene_device_bytes(const mh_scene* s) { return s ? s->device_bytes : 0; }
+
+int mh_trace_closest(mh_scene* s, const mh_ray* rays, int n, double tmin, double tmax, int precision, mh_hit* out)
+{
+    if (!s || (!rays && n > 0) || (!out && n > 0)) return fail(1, "NULL argument");
+    if (n <= 0) return 0;
+    size_t rb = (size_t)n * sizeof(mh_ray), hb = (size_t)n * sizeof(mh_hit);
+    int rc = ensure_query(s, rb + hb);
+    if (rc) return rc;
+    mh_ray* d_r = (mh_ray*)s->qbuf;
+    mh_hit* d_h = (mh_hit*)((char*)s->qbuf + rb);
+    HIP_TRY(hipMemcpyAsync(d_r, rays, rb, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+    if (precision == MH_PREC_FP64) HIP_TRY(launch_closest<double>(s->v64, d_r, n, tmin, tmax, d_h, s->stream));
+    else HIP_TRY(launch_closest<float>(s->v32, d_r, n, tmin, tmax, d_h, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+    HIP_TRY(hipMemcpyAsync(out, d_h, hb, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    memset(&g_timing, 0, sizeof g_timing);
+    g_timing.kernel_ms = g_timing.total_ms = ms_between(s->ev[0], s->ev[1]);
+    g_timing.rays = n;
+    return 0;
+}
+
+int mh_trace_any(mh_scene* s, const mh_ray* rays, int n, double tmin, double tmax, int precision, uint8_t* out)
+{
+    if (!s || (!rays && n > 0) || (!out && n > 0)) return fail(1, "NULL argument");
+    if (n <= 0) return 0;
+    size_t rb = (size_t)n * sizeof(mh_ray);
+    int rc = ensure_query(s, rb + (size_t)n);
+    if (rc) return rc;
+    mh_ray* d_r = (mh_ray*)s->qbuf;
+    uint8_t* d_o = (uint8_t*)s->qbuf + rb;
+    HIP_TRY(hipMemcpyAsync(d_r, rays, rb, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+    if (precision == MH_PREC_FP64) HIP_TRY(launch_any<double>(s->v64, d_r, n, tmin, tmax, d_o, s->stream));
+    else HIP_TRY(launch_any<float>(s->v32, d_r, n, tmin, tmax, d_o, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+    HIP_TRY(hipMemcpyAsync(out, d_o, (size_t)n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    memset(&g_timing, 0, sizeof g_timing);
+    g_timing.kernel_ms = g_timing.total_ms = ms_between(s->ev[0], s->ev[1]);
+    g_timing.rays = n;
+    return 0;
+}
+
+int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsigned char* rgb_out, float* linear_out)
+{
+    if (!s || !cam || !p) return fail(1, "NULL argument");
+    if (p->width <= 0 || p->height <= 0) return fail(2, "non-positive image size %dx%d", p->width, p->height);
+    if (p->spp < 1 || p->max_depth < 0) return fail(3, "bad spp %d / depth %d", p->spp, p->max_depth);
+    if ((long long)p->width * p->height > (1LL << 28)) return fail(2, "image too large");
+    if (p->mode != MH_MODE_PATHTRACE && p->mode != MH_MODE_WHITTED) return fail(5, "unknown mode %d", p->mode);
+    if (p->sample_offset < 0) return fail(3, "negative sample offset");
+    int rc = ensure_fb(s, p->width, p->height);
+    if (rc) return rc;
+    LaunchOut lo{s->fb_linear, s->fb_rgb, s->d_rays, s->d_overflow};
+    HIP_TRY(hipMemsetAsync(s->d_rays, 0, sizeof(unsigned long long), s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_overflow, 0, sizeof(int), s->stream));
+    bool f64 = p->precision == MH_PREC_FP64;
+    HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+    if (p->mode == MH_MODE_PATHTRACE) {
+        if (f64)
+            HIP_TRY(launch_pathtrace<double>(s->v64, conv_cam<double>(*cam), p->width, p->height, p->spp,
+                                             p->sample_offset, p->max_depth, lo, p->block_size, s->stream));
+        else
+            HIP_TRY(launch_pathtrace<float>(s->v32, conv_cam<float>(*cam), p->width, p->height, p->spp,
+                                            p->sample_offset, p->max_depth, lo, p->block_size, s->stream));
+    } else {
+        // explicit Whitted work stack: 17 pending rays per level suffices
+        int cap = 17 * (p->max_depth + 2) + 4;
+        int threads = 256 * 1024;
+        size_t px = (size_t)p->width * p->height;
+        if ((size_t)threads > px) threads = (int)((px + 255) / 256 * 256);
+        size_t item = f64 ? whitted_item_bytes<double>() : whitted_item_bytes<float>();
+        size_t need = item * (size_t)cap * (size_t)threads;
+        if (need > s->wstack_bytes) {
+            if (s->wstack) (void)hipFree(s->wstack);
+            s->wstack = nullptr;
+            s->wstack_bytes = 0;
+            HIP_TRY(hipMalloc(&s->wstack, need));
+            s->wstack_bytes = need;
+        }
+        if (f64)
+            HIP_TRY(launch_whitted<double>(s->v64, conv_cam<double>(*cam), p->width, p->height, p->spp,
+                                           p->sample_offset, p->max_depth, lo, s->wstack, cap, threads, s->stream));
+        else
+            HIP_TRY(launch_whitted<float>(s->v32, conv_cam<float>(*cam), p->width, p->height, p->spp,
+                                          p->sample_offset, p->max_depth, lo, s->wstack, cap, threads, s->stream));
+    }
+    HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+    size_t n3 = (size_t)p->width * p->height * 3;
+    if (rgb_out) HIP_TRY(hipMemcpyAsync(rgb_out, s->fb_rgb, n3, hipMemcpyDeviceToHost, s->stream));
+    if (linear_out) HIP_TRY(hipMemcpyAsync(linear_out, s->fb_linear, n3 * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    unsigned long long rays = 0;
+    int ovf = 0;
+    HIP_TRY(hipMemcpyAsync(&rays, s->d_rays, sizeof rays, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(&ovf, s->d_overflow, sizeof ovf, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[2], s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    memset(&g_timing, 0, sizeof g_timing);
+    g_timing.kernel_ms = ms_between(s->ev[0], s->ev[1]);
+    g_timing.total_ms = ms_between(s->ev[0], s->ev[2]);
+    g_timing.samples = (int64_t)p->width * p->height * p->spp;
+    g_timing.rays = (int64_t)rays;
+    g_seconds = g_timing.kernel_ms * 1e-3;
+    if (ovf) return fail(6, "Whitted work stack overflow");
+    return 0;
+}
+
+int mh_framebuffer_download(mh_scene* s, unsigned char* rgb_out, float* linear_out)
+{
+    if (!s) return fail(1, "NULL argument");
+    if (!s->fb_rgb) return fail(5, "no framebuffer yet");
+    size_t n3 = (size_t)s->fb_w * s->fb_h * 3;
+    if (rgb_out) HIP_TRY(hipMemcpyAsync(rgb_out, s->fb_rgb, n3, hipMemcpyDeviceToHost, s->stream));
+    if (linear_out) HIP_TRY(hipMemcpyAsync(linear_out, s->fb_linear, n3 * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+static int render_drop_in(mh_scene* s, const mh_camera* cam, int w, int h, int spp, int depth, unsigned char* rgb,
+                          int mode)
+{
+    if (!s || !cam || !rgb) return fail(1, "NULL argument");
+    if (w <= 0 || h <= 0) return fail(2, "non-positive image size");
+    if (spp < 1 || depth < 0) return fail(3, "bad spp / depth");
+    mh_render_params p;
+    memset(&p, 0, sizeof p);
+    p.width = w;
+    p.height = h;
+    p.spp = spp;
+    p.max_depth = depth;
+    p.mode = mode;
+    p.precision = MH_PREC_FP32;
+    return mh_render(s, cam, &p, rgb, nullptr);
+}
+
+int mh_pathtrace_render(mh_scene* s, const mh_camera* cam, int width, int height, int spp, int max_depth,
+                        unsigned char* rgb_out)
+{
+    return render_drop_in(s, cam, width, height, spp, max_depth, rgb_out, MH_MODE_PATHTRACE);
+}
+
+int mh_render_image(mh_scene* s, const mh_camera* cam, int width, int height, int spp, int max_depth,
+                    unsigned char* rgb_out)
+{
+    return render_drop_in(s, cam, width, height, spp, max_depth, rgb_out, MH_MODE_WHITTED);
+}
+
+int mh_last_timing(mh_timing* out)
+{
+    if (!out) return fail(1, "NULL argument");
+    *out = g_timing;
+    return 0;
+}
+
+double mh_last_seconds(void) { return g_seconds; }
+
+const char* mh_last_error(void) { return g_err.c_str(); }
+
+} // extern "C"

@@ -1,0 +1,214 @@
+// mh_kernels_impl.h - kernel bodies and launchers, included by the two
+// per-precision translation units.
+#pragma once
+
+#include "mh_kernels.h"
+#include "mh_render.h"
+
+namespace mh {
+
+// Blocks are dealt round-robin to the 8 XCDs of an MI355X; remapping block b
+// to tile (b % 8) * per + b / 8 gives every XCD a contiguous band of the
+// image, so rays that share BVH nodes share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblocks)
+{
+    int per = nblocks >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+__device__ __forceinline__ void count_rays(unsigned long long* ctr, int nq)
+{
+    // wave-level reduction, one atomic per wave
+    unsigned long long v = (unsigned long long)nq;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
+}
+
+// One thread per pixel, 16x16-pixel tiles per 256-thread block; the thread
+// loops over the pixel's samples in order (same summation order as the
+// reference pixel loop), then writes the mean and its 8-bit tone map.
+template <typename R>
+__global__ __launch_bounds__(256) void k_pathtrace(SceneView<R> sc, CamView<R> cam, int w, int h, int spp, int s0,
+                                                   int depth, int tiles_x, int ntiles, LaunchOut out)
+{
+    int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int nq = 0;
+    if (tile < ntiles) {
+        int tx = tile % tiles_x, ty = tile / tiles_x;
+        int x = tx * 16 + (threadIdx.x & 15);
+        int y = ty * 16 + (threadIdx.x >> 4);
+        if (x < w && y < h) {
+            V3<R> acc = zero3<R>();
+            for (int si = 0; si < spp; ++si) {
+                uint32_t s = (uint32_t)(s0 + si);
+                R jx = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 0u);
+                R jy = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 1u);
+                R l1 = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 100u);
+                R l2 = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 101u);
+                R uu = ((R)x + jx) / (R)w;
+                R vv = R(1) - ((R)y + jy) / (R)h;
+                V3<R> o, d;
+                camera_ray(cam, uu, vv, l1, l2, &o, &d);
+                if (cam.dome > R(1e-4) && sc.water_material >= 0) {
+                    V3<R> pd = o + d * cam.dome;
+                    if (pd.y < sc.water_level) o = pd - mk(R(0), R(1e-3), R(0));
+                }
+                acc = acc + pt_radiance(sc, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq);
+            }
+            acc = acc * (R(1) / (R)spp);
+            size_t p = ((size_t)y * (size_t)w + (size_t)x) * 3;
+            out.linear[p] = (float)acc.x;
+            out.linear[p + 1] = (float)acc.y;
+            out.linear[p + 2] = (float)acc.z;
+            out.rgb[p] = to_byte(acc.x);
+            out.rgb[p + 1] = to_byte(acc.y);
+            out.rgb[p + 2] = to_byte(acc.z);
+        }
+    }
+    count_rays(out.rays, nq);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_whitted(SceneView<R> sc, CamView<R> cam, int w, int h, int spp, int s0,
+                                                 int depth, LaunchOut out, WItem<R>* stack, int cap)
+{
+    int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    int nlanes = gridDim.x * blockDim.x;
+    WItem<R>* stk = stack + (size_t)lane * (size_t)cap;
+    int nq = 0, ovf = 0;
+    int g = (int)(msqrt((double)spp) + 0.5);
+    bool strat = g > 0 && g * g == spp;
+    for (int pix = lane; pix < w * h; pix += nlanes) {
+        int x = pix % w, y = pix / w;
+        V3<R> acc = zero3<R>();
+        for (int si = 0; si < spp; ++si) {
+            int s = s0 + si;
+            R du, dv;
+            R jx = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 0));
+            R jy = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 1));
+            if (strat) {
+                du = ((R)(s % g) + jx) / (R)g;
+                dv = ((R)(s / g) + jy) / (R)g;
+            } else {
+                du = jx;
+                dv = jy;
+            }
+            R uu = ((R)x + du) / (R)w;
+            R vv = R(1) - ((R)y + dv) / (R)h;
+            R l1 = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 2));
+            R l2 = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 3));
+            V3<R> o, d;
+            camera_ray(cam, uu, vv, l1, l2, &o, &d);
+            acc = acc + whitted_sample(sc, o, d, depth, hash3((uint32_t)x, (uint32_t)y, (uint32_t)s), stk, cap, &ovf,
+                                       &nq);
+        }
+        acc = acc * (R(1) / (R)spp);
+        size_t p = ((size_t)y * (size_t)w + (size_t)x) * 3;
+        out.linear[p] = (float)acc.x;
+        out.linear[p + 1] = (float)acc.y;
+        out.linear[p + 2] = (float)acc.z;
+        out.rgb[p] = to_byte(acc.x);
+        out.rgb[p + 1] = to_byte(acc.y);
+        out.rgb[p + 2] = to_byte(acc.z);
+    }
+    if (ovf) atomicOr(out.overflow, 1);
+    count_rays(out.rays, nq);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_closest(SceneView<R> sc, const mh_ray* rays, int n, double tmin, double tmax,
+                                                 mh_hit* out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    mh_ray r = rays[i];
+    V3<R> o = mk((R)r.ox, (R)r.oy, (R)r.oz);
+    V3<R> d = unit_dir(mk((R)r.dx, (R)r.dy, (R)r.dz));
+    DHit<R> h;
+    mh_hit res;
+    res.reserved = 0;
+    if (intersect(sc, o, d, (R)tmin, (R)tmax, &h)) {
+        res.t = (double)h.t;
+        res.prim_index = h.prim;
+        res.material = h.mat;
+        res.front_face = h.front;
+        res.nx = (double)h.n.x;
+        res.ny = (double)h.n.y;
+        res.nz = (double)h.n.z;
+        res.u = (double)h.u;
+        res.v = (double)h.v;
+    } else {
+        res.t = 0.0;
+        res.prim_index = -1;
+        res.material = 0;
+        res.front_face = 0;
+        res.nx = res.ny = res.nz = res.u = res.v = 0.0;
+    }
+    out[i] = res;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_any(SceneView<R> sc, const mh_ray* rays, int n, double tmin, double tmax,
+                                             uint8_t* out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    mh_ray r = rays[i];
+    V3<R> o = mk((R)r.ox, (R)r.oy, (R)r.oz);
+    V3<R> d = unit_dir(mk((R)r.dx, (R)r.dy, (R)r.dz));
+    out[i] = any_hit(sc, o, d, (R)tmin, (R)tmax) ? 1 : 0;
+}
+
+template <typename R>
+hipError_t launch_pathtrace(const SceneView<R>& sc, const CamView<R>& cam, int w, int h, int spp, int s0, int depth,
+                            const LaunchOut& out, int block, hipStream_t st)
+{
+    (void)block;
+    int tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16;
+    int ntiles = tiles_x * tiles_y;
+    int nblocks = (ntiles + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_pathtrace<R>, dim3(nblocks), dim3(256), 0, st, sc, cam, w, h, spp, s0, depth, tiles_x, ntiles,
+                       out);
+    return hipGetLastError();
+}
+
+template <typename R> size_t whitted_item_bytes() { return sizeof(WItem<R>); }
+
+template <typename R>
+hipError_t launch_whitted(const SceneView<R>& sc, const CamView<R>& cam, int w, int h, int spp, int s0, int depth,
+                          const LaunchOut& out, void* stack, int cap, int threads, hipStream_t st)
+{
+    int nblocks = threads / 256;
+    hipLaunchKernelGGL(k_whitted<R>, dim3(nblocks), dim3(256), 0, st, sc, cam, w, h, spp, s0, depth, out,
+                       (WItem<R>*)stack, cap);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_closest(const SceneView<R>& sc, const mh_ray* rays, int n, double tmin, double tmax, mh_hit* out,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_closest<R>, dim3((n + 255) / 256), dim3(256), 0, st, sc, rays, n, tmin, tmax, out);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_any(const SceneView<R>& sc, const mh_ray* rays, int n, double tmin, double tmax, uint8_t* out,
+                      hipStream_t st)
+{
+    hipLaunchKernelGGL(k_any<R>, dim3((n + 255) / 256), dim3(256), 0, st, sc, rays, n, tmin, tmax, out);
+    return hipGetLastError();
+}
+
+} // namespace mh
+
+#define MH_INSTANTIATE(R)                                                                                          \
+    template hipError_t mh::launch_pathtrace<R>(const SceneView<R>&, const CamView<R>&, int, int, int, int, int,  \
+                                                const LaunchOut&, int, hipStream_t);                               \
+    template hipError_t mh::launch_whitted<R>(const SceneView<R>&, const CamView<R>&, int, int, int, int, int,    \
+                                              const LaunchOut&, void*, int, int, hipStream_t);                     \
+    template size_t mh::whitted_item_bytes<R>();                                                                   \
+    template hipError_t mh::launch_closest<R>(const SceneView<R>&, const mh_ray*, int, double, double, mh_hit*,    \
+                                              hipStream_t);                                                        \
+    template hipError_t mh::launch_any<R>(const SceneView<R>&, const mh_ray*, int, double, double, uint8_t*,       \
+                                          hipStream_t);

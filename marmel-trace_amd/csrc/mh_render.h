@@ -1,0 +1,518 @@
+// mh_render.h - GPU path tracer and fixed-spp Whitted renderer.
+//
+// Path tracer: reference src/pathtrace.c (lobe sampling 251-453, medium
+// 220-233, NEE + shadow segments 504-606, radiance 800-1028, pixel loop
+// 1047-1096). Whitted: reference src/render.c:507-970. Every random number is
+// the same pure function of (pixel, sample, bounce, channel) as in the
+// reference, so the GPU and CPU images differ only by floating-point rounding.
+#pragma once
+
+#include "mh_shade.h"
+#include "mh_trace.h"
+
+namespace mh {
+
+// ---- path tracer --------------------------------------------------------------
+
+template <typename R> struct Lobe {
+    V3<R> wi, w;
+    int refr, spec;
+};
+
+template <typename R>
+__device__ bool sample_pbr(const DMat<R>& m, V3<R> alb, V3<R> N, V3<R> d, V3<R> V, R u1, R u2, R ul, Lobe<R>* out)
+{
+    out->spec = 0;
+    out->refr = 0;
+    R met = clamp01n(m.metallic), rough = clamp01n(m.roughness);
+    R alpha = rough_alpha(rough);
+    V3<R> f0 = lerp(mk(R(0.04), R(0.04), R(0.04)), alb, met);
+    R lum = R(0.2126) * f0.x + R(0.7152) * f0.y + R(0.0722) * f0.z;
+    lum = lum > R(0) ? lum : R(0);
+    R ps = p_specular(met, rough, lum);
+    R ndv = dot(N, V);
+    if (!(ndv > R(0))) return false;
+    if (ul < ps) {
+        V3<R> H = ggx_half(N, alpha, u1, u2);
+        V3<R> wi = reflect(d, H);
+        R ndl = dot(N, wi), ndh = dot(N, H), vdh = dot(V, H);
+        if (!(ndl > R(0)) || !(ndh > R(0)) || !(vdh > R(0))) return false;
+        R D = d_ggx(ndh, alpha);
+        R k = clamp01n((rough + R(1)) * (rough + R(1)) / R(8));
+        R G = (ndl / (ndl * (R(1) - k) + k)) * (ndv / (ndv * (R(1) - k) + k));
+        V3<R> F = schlick_s3(vdh, f0);
+        R pdf = ggx_pdf(ndh, vdh, alpha);
+        if (!(pdf > R(0)) || !(ps > R(0))) return false;
+        out->wi = wi;
+        out->w = F * (D * G / (R(4) * ndv * pdf * ps));
+        out->spec = 1;
+        return true;
+    }
+    R den = R(1) - ps;
+    if (!(den > R(0))) return false;
+    V3<R> wi = cosine_dir(N, u1, u2);
+    if (!(dot(N, wi) > R(0))) return false;
+    V3<R> F = schlick_s3(ndv, f0);
+    V3<R> kd = (mk(R(1), R(1), R(1)) - F) * (R(1) - met);
+    out->wi = wi;
+    out->w = vmul(kd, alb) * (R(1) / den);
+    return true;
+}
+
+template <typename R>
+__device__ bool sample_legacy(const DMat<R>& m, V3<R> alb, V3<R> N, V3<R> d, V3<R> V, int front, R u1, R u2, R ul,
+                              R ug, Lobe<R>* out)
+{
+    out->refr = 0;
+    out->spec = 0;
+    R rm = clamp01n(m.reflectivity), rt = clamp01n(m.transparency);
+    R rd = R(1) - rm - rt;
+    if (rd < R(0)) rd = R(0);
+    R pm = rm, pt = rt, pd = R(1) - pm - pt, norm = R(1);
+    if (pd < R(0)) {
+        R s = pm + pt;
+        if (s > R(0)) {
+            pm /= s;
+            pt /= s;
+            if (s > R(1)) norm = s;
+        }
+        pd = R(0);
+    }
+    if (ul < pm) {
+        if (!(pm > R(0))) return false;
+        R w = rm / (pm * norm);
+        out->wi = reflect(d, N);
+        out->w = mk(w, w, w);
+        out->spec = 1;
+        return true;
+    }
+    if (ul < pm + pt) {
+        if (!(pt > R(0))) return false;
+        out->spec = 1;
+        R ior = m.ior > R(0) ? m.ior : R(1);
+        R eta = front ? R(1) / ior : ior;
+        R F = fresnel_diel(clamp01n(dot(N, V)), eta);
+        R lobe = rt / (pt * norm);
+        if (F >= R(1)) {
+            out->wi = reflect(d, N);
+            out->w = mk(lobe, lobe, lobe);
+            return true;
+        }
+        R pr = clamp_n(F, R(0.05), R(0.95));
+        if (ug < pr) {
+            R w = lobe * (F / pr);
+            out->wi = reflect(d, N);
+            out->w = mk(w, w, w);
+            return true;
+        }
+        V3<R> wi;
+        if (!refract_n(d, N, eta, &wi)) {
+            R w = lobe * (F / pr);
+            out->wi = reflect(d, N);
+            out->w = mk(w, w, w);
+            return true;
+        }
+        R w = lobe * ((R(1) - F) / (R(1) - pr)) * eta * eta;
+        out->wi = wi;
+        out->w = mk(w, w, w);
+        out->refr = 1;
+        return true;
+    }
+    if (!(pd > R(0))) return false;
+    V3<R> wi = cosine_dir(N, u1, u2);
+    if (!(dot(N, wi) > R(0))) return false;
+    out->wi = wi;
+    out->w = alb * (rd / pd);
+    return true;
+}
+
+// Shadow connection through up to 6 closest hits; water and near-clear
+// (transparency > 0.8) surfaces let light through with Beer-Lambert decay.
+template <typename R>
+__device__ bool occluded_segment(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax, V3<R>* att, int* nq)
+{
+    *att = mk(R(1), R(1), R(1));
+    R rem = tmax;
+    for (int step = 0; step < 6; ++step) {
+        DHit<R> h;
+        ++*nq;
+        if (!intersect(sc, o, d, tmin, rem, &h)) return false;
+        if (h.mat < 0 || h.mat >= sc.n_mats) return true;
+        const DMat<R>& m = sc.mats[h.mat];
+        if (m.is_water || m.transparency > R(0.8)) {
+            if (m.beer_lambert || m.is_water)
+                *att = vmul(*att, mk(mexp(-m.absorption.x * h.t), mexp(-m.absorption.y * h.t),
+                                     mexp(-m.absorption.z * h.t)));
+            o = h.p + d * R(1e-3);
+            rem -= h.t;
+            if (rem <= R(1e-3)) return false;
+            tmin = R(1e-3);
+            continue;
+        }
+        return true;
+    }
+    return true;
+}
+
+template <typename R>
+__device__ void apply_medium(const DMat<R>& m, R dist, V3<R>* beta, V3<R>* L)
+{
+    V3<R> T, deep;
+    medium_terms(m, dist, &T, &deep);
+    V3<R> white = mk(R(1) * T.x + deep.x, R(1) * T.y + deep.y, R(1) * T.z + deep.z);
+    V3<R> Tm = white - deep;
+    *L = *L + vmul(*beta, deep);
+    *beta = vmul(*beta, Tm);
+}
+
+template <typename R>
+__device__ V3<R> pt_radiance(const SceneView<R>& sc, V3<R> ro, V3<R> rd0, int max_depth, uint32_t seed, int* nq)
+{
+    V3<R> L = zero3<R>();
+    V3<R> o = ro, d = normalize(rd0);
+    if (len2(d) <= R(0)) return L;
+    V3<R> beta = mk(R(1), R(1), R(1));
+    int medium = -1;
+    if (sc.water_material >= 0 && ro.y < sc.water_level && sc.water_material < sc.n_mats) medium = sc.water_material;
+    int last_spec = 1;
+    for (int b = 0; b <= max_depth; ++b) {
+        DHit<R> h;
+        ++*nq;
+        if (!intersect(sc, o, d, R(1e-4), R(1e30), &h)) {
+            if (medium >= 0) {
+                apply_medium(sc.mats[medium], R(5), &beta, &L);
+                medium = -1;
+            }
+            L = L + vmul(beta, sky_sample(d, sc.sky));
+            break;
+        }
+        if (medium >= 0) apply_medium(sc.mats[medium], h.t, &beta, &L);
+        if (h.mat < 0 || h.mat >= sc.n_mats) {
+            L = L + vmul(beta, sky_sample(d, sc.sky));
+            break;
+        }
+        const DMat<R>& m = sc.mats[h.mat];
+        V3<R> P = h.p, N = h.n;
+        if (m.is_water && mfabs(h.n.y) > R(0.999) && mfabs(h.n.x) < R(1e-4) && mfabs(h.n.z) < R(1e-4))
+            N = water_normal<R>(P.x, P.z);
+        if (dot(N, d) > R(0)) N = -N;
+        N = normalize(N);
+        if (len2(N) <= R(0)) break;
+        V3<R> V = -d;
+        V3<R> alb = texture_albedo(m, P, h.u, h.v);
+        if ((b == 0 || last_spec) && m.has_emission) L = L + vmul(beta, m.emissive);
+        if (b == 0 && m.has_glow) L = L + vmul(beta, limb_glow(m, N, V, sc.sky.sun_dir));
+
+        // emissive-sphere NEE, one cone sample per light
+        for (int li = 0; li < sc.n_lights; ++li) {
+            const DLight<R>& lt = sc.lights[li];
+            if (lt.prim == h.prim) continue;
+            V3<R> toc = lt.center - P;
+            R dc2 = len2(toc), dc = msqrt(dc2);
+            if (dc <= lt.radius) continue;
+            R cmx = msqrt(R(1) - (lt.radius * lt.radius) / dc2);
+            R omega = R(2) * Consts<R>::pi * (R(1) - cmx);
+            if (!(omega > R(0))) continue;
+            V3<R> w = toc * (R(1) / dc);
+            R e1 = rand01<R>(seed ^ (uint32_t)li, (uint32_t)b, 0xA201u);
+            R e2 = rand01<R>(seed ^ (uint32_t)li, (uint32_t)b, 0xA202u);
+            V3<R> wi = cone_dir(w, cmx, e1, e2);
+            if (!finite3(wi) || dot(N, wi) <= R(0)) continue;
+            V3<R> so = P + N * R(1e-3);
+            PD<R> lp;
+            lp.kind = K_SPHERE;
+            lp.p0 = lt.center;
+            lp.r0 = lt.radius;
+            R lt_t;
+            if (!hit_sphere<R, false>(lp, so, unit_dir(wi), R(1e-3), R(1e30), &lt_t, nullptr)) continue;
+            V3<R> att;
+            if (occluded_segment(sc, so, wi, R(1e-3), lt_t - R(1e-3), &att, nq)) continue;
+            V3<R> lit = shade(m, alb, N, wi, V, vmul(lt.emissive, att));
+            L = L + vmul(beta, lit * omega);
+        }
+        // sun NEE at the primary hit (or the first hit inside a medium after refraction)
+        if (b == 0 || (b == 1 && medium >= 0 && last_spec)) {
+            R s1 = rand01<R>(seed, (uint32_t)b, 0xA101u);
+            R s2 = rand01<R>(seed, (uint32_t)b, 0xA102u);
+            V3<R> Ls = sun_disk_dir(sc.sky.sun_dir, sc.sky.sun_radius, s1, s2);
+            if (finite3(Ls) && dot(N, Ls) > R(0)) {
+                V3<R> att;
+                if (!occluded_segment(sc, P + N * R(1e-3), Ls, R(1e-3), R(1e30), &att, nq)) {
+                    if (!(att.x < R(1e-4) && att.y < R(1e-4) && att.z < R(1e-4))) {
+                        V3<R> lit = shade(m, alb, N, Ls, V, vmul(sc.sky.sun_color, att));
+                        L = L + vmul(beta, lit * sc.sky.sun_omega);
+                    }
+                }
+            }
+        }
+
+        uint32_t kb = hash3(seed ^ 0x5054u, (uint32_t)b, 0u);
+        R u1 = rand01<R>(kb, (uint32_t)b, 0xB501u);
+        R u2 = rand01<R>(kb, (uint32_t)b, 0xB502u);
+        R ul = rand01<R>(kb, (uint32_t)b, 0x1001u);
+        R ug = rand01<R>(kb, (uint32_t)b, 0x1002u);
+        Lobe<R> lb;
+        bool ok = m.pbr ? sample_pbr(m, alb, N, d, V, u1, u2, ul, &lb)
+                        : sample_legacy(m, alb, N, d, V, h.front, u1, u2, ul, ug, &lb);
+        if (!ok) break;
+        last_spec = lb.spec;
+        beta = vmul(beta, lb.w);
+        R mc = beta.x;
+        if (beta.y > mc) mc = beta.y;
+        if (beta.z > mc) mc = beta.z;
+        if (!(mc > R(0))) mc = R(0);
+        if (!finite3(beta) || !(mc > R(0))) break;
+        if (lb.refr && (m.is_water || m.beer_lambert)) medium = h.front ? h.mat : -1;
+        if (b >= 2) {
+            R q = clamp01n(mc);
+            if (!(q > R(0))) break;
+            if (rand01<R>(kb, (uint32_t)b, 0x5201u) >= q) break;
+            beta = beta * (R(1) / q);
+        }
+        V3<R> off = lb.refr ? lb.wi * R(1e-4) : N * (dot(lb.wi, N) >= R(0) ? R(1e-4) : R(-1e-4));
+        o = P + off;
+        d = lb.wi;
+    }
+    if (!finite3(L)) return zero3<R>();
+    if (L.x < R(0)) L.x = R(0);
+    if (L.y < R(0)) L.y = R(0);
+    if (L.z < R(0)) L.z = R(0);
+    return L;
+}
+
+template <typename R>
+__device__ __forceinline__ void camera_ray(const CamView<R>& c, R u, R v, R r1, R r2, V3<R>* o, V3<R>* d)
+{
+    V3<R> pt = c.ll + (c.hor * u + c.ver * v);
+    V3<R> dir = normalize(pt - c.pos);
+    if (c.aperture <= R(1e-12)) {
+        *o = c.pos;
+        *d = dir;
+        return;
+    }
+    R fd = c.focus > R(0) ? c.focus : R(1);
+    V3<R> focal = c.pos + dir * fd;
+    R rad = c.aperture * msqrt(r1);
+    R th = R(2) * Consts<R>::pi * r2;
+    R s, cs;
+    msincos(th, &s, &cs);
+    R sq = c.squeeze;
+    V3<R> off;
+    if (sq <= R(0) || mfabs(sq - R(1)) < R(1e-12)) off = c.right_n * (rad * cs) + c.up_n * (rad * s);
+    else off = c.right_n * ((rad / sq) * cs) + c.up_n * (rad * s);
+    *o = c.pos + off;
+    *d = normalize(focal - *o);
+}
+
+template <typename R> __device__ __forceinline__ unsigned char to_byte(R c)
+{
+    if (!(c >= R(0))) c = R(0);
+    else if (c > R(1)) c = R(1);
+    int iv = (int)(mpow(c, R(1.0 / 2.2)) * R(255) + R(0.5));
+    iv = iv < 0 ? 0 : (iv > 255 ? 255 : iv);
+    return (unsigned char)iv;
+}
+
+// ---- Whitted (explicit work stack) ------------------------------------------------
+//
+// The reference recursion trace()/trace_hit() is linear in the colours its
+// children return, so a pixel sample is the sum over a tree of rays of
+// (weight x local term). Each stack item carries its ray, its weight and,
+// for rays refracted into an absorbing medium, the material whose
+// Beer-Lambert blend applies once the segment length is known.
+
+template <typename R> struct WItem {
+    V3<R> o, d, w;
+    int depth;
+    int medium; // material index of the absorbing medium, or -1
+};
+
+template <typename R>
+__device__ V3<R> emissive_direct(const SceneView<R>& sc, const DMat<R>& m, V3<R> alb, V3<R> P, V3<R> N, V3<R> V,
+                                 int self, uint32_t seed, int* nq)
+{
+    V3<R> sum = zero3<R>();
+    V3<R> so = P + N * R(1e-3);
+    for (int li = 0; li < sc.n_lights; ++li) {
+        const DLight<R>& lt = sc.lights[li];
+        if (lt.prim == self) continue;
+        V3<R> toc = lt.center - P;
+        R dc2 = len2(toc), dc = msqrt(dc2);
+        if (dc <= lt.radius) continue;
+        R cmx = msqrt(R(1) - (lt.radius * lt.radius) / dc2);
+        R solid = R(2) * Consts<R>::pi * (R(1) - cmx);
+        if (!(solid > R(0))) continue;
+        V3<R> w = toc * (R(1) / dc);
+        for (int si = 0; si < 16; ++si) {
+            uint32_t k = hash3(seed ^ 0xE311u, (uint32_t)li, (uint32_t)si);
+            R e1 = rand01<R>(k, (uint32_t)si, 0x4c11u);
+            R e2 = rand01<R>(k, (uint32_t)si, 0x9d27u);
+            V3<R> wi = cone_dir(w, cmx, e1, e2);
+            if (dot(N, wi) <= R(0)) continue;
+            PD<R> lp;
+            lp.kind = K_SPHERE;
+            lp.p0 = lt.center;
+            lp.r0 = lt.radius;
+            R lt_t;
+            if (!hit_sphere<R, false>(lp, so, unit_dir(wi), R(1e-3), R(1e30), &lt_t, nullptr)) continue;
+            ++*nq;
+            if (any_hit(sc, so, unit_dir(wi), R(1e-3), lt_t - R(1e-3))) continue;
+            sum = sum + shade(m, alb, N, wi, V, lt.emissive * solid);
+        }
+    }
+    return sum * (R(1) / R(16));
+}
+
+// Shade one Whitted hit: add weight x local terms to acc, push children.
+// Returns false on stack overflow.
+template <typename R>
+__device__ bool whitted_hit(const SceneView<R>& sc, const WItem<R>& it, const DHit<R>& h, int max_depth, uint32_t seed,
+                            V3<R>* acc, WItem<R>* stk, int* sp, int cap, int* nq)
+{
+    if (it.depth > max_depth) return true;
+    V3<R> d = it.d;
+    if (h.mat < 0 || h.mat >= sc.n_mats) {
+        *acc = *acc + vmul(it.w, sky_sample(d, sc.sky));
+        return true;
+    }
+    const DMat<R>& m = sc.mats[h.mat];
+    V3<R> P = h.p, N = h.n;
+    if (m.is_water) N = water_normal<R>(P.x, P.z);
+    if (dot(N, d) > R(0)) N = -N;
+    V3<R> V = -d;
+    V3<R> alb = texture_albedo(m, P, h.u, h.v);
+    V3<R> color = ambient(alb, N, sc.sky);
+    V3<R> Ls = sc.sky.sun_dir;
+    V3<R> so = P + N * R(1e-3);
+    if (sc.sky.sun_radius > R(1e-9)) {
+        R lit = R(0);
+        for (int i = 0; i < 16; ++i) {
+            uint32_t k = seed * 0x9e3779b9u + (uint32_t)i * 2u;
+            R r1 = rand01<R>(k, (uint32_t)i, 0x5a17u);
+            R r2 = rand01<R>(k, (uint32_t)i, 0x7c3du);
+            V3<R> Ld = sun_disk_dir(Ls, sc.sky.sun_radius, r1, r2);
+            ++*nq;
+            if (!any_hit(sc, so, unit_dir(Ld), R(1e-3), R(1e30))) lit += R(1);
+        }
+        if (lit > R(0)) color = color + shade(m, alb, N, Ls, V, sc.sky.sun_color) * (lit / R(16));
+    } else {
+        ++*nq;
+        if (!any_hit(sc, so, unit_dir(Ls), R(1e-3), R(1e30))) color = color + shade(m, alb, N, Ls, V, sc.sky.sun_color);
+    }
+    if (sc.n_lights > 0) color = color + emissive_direct(sc, m, alb, P, N, V, h.prim, seed, nq);
+
+    // Fresnel at the refracted cosine (Schlick), TIR detection
+    R f0 = (R(1) - m.ior) / (R(1) + m.ior);
+    f0 = f0 * f0;
+    R ci = dot(N, V);
+    ci = ci < R(0) ? R(0) : (ci > R(1) ? R(1) : ci);
+    R eta = h.front == 1 ? R(1) / m.ior : m.ior;
+    R s2t = eta * eta * (R(1) - ci * ci);
+    R F;
+    bool tir = false;
+    if (s2t > R(1)) {
+        F = R(1);
+        tir = true;
+    } else {
+        R c = clamp01m(msqrt(R(1) - s2t)), q = R(1) - c, q2 = q * q;
+        F = f0 + (R(1) - f0) * (q2 * q2 * q);
+    }
+    R wrefl, wrefr = R(0);
+    V3<R> local;
+    if (m.transparency > R(0)) {
+        R tr = m.transparency;
+        local = color * (R(1) - tr);
+        wrefl = tir ? tr : tr * F;
+        if (!tir) wrefr = tr * (R(1) - F);
+    } else {
+        local = color * (R(1) - m.reflectivity);
+        wrefl = m.reflectivity;
+        if (it.depth == 0 && m.has_glow) local = local + limb_glow(m, N, V, sc.sky.sun_dir);
+    }
+    if (m.pbr && m.has_emission) local = local + m.emissive;
+    *acc = *acc + vmul(it.w, local);
+
+    // refraction child (its intersection starts at tmin 1e-4 from P + rd*1e-3)
+    if (wrefr != R(0)) {
+        R cosi = -dot(d, N);
+        R s2 = eta * eta * (R(1) - cosi * cosi);
+        V3<R> rd = zero3<R>();
+        if (!(s2 > R(1))) {
+            R ct = msqrt(R(1) - s2), k = eta * cosi - ct;
+            rd = mk(eta * d.x + k * N.x, eta * d.y + k * N.y, eta * d.z + k * N.z);
+        }
+        if (*sp >= cap) return false;
+        WItem<R>& c = stk[(*sp)++];
+        c.o = P + rd * R(1e-3);
+        c.d = rd;
+        c.w = it.w * wrefr;
+        c.depth = it.depth + 1;
+        c.medium = (m.is_water || m.beer_lambert) ? h.mat : -2; // -2: refracted, no medium
+    }
+    if (it.depth < max_depth && wrefl != R(0)) {
+        if (m.pbr != 0 && m.reflectivity > R(0) && m.roughness > R(1e-6)) {
+            for (int i = 0; i < 16; ++i) {
+                uint32_t k = seed * 0x9e3779b9u + (uint32_t)it.depth * 0x85ebca6bu + (uint32_t)i * 2u;
+                R r1 = rand01<R>(k, (uint32_t)i, 0x9e37u);
+                R r2 = rand01<R>(k, (uint32_t)i, 0x85ebu);
+                if (*sp >= cap) return false;
+                WItem<R>& c = stk[(*sp)++];
+                c.o = P + N * R(1e-3);
+                c.d = glossy_dir(m, N, d, r1, r2);
+                c.w = it.w * (wrefl * (R(1) / R(16)));
+                c.depth = it.depth + 1;
+                c.medium = -1;
+            }
+        } else {
+            if (*sp >= cap) return false;
+            WItem<R>& c = stk[(*sp)++];
+            c.o = P + N * R(1e-3);
+            c.d = reflect(d, N);
+            c.w = it.w * wrefl;
+            c.depth = it.depth + 1;
+            c.medium = -1;
+        }
+    }
+    return true;
+}
+
+template <typename R>
+__device__ V3<R> whitted_sample(const SceneView<R>& sc, V3<R> o, V3<R> d, int max_depth, uint32_t seed,
+                                WItem<R>* stk, int cap, int* overflow, int* nq)
+{
+    V3<R> acc = zero3<R>();
+    int sp = 0;
+    WItem<R>& root = stk[sp++];
+    root.o = o;
+    root.d = d;
+    root.w = mk(R(1), R(1), R(1));
+    root.depth = 0;
+    root.medium = -1;
+    while (sp > 0) {
+        WItem<R> it = stk[--sp];
+        if (it.depth > max_depth && it.medium == -1) continue;
+        DHit<R> h;
+        ++*nq;
+        bool hit = intersect(sc, it.o, unit_dir(it.d), R(1e-4), R(1e30), &h);
+        if (it.medium >= 0) {
+            // Beer-Lambert blend over the in-medium segment (5 when it escapes)
+            const DMat<R>& mm = sc.mats[it.medium];
+            V3<R> T, deep;
+            medium_terms(mm, hit ? h.t : R(5), &T, &deep);
+            acc = acc + vmul(it.w, deep);
+            it.w = vmul(it.w, T);
+        }
+        if (!hit) {
+            acc = acc + vmul(it.w, sky_sample(it.d, sc.sky));
+            continue;
+        }
+        if (it.depth > max_depth) continue;
+        if (!whitted_hit(sc, it, h, max_depth, seed, &acc, stk, &sp, cap, nq)) {
+            *overflow = 1;
+            break;
+        }
+    }
+    return acc;
+}
+
+} // namespace mh

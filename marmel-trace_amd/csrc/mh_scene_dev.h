@@ -1,0 +1,109 @@
+// mh_scene_dev.h - device-side scene layout of libmarmel_hip.
+//
+// HBM layout (one upload in mh_scene_create):
+//   nodes   BVH2, 64 B per node: the two CHILD boxes of the node as fp32
+//           (rounded outward from the host's double bounds) + two child refs.
+//           A child ref >= 0 is a node index; a ref < 0 is a leaf,
+//           ~ref = (first << 3) | (count - 1) into the leaf-ordered records.
+//   prims32 leaf-ordered primitive records, 48 B (fp32 throughput path)
+//   prims64 leaf-ordered primitive records, 104 B (fp64 parity path)
+//   planes  plane side list (not in the tree), fp32 and fp64 records
+//   attrs   leaf-ordered smooth normals / uvs (only when a scene has them)
+//   mats    material table (fp32 and fp64)
+// Sky, lights and water metadata travel in the kernel argument block.
+#pragma once
+
+#include "mh_math.h"
+
+namespace mh {
+
+enum { K_SPHERE = 0, K_PLANE = 1, K_BOX = 2, K_TRIANGLE = 3, K_CYLINDER = 4 };
+enum { F_SMOOTH = 1, F_UV = 2 };
+
+struct __attribute__((aligned(16))) DNode {
+    float4 c0; // left  (minx, maxx, miny, maxy)
+    float4 c1; // left  (minz, maxz) right (minx, maxx)
+    float4 c2; // right (miny, maxy, minz, maxz)
+    int4 c3;   // (left ref, right ref, -, -)
+};
+
+// fp32 record: a.w = kind | flags << 8, b.w = prim index, c.w = material
+//   TRIANGLE a = v0, b = v1 - v0, c = v2 - v0
+//   SPHERE   a = centre, b.x = radius
+//   BOX      a = centre, b = half extents
+//   CYLINDER a = base, b = top, c.x = r0, c.y = r1
+//   PLANE    a = point, b = unit normal
+struct __attribute__((aligned(16))) P32 {
+    float4 a, b, c;
+};
+
+// fp64 record: the host's double data verbatim.
+struct P64 {
+    double p0[3], p1[3], p2[3];
+    double r0, r1;
+    int32_t kind, prim, material, flags;
+};
+
+template <typename R> struct DAttr {
+    V3<R> n0, n1, n2, uv0, uv1, uv2;
+};
+
+template <typename R> struct DMat {
+    V3<R> albedo, specular;
+    R shininess, reflectivity, transparency, ior;
+    int is_water, beer_lambert, pbr, texture_kind;
+    V3<R> absorption, deep_color;
+    R metallic, roughness;
+    V3<R> emissive;
+    R texture_scale;
+    V3<R> tex_a, tex_b, glow;
+    int has_emission;   // emissive != 0
+    int has_glow;       // |glow|^2 > 1e-6
+};
+
+template <typename R> struct DSky {
+    V3<R> sun_dir, sun_color, horizon, zenith;
+    R gamma, glow_exp, glow_strength;
+    R cloud_height, cloud_scale, coverage, softness, sharpness;
+    int octaves;
+    uint32_t seed;
+    R sun_radius;      // degrees
+    R sun_omega;       // solid angle used by sun NEE (1 for a point sun)
+};
+
+template <typename R> struct DLight {
+    int prim;
+    V3<R> center;
+    R radius;
+    V3<R> emissive;
+};
+
+template <typename R> struct PrimT;
+template <> struct PrimT<float> { typedef P32 type; };
+template <> struct PrimT<double> { typedef P64 type; };
+
+template <typename R> struct SceneView {
+    const DNode* nodes;
+    int n_nodes;
+    int root_ref;                  // ref of the tree root, or -2^31 when no tree
+    const typename PrimT<R>::type* prims;
+    const typename PrimT<R>::type* planes;
+    int n_planes;
+    const int* plane_index;        // prim index of each plane
+    const DAttr<R>* attrs;         // leaf order, may be null
+    const DMat<R>* mats;
+    int n_mats;
+    DSky<R> sky;
+    R water_level;
+    int water_material;
+    int n_lights;
+    DLight<R> lights[8];
+};
+
+template <typename R> struct CamView {
+    V3<R> pos, ll, hor, ver;
+    R aperture, focus, dome, squeeze;
+    V3<R> right_n, up_n; // normalized horizontal / vertical
+};
+
+} // namespace mh

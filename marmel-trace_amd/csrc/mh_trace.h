@@ -1,0 +1,677 @@
+// mh_trace.h - ray/primitive intersection and BVH2 traversal on the GPU.
+//
+// Primitive tests follow reference src/geometry.c (sphere 338-369, plane
+// 372-393, box 396-479, triangle 482-538, cylinder 549-699, occlusion
+// 1373-1461); traversal follows the nearest-hit semantics of reference
+// src/bvh.c:557-712 (plane side list first, then the tree; equal t resolves
+// to the lowest prim index) and the any-hit semantics of src/bvh.c:714-813.
+//
+// The traversal is a stack walk over 64-byte nodes holding both child boxes,
+// so one node fetch tests two boxes and pushes the far child only. Nodes are
+// culled when their entry distance exceeds the running closest hit (strict >,
+// which keeps equal-t ties reachable). The fp32 path widens every slab
+// interval by a few ulps (robust traversal), the fp64 path runs the slab test
+// in double over fp32 boxes that the host rounded outward, so it never culls a
+// node the double-precision reference would visit.
+#pragma once
+
+#include "mh_scene_dev.h"
+
+namespace mh {
+
+template <typename R> struct PD {
+    int kind, flags, prim, mat;
+    V3<R> p0, p1, p2; // fp32: triangles carry (v0, e1, e2)
+    R r0, r1;
+};
+
+__device__ __forceinline__ PD<float> load_prim(const P32* p)
+{
+    float4 a = p->a, b = p->b, c = p->c;
+    PD<float> d;
+    int kf = __float_as_int(a.w);
+    d.kind = kf & 0xff;
+    d.flags = kf >> 8;
+    d.prim = __float_as_int(b.w);
+    d.mat = __float_as_int(c.w);
+    d.p0 = mk(a.x, a.y, a.z);
+    d.p1 = mk(b.x, b.y, b.z);
+    d.p2 = mk(c.x, c.y, c.z);
+    d.r0 = d.kind == K_SPHERE ? b.x : c.x;
+    d.r1 = c.y;
+    return d;
+}
+
+__device__ __forceinline__ PD<double> load_prim(const P64* p)
+{
+    PD<double> d;
+    d.kind = p->kind;
+    d.flags = p->flags;
+    d.prim = p->prim;
+    d.mat = p->material;
+    d.p0 = mk(p->p0[0], p->p0[1], p->p0[2]);
+    d.p1 = mk(p->p1[0], p->p1[1], p->p1[2]);
+    d.p2 = mk(p->p2[0], p->p2[1], p->p2[2]);
+    d.r0 = p->r0;
+    d.r1 = p->r1;
+    return d;
+}
+
+template <typename R> struct Eps;
+template <> struct Eps<float> {
+    static constexpr float bary = 1e-6f;   // closes fp32 cracks along shared edges
+};
+template <> struct Eps<double> {
+    static constexpr double bary = 1e-9;   // reference GEO_BARY_EPS
+};
+
+template <typename R> struct DHit {
+    R t, u, v;
+    V3<R> p, n;
+    int front, prim, mat;
+};
+
+template <typename R>
+__device__ __forceinline__ void store_hit(DHit<R>* h, R t, V3<R> p, V3<R> n, V3<R> d, int mat, R u, R v)
+{
+    bool front = dot(d, n) < R(0);
+    h->t = t;
+    h->p = p;
+    h->n = front ? n : -n;
+    h->front = front ? 1 : 0;
+    h->mat = mat;
+    h->u = u;
+    h->v = v;
+}
+
+// ---- sphere -------------------------------------------------------------
+
+template <typename R, bool FULL>
+__device__ __forceinline__ bool hit_sphere(const PD<R>& p, V3<R> o, V3<R> d, R tmin, R tmax, R* tout, DHit<R>* h)
+{
+    if (p.r0 <= R(0)) return false;
+    V3<R> l = o - p.p0;
+    R hb = dot(l, d);
+    R c = len2(l) - p.r0 * p.r0;
+    R disc = hb * hb - c;
+    if (disc < R(0)) return false;
+    R sq = msqrt(disc);
+    R t = -hb - sq;
+    if (t < tmin) {
+        t = -hb + sq;
+        if (t < tmin || t > tmax) return false;
+    } else if (t > tmax) {
+        return false;
+    }
+    *tout = t;
+    if (FULL) {
+        V3<R> pt = at(o, d, t);
+        V3<R> n = (pt - p.p0) * (R(1) / p.r0);
+        R phi = matan2(n.z, n.x);
+        R th = masin(mfmax(R(-1), mfmin(R(1), n.y)));
+        R u = R(0.5) + phi / (R(2) * Consts<R>::pi);
+        R v = R(0.5) + th / Consts<R>::pi;
+        u = u - mfloor(u);
+        v = mfmax(R(0), mfmin(R(1), v));
+        store_hit(h, t, pt, n, d, p.mat, u, v);
+    }
+    return true;
+}
+
+// any-hit sphere: nearest root of t^2 + 2 hb t + c in [tmin, tmax]
+template <typename R>
+__device__ __forceinline__ bool occ_sphere(const PD<R>& p, V3<R> o, V3<R> d, R tmin, R tmax)
+{
+    if (p.r0 <= R(0)) return false;
+    V3<R> l = o - p.p0;
+    R hb = dot(l, d);
+    R c = len2(l) - p.r0 * p.r0;
+    R disc = hb * hb - c;
+    if (disc < R(0)) return false;
+    R sq = msqrt(disc);
+    R t0 = -hb - sq, t1 = -hb + sq;
+    if (t0 > t1) { R tt = t0; t0 = t1; t1 = tt; }
+    return (t0 >= tmin && t0 <= tmax) || (t1 >= tmin && t1 <= tmax);
+}
+
+// ---- plane ----------------------------------------------------------------
+
+template <typename R, bool FULL>
+__device__ __forceinline__ bool hit_plane(const PD<R>& p, V3<R> o, V3<R> d, R tmin, R tmax, R* tout, DHit<R>* h)
+{
+    V3<R> n = normalize(p.p1);
+    R den = dot(d, n);
+    if (mfabs(den) < R(1e-12)) return false;
+    R t = dot(p.p0 - o, n) / den;
+    if (t < tmin || t > tmax) return false;
+    *tout = t;
+    if (FULL) {
+        V3<R> pt = at(o, d, t);
+        V3<R> hint = mfabs(n.y) > R(0.9) ? mk(R(0), R(0), R(1)) : mk(R(0), R(1), R(0));
+        V3<R> T = normalize(cross(hint, n));
+        V3<R> B = cross(n, T);
+        V3<R> df = pt - p.p0;
+        R u = dot(df, T), v = dot(df, B);
+        u = u - mfloor(u);
+        v = v - mfloor(v);
+        store_hit(h, t, pt, n, d, p.mat, u, v);
+    }
+    return true;
+}
+
+// ---- box ------------------------------------------------------------------
+
+template <typename R, bool FULL>
+__device__ bool hit_box(const PD<R>& p, V3<R> o, V3<R> d, R tmin, R tmax, R* tout, DHit<R>* h)
+{
+    V3<R> lo = p.p0 - p.p1, hi = p.p0 + p.p1;
+    R os[3] = {o.x, o.y, o.z}, ds[3] = {d.x, d.y, d.z};
+    R los[3] = {lo.x, lo.y, lo.z}, his[3] = {hi.x, hi.y, hi.z};
+    R tn = -INFINITY, tf = INFINITY;
+    int ea = -1, es = 0, xa = -1, xs = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        R inv = R(1) / ds[i];
+        R t1 = (los[i] - os[i]) * inv, t2 = (his[i] - os[i]) * inv;
+        R tnear = mfmin(t1, t2), tfar = mfmax(t1, t2);
+        if (tnear > tn) { tn = tnear; ea = i; es = inv > R(0) ? -1 : 1; }
+        if (tfar < tf) { tf = tfar; xa = i; xs = inv > R(0) ? 1 : -1; }
+        if (tf < tn) return false;
+    }
+    R th;
+    int axis, sgn;
+    if (tn >= tmin && tn <= tmax) { th = tn; axis = ea; sgn = es; }
+    else if (tf >= tmin && tf <= tmax) { th = tf; axis = xa; sgn = xs; }
+    else return false;
+    *tout = th;
+    if (FULL) {
+        V3<R> pt = at(o, d, th);
+        V3<R> n;
+        if (axis < 0) {
+            n = -d;
+        } else {
+            n = zero3<R>();
+            if (axis == 0) n.x = (R)sgn;
+            else if (axis == 1) n.y = (R)sgn;
+            else n.z = (R)sgn;
+        }
+        R u = R(0), v = R(0);
+        R dx = his[0] - los[0], dy = his[1] - los[1], dz = his[2] - los[2];
+        if (axis == 0) {
+            u = dz > R(1e-12) ? (pt.z - los[2]) / dz : R(0);
+            v = dy > R(1e-12) ? (pt.y - los[1]) / dy : R(0);
+        } else if (axis == 1) {
+            u = dx > R(1e-12) ? (pt.x - los[0]) / dx : R(0);
+            v = dz > R(1e-12) ? (pt.z - los[2]) / dz : R(0);
+        } else {
+            u = dx > R(1e-12) ? (pt.x - los[0]) / dx : R(0);
+            v = dy > R(1e-12) ? (pt.y - los[1]) / dy : R(0);
+        }
+        u = mfmax(R(0), mfmin(R(1), u));
+        v = mfmax(R(0), mfmin(R(1), v));
+        store_hit(h, th, pt, n, d, p.mat, u, v);
+    }
+    return true;
+}
+
+template <typename R>
+__device__ bool occ_box(const PD<R>& p, V3<R> o, V3<R> d, R tmin, R tmax)
+{
+    V3<R> lo = p.p0 - p.p1, hi = p.p0 + p.p1;
+    R os[3] = {o.x, o.y, o.z}, ds[3] = {d.x, d.y, d.z};
+    R los[3] = {lo.x, lo.y, lo.z}, his[3] = {hi.x, hi.y, hi.z};
+    R tn = -INFINITY, tf = INFINITY;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        R inv = R(1) / ds[i];
+        R t1 = (los[i] - os[i]) * inv, t2 = (his[i] - os[i]) * inv;
+        if (t1 > t2) { R tt = t1; t1 = t2; t2 = tt; }
+        if (t1 > tn) tn = t1;
+        if (t2 < tf) tf = t2;
+        if (tn > tf) return false;
+    }
+    return (tn >= tmin && tn <= tmax) || (tf >= tmin && tf <= tmax);
+}
+
+// ---- triangle (Moller-Trumbore, two-sided) ---------------------------------
+
+template <typename R> __device__ __forceinline__ void tri_edges(const PD<R>& p, V3<R>* e1, V3<R>* e2);
+template <> __device__ __forceinline__ void tri_edges<float>(const PD<float>& p, V3<float>* e1, V3<float>* e2)
+{
+    *e1 = p.p1;
+    *e2 = p.p2;
+}
+template <> __device__ __forceinline__ void tri_edges<double>(const PD<double>& p, V3<double>* e1, V3<double>* e2)
+{
+    *e1 = p.p1 - p.p0;
+    *e2 = p.p2 - p.p0;
+}
+
+template <typename R, bool FULL>
+__device__ __forceinline__ bool hit_tri(const PD<R>& p, const DAttr<R>* attr, V3<R> o, V3<R> d, R tmin, R tmax, R* tout,
+                                        DHit<R>* h)
+{
+    V3<R> e1, e2;
+    tri_edges(p, &e1, &e2);
+    V3<R> pv = cross(d, e2);
+    R det = dot(e1, pv);
+    if (mfabs(det) < R(1e-9)) return false;
+    R idet = R(1) / det;
+    V3<R> tv = o - p.p0;
+    R u = dot(tv, pv) * idet;
+    if (u < -Eps<R>::bary || u > R(1) + Eps<R>::bary) return false;
+    V3<R> qv = cross(tv, e1);
+    R v = dot(d, qv) * idet;
+    if (v < -Eps<R>::bary || u + v > R(1) + Eps<R>::bary) return false;
+    R t = dot(e2, qv) * idet;
+    if (t < tmin || t > tmax) return false;
+    *tout = t;
+    if (FULL) {
+        V3<R> pt = at(o, d, t);
+        V3<R> n;
+        bool have = false;
+        if ((p.flags & F_SMOOTH) && attr) {
+            R w = R(1) - u - v;
+            n = attr->n0 * w + (attr->n1 * u + attr->n2 * v);
+            R nl = msqrt(len2(n));
+            if (nl > R(1e-12)) {
+                n = n * (R(1) / nl);
+                have = true;
+            }
+        }
+        if (!have) {
+            n = cross(e1, e2);
+            R nl = msqrt(len2(n));
+            if (nl < R(1e-15)) return false;
+            n = n * (R(1) / nl);
+        }
+        R hu = u, hv = v;
+        if ((p.flags & F_UV) && attr) {
+            R w = R(1) - u - v;
+            hu = w * attr->uv0.x + u * attr->uv1.x + v * attr->uv2.x;
+            hv = w * attr->uv0.y + u * attr->uv1.y + v * attr->uv2.y;
+        }
+        store_hit(h, t, pt, n, d, p.mat, hu, hv);
+    }
+    return true;
+}
+
+// ---- tapered capped cylinder -------------------------------------------------
+
+template <typename R, bool FULL>
+__device__ bool hit_cyl(const PD<R>& p, V3<R> o, V3<R> d, R tmin, R tmax, R* tout, DHit<R>* h)
+{
+    V3<R> A = p.p1 - p.p0;
+    R hgt = msqrt(len2(A));
+    if (hgt < R(1e-12)) return false;
+    V3<R> ax = A * (R(1) / hgt);
+    R r0 = p.r0, r1 = p.r1;
+    R k = (r1 - r0) / hgt;
+    V3<R> d0 = o - p.p0;
+    R od = dot(d, ax), d0d = dot(d0, ax);
+    V3<R> dp = d - ax * od;
+    V3<R> d0p = d0 - ax * d0d;
+    R ra = k * od, rb = r0 + k * d0d;
+    R qa = len2(dp) - ra * ra;
+    R qb = dot(dp, d0p) - rb * ra;
+    R qc = len2(d0p) - rb * rb;
+    bool found = false;
+    int is_cap = 0, cap_id = 0;
+    R bt = R(0), bs = R(0);
+    V3<R> bn = zero3<R>();
+    if (mfabs(qa) > R(1e-15)) {
+        R disc = qb * qb - qa * qc;
+        if (disc >= R(0)) {
+            R sq = msqrt(disc);
+            R ro0 = (-qb - sq) / qa, ro1 = (-qb + sq) / qa;
+            if (ro0 > ro1) { R tt = ro0; ro0 = ro1; ro1 = tt; }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                R t = i == 0 ? ro0 : ro1;
+                if (t < tmin || t > tmax) continue;
+                R s = (d0d + t * od) / hgt;
+                if (s < R(0) || s > R(1)) continue;
+                if (!found || t < bt) {
+                    found = true;
+                    bt = t;
+                    bs = s;
+                    is_cap = 0;
+                    if (FULL) {
+                        V3<R> pt = at(o, d, t);
+                        V3<R> ap = p.p0 + ax * (s * hgt);
+                        V3<R> rad = pt - ap;
+                        R rl = msqrt(len2(rad));
+                        V3<R> rd = rl > R(1e-15) ? rad * (R(1) / rl) : zero3<R>();
+                        V3<R> n = normalize(rd * hgt - ax * (r1 - r0));
+                        if (len2(n) < R(0.5)) n = rd;
+                        bn = n;
+                    }
+                }
+            }
+        }
+    } else if (mfabs(qb) > R(1e-15)) {
+        R t = -qc / (R(2) * qb);
+        if (t >= tmin && t <= tmax) {
+            R s = (d0d + t * od) / hgt;
+            if (s >= R(0) && s <= R(1)) {
+                found = true;
+                bt = t;
+                bs = s;
+                is_cap = 0;
+                if (FULL) {
+                    V3<R> pt = at(o, d, t);
+                    V3<R> ap = p.p0 + ax * (s * hgt);
+                    V3<R> rd = normalize(pt - ap);
+                    V3<R> n = normalize(rd * hgt - ax * (r1 - r0));
+                    if (len2(n) < R(0.5)) n = rd;
+                    bn = n;
+                }
+            }
+        }
+    }
+    if (mfabs(od) > R(1e-15)) {
+#pragma unroll
+        for (int cap = 0; cap < 2; ++cap) {
+            R sc = cap == 0 ? R(0) : R(1);
+            R rc = cap == 0 ? r0 : r1;
+            if (rc <= R(0)) continue;
+            V3<R> cc = cap == 0 ? p.p0 : p.p1;
+            R t = ((sc * hgt) - d0d) / od;
+            if (t < tmin || t > tmax) continue;
+            V3<R> pt = at(o, d, t);
+            V3<R> rad = pt - cc;
+            if (len2(rad) > rc * rc) continue;
+            if (!found || t < bt) {
+                found = true;
+                bt = t;
+                bn = cap == 0 ? -ax : ax;
+                is_cap = 1;
+                cap_id = cap;
+            }
+        }
+    }
+    if (!found) return false;
+    *tout = bt;
+    if (FULL) {
+        V3<R> pt = at(o, d, bt);
+        R u, v;
+        V3<R> hint = mfabs(ax.y) > R(0.9) ? mk(R(0), R(0), R(1)) : mk(R(0), R(1), R(0));
+        V3<R> T = normalize(cross(hint, ax));
+        V3<R> B = cross(ax, T);
+        if (is_cap) {
+            V3<R> cc = cap_id == 0 ? p.p0 : p.p1;
+            R rc = cap_id == 0 ? r0 : r1;
+            V3<R> rad = pt - cc;
+            R rx = rc > R(1e-12) ? dot(rad, T) / (R(2) * rc) : R(0);
+            R ry = rc > R(1e-12) ? dot(rad, B) / (R(2) * rc) : R(0);
+            u = mfmax(R(0), mfmin(R(1), R(0.5) + rx));
+            v = mfmax(R(0), mfmin(R(1), R(0.5) + ry));
+        } else {
+            V3<R> ap = p.p0 + ax * (bs * hgt);
+            V3<R> rad = pt - ap;
+            R phi = matan2(dot(rad, B), dot(rad, T));
+            u = R(0.5) + phi / (R(2) * Consts<R>::pi);
+            u = u - mfloor(u);
+            v = mfmax(R(0), mfmin(R(1), bs));
+        }
+        store_hit(h, bt, pt, bn, d, p.mat, u, v);
+    }
+    return true;
+}
+
+// ---- dispatch -------------------------------------------------------------------
+
+template <typename R, bool FULL>
+__device__ __forceinline__ bool hit_prim(const PD<R>& p, const DAttr<R>* attr, V3<R> o, V3<R> d, R tmin, R tmax,
+                                         R* t, DHit<R>* h)
+{
+    switch (p.kind) {
+    case K_TRIANGLE: return hit_tri<R, FULL>(p, attr, o, d, tmin, tmax, t, h);
+    case K_SPHERE: return hit_sphere<R, FULL>(p, o, d, tmin, tmax, t, h);
+    case K_BOX: return hit_box<R, FULL>(p, o, d, tmin, tmax, t, h);
+    case K_CYLINDER: return hit_cyl<R, FULL>(p, o, d, tmin, tmax, t, h);
+    case K_PLANE: return hit_plane<R, FULL>(p, o, d, tmin, tmax, t, h);
+    default: return false;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ bool occ_prim(const PD<R>& p, V3<R> o, V3<R> d, R tmin, R tmax)
+{
+    R t;
+    switch (p.kind) {
+    case K_TRIANGLE: return hit_tri<R, false>(p, nullptr, o, d, tmin, tmax, &t, nullptr);
+    case K_SPHERE: return occ_sphere<R>(p, o, d, tmin, tmax);
+    case K_BOX: return occ_box<R>(p, o, d, tmin, tmax);
+    case K_CYLINDER: return hit_cyl<R, false>(p, o, d, tmin, tmax, &t, nullptr);
+    case K_PLANE: return hit_plane<R, false>(p, o, d, tmin, tmax, &t, nullptr);
+    default: return false;
+    }
+}
+
+// ---- BVH2 traversal -------------------------------------------------------------
+
+#ifndef MH_STACK
+#define MH_STACK 64
+#endif
+
+// Both child slabs of one node. fp32: origin-scaled planes and a widened exit
+// (robust traversal); fp64: the reference's slab arithmetic in double.
+template <typename R> struct Slab;
+
+template <> struct Slab<float> {
+    float ix, iy, iz, ox, oy, oz;
+    __device__ __forceinline__ void init(V3<float> o, V3<float> d)
+    {
+        ix = 1.0f / d.x; iy = 1.0f / d.y; iz = 1.0f / d.z;
+        ox = o.x * ix; oy = o.y * iy; oz = o.z * iz;
+    }
+    __device__ __forceinline__ void test(const DNode& n, float tmin, float tmax, bool* hl, bool* hr, float* tl,
+                                         float* tr) const
+    {
+        float lx0 = fmaf(n.c0.x, ix, -ox), lx1 = fmaf(n.c0.y, ix, -ox);
+        float ly0 = fmaf(n.c0.z, iy, -oy), ly1 = fmaf(n.c0.w, iy, -oy);
+        float lz0 = fmaf(n.c1.x, iz, -oz), lz1 = fmaf(n.c1.y, iz, -oz);
+        float rx0 = fmaf(n.c1.z, ix, -ox), rx1 = fmaf(n.c1.w, ix, -ox);
+        float ry0 = fmaf(n.c2.x, iy, -oy), ry1 = fmaf(n.c2.y, iy, -oy);
+        float rz0 = fmaf(n.c2.z, iz, -oz), rz1 = fmaf(n.c2.w, iz, -oz);
+        float len = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), tmin));
+        float lex = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), tmax));
+        float ren = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), tmin));
+        float rex = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), tmax));
+        const float widen = 1.0000004f; // 1 + 2 gamma(3)
+        *hl = len <= lex * widen;
+        *hr = ren <= rex * widen;
+        *tl = len;
+        *tr = ren;
+    }
+};
+
+template <> struct Slab<double> {
+    double ix, iy, iz, ox, oy, oz;
+    __device__ __forceinline__ void init(V3<double> o, V3<double> d)
+    {
+        ix = 1.0 / d.x; iy = 1.0 / d.y; iz = 1.0 / d.z;
+        ox = o.x; oy = o.y; oz = o.z;
+    }
+    __device__ __forceinline__ void one(double x0, double x1, double y0, double y1, double z0, double z1,
+                                        double tmin, double tmax, bool* hit, float* te) const
+    {
+        double ax = (x0 - ox) * ix, bx = (x1 - ox) * ix;
+        double ay = (y0 - oy) * iy, by = (y1 - oy) * iy;
+        double az = (z0 - oz) * iz, bz = (z1 - oz) * iz;
+        double en = fmax(fmax(fmin(ax, bx), fmin(ay, by)), fmin(az, bz));
+        double ex = fmin(fmin(fmax(ax, bx), fmax(ay, by)), fmax(az, bz));
+        *hit = ex >= en && ex >= tmin && en <= tmax;
+        double e = en < tmin ? tmin : en;
+        // round DOWN to float: culling on a lower bound never drops a node
+        float f = (float)e;
+        if ((double)f > e) f = nextafterf(f, -INFINITY);
+        *te = f;
+    }
+    __device__ __forceinline__ void test(const DNode& n, double tmin, double tmax, bool* hl, bool* hr, float* tl,
+                                         float* tr) const
+    {
+        one(n.c0.x, n.c0.y, n.c0.z, n.c0.w, n.c1.x, n.c1.y, tmin, tmax, hl, tl);
+        one(n.c1.z, n.c1.w, n.c2.x, n.c2.y, n.c2.z, n.c2.w, tmin, tmax, hr, tr);
+    }
+};
+
+// Closest hit. Returns the leaf-record index of the winner (or -1 - plane
+// slot for a plane, or INT_MIN for a miss) and its t / prim index.
+template <typename R>
+__device__ int closest_hit(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax, R* best_t, int* best_prim)
+{
+    int best = INT_MIN;
+    int bprim = 0x7fffffff;
+    R closest = tmax;
+    for (int i = 0; i < sc.n_planes; ++i) {
+        PD<R> p = load_prim(&sc.planes[i]);
+        R t;
+        if (hit_plane<R, false>(p, o, d, tmin, closest, &t, nullptr)) {
+            int pi = sc.plane_index[i];
+            if (best == INT_MIN || t < closest || (t == closest && pi < bprim)) {
+                closest = t;
+                bprim = pi;
+                best = -1 - i;
+            }
+        }
+    }
+    if (sc.root_ref != INT_MIN) {
+        Slab<R> sl;
+        sl.init(o, d);
+        int stk[MH_STACK];
+        float stt[MH_STACK];
+        int sp = 0;
+        stk[sp] = sc.root_ref;
+        stt[sp] = (float)tmin;
+        ++sp;
+        while (sp > 0) {
+            --sp;
+            int ref = stk[sp];
+            if (best != INT_MIN && (R)stt[sp] > closest) continue;
+            if (ref >= 0) {
+                DNode n = sc.nodes[ref];
+                bool hl, hr;
+                float tl, tr;
+                sl.test(n, tmin, closest, &hl, &hr, &tl, &tr);
+                if (hl && hr) {
+                    bool lfirst = tl <= tr;
+                    stk[sp] = lfirst ? n.c3.y : n.c3.x;
+                    stt[sp] = lfirst ? tr : tl;
+                    stk[sp + 1] = lfirst ? n.c3.x : n.c3.y;
+                    stt[sp + 1] = lfirst ? tl : tr;
+                    sp += 2;
+                } else if (hl) {
+                    stk[sp] = n.c3.x;
+                    stt[sp] = tl;
+                    ++sp;
+                } else if (hr) {
+                    stk[sp] = n.c3.y;
+                    stt[sp] = tr;
+                    ++sp;
+                }
+                if (sp > MH_STACK - 2) sp = MH_STACK - 2; // unreachable for host-built trees (depth checked)
+            } else {
+                int code = ~ref;
+                int first = code >> 3, cnt = (code & 7) + 1;
+                for (int i = 0; i < cnt; ++i) {
+                    PD<R> p = load_prim(&sc.prims[first + i]);
+                    R t;
+                    if (hit_prim<R, false>(p, nullptr, o, d, tmin, closest, &t, nullptr)) {
+                        if (best == INT_MIN || t < closest || (t == closest && p.prim < bprim)) {
+                            closest = t;
+                            bprim = p.prim;
+                            best = first + i;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    *best_t = closest;
+    *best_prim = bprim;
+    return best;
+}
+
+// Recompute the full hit record of the winner found by closest_hit.
+template <typename R>
+__device__ __forceinline__ void finish_hit(const SceneView<R>& sc, int rec, V3<R> o, V3<R> d, R tmin, R tmax, DHit<R>* h)
+{
+    R t;
+    if (rec < 0) {
+        int slot = -1 - rec;
+        PD<R> p = load_prim(&sc.planes[slot]);
+        hit_plane<R, true>(p, o, d, tmin, tmax, &t, h);
+        h->prim = sc.plane_index[slot];
+    } else {
+        PD<R> p = load_prim(&sc.prims[rec]);
+        hit_prim<R, true>(p, sc.attrs ? &sc.attrs[rec] : nullptr, o, d, tmin, tmax, &t, h);
+        h->prim = p.prim;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ bool intersect(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax, DHit<R>* h)
+{
+    R t;
+    int prim;
+    int rec = closest_hit(sc, o, d, tmin, tmax, &t, &prim);
+    if (rec == INT_MIN) return false;
+    finish_hit(sc, rec, o, d, tmin, t, h);
+    return true;
+}
+
+template <typename R>
+__device__ bool any_hit(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax)
+{
+    for (int i = 0; i < sc.n_planes; ++i) {
+        PD<R> p = load_prim(&sc.planes[i]);
+        if (occ_prim(p, o, d, tmin, tmax)) return true;
+    }
+    if (sc.root_ref == INT_MIN) return false;
+    Slab<R> sl;
+    sl.init(o, d);
+    int stk[MH_STACK];
+    int sp = 0;
+    stk[sp++] = sc.root_ref;
+    while (sp > 0) {
+        int ref = stk[--sp];
+        if (ref >= 0) {
+            DNode n = sc.nodes[ref];
+            bool hl, hr;
+            float tl, tr;
+            sl.test(n, tmin, tmax, &hl, &hr, &tl, &tr);
+            if (hl && hr) {
+                bool lfirst = tl <= tr;
+                stk[sp] = lfirst ? n.c3.y : n.c3.x;
+                stk[sp + 1] = lfirst ? n.c3.x : n.c3.y;
+                sp += 2;
+            } else if (hl) {
+                stk[sp++] = n.c3.x;
+            } else if (hr) {
+                stk[sp++] = n.c3.y;
+            }
+            if (sp > MH_STACK - 2) sp = MH_STACK - 2;
+        } else {
+            int code = ~ref;
+            int first = code >> 3, cnt = (code & 7) + 1;
+            for (int i = 0; i < cnt; ++i) {
+                PD<R> p = load_prim(&sc.prims[first + i]);
+                if (occ_prim(p, o, d, tmin, tmax)) return true;
+            }
+        }
+    }
+    return false;
+}
+
+// Normalise a direction unless it is already unit within 1e-12 in |d|^2,
+// as the reference's bvh_intersect / primitive_intersect do on entry.
+template <typename R> __device__ __forceinline__ V3<R> unit_dir(V3<R> d)
+{
+    R l2 = len2(d);
+    if (l2 > R(0) && mfabs(l2 - R(1)) > R(1e-12)) d = normalize(d);
+    return d;
+}
+
+} // namespace mh

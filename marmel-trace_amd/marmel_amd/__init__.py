@@ -1,0 +1,11 @@
+"""marmel_amd -- MI355X (gfx950) path-tracing hot path of marmel-trace.
+
+The GPU library (libmarmel_hip) and the C11 host (libmarmel_host) are native;
+this package only binds them for tests, benchmarks and Python callers.
+"""
+from . import abi, native
+from .scene import (GpuScene, HostScene, MarmelError, device_info, encode_bmp, make_camera,
+                    write_bmp)
+
+__all__ = ["abi", "native", "GpuScene", "HostScene", "MarmelError", "device_info",
+           "encode_bmp", "make_camera", "write_bmp"]

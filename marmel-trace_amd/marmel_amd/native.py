@@ -1,0 +1,115 @@
+"""Loaders for the in-tree native libraries of the package.
+
+libmarmel_hip.so (GPU path) and libmarmel_host.so (C11 host) are built by
+marmel-trace_amd/Makefile into marmel-trace_amd/lib/. Loading fails loudly
+when they are missing: there is no Python or CPU fallback for the GPU path.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+
+_hip = None
+_host = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def _load(name):
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise NativeLibraryMissing(
+            f"{path} is not built; run `make -C marmel-trace_amd` (or __graft_entry__.build())")
+    return C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+def host():
+    """libmarmel_host.so with argtypes set."""
+    global _host
+    if _host is not None:
+        return _host
+    L = _load("libmarmel_host.so")
+    P = C.c_void_p
+    _sig(L, "mt_scene_new", P, [])
+    _sig(L, "mt_scene_free", None, [P])
+    _sig(L, "mt_sky_default", None, [C.POINTER(abi.Sky)])
+    _sig(L, "mt_material_default", None, [C.POINTER(abi.Material)])
+    _sig(L, "mt_material_preset", C.c_int, [C.c_char_p, C.POINTER(abi.Material)])
+    _sig(L, "mt_scene_add_material", C.c_int, [P, C.POINTER(abi.Material)])
+    _sig(L, "mt_scene_add_prims", C.c_int, [P, P, P, C.c_int])
+    _sig(L, "mt_scene_set_sky", None, [P, C.POINTER(abi.Sky)])
+    _sig(L, "mt_scene_set_water", None, [P, C.c_double, C.c_int])
+    _sig(L, "mt_scene_counts", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)])
+    _sig(L, "mt_scene_build_bvh", C.c_int, [P, C.POINTER(BvhOptions), C.POINTER(BvhStats)])
+    _sig(L, "mt_scene_soa", C.c_int, [P, C.POINTER(abi.SceneSoa)])
+    _sig(L, "mt_camera_create", None, [C.POINTER(C.c_double)] * 3 + [C.c_double, C.c_double,
+                                                                      C.POINTER(abi.Camera)])
+    _sig(L, "mt_scene_generate", C.c_int, [P, C.POINTER(GenParams), C.POINTER(GenInfo)])
+    _sig(L, "mt_write_bmp", C.c_int, [C.c_char_p, P, C.c_int, C.c_int])
+    _sig(L, "mt_write_ppm", C.c_int, [C.c_char_p, P, C.c_int, C.c_int])
+    _sig(L, "mt_encode_bmp", C.c_long, [P, C.c_int, C.c_int, P, C.c_long])
+    _sig(L, "mt_last_error", C.c_char_p, [])
+    _host = L
+    return L
+
+
+def hip():
+    """libmarmel_hip.so with argtypes set."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    L = _load("libmarmel_hip.so")
+    P = C.c_void_p
+    _sig(L, "mh_device_info", C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_size_t)])
+    _sig(L, "mh_scene_create", C.c_int, [C.POINTER(abi.SceneSoa), C.POINTER(P)])
+    _sig(L, "mh_scene_destroy", None, [P])
+    _sig(L, "mh_scene_device_bytes", C.c_size_t, [P])
+    _sig(L, "mh_trace_closest", C.c_int, [P, P, C.c_int, C.c_double, C.c_double, C.c_int, P])
+    _sig(L, "mh_trace_any", C.c_int, [P, P, C.c_int, C.c_double, C.c_double, C.c_int, P])
+    _sig(L, "mh_pathtrace_render", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int, C.c_int,
+                                             C.c_int, P])
+    _sig(L, "mh_render_image", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int, C.c_int,
+                                         C.c_int, P])
+    _sig(L, "mh_render", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P, P])
+    _sig(L, "mh_framebuffer_download", C.c_int, [P, P, P])
+    _sig(L, "mh_last_timing", C.c_int, [C.POINTER(abi.Timing)])
+    _sig(L, "mh_last_seconds", C.c_double, [])
+    _sig(L, "mh_last_error", C.c_char_p, [])
+    _hip = L
+    return L
+
+
+class BvhOptions(C.Structure):
+    _fields_ = [("max_leaf", C.c_int), ("bins", C.c_int), ("threads", C.c_int)]
+
+
+class BvhStats(C.Structure):
+    _fields_ = [("n_nodes", C.c_int), ("n_leaves", C.c_int), ("max_depth", C.c_int),
+                ("sah_cost", C.c_double), ("build_seconds", C.c_double)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("kind", C.c_int), ("seed", C.c_uint), ("grid", C.c_int),
+                ("n_spheres", C.c_int), ("n_emissive", C.c_int), ("subdiv", C.c_int)]
+
+
+class GenInfo(C.Structure):
+    _fields_ = [("eye", C.c_double * 3), ("at", C.c_double * 3), ("up", C.c_double * 3),
+                ("vfov", C.c_double), ("n_prims", C.c_int), ("n_triangles", C.c_int),
+                ("n_spheres", C.c_int), ("n_emissive", C.c_int)]
+
+
+GEN_SPHERES, GEN_TERRAIN, GEN_ICOSPHERE = 1, 2, 3

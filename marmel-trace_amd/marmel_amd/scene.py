@@ -1,0 +1,248 @@
+"""Host scene (libmarmel_host) and device scene (libmarmel_hip) wrappers.
+
+HostScene  -- the C11 host scene: materials, primitives in file order, sky,
+              BVH build, flattened mh_scene_soa view, look-at camera.
+GpuScene   -- a scene uploaded to the GPU; render / trace entry points.
+
+This mirrors the reference's render interface (reference src/pathtrace.h:78-80
+pathtrace_render, src/render.h:29-31 render_image, src/scene.h:162-169
+scene_intersect / scene_occluded) with the same return codes.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from . import native
+
+
+class MarmelError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"code {code}: {msg}")
+        self.code = code
+
+
+class HostScene:
+    def __init__(self):
+        self._L = native.host()
+        self._h = C.c_void_p(self._L.mt_scene_new())
+        if not self._h:
+            raise MemoryError("mt_scene_new failed")
+        self.info = None
+        self.bvh_stats = None
+        self._keep = []
+
+    def close(self):
+        if self._h:
+            self._L.mt_scene_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, rc):
+        raise MarmelError(rc, self._L.mt_last_error().decode())
+
+    # ---- construction -------------------------------------------------
+    def generate(self, kind, seed=1, grid=0, n_spheres=0, n_emissive=0, subdiv=0):
+        p = native.GenParams(kind=kind, seed=seed, grid=grid, n_spheres=n_spheres,
+                             n_emissive=n_emissive, subdiv=subdiv)
+        info = native.GenInfo()
+        rc = self._L.mt_scene_generate(self._h, C.byref(p), C.byref(info))
+        if rc:
+            raise MarmelError(rc, "scene generation failed")
+        self.info = info
+        return info
+
+    def add_material(self, mat):
+        idx = self._L.mt_scene_add_material(self._h, C.byref(mat))
+        if idx < 0:
+            raise MemoryError("add_material failed")
+        return idx
+
+    def add_materials(self, mats):
+        """mats: numpy array of MATERIAL_DTYPE."""
+        mats = np.ascontiguousarray(mats, dtype=abi.MATERIAL_DTYPE)
+        first = None
+        for i in range(len(mats)):
+            m = abi.Material.from_buffer_copy(mats[i:i + 1].tobytes())
+            idx = self.add_material(m)
+            first = idx if first is None else first
+        return first
+
+    def add_prims(self, prims, attrs=None):
+        prims = np.ascontiguousarray(prims, dtype=abi.PRIM_DTYPE)
+        if attrs is not None:
+            attrs = np.ascontiguousarray(attrs, dtype=abi.ATTR_DTYPE)
+        first = self._L.mt_scene_add_prims(self._h, abi.ptr(prims), abi.ptr(attrs), len(prims))
+        if first < 0:
+            self._err(-1)
+        return first
+
+    def set_sky(self, sky):
+        self._L.mt_scene_set_sky(self._h, C.byref(sky))
+
+    def set_water(self, level, material):
+        self._L.mt_scene_set_water(self._h, level, material)
+
+    def counts(self):
+        n, m = C.c_int(), C.c_int()
+        self._L.mt_scene_counts(self._h, C.byref(n), C.byref(m))
+        return n.value, m.value
+
+    def build_bvh(self, max_leaf=4, bins=16):
+        opts = native.BvhOptions(max_leaf=max_leaf, bins=bins, threads=0)
+        st = native.BvhStats()
+        rc = self._L.mt_scene_build_bvh(self._h, C.byref(opts), C.byref(st))
+        if rc:
+            self._err(rc)
+        self.bvh_stats = st
+        return st
+
+    def soa(self):
+        s = abi.SceneSoa()
+        rc = self._L.mt_scene_soa(self._h, C.byref(s))
+        if rc:
+            self._err(rc)
+        return s
+
+    def camera(self, aspect, eye=None, at=None, up=None, vfov=None):
+        info = self.info
+        eye = eye if eye is not None else list(info.eye)
+        at = at if at is not None else list(info.at)
+        up = up if up is not None else list(info.up)
+        vfov = vfov if vfov is not None else info.vfov
+        return make_camera(eye, at, up, vfov, aspect)
+
+    # ---- numpy views of the flattened scene (for checkers) ----------------
+    def arrays(self):
+        s = self.soa()
+        n, m = s.n_prims, s.n_materials
+        prims = np.ctypeslib.as_array(C.cast(s.prims, C.POINTER(C.c_uint8)),
+                                      (n * abi.PRIM_DTYPE.itemsize,)).view(abi.PRIM_DTYPE).copy()
+        mats = np.ctypeslib.as_array(C.cast(s.materials, C.POINTER(C.c_uint8)),
+                                     (m * abi.MATERIAL_DTYPE.itemsize,)).view(abi.MATERIAL_DTYPE).copy()
+        attrs = None
+        if s.attrs:
+            attrs = np.ctypeslib.as_array(C.cast(s.attrs, C.POINTER(C.c_uint8)),
+                                          (n * abi.ATTR_DTYPE.itemsize,)).view(abi.ATTR_DTYPE).copy()
+        return prims, attrs, mats, s.sky, s.water_level, s.water_material
+
+
+def make_camera(eye, at, up, vfov, aspect):
+    L = native.host()
+    cam = abi.Camera()
+    e = (C.c_double * 3)(*eye)
+    a = (C.c_double * 3)(*at)
+    u = (C.c_double * 3)(*up)
+    L.mt_camera_create(e, a, u, vfov, aspect, C.byref(cam))
+    return cam
+
+
+def write_bmp(path, rgb):
+    h, w, _ = rgb.shape
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    rc = native.host().mt_write_bmp(path.encode(), abi.ptr(rgb), w, h)
+    if rc:
+        raise MarmelError(rc, native.host().mt_last_error().decode())
+
+
+def encode_bmp(rgb):
+    h, w, _ = rgb.shape
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    L = native.host()
+    n = L.mt_encode_bmp(abi.ptr(rgb), w, h, None, 0)
+    buf = np.zeros(n, np.uint8)
+    L.mt_encode_bmp(abi.ptr(rgb), w, h, abi.ptr(buf), n)
+    return buf.tobytes()
+
+
+class GpuScene:
+    """A scene resident in GPU memory (mh_scene_create)."""
+
+    def __init__(self, soa):
+        self._L = native.hip()
+        h = C.c_void_p()
+        rc = self._L.mh_scene_create(C.byref(soa), C.byref(h))
+        if rc:
+            raise MarmelError(rc, self._L.mh_last_error().decode())
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._L.mh_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise MarmelError(rc, self._L.mh_last_error().decode())
+
+    @property
+    def device_bytes(self):
+        return self._L.mh_scene_device_bytes(self._h)
+
+    def render(self, cam, width, height, spp, depth, mode=abi.MODE_PATHTRACE,
+               precision=abi.PREC_FP32, sample_offset=0, fetch=True):
+        p = abi.RenderParams(width=width, height=height, spp=spp, max_depth=depth, mode=mode,
+                             precision=precision, sample_offset=sample_offset, block_size=0)
+        rgb = lin = None
+        if fetch:
+            rgb = np.zeros((height, width, 3), np.uint8)
+            lin = np.zeros((height, width, 3), np.float32)
+        rc = self._L.mh_render(self._h, C.byref(cam), C.byref(p), abi.ptr(rgb), abi.ptr(lin))
+        self._check(rc)
+        return rgb, lin
+
+    def pathtrace_render(self, cam, width, height, spp, depth):
+        """Drop-in form of reference pathtrace_render: returns (code, rgb)."""
+        rgb = np.zeros((height, width, 3), np.uint8)
+        rc = self._L.mh_pathtrace_render(self._h, C.byref(cam), width, height, spp, depth,
+                                         abi.ptr(rgb))
+        return rc, rgb
+
+    def render_image(self, cam, width, height, spp, depth):
+        """Drop-in form of reference render_image (fixed-spp Whitted)."""
+        rgb = np.zeros((height, width, 3), np.uint8)
+        rc = self._L.mh_render_image(self._h, C.byref(cam), width, height, spp, depth,
+                                     abi.ptr(rgb))
+        return rc, rgb
+
+    def trace_closest(self, rays, tmin=1e-4, tmax=1e30, precision=abi.PREC_FP32):
+        rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
+        out = np.zeros(len(rays), abi.HIT_DTYPE)
+        self._check(self._L.mh_trace_closest(self._h, abi.ptr(rays), len(rays), tmin, tmax,
+                                             precision, abi.ptr(out)))
+        return out
+
+    def trace_any(self, rays, tmin=1e-3, tmax=1e30, precision=abi.PREC_FP32):
+        rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
+        out = np.zeros(len(rays), np.uint8)
+        self._check(self._L.mh_trace_any(self._h, abi.ptr(rays), len(rays), tmin, tmax,
+                                         precision, abi.ptr(out)))
+        return out
+
+    @staticmethod
+    def last_timing():
+        t = abi.Timing()
+        native.hip().mh_last_timing(C.byref(t))
+        return t
+
+
+def device_info():
+    L = native.hip()
+    name = C.create_string_buffer(256)
+    cu = C.c_int()
+    mem = C.c_size_t()
+    rc = L.mh_device_info(name, 256, C.byref(cu), C.byref(mem))
+    if rc:
+        raise MarmelError(rc, L.mh_last_error().decode())
+    return name.value.decode(), cu.value, mem.value

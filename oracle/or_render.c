@@ -1,0 +1,945 @@
+/*
+ * or_render.c - oracle: the unidirectional path tracer, the fixed-spp
+ * Whitted renderer, camera rays, tone map, the pthread tile pool and the
+ * public liboracle API. FP64 restatement; TEST INFRASTRUCTURE ONLY.
+ *
+ * Follows:
+ *   camera                 reference src/camera.c:24-134
+ *   PT lobe sampling       reference src/pathtrace.c:141-200, 220-233, 251-453
+ *   PT NEE + shadow        reference src/pathtrace.c:504-606
+ *   PT radiance            reference src/pathtrace.c:800-1028 (fog and the
+ *                          underwater in-scatter march are off: fog density 0)
+ *   PT pixel + pool        reference src/pathtrace.c:1047-1096, 1134-1234
+ *   tone map               reference src/pathtrace.c:464-480, src/render.c:892-907
+ *   Whitted                reference src/render.c:507-863 (trace/trace_hit,
+ *                          emissive_direct), 874-889 (sample_offset),
+ *                          924-970 (render_region)
+ */
+#define _POSIX_C_SOURCE 200809L
+
+#include "oracle.h"
+#include "or_internal.h"
+
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+/* ---- helpers --------------------------------------------------------- */
+
+static double cl01n(double v) /* NaN -> 0 */
+{
+    if (!(v > 0.0)) return 0.0;
+    if (v > 1.0) return 1.0;
+    return v;
+}
+
+static double clamp_n(double v, double lo, double hi)
+{
+    if (!(v > lo)) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
+static double max_comp(dv3 v)
+{
+    double m = v.x;
+    if (v.y > m) m = v.y;
+    if (v.z > m) m = v.z;
+    if (!(m > 0.0)) m = 0.0;
+    return m;
+}
+
+static double luma(dv3 c)
+{
+    double y = 0.2126 * c.x + 0.7152 * c.y + 0.0722 * c.z;
+    return y > 0.0 ? y : 0.0;
+}
+
+static const mh_material *mat_of(const or_scene *s, int i)
+{
+    if (i < 0 || i >= s->n_mats) return NULL;
+    return &s->mats[i];
+}
+
+unsigned char or_to_byte(double c)
+{
+    if (!(c >= 0.0)) c = 0.0;
+    else if (c > 1.0) c = 1.0;
+    int iv = (int)(pow(c, 1.0 / 2.2) * 255.0 + 0.5);
+    if (iv < 0) iv = 0;
+    else if (iv > 255) iv = 255;
+    return (unsigned char)iv;
+}
+
+/* ---- camera ---------------------------------------------------------- */
+
+void or_camera_create(const double eye[3], const double at[3], const double up[3],
+                      double vfov, double aspect, mh_camera *out)
+{
+    dv3 from = d3(eye[0], eye[1], eye[2]), to = d3(at[0], at[1], at[2]);
+    dv3 upv = d3(up[0], up[1], up[2]);
+    dv3 fwd = dnorm(dsub(to, from));
+    if (dlen2(fwd) <= 1e-12) fwd = d3(0.0, 0.0, -1.0);
+    dv3 right = dcross(fwd, upv);
+    if (dlen2(right) <= 1e-12) {
+        dv3 alt = fabs(fwd.y) < 0.99 ? d3(0.0, 1.0, 0.0) : d3(1.0, 0.0, 0.0);
+        right = dcross(fwd, alt);
+    }
+    right = dnorm(right);
+    dv3 tup = dcross(right, fwd);
+    double hh = tan(vfov * (OR_PI / 180.0) * 0.5);
+    double hw = aspect * hh;
+    dv3 hor = dscale(right, 2.0 * hw), ver = dscale(tup, 2.0 * hh);
+    dv3 ll = dsub(dadd(from, fwd), dadd(dscale(hor, 0.5), dscale(ver, 0.5)));
+    out->position = (mh_vec3){ from.x, from.y, from.z };
+    out->horizontal = (mh_vec3){ hor.x, hor.y, hor.z };
+    out->vertical = (mh_vec3){ ver.x, ver.y, ver.z };
+    out->lower_left = (mh_vec3){ ll.x, ll.y, ll.z };
+    out->aperture = 0.0;
+    out->focus_distance = dlen(dsub(to, from));
+    out->dome_radius = 0.0;
+    out->anamorphic_squeeze = 1.0;
+}
+
+static oray cam_ray(const mh_camera *c, double u, double v, double r1, double r2)
+{
+    dv3 pos = fromm(c->position), hor = fromm(c->horizontal), ver = fromm(c->vertical);
+    dv3 pt = dadd(fromm(c->lower_left), dadd(dscale(hor, u), dscale(ver, v)));
+    oray pin = { pos, dnorm(dsub(pt, pos)) };
+    if (c->aperture <= 1e-12) return pin;
+    double fd = c->focus_distance;
+    if (fd <= 0.0) fd = 1.0;
+    dv3 focal = dadd(pos, dscale(pin.d, fd));
+    dv3 rt = dnorm(hor), upv = dnorm(ver);
+    double rad = c->aperture * sqrt(r1);
+    double th = 2.0 * OR_PI * r2;
+    double sq = c->anamorphic_squeeze;
+    dv3 off;
+    if (sq <= 0.0 || fabs(sq - 1.0) < 1e-12)
+        off = dadd(dscale(rt, rad * cos(th)), dscale(upv, rad * sin(th)));
+    else
+        off = dadd(dscale(rt, (rad / sq) * cos(th)), dscale(upv, rad * sin(th)));
+    oray r;
+    r.o = dadd(pos, off);
+    r.d = dnorm(dsub(focal, r.o));
+    return r;
+}
+
+void or_camera_ray(const mh_camera *cam, double u, double v, double r1, double r2, double out[6])
+{
+    oray r = cam_ray(cam, u, v, r1, r2);
+    out[0] = r.o.x; out[1] = r.o.y; out[2] = r.o.z;
+    out[3] = r.d.x; out[4] = r.d.y; out[5] = r.d.z;
+}
+
+/* ---- path tracer ----------------------------------------------------- */
+
+#define K_BOUNCE 0x5054u
+#define CH_U1 0xB501u
+#define CH_U2 0xB502u
+#define CH_LOBE 0x1001u
+#define CH_GLASS 0x1002u
+#define CH_RR 0x5201u
+#define CH_SUN_A 0xA101u
+#define CH_SUN_B 0xA102u
+#define CH_EM_A 0xA201u
+#define CH_EM_B 0xA202u
+
+static void apply_medium(const mh_material *m, double dist, dv3 *beta, dv3 *rad)
+{
+    dv3 white = or_water_attenuate(m, d3(1.0, 1.0, 1.0), dist);
+    dv3 deep = or_water_attenuate(m, d3(0.0, 0.0, 0.0), dist);
+    dv3 T = dsub(white, deep);
+    *rad = dadd(*rad, dmul(*beta, deep));
+    *beta = dmul(*beta, T);
+}
+
+static double pt_smith(double ndl, double ndv, double rough)
+{
+    double k = cl01n((rough + 1.0) * (rough + 1.0) / 8.0);
+    return (ndl / (ndl * (1.0 - k) + k)) * (ndv / (ndv * (1.0 - k) + k));
+}
+
+static int sample_pbr(const mh_material *m, dv3 alb, dv3 N, dv3 d, dv3 V, double u1, double u2,
+                      double ul, dv3 *wi_o, dv3 *w_o, int *spec)
+{
+    *spec = 0;
+    double met = cl01n(m->metallic), rough = cl01n(m->roughness);
+    double alpha = or_roughness_to_alpha(rough);
+    dv3 f0 = dlerp(d3(0.04, 0.04, 0.04), alb, met);
+    double ps = or_specular_probability(met, rough, luma(f0));
+    double ndv = ddot(N, V);
+    if (!(ndv > 0.0)) return 0;
+    if (ul < ps) {
+        dv3 H = or_ggx_half(N, alpha, u1, u2);
+        dv3 wi = dreflect(d, H);
+        double ndl = ddot(N, wi), ndh = ddot(N, H), vdh = ddot(V, H);
+        if (!(ndl > 0.0) || !(ndh > 0.0) || !(vdh > 0.0)) return 0;
+        double D = or_d_ggx(ndh, alpha);
+        double G = pt_smith(ndl, ndv, rough);
+        dv3 F = or_fresnel_schlick_rgb(vdh, f0);
+        double pdf = or_ggx_pdf(ndh, vdh, alpha);
+        if (!(pdf > 0.0) || !(ps > 0.0)) return 0;
+        *wi_o = wi;
+        *w_o = dscale(F, D * G / (4.0 * ndv * pdf * ps));
+        *spec = 1;
+        return 1;
+    }
+    double den = 1.0 - ps;
+    if (!(den > 0.0)) return 0;
+    dv3 wi = or_cosine_hemisphere(N, u1, u2);
+    if (!(ddot(N, wi) > 0.0)) return 0;
+    dv3 F = or_fresnel_schlick_rgb(ndv, f0);
+    dv3 kd = dscale(dsub(d3(1.0, 1.0, 1.0), F), 1.0 - met);
+    *wi_o = wi;
+    *w_o = dscale(dmul(kd, alb), 1.0 / den);
+    return 1;
+}
+
+static int sample_legacy(const mh_material *m, dv3 alb, dv3 N, dv3 d, dv3 V, int front,
+                         double u1, double u2, double ul, double ug,
+                         dv3 *wi_o, dv3 *w_o, int *refr, int *spec)
+{
+    *refr = 0;
+    *spec = 0;
+    double rm = cl01n(m->reflectivity), rt = cl01n(m->transparency);
+    double rd = 1.0 - rm - rt;
+    if (rd < 0.0) rd = 0.0;
+    double pm = rm, pt = rt, pd = 1.0 - pm - pt, norm = 1.0;
+    if (pd < 0.0) {
+        double s = pm + pt;
+        if (s > 0.0) {
+            pm /= s;
+            pt /= s;
+            if (s > 1.0) norm = s;
+        }
+        pd = 0.0;
+    }
+    if (ul < pm) {
+        if (!(pm > 0.0)) return 0;
+        double w = rm / (pm * norm);
+        *wi_o = dreflect(d, N);
+        *w_o = d3(w, w, w);
+        *spec = 1;
+        return 1;
+    }
+    if (ul < pm + pt) {
+        if (!(pt > 0.0)) return 0;
+        *spec = 1;
+        double ior = m->ior > 0.0 ? m->ior : 1.0;
+        double eta = front ? 1.0 / ior : ior;
+        double ci = cl01n(ddot(N, V));
+        double F = or_fresnel_dielectric(ci, eta);
+        double lobe = rt / (pt * norm);
+        if (F >= 1.0) {
+            *wi_o = dreflect(d, N);
+            *w_o = d3(lobe, lobe, lobe);
+            return 1;
+        }
+        double pr = clamp_n(F, 0.05, 0.95);
+        if (ug < pr) {
+            double w = lobe * (F / pr);
+            *wi_o = dreflect(d, N);
+            *w_o = d3(w, w, w);
+            return 1;
+        }
+        dv3 wi;
+        if (!or_refract(d, N, eta, &wi)) {
+            double w = lobe * (F / pr);
+            *wi_o = dreflect(d, N);
+            *w_o = d3(w, w, w);
+            return 1;
+        }
+        double w = lobe * ((1.0 - F) / (1.0 - pr)) * eta * eta;
+        *wi_o = wi;
+        *w_o = d3(w, w, w);
+        *refr = 1;
+        return 1;
+    }
+    if (!(pd > 0.0)) return 0;
+    dv3 wi = or_cosine_hemisphere(N, u1, u2);
+    if (!(ddot(N, wi) > 0.0)) return 0;
+    *wi_o = wi;
+    *w_o = dscale(alb, rd / pd);
+    return 1;
+}
+
+/* Shadow connection through up to 6 closest hits; water and near-clear
+ * (transparency > 0.8) surfaces pass light with Beer-Lambert attenuation. */
+static int occluded_segment(const or_scene *s, oray r, double tmin, double tmax, dv3 *att)
+{
+    *att = d3(1.0, 1.0, 1.0);
+    oray cur = r;
+    double rem = tmax;
+    for (int step = 0; step < 6; ++step) {
+        ohit h;
+        if (!or_intersect1(s, cur, tmin, rem, &h)) return 0;
+        const mh_material *m = mat_of(s, h.material);
+        if (!m) return 1;
+        if (m->is_water || m->transparency > 0.8) {
+            if (m->beer_lambert || m->is_water) {
+                dv3 seg = d3(exp(-m->absorption.x * h.t), exp(-m->absorption.y * h.t),
+                             exp(-m->absorption.z * h.t));
+                *att = dmul(*att, seg);
+            }
+            cur.o = dadd(h.point, dscale(cur.d, 1e-3));
+            rem -= h.t;
+            if (rem <= 1e-3) return 0;
+            tmin = 1e-3;
+            continue;
+        }
+        return 1;
+    }
+    return 1;
+}
+
+static dv3 shade(const mh_material *m, dv3 N, dv3 L, dv3 V, dv3 c)
+{
+    return m->pbr ? or_shade_pbr(m, N, L, V, c) : or_shade_local(m, N, L, V, c);
+}
+
+static dv3 nee_sun(const or_scene *s, const mh_material *mm, dv3 P, dv3 N, dv3 V, dv3 beta,
+                   unsigned seed, int b)
+{
+    double r1 = or_rand01(seed, (unsigned)b, CH_SUN_A);
+    double r2 = or_rand01(seed, (unsigned)b, CH_SUN_B);
+    dv3 L = or_sun_disk_dir(fromm(s->sky.sun_dir), s->sky.sun_radius, r1, r2);
+    if (!dfinite(L) || ddot(N, L) <= 0.0) return d3(0.0, 0.0, 0.0);
+    oray sr = { dadd(P, dscale(N, 1e-3)), L };
+    dv3 att;
+    if (occluded_segment(s, sr, 1e-3, 1e30, &att)) return d3(0.0, 0.0, 0.0);
+    if (att.x < 1e-4 && att.y < 1e-4 && att.z < 1e-4) return d3(0.0, 0.0, 0.0);
+    double rad = s->sky.sun_radius * OR_PI / 180.0;
+    double omega = s->sky.sun_radius > 0.0 ? 2.0 * OR_PI * (1.0 - cos(rad)) : 1.0;
+    dv3 lit = shade(mm, N, L, V, dmul(fromm(s->sky.sun_color), att));
+    return dmul(beta, dscale(lit, omega));
+}
+
+static dv3 nee_emissive(const or_scene *s, const mh_material *mm, dv3 P, dv3 N, dv3 V,
+                        dv3 beta, int self, unsigned seed, int b)
+{
+    dv3 sum = d3(0.0, 0.0, 0.0);
+    for (int li = 0; li < s->n_lights; ++li) {
+        const olight *lt = &s->lights[li];
+        if (lt->prim == self) continue;
+        dv3 toc = dsub(lt->center, P);
+        double dc2 = dlen2(toc), dc = sqrt(dc2);
+        if (dc <= lt->radius) continue;
+        double cmx = sqrt(1.0 - (lt->radius * lt->radius) / dc2);
+        double omega = 2.0 * OR_PI * (1.0 - cmx);
+        if (!(omega > 0.0)) continue;
+        dv3 w = dscale(toc, 1.0 / dc);
+        double u1 = or_rand01(seed ^ (unsigned)li, (unsigned)b, CH_EM_A);
+        double u2 = or_rand01(seed ^ (unsigned)li, (unsigned)b, CH_EM_B);
+        dv3 wi = or_light_cone_dir(w, cmx, u1, u2);
+        if (!dfinite(wi) || ddot(N, wi) <= 0.0) continue;
+        oray sr = { dadd(P, dscale(N, 1e-3)), wi };
+        ohit lh;
+        oray srn = sr;
+        double l2 = dlen2(srn.d);
+        if (l2 > 0.0 && fabs(l2 - 1.0) > 1e-12) srn.d = dnorm(srn.d);
+        if (!or_prim_intersect(&s->prims[lt->prim], srn, 1e-3, 1e30, &lh)) continue;
+        dv3 att;
+        if (occluded_segment(s, sr, 1e-3, lh.t - 1e-3, &att)) continue;
+        dv3 lit = shade(mm, N, wi, V, dmul(lt->emissive, att));
+        sum = dadd(sum, dmul(beta, dscale(lit, omega)));
+    }
+    return sum;
+}
+
+static dv3 limb_glow(const mh_material *mm, dv3 N, dv3 V, dv3 sun)
+{
+    double cv = fmax(0.0, ddot(N, V));
+    double limb = pow(1.0 - cv, 3.5);
+    double sd = ddot(N, sun);
+    double day = 0.0;
+    if (sd > -0.15) {
+        day = (sd + 0.15) / 0.35;
+        if (day > 1.0) day = 1.0;
+    }
+    dv3 g = fromm(mm->atmosphere_glow);
+    if (sd > -0.10 && sd < 0.25) {
+        double st = 1.0 - fabs(sd - 0.05) / 0.20;
+        if (st > 0.0) g = dlerp(g, d3(1.0, 0.45, 0.15), st * 0.6);
+    }
+    return dscale(g, limb * day * 2.5);
+}
+
+static dv3 pt_radiance(const or_scene *s, oray primary, int max_depth, unsigned seed)
+{
+    dv3 L = d3(0.0, 0.0, 0.0);
+    if (max_depth < 0) max_depth = 0;
+    oray r = { primary.o, dnorm(primary.d) };
+    if (dlen2(r.d) <= 0.0) return L;
+    dv3 beta = d3(1.0, 1.0, 1.0);
+    const mh_material *medium = NULL;
+    if (s->water_material >= 0 && primary.o.y < s->water_level) medium = mat_of(s, s->water_material);
+    int last_spec = 1;
+    for (int b = 0; b <= max_depth; ++b) {
+        ohit h;
+        if (!or_intersect1(s, r, 1e-4, 1e30, &h)) {
+            if (medium) {
+                apply_medium(medium, 5.0, &beta, &L);
+                medium = NULL;
+            }
+            L = dadd(L, dmul(beta, or_sky_sample(r.d, &s->sky)));
+            break;
+        }
+        if (medium) apply_medium(medium, h.t, &beta, &L);
+        const mh_material *m = mat_of(s, h.material);
+        if (!m) {
+            L = dadd(L, dmul(beta, or_sky_sample(r.d, &s->sky)));
+            break;
+        }
+        dv3 P = h.point, d = r.d, N = h.normal;
+        if (m->is_water && fabs(h.normal.y) > 0.999 && fabs(h.normal.x) < 1e-4 &&
+            fabs(h.normal.z) < 1e-4)
+            N = or_water_normal(P.x, P.z, 0.0);
+        if (ddot(N, d) > 0.0) N = dneg(N);
+        N = dnorm(N);
+        if (dlen2(N) <= 0.0) break;
+        dv3 V = dneg(d);
+        mh_material ml = *m;
+        dv3 alb = or_texture_albedo(m, P, h.u, h.v);
+        ml.albedo = (mh_vec3){ alb.x, alb.y, alb.z };
+        if (b == 0 || last_spec) L = dadd(L, dmul(beta, fromm(ml.emissive)));
+        if (b == 0 && dlen2(fromm(ml.atmosphere_glow)) > 1e-6)
+            L = dadd(L, dmul(beta, limb_glow(&ml, N, V, fromm(s->sky.sun_dir))));
+        L = dadd(L, nee_emissive(s, &ml, P, N, V, beta, h.prim, seed, b));
+        if (b == 0 || (b == 1 && medium != NULL && last_spec))
+            L = dadd(L, nee_sun(s, &ml, P, N, V, beta, seed, b));
+        unsigned kb = or_hash3(seed ^ K_BOUNCE, (unsigned)b, 0u);
+        double u1 = or_rand01(kb, (unsigned)b, CH_U1);
+        double u2 = or_rand01(kb, (unsigned)b, CH_U2);
+        double ul = or_rand01(kb, (unsigned)b, CH_LOBE);
+        double ug = or_rand01(kb, (unsigned)b, CH_GLASS);
+        dv3 wi, w;
+        int ok, refr = 0, spec = 0;
+        if (ml.pbr) ok = sample_pbr(&ml, alb, N, d, V, u1, u2, ul, &wi, &w, &spec);
+        else ok = sample_legacy(&ml, alb, N, d, V, h.front, u1, u2, ul, ug, &wi, &w, &refr, &spec);
+        if (!ok) break;
+        last_spec = spec;
+        beta = dmul(beta, w);
+        if (!dfinite(beta) || !(max_comp(beta) > 0.0)) break;
+        if (refr && (m->is_water || m->beer_lambert)) medium = h.front ? m : NULL;
+        if (b >= 2) {
+            double q = cl01n(max_comp(beta));
+            if (!(q > 0.0)) break;
+            if (or_rand01(kb, (unsigned)b, CH_RR) >= q) break;
+            beta = dscale(beta, 1.0 / q);
+        }
+        dv3 off = refr ? dscale(wi, 1e-4) : dscale(N, ddot(wi, N) >= 0.0 ? 1e-4 : -1e-4);
+        r.o = dadd(P, off);
+        r.d = wi;
+    }
+    if (!dfinite(L)) return d3(0.0, 0.0, 0.0);
+    if (L.x < 0.0) L.x = 0.0;
+    if (L.y < 0.0) L.y = 0.0;
+    if (L.z < 0.0) L.z = 0.0;
+    return L;
+}
+
+/* ---- Whitted --------------------------------------------------------- */
+
+static dv3 w_trace(const or_scene *s, oray r, int depth, int maxd, unsigned seed);
+
+static dv3 emissive_direct(const or_scene *s, const mh_material *mm, dv3 P, dv3 N, dv3 V,
+                           int self, unsigned seed)
+{
+    dv3 sum = d3(0.0, 0.0, 0.0);
+    dv3 so = dadd(P, dscale(N, 1e-3));
+    for (int li = 0; li < s->n_lights; ++li) {
+        const olight *lt = &s->lights[li];
+        if (lt->prim == self) continue;
+        dv3 toc = dsub(lt->center, P);
+        double dc2 = dlen2(toc), dc = sqrt(dc2);
+        if (dc <= lt->radius) continue;
+        double cmx = sqrt(1.0 - (lt->radius * lt->radius) / dc2);
+        double solid = 2.0 * OR_PI * (1.0 - cmx);
+        if (!(solid > 0.0)) continue;
+        dv3 w = dscale(toc, 1.0 / dc);
+        for (int si = 0; si < 16; ++si) {
+            unsigned k = or_hash3(seed ^ 0xE311u, (unsigned)li, (unsigned)si);
+            double u1 = or_rand01(k, (unsigned)si, 0x4c11u);
+            double u2 = or_rand01(k, (unsigned)si, 0x9d27u);
+            dv3 wi = or_light_cone_dir(w, cmx, u1, u2);
+            if (ddot(N, wi) <= 0.0) continue;
+            oray sr = { so, wi };
+            oray srn = sr;
+            double l2 = dlen2(srn.d);
+            if (l2 > 0.0 && fabs(l2 - 1.0) > 1e-12) srn.d = dnorm(srn.d);
+            ohit lh;
+            int vis = 0;
+            if (or_prim_intersect(&s->prims[lt->prim], srn, 1e-3, 1e30, &lh))
+                vis = !or_occluded1(s, sr, 1e-3, lh.t - 1e-3);
+            if (!vis) continue;
+            sum = dadd(sum, shade(mm, N, wi, V, dscale(lt->emissive, solid)));
+        }
+    }
+    return dscale(sum, 1.0 / 16.0);
+}
+
+static dv3 w_trace_hit(const or_scene *s, oray r, int depth, int maxd, const ohit *h, unsigned seed)
+{
+    if (depth > maxd) return d3(0.0, 0.0, 0.0);
+    const mh_material *m = mat_of(s, h->material);
+    if (!m) return or_sky_sample(r.d, &s->sky);
+    dv3 P = h->point, N = h->normal;
+    if (m->is_water) N = or_water_normal(P.x, P.z, 0.0);
+    if (ddot(N, r.d) > 0) N = dneg(N);
+    dv3 V = dneg(r.d);
+    mh_material ml = *m;
+    dv3 alb = or_texture_albedo(m, P, h->u, h->v);
+    ml.albedo = (mh_vec3){ alb.x, alb.y, alb.z };
+    const mh_material *mm = &ml;
+
+    dv3 color = or_ambient(mm, N, &s->sky);
+    dv3 L = fromm(s->sky.sun_dir), sunc = fromm(s->sky.sun_color);
+    dv3 so = dadd(P, dscale(N, 1e-3));
+    double srad = s->sky.sun_radius;
+    if (srad > 1e-9) {
+        double lit = 0.0;
+        for (int i = 0; i < 16; ++i) {
+            unsigned k = seed * 0x9e3779b9u + (unsigned)i * 2u;
+            double r1 = or_rand01(k, (unsigned)i, 0x5a17u);
+            double r2 = or_rand01(k, (unsigned)i, 0x7c3du);
+            dv3 Ld = or_sun_disk_dir(L, srad, r1, r2);
+            oray sr = { so, Ld };
+            if (!or_occluded1(s, sr, 1e-3, 1e30)) lit += 1.0;
+        }
+        if (lit > 0.0) color = dadd(color, dscale(shade(mm, N, L, V, sunc), lit / 16.0));
+    } else {
+        oray sr = { so, L };
+        if (!or_occluded1(s, sr, 1e-3, 1e30)) color = dadd(color, shade(mm, N, L, V, sunc));
+    }
+    if (s->n_lights > 0) color = dadd(color, emissive_direct(s, mm, P, N, V, h->prim, seed));
+
+    dv3 refl = d3(0.0, 0.0, 0.0);
+    if (depth < maxd) {
+        if (mm->pbr != 0 && m->reflectivity > 0.0 && m->roughness > 1e-6) {
+            const double inv = 1.0 / 16.0;
+            for (int i = 0; i < 16; ++i) {
+                unsigned k = seed * 0x9e3779b9u + (unsigned)depth * 0x85ebca6bu + (unsigned)i * 2u;
+                double r1 = or_rand01(k, (unsigned)i, 0x9e37u);
+                double r2 = or_rand01(k, (unsigned)i, 0x85ebu);
+                dv3 rd = or_sample_glossy_dir(mm, N, r.d, r1, r2);
+                oray rr = { dadd(P, dscale(N, 1e-3)), rd };
+                refl = dadd(refl, dscale(w_trace(s, rr, depth + 1, maxd, seed), inv));
+            }
+        } else {
+            oray rr = { dadd(P, dscale(N, 1e-3)), dreflect(r.d, N) };
+            refl = w_trace(s, rr, depth + 1, maxd, seed);
+        }
+    }
+    double f0 = (1.0 - m->ior) / (1.0 + m->ior);
+    f0 = f0 * f0;
+    double ci = ddot(N, V);
+    if (ci < 0.0) ci = 0.0;
+    else if (ci > 1.0) ci = 1.0;
+    double eta = h->front == 1 ? 1.0 / m->ior : m->ior;
+    double s2t = eta * eta * (1.0 - ci * ci);
+    double F;
+    int tir = 0;
+    if (s2t > 1.0) {
+        F = 1.0;
+        tir = 1;
+    } else {
+        /* material.c Schlick: cos clamped to [0,1], f0 unclamped */
+        double c = or_clamp01(sqrt(1.0 - s2t)), mm5 = 1.0 - c, m2 = mm5 * mm5;
+        F = f0 + (1.0 - f0) * (m2 * m2 * mm5);
+    }
+    if (m->transparency > 0.0) {
+        dv3 refr;
+        if (tir) {
+            refr = refl;
+        } else {
+            double cosi = -ddot(r.d, N);
+            double s2 = eta * eta * (1.0 - cosi * cosi);
+            dv3 rd;
+            if (s2 > 1.0) {
+                rd = d3(0.0, 0.0, 0.0);
+            } else {
+                double ct = sqrt(1.0 - s2), k = eta * cosi - ct;
+                rd = d3(eta * r.d.x + k * N.x, eta * r.d.y + k * N.y, eta * r.d.z + k * N.z);
+            }
+            oray rr = { dadd(P, dscale(rd, 1e-3)), rd };
+            ohit h2;
+            if (or_intersect1(s, rr, 1e-4, 1e30, &h2)) {
+                dv3 inner = w_trace_hit(s, rr, depth + 1, maxd, &h2, seed);
+                refr = (m->is_water || m->beer_lambert) ? or_water_attenuate(m, inner, h2.t) : inner;
+            } else {
+                dv3 inner = or_sky_sample(rd, &s->sky);
+                refr = (m->is_water || m->beer_lambert) ? or_water_attenuate(m, inner, 5.0) : inner;
+            }
+        }
+        dv3 surf = dadd(dscale(refl, F), dscale(refr, 1.0 - F));
+        dv3 out = dlerp(color, surf, m->transparency);
+        if (mm->pbr && (m->emissive.x != 0.0 || m->emissive.y != 0.0 || m->emissive.z != 0.0))
+            out = dadd(out, fromm(m->emissive));
+        return out;
+    }
+    dv3 out = dlerp(color, refl, m->reflectivity);
+    if (mm->pbr && (m->emissive.x != 0.0 || m->emissive.y != 0.0 || m->emissive.z != 0.0))
+        out = dadd(out, fromm(m->emissive));
+    if (depth == 0 && dlen2(fromm(mm->atmosphere_glow)) > 1e-6)
+        out = dadd(out, limb_glow(mm, N, V, fromm(s->sky.sun_dir)));
+    return out;
+}
+
+static dv3 w_trace(const or_scene *s, oray r, int depth, int maxd, unsigned seed)
+{
+    if (depth > maxd) return d3(0.0, 0.0, 0.0);
+    ohit h;
+    if (!or_intersect1(s, r, 1e-4, 1e30, &h)) return or_sky_sample(r.d, &s->sky);
+    return w_trace_hit(s, r, depth, maxd, &h, seed);
+}
+
+/* ---- pixel kernels ---------------------------------------------------- */
+
+typedef struct {
+    const or_scene *s;
+    const mh_camera *cam;
+    int w, h, spp, depth, mode, s0, y0, y1;
+    unsigned char *rgb;
+    double *mean, *var;
+    int tiles_x, tiles_y;
+    atomic_int next;
+    pthread_mutex_t lock;
+    long long stats[5];
+} job_t;
+
+static void render_pixel(job_t *j, int x, int y)
+{
+    const or_scene *s = j->s;
+    dv3 acc = d3(0.0, 0.0, 0.0), acc2 = d3(0.0, 0.0, 0.0);
+    int g = (int)(sqrt((double)j->spp) + 0.5);
+    for (int si = 0; si < j->spp; ++si) {
+        int sidx = j->s0 + si;
+        dv3 c;
+        if (j->mode == 0) {
+            double jx = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 0));
+            double jy = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 1));
+            double l1 = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 100));
+            double l2 = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 101));
+            double uu = ((double)x + jx) / (double)j->w;
+            double vv = 1.0 - ((double)y + jy) / (double)j->h;
+            oray ray = cam_ray(j->cam, uu, vv, l1, l2);
+            if (j->cam->dome_radius > 1e-4 && s->water_material >= 0) {
+                dv3 pd = dadd(ray.o, dscale(ray.d, j->cam->dome_radius));
+                if (pd.y < s->water_level) ray.o = dsub(pd, d3(0.0, 1e-3, 0.0));
+            }
+            c = pt_radiance(s, ray, j->depth, or_hash3((unsigned)x, (unsigned)y, (unsigned)sidx));
+        } else {
+            double du, dv;
+            if (g > 0 && g * g == j->spp) {
+                int gi = sidx % g, gj = sidx / g;
+                double jx = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 0));
+                double jy = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 1));
+                du = ((double)gi + jx) / (double)g;
+                dv = ((double)gj + jy) / (double)g;
+            } else {
+                du = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 0));
+                dv = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 1));
+            }
+            double uu = ((double)x + du) / (double)j->w;
+            double vv = 1.0 - ((double)y + dv) / (double)j->h;
+            double l1 = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 2));
+            double l2 = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 3));
+            oray ray = cam_ray(j->cam, uu, vv, l1, l2);
+            c = w_trace(s, ray, 0, j->depth, or_hash3((unsigned)x, (unsigned)y, (unsigned)sidx));
+        }
+        acc = dadd(acc, c);
+        acc2 = dadd(acc2, dmul(c, c));
+    }
+    dv3 m = dscale(acc, 1.0 / (double)j->spp);
+    size_t o = ((size_t)y * (size_t)j->w + (size_t)x) * 3;
+    if (j->rgb) {
+        j->rgb[o] = or_to_byte(m.x);
+        j->rgb[o + 1] = or_to_byte(m.y);
+        j->rgb[o + 2] = or_to_byte(m.z);
+    }
+    if (j->mean) {
+        j->mean[o] = m.x;
+        j->mean[o + 1] = m.y;
+        j->mean[o + 2] = m.z;
+    }
+    if (j->var) {
+        double n = (double)j->spp;
+        double k = n > 1.0 ? 1.0 / (n - 1.0) : 0.0;
+        j->var[o] = (acc2.x - acc.x * m.x) * k;
+        j->var[o + 1] = (acc2.y - acc.y * m.y) * k;
+        j->var[o + 2] = (acc2.z - acc.z * m.z) * k;
+    }
+}
+
+#define TILE 16
+
+static void *worker(void *arg)
+{
+    job_t *j = (job_t *)arg;
+    int nt = j->tiles_x * j->tiles_y;
+    or_stats_reset();
+    for (;;) {
+        int t = atomic_fetch_add(&j->next, 1);
+        if (t >= nt) break;
+        int ty = t / j->tiles_x, tx = t - ty * j->tiles_x;
+        int x0 = tx * TILE, yy0 = j->y0 + ty * TILE;
+        int x1 = x0 + TILE < j->w ? x0 + TILE : j->w;
+        int yy1 = yy0 + TILE < j->y1 ? yy0 + TILE : j->y1;
+        for (int y = yy0; y < yy1; ++y)
+            for (int x = x0; x < x1; ++x) render_pixel(j, x, y);
+    }
+    long long st[5];
+    or_stats_get(st);
+    pthread_mutex_lock(&j->lock);
+    for (int i = 0; i < 5; ++i) j->stats[i] += st[i];
+    pthread_mutex_unlock(&j->lock);
+    return NULL;
+}
+
+static long long g_render_stats[5];
+
+int or_render(const or_scene *s, const mh_camera *cam, int width, int height, int spp,
+              int depth, int mode, int sample_offset, int y0, int y1, int threads,
+              unsigned char *rgb, double *mean, double *var, double *seconds)
+{
+    if (!s || !cam) return 1;
+    if (width <= 0 || height <= 0) return 2;
+    if (spp < 1 || depth < 0) return 3;
+    if (y1 <= 0 || y1 > height) y1 = height;
+    if (y0 < 0) y0 = 0;
+    if (y0 >= y1) return 2;
+    job_t *j = (job_t *)calloc(1, sizeof(job_t));
+    if (!j) return 4;
+    j->s = s; j->cam = cam; j->w = width; j->h = height; j->spp = spp; j->depth = depth;
+    j->mode = mode; j->s0 = sample_offset; j->y0 = y0; j->y1 = y1;
+    j->rgb = rgb; j->mean = mean; j->var = var;
+    j->tiles_x = (width + TILE - 1) / TILE;
+    j->tiles_y = (y1 - y0 + TILE - 1) / TILE;
+    atomic_init(&j->next, 0);
+    pthread_mutex_init(&j->lock, NULL);
+    int nt = threads;
+    if (nt <= 0) {
+        long nc = sysconf(_SC_NPROCESSORS_ONLN);
+        nt = nc >= 1 ? (int)nc : 4;
+    }
+    if (nt > 64) nt = 64;
+    if (nt > j->tiles_x * j->tiles_y) nt = j->tiles_x * j->tiles_y;
+    if (nt < 1) nt = 1;
+    struct timespec a, b;
+    clock_gettime(CLOCK_MONOTONIC, &a);
+    pthread_t tid[64];
+    int started = 0;
+    for (int i = 1; i < nt; ++i)
+        if (pthread_create(&tid[started], NULL, worker, j) == 0) ++started;
+    worker(j);
+    for (int i = 0; i < started; ++i) pthread_join(tid[i], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &b);
+    if (seconds) *seconds = (double)(b.tv_sec - a.tv_sec) + (double)(b.tv_nsec - a.tv_nsec) * 1e-9;
+    memcpy(g_render_stats, j->stats, sizeof g_render_stats);
+    pthread_mutex_destroy(&j->lock);
+    free(j);
+    return 0;
+}
+
+void or_render_stats_get(long long out[5]) { memcpy(out, g_render_stats, sizeof g_render_stats); }
+
+void or_stats_reset(void) { memset(&or_tls_stats, 0, sizeof or_tls_stats); }
+
+void or_stats_get(long long out[5])
+{
+    out[0] = or_tls_stats.closest_queries;
+    out[1] = or_tls_stats.any_queries;
+    out[2] = or_tls_stats.nodes_visited;
+    out[3] = or_tls_stats.prims_tested;
+    out[4] = or_tls_stats.plane_tests;
+}
+
+/* ---- scene ------------------------------------------------------------ */
+
+or_scene *or_scene_create(const mh_scene_soa *soa)
+{
+    if (!soa || soa->n_prims < 0 || soa->n_materials < 0) return NULL;
+    or_scene *s = (or_scene *)calloc(1, sizeof(or_scene));
+    if (!s) return NULL;
+    s->n_prims = soa->n_prims;
+    s->prims = (oprim *)calloc((size_t)(soa->n_prims > 0 ? soa->n_prims : 1), sizeof(oprim));
+    s->n_mats = soa->n_materials;
+    s->mats = (mh_material *)calloc((size_t)(soa->n_materials > 0 ? soa->n_materials : 1), sizeof(mh_material));
+    if (!s->prims || !s->mats) {
+        or_scene_destroy(s);
+        return NULL;
+    }
+    if (soa->n_materials > 0) memcpy(s->mats, soa->materials, sizeof(mh_material) * (size_t)soa->n_materials);
+    s->sky = soa->sky;
+    s->water_level = soa->water_level;
+    s->water_material = soa->water_material;
+    for (int i = 0; i < soa->n_prims; ++i) {
+        const mh_prim *q = &soa->prims[i];
+        oprim *p = &s->prims[i];
+        p->kind = q->kind;
+        p->material = q->material;
+        p->flags = soa->attrs ? q->flags : 0;
+        p->p0 = fromm(q->p0);
+        p->p1 = fromm(q->p1);
+        p->p2 = fromm(q->p2);
+        p->r0 = q->r0;
+        p->r1 = q->r1;
+        p->e1 = dsub(p->p1, p->p0);
+        p->e2 = dsub(p->p2, p->p0);
+        if (soa->attrs) {
+            const mh_prim_attr *a = &soa->attrs[i];
+            p->n0 = fromm(a->n0); p->n1 = fromm(a->n1); p->n2 = fromm(a->n2);
+            p->uv0 = fromm(a->uv0); p->uv1 = fromm(a->uv1); p->uv2 = fromm(a->uv2);
+        }
+    }
+    /* emissive area lights: first 8 PBR emitting spheres, file order */
+    for (int i = 0; i < s->n_prims && s->n_lights < 8; ++i) {
+        const oprim *p = &s->prims[i];
+        if (p->kind != MH_PRIM_SPHERE) continue;
+        const mh_material *m = mat_of(s, p->material);
+        if (!m || !m->pbr) continue;
+        if (m->emissive.x == 0.0 && m->emissive.y == 0.0 && m->emissive.z == 0.0) continue;
+        if (!(p->r0 > 0.0)) continue;
+        olight *l = &s->lights[s->n_lights++];
+        l->prim = i;
+        l->center = p->p0;
+        l->radius = p->r0;
+        l->emissive = fromm(m->emissive);
+    }
+    if (or_bvh_build(s) != 0) {
+        or_scene_destroy(s);
+        return NULL;
+    }
+    return s;
+}
+
+void or_scene_destroy(or_scene *s)
+{
+    if (!s) return;
+    free(s->prims);
+    free(s->mats);
+    free(s->nodes);
+    free(s->order);
+    free(s->leaf_prims);
+    free(s->planes);
+    free(s);
+}
+
+int or_scene_info(const or_scene *s, int *n_prims, int *n_nodes, int *n_lights)
+{
+    if (!s) return 1;
+    if (n_prims) *n_prims = s->n_prims;
+    if (n_nodes) *n_nodes = s->n_nodes;
+    if (n_lights) *n_lights = s->n_lights;
+    return 0;
+}
+
+/* ---- batch queries & vectors ----------------------------------------- */
+
+static oray from_mh(const mh_ray *r)
+{
+    oray o = { d3(r->ox, r->oy, r->oz), d3(r->dx, r->dy, r->dz) };
+    return o;
+}
+
+int or_intersect(const or_scene *s, const mh_ray *rays, int n, double tmin, double tmax, mh_hit *out)
+{
+    if (!s || (!rays && n > 0) || (!out && n > 0)) return 1;
+    for (int i = 0; i < n; ++i) {
+        ohit h;
+        memset(&out[i], 0, sizeof out[i]);
+        if (or_intersect1(s, from_mh(&rays[i]), tmin, tmax, &h)) {
+            out[i].t = h.t;
+            out[i].prim_index = h.prim;
+            out[i].material = h.material;
+            out[i].front_face = h.front;
+            out[i].nx = h.normal.x; out[i].ny = h.normal.y; out[i].nz = h.normal.z;
+            out[i].u = h.u; out[i].v = h.v;
+        } else {
+            out[i].prim_index = -1;
+        }
+    }
+    return 0;
+}
+
+int or_occluded(const or_scene *s, const mh_ray *rays, int n, double tmin, double tmax, unsigned char *out)
+{
+    if (!s || (!rays && n > 0) || (!out && n > 0)) return 1;
+    for (int i = 0; i < n; ++i) out[i] = (unsigned char)or_occluded1(s, from_mh(&rays[i]), tmin, tmax);
+    return 0;
+}
+
+int or_radiance(const or_scene *s, const mh_ray *ray, int depth, unsigned seed, double out[3])
+{
+    if (!s || !ray) return 1;
+    dv3 c = pt_radiance(s, from_mh(ray), depth, seed);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+    return 0;
+}
+
+void or_sky_samples(const or_scene *s, const double *dirs, int n, double *out)
+{
+    for (int i = 0; i < n; ++i) {
+        dv3 c = or_sky_sample(d3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), &s->sky);
+        out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+}
+
+void or_water_normals(const double *xz, int n, double *out)
+{
+    for (int i = 0; i < n; ++i) {
+        dv3 c = or_water_normal(xz[2 * i], xz[2 * i + 1], 0.0);
+        out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+}
+
+void or_noise(const double *xy, int n, int octaves, unsigned seed, double *perlin, double *fbm)
+{
+    for (int i = 0; i < n; ++i) {
+        perlin[i] = or_perlin2(xy[2 * i], xy[2 * i + 1], seed);
+        fbm[i] = or_fbm2(xy[2 * i], xy[2 * i + 1], octaves, 2.0, 0.5, seed);
+    }
+}
+
+static void put3(double *o, dv3 v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
+
+void or_sun_disk_dirs(const double sun[3], double radius_deg, const double *u, int n, double *out)
+{
+    for (int i = 0; i < n; ++i)
+        put3(out + 3 * i, or_sun_disk_dir(d3(sun[0], sun[1], sun[2]), radius_deg, u[2 * i], u[2 * i + 1]));
+}
+
+void or_light_cone_dirs(const double w[3], double cos_max, const double *u, int n, double *out)
+{
+    for (int i = 0; i < n; ++i)
+        put3(out + 3 * i, or_light_cone_dir(d3(w[0], w[1], w[2]), cos_max, u[2 * i], u[2 * i + 1]));
+}
+
+void or_cosine_dirs(const double nrm[3], const double *u, int n, double *out)
+{
+    for (int i = 0; i < n; ++i)
+        put3(out + 3 * i, or_cosine_hemisphere(d3(nrm[0], nrm[1], nrm[2]), u[2 * i], u[2 * i + 1]));
+}
+
+void or_ggx_halfs(const double nrm[3], double alpha, const double *u, int n, double *out)
+{
+    for (int i = 0; i < n; ++i)
+        put3(out + 3 * i, or_ggx_half(d3(nrm[0], nrm[1], nrm[2]), alpha, u[2 * i], u[2 * i + 1]));
+}
+
+double or_fresnel_dielectric_x(double cos_i, double eta) { return or_fresnel_dielectric(cos_i, eta); }
+
+double or_specular_probability_x(double m, double r, double f0) { return or_specular_probability(m, r, f0); }
+
+void or_shade(const or_scene *s, int material, const double N[3], const double L[3],
+              const double V[3], const double light[3], double out[3])
+{
+    const mh_material *m = mat_of(s, material);
+    if (!m) { out[0] = out[1] = out[2] = 0.0; return; }
+    put3(out, shade(m, d3(N[0], N[1], N[2]), d3(L[0], L[1], L[2]), d3(V[0], V[1], V[2]),
+                    d3(light[0], light[1], light[2])));
+}

@@ -1,0 +1,259 @@
+"""Test-side loaders of the CPU checkers (oracle/liboracle.so and, when it was
+built here, the reference itself in oracle/_ref/libmtref.so), plus helpers
+shared by the tests. Only tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline import this module.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "marmel-trace_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+from marmel_amd import abi  # noqa: E402
+
+ORACLE_SO = os.path.join(REPO, "oracle", "liboracle.so")
+REF_SO = os.path.join(REPO, "oracle", "_ref", "libmtref.so")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+_oracle = None
+_ref = None
+
+P = C.c_void_p
+D = C.c_double
+DP = C.POINTER(C.c_double)
+
+
+def _sig(L, name, res, args):
+    f = getattr(L, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: run make -C oracle")
+        L = C.CDLL(ORACLE_SO)
+        _sig(L, "or_scene_create", P, [C.POINTER(abi.SceneSoa)])
+        _sig(L, "or_scene_destroy", None, [P])
+        _sig(L, "or_scene_info", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)])
+        _sig(L, "or_intersect", C.c_int, [P, P, C.c_int, D, D, P])
+        _sig(L, "or_occluded", C.c_int, [P, P, C.c_int, D, D, P])
+        _sig(L, "or_radiance", C.c_int, [P, P, C.c_int, C.c_uint, DP])
+        _sig(L, "or_render", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P,
+                                       P, P, DP])
+        _sig(L, "or_stats_reset", None, [])
+        _sig(L, "or_stats_get", None, [C.POINTER(C.c_longlong)])
+        _sig(L, "or_render_stats_get", None, [C.POINTER(C.c_longlong)])
+        _sig(L, "or_hash3", C.c_uint, [C.c_uint, C.c_uint, C.c_uint])
+        _sig(L, "or_rand01", D, [C.c_uint, C.c_uint, C.c_uint])
+        _sig(L, "or_camera_create", None, [DP, DP, DP, D, D, C.POINTER(abi.Camera)])
+        _sig(L, "or_camera_ray", None, [C.POINTER(abi.Camera), D, D, D, D, DP])
+        _sig(L, "or_sky_samples", None, [P, P, C.c_int, P])
+        _sig(L, "or_water_normals", None, [P, C.c_int, P])
+        _sig(L, "or_noise", None, [P, C.c_int, C.c_int, C.c_uint, P, P])
+        _sig(L, "or_sun_disk_dirs", None, [DP, D, P, C.c_int, P])
+        _sig(L, "or_light_cone_dirs", None, [DP, D, P, C.c_int, P])
+        _sig(L, "or_cosine_dirs", None, [DP, P, C.c_int, P])
+        _sig(L, "or_ggx_halfs", None, [DP, D, P, C.c_int, P])
+        _sig(L, "or_fresnel_dielectric_x", D, [D, D])
+        _sig(L, "or_specular_probability_x", D, [D, D, D])
+        _sig(L, "or_shade", None, [P, C.c_int, DP, DP, DP, DP, DP])
+        _sig(L, "or_to_byte", C.c_ubyte, [D])
+        _oracle = L
+    return _oracle
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_SO)
+        _sig(L, "mtref_load_text", P, [C.c_char_p, C.c_char_p, C.c_size_t])
+        _sig(L, "mtref_from_soa", P, [C.POINTER(abi.SceneSoa)])
+        _sig(L, "mtref_free", None, [P])
+        _sig(L, "mtref_counts", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)])
+        _sig(L, "mtref_export", C.c_int, [P, P, P, P, C.POINTER(abi.Sky), DP,
+                                          C.POINTER(C.c_int)])
+        _sig(L, "mtref_camera_create", None, [DP, DP, DP, D, D, C.POINTER(abi.Camera)])
+        _sig(L, "mtref_scene_camera", C.c_int, [P, D, C.POINTER(abi.Camera)])
+        _sig(L, "mtref_render", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int, C.c_int,
+                                          C.c_int, C.c_int, P, C.c_int, DP])
+        _sig(L, "mtref_intersect", C.c_int, [P, P, C.c_int, D, D, P])
+        _sig(L, "mtref_occluded", C.c_int, [P, P, C.c_int, D, D, P])
+        _sig(L, "mtref_radiance", C.c_int, [P, P, C.c_int, C.c_uint, DP])
+        _sig(L, "mtref_sky_sample", None, [P, P, C.c_int, P])
+        _sig(L, "mtref_water_normal", None, [P, C.c_int, P])
+        _sig(L, "mtref_noise", None, [P, C.c_int, C.c_int, C.c_uint, P, P])
+        _sig(L, "mtref_sun_disk_dir", None, [DP, D, P, C.c_int, P])
+        _sig(L, "mtref_light_cone_dir", None, [DP, D, P, C.c_int, P])
+        _sig(L, "mtref_cosine_dir", None, [DP, P, C.c_int, P])
+        _sig(L, "mtref_ggx_half", None, [DP, D, P, C.c_int, P])
+        _sig(L, "mtref_fresnel_dielectric", D, [D, D])
+        _sig(L, "mtref_specular_probability", D, [D, D, D])
+        _sig(L, "mtref_shade", None, [P, C.c_int, DP, DP, DP, DP, DP])
+        _sig(L, "mtref_to_byte", C.c_ubyte, [D])
+        _ref = L
+    return _ref
+
+
+def d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+class SoaHolder:
+    """Keeps numpy arrays alive behind an mh_scene_soa (no BVH)."""
+
+    def __init__(self, prims, attrs, mats, sky, water_level=0.0, water_material=-1):
+        self.prims = np.ascontiguousarray(prims, dtype=abi.PRIM_DTYPE)
+        self.attrs = None if attrs is None else np.ascontiguousarray(attrs, dtype=abi.ATTR_DTYPE)
+        self.mats = np.ascontiguousarray(mats, dtype=abi.MATERIAL_DTYPE)
+        self.sky = sky
+        self.soa = abi.SceneSoa(abi_version=abi.ABI_VERSION, n_prims=len(self.prims),
+                                prims=abi.ptr(self.prims), attrs=abi.ptr(self.attrs),
+                                n_materials=len(self.mats), materials=abi.ptr(self.mats),
+                                sky=sky, water_level=water_level, water_material=water_material)
+
+
+class OracleScene:
+    def __init__(self, soa):
+        self.L = oracle()
+        self.h = C.c_void_p(self.L.or_scene_create(C.byref(soa)))
+        if not self.h:
+            raise RuntimeError("or_scene_create failed")
+
+    def __del__(self):
+        try:
+            self.L.or_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    def render(self, cam, w, h, spp, depth, mode=0, sample_offset=0, y0=0, y1=0, threads=0,
+               want_var=False):
+        rgb = np.zeros((h, w, 3), np.uint8)
+        mean = np.zeros((h, w, 3), np.float64)
+        var = np.zeros((h, w, 3), np.float64) if want_var else None
+        sec = C.c_double()
+        rc = self.L.or_render(self.h, C.byref(cam), w, h, spp, depth, mode, sample_offset, y0, y1,
+                              threads, abi.ptr(rgb), abi.ptr(mean), abi.ptr(var), C.byref(sec))
+        if rc:
+            raise RuntimeError(f"or_render rc {rc}")
+        return rgb, mean, var, sec.value
+
+    def intersect(self, rays, tmin=1e-4, tmax=1e30):
+        rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
+        out = np.zeros(len(rays), abi.HIT_DTYPE)
+        self.L.or_intersect(self.h, abi.ptr(rays), len(rays), tmin, tmax, abi.ptr(out))
+        return out
+
+    def occluded(self, rays, tmin=1e-3, tmax=1e30):
+        rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
+        out = np.zeros(len(rays), np.uint8)
+        self.L.or_occluded(self.h, abi.ptr(rays), len(rays), tmin, tmax, abi.ptr(out))
+        return out
+
+    def stats(self):
+        a = (C.c_longlong * 5)()
+        self.L.or_render_stats_get(a)
+        return list(a)
+
+
+class RefScene:
+    """The reference renderer (oracle/_ref) on a text scene or on arrays."""
+
+    def __init__(self, text=None, soa=None):
+        self.L = ref()
+        if text is not None:
+            err = C.create_string_buffer(512)
+            self.h = C.c_void_p(self.L.mtref_load_text(text.encode() if isinstance(text, str)
+                                                       else text, err, 512))
+            if not self.h:
+                raise RuntimeError("mtref_load_text: " + err.value.decode())
+        else:
+            self.h = C.c_void_p(self.L.mtref_from_soa(C.byref(soa)))
+            if not self.h:
+                raise RuntimeError("mtref_from_soa failed")
+
+    def __del__(self):
+        try:
+            self.L.mtref_free(self.h)
+        except Exception:
+            pass
+
+    def export(self):
+        n_p, n_m, n_l = C.c_int(), C.c_int(), C.c_int()
+        self.L.mtref_counts(self.h, C.byref(n_p), C.byref(n_m), C.byref(n_l))
+        prims = np.zeros(n_p.value, abi.PRIM_DTYPE)
+        attrs = np.zeros(n_p.value, abi.ATTR_DTYPE)
+        mats = np.zeros(n_m.value, abi.MATERIAL_DTYPE)
+        sky = abi.Sky()
+        wl, wm = C.c_double(), C.c_int()
+        rc = self.L.mtref_export(self.h, abi.ptr(prims), abi.ptr(attrs), abi.ptr(mats),
+                                 C.byref(sky), C.byref(wl), C.byref(wm))
+        return rc, prims, attrs, mats, sky, wl.value, wm.value
+
+    def camera(self, aspect):
+        cam = abi.Camera()
+        self.L.mtref_scene_camera(self.h, aspect, C.byref(cam))
+        return cam
+
+    def render(self, cam, w, h, spp, depth, mode=0, threads=0):
+        rgb = np.zeros((h, w, 3), np.uint8)
+        sec = C.c_double()
+        rc = self.L.mtref_render(self.h, C.byref(cam), w, h, spp, depth, mode, abi.ptr(rgb),
+                                 threads, C.byref(sec))
+        if rc:
+            raise RuntimeError(f"mtref_render rc {rc}")
+        return rgb, sec.value
+
+    def intersect(self, rays, tmin=1e-4, tmax=1e30):
+        rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
+        out = np.zeros(len(rays), abi.HIT_DTYPE)
+        self.L.mtref_intersect(self.h, abi.ptr(rays), len(rays), tmin, tmax, abi.ptr(out))
+        return out
+
+
+def seeded_rays(n, seed, lo, hi, towards=None):
+    """Seeded rays: origins uniform in the box [lo, hi], directions uniform on
+    the sphere (or aimed at random points of `towards` box when given)."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(lo, hi, size=(n, 3))
+    if towards is not None:
+        t = rng.uniform(towards[0], towards[1], size=(n, 3))
+        d = t - o
+    else:
+        d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, abi.RAY_DTYPE)
+    rays["ox"], rays["oy"], rays["oz"] = o[:, 0], o[:, 1], o[:, 2]
+    rays["dx"], rays["dy"], rays["dz"] = d[:, 0], d[:, 1], d[:, 2]
+    return rays
+
+
+def camera_rays(cam, w, h, seed=0):
+    """Jittered primary rays through a w x h grid, built with the oracle camera."""
+    L = oracle()
+    rng = np.random.default_rng(seed)
+    rays = np.zeros(w * h, abi.RAY_DTYPE)
+    out = (C.c_double * 6)()
+    k = 0
+    for y in range(h):
+        for x in range(w):
+            u = (x + rng.random()) / w
+            v = 1.0 - (y + rng.random()) / h
+            L.or_camera_ray(C.byref(cam), u, v, 0.0, 0.0, out)
+            rays[k] = tuple(out)
+            k += 1
+    return rays

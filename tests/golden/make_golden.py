@@ -1,0 +1,139 @@
+"""Generate the golden fixtures under tests/golden/ from the REFERENCE itself.
+
+Runs in the build container only, where oracle/_ref/libmtref.so was compiled
+from the reference sources (oracle/Makefile). Writes data only: the default
+scene exported in the flattened ABI layout, reference-rendered images, and
+reference input -> output vectors for the per-function checks. Nothing of the
+reference's source text is copied.
+
+    python tests/golden/make_golden.py
+"""
+import ctypes as C
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import checkers as ck  # noqa: E402
+from marmel_amd import abi, native, HostScene, encode_bmp  # noqa: E402
+
+REF_SCENE = "/root/reference/scenes/default.scene"
+
+
+def sbytes(struct):
+    return np.frombuffer(bytes(struct), np.uint8).copy()
+
+
+def default_scene():
+    text = open(REF_SCENE, "rb").read()
+    r = ck.RefScene(text=text)
+    rc, prims, attrs, mats, sky, wl, wm = r.export()
+    assert rc == 0
+    out = {}
+    for (w, h, spp, depth) in [(320, 180, 4, 4), (160, 90, 16, 6)]:
+        cam = r.camera(w / h)
+        out[f"cam_{w}x{h}"] = sbytes(cam)
+        for mode, tag in [(1, "whitted"), (0, "pt")]:
+            rgb, _ = r.render(cam, w, h, spp, depth, mode, threads=8)
+            key = f"{tag}_{w}x{h}_s{spp}_d{depth}"
+            out[key] = rgb
+            out[key + "_sha_bmp"] = np.frombuffer(
+                hashlib.sha256(encode_bmp(rgb)).hexdigest().encode(), np.uint8).copy()
+    np.savez_compressed(os.path.join(HERE, "default_scene.npz"), prims=prims, attrs=attrs,
+                        mats=mats, sky=sbytes(sky), water_level=np.float64(wl),
+                        water_material=np.int32(wm), **out)
+    return r, prims, attrs, mats, sky, wl, wm
+
+
+def host_scene_images():
+    """Reference renders of the host-generated benchmark scenes (small sizes)."""
+    out = {}
+    cases = [("spheres", native.GEN_SPHERES, {}, (160, 120, 4, 6)),
+             ("terrain", native.GEN_TERRAIN, dict(grid=48), (120, 68, 2, 8)),
+             ("icosphere", native.GEN_ICOSPHERE, dict(subdiv=2), (96, 72, 2, 6))]
+    for name, kind, kw, (w, h, spp, depth) in cases:
+        hs = HostScene()
+        info = hs.generate(kind, seed=1, **kw)
+        hs.build_bvh()
+        prims, attrs, mats, sky, wl, wm = hs.arrays()
+        hold = ck.SoaHolder(prims, attrs, mats, sky, wl, wm)
+        r = ck.RefScene(soa=hold.soa)
+        cam = hs.camera(w / h)
+        out[f"{name}_cam"] = sbytes(cam)
+        out[f"{name}_prims_sha"] = np.frombuffer(
+            hashlib.sha256(prims.tobytes() + mats.tobytes()).hexdigest().encode(), np.uint8).copy()
+        for mode, tag in [(0, "pt"), (1, "whitted")]:
+            rgb, _ = r.render(cam, w, h, spp, depth, mode, threads=8)
+            out[f"{name}_{tag}_{w}x{h}_s{spp}_d{depth}"] = rgb
+        rays = ck.seeded_rays(2000, 11, [-12, 0.05, -12], [12, 6, 12])
+        hits = r.intersect(rays)
+        out[f"{name}_rays_sha"] = np.frombuffer(
+            hashlib.sha256(rays.tobytes()).hexdigest().encode(), np.uint8).copy()
+        out[f"{name}_hits_prim"] = hits["prim_index"]
+        out[f"{name}_hits_t"] = hits["t"]
+    np.savez_compressed(os.path.join(HERE, "host_scenes.npz"), **out)
+
+
+def vectors(r, sky):
+    """Reference outputs of the per-function routines on seeded inputs."""
+    L = ck.ref()
+    rng = np.random.default_rng(20261203)
+    out = {}
+    # sky: directions over the upper and lower hemisphere incl. near the sun
+    d = rng.normal(size=(3000, 3))
+    sun = np.array([sky.sun_dir.x, sky.sun_dir.y, sky.sun_dir.z])
+    d[:300] = sun + 0.02 * rng.normal(size=(300, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = np.ascontiguousarray(d)
+    s = np.zeros_like(d)
+    L.mtref_sky_sample(r.h, abi.ptr(d), len(d), abi.ptr(s))
+    out["sky_dirs"], out["sky_out"] = d, s
+    xz = np.ascontiguousarray(rng.uniform(-60, 60, size=(1000, 2)))
+    wn = np.zeros((1000, 3))
+    L.mtref_water_normal(abi.ptr(xz), 1000, abi.ptr(wn))
+    out["water_xz"], out["water_n"] = xz, wn
+    xy = np.ascontiguousarray(rng.uniform(-300, 300, size=(2000, 2)))
+    per = np.zeros(2000)
+    fbm = np.zeros(2000)
+    L.mtref_noise(abi.ptr(xy), 2000, 5, 1337, abi.ptr(per), abi.ptr(fbm))
+    out["noise_xy"], out["perlin2"], out["fbm2_o5"] = xy, per, fbm
+    u = np.ascontiguousarray(rng.random(size=(1000, 2)))
+    out["u2"] = u
+    for name, fn, args in [("sun_disk", L.mtref_sun_disk_dir, (ck.d3(sun), 0.5)),
+                           ("light_cone", L.mtref_light_cone_dir, (ck.d3([0.3, 0.8, -0.52]), 0.93)),
+                           ("ggx_half", L.mtref_ggx_half, (ck.d3([0.0, 0.6, 0.8]), 0.09))]:
+        o = np.zeros((1000, 3))
+        fn(args[0], args[1], abi.ptr(u), 1000, abi.ptr(o))
+        out[name] = o
+    o = np.zeros((1000, 3))
+    L.mtref_cosine_dir(ck.d3([0.2, 0.9, -0.38]), abi.ptr(u), 1000, abi.ptr(o))
+    out["cosine"] = o
+    ci = rng.uniform(-0.1, 1.1, size=500)
+    eta = rng.uniform(0.5, 1.6, size=500)
+    out["fresnel_in"] = np.stack([ci, eta], 1)
+    out["fresnel_out"] = np.array([L.mtref_fresnel_dielectric(a, b) for a, b in zip(ci, eta)])
+    ps_in = rng.uniform(-0.1, 1.1, size=(500, 3))
+    out["pspec_in"] = ps_in
+    out["pspec_out"] = np.array([L.mtref_specular_probability(*x) for x in ps_in])
+    tb = rng.uniform(-0.2, 1.2, size=2000)
+    out["tobyte_in"] = tb
+    out["tobyte_out"] = np.array([L.mtref_to_byte(x) for x in tb], np.uint8)
+    np.savez_compressed(os.path.join(HERE, "vectors.npz"), **out)
+
+
+def main():
+    if not ck.have_ref():
+        raise SystemExit("oracle/_ref/libmtref.so missing: make -C oracle ref")
+    r, prims, attrs, mats, sky, wl, wm = default_scene()
+    vectors(r, sky)
+    host_scene_images()
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,94 @@
+"""Small hand-built scenes with closed-form answers, shared by the CPU
+(oracle) and GPU tests. Built through the product host API (HostScene)."""
+import numpy as np
+
+from marmel_amd import HostScene, abi
+
+
+def material(**kw):
+    m = np.zeros(1, abi.MATERIAL_DTYPE)[0]
+    m["ior"] = 1.0
+    m["texture_scale"] = 1.0
+    m["texture_color_a"] = (1.0, 1.0, 1.0)
+    for k, v in kw.items():
+        m[k] = v
+    return m
+
+
+def prim(kind, mat, p0=(0, 0, 0), p1=(0, 0, 0), p2=(0, 0, 0), r0=0.0, r1=0.0):
+    p = np.zeros(1, abi.PRIM_DTYPE)[0]
+    p["kind"], p["material"] = kind, mat
+    p["p0"], p["p1"], p["p2"] = p0, p1, p2
+    p["r0"], p["r1"] = r0, r1
+    return p
+
+
+def sky(horizon=(0, 0, 0), zenith=(0, 0, 0), sun_color=(0, 0, 0), sun_dir=(0.0, 1.0, 0.0),
+        glow=0.0, clouds=False, sun_radius=0.0):
+    s = abi.Sky()
+    s.sun_dir = abi.vec3(sun_dir)
+    s.sun_color = abi.vec3(sun_color)
+    s.horizon_color = abi.vec3(horizon)
+    s.zenith_color = abi.vec3(zenith)
+    s.gradient_gamma = 0.6
+    s.sun_glow_exponent = 350.0
+    s.sun_glow_strength = glow
+    s.cloud_height = 120.0
+    s.cloud_scale = 0.0025
+    # coverage above every density value switches the cloud layer off
+    s.cloud_coverage = 0.5 if clouds else 5.0
+    s.cloud_softness = 0.12
+    s.cloud_sharpness = 1.5
+    s.cloud_octaves = 5
+    s.seed = 1337
+    s.sun_radius = sun_radius
+    return s
+
+
+def build(prims, mats, sk, water=(0.0, -1)):
+    hs = HostScene()
+    hs.add_materials(np.array(mats, dtype=abi.MATERIAL_DTYPE))
+    hs.add_prims(np.array(prims, dtype=abi.PRIM_DTYPE))
+    hs.set_sky(sk)
+    hs.set_water(*water)
+    hs.build_bvh()
+    return hs
+
+
+def white_furnace():
+    """One white Lambert sphere under a uniform sky of radiance 1, no sun: every
+    path that leaves the convex sphere returns exactly 1."""
+    return build([prim(abi.PRIM_SPHERE, 0, p0=(0, 0, 0), r0=1.0)],
+                 [material(albedo=(1, 1, 1))], sky(horizon=(1, 1, 1), zenith=(1, 1, 1)))
+
+
+def sun_plane(albedo=0.6, sun=(0.3, 0.8, -0.52)):
+    """A Lambert plane under a point sun in a black sky: the primary-hit
+    radiance is albedo * sun_color * (N . L) exactly (Blinn-Phong NEE has no
+    1/pi, the point sun has solid angle 1)."""
+    s = np.array(sun, float)
+    s /= np.linalg.norm(s)
+    return build([prim(abi.PRIM_PLANE, 0, p0=(0, 0, 0), p1=(0, 1, 0))],
+                 [material(albedo=(albedo, albedo, albedo))],
+                 sky(sun_color=(1.0, 0.9, 0.8), sun_dir=tuple(s))), s
+
+
+def lamp_over_plane(h=4.0, R=0.5, le=10.0, albedo=0.5):
+    """A Lambert plane and one emissive PBR sphere of radiance le straight
+    above at height h, black sky: the NEE estimate at the foot point has mean
+    albedo * le * pi * R^2 / h^2 (cap of half-angle asin(R/h) about the
+    normal; Blinn-Phong NEE without 1/pi)."""
+    mats = [material(albedo=(albedo, albedo, albedo)),
+            material(pbr=1, albedo=(1, 1, 1), emissive=(le, le, le), roughness=1.0)]
+    prims = [prim(abi.PRIM_SPHERE, 1, p0=(0, h, 0), r0=R),
+             prim(abi.PRIM_PLANE, 0, p0=(0, 0, 0), p1=(0, 1, 0))]
+    return build(prims, mats, sky())
+
+
+def down_ray(x=0.0, z=0.0, y=3.0, tilt=(0.0, 0.0)):
+    r = np.zeros(1, abi.RAY_DTYPE)
+    d = np.array([tilt[0], -1.0, tilt[1]])
+    d /= np.linalg.norm(d)
+    r["ox"], r["oy"], r["oz"] = x, y, z
+    r["dx"], r["dy"], r["dz"] = d
+    return r
