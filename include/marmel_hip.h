@@ -226,6 +226,11 @@ enum {
     MH_PREC_FP64 = 1        /* fp64 throughout (parity path)               */
 };
 
+enum {
+    MH_ENGINE_MEGAKERNEL = 0, /* one thread per pixel, whole path in one kernel */
+    MH_ENGINE_WAVEFRONT  = 1  /* queue-based extend / shade / shadow kernels (fp32) */
+};
+
 typedef struct mh_render_params {
     int32_t width;
     int32_t height;
@@ -234,7 +239,11 @@ typedef struct mh_render_params {
     int32_t mode;           /* MH_MODE_*  */
     int32_t precision;      /* MH_PREC_*  */
     int32_t sample_offset;  /* first sample index (progressive chunks) */
-    int32_t block_size;     /* 0 = library default; otherwise a multiple of 64 */
+    int32_t engine;         /* MH_ENGINE_* (path tracing only)          */
+    int32_t pool;           /* wavefront: paths per batch (0 = default, capped at w*h) */
+    int32_t grid_blocks;    /* wavefront: persistent blocks of 256 (0 = default)       */
+    int32_t use_graph;      /* wavefront: replay the bounce sequence as a hipGraph     */
+    int32_t reserved;
 } mh_render_params;
 
 /* Per-call timing of the last render on the calling thread, measured with

@@ -182,6 +182,15 @@ struct mh_scene {
     size_t wstack_bytes = 0;
     void* qbuf = nullptr;
     size_t qbuf_bytes = 0;
+    // wavefront state (grown on demand) and the captured batch graph
+    void* wf_mem = nullptr;
+    size_t wf_bytes = 0;
+    int wf_pool = 0;
+    size_t wf_npix = 0;
+    WfState ws{};
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    std::vector<unsigned char> graph_key;
 };
 
 namespace {
@@ -334,6 +343,57 @@ int ensure_query(mh_scene* s, size_t bytes)
     return 0;
 }
 
+// Carve the wavefront state for `pool` paths and `npix` pixels out of one allocation.
+int ensure_wf(mh_scene* s, int pool, size_t npix)
+{
+    if (pool <= s->wf_pool && npix <= s->wf_npix) return 0;
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    if (s->graph_def) (void)hipGraphDestroy(s->graph_def);
+    s->graph = nullptr;
+    s->graph_def = nullptr;
+    s->graph_key.clear();
+    if (s->wf_mem) (void)hipFree(s->wf_mem);
+    s->wf_mem = nullptr;
+    s->wf_bytes = 0;
+    s->wf_pool = 0;
+    s->wf_npix = 0;
+    size_t P = (size_t)pool;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t sz[] = {al(P * 16), al(P * 16), al(P * 16), al(P * 16), al(P * 8), al(P * 9 * 16), al(P * 4),
+                   al(P * 4),  al(P * 4),  al(P * 4),  al(P * 4),  al(P * 9 * 16), al(P * 9 * 16), al(64),
+                   al(npix * 3 * 4), al(64)};
+    size_t total = 0;
+    for (size_t v : sz) total += v;
+    HIP_TRY(hipMalloc(&s->wf_mem, total));
+    s->wf_bytes = total;
+    char* p = (char*)s->wf_mem;
+    void* ptrs[16];
+    for (int i = 0; i < 16; ++i) {
+        ptrs[i] = p;
+        p += sz[i];
+    }
+    WfState& w = s->ws;
+    w.p0 = (float4*)ptrs[0];
+    w.p1 = (float4*)ptrs[1];
+    w.p2 = (float4*)ptrs[2];
+    w.p3 = (float4*)ptrs[3];
+    w.hit = (float2*)ptrs[4];
+    w.pend = (float4*)ptrs[5];
+    w.vis = (uint32_t*)ptrs[6];
+    w.npend = (uint32_t*)ptrs[7];
+    w.q[0] = (uint32_t*)ptrs[8];
+    w.q[1] = (uint32_t*)ptrs[9];
+    w.term = (uint32_t*)ptrs[10];
+    w.sh0 = (float4*)ptrs[11];
+    w.sh1 = (float4*)ptrs[12];
+    w.cnt = (uint32_t*)ptrs[13];
+    w.acc = (float*)ptrs[14];
+    w.batch = (uint32_t*)ptrs[15];
+    s->wf_pool = pool;
+    s->wf_npix = npix;
+    return 0;
+}
+
 float ms_between(hipEvent_t a, hipEvent_t b)
 {
     float ms = 0.f;
@@ -472,6 +532,10 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         ++nl;
     }
 
+    int has_transmissive = 0;
+    for (int i = 0; i < soa->n_materials; ++i)
+        if (soa->materials[i].is_water || soa->materials[i].transparency > 0.8) has_transmissive = 1;
+
     auto fill = [&](auto& v, const void* pr, const void* pl, const void* at, const void* mt, auto* lights, auto tag) {
         typedef decltype(tag) R;
         v.nodes = (const DNode*)d_nodes;
@@ -487,6 +551,7 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         v.sky = conv_sky<R>(soa->sky);
         v.water_level = (R)soa->water_level;
         v.water_material = soa->water_material;
+        v.has_transmissive = has_transmissive;
         v.n_lights = nl;
         for (int i = 0; i < 8; ++i) v.lights[i] = lights[i];
     };
@@ -507,6 +572,9 @@ void mh_scene_destroy(mh_scene* s)
     if (s->d_overflow) (void)hipFree(s->d_overflow);
     if (s->wstack) (void)hipFree(s->wstack);
     if (s->qbuf) (void)hipFree(s->qbuf);
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    if (s->graph_def) (void)hipGraphDestroy(s->graph_def);
+    if (s->wf_mem) (void)hipFree(s->wf_mem);
     for (auto& e : s->ev)
         if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -574,13 +642,50 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
     HIP_TRY(hipMemsetAsync(s->d_overflow, 0, sizeof(int), s->stream));
     bool f64 = p->precision == MH_PREC_FP64;
     HIP_TRY(hipEventRecord(s->ev[0], s->stream));
-    if (p->mode == MH_MODE_PATHTRACE) {
+    if (p->mode == MH_MODE_PATHTRACE && p->engine == MH_ENGINE_WAVEFRONT) {
+        if (f64) return fail(5, "the wavefront engine computes in fp32 only");
+        int npix = p->width * p->height;
+        int pool = p->pool > 0 ? p->pool : (1 << 21);
+        if (pool > npix) pool = npix; // one sample per pixel per batch
+        rc = ensure_wf(s, pool, (size_t)npix);
+        if (rc) return rc;
+        int grid = p->grid_blocks > 0 ? p->grid_blocks : 2048;
+        CamView<float> cvw = conv_cam<float>(*cam);
+        WfJob job{p->width, p->height, p->spp, p->sample_offset, p->max_depth, npix,
+                  (long long)npix * p->spp, pool};
+        long long nb = (job.total + pool - 1) / pool;
+        HIP_TRY(wf_enqueue_begin(s->ws, npix, s->stream));
+        if (p->use_graph) {
+            std::vector<unsigned char> key(sizeof job + sizeof cvw + sizeof grid);
+            memcpy(key.data(), &job, sizeof job);
+            memcpy(key.data() + sizeof job, &cvw, sizeof cvw);
+            memcpy(key.data() + sizeof job + sizeof cvw, &grid, sizeof grid);
+            if (!s->graph || key != s->graph_key) {
+                if (s->graph) (void)hipGraphExecDestroy(s->graph);
+                if (s->graph_def) (void)hipGraphDestroy(s->graph_def);
+                s->graph = nullptr;
+                s->graph_def = nullptr;
+                HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+                hipError_t ce = wf_enqueue_batch(s->v32, cvw, job, s->ws, s->d_rays, grid, s->stream);
+                hipError_t ee = hipStreamEndCapture(s->stream, &s->graph_def);
+                HIP_TRY(ce);
+                HIP_TRY(ee);
+                HIP_TRY(hipGraphInstantiate(&s->graph, s->graph_def, nullptr, nullptr, 0));
+                s->graph_key = key;
+            }
+            for (long long b = 0; b < nb; ++b) HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+        } else {
+            for (long long b = 0; b < nb; ++b)
+                HIP_TRY(wf_enqueue_batch(s->v32, cvw, job, s->ws, s->d_rays, grid, s->stream));
+        }
+        HIP_TRY(wf_enqueue_end(s->ws, npix, p->spp, lo, s->stream));
+    } else if (p->mode == MH_MODE_PATHTRACE) {
         if (f64)
             HIP_TRY(launch_pathtrace<double>(s->v64, conv_cam<double>(*cam), p->width, p->height, p->spp,
-                                             p->sample_offset, p->max_depth, lo, p->block_size, s->stream));
+                                             p->sample_offset, p->max_depth, lo, 256, s->stream));
         else
             HIP_TRY(launch_pathtrace<float>(s->v32, conv_cam<float>(*cam), p->width, p->height, p->spp,
-                                            p->sample_offset, p->max_depth, lo, p->block_size, s->stream));
+                                            p->sample_offset, p->max_depth, lo, 256, s->stream));
     } else {
         // explicit Whitted work stack: 17 pending rays per level suffices
         int cap = 17 * (p->max_depth + 2) + 4;
@@ -647,7 +752,9 @@ static int render_drop_in(mh_scene* s, const mh_camera* cam, int w, int h, int s
     p.spp = spp;
     p.max_depth = depth;
     p.mode = mode;
-    p.precision = MH_PREC_FP32;
+    // Whitted is the byte-parity renderer: run it in fp64. The path tracer
+    // is a Monte Carlo estimate: fp32 keeps it within its own noise.
+    p.precision = mode == MH_MODE_WHITTED ? MH_PREC_FP64 : MH_PREC_FP32;
     return mh_render(s, cam, &p, rgb, nullptr);
 }
 

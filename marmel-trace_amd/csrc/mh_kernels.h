@@ -22,6 +22,43 @@ template <typename R>
 hipError_t launch_pathtrace(const SceneView<R>& sc, const CamView<R>& cam, int w, int h, int spp, int s0, int depth,
                             const LaunchOut& out, int block, hipStream_t st);
 
+// ---- wavefront engine (fp32) ---------------------------------------------
+
+// Device state of the wavefront renderer for `pool` paths in flight.
+struct WfState {
+    float4* p0;       // origin.xyz, seed (bits)
+    float4* p1;       // direction.xyz, meta (bits): bounce | last_spec << 8 | (medium + 1) << 9
+    float4* p2;       // throughput.xyz, pixel (bits)
+    float4* p3;       // radiance.xyz, sample index (bits)
+    float2* hit;      // t, leaf record (bits)
+    float4* pend;     // 9 pending NEE estimates per path (rgb)
+    uint32_t* vis;    // visibility bits of the pending estimates
+    uint32_t* npend;  // pending estimates: lights in bits 0..3, sun in bit 4
+    uint32_t* q[2];   // extension queues (ping-pong)
+    uint32_t* term;   // paths that ended this bounce
+    float4* sh0;      // shadow rays: origin.xyz, tmax
+    float4* sh1;      // direction.xyz, (slot << 4 | estimate) bits
+    uint32_t* cnt;    // [0..1] ext queue lengths, [2] shadow, [3] term
+    float* acc;       // per-pixel radiance sums (w*h*3)
+    uint32_t* batch;  // [0] first global sample index of the current batch
+};
+
+struct WfJob {
+    int w, h, spp, s0, depth;
+    int npix;
+    long long total;  // npix * spp
+    int pool;         // batch size (<= npix)
+};
+
+// Bytes of device state per path slot (state + pending + queues + shadow rays).
+size_t wf_bytes_per_path();
+
+// Record the kernels of one batch (gen, depth+1 bounces, advance) on `st`.
+hipError_t wf_enqueue_batch(const SceneView<float>& sc, const CamView<float>& cam, const WfJob& job,
+                            const WfState& ws, unsigned long long* rays, int grid, hipStream_t st);
+hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st);
+hipError_t wf_enqueue_end(const WfState& ws, int npix, int spp, const LaunchOut& out, hipStream_t st);
+
 // Whitted: persistent grid of `threads` lanes, each with `cap` stack items
 // of `item_bytes` at stack + lane * cap * item_bytes.
 template <typename R>

@@ -132,6 +132,11 @@ template <typename R>
 __device__ bool occluded_segment(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax, V3<R>* att, int* nq)
 {
     *att = mk(R(1), R(1), R(1));
+    if (!sc.has_transmissive) {
+        // every first hit would be opaque: blocked iff any hit in [tmin, tmax]
+        ++*nq;
+        return any_hit(sc, o, unit_dir(d), tmin, tmax);
+    }
     R rem = tmax;
     for (int step = 0; step < 6; ++step) {
         DHit<R> h;

@@ -96,6 +96,9 @@ template <typename R> struct SceneView {
     DSky<R> sky;
     R water_level;
     int water_material;
+    // 0 when no material is water or has transparency > 0.8: a shadow
+    // segment is then blocked exactly when anything lies on it (any-hit)
+    int has_transmissive;
     int n_lights;
     DLight<R> lights[8];
 };

@@ -1,0 +1,365 @@
+// mh_wavefront.h - wavefront (queue-based) path tracer, fp32.
+//
+// The per-path bounce loop of the reference (src/pathtrace.c:800-1028) is
+// split into kernels that each run over a compacted queue:
+//
+//   gen       start a batch of paths: camera ray, seed, throughput 1
+//   extend    nearest hit of every queued path (small register footprint,
+//             high occupancy: the traversal is the HBM-bound part)
+//   shade     resolve the previous bounce's next-event estimates, add
+//             emission / sky, emit this bounce's NEE shadow rays into the
+//             shadow queue, sample the BSDF, Russian roulette, and append
+//             the survivors to the next extension queue
+//   shadow    any-hit test of every queued shadow ray (or the transmissive
+//             segment walk), setting one visibility bit per estimate
+//   finalize  resolve the last estimates of paths that ended and add the
+//             path radiance to its pixel
+//
+// A batch holds at most one sample per pixel (batch size <= pixel count), and
+// batches advance through the samples in increasing order, so each pixel's
+// radiance is summed in the reference's sample order without atomics.
+// Appends use one atomic per wave: ballot of the predicate, popcount, and the
+// lane's rank from mbcnt. Every kernel is a persistent grid that reads its
+// queue length from device memory, so the bounce sequence of a batch has
+// fixed launch parameters and is captured once into a hipGraph.
+#pragma once
+
+#include "mh_kernels_impl.h"
+
+namespace mh {
+
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+
+// Append `want` lanes of the wave to a queue: one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want, bool* ok)
+{
+    uint64_t mask = __ballot(want);
+    *ok = want;
+    if (!mask) return 0;
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    return base + rank;
+}
+
+__global__ __launch_bounds__(64) void k_wf_reset(uint32_t* cnt, int which)
+{
+    // zero the output queue of this bounce plus the shadow / term counters
+    if (threadIdx.x == 0) {
+        cnt[which] = 0;
+        cnt[2] = 0;
+        cnt[3] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_wf_gen(SceneView<float> sc, CamView<float> cam, WfJob job, WfState st)
+{
+    uint32_t g0 = st.batch[0];
+    long long remain = job.total - (long long)g0;
+    int n = (int)(remain < job.pool ? remain : job.pool);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.cnt[0] = (uint32_t)(n > 0 ? n : 0);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t g = g0 + (uint32_t)i;
+        uint32_t pix = g % (uint32_t)job.npix;
+        uint32_t s = (uint32_t)job.s0 + g / (uint32_t)job.npix;
+        uint32_t x = pix % (uint32_t)job.w, y = pix / (uint32_t)job.w;
+        float jx = rand01<float>(x, y, s * 2u + 0u), jy = rand01<float>(x, y, s * 2u + 1u);
+        float l1 = rand01<float>(x, y, s * 2u + 100u), l2 = rand01<float>(x, y, s * 2u + 101u);
+        float uu = ((float)x + jx) / (float)job.w;
+        float vv = 1.0f - ((float)y + jy) / (float)job.h;
+        V3<float> o, d;
+        camera_ray(cam, uu, vv, l1, l2, &o, &d);
+        if (cam.dome > 1e-4f && sc.water_material >= 0) {
+            V3<float> pd = o + d * cam.dome;
+            if (pd.y < sc.water_level) o = pd - mk(0.f, 1e-3f, 0.f);
+        }
+        int medium = -1;
+        if (sc.water_material >= 0 && o.y < sc.water_level && sc.water_material < sc.n_mats) medium = sc.water_material;
+        d = normalize(d);
+        uint32_t seed = hash3(x, y, s);
+        uint32_t meta = 0u | (1u << 8) | ((uint32_t)(medium + 1) << 9);
+        st.p0[i] = make_float4(o.x, o.y, o.z, u2f(seed));
+        st.p1[i] = make_float4(d.x, d.y, d.z, u2f(len2(d) > 0.f ? meta : 0xffffffffu));
+        st.p2[i] = make_float4(1.f, 1.f, 1.f, u2f(pix));
+        st.p3[i] = make_float4(0.f, 0.f, 0.f, u2f(s));
+        st.npend[i] = 0;
+        st.q[0][i] = (uint32_t)i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc, WfState st, int which)
+{
+    uint32_t n = st.cnt[which];
+    const uint32_t* q = st.q[which];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t slot = q[i];
+        float4 a = st.p0[slot], b = st.p1[slot];
+        float t;
+        int prim;
+        int rec = closest_hit(sc, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), 1e-4f, 1e30f, &t, &prim);
+        st.hit[slot] = make_float2(t, __int_as_float(rec));
+    }
+}
+
+// add the visible pending estimates of the previous bounce, in estimate order
+__device__ __forceinline__ V3<float> resolve_pending(const WfState& st, uint32_t slot, V3<float> L)
+{
+    uint32_t np = st.npend[slot];
+    if (np == 0) return L;
+    uint32_t vis = st.vis[slot];
+    int nl = (int)(np & 15u);        // light estimates 0..nl-1
+    bool sun = (np >> 4) & 1u;       // sun estimate in slot 8
+    V3<float> sum = zero3<float>();
+    for (int k = 0; k < nl; ++k)
+        if (vis & (1u << k)) {
+            float4 c = st.pend[(size_t)slot * 9 + k];
+            sum = sum + mk(c.x, c.y, c.z);
+        }
+    L = L + sum;
+    if (sun && (vis & (1u << 8))) {
+        float4 c = st.pend[(size_t)slot * 9 + 8];
+        L = L + mk(c.x, c.y, c.z);
+    }
+    return L;
+}
+
+__global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc, WfState st, WfJob job, int which, int bounce,
+                                                   unsigned long long* rays)
+{
+    uint32_t n = st.cnt[which];
+    const uint32_t* qin = st.q[which];
+    uint32_t* qout = st.q[which ^ 1];
+    int nq = 0;
+    uint32_t n_round = (n + 63u) & ~63u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += gridDim.x * blockDim.x) {
+        bool active = i < n;
+        uint32_t slot = active ? qin[i] : 0u;
+        bool alive = false, ended = false;
+        V3<float> nray_o = zero3<float>(), nray_d = zero3<float>();
+        if (active) {
+            float4 a = st.p0[slot], b = st.p1[slot], c = st.p2[slot], e = st.p3[slot];
+            float2 hh = st.hit[slot];
+            V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+            V3<float> beta = mk(c.x, c.y, c.z), L = mk(e.x, e.y, e.z);
+            uint32_t seed = f2u(a.w), meta = f2u(b.w);
+            int last_spec = (meta >> 8) & 1u;
+            int medium = (int)(meta >> 9) - 1;
+            int b_idx = bounce;
+            int rec = __float_as_int(hh.y);
+            L = resolve_pending(st, slot, L);
+            st.npend[slot] = 0;
+            ended = true;
+            if (meta == 0xffffffffu) {
+                // degenerate primary ray: zero radiance
+            } else if (rec == INT_MIN) {
+                if (medium >= 0) apply_medium(sc.mats[medium], 5.f, &beta, &L);
+                L = L + vmul(beta, sky_sample(d, sc.sky));
+            } else {
+                DHit<float> h;
+                finish_hit(sc, rec, o, d, 1e-4f, hh.x, &h);
+                if (medium >= 0) apply_medium(sc.mats[medium], h.t, &beta, &L);
+                if (h.mat < 0 || h.mat >= sc.n_mats) {
+                    L = L + vmul(beta, sky_sample(d, sc.sky));
+                } else {
+                    const DMat<float>& m = sc.mats[h.mat];
+                    V3<float> P = h.p, N = h.n;
+                    if (m.is_water && fabsf(h.n.y) > 0.999f && fabsf(h.n.x) < 1e-4f && fabsf(h.n.z) < 1e-4f)
+                        N = water_normal<float>(P.x, P.z);
+                    if (dot(N, d) > 0.f) N = -N;
+                    N = normalize(N);
+                    if (len2(N) > 0.f) {
+                        V3<float> V = -d;
+                        V3<float> alb = texture_albedo(m, P, h.u, h.v);
+                        if ((b_idx == 0 || last_spec) && m.has_emission) L = L + vmul(beta, m.emissive);
+                        if (b_idx == 0 && m.has_glow) L = L + vmul(beta, limb_glow(m, N, V, sc.sky.sun_dir));
+                        // ---- next-event estimates -> shadow queue
+                        uint32_t npend = 0;
+                        V3<float> so = P + N * 1e-3f;
+                        for (int li = 0; li < sc.n_lights; ++li) {
+                            const DLight<float>& lt = sc.lights[li];
+                            bool emit = false;
+                            V3<float> wi = zero3<float>(), cc = zero3<float>();
+                            float tmax = 0.f;
+                            if (lt.prim != h.prim) {
+                                V3<float> toc = lt.center - P;
+                                float dc2 = len2(toc), dc = sqrtf(dc2);
+                                if (dc > lt.radius) {
+                                    float cmx = sqrtf(1.f - (lt.radius * lt.radius) / dc2);
+                                    float omega = 2.f * Consts<float>::pi * (1.f - cmx);
+                                    if (omega > 0.f) {
+                                        V3<float> w = toc * (1.f / dc);
+                                        float e1 = rand01<float>(seed ^ (uint32_t)li, (uint32_t)b_idx, 0xA201u);
+                                        float e2 = rand01<float>(seed ^ (uint32_t)li, (uint32_t)b_idx, 0xA202u);
+                                        wi = cone_dir(w, cmx, e1, e2);
+                                        if (finite3(wi) && dot(N, wi) > 0.f) {
+                                            PD<float> lp;
+                                            lp.kind = K_SPHERE;
+                                            lp.p0 = lt.center;
+                                            lp.r0 = lt.radius;
+                                            float lt_t;
+                                            if (hit_sphere<float, false>(lp, so, wi, 1e-3f, 1e30f, &lt_t, nullptr)) {
+                                                emit = true;
+                                                tmax = lt_t - 1e-3f;
+                                                cc = vmul(beta, shade(m, alb, N, wi, V, lt.emissive) * omega);
+                                            }
+                                        }
+                                    }
+                                }
+                            }
+                            bool ok;
+                            uint32_t at = wave_append(&st.cnt[2], emit, &ok);
+                            if (ok) {
+                                st.sh0[at] = make_float4(so.x, so.y, so.z, tmax);
+                                st.sh1[at] = make_float4(wi.x, wi.y, wi.z, u2f((slot << 4) | (uint32_t)li));
+                                st.pend[(size_t)slot * 9 + li] = make_float4(cc.x, cc.y, cc.z, 0.f);
+                            }
+                        }
+                        npend = (uint32_t)sc.n_lights;
+                        {
+                            bool emit = false;
+                            V3<float> Ls = zero3<float>(), cc = zero3<float>();
+                            if (b_idx == 0 || (b_idx == 1 && medium >= 0 && last_spec)) {
+                                float s1 = rand01<float>(seed, (uint32_t)b_idx, 0xA101u);
+                                float s2 = rand01<float>(seed, (uint32_t)b_idx, 0xA102u);
+                                Ls = sun_disk_dir(sc.sky.sun_dir, sc.sky.sun_radius, s1, s2);
+                                if (finite3(Ls) && dot(N, Ls) > 0.f) {
+                                    emit = true;
+                                    cc = vmul(beta, shade(m, alb, N, Ls, V, sc.sky.sun_color) * sc.sky.sun_omega);
+                                }
+                            }
+                            bool ok;
+                            uint32_t at = wave_append(&st.cnt[2], emit, &ok);
+                            if (ok) {
+                                st.sh0[at] = make_float4(so.x, so.y, so.z, 1e30f);
+                                st.sh1[at] = make_float4(Ls.x, Ls.y, Ls.z, u2f((slot << 4) | 8u));
+                                st.pend[(size_t)slot * 9 + 8] = make_float4(cc.x, cc.y, cc.z, 0.f);
+                                npend |= 16u;
+                            }
+                        }
+                        st.vis[slot] = 0u;
+                        st.npend[slot] = npend;
+                        // ---- BSDF sample, throughput, roulette
+                        uint32_t kb = hash3(seed ^ 0x5054u, (uint32_t)b_idx, 0u);
+                        float u1 = rand01<float>(kb, (uint32_t)b_idx, 0xB501u);
+                        float u2 = rand01<float>(kb, (uint32_t)b_idx, 0xB502u);
+                        float ul = rand01<float>(kb, (uint32_t)b_idx, 0x1001u);
+                        float ug = rand01<float>(kb, (uint32_t)b_idx, 0x1002u);
+                        Lobe<float> lb;
+                        bool ok = m.pbr ? sample_pbr(m, alb, N, d, V, u1, u2, ul, &lb)
+                                        : sample_legacy(m, alb, N, d, V, h.front, u1, u2, ul, ug, &lb);
+                        if (ok) {
+                            last_spec = lb.spec;
+                            beta = vmul(beta, lb.w);
+                            float mc = fmaxf(fmaxf(beta.x, beta.y), beta.z);
+                            if (!(mc > 0.f)) mc = 0.f;
+                            if (finite3(beta) && mc > 0.f) {
+                                if (lb.refr && (m.is_water || m.beer_lambert)) medium = h.front ? h.mat : -1;
+                                bool live = true;
+                                if (b_idx >= 2) {
+                                    float q = clamp01n(mc);
+                                    if (!(q > 0.f) || rand01<float>(kb, (uint32_t)b_idx, 0x5201u) >= q) live = false;
+                                    else beta = beta * (1.f / q);
+                                }
+                                if (live && b_idx < job.depth) {
+                                    V3<float> off = lb.refr ? lb.wi * 1e-4f
+                                                            : N * (dot(lb.wi, N) >= 0.f ? 1e-4f : -1e-4f);
+                                    nray_o = P + off;
+                                    nray_d = lb.wi;
+                                    alive = true;
+                                    ended = false;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            uint32_t nmeta = (uint32_t)(b_idx + 1) | ((uint32_t)last_spec << 8) | ((uint32_t)(medium + 1) << 9);
+            if (alive) {
+                st.p0[slot] = make_float4(nray_o.x, nray_o.y, nray_o.z, a.w);
+                st.p1[slot] = make_float4(nray_d.x, nray_d.y, nray_d.z, u2f(nmeta));
+            }
+            st.p2[slot] = make_float4(beta.x, beta.y, beta.z, c.w);
+            st.p3[slot] = make_float4(L.x, L.y, L.z, e.w);
+            nq += 1;
+        }
+        bool ok;
+        uint32_t at = wave_append(&st.cnt[which ^ 1], alive, &ok);
+        if (ok) qout[at] = slot;
+        uint32_t at2 = wave_append(&st.cnt[3], ended, &ok);
+        if (ok) st.term[at2] = slot;
+    }
+    count_rays(rays, nq);
+}
+
+__global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc, WfState st, unsigned long long* rays)
+{
+    uint32_t n = st.cnt[2];
+    int nq = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 a = st.sh0[i], b = st.sh1[i];
+        uint32_t tag = f2u(b.w);
+        uint32_t slot = tag >> 4, k = tag & 15u;
+        V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+        V3<float> att;
+        bool occ = occluded_segment(sc, o, d, 1e-3f, a.w, &att, &nq);
+        if (!occ && k == 8 && att.x < 1e-4f && att.y < 1e-4f && att.z < 1e-4f) occ = true;
+        if (!occ) {
+            if (sc.has_transmissive) {
+                float4 c = st.pend[(size_t)slot * 9 + k];
+                st.pend[(size_t)slot * 9 + k] = make_float4(c.x * att.x, c.y * att.y, c.z * att.z, 0.f);
+            }
+            atomicOr(&st.vis[slot], 1u << k);
+        }
+    }
+    count_rays(rays, nq);
+}
+
+__global__ __launch_bounds__(256) void k_wf_finalize(WfState st)
+{
+    uint32_t n = st.cnt[3];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t slot = st.term[i];
+        float4 c = st.p2[slot], e = st.p3[slot];
+        V3<float> L = resolve_pending(st, slot, mk(e.x, e.y, e.z));
+        st.npend[slot] = 0;
+        if (!finite3(L)) L = zero3<float>();
+        L.x = L.x < 0.f ? 0.f : L.x;
+        L.y = L.y < 0.f ? 0.f : L.y;
+        L.z = L.z < 0.f ? 0.f : L.z;
+        uint32_t pix = f2u(c.w);
+        // one sample per pixel per batch: no other thread touches this pixel
+        st.acc[(size_t)pix * 3] += L.x;
+        st.acc[(size_t)pix * 3 + 1] += L.y;
+        st.acc[(size_t)pix * 3 + 2] += L.z;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_wf_next_batch(WfState st, int pool)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.batch[0] += (uint32_t)pool;
+}
+
+__global__ __launch_bounds__(256) void k_wf_clear(WfState st, int npix)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npix * 3; i += gridDim.x * blockDim.x) st.acc[i] = 0.f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.batch[0] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_wf_resolve(WfState st, int npix, int spp, LaunchOut out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+        float k = 1.0f / (float)spp;
+        float r = st.acc[(size_t)i * 3] * k, g = st.acc[(size_t)i * 3 + 1] * k, b = st.acc[(size_t)i * 3 + 2] * k;
+        out.linear[(size_t)i * 3] = r;
+        out.linear[(size_t)i * 3 + 1] = g;
+        out.linear[(size_t)i * 3 + 2] = b;
+        out.rgb[(size_t)i * 3] = to_byte(r);
+        out.rgb[(size_t)i * 3 + 1] = to_byte(g);
+        out.rgb[(size_t)i * 3 + 2] = to_byte(b);
+    }
+}
+
+} // namespace mh

@@ -95,9 +95,13 @@ class Hit(C.Structure):
                 ("u", C.c_double), ("v", C.c_double)]
 
 
+ENGINE_MEGAKERNEL, ENGINE_WAVEFRONT = 0, 1
+
+
 class RenderParams(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("width", "height", "spp", "max_depth", "mode",
-                                          "precision", "sample_offset", "block_size")]
+                                          "precision", "sample_offset", "engine", "pool",
+                                          "grid_blocks", "use_graph", "reserved")]
 
 
 class Timing(C.Structure):

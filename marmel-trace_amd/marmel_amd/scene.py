@@ -191,9 +191,11 @@ class GpuScene:
         return self._L.mh_scene_device_bytes(self._h)
 
     def render(self, cam, width, height, spp, depth, mode=abi.MODE_PATHTRACE,
-               precision=abi.PREC_FP32, sample_offset=0, fetch=True):
+               precision=abi.PREC_FP32, sample_offset=0, fetch=True,
+               engine=abi.ENGINE_MEGAKERNEL, pool=0, grid_blocks=0, use_graph=False):
         p = abi.RenderParams(width=width, height=height, spp=spp, max_depth=depth, mode=mode,
-                             precision=precision, sample_offset=sample_offset, block_size=0)
+                             precision=precision, sample_offset=sample_offset, engine=engine,
+                             pool=pool, grid_blocks=grid_blocks, use_graph=int(use_graph))
         rgb = lin = None
         if fetch:
             rgb = np.zeros((height, width, 3), np.uint8)
@@ -204,14 +206,14 @@ class GpuScene:
 
     def pathtrace_render(self, cam, width, height, spp, depth):
         """Drop-in form of reference pathtrace_render: returns (code, rgb)."""
-        rgb = np.zeros((height, width, 3), np.uint8)
+        rgb = np.zeros((max(height, 1), max(width, 1), 3), np.uint8)
         rc = self._L.mh_pathtrace_render(self._h, C.byref(cam), width, height, spp, depth,
                                          abi.ptr(rgb))
         return rc, rgb
 
     def render_image(self, cam, width, height, spp, depth):
         """Drop-in form of reference render_image (fixed-spp Whitted)."""
-        rgb = np.zeros((height, width, 3), np.uint8)
+        rgb = np.zeros((max(height, 1), max(width, 1), 3), np.uint8)
         rc = self._L.mh_render_image(self._h, C.byref(cam), width, height, spp, depth,
                                      abi.ptr(rgb))
         return rc, rgb

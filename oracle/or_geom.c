@@ -635,6 +635,7 @@ int or_intersect1(const or_scene *s, oray r, double tmin, double tmax, ohit *out
                     int pi = s->order[n->first + i];
                     ohit h;
                     or_tls_stats.prims_tested++;
+                    if (s->leaf_prims[n->first + i].kind == MH_PRIM_TRIANGLE) or_tls_stats.tri_tests++;
                     if (or_prim_intersect(&s->leaf_prims[n->first + i], r, tmin, closest, &h)) {
                         h.prim = pi;
                         if (!found || h.t < best.t || (h.t == best.t && pi < best.prim)) {
@@ -702,6 +703,7 @@ int or_occluded1(const or_scene *s, oray r, double tmin, double tmax)
         if (n->count > 0) {
             for (int i = 0; i < n->count; ++i) {
                 or_tls_stats.prims_tested++;
+                if (s->leaf_prims[n->first + i].kind == MH_PRIM_TRIANGLE) or_tls_stats.tri_tests++;
                 if (or_prim_occluded(&s->leaf_prims[n->first + i], r, tmin, tmax)) { hit = 1; break; }
             }
         } else {

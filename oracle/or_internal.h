@@ -110,7 +110,9 @@ typedef struct or_scene {
 typedef struct {
     long long closest_queries, any_queries;
     long long nodes_visited, prims_tested, plane_tests;
+    long long tri_tests, ext_queries;
 } or_stats;
+#define OR_NSTATS 7
 extern __thread or_stats or_tls_stats;
 
 /* or_math.c */

@@ -380,6 +380,7 @@ static dv3 pt_radiance(const or_scene *s, oray primary, int max_depth, unsigned 
     int last_spec = 1;
     for (int b = 0; b <= max_depth; ++b) {
         ohit h;
+        or_tls_stats.ext_queries++;
         if (!or_intersect1(s, r, 1e-4, 1e30, &h)) {
             if (medium) {
                 apply_medium(medium, 5.0, &beta, &L);
@@ -608,13 +609,15 @@ typedef struct {
     int tiles_x, tiles_y;
     atomic_int next;
     pthread_mutex_t lock;
-    long long stats[5];
+    long long stats[OR_NSTATS];
 } job_t;
 
 static void render_pixel(job_t *j, int x, int y)
 {
     const or_scene *s = j->s;
-    dv3 acc = d3(0.0, 0.0, 0.0), acc2 = d3(0.0, 0.0, 0.0);
+    /* acc: the reference's running sum; wm / wq: Welford mean and squared
+     * deviations (numerically stable variance) */
+    dv3 acc = d3(0.0, 0.0, 0.0), wm = d3(0.0, 0.0, 0.0), wq = d3(0.0, 0.0, 0.0);
     int g = (int)(sqrt((double)j->spp) + 0.5);
     for (int si = 0; si < j->spp; ++si) {
         int sidx = j->s0 + si;
@@ -652,7 +655,10 @@ static void render_pixel(job_t *j, int x, int y)
             c = w_trace(s, ray, 0, j->depth, or_hash3((unsigned)x, (unsigned)y, (unsigned)sidx));
         }
         acc = dadd(acc, c);
-        acc2 = dadd(acc2, dmul(c, c));
+        double k = 1.0 / (double)(si + 1);
+        dv3 dl = dsub(c, wm);
+        wm = dadd(wm, dscale(dl, k));
+        wq = dadd(wq, dmul(dl, dsub(c, wm)));
     }
     dv3 m = dscale(acc, 1.0 / (double)j->spp);
     size_t o = ((size_t)y * (size_t)j->w + (size_t)x) * 3;
@@ -669,9 +675,9 @@ static void render_pixel(job_t *j, int x, int y)
     if (j->var) {
         double n = (double)j->spp;
         double k = n > 1.0 ? 1.0 / (n - 1.0) : 0.0;
-        j->var[o] = (acc2.x - acc.x * m.x) * k;
-        j->var[o + 1] = (acc2.y - acc.y * m.y) * k;
-        j->var[o + 2] = (acc2.z - acc.z * m.z) * k;
+        j->var[o] = wq.x * k;
+        j->var[o + 1] = wq.y * k;
+        j->var[o + 2] = wq.z * k;
     }
 }
 
@@ -692,15 +698,15 @@ static void *worker(void *arg)
         for (int y = yy0; y < yy1; ++y)
             for (int x = x0; x < x1; ++x) render_pixel(j, x, y);
     }
-    long long st[5];
+    long long st[OR_NSTATS];
     or_stats_get(st);
     pthread_mutex_lock(&j->lock);
-    for (int i = 0; i < 5; ++i) j->stats[i] += st[i];
+    for (int i = 0; i < OR_NSTATS; ++i) j->stats[i] += st[i];
     pthread_mutex_unlock(&j->lock);
     return NULL;
 }
 
-static long long g_render_stats[5];
+static long long g_render_stats[OR_NSTATS];
 
 int or_render(const or_scene *s, const mh_camera *cam, int width, int height, int spp,
               int depth, int mode, int sample_offset, int y0, int y1, int threads,
@@ -745,17 +751,19 @@ int or_render(const or_scene *s, const mh_camera *cam, int width, int height, in
     return 0;
 }
 
-void or_render_stats_get(long long out[5]) { memcpy(out, g_render_stats, sizeof g_render_stats); }
+void or_render_stats_get(long long out[7]) { memcpy(out, g_render_stats, sizeof g_render_stats); }
 
 void or_stats_reset(void) { memset(&or_tls_stats, 0, sizeof or_tls_stats); }
 
-void or_stats_get(long long out[5])
+void or_stats_get(long long out[7])
 {
     out[0] = or_tls_stats.closest_queries;
     out[1] = or_tls_stats.any_queries;
     out[2] = or_tls_stats.nodes_visited;
     out[3] = or_tls_stats.prims_tested;
     out[4] = or_tls_stats.plane_tests;
+    out[5] = or_tls_stats.tri_tests;
+    out[6] = or_tls_stats.ext_queries;
 }
 
 /* ---- scene ------------------------------------------------------------ */

@@ -54,9 +54,9 @@ int or_render(const or_scene *s, const mh_camera *cam, int width, int height,
 /* Traversal counts accumulated by the calling thread's queries since the
  * last reset (used to price algorithmic bytes per sample). */
 void or_stats_reset(void);
-void or_stats_get(long long out[5]); /* closest, any, nodes, prims, planes */
+void or_stats_get(long long out[7]); /* closest, any, nodes, prims, planes, triangle tests, extension queries */
 /* Counts of a full render (all worker threads), filled by or_render. */
-void or_render_stats_get(long long out[5]);
+void or_render_stats_get(long long out[7]);
 
 /* Pure-function vectors. */
 unsigned or_hash3(unsigned a, unsigned b, unsigned c);

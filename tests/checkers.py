@@ -165,7 +165,7 @@ class OracleScene:
         return out
 
     def stats(self):
-        a = (C.c_longlong * 5)()
+        a = (C.c_longlong * 7)()
         self.L.or_render_stats_get(a)
         return list(a)
 
