@@ -75,7 +75,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-scale", type=int, default=8)
+    ap.add_argument("--cpu-scale", type=int, default=3)
+    ap.add_argument("--engine", choices=["wavefront", "megakernel"], default="wavefront")
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="frames of the other engine timed for the batched-vs-plain ratio")
+    ap.add_argument("--pool", type=int, default=0, help="wavefront paths per batch (0 = default)")
+    ap.add_argument("--grid-blocks", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -94,22 +99,38 @@ def main():
     log(f"bench: device {name}, scene resident: {g.device_bytes / 2**20:.1f} MiB")
     cam = hs.camera(W / H)
 
+    engines = {"wavefront": dict(engine=abi.ENGINE_WAVEFRONT, pool=args.pool, grid_blocks=args.grid_blocks,
+                                 use_graph=True),
+               "megakernel": dict(engine=abi.ENGINE_MEGAKERNEL)}
+    main_kw = engines[args.engine]
     for i in range(args.warmup):
-        g.render(cam, W, H, SPP, DEPTH, fetch=False)
+        g.render(cam, W, H, SPP, DEPTH, fetch=False, **main_kw)
         log(f"bench: warmup {i + 1}/{args.warmup} done")
     kernel_ms, rays = [], 0
     t_start = time.perf_counter()
     for i in range(args.steps):
-        g.render(cam, W, H, SPP, DEPTH, fetch=False, sample_offset=i * SPP)
+        g.render(cam, W, H, SPP, DEPTH, fetch=False, sample_offset=i * SPP, **main_kw)
         t = g.last_timing()
         kernel_ms.append(t.kernel_ms)
         rays += t.rays
-        log(f"bench: step {i + 1}/{args.steps} kernel {t.kernel_ms:.1f} ms")
+        log(f"bench: step {i + 1}/{args.steps} {args.engine} {t.kernel_ms:.1f} ms")
     wall = time.perf_counter() - t_start
     ms_step = wall / args.steps * 1e3
     samples = W * H * SPP
     value = samples * args.steps / wall / 1e6
     mean_kernel_ms = sum(kernel_ms) / len(kernel_ms)
+
+    # the plain GPU kernel on the same frame and sample streams (equal RMSE:
+    # both engines evaluate the identical estimator sample for sample)
+    plain = None
+    other = "megakernel" if args.engine == "wavefront" else "wavefront"
+    if args.plain_steps > 0:
+        g.render(cam, W, H, SPP, DEPTH, fetch=False, **engines[other])
+        t0 = time.perf_counter()
+        for i in range(args.plain_steps):
+            g.render(cam, W, H, SPP, DEPTH, fetch=False, sample_offset=i * SPP, **engines[other])
+            log(f"bench: {other} step {i + 1}/{args.plain_steps} {g.last_timing().kernel_ms:.1f} ms")
+        plain = samples * args.plain_steps / (time.perf_counter() - t0) / 1e6
 
     cpu = None
     bytes_per_sample = None
@@ -130,11 +151,14 @@ def main():
         "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "plain GPU path tracer: 1024x1024-grid fBm terrain (2,093,058 tris) "
+        "config": {"workload": "GPU path tracer at scale: 1024x1024-grid fBm terrain (2,093,058 tris) "
                                "+ 200 spheres (8 emissive), 1920x1080, 64 spp, depth 8, NEE + RR",
-                   "mode": "plain", "width": W, "height": H, "spp": SPP, "depth": DEPTH,
+                   "engine": args.engine, "width": W, "height": H, "spp": SPP, "depth": DEPTH,
                    "grid": args.grid},
         "mrays_per_s": round(rays / (sum(kernel_ms) * 1e-3) / 1e6, 1),
+        f"{other}_msamples_per_s": None if plain is None else round(plain, 2),
+        "batched_vs_plain": None if plain is None else round(
+            (value / plain) if args.engine == "wavefront" else (plain / value), 3),
         "roofline": roof,
         "cpu_baseline": cpu,
         "device": name,

@@ -31,6 +31,11 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_pathtrace(SceneView<R> sc, CamView<R> cam, int w, int h, int spp, int s0,
                                                    int depth, int tiles_x, int ntiles, LaunchOut out)
 {
+#if MH_MEGA_LDS
+    MH_STACK_LDS(ts, MH_MEGA_LDS, 256)
+#else
+    MH_STACK_PRIV(ts)
+#endif
     int tile = xcd_tile(blockIdx.x, gridDim.x);
     int nq = 0;
     if (tile < ntiles) {
@@ -53,7 +58,7 @@ __global__ __launch_bounds__(256) void k_pathtrace(SceneView<R> sc, CamView<R> c
                     V3<R> pd = o + d * cam.dome;
                     if (pd.y < sc.water_level) o = pd - mk(R(0), R(1e-3), R(0));
                 }
-                acc = acc + pt_radiance(sc, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq);
+                acc = acc + pt_radiance(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq);
             }
             acc = acc * (R(1) / (R)spp);
             size_t p = ((size_t)y * (size_t)w + (size_t)x) * 3;
@@ -72,6 +77,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_whitted(SceneView<R> sc, CamView<R> cam, int w, int h, int spp, int s0,
                                                  int depth, LaunchOut out, WItem<R>* stack, int cap)
 {
+    MH_STACK_PRIV(ts)
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     int nlanes = gridDim.x * blockDim.x;
     WItem<R>* stk = stack + (size_t)lane * (size_t)cap;
@@ -99,7 +105,7 @@ __global__ __launch_bounds__(256) void k_whitted(SceneView<R> sc, CamView<R> cam
             R l2 = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 3));
             V3<R> o, d;
             camera_ray(cam, uu, vv, l1, l2, &o, &d);
-            acc = acc + whitted_sample(sc, o, d, depth, hash3((uint32_t)x, (uint32_t)y, (uint32_t)s), stk, cap, &ovf,
+            acc = acc + whitted_sample(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, (uint32_t)s), stk, cap, &ovf,
                                        &nq);
         }
         acc = acc * (R(1) / (R)spp);
@@ -119,6 +125,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_closest(SceneView<R> sc, const mh_ray* rays, int n, double tmin, double tmax,
                                                  mh_hit* out)
 {
+    MH_STACK_PRIV(ts)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     mh_ray r = rays[i];
@@ -127,7 +134,7 @@ __global__ __launch_bounds__(256) void k_closest(SceneView<R> sc, const mh_ray* 
     DHit<R> h;
     mh_hit res;
     res.reserved = 0;
-    if (intersect(sc, o, d, (R)tmin, (R)tmax, &h)) {
+    if (intersect(sc, ts, o, d, (R)tmin, (R)tmax, &h)) {
         res.t = (double)h.t;
         res.prim_index = h.prim;
         res.material = h.mat;
@@ -151,12 +158,13 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_any(SceneView<R> sc, const mh_ray* rays, int n, double tmin, double tmax,
                                              uint8_t* out)
 {
+    MH_STACK_PRIV(ts)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     mh_ray r = rays[i];
     V3<R> o = mk((R)r.ox, (R)r.oy, (R)r.oz);
     V3<R> d = unit_dir(mk((R)r.dx, (R)r.dy, (R)r.dz));
-    out[i] = any_hit(sc, o, d, (R)tmin, (R)tmax) ? 1 : 0;
+    out[i] = any_hit(sc, ts, o, d, (R)tmin, (R)tmax) ? 1 : 0;
 }
 
 template <typename R>

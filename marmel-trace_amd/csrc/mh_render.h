@@ -128,20 +128,21 @@ __device__ bool sample_legacy(const DMat<R>& m, V3<R> alb, V3<R> N, V3<R> d, V3<
 
 // Shadow connection through up to 6 closest hits; water and near-clear
 // (transparency > 0.8) surfaces let light through with Beer-Lambert decay.
-template <typename R>
-__device__ bool occluded_segment(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax, V3<R>* att, int* nq)
+template <typename R, typename TS>
+__device__ bool occluded_segment(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin, R tmax, V3<R>* att,
+                                 int* nq)
 {
     *att = mk(R(1), R(1), R(1));
     if (!sc.has_transmissive) {
         // every first hit would be opaque: blocked iff any hit in [tmin, tmax]
         ++*nq;
-        return any_hit(sc, o, unit_dir(d), tmin, tmax);
+        return any_hit(sc, ts, o, unit_dir(d), tmin, tmax);
     }
     R rem = tmax;
     for (int step = 0; step < 6; ++step) {
         DHit<R> h;
         ++*nq;
-        if (!intersect(sc, o, d, tmin, rem, &h)) return false;
+        if (!intersect(sc, ts, o, d, tmin, rem, &h)) return false;
         if (h.mat < 0 || h.mat >= sc.n_mats) return true;
         const DMat<R>& m = sc.mats[h.mat];
         if (m.is_water || m.transparency > R(0.8)) {
@@ -170,8 +171,9 @@ __device__ void apply_medium(const DMat<R>& m, R dist, V3<R>* beta, V3<R>* L)
     *beta = vmul(*beta, Tm);
 }
 
-template <typename R>
-__device__ V3<R> pt_radiance(const SceneView<R>& sc, V3<R> ro, V3<R> rd0, int max_depth, uint32_t seed, int* nq)
+template <typename R, typename TS>
+__device__ V3<R> pt_radiance(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0, int max_depth, uint32_t seed,
+                             int* nq)
 {
     V3<R> L = zero3<R>();
     V3<R> o = ro, d = normalize(rd0);
@@ -183,7 +185,7 @@ __device__ V3<R> pt_radiance(const SceneView<R>& sc, V3<R> ro, V3<R> rd0, int ma
     for (int b = 0; b <= max_depth; ++b) {
         DHit<R> h;
         ++*nq;
-        if (!intersect(sc, o, d, R(1e-4), R(1e30), &h)) {
+        if (!intersect(sc, ts, o, d, R(1e-4), R(1e30), &h)) {
             if (medium >= 0) {
                 apply_medium(sc.mats[medium], R(5), &beta, &L);
                 medium = -1;
@@ -231,7 +233,7 @@ __device__ V3<R> pt_radiance(const SceneView<R>& sc, V3<R> ro, V3<R> rd0, int ma
             R lt_t;
             if (!hit_sphere<R, false>(lp, so, unit_dir(wi), R(1e-3), R(1e30), &lt_t, nullptr)) continue;
             V3<R> att;
-            if (occluded_segment(sc, so, wi, R(1e-3), lt_t - R(1e-3), &att, nq)) continue;
+            if (occluded_segment(sc, ts, so, wi, R(1e-3), lt_t - R(1e-3), &att, nq)) continue;
             V3<R> lit = shade(m, alb, N, wi, V, vmul(lt.emissive, att));
             L = L + vmul(beta, lit * omega);
         }
@@ -242,7 +244,7 @@ __device__ V3<R> pt_radiance(const SceneView<R>& sc, V3<R> ro, V3<R> rd0, int ma
             V3<R> Ls = sun_disk_dir(sc.sky.sun_dir, sc.sky.sun_radius, s1, s2);
             if (finite3(Ls) && dot(N, Ls) > R(0)) {
                 V3<R> att;
-                if (!occluded_segment(sc, P + N * R(1e-3), Ls, R(1e-3), R(1e30), &att, nq)) {
+                if (!occluded_segment(sc, ts, P + N * R(1e-3), Ls, R(1e-3), R(1e30), &att, nq)) {
                     if (!(att.x < R(1e-4) && att.y < R(1e-4) && att.z < R(1e-4))) {
                         V3<R> lit = shade(m, alb, N, Ls, V, vmul(sc.sky.sun_color, att));
                         L = L + vmul(beta, lit * sc.sky.sun_omega);
@@ -332,8 +334,9 @@ template <typename R> struct WItem {
     int medium; // material index of the absorbing medium, or -1
 };
 
-template <typename R>
-__device__ V3<R> emissive_direct(const SceneView<R>& sc, const DMat<R>& m, V3<R> alb, V3<R> P, V3<R> N, V3<R> V,
+template <typename R, typename TS>
+__device__ V3<R> emissive_direct(const SceneView<R>& sc, TS ts, const DMat<R>& m, V3<R> alb, V3<R> P, V3<R> N,
+                                 V3<R> V,
                                  int self, uint32_t seed, int* nq)
 {
     V3<R> sum = zero3<R>();
@@ -361,7 +364,7 @@ __device__ V3<R> emissive_direct(const SceneView<R>& sc, const DMat<R>& m, V3<R>
             R lt_t;
             if (!hit_sphere<R, false>(lp, so, unit_dir(wi), R(1e-3), R(1e30), &lt_t, nullptr)) continue;
             ++*nq;
-            if (any_hit(sc, so, unit_dir(wi), R(1e-3), lt_t - R(1e-3))) continue;
+            if (any_hit(sc, ts, so, unit_dir(wi), R(1e-3), lt_t - R(1e-3))) continue;
             sum = sum + shade(m, alb, N, wi, V, lt.emissive * solid);
         }
     }
@@ -370,8 +373,9 @@ __device__ V3<R> emissive_direct(const SceneView<R>& sc, const DMat<R>& m, V3<R>
 
 // Shade one Whitted hit: add weight x local terms to acc, push children.
 // Returns false on stack overflow.
-template <typename R>
-__device__ bool whitted_hit(const SceneView<R>& sc, const WItem<R>& it, const DHit<R>& h, int max_depth, uint32_t seed,
+template <typename R, typename TS>
+__device__ bool whitted_hit(const SceneView<R>& sc, TS ts, const WItem<R>& it, const DHit<R>& h, int max_depth,
+                            uint32_t seed,
                             V3<R>* acc, WItem<R>* stk, int* sp, int cap, int* nq)
 {
     if (it.depth > max_depth) return true;
@@ -397,14 +401,15 @@ __device__ bool whitted_hit(const SceneView<R>& sc, const WItem<R>& it, const DH
             R r2 = rand01<R>(k, (uint32_t)i, 0x7c3du);
             V3<R> Ld = sun_disk_dir(Ls, sc.sky.sun_radius, r1, r2);
             ++*nq;
-            if (!any_hit(sc, so, unit_dir(Ld), R(1e-3), R(1e30))) lit += R(1);
+            if (!any_hit(sc, ts, so, unit_dir(Ld), R(1e-3), R(1e30))) lit += R(1);
         }
         if (lit > R(0)) color = color + shade(m, alb, N, Ls, V, sc.sky.sun_color) * (lit / R(16));
     } else {
         ++*nq;
-        if (!any_hit(sc, so, unit_dir(Ls), R(1e-3), R(1e30))) color = color + shade(m, alb, N, Ls, V, sc.sky.sun_color);
+        if (!any_hit(sc, ts, so, unit_dir(Ls), R(1e-3), R(1e30)))
+            color = color + shade(m, alb, N, Ls, V, sc.sky.sun_color);
     }
-    if (sc.n_lights > 0) color = color + emissive_direct(sc, m, alb, P, N, V, h.prim, seed, nq);
+    if (sc.n_lights > 0) color = color + emissive_direct(sc, ts, m, alb, P, N, V, h.prim, seed, nq);
 
     // Fresnel at the refracted cosine (Schlick), TIR detection
     R f0 = (R(1) - m.ior) / (R(1) + m.ior);
@@ -481,8 +486,8 @@ __device__ bool whitted_hit(const SceneView<R>& sc, const WItem<R>& it, const DH
     return true;
 }
 
-template <typename R>
-__device__ V3<R> whitted_sample(const SceneView<R>& sc, V3<R> o, V3<R> d, int max_depth, uint32_t seed,
+template <typename R, typename TS>
+__device__ V3<R> whitted_sample(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, int max_depth, uint32_t seed,
                                 WItem<R>* stk, int cap, int* overflow, int* nq)
 {
     V3<R> acc = zero3<R>();
@@ -498,7 +503,7 @@ __device__ V3<R> whitted_sample(const SceneView<R>& sc, V3<R> o, V3<R> d, int ma
         if (it.depth > max_depth && it.medium == -1) continue;
         DHit<R> h;
         ++*nq;
-        bool hit = intersect(sc, it.o, unit_dir(it.d), R(1e-4), R(1e30), &h);
+        bool hit = intersect(sc, ts, it.o, unit_dir(it.d), R(1e-4), R(1e30), &h);
         if (it.medium >= 0) {
             // Beer-Lambert blend over the in-medium segment (5 when it escapes)
             const DMat<R>& mm = sc.mats[it.medium];
@@ -512,7 +517,7 @@ __device__ V3<R> whitted_sample(const SceneView<R>& sc, V3<R> o, V3<R> d, int ma
             continue;
         }
         if (it.depth > max_depth) continue;
-        if (!whitted_hit(sc, it, h, max_depth, seed, &acc, stk, &sp, cap, nq)) {
+        if (!whitted_hit(sc, ts, it, h, max_depth, seed, &acc, stk, &sp, cap, nq)) {
             *overflow = 1;
             break;
         }

@@ -455,6 +455,109 @@ __device__ __forceinline__ bool occ_prim(const PD<R>& p, V3<R> o, V3<R> d, R tmi
 #define MH_STACK 64
 #endif
 
+// Traversal stacks. A stack policy is a small per-lane handle type passed by
+// value down to the traversal routines; each traversal builds its own
+// Lane<KeepT> from it, so the stack pointer lives in that traversal's
+// registers. KeepT: entries also carry their entry distance (closest hit culls
+// on it); any-hit keeps node references only. Host-built trees have depth
+// <= 60, so MH_STACK = 64 entries never overflow; indices are masked so that
+// even a malformed tree cannot address outside the arrays.
+//
+// PrivStack: the whole stack in private (scratch) memory, cached in L1/L2.
+// LdsStack<K>: the top K entries of each lane in LDS, lane-interleaved so a
+// wave's pushes hit consecutive banks; when full, the OLDEST entry moves to a
+// private overflow array.
+struct PrivStack {
+    template <bool KeepT> struct Lane {
+        int sp;
+        int pref[MH_STACK];
+        float pt[KeepT ? MH_STACK : 1];
+        __device__ __forceinline__ explicit Lane(PrivStack) : sp(0) {}
+        __device__ __forceinline__ bool empty() const { return sp == 0; }
+        __device__ __forceinline__ void push(int r, float t)
+        {
+            pref[sp & (MH_STACK - 1)] = r;
+            if (KeepT) pt[sp & (MH_STACK - 1)] = t;
+            ++sp;
+        }
+        __device__ __forceinline__ int pop(float* t)
+        {
+            --sp;
+            *t = KeepT ? pt[sp & (MH_STACK - 1)] : 0.f;
+            return pref[sp & (MH_STACK - 1)];
+        }
+    };
+};
+
+template <int K> struct LdsStack {
+    static_assert(K > 0 && (K & (K - 1)) == 0 && K < MH_STACK, "LDS stack depth must be a power of two");
+    int* ref;   // this lane's first LDS slot; slot k at ref[k * stride]
+    float* tt;  // entry distances, same layout
+    int stride;
+    template <bool KeepT> struct Lane {
+        int* ref;
+        float* tt;
+        int stride;
+        int sp, base, psp;
+        int pref[MH_STACK - K];
+        float pt[KeepT ? MH_STACK - K : 1];
+        __device__ __forceinline__ explicit Lane(LdsStack h)
+            : ref(h.ref), tt(h.tt), stride(h.stride), sp(0), base(0), psp(0)
+        {
+        }
+        __device__ __forceinline__ bool empty() const { return sp == 0 && psp == 0; }
+        __device__ __forceinline__ void push(int r, float t)
+        {
+            if (sp == K) {
+                int j = psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1;
+                pref[j] = ref[base * stride];
+                if (KeepT) pt[j] = tt[base * stride];
+                ++psp;
+                base = (base + 1) & (K - 1);
+                --sp;
+            }
+            int k = (base + sp) & (K - 1);
+            ref[k * stride] = r;
+            if (KeepT) tt[k * stride] = t;
+            ++sp;
+        }
+        __device__ __forceinline__ int pop(float* t)
+        {
+            if (sp > 0) {
+                --sp;
+                int k = (base + sp) & (K - 1);
+                *t = KeepT ? tt[k * stride] : 0.f;
+                return ref[k * stride];
+            }
+            --psp;
+            int j = psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1;
+            *t = KeepT ? pt[j] : 0.f;
+            return pref[j];
+        }
+    };
+};
+
+// Per-kernel stack choice (0 = PrivStack, K = LdsStack<K>), measured on the
+// 1080p terrain workload: LDS pays off for the closest-hit extend kernel,
+// whose short-lived lanes keep the top of the stack hot; the shadow and
+// megakernel traversals run faster from private memory at their occupancy.
+#ifndef MH_EXTEND_LDS
+#define MH_EXTEND_LDS 8
+#endif
+#ifndef MH_SHADOW_LDS
+#define MH_SHADOW_LDS 0
+#endif
+#ifndef MH_MEGA_LDS
+#define MH_MEGA_LDS 0
+#endif
+
+// Stack declarations at the top of a kernel of `threads` lanes per block.
+#define MH_STACK_PRIV(ts) ::mh::PrivStack ts;
+#define MH_STACK_LDS(ts, K, threads)                                                                      \
+    __shared__ int ts##_lds_ref[(K) * (threads)];                                                       \
+    __shared__ float ts##_lds_t[(K) * (threads)];                                                       \
+    ::mh::LdsStack<K> ts{ts##_lds_ref + threadIdx.x, ts##_lds_t + threadIdx.x, (int)blockDim.x};
+
 // Both child slabs of one node. fp32: origin-scaled planes and a widened exit
 // (robust traversal); fp64: the reference's slab arithmetic in double.
 template <typename R> struct Slab;
@@ -519,8 +622,9 @@ template <> struct Slab<double> {
 
 // Closest hit. Returns the leaf-record index of the winner (or -1 - plane
 // slot for a plane, or INT_MIN for a miss) and its t / prim index.
-template <typename R>
-__device__ int closest_hit(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax, R* best_t, int* best_prim)
+template <typename R, typename TS>
+__device__ int closest_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin, R tmax, R* best_t,
+                           int* best_prim)
 {
     int best = INT_MIN;
     int bprim = 0x7fffffff;
@@ -540,16 +644,12 @@ __device__ int closest_hit(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R t
     if (sc.root_ref != INT_MIN) {
         Slab<R> sl;
         sl.init(o, d);
-        int stk[MH_STACK];
-        float stt[MH_STACK];
-        int sp = 0;
-        stk[sp] = sc.root_ref;
-        stt[sp] = (float)tmin;
-        ++sp;
-        while (sp > 0) {
-            --sp;
-            int ref = stk[sp];
-            if (best != INT_MIN && (R)stt[sp] > closest) continue;
+        typename TS::template Lane<true> st(ts);
+        st.push(sc.root_ref, (float)tmin);
+        while (!st.empty()) {
+            float te;
+            int ref = st.pop(&te);
+            if (best != INT_MIN && (R)te > closest) continue;
             if (ref >= 0) {
                 DNode n = sc.nodes[ref];
                 bool hl, hr;
@@ -557,21 +657,13 @@ __device__ int closest_hit(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R t
                 sl.test(n, tmin, closest, &hl, &hr, &tl, &tr);
                 if (hl && hr) {
                     bool lfirst = tl <= tr;
-                    stk[sp] = lfirst ? n.c3.y : n.c3.x;
-                    stt[sp] = lfirst ? tr : tl;
-                    stk[sp + 1] = lfirst ? n.c3.x : n.c3.y;
-                    stt[sp + 1] = lfirst ? tl : tr;
-                    sp += 2;
+                    st.push(lfirst ? n.c3.y : n.c3.x, lfirst ? tr : tl);
+                    st.push(lfirst ? n.c3.x : n.c3.y, lfirst ? tl : tr);
                 } else if (hl) {
-                    stk[sp] = n.c3.x;
-                    stt[sp] = tl;
-                    ++sp;
+                    st.push(n.c3.x, tl);
                 } else if (hr) {
-                    stk[sp] = n.c3.y;
-                    stt[sp] = tr;
-                    ++sp;
+                    st.push(n.c3.y, tr);
                 }
-                if (sp > MH_STACK - 2) sp = MH_STACK - 2; // unreachable for host-built trees (depth checked)
             } else {
                 int code = ~ref;
                 int first = code >> 3, cnt = (code & 7) + 1;
@@ -611,19 +703,20 @@ __device__ __forceinline__ void finish_hit(const SceneView<R>& sc, int rec, V3<R
     }
 }
 
-template <typename R>
-__device__ __forceinline__ bool intersect(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax, DHit<R>* h)
+template <typename R, typename TS>
+__device__ __forceinline__ bool intersect(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin, R tmax,
+                                          DHit<R>* h)
 {
     R t;
     int prim;
-    int rec = closest_hit(sc, o, d, tmin, tmax, &t, &prim);
+    int rec = closest_hit(sc, ts, o, d, tmin, tmax, &t, &prim);
     if (rec == INT_MIN) return false;
     finish_hit(sc, rec, o, d, tmin, t, h);
     return true;
 }
 
-template <typename R>
-__device__ bool any_hit(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax)
+template <typename R, typename TS>
+__device__ bool any_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin, R tmax)
 {
     for (int i = 0; i < sc.n_planes; ++i) {
         PD<R> p = load_prim(&sc.planes[i]);
@@ -632,11 +725,11 @@ __device__ bool any_hit(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax
     if (sc.root_ref == INT_MIN) return false;
     Slab<R> sl;
     sl.init(o, d);
-    int stk[MH_STACK];
-    int sp = 0;
-    stk[sp++] = sc.root_ref;
-    while (sp > 0) {
-        int ref = stk[--sp];
+    typename TS::template Lane<false> st(ts);
+    st.push(sc.root_ref, 0.f);
+    while (!st.empty()) {
+        float te;
+        int ref = st.pop(&te);
         if (ref >= 0) {
             DNode n = sc.nodes[ref];
             bool hl, hr;
@@ -644,15 +737,13 @@ __device__ bool any_hit(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax
             sl.test(n, tmin, tmax, &hl, &hr, &tl, &tr);
             if (hl && hr) {
                 bool lfirst = tl <= tr;
-                stk[sp] = lfirst ? n.c3.y : n.c3.x;
-                stk[sp + 1] = lfirst ? n.c3.x : n.c3.y;
-                sp += 2;
+                st.push(lfirst ? n.c3.y : n.c3.x, 0.f);
+                st.push(lfirst ? n.c3.x : n.c3.y, 0.f);
             } else if (hl) {
-                stk[sp++] = n.c3.x;
+                st.push(n.c3.x, 0.f);
             } else if (hr) {
-                stk[sp++] = n.c3.y;
+                st.push(n.c3.y, 0.f);
             }
-            if (sp > MH_STACK - 2) sp = MH_STACK - 2;
         } else {
             int code = ~ref;
             int first = code >> 3, cnt = (code & 7) + 1;

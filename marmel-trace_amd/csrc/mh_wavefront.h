@@ -92,6 +92,11 @@ __global__ __launch_bounds__(256) void k_wf_gen(SceneView<float> sc, CamView<flo
 
 __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc, WfState st, int which)
 {
+#if MH_EXTEND_LDS
+    MH_STACK_LDS(ts, MH_EXTEND_LDS, 256)
+#else
+    MH_STACK_PRIV(ts)
+#endif
     uint32_t n = st.cnt[which];
     const uint32_t* q = st.q[which];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -99,7 +104,7 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc, WfState 
         float4 a = st.p0[slot], b = st.p1[slot];
         float t;
         int prim;
-        int rec = closest_hit(sc, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), 1e-4f, 1e30f, &t, &prim);
+        int rec = closest_hit(sc, ts, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), 1e-4f, 1e30f, &t, &prim);
         st.hit[slot] = make_float2(t, __int_as_float(rec));
     }
 }
@@ -296,6 +301,11 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc, WfState s
 
 __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc, WfState st, unsigned long long* rays)
 {
+#if MH_SHADOW_LDS
+    MH_STACK_LDS(ts, MH_SHADOW_LDS, 256)
+#else
+    MH_STACK_PRIV(ts)
+#endif
     uint32_t n = st.cnt[2];
     int nq = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -304,7 +314,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc, WfState 
         uint32_t slot = tag >> 4, k = tag & 15u;
         V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
         V3<float> att;
-        bool occ = occluded_segment(sc, o, d, 1e-3f, a.w, &att, &nq);
+        bool occ = occluded_segment(sc, ts, o, d, 1e-3f, a.w, &att, &nq);
         if (!occ && k == 8 && att.x < 1e-4f && att.y < 1e-4f && att.z < 1e-4f) occ = true;
         if (!occ) {
             if (sc.has_transmissive) {

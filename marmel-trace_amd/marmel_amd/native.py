@@ -10,7 +10,9 @@ import os
 from . import abi
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_DIR = os.path.join(PKG_DIR, "lib")
+# MARMEL_LIB_DIR selects another build of the same sources (tuning builds made
+# with `make OUT=<dir> EXTRA=...`); the default is the in-tree build.
+LIB_DIR = os.environ.get("MARMEL_LIB_DIR") or os.path.join(PKG_DIR, "lib")
 
 _hip = None
 _host = None
