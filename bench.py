@@ -8,16 +8,24 @@ the 8 emitters, Russian roulette. One step = one full frame rendered into the
 scene's device framebuffer (scene already resident in HBM).
 
 Prints ONE JSON line. Also reports:
-  roofline      algorithmic bytes per frame (traversal counts of the CPU oracle
-                on a downscaled render of the same scene, priced with the fixed
-                record sizes: 64 B node, 48 B triangle, 32 B sphere, 16 B plane
-                test, 48 B ray+hit per closest query, 36 B per any-hit query,
-                192 B path state + material per bounce, 12 B per sample) over
-                the mean kernel time measured with HIP events on the render
-                stream; peak = MI355X HBM3E 8 TB/s.
+  roofline      for the dominant kernel, k_wf_shadow (the any-hit shadow
+                queries): algorithmic bytes per launch over its average launch
+                duration, both from this run. The duration comes from HIP
+                event pairs the library records around every launch on its
+                render stream during the timed frames (kernel_clock). The
+                bytes are the shadow queries' traversal counts from the CPU
+                oracle (run with shadow segments as one any-hit query, which
+                is what the GPU does and gives the identical image) on a
+                downscaled render of the same scene, priced per query as
+                36 B (32 B ray in + 4 B visibility) + 64 B per interior node
+                + 48 B per triangle + 32 B per sphere tested, scaled to the
+                frame's samples and divided by the launches.
+                peak = MI355X HBM3E 8 TB/s. traffic = measured HBM bytes per
+                launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a
+                copy kernel) when known, else null.
   cpu_baseline  the oracle (FP64 C restatement of the reference renderer,
                 byte-identical to it) on the host cores, on a downscaled
-                (1/8 per axis) render of the same scene at the same spp/depth.
+                (1/3 per axis) render of the same scene at the same spp/depth.
 """
 import argparse
 import json
@@ -29,6 +37,10 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "marmel-trace_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# Measured HBM bytes per k_wf_shadow launch on this workload (rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE, scaled by the copy-kernel calibration); None until
+# measured for the current kernels.
+SHADOW_TRAFFIC_PER_LAUNCH = None
 W, H, SPP, DEPTH = 1920, 1080, 64, 8
 GRID = 1024
 
@@ -45,27 +57,50 @@ def build_scene(grid):
     return hs, info, st
 
 
-def algorithmic_bytes(stats, samples):
-    closest, anyq, nodes, prims, planes, tris, ext = stats
-    sph = prims - tris
-    return (closest * 48 + anyq * 36 + nodes * 64 + tris * 48 + sph * 32 + planes * 16
-            + ext * 192 + samples * 12)
+# Per-unit algorithmic bytes (fixed fp32 record sizes of the device layout).
+B_NODE, B_TRI, B_SPH, B_PLANE = 64, 48, 32, 16
+B_CLOSEST_IO, B_ANY_IO = 48, 36     # ray in (32 B) + hit (16 B) / visibility (4 B)
+B_BOUNCE, B_SAMPLE = 192, 12        # path state + material per bounce; accumulator per sample
 
 
-def cpu_baseline(hs, cam_fn, scale, threads):
-    """Oracle render of a downscaled frame; returns (Msamples/s, bytes/sample, sample, cores)."""
+def shadow_bytes(st):
+    """Bytes of the any-hit shadow queries the stats cover (k_wf_shadow's work)."""
+    return (st["any"] * B_ANY_IO + st["a_inner"] * B_NODE + st["a_tris"] * B_TRI
+            + (st["a_prims"] - st["a_tris"]) * B_SPH)
+
+
+def frame_bytes(st, samples):
+    """Bytes of all kernels of the frame."""
+    return (shadow_bytes(st) + st["closest"] * B_CLOSEST_IO + st["c_inner"] * B_NODE
+            + st["c_tris"] * B_TRI + (st["c_prims"] - st["c_tris"]) * B_SPH
+            + st["planes"] * B_PLANE + st["ext"] * B_BOUNCE + samples * B_SAMPLE)
+
+
+def oracle_scene(hs):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import checkers as ck
     prims, attrs, mats, sky, wl, wm = hs.arrays()
     hold = ck.SoaHolder(prims, attrs, mats, sky, wl, wm)
-    o = ck.OracleScene(hold.soa)
+    return hold, ck.OracleScene(hold.soa)
+
+
+def traversal_stats(o, cam, scale, threads):
+    """GPU-algorithm traversal counts per sample on a downscaled frame."""
     w, h = W // scale, H // scale
-    cam = cam_fn(W / H)
+    o.set_anyhit_shadow(True)
+    try:
+        o.render(cam, w, h, SPP, DEPTH, mode=0, threads=threads)
+    finally:
+        o.set_anyhit_shadow(False)
+    return o.stats(), w * h * SPP, f"{w}x{h}"
+
+
+def cpu_baseline(o, cam, scale, threads):
+    """Oracle render (reference algorithm) of a downscaled frame."""
+    w, h = W // scale, H // scale
     _, _, _, sec = o.render(cam, w, h, SPP, DEPTH, mode=0, threads=threads)
-    st = o.stats()
     samples = w * h * SPP
-    return samples / sec / 1e6, algorithmic_bytes(st, samples) / samples, \
-        f"{w}x{h} downscaled frame of the same scene/camera, {SPP} spp, depth {DEPTH}", sec
+    return samples / sec / 1e6, f"{w}x{h} downscaled frame of the same scene/camera, {SPP} spp, depth {DEPTH}", sec
 
 
 def main():
@@ -76,6 +111,10 @@ def main():
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-scale", type=int, default=3)
+    ap.add_argument("--stats-scale", type=int, default=8,
+                    help="downscale of the oracle render that counts traversal work")
+    ap.add_argument("--graph", action="store_true",
+                    help="time hipGraph replays instead of clocked direct launches (no per-kernel times)")
     ap.add_argument("--engine", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="frames of the other engine timed for the batched-vs-plain ratio")
@@ -84,7 +123,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the path shards by image tiles / samples, see DESIGN.md", file=sys.stderr)
+              "the path would shard by image tiles or sample ranges with one reduction", file=sys.stderr)
         sys.exit(2)
 
     from marmel_amd import GpuScene, abi, device_info
@@ -100,19 +139,24 @@ def main():
     cam = hs.camera(W / H)
 
     engines = {"wavefront": dict(engine=abi.ENGINE_WAVEFRONT, pool=args.pool, grid_blocks=args.grid_blocks,
-                                 use_graph=True),
+                                 use_graph=args.graph, kernel_clock=not args.graph),
                "megakernel": dict(engine=abi.ENGINE_MEGAKERNEL)}
     main_kw = engines[args.engine]
     for i in range(args.warmup):
         g.render(cam, W, H, SPP, DEPTH, fetch=False, **main_kw)
         log(f"bench: warmup {i + 1}/{args.warmup} done")
     kernel_ms, rays = [], 0
+    ktimes = {}
     t_start = time.perf_counter()
     for i in range(args.steps):
         g.render(cam, W, H, SPP, DEPTH, fetch=False, sample_offset=i * SPP, **main_kw)
         t = g.last_timing()
         kernel_ms.append(t.kernel_ms)
         rays += t.rays
+        for k, (ms, n) in g.last_kernel_times().items():
+            a = ktimes.setdefault(k, [0.0, 0])
+            a[0] += ms
+            a[1] += n
         log(f"bench: step {i + 1}/{args.steps} {args.engine} {t.kernel_ms:.1f} ms")
     wall = time.perf_counter() - t_start
     ms_step = wall / args.steps * 1e3
@@ -132,20 +176,32 @@ def main():
             log(f"bench: {other} step {i + 1}/{args.plain_steps} {g.last_timing().kernel_ms:.1f} ms")
         plain = samples * args.plain_steps / (time.perf_counter() - t0) / 1e6
 
+    # host threads: the GPU's share of the host (16 cores), as the reference
+    # caps its pool at min(nproc, 64) (pathtrace.c:1210-1234)
+    threads = min(os.cpu_count() or 1, 16)
     cpu = None
-    bytes_per_sample = None
+    roof = None
+    frame_gbs = None
+    hold, o = oracle_scene(hs)
+    st, st_samples, st_size = traversal_stats(o, cam, args.stats_scale, threads)
+    log(f"bench: traversal counts from a {st_size} oracle render: {st}")
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 64)
-        cpu_msps, bytes_per_sample, sample, sec = cpu_baseline(hs, hs.camera, args.cpu_scale, threads)
+        cpu_msps, sample, sec = cpu_baseline(o, cam, args.cpu_scale, threads)
         log(f"bench: cpu baseline {cpu_msps:.3f} Msamples/s on {threads} threads ({sec:.1f}s)")
         cpu = {"value": round(cpu_msps, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
                "sample": sample}
-    roof = None
-    if bytes_per_sample is not None:
-        achieved = bytes_per_sample * samples / (mean_kernel_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_sample": round(bytes_per_sample, 1), "kernel_ms": round(mean_kernel_ms, 2)}
+    scale = samples / st_samples
+    frame_gbs = frame_bytes(st, st_samples) * scale / (mean_kernel_ms * 1e-3) / 1e9
+    sh_ms, sh_n = ktimes.get("shadow", (0.0, 0))
+    if sh_n > 0:
+        per_launch = shadow_bytes(st) * scale * args.steps / sh_n
+        avg_s = sh_ms / sh_n * 1e-3
+        achieved = per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": "k_wf_shadow", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": SHADOW_TRAFFIC_PER_LAUNCH, "bytes_per_launch": round(per_launch),
+                "avg_launch_us": round(avg_s * 1e6, 1), "launches": sh_n,
+                "kernel_share": round(sh_ms / sum(v[0] for v in ktimes.values()), 3)}
     line = {
         "metric": "Msamples/s at 1080p, batched vs plain GPU at equal RMSE; adjoint ms/grad",
         "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
@@ -156,6 +212,8 @@ def main():
                    "engine": args.engine, "width": W, "height": H, "spp": SPP, "depth": DEPTH,
                    "grid": args.grid},
         "mrays_per_s": round(rays / (sum(kernel_ms) * 1e-3) / 1e6, 1),
+        "frame_algorithmic_gbs": None if frame_gbs is None else round(frame_gbs, 1),
+        "kernel_ms_per_frame": {k: round(v[0] / args.steps, 2) for k, v in ktimes.items() if v[1]},
         f"{other}_msamples_per_s": None if plain is None else round(plain, 2),
         "batched_vs_plain": None if plain is None else round(
             (value / plain) if args.engine == "wavefront" else (plain / value), 3),

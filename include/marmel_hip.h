@@ -243,7 +243,9 @@ typedef struct mh_render_params {
     int32_t pool;           /* wavefront: paths per batch (0 = default, capped at w*h) */
     int32_t grid_blocks;    /* wavefront: persistent blocks of 256 (0 = default)       */
     int32_t use_graph;      /* wavefront: replay the bounce sequence as a hipGraph     */
-    int32_t reserved;
+    int32_t kernel_clock;   /* wavefront: 1 = record a HIP event pair around every
+                               kernel launch (launched directly, not from a graph);
+                               per-kernel totals via mh_last_kernel_times          */
 } mh_render_params;
 
 /* Per-call timing of the last render on the calling thread, measured with
@@ -255,6 +257,23 @@ typedef struct mh_timing {
     int64_t samples;        /* width * height * spp                    */
     int64_t rays;           /* extension + shadow queries (counted)    */
 } mh_timing;
+
+/* Per-kernel totals of the last wavefront render made with kernel_clock = 1:
+ * summed HIP-event durations and launch counts, indexed by MH_K_*. */
+enum {
+    MH_K_GEN = 0,       /* camera rays of a batch                  */
+    MH_K_EXTEND = 1,    /* nearest hit of the extension queue      */
+    MH_K_SHADE = 2,     /* shading, NEE emission, BSDF sampling    */
+    MH_K_SHADOW = 3,    /* any-hit / transmissive shadow queries   */
+    MH_K_FINALIZE = 4,  /* ended paths -> pixel accumulators       */
+    MH_K_OTHER = 5,     /* queue resets, batch advance, resolve    */
+    MH_K_COUNT = 6
+};
+
+typedef struct mh_kernel_times {
+    double  ms[8];
+    int64_t launches[8];
+} mh_kernel_times;
 
 /* ------------------------------------------------------------------ */
 /* Entry points                                                        */
@@ -306,8 +325,61 @@ int mh_render(struct mh_scene *s, const mh_camera *cam,
 int mh_framebuffer_download(struct mh_scene *s, unsigned char *rgb_out,
                             float *linear_out);
 
+/* ------------------------------------------------------------------ */
+/* Batched rendering (no reference counterpart: the reference renders  */
+/* one image per pathtrace_render call, src/pathtrace.c:1256-1363)      */
+/* ------------------------------------------------------------------ */
+
+/* One image of a batch: a camera view of scenes[scene] with its own size,
+ * spp, depth and first sample index. Each job's pixels are the same
+ * estimator as mh_render's path tracer (same sample streams). */
+typedef struct mh_job {
+    int32_t   scene;
+    int32_t   width;
+    int32_t   height;
+    int32_t   spp;
+    int32_t   max_depth;
+    int32_t   sample_offset;
+    mh_camera cam;
+} mh_job;
+
+/* Launch settings of the batched engine. Zeros pick the defaults. On
+ * return from mh_render_batch, mh_batch_last_tune gives what was used. */
+typedef struct mh_tune {
+    int32_t pool;          /* paths in flight per batch (queue capacity); 0 = 2^21       */
+    int32_t grid_blocks;   /* workgroups of each persistent kernel; 0 = 2048             */
+    int32_t block;         /* lanes per workgroup: 64, 128 or 256; 0 = 256               */
+    int32_t use_graph;     /* replay each batch's bounce sequence as a hipGraph          */
+    int32_t autotune;      /* 1 = choose pool / grid_blocks / block by timing trial
+                              batches of this workload (cached per workload on the handle) */
+    int32_t kernel_clock;  /* 1 = per-kernel HIP-event timing (direct launches)          */
+    int32_t bvh_width;     /* BVH node width: 0 or 2 (binary nodes, the one layout built) */
+    int32_t reserved;
+} mh_tune;
+
+struct mh_batch;
+typedef struct mh_batch mh_batch;
+
+/* A batch renderer owns its stream, path state and graph. One handle must
+ * not be used from two threads at once. */
+int  mh_batch_create(mh_batch **out);
+void mh_batch_destroy(mh_batch *b);
+
+/* Render n_jobs images of n_scenes uploaded scenes in one persistent
+ * wavefront launch sequence (fp32). Outputs are concatenated in job order,
+ * each job top-down RGB: rgb_out holds sum(w*h*3) bytes, linear_out the same
+ * count of floats; either may be NULL. Return codes as mh_render; 4 when a
+ * job's max_depth exceeds 254. */
+int mh_render_batch(mh_batch *b, struct mh_scene *const *scenes, int n_scenes,
+                    const mh_job *jobs, int n_jobs, const mh_tune *tune,
+                    unsigned char *rgb_out, float *linear_out);
+
+/* Settings used by the last mh_render_batch on this handle. */
+int mh_batch_last_tune(const mh_batch *b, mh_tune *out);
+
 /* Timing and error state of the last call on the calling thread. */
 int mh_last_timing(mh_timing *out);
+int mh_last_kernel_times(mh_kernel_times *out);  /* no reference counterpart */
 double mh_last_seconds(void);   /* like reference render_last_seconds() */
 const char *mh_last_error(void);
 

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 using namespace mh;
@@ -24,6 +25,7 @@ namespace {
 thread_local std::string g_err;
 thread_local mh_timing g_timing;
 thread_local double g_seconds = 0.0;
+thread_local mh_kernel_times g_ktimes;
 
 int fail(int code, const char* fmt, ...)
 {
@@ -162,6 +164,84 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Event pairs recorded around kernel launches. A pair is read back only when
+// its slot is reused (N launches later) or at the end of the render, so the
+// host never waits on the launches it has just queued.
+class EventClock : public KernelClock {
+public:
+    static const int N = 1024;
+    double ms[8] = {};
+    long long launches[8] = {};
+
+    ~EventClock() override
+    {
+        for (auto e : a_) (void)hipEventDestroy(e);
+        for (auto e : b_) (void)hipEventDestroy(e);
+    }
+    hipError_t start()
+    {
+        if (a_.empty()) {
+            a_.assign(N, nullptr);
+            b_.assign(N, nullptr);
+            kind_.assign(N, -1);
+            for (int i = 0; i < N; ++i) {
+                hipError_t e = hipEventCreate(&a_[i]);
+                if (e == hipSuccess) e = hipEventCreate(&b_[i]);
+                if (e != hipSuccess) return e;
+            }
+        }
+        for (int i = 0; i < N; ++i) kind_[i] = -1;
+        for (int k = 0; k < 8; ++k) {
+            ms[k] = 0.0;
+            launches[k] = 0;
+        }
+        next_ = 0;
+        return hipSuccess;
+    }
+    hipError_t begin(int kind, hipStream_t st) override
+    {
+        (void)kind;
+        int i = (int)(next_ % N);
+        hipError_t e = harvest(i);
+        if (e != hipSuccess) return e;
+        return hipEventRecord(a_[i], st);
+    }
+    hipError_t end(int kind, hipStream_t st) override
+    {
+        int i = (int)(next_ % N);
+        kind_[i] = kind;
+        ++next_;
+        return hipEventRecord(b_[i], st);
+    }
+    hipError_t drain()
+    {
+        for (int i = 0; i < (int)kind_.size(); ++i) {
+            hipError_t e = harvest(i);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+
+private:
+    std::vector<hipEvent_t> a_, b_;
+    std::vector<int> kind_;
+    long long next_ = 0;
+    hipError_t harvest(int i)
+    {
+        int k = kind_[i];
+        if (k < 0) return hipSuccess;
+        hipError_t e = hipEventSynchronize(b_[i]);
+        if (e != hipSuccess) return e;
+        float t = 0.f;
+        e = hipEventElapsedTime(&t, a_[i], b_[i]);
+        if (e != hipSuccess) return e;
+        ms[k & 7] += t;
+        launches[k & 7] += 1;
+        kind_[i] = -1;
+        return hipSuccess;
+    }
+};
+
 } // namespace
 
 struct mh_scene {
@@ -191,6 +271,7 @@ struct mh_scene {
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
     std::vector<unsigned char> graph_key;
+    EventClock clock;
 };
 
 namespace {
@@ -343,7 +424,46 @@ int ensure_query(mh_scene* s, size_t bytes)
     return 0;
 }
 
-// Carve the wavefront state for `pool` paths and `npix` pixels out of one allocation.
+// Carve the wavefront state for `pool` paths and `npix` pixels out of one
+// allocation (`jid`: per-slot job ids of the batched engine).
+int alloc_wf(int pool, size_t npix, bool jid, void** mem, size_t* bytes, WfState* w)
+{
+    size_t P = (size_t)pool;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t sz[] = {al(P * 16), al(P * 16), al(P * 16), al(P * 16), al(P * 8), al(P * 9 * 16), al(P * 4),
+                   al(P * 4),  al(P * 4),  al(P * 4),  al(P * 4),  al(P * 9 * 16), al(P * 9 * 16), al(64),
+                   al(npix * 3 * 4), al(64), jid ? al(P * 4) : 0};
+    const int nbuf = (int)(sizeof sz / sizeof sz[0]);
+    size_t total = 0;
+    for (size_t v : sz) total += v;
+    HIP_TRY(hipMalloc(mem, total));
+    *bytes = total;
+    char* p = (char*)*mem;
+    void* ptrs[nbuf];
+    for (int i = 0; i < nbuf; ++i) {
+        ptrs[i] = sz[i] ? p : nullptr;
+        p += sz[i];
+    }
+    w->p0 = (float4*)ptrs[0];
+    w->p1 = (float4*)ptrs[1];
+    w->p2 = (float4*)ptrs[2];
+    w->p3 = (float4*)ptrs[3];
+    w->hit = (float2*)ptrs[4];
+    w->pend = (float4*)ptrs[5];
+    w->vis = (uint32_t*)ptrs[6];
+    w->npend = (uint32_t*)ptrs[7];
+    w->q[0] = (uint32_t*)ptrs[8];
+    w->q[1] = (uint32_t*)ptrs[9];
+    w->term = (uint32_t*)ptrs[10];
+    w->sh0 = (float4*)ptrs[11];
+    w->sh1 = (float4*)ptrs[12];
+    w->cnt = (uint32_t*)ptrs[13];
+    w->acc = (float*)ptrs[14];
+    w->batch = (unsigned long long*)ptrs[15];
+    w->jid = (uint32_t*)ptrs[16];
+    return 0;
+}
+
 int ensure_wf(mh_scene* s, int pool, size_t npix)
 {
     if (pool <= s->wf_pool && npix <= s->wf_npix) return 0;
@@ -357,38 +477,8 @@ int ensure_wf(mh_scene* s, int pool, size_t npix)
     s->wf_bytes = 0;
     s->wf_pool = 0;
     s->wf_npix = 0;
-    size_t P = (size_t)pool;
-    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    size_t sz[] = {al(P * 16), al(P * 16), al(P * 16), al(P * 16), al(P * 8), al(P * 9 * 16), al(P * 4),
-                   al(P * 4),  al(P * 4),  al(P * 4),  al(P * 4),  al(P * 9 * 16), al(P * 9 * 16), al(64),
-                   al(npix * 3 * 4), al(64)};
-    size_t total = 0;
-    for (size_t v : sz) total += v;
-    HIP_TRY(hipMalloc(&s->wf_mem, total));
-    s->wf_bytes = total;
-    char* p = (char*)s->wf_mem;
-    void* ptrs[16];
-    for (int i = 0; i < 16; ++i) {
-        ptrs[i] = p;
-        p += sz[i];
-    }
-    WfState& w = s->ws;
-    w.p0 = (float4*)ptrs[0];
-    w.p1 = (float4*)ptrs[1];
-    w.p2 = (float4*)ptrs[2];
-    w.p3 = (float4*)ptrs[3];
-    w.hit = (float2*)ptrs[4];
-    w.pend = (float4*)ptrs[5];
-    w.vis = (uint32_t*)ptrs[6];
-    w.npend = (uint32_t*)ptrs[7];
-    w.q[0] = (uint32_t*)ptrs[8];
-    w.q[1] = (uint32_t*)ptrs[9];
-    w.term = (uint32_t*)ptrs[10];
-    w.sh0 = (float4*)ptrs[11];
-    w.sh1 = (float4*)ptrs[12];
-    w.cnt = (uint32_t*)ptrs[13];
-    w.acc = (float*)ptrs[14];
-    w.batch = (uint32_t*)ptrs[15];
+    int rc = alloc_wf(pool, npix, false, &s->wf_mem, &s->wf_bytes, &s->ws);
+    if (rc) return rc;
     s->wf_pool = pool;
     s->wf_npix = npix;
     return 0;
@@ -399,6 +489,181 @@ float ms_between(hipEvent_t a, hipEvent_t b)
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 0.f;
     return ms;
+}
+
+} // namespace
+
+
+// Batched renderer: its own stream, path state, tables and graph.
+struct mh_batch {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    void* wf_mem = nullptr;
+    size_t wf_bytes = 0;
+    int wf_pool = 0;
+    WfState ws{};
+    void* tab = nullptr;          // scene table then job table
+    size_t tab_bytes = 0;
+    unsigned long long* acc64 = nullptr;
+    size_t acc_pix = 0;
+    float* fb_linear = nullptr;
+    unsigned char* fb_rgb = nullptr;
+    size_t fb_pix = 0;
+    unsigned long long* d_rays = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    std::vector<unsigned char> graph_key;
+    EventClock clock;
+    std::vector<std::pair<std::vector<long long>, mh_tune>> tuned;  // autotune cache
+    mh_tune last{};
+};
+
+namespace {
+
+void drop_graph(mh_batch* b)
+{
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
+    b->graph = nullptr;
+    b->graph_def = nullptr;
+    b->graph_key.clear();
+}
+
+template <typename T> int grow(T** p, size_t* have, size_t need, size_t elem)
+{
+    if (need <= *have) return 0;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    HIP_TRY(hipMalloc((void**)p, need * elem));
+    *have = need;
+    return 0;
+}
+
+// Run the batches of one render: `limit` paths (all when < 0) with settings t.
+int batch_run(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, const mh_tune& t, long long limit)
+{
+    long long total = limit >= 0 && limit < mj.total ? limit : mj.total;
+    long long nb = (total + t.pool - 1) / t.pool;
+    if (t.kernel_clock) {
+        HIP_TRY(b->clock.start());
+        for (long long i = 0; i < nb; ++i)
+            HIP_TRY(wf_multi_batch(mj, b->ws, t.pool, max_depth, b->d_rays, t.grid_blocks, t.block, trans,
+                                   b->stream, &b->clock));
+    } else if (t.use_graph) {
+        std::vector<unsigned char> key(sizeof mj + sizeof b->ws + 5 * sizeof(int));
+        int extra[5] = {t.pool, max_depth, t.grid_blocks, t.block, trans ? 1 : 0};
+        memcpy(key.data(), &mj, sizeof mj);
+        memcpy(key.data() + sizeof mj, &b->ws, sizeof b->ws);
+        memcpy(key.data() + sizeof mj + sizeof b->ws, extra, sizeof extra);
+        if (!b->graph || key != b->graph_key) {
+            drop_graph(b);
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = wf_multi_batch(mj, b->ws, t.pool, max_depth, b->d_rays, t.grid_blocks, t.block, trans,
+                                           b->stream);
+            hipError_t ee = hipStreamEndCapture(b->stream, &b->graph_def);
+            HIP_TRY(ce);
+            HIP_TRY(ee);
+            HIP_TRY(hipGraphInstantiate(&b->graph, b->graph_def, nullptr, nullptr, 0));
+            b->graph_key = key;
+        }
+        for (long long i = 0; i < nb; ++i) HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+    } else {
+        for (long long i = 0; i < nb; ++i)
+            HIP_TRY(wf_multi_batch(mj, b->ws, t.pool, max_depth, b->d_rays, t.grid_blocks, t.block, trans, b->stream));
+    }
+    return 0;
+}
+
+int ensure_batch_state(mh_batch* b, int pool)
+{
+    if (pool <= b->wf_pool) return 0;
+    drop_graph(b);
+    if (b->wf_mem) (void)hipFree(b->wf_mem);
+    b->wf_mem = nullptr;
+    b->wf_bytes = 0;
+    b->wf_pool = 0;
+    int rc = alloc_wf(pool, 1, true, &b->wf_mem, &b->wf_bytes, &b->ws);
+    if (rc) return rc;
+    b->wf_pool = pool;
+    return 0;
+}
+
+// Paths per ms of `trial` paths of the workload under settings t.
+int batch_trial(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, mh_tune t, long long trial, double* rate)
+{
+    int rc = ensure_batch_state(b, t.pool);
+    if (rc) return rc;
+    t.use_graph = 0;
+    t.kernel_clock = 0;
+    HIP_TRY(wf_multi_begin(b->ws, mj, b->stream));
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    rc = batch_run(b, mj, max_depth, trans, t, trial);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    float ms = ms_between(b->ev[0], b->ev[1]);
+    long long n = trial < mj.total ? trial : mj.total;
+    long long done = (n + t.pool - 1) / t.pool * (long long)t.pool;  // whole batches ran
+    if (done > mj.total) done = mj.total;
+    *rate = ms > 0.f ? (double)done / ms : 0.0;
+    return 0;
+}
+
+// Coordinate search: pool, then grid blocks, then block size, each by the
+// path rate of a fixed trial of the workload's first paths.
+int autotune(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, mh_tune* t)
+{
+    long long trial = std::min<long long>(mj.total, 1LL << 23);
+    double best = -1.0;
+    mh_tune cur = *t;
+    const int pools[] = {1 << 19, 1 << 20, 1 << 21, 1 << 22};
+    int pool_best = cur.pool;
+    for (int p : pools) {
+        if (p > 2 * trial && p != pools[0]) break;  // larger pools cannot fill up
+        mh_tune c = cur;
+        c.pool = p;
+        double r;
+        int rc = batch_trial(b, mj, max_depth, trans, c, std::max<long long>(trial, p), &r);
+        if (rc) return rc;
+        if (r > best) {
+            best = r;
+            pool_best = p;
+        }
+    }
+    cur.pool = pool_best;
+    best = -1.0;
+    const int grids[] = {1024, 2048, 4096};
+    int grid_best = cur.grid_blocks;
+    for (int g : grids) {
+        mh_tune c = cur;
+        c.grid_blocks = g;
+        double r;
+        int rc = batch_trial(b, mj, max_depth, trans, c, std::max<long long>(trial, cur.pool), &r);
+        if (rc) return rc;
+        if (r > best) {
+            best = r;
+            grid_best = g;
+        }
+    }
+    cur.grid_blocks = grid_best;
+    best = -1.0;
+    const int blocks[] = {128, 256};
+    int block_best = cur.block;
+    for (int bl : blocks) {
+        mh_tune c = cur;
+        c.block = bl;
+        double r;
+        int rc = batch_trial(b, mj, max_depth, trans, c, std::max<long long>(trial, cur.pool), &r);
+        if (rc) return rc;
+        if (r > best) {
+            best = r;
+            block_best = bl;
+        }
+    }
+    cur.block = block_best;
+    *t = cur;
+    return 0;
 }
 
 } // namespace
@@ -655,7 +920,11 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
                   (long long)npix * p->spp, pool};
         long long nb = (job.total + pool - 1) / pool;
         HIP_TRY(wf_enqueue_begin(s->ws, npix, s->stream));
-        if (p->use_graph) {
+        if (p->kernel_clock) {
+            HIP_TRY(s->clock.start());
+            for (long long b = 0; b < nb; ++b)
+                HIP_TRY(wf_enqueue_batch(s->v32, cvw, job, s->ws, s->d_rays, grid, s->stream, &s->clock));
+        } else if (p->use_graph) {
             std::vector<unsigned char> key(sizeof job + sizeof cvw + sizeof grid);
             memcpy(key.data(), &job, sizeof job);
             memcpy(key.data() + sizeof job, &cvw, sizeof cvw);
@@ -724,6 +993,14 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
     g_timing.samples = (int64_t)p->width * p->height * p->spp;
     g_timing.rays = (int64_t)rays;
     g_seconds = g_timing.kernel_ms * 1e-3;
+    memset(&g_ktimes, 0, sizeof g_ktimes);
+    if (p->kernel_clock && p->mode == MH_MODE_PATHTRACE && p->engine == MH_ENGINE_WAVEFRONT) {
+        HIP_TRY(s->clock.drain());
+        for (int k = 0; k < 8; ++k) {
+            g_ktimes.ms[k] = s->clock.ms[k];
+            g_ktimes.launches[k] = s->clock.launches[k];
+        }
+    }
     if (ovf) return fail(6, "Whitted work stack overflow");
     return 0;
 }
@@ -770,10 +1047,203 @@ int mh_render_image(mh_scene* s, const mh_camera* cam, int width, int height, in
     return render_drop_in(s, cam, width, height, spp, max_depth, rgb_out, MH_MODE_WHITTED);
 }
 
+int mh_batch_create(mh_batch** out)
+{
+    if (!out) return fail(1, "NULL argument");
+    *out = nullptr;
+    mh_batch* b = new mh_batch();
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        mh_batch_destroy(b);
+        return fail(101, "hipStreamCreate failed");
+    }
+    for (auto& e : b->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            mh_batch_destroy(b);
+            return fail(101, "hipEventCreate failed");
+        }
+    if (hipMalloc(&b->d_rays, sizeof(unsigned long long)) != hipSuccess) {
+        mh_batch_destroy(b);
+        return fail(102, "hipMalloc failed");
+    }
+    *out = b;
+    return 0;
+}
+
+void mh_batch_destroy(mh_batch* b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    if (b->wf_mem) (void)hipFree(b->wf_mem);
+    if (b->tab) (void)hipFree(b->tab);
+    if (b->acc64) (void)hipFree(b->acc64);
+    if (b->fb_linear) (void)hipFree(b->fb_linear);
+    if (b->fb_rgb) (void)hipFree(b->fb_rgb);
+    if (b->d_rays) (void)hipFree(b->d_rays);
+    for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh_job* jobs, int n_jobs,
+                    const mh_tune* tune, unsigned char* rgb_out, float* linear_out)
+{
+    if (!b || !scenes || !jobs) return fail(1, "NULL argument");
+    if (n_scenes < 1 || n_jobs < 1) return fail(2, "empty batch (%d scenes, %d jobs)", n_scenes, n_jobs);
+    for (int i = 0; i < n_scenes; ++i)
+        if (!scenes[i]) return fail(1, "NULL scene %d", i);
+    mh_tune t{};
+    if (tune) t = *tune;
+    if (t.bvh_width != 0 && t.bvh_width != 2) return fail(4, "BVH node width %d is not built (binary only)", t.bvh_width);
+    if (t.pool < 0 || t.grid_blocks < 0 || (t.block != 0 && t.block != 64 && t.block != 128 && t.block != 256))
+        return fail(4, "bad launch settings (pool %d, grid %d, block %d)", t.pool, t.grid_blocks, t.block);
+
+    std::vector<SceneView<float>> views((size_t)n_scenes);
+    bool trans = false;
+    for (int i = 0; i < n_scenes; ++i) {
+        views[i] = scenes[i]->v32;
+        if (views[i].has_transmissive) trans = true;
+    }
+    std::vector<BJob> bj((size_t)n_jobs);
+    long long begin = 0, pix = 0;
+    int max_depth = 0;
+    for (int i = 0; i < n_jobs; ++i) {
+        const mh_job& j = jobs[i];
+        if (j.scene < 0 || j.scene >= n_scenes) return fail(5, "job %d: scene %d out of range", i, j.scene);
+        if (j.width <= 0 || j.height <= 0) return fail(2, "job %d: non-positive image size %dx%d", i, j.width, j.height);
+        if ((long long)j.width * j.height > (1LL << 28)) return fail(2, "job %d: image too large", i);
+        if (j.spp < 1 || j.max_depth < 0 || j.sample_offset < 0)
+            return fail(3, "job %d: bad spp %d / depth %d / offset %d", i, j.spp, j.max_depth, j.sample_offset);
+        if (j.max_depth > 254) return fail(4, "job %d: depth %d above the batched engine's 254", i, j.max_depth);
+        BJob& d = bj[i];
+        d.scene = j.scene;
+        d.w = j.width;
+        d.h = j.height;
+        d.spp = j.spp;
+        d.s0 = j.sample_offset;
+        d.depth = j.max_depth;
+        d.npix = j.width * j.height;
+        d.begin = begin;
+        d.acc_off = pix;
+        d.cam = conv_cam<float>(j.cam);
+        begin += (long long)d.npix * j.spp;
+        pix += d.npix;
+        if (j.max_depth > max_depth) max_depth = j.max_depth;
+    }
+    if (pix * 3 > (1LL << 31)) return fail(2, "batch output too large (%lld pixels)", pix);
+
+    // tables, accumulators, framebuffer
+    size_t vb = (views.size() * sizeof(SceneView<float>) + 255) & ~(size_t)255;
+    size_t need = vb + bj.size() * sizeof(BJob);
+    if (need > b->tab_bytes) {
+        drop_graph(b);
+        if (b->tab) (void)hipFree(b->tab);
+        b->tab = nullptr;
+        b->tab_bytes = 0;
+        HIP_TRY(hipMalloc(&b->tab, need));
+        b->tab_bytes = need;
+    }
+    size_t old_acc = b->acc_pix;
+    int rc = grow(&b->acc64, &b->acc_pix, (size_t)pix * 3, sizeof(unsigned long long));
+    if (rc) return rc;
+    if (b->acc_pix != old_acc) drop_graph(b);
+    if ((size_t)pix > b->fb_pix) {
+        if (b->fb_linear) (void)hipFree(b->fb_linear);
+        if (b->fb_rgb) (void)hipFree(b->fb_rgb);
+        b->fb_linear = nullptr;
+        b->fb_rgb = nullptr;
+        b->fb_pix = 0;
+        HIP_TRY(hipMalloc(&b->fb_linear, (size_t)pix * 3 * sizeof(float)));
+        HIP_TRY(hipMalloc(&b->fb_rgb, (size_t)pix * 3));
+        b->fb_pix = (size_t)pix;
+    }
+    HIP_TRY(hipMemcpyAsync(b->tab, views.data(), views.size() * sizeof(SceneView<float>), hipMemcpyHostToDevice,
+                           b->stream));
+    HIP_TRY(hipMemcpyAsync((char*)b->tab + vb, bj.data(), bj.size() * sizeof(BJob), hipMemcpyHostToDevice, b->stream));
+    WfMulti mj{};
+    mj.scenes = (const SceneView<float>*)b->tab;
+    mj.jobs = (const BJob*)((char*)b->tab + vb);
+    mj.n_jobs = n_jobs;
+    mj.total = begin;
+    mj.total_pix = pix;
+    mj.acc64 = b->acc64;
+
+    if (t.pool == 0) t.pool = 1 << 21;
+    if (t.grid_blocks == 0) t.grid_blocks = 2048;
+    if (t.block == 0) t.block = 256;
+    if (t.autotune) {
+        std::vector<long long> key = {n_jobs, begin, pix, max_depth};
+        for (int i = 0; i < n_scenes; ++i) key.push_back((long long)(uintptr_t)scenes[i]);
+        bool hit = false;
+        for (auto& kv : b->tuned)
+            if (kv.first == key) {
+                t.pool = kv.second.pool;
+                t.grid_blocks = kv.second.grid_blocks;
+                t.block = kv.second.block;
+                hit = true;
+            }
+        if (!hit) {
+            rc = autotune(b, mj, max_depth, trans, &t);
+            if (rc) return rc;
+            b->tuned.push_back({key, t});
+        }
+    }
+    t.bvh_width = 2;
+    rc = ensure_batch_state(b, t.pool);
+    if (rc) return rc;
+
+    LaunchOut lo{b->fb_linear, b->fb_rgb, b->d_rays, nullptr};
+    HIP_TRY(hipMemsetAsync(b->d_rays, 0, sizeof(unsigned long long), b->stream));
+    HIP_TRY(wf_multi_begin(b->ws, mj, b->stream));
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    rc = batch_run(b, mj, max_depth, trans, t, -1);
+    if (rc) return rc;
+    HIP_TRY(wf_multi_end(mj, lo, b->stream));
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    size_t n3 = (size_t)pix * 3;
+    if (rgb_out) HIP_TRY(hipMemcpyAsync(rgb_out, b->fb_rgb, n3, hipMemcpyDeviceToHost, b->stream));
+    if (linear_out) HIP_TRY(hipMemcpyAsync(linear_out, b->fb_linear, n3 * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    unsigned long long rays = 0;
+    HIP_TRY(hipMemcpyAsync(&rays, b->d_rays, sizeof rays, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipEventRecord(b->ev[2], b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    memset(&g_timing, 0, sizeof g_timing);
+    g_timing.kernel_ms = ms_between(b->ev[0], b->ev[1]);
+    g_timing.total_ms = ms_between(b->ev[0], b->ev[2]);
+    g_timing.samples = begin;
+    g_timing.rays = (int64_t)rays;
+    g_seconds = g_timing.kernel_ms * 1e-3;
+    memset(&g_ktimes, 0, sizeof g_ktimes);
+    if (t.kernel_clock) {
+        HIP_TRY(b->clock.drain());
+        for (int k = 0; k < 8; ++k) {
+            g_ktimes.ms[k] = b->clock.ms[k];
+            g_ktimes.launches[k] = b->clock.launches[k];
+        }
+    }
+    b->last = t;
+    return 0;
+}
+
+int mh_batch_last_tune(const mh_batch* b, mh_tune* out)
+{
+    if (!b || !out) return fail(1, "NULL argument");
+    *out = b->last;
+    return 0;
+}
+
 int mh_last_timing(mh_timing* out)
 {
     if (!out) return fail(1, "NULL argument");
     *out = g_timing;
+    return 0;
+}
+
+int mh_last_kernel_times(mh_kernel_times* out)
+{
+    if (!out) return fail(1, "NULL argument");
+    *out = g_ktimes;
     return 0;
 }
 

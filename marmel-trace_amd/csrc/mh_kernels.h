@@ -40,7 +40,28 @@ struct WfState {
     float4* sh1;      // direction.xyz, (slot << 4 | estimate) bits
     uint32_t* cnt;    // [0..1] ext queue lengths, [2] shadow, [3] term
     float* acc;       // per-pixel radiance sums (w*h*3)
-    uint32_t* batch;  // [0] first global sample index of the current batch
+    unsigned long long* batch;  // [0] first global path index of the current batch
+    uint32_t* jid;    // batched engine: job of each path slot
+};
+
+// One render job of the batched engine: a camera view of one scene.
+struct BJob {
+    int scene;              // index into the scene table
+    int w, h, spp, s0, depth;
+    int npix;
+    long long begin;        // first global path index (paths ordered job, sample, pixel)
+    long long acc_off;      // first pixel of this job in the accumulators / output
+    CamView<float> cam;
+};
+
+// Batched engine tables (device pointers). Single-scene launches pass zeros.
+struct WfMulti {
+    const SceneView<float>* scenes;
+    const BJob* jobs;
+    int n_jobs;
+    long long total;             // paths over all jobs
+    long long total_pix;         // pixels over all jobs
+    unsigned long long* acc64;   // fixed-point (2^-24) per-pixel sums, 3 per pixel
 };
 
 struct WfJob {
@@ -53,11 +74,29 @@ struct WfJob {
 // Bytes of device state per path slot (state + pending + queues + shadow rays).
 size_t wf_bytes_per_path();
 
+// Optional per-launch timing: begin/end are called around each kernel launch
+// on the launch stream (direct launches only, never during graph capture).
+struct KernelClock {
+    virtual ~KernelClock() {}
+    virtual hipError_t begin(int kind, hipStream_t st) = 0;
+    virtual hipError_t end(int kind, hipStream_t st) = 0;
+};
+
 // Record the kernels of one batch (gen, depth+1 bounces, advance) on `st`.
 hipError_t wf_enqueue_batch(const SceneView<float>& sc, const CamView<float>& cam, const WfJob& job,
-                            const WfState& ws, unsigned long long* rays, int grid, hipStream_t st);
+                            const WfState& ws, unsigned long long* rays, int grid, hipStream_t st,
+                            KernelClock* kc = nullptr);
 hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st);
 hipError_t wf_enqueue_end(const WfState& ws, int npix, int spp, const LaunchOut& out, hipStream_t st);
+
+// Batched engine: one batch of `pool` paths drawn from all jobs, bounce
+// sequence to the deepest job's depth. `trans`: some scene of the batch has a
+// transmissive material (shadow rays walk through it).
+hipError_t wf_multi_begin(const WfState& ws, const WfMulti& mj, hipStream_t st);
+hipError_t wf_multi_batch(const WfMulti& mj, const WfState& ws, int pool, int max_depth,
+                          unsigned long long* rays, int grid, int block, bool trans, hipStream_t st,
+                          KernelClock* kc = nullptr);
+hipError_t wf_multi_end(const WfMulti& mj, const LaunchOut& out, hipStream_t st);
 
 // Whitted: persistent grid of `threads` lanes, each with `cap` stack items
 // of `item_bytes` at stack + lane * cap * item_bytes.

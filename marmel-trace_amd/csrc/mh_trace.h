@@ -538,14 +538,19 @@ template <int K> struct LdsStack {
 };
 
 // Per-kernel stack choice (0 = PrivStack, K = LdsStack<K>), measured on the
-// 1080p terrain workload: LDS pays off for the closest-hit extend kernel,
-// whose short-lived lanes keep the top of the stack hot; the shadow and
-// megakernel traversals run faster from private memory at their occupancy.
+// 1080p terrain workload: LDS pays off for the closest-hit extend kernel and
+// for the any-hit-only shadow kernel (occupancy 8: a private stack there is
+// ~130 MB of scratch, beyond L2 and MALL, written back to HBM); the
+// transmissive shadow walk and the megakernel, at occupancy 5-6, run faster
+// from private memory.
 #ifndef MH_EXTEND_LDS
 #define MH_EXTEND_LDS 8
 #endif
 #ifndef MH_SHADOW_LDS
 #define MH_SHADOW_LDS 0
+#endif
+#ifndef MH_SHADOW_ANY_LDS
+#define MH_SHADOW_ANY_LDS 16  // any-hit-only shadow kernel (no transmissive material)
 #endif
 #ifndef MH_MEGA_LDS
 #define MH_MEGA_LDS 0
@@ -557,6 +562,10 @@ template <int K> struct LdsStack {
     __shared__ int ts##_lds_ref[(K) * (threads)];                                                       \
     __shared__ float ts##_lds_t[(K) * (threads)];                                                       \
     ::mh::LdsStack<K> ts{ts##_lds_ref + threadIdx.x, ts##_lds_t + threadIdx.x, (int)blockDim.x};
+// References only: for kernels whose traversals are all any-hit.
+#define MH_STACK_LDS_REFS(ts, K, threads)                                                                 \
+    __shared__ int ts##_lds_ref[(K) * (threads)];                                                       \
+    ::mh::LdsStack<K> ts{ts##_lds_ref + threadIdx.x, nullptr, (int)blockDim.x};
 
 // Both child slabs of one node. fp32: origin-scaled planes and a widened exit
 // (robust traversal); fp64: the reference's slab arithmetic in double.

@@ -22,6 +22,15 @@
 // lane's rank from mbcnt. Every kernel is a persistent grid that reads its
 // queue length from device memory, so the bounce sequence of a batch has
 // fixed launch parameters and is captured once into a hipGraph.
+//
+// Batched engine (MULTI = true): the same kernels over a table of jobs (a
+// camera view of one scene each, with its own size, spp and depth). Global
+// path indices run job by job, sample-major; a batch takes the next `pool`
+// of them whatever jobs they belong to, so small jobs fill the machine
+// together. Each slot records its job; the kernels look up the job's scene.
+// A batch may then hold several samples of one pixel, so pixel sums are
+// 64-bit fixed-point atomics (2^-24 units): integer addition is associative,
+// and the result does not depend on the order in which paths end.
 #pragma once
 
 #include "mh_kernels_impl.h"
@@ -55,21 +64,50 @@ __global__ __launch_bounds__(64) void k_wf_reset(uint32_t* cnt, int which)
     }
 }
 
-__global__ __launch_bounds__(256) void k_wf_gen(SceneView<float> sc, CamView<float> cam, WfJob job, WfState st)
+// Job of global path index g: the last job whose first index is <= g.
+__device__ __forceinline__ int find_job(const WfMulti& mj, long long g)
 {
-    uint32_t g0 = st.batch[0];
-    long long remain = job.total - (long long)g0;
+    int lo = 0, hi = mj.n_jobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (mj.jobs[mid].begin <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The scene a path slot renders.
+template <bool MULTI>
+__device__ __forceinline__ const SceneView<float>& slot_scene(const SceneView<float>& sc, const WfMulti& mj,
+                                                              const WfState& st, uint32_t slot)
+{
+    return MULTI ? mj.scenes[mj.jobs[st.jid[slot]].scene] : sc;
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_wf_gen(SceneView<float> sc0, CamView<float> cam0, WfJob job, WfState st,
+                                                WfMulti mj)
+{
+    unsigned long long g0 = st.batch[0];
+    long long remain = (MULTI ? mj.total : job.total) - (long long)g0;
     int n = (int)(remain < job.pool ? remain : job.pool);
     if (blockIdx.x == 0 && threadIdx.x == 0) st.cnt[0] = (uint32_t)(n > 0 ? n : 0);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t g = g0 + (uint32_t)i;
-        uint32_t pix = g % (uint32_t)job.npix;
-        uint32_t s = (uint32_t)job.s0 + g / (uint32_t)job.npix;
-        uint32_t x = pix % (uint32_t)job.w, y = pix / (uint32_t)job.w;
+        long long g = (long long)g0 + i;
+        int j = MULTI ? find_job(mj, g) : 0;
+        const BJob* bj = MULTI ? &mj.jobs[j] : nullptr;
+        const SceneView<float>& sc = MULTI ? mj.scenes[bj->scene] : sc0;
+        const CamView<float>& cam = MULTI ? bj->cam : cam0;
+        long long l = MULTI ? g - bj->begin : g;
+        int npix = MULTI ? bj->npix : job.npix;
+        int jw = MULTI ? bj->w : job.w, jh = MULTI ? bj->h : job.h;
+        uint32_t pix = (uint32_t)(l % npix);
+        uint32_t s = (uint32_t)(MULTI ? bj->s0 : job.s0) + (uint32_t)(l / npix);
+        uint32_t x = pix % (uint32_t)jw, y = pix / (uint32_t)jw;
         float jx = rand01<float>(x, y, s * 2u + 0u), jy = rand01<float>(x, y, s * 2u + 1u);
         float l1 = rand01<float>(x, y, s * 2u + 100u), l2 = rand01<float>(x, y, s * 2u + 101u);
-        float uu = ((float)x + jx) / (float)job.w;
-        float vv = 1.0f - ((float)y + jy) / (float)job.h;
+        float uu = ((float)x + jx) / (float)jw;
+        float vv = 1.0f - ((float)y + jy) / (float)jh;
         V3<float> o, d;
         camera_ray(cam, uu, vv, l1, l2, &o, &d);
         if (cam.dome > 1e-4f && sc.water_material >= 0) {
@@ -86,11 +124,13 @@ __global__ __launch_bounds__(256) void k_wf_gen(SceneView<float> sc, CamView<flo
         st.p2[i] = make_float4(1.f, 1.f, 1.f, u2f(pix));
         st.p3[i] = make_float4(0.f, 0.f, 0.f, u2f(s));
         st.npend[i] = 0;
+        if (MULTI) st.jid[i] = (uint32_t)j;
         st.q[0][i] = (uint32_t)i;
     }
 }
 
-__global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc, WfState st, int which)
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState st, int which, WfMulti mj)
 {
 #if MH_EXTEND_LDS
     MH_STACK_LDS(ts, MH_EXTEND_LDS, 256)
@@ -101,6 +141,7 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc, WfState 
     const uint32_t* q = st.q[which];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t slot = q[i];
+        const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
         float4 a = st.p0[slot], b = st.p1[slot];
         float t;
         int prim;
@@ -131,8 +172,9 @@ __device__ __forceinline__ V3<float> resolve_pending(const WfState& st, uint32_t
     return L;
 }
 
-__global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc, WfState st, WfJob job, int which, int bounce,
-                                                   unsigned long long* rays)
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState st, WfJob job, int which, int bounce,
+                                                   unsigned long long* rays, WfMulti mj)
 {
     uint32_t n = st.cnt[which];
     const uint32_t* qin = st.q[which];
@@ -145,6 +187,8 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc, WfState s
         bool alive = false, ended = false;
         V3<float> nray_o = zero3<float>(), nray_d = zero3<float>();
         if (active) {
+            const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
+            int depth = MULTI ? mj.jobs[st.jid[slot]].depth : job.depth;
             float4 a = st.p0[slot], b = st.p1[slot], c = st.p2[slot], e = st.p3[slot];
             float2 hh = st.hit[slot];
             V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
@@ -268,7 +312,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc, WfState s
                                     if (!(q > 0.f) || rand01<float>(kb, (uint32_t)b_idx, 0x5201u) >= q) live = false;
                                     else beta = beta * (1.f / q);
                                 }
-                                if (live && b_idx < job.depth) {
+                                if (live && b_idx < depth) {
                                     V3<float> off = lb.refr ? lb.wi * 1e-4f
                                                             : N * (dot(lb.wi, N) >= 0.f ? 1e-4f : -1e-4f);
                                     nray_o = P + off;
@@ -299,12 +343,24 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc, WfState s
     count_rays(rays, nq);
 }
 
-__global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc, WfState st, unsigned long long* rays)
+// TRANS: some scene of the launch has a transmissive material, so a shadow
+// ray walks through water / near-clear surfaces (closest-hit steps with
+// Beer-Lambert attenuation). Otherwise every first hit is opaque and the
+// query is a single any-hit traversal (same answer), with a smaller stack and
+// register footprint.
+template <bool MULTI, bool TRANS>
+__global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState st, unsigned long long* rays,
+                                                    WfMulti mj)
 {
 #if MH_SHADOW_LDS
-    MH_STACK_LDS(ts, MH_SHADOW_LDS, 256)
+    MH_STACK_LDS(tsw, MH_SHADOW_LDS, 256)
 #else
-    MH_STACK_PRIV(ts)
+    MH_STACK_PRIV(tsw)
+#endif
+#if MH_SHADOW_ANY_LDS
+    MH_STACK_LDS_REFS(tsa, MH_SHADOW_ANY_LDS, 256)
+#else
+    MH_STACK_PRIV(tsa)
 #endif
     uint32_t n = st.cnt[2];
     int nq = 0;
@@ -312,22 +368,35 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc, WfState 
         float4 a = st.sh0[i], b = st.sh1[i];
         uint32_t tag = f2u(b.w);
         uint32_t slot = tag >> 4, k = tag & 15u;
+        const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
         V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
-        V3<float> att;
-        bool occ = occluded_segment(sc, ts, o, d, 1e-3f, a.w, &att, &nq);
-        if (!occ && k == 8 && att.x < 1e-4f && att.y < 1e-4f && att.z < 1e-4f) occ = true;
-        if (!occ) {
-            if (sc.has_transmissive) {
+        bool occ;
+        if (TRANS) {
+            V3<float> att;
+            occ = occluded_segment(sc, tsw, o, d, 1e-3f, a.w, &att, &nq);
+            if (!occ && k == 8 && att.x < 1e-4f && att.y < 1e-4f && att.z < 1e-4f) occ = true;
+            if (!occ && sc.has_transmissive) {
                 float4 c = st.pend[(size_t)slot * 9 + k];
                 st.pend[(size_t)slot * 9 + k] = make_float4(c.x * att.x, c.y * att.y, c.z * att.z, 0.f);
             }
-            atomicOr(&st.vis[slot], 1u << k);
+        } else {
+            ++nq;
+            occ = any_hit(sc, tsa, o, unit_dir(d), 1e-3f, a.w);
         }
+        if (!occ) atomicOr(&st.vis[slot], 1u << k);
     }
     count_rays(rays, nq);
 }
 
-__global__ __launch_bounds__(256) void k_wf_finalize(WfState st)
+// Radiance in 2^-24 fixed point. A sample is capped at 2^30 (no physical
+// estimate here comes near it), so 2^10 samples cannot overflow a pixel sum.
+__device__ __forceinline__ unsigned long long to_fixed(float v)
+{
+    return __float2ull_rn(fminf(v, 1073741824.0f) * 16777216.0f);
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_wf_finalize(WfState st, WfMulti mj)
 {
     uint32_t n = st.cnt[3];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -340,16 +409,23 @@ __global__ __launch_bounds__(256) void k_wf_finalize(WfState st)
         L.y = L.y < 0.f ? 0.f : L.y;
         L.z = L.z < 0.f ? 0.f : L.z;
         uint32_t pix = f2u(c.w);
-        // one sample per pixel per batch: no other thread touches this pixel
-        st.acc[(size_t)pix * 3] += L.x;
-        st.acc[(size_t)pix * 3 + 1] += L.y;
-        st.acc[(size_t)pix * 3 + 2] += L.z;
+        if (MULTI) {
+            size_t p = ((size_t)mj.jobs[st.jid[slot]].acc_off + pix) * 3;
+            atomicAdd(&mj.acc64[p], to_fixed(L.x));
+            atomicAdd(&mj.acc64[p + 1], to_fixed(L.y));
+            atomicAdd(&mj.acc64[p + 2], to_fixed(L.z));
+        } else {
+            // one sample per pixel per batch: no other thread touches this pixel
+            st.acc[(size_t)pix * 3] += L.x;
+            st.acc[(size_t)pix * 3 + 1] += L.y;
+            st.acc[(size_t)pix * 3 + 2] += L.z;
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_wf_next_batch(WfState st, int pool)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) st.batch[0] += (uint32_t)pool;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.batch[0] += (unsigned long long)pool;
 }
 
 __global__ __launch_bounds__(256) void k_wf_clear(WfState st, int npix)
@@ -369,6 +445,36 @@ __global__ __launch_bounds__(256) void k_wf_resolve(WfState st, int npix, int sp
         out.rgb[(size_t)i * 3] = to_byte(r);
         out.rgb[(size_t)i * 3 + 1] = to_byte(g);
         out.rgb[(size_t)i * 3 + 2] = to_byte(b);
+    }
+}
+
+// Batched engine: zero the fixed-point sums and the batch cursor.
+__global__ __launch_bounds__(256) void k_wf_multi_clear(WfState st, WfMulti mj)
+{
+    size_t n = (size_t)mj.total_pix * 3;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        mj.acc64[i] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.batch[0] = 0;
+}
+
+// Batched engine: per-pixel means and tone map of every job, written to the
+// concatenated output (job order, each job top-down RGB).
+__global__ __launch_bounds__(256) void k_wf_multi_resolve(WfMulti mj, LaunchOut out)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < mj.total_pix;
+         i += (long long)gridDim.x * blockDim.x) {
+        int lo = 0, hi = mj.n_jobs - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (mj.jobs[mid].acc_off <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        double k = 1.0 / (16777216.0 * (double)mj.jobs[lo].spp);
+        for (int c = 0; c < 3; ++c) {
+            float v = (float)((double)mj.acc64[i * 3 + c] * k);
+            out.linear[i * 3 + c] = v;
+            out.rgb[i * 3 + c] = to_byte(v);
+        }
     }
 }
 

@@ -18,19 +18,76 @@ hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t wf_enqueue_batch(const SceneView<float>& sc, const CamView<float>& cam, const WfJob& job,
-                            const WfState& ws, unsigned long long* rays, int grid, hipStream_t st)
+// Launch wrapped in the optional clock.
+#define MH_TIMED(kind, launch)                                                                            \
+    do {                                                                                                  \
+        if (kc) {                                                                                         \
+            hipError_t e_ = kc->begin(kind, st);                                                          \
+            if (e_ != hipSuccess) return e_;                                                              \
+        }                                                                                                 \
+        launch;                                                                                           \
+        if (kc) {                                                                                         \
+            hipError_t e_ = kc->end(kind, st);                                                            \
+            if (e_ != hipSuccess) return e_;                                                              \
+        }                                                                                                 \
+    } while (0)
+
+// The bounce sequence of one batch, shared by both engines.
+template <bool MULTI>
+static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<float>& cam, const WfJob& job,
+                                  const WfState& ws, const WfMulti& mj, unsigned long long* rays, int grid, int block,
+                                  bool trans, hipStream_t st, KernelClock* kc)
 {
-    hipLaunchKernelGGL(k_wf_gen, dim3(grid), dim3(256), 0, st, sc, cam, job, ws);
+    auto shadow = trans ? k_wf_shadow<MULTI, true> : k_wf_shadow<MULTI, false>;
+    dim3 G(grid), B(block);
+    MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_gen<MULTI>, G, B, 0, st, sc, cam, job, ws, mj));
     for (int b = 0; b <= job.depth; ++b) {
         int which = b & 1;
-        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, ws.cnt, which ^ 1);
-        hipLaunchKernelGGL(k_wf_extend, dim3(grid), dim3(256), 0, st, sc, ws, which);
-        hipLaunchKernelGGL(k_wf_shade, dim3(grid), dim3(256), 0, st, sc, ws, job, which, b, rays);
-        hipLaunchKernelGGL(k_wf_shadow, dim3(grid), dim3(256), 0, st, sc, ws, rays);
-        hipLaunchKernelGGL(k_wf_finalize, dim3(grid), dim3(256), 0, st, ws);
+        MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, ws.cnt, which ^ 1));
+        MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(k_wf_extend<MULTI>, G, B, 0, st, sc, ws, which, mj));
+        MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(k_wf_shade<MULTI>, G, B, 0, st, sc, ws, job, which, b, rays, mj));
+        MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
+        MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MULTI>, G, B, 0, st, ws, mj));
     }
-    hipLaunchKernelGGL(k_wf_next_batch, dim3(1), dim3(64), 0, st, ws, job.pool);
+    MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_next_batch, dim3(1), dim3(64), 0, st, ws, job.pool));
+    return hipGetLastError();
+}
+
+hipError_t wf_enqueue_batch(const SceneView<float>& sc, const CamView<float>& cam, const WfJob& job,
+                            const WfState& ws, unsigned long long* rays, int grid, hipStream_t st, KernelClock* kc)
+{
+    WfMulti none{};
+    return enqueue_bounces<false>(sc, cam, job, ws, none, rays, grid, 256, sc.has_transmissive != 0, st, kc);
+}
+
+hipError_t wf_multi_begin(const WfState& ws, const WfMulti& mj, hipStream_t st)
+{
+    long long n = mj.total_pix * 3;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_wf_multi_clear, dim3(blocks), dim3(256), 0, st, ws, mj);
+    return hipGetLastError();
+}
+
+hipError_t wf_multi_batch(const WfMulti& mj, const WfState& ws, int pool, int max_depth, unsigned long long* rays,
+                          int grid, int block, bool trans, hipStream_t st, KernelClock* kc)
+{
+    SceneView<float> sc{};
+    CamView<float> cam{};
+    WfJob job{};
+    job.depth = max_depth;
+    job.pool = pool;
+    return enqueue_bounces<true>(sc, cam, job, ws, mj, rays, grid, block, trans, st, kc);
+}
+
+hipError_t wf_multi_end(const WfMulti& mj, const LaunchOut& out, hipStream_t st)
+{
+    long long n = mj.total_pix;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_wf_multi_resolve, dim3(blocks), dim3(256), 0, st, mj, out);
     return hipGetLastError();
 }
 

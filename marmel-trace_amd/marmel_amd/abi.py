@@ -101,7 +101,25 @@ ENGINE_MEGAKERNEL, ENGINE_WAVEFRONT = 0, 1
 class RenderParams(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("width", "height", "spp", "max_depth", "mode",
                                           "precision", "sample_offset", "engine", "pool",
-                                          "grid_blocks", "use_graph", "reserved")]
+                                          "grid_blocks", "use_graph", "kernel_clock")]
+
+
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("max_depth", C.c_int32), ("sample_offset", C.c_int32), ("cam", Camera)]
+
+
+class Tune(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("pool", "grid_blocks", "block", "use_graph", "autotune",
+                                          "kernel_clock", "bvh_width", "reserved")]
+
+
+K_GEN, K_EXTEND, K_SHADE, K_SHADOW, K_FINALIZE, K_OTHER = range(6)
+K_NAMES = ("gen", "extend", "shade", "shadow", "finalize", "other")
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_int64 * 8)]
 
 
 class Timing(C.Structure):

@@ -2,10 +2,11 @@
 
     python -m marmel_amd.calib [MiB]
 
-Runs one device-to-device copy of a known size through a plain elementwise
-copy kernel (torch's copy_), so that rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-readings of the render kernels can be scaled by what the same counters report
-for exactly N bytes read and N bytes written.
+Streams a buffer of known size through two plain kernels, a device copy
+(torch's copy_, the runtime's copyBuffer kernel) and an elementwise multiply
+(y = 2x), each reading exactly N bytes and writing exactly N bytes, so that
+rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE readings of the render kernels can be
+scaled by what the same counters report for known traffic.
 """
 import sys
 
@@ -20,7 +21,9 @@ def main():
     torch.cuda.synchronize()
     y.copy_(x)
     torch.cuda.synchronize()
-    print(f"calib: copied {n * 4} bytes (read {n * 4}, write {n * 4})", flush=True)
+    torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    print(f"calib: copy and multiply, each read {n * 4} bytes and wrote {n * 4} bytes", flush=True)
 
 
 if __name__ == "__main__":

@@ -87,7 +87,13 @@ def hip():
                                          C.c_int, P])
     _sig(L, "mh_render", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P, P])
     _sig(L, "mh_framebuffer_download", C.c_int, [P, P, P])
+    _sig(L, "mh_batch_create", C.c_int, [C.POINTER(P)])
+    _sig(L, "mh_batch_destroy", None, [P])
+    _sig(L, "mh_render_batch", C.c_int, [P, P, C.c_int, C.POINTER(abi.Job), C.c_int,
+                                         C.POINTER(abi.Tune), P, P])
+    _sig(L, "mh_batch_last_tune", C.c_int, [P, C.POINTER(abi.Tune)])
     _sig(L, "mh_last_timing", C.c_int, [C.POINTER(abi.Timing)])
+    _sig(L, "mh_last_kernel_times", C.c_int, [C.POINTER(abi.KernelTimes)])
     _sig(L, "mh_last_seconds", C.c_double, [])
     _sig(L, "mh_last_error", C.c_char_p, [])
     _hip = L

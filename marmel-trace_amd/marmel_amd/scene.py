@@ -192,10 +192,12 @@ class GpuScene:
 
     def render(self, cam, width, height, spp, depth, mode=abi.MODE_PATHTRACE,
                precision=abi.PREC_FP32, sample_offset=0, fetch=True,
-               engine=abi.ENGINE_MEGAKERNEL, pool=0, grid_blocks=0, use_graph=False):
+               engine=abi.ENGINE_MEGAKERNEL, pool=0, grid_blocks=0, use_graph=False,
+               kernel_clock=False):
         p = abi.RenderParams(width=width, height=height, spp=spp, max_depth=depth, mode=mode,
                              precision=precision, sample_offset=sample_offset, engine=engine,
-                             pool=pool, grid_blocks=grid_blocks, use_graph=int(use_graph))
+                             pool=pool, grid_blocks=grid_blocks, use_graph=int(use_graph),
+                             kernel_clock=int(kernel_clock))
         rgb = lin = None
         if fetch:
             rgb = np.zeros((height, width, 3), np.uint8)
@@ -233,10 +235,75 @@ class GpuScene:
         return out
 
     @staticmethod
+    def last_kernel_times():
+        """{kernel: (total_ms, launches)} of the last render made with kernel_clock."""
+        k = abi.KernelTimes()
+        native.hip().mh_last_kernel_times(C.byref(k))
+        return {n: (k.ms[i], int(k.launches[i])) for i, n in enumerate(abi.K_NAMES)}
+
+    @staticmethod
     def last_timing():
         t = abi.Timing()
         native.hip().mh_last_timing(C.byref(t))
         return t
+
+
+class Batch:
+    """Batched renderer (mh_batch): many jobs over many GPU scenes in one
+    persistent wavefront launch sequence."""
+
+    def __init__(self):
+        self._L = native.hip()
+        h = C.c_void_p()
+        rc = self._L.mh_batch_create(C.byref(h))
+        if rc:
+            raise MarmelError(rc, self._L.mh_last_error().decode())
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._L.mh_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, scenes, jobs, pool=0, grid_blocks=0, block=0, use_graph=True, autotune=False,
+               kernel_clock=False, fetch=True):
+        """scenes: list of GpuScene; jobs: list of dicts (scene, cam, width, height, spp, depth,
+        sample_offset). Returns [(rgb, linear)] per job (None when fetch is False)."""
+        arr = (C.c_void_p * len(scenes))(*[s._h.value for s in scenes])
+        js = (abi.Job * len(jobs))()
+        npx = []
+        for i, j in enumerate(jobs):
+            js[i] = abi.Job(scene=j["scene"], width=j["width"], height=j["height"], spp=j["spp"],
+                            max_depth=j["depth"], sample_offset=j.get("sample_offset", 0), cam=j["cam"])
+            npx.append(max(j["width"], 0) * max(j["height"], 0))
+        t = abi.Tune(pool=pool, grid_blocks=grid_blocks, block=block, use_graph=int(use_graph),
+                     autotune=int(autotune), kernel_clock=int(kernel_clock))
+        tot = max(sum(npx), 1)
+        rgb = np.zeros(tot * 3, np.uint8) if fetch else None
+        lin = np.zeros(tot * 3, np.float32) if fetch else None
+        rc = self._L.mh_render_batch(self._h, arr, len(scenes), js, len(jobs), C.byref(t),
+                                     abi.ptr(rgb) if fetch else None, abi.ptr(lin) if fetch else None)
+        if rc:
+            raise MarmelError(rc, self._L.mh_last_error().decode())
+        if not fetch:
+            return None
+        out, off = [], 0
+        for j, n in zip(jobs, npx):
+            shape = (j["height"], j["width"], 3)
+            out.append((rgb[off * 3:(off + n) * 3].reshape(shape), lin[off * 3:(off + n) * 3].reshape(shape)))
+            off += n
+        return out
+
+    def last_tune(self):
+        t = abi.Tune()
+        self._L.mh_batch_last_tune(self._h, C.byref(t))
+        return {n: getattr(t, n) for n, _ in abi.Tune._fields_ if n != "reserved"}
 
 
 def device_info():

@@ -629,13 +629,12 @@ int or_intersect1(const or_scene *s, oray r, double tmin, double tmax, ohit *out
             sentry e = st[--sp];
             if (found && e.t > closest) continue;
             const onode *n = &s->nodes[e.idx];
-            or_tls_stats.nodes_visited++;
             if (n->count > 0) {
                 for (int i = 0; i < n->count; ++i) {
                     int pi = s->order[n->first + i];
                     ohit h;
-                    or_tls_stats.prims_tested++;
-                    if (s->leaf_prims[n->first + i].kind == MH_PRIM_TRIANGLE) or_tls_stats.tri_tests++;
+                    or_tls_stats.c_prims++;
+                    if (s->leaf_prims[n->first + i].kind == MH_PRIM_TRIANGLE) or_tls_stats.c_tris++;
                     if (or_prim_intersect(&s->leaf_prims[n->first + i], r, tmin, closest, &h)) {
                         h.prim = pi;
                         if (!found || h.t < best.t || (h.t == best.t && pi < best.prim)) {
@@ -647,6 +646,7 @@ int or_intersect1(const or_scene *s, oray r, double tmin, double tmax, ohit *out
                 }
             } else {
                 double tl, tr;
+                or_tls_stats.c_inner++;
                 int hl = slab_child(&s->nodes[n->left], r.o, inv, tmin, closest, &tl);
                 int hr = slab_child(&s->nodes[n->right], r.o, inv, tmin, closest, &tr);
                 if (hl || hr) {
@@ -699,15 +699,15 @@ int or_occluded1(const or_scene *s, oray r, double tmin, double tmax)
     if (slab(s->nodes[0].bmin, s->nodes[0].bmax, r.o, inv, tmin, tmax, &t0)) st[sp++] = 0;
     while (sp > 0 && !hit) {
         const onode *n = &s->nodes[st[--sp]];
-        or_tls_stats.nodes_visited++;
         if (n->count > 0) {
             for (int i = 0; i < n->count; ++i) {
-                or_tls_stats.prims_tested++;
-                if (s->leaf_prims[n->first + i].kind == MH_PRIM_TRIANGLE) or_tls_stats.tri_tests++;
+                or_tls_stats.a_prims++;
+                if (s->leaf_prims[n->first + i].kind == MH_PRIM_TRIANGLE) or_tls_stats.a_tris++;
                 if (or_prim_occluded(&s->leaf_prims[n->first + i], r, tmin, tmax)) { hit = 1; break; }
             }
         } else {
             double tl, tr;
+            or_tls_stats.a_inner++;
             int hl = slab_child(&s->nodes[n->left], r.o, inv, tmin, tmax, &tl);
             int hr = slab_child(&s->nodes[n->right], r.o, inv, tmin, tmax, &tr);
             if (hl || hr) {

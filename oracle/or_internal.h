@@ -94,6 +94,7 @@ typedef struct or_scene {
     mh_sky sky;
     double water_level;
     int water_material;
+    int has_transmissive;   /* some material is water or transparency > 0.8 */
     int n_lights;
     olight lights[8];
     /* binary SAH BVH restated from the reference build */
@@ -106,14 +107,22 @@ typedef struct or_scene {
     int n_planes;
 } or_scene;
 
-/* traversal statistics (thread-local), used to price algorithmic bytes */
+/* traversal statistics (thread-local), used to price algorithmic bytes:
+ * per query kind, interior nodes whose two child boxes were tested and
+ * primitives tested (all / triangles), plus plane tests and path bounces */
 typedef struct {
     long long closest_queries, any_queries;
-    long long nodes_visited, prims_tested, plane_tests;
-    long long tri_tests, ext_queries;
+    long long c_inner, c_prims, c_tris;
+    long long a_inner, a_prims, a_tris;
+    long long plane_tests, ext_queries;
 } or_stats;
-#define OR_NSTATS 7
+#define OR_NSTATS 10
 extern __thread or_stats or_tls_stats;
+/* 1: the path tracer's shadow segment runs as one any-hit query when the
+ * scene has no transmissive material (same answer: the first hit would be
+ * opaque), so the counts price the GPU's traversal; 0: the reference's
+ * closest-hit walk. */
+extern int or_anyhit_shadow;
 
 /* or_math.c */
 unsigned or_hash_u32(unsigned x);

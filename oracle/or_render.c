@@ -269,9 +269,12 @@ static int sample_legacy(const mh_material *m, dv3 alb, dv3 N, dv3 d, dv3 V, int
 
 /* Shadow connection through up to 6 closest hits; water and near-clear
  * (transparency > 0.8) surfaces pass light with Beer-Lambert attenuation. */
+int or_anyhit_shadow = 0;
+
 static int occluded_segment(const or_scene *s, oray r, double tmin, double tmax, dv3 *att)
 {
     *att = d3(1.0, 1.0, 1.0);
+    if (or_anyhit_shadow && !s->has_transmissive) return or_occluded1(s, r, tmin, tmax);
     oray cur = r;
     double rem = tmax;
     for (int step = 0; step < 6; ++step) {
@@ -751,19 +754,24 @@ int or_render(const or_scene *s, const mh_camera *cam, int width, int height, in
     return 0;
 }
 
-void or_render_stats_get(long long out[7]) { memcpy(out, g_render_stats, sizeof g_render_stats); }
+void or_render_stats_get(long long out[OR_NSTATS]) { memcpy(out, g_render_stats, sizeof g_render_stats); }
+
+void or_set_anyhit_shadow(int on) { or_anyhit_shadow = on ? 1 : 0; }
 
 void or_stats_reset(void) { memset(&or_tls_stats, 0, sizeof or_tls_stats); }
 
-void or_stats_get(long long out[7])
+void or_stats_get(long long out[OR_NSTATS])
 {
     out[0] = or_tls_stats.closest_queries;
     out[1] = or_tls_stats.any_queries;
-    out[2] = or_tls_stats.nodes_visited;
-    out[3] = or_tls_stats.prims_tested;
-    out[4] = or_tls_stats.plane_tests;
-    out[5] = or_tls_stats.tri_tests;
-    out[6] = or_tls_stats.ext_queries;
+    out[2] = or_tls_stats.c_inner;
+    out[3] = or_tls_stats.c_prims;
+    out[4] = or_tls_stats.c_tris;
+    out[5] = or_tls_stats.a_inner;
+    out[6] = or_tls_stats.a_prims;
+    out[7] = or_tls_stats.a_tris;
+    out[8] = or_tls_stats.plane_tests;
+    out[9] = or_tls_stats.ext_queries;
 }
 
 /* ---- scene ------------------------------------------------------------ */
@@ -785,6 +793,8 @@ or_scene *or_scene_create(const mh_scene_soa *soa)
     s->sky = soa->sky;
     s->water_level = soa->water_level;
     s->water_material = soa->water_material;
+    for (int i = 0; i < s->n_mats; ++i)
+        if (s->mats[i].is_water || s->mats[i].transparency > 0.8) s->has_transmissive = 1;
     for (int i = 0; i < soa->n_prims; ++i) {
         const mh_prim *q = &soa->prims[i];
         oprim *p = &s->prims[i];

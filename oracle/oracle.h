@@ -54,9 +54,16 @@ int or_render(const or_scene *s, const mh_camera *cam, int width, int height,
 /* Traversal counts accumulated by the calling thread's queries since the
  * last reset (used to price algorithmic bytes per sample). */
 void or_stats_reset(void);
-void or_stats_get(long long out[7]); /* closest, any, nodes, prims, planes, triangle tests, extension queries */
+/* out[10]: closest queries, any-hit queries, then for closest and for any-hit
+ * queries in turn: interior nodes tested, primitives tested, triangles
+ * tested; then plane tests and path bounces (extension queries). */
+void or_stats_get(long long out[10]);
 /* Counts of a full render (all worker threads), filled by or_render. */
-void or_render_stats_get(long long out[7]);
+void or_render_stats_get(long long out[10]);
+/* 1: price shadow segments as the GPU runs them (one any-hit query when no
+ * material is transmissive; identical result). 0 (default): the reference's
+ * closest-hit walk. */
+void or_set_anyhit_shadow(int on);
 
 /* Pure-function vectors. */
 unsigned or_hash3(unsigned a, unsigned b, unsigned c);

@@ -53,6 +53,7 @@ def oracle():
         _sig(L, "or_stats_reset", None, [])
         _sig(L, "or_stats_get", None, [C.POINTER(C.c_longlong)])
         _sig(L, "or_render_stats_get", None, [C.POINTER(C.c_longlong)])
+        _sig(L, "or_set_anyhit_shadow", None, [C.c_int])
         _sig(L, "or_hash3", C.c_uint, [C.c_uint, C.c_uint, C.c_uint])
         _sig(L, "or_rand01", D, [C.c_uint, C.c_uint, C.c_uint])
         _sig(L, "or_camera_create", None, [DP, DP, DP, D, D, C.POINTER(abi.Camera)])
@@ -164,10 +165,17 @@ class OracleScene:
         self.L.or_occluded(self.h, abi.ptr(rays), len(rays), tmin, tmax, abi.ptr(out))
         return out
 
+    STAT_NAMES = ("closest", "any", "c_inner", "c_prims", "c_tris", "a_inner", "a_prims", "a_tris",
+                  "planes", "ext")
+
     def stats(self):
-        a = (C.c_longlong * 7)()
+        """Traversal counts of the last render, by STAT_NAMES."""
+        a = (C.c_longlong * 10)()
         self.L.or_render_stats_get(a)
-        return list(a)
+        return dict(zip(self.STAT_NAMES, list(a)))
+
+    def set_anyhit_shadow(self, on):
+        self.L.or_set_anyhit_shadow(1 if on else 0)
 
 
 class RefScene:

@@ -1,0 +1,143 @@
+"""GPU tests of the batched engine (mh_render_batch): many jobs over several
+scenes in one persistent wavefront launch sequence.
+
+  * every job's image is the same estimator as a single render of that job
+    (same sample streams): equal to the megakernel image up to fp32 rounding,
+  * the fixed-point pixel sums make the result independent of the launch
+    settings (pool, grid, block, graph, autotune): bit for bit,
+  * a job's per-pixel means are consistent with a converged FP64 oracle
+    render (3 sigma), and bad jobs return the reference's error codes.
+"""
+import numpy as np
+import pytest
+
+import checkers as ck
+from marmel_amd import Batch, GpuScene, HostScene, MarmelError, abi, native
+
+pytestmark = pytest.mark.gpu
+
+HOST = {"spheres": (native.GEN_SPHERES, {}), "terrain": (native.GEN_TERRAIN, dict(grid=48)),
+        "icosphere": (native.GEN_ICOSPHERE, dict(subdiv=2))}
+
+
+@pytest.fixture(scope="module")
+def world():
+    out = {}
+    for name, (kind, kw) in HOST.items():
+        hs = HostScene()
+        hs.generate(kind, seed=1, **kw)
+        hs.build_bvh()
+        out[name] = (hs, GpuScene(hs.soa()))
+    yield out
+    for hs, gs in out.values():
+        gs.close()
+
+
+def jobs_of(world, names):
+    """Uneven jobs: sizes, spp, depths and sample offsets differ."""
+    spec = [("spheres", 64, 48, 8, 6, 0), ("terrain", 80, 45, 4, 8, 0), ("icosphere", 40, 40, 16, 5, 0),
+            ("spheres", 33, 17, 12, 3, 100), ("terrain", 24, 56, 2, 10, 7)]
+    jobs = []
+    for sc, w, h, spp, depth, off in spec:
+        hs, _ = world[sc]
+        jobs.append(dict(scene=names.index(sc), cam=hs.camera(w / h), width=w, height=h, spp=spp,
+                         depth=depth, sample_offset=off))
+    return jobs
+
+
+NAMES = ["spheres", "terrain", "icosphere"]
+
+
+def within(a, b, tol=1):
+    return (np.abs(a.astype(int) - b.astype(int)) <= tol).all(axis=-1).mean()
+
+
+def test_batch_jobs_equal_single_renders(world):
+    b = Batch()
+    scenes = [world[n][1] for n in NAMES]
+    jobs = jobs_of(world, NAMES)
+    res = b.render(scenes, jobs, pool=4096, grid_blocks=64)
+    for j, (rgb, lin) in zip(jobs, res):
+        gs = scenes[j["scene"]]
+        rgb_m, lin_m = gs.render(j["cam"], j["width"], j["height"], j["spp"], j["depth"],
+                                 sample_offset=j["sample_offset"])
+        frac = within(rgb, rgb_m)
+        d = np.abs(lin - lin_m).mean()
+        print(NAMES[j["scene"]], j["width"], j["height"], "within 1/255:", frac, "mean |d|", d)
+        assert frac >= 0.995
+        assert d < 2e-3
+    b.close()
+
+
+def test_batch_is_independent_of_launch_settings(world):
+    b = Batch()
+    scenes = [world[n][1] for n in NAMES]
+    jobs = jobs_of(world, NAMES)
+    ref = b.render(scenes, jobs, pool=1 << 21, grid_blocks=2048, block=256, use_graph=True)
+    for kw in (dict(pool=1000, grid_blocks=64, block=128, use_graph=False),
+               dict(pool=777, grid_blocks=300, block=64, use_graph=True),
+               dict(pool=50000, grid_blocks=512, block=256, use_graph=True, kernel_clock=True)):
+        res = b.render(scenes, jobs, **kw)
+        for (r0, l0), (r1, l1) in zip(ref, res):
+            assert np.array_equal(l0, l1), kw
+            assert np.array_equal(r0, r1), kw
+    # graph replay again after other settings: same bits
+    res = b.render(scenes, jobs, pool=1 << 21, grid_blocks=2048, block=256, use_graph=True)
+    for (r0, l0), (r1, l1) in zip(ref, res):
+        assert np.array_equal(l0, l1)
+    b.close()
+
+
+def test_batch_autotune_keeps_the_image(world):
+    b = Batch()
+    scenes = [world[n][1] for n in NAMES]
+    jobs = jobs_of(world, NAMES)
+    ref = b.render(scenes, jobs)
+    res = b.render(scenes, jobs, autotune=True)
+    t = b.last_tune()
+    print("autotuned", t)
+    assert t["pool"] in (1 << 19, 1 << 20, 1 << 21, 1 << 22)
+    assert t["grid_blocks"] in (1024, 2048, 4096) and t["block"] in (128, 256)
+    assert t["bvh_width"] == 2
+    for (r0, l0), (r1, l1) in zip(ref, res):
+        assert np.array_equal(l0, l1)
+    res2 = b.render(scenes, jobs, autotune=True)  # cached choice
+    assert b.last_tune() == t
+    for (r0, l0), (r1, l1) in zip(ref, res2):
+        assert np.array_equal(l0, l1)
+    b.close()
+
+
+def test_batch_job_statistics_vs_oracle(world):
+    hs, gs = world["terrain"]
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    o = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm).soa)
+    w, h = 48, 36
+    cam = hs.camera(w / h)
+    _, ref_mean, ref_var, _ = o.render(cam, w, h, 512, 8, want_var=True, sample_offset=9000)
+    hs2, gs2 = world["spheres"]
+    b = Batch()
+    jobs = [dict(scene=1, cam=hs2.camera(1.0), width=16, height=16, spp=4, depth=4),
+            dict(scene=0, cam=cam, width=w, height=h, spp=64, depth=8)]
+    (_, _), (_, lin) = b.render([gs, gs2], jobs)
+    se = np.sqrt(ref_var / 64 + ref_var / 512) + 1e-6
+    z = (lin - ref_mean) / se
+    print("batched terrain within 3 sigma", (np.abs(z) <= 3).mean(), "mean z", z.mean())
+    assert (np.abs(z) <= 3).mean() >= 0.97
+    assert abs(z.mean()) < 0.1
+    assert (z * z).mean() < 1.6
+    b.close()
+
+
+@pytest.mark.parametrize("patch,code", [(dict(scene=5), 5), (dict(width=0), 2), (dict(height=-3), 2),
+                                        (dict(spp=0), 3), (dict(depth=-1), 3), (dict(depth=300), 4),
+                                        (dict(sample_offset=-1), 3)])
+def test_batch_error_codes(world, patch, code):
+    hs, gs = world["spheres"]
+    job = dict(scene=0, cam=hs.camera(1.0), width=8, height=8, spp=1, depth=2)
+    job.update(patch)
+    b = Batch()
+    with pytest.raises(MarmelError) as e:
+        b.render([gs], [job])
+    assert e.value.code == code
+    b.close()

@@ -39,6 +39,10 @@ def test_gpu_entry_points_reject_null_arguments():
     assert L.mh_render(None, None, None, None, None) == 1
     assert L.mh_pathtrace_render(None, None, 10, 10, 1, 1, None) == 1
     assert L.mh_render_image(None, None, 10, 10, 1, 1, None) == 1
+    assert L.mh_batch_create(None) == 1
+    assert L.mh_render_batch(None, None, 0, None, 0, None, None, None) == 1
+    assert L.mh_batch_last_tune(None, None) == 1
+    assert L.mh_last_kernel_times(None) == 1
 
 
 def bvh_arrays(hs):

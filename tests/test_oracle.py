@@ -130,6 +130,35 @@ def test_oracle_host_scenes_match_reference_images(golden, name, kind, kw, size)
     assert np.array_equal(hits["t"], g[f"{name}_hits_t"])
 
 
+@pytest.mark.parametrize("name,kind,kw,size", HOST_CASES[1:])
+def test_anyhit_shadow_segment_is_the_reference_walk(golden, name, kind, kw, size):
+    """With no transmissive material the reference's closest-hit shadow walk
+    (pathtrace.c:504-532) equals one any-hit query: the GPU relies on this.
+    The oracle's any-hit pricing mode must leave every byte of the image and
+    every mean unchanged, and move the shadow queries to any-hit."""
+    hs, hold, prims, mats = host_scene(kind, kw)
+    transmissive = any(m["is_water"] or m["transparency"] > 0.8 for m in mats)
+    o = ck.OracleScene(hold.soa)
+    w, h, spp, depth = size
+    cam = hs.camera(w / h)
+    rgb0, mean0, _, _ = o.render(cam, w, h, spp, depth, mode=0)
+    s0 = o.stats()
+    try:
+        o.set_anyhit_shadow(True)
+        rgb1, mean1, _, _ = o.render(cam, w, h, spp, depth, mode=0)
+        s1 = o.stats()
+    finally:
+        o.set_anyhit_shadow(False)
+    assert np.array_equal(rgb0, rgb1) and np.array_equal(mean0, mean1)
+    if transmissive:  # the walk must stay: nothing changes
+        assert s1 == s0 and s1["any"] == 0
+        return
+    assert s0["any"] == 0 and s1["any"] > 0
+    assert s1["closest"] == s0["closest"] - s1["any"]
+    assert s1["ext"] == s0["ext"]
+    assert s1["a_inner"] < s0["c_inner"] - s1["c_inner"]
+
+
 # ---- closed forms ------------------------------------------------------------
 
 def radiance(o, ray, depth, seed):
