@@ -325,6 +325,36 @@ int mh_render(struct mh_scene *s, const mh_camera *cam,
 int mh_framebuffer_download(struct mh_scene *s, unsigned char *rgb_out,
                             float *linear_out);
 
+/* Replace every material (same count as at creation) / the sky of an
+ * uploaded scene, keeping geometry and BVH. The emissive-light list and the
+ * transmissive flag are recomputed as mh_scene_create would. No reference
+ * counterpart (the reference rebuilds its Scene); used by the differentiable
+ * renderer and parameter sweeps. */
+int mh_scene_set_materials(struct mh_scene *s, const mh_material *mats, int n);
+int mh_scene_set_sky(struct mh_scene *s, const mh_sky *sky);
+
+/* Path-replay adjoint of the fp32 path tracer (no reference counterpart).
+ * d_image: dLoss/d(linear pixel mean), w*h*3 floats, top-down like the
+ * render. grad receives n_materials*8 + 3 floats: for each material
+ * {albedo r, g, b, roughness, metallic, emissive r, g, b}, then
+ * dLoss/d(sky sun_color) r, g, b. Same sample streams as mh_render's path
+ * tracer with the same params (mode PATHTRACE, precision FP32). Sampling
+ * decisions are held fixed (gradient of the estimator's expectation);
+ * visibility is not differentiated. Returns 4 for Whitted or fp64 params. */
+int mh_adjoint(struct mh_scene *s, const mh_camera *cam,
+               const mh_render_params *p, const float *d_image, float *grad);
+
+/* fp32 path tracer whose sampling decisions (lobe choice and probability,
+ * GGX half vector and pdf, Russian roulette on its own throughput) are taken
+ * with the materials `frozen` (same count as the scene's) and whose BSDF
+ * values use the scene's current materials. The image is then a smooth
+ * function of the current parameters, with derivative at `frozen` equal to
+ * what mh_adjoint computes: finite differences of it check the adjoint
+ * without lobe-switching noise. No reference counterpart. */
+int mh_render_frozen(struct mh_scene *s, const mh_camera *cam,
+                     const mh_render_params *p, const mh_material *frozen, int n,
+                     unsigned char *rgb_out, float *linear_out);
+
 /* ------------------------------------------------------------------ */
 /* Batched rendering (no reference counterpart: the reference renders  */
 /* one image per pathtrace_render call, src/pathtrace.c:1256-1363)      */

@@ -272,6 +272,22 @@ struct mh_scene {
     hipGraph_t graph_def = nullptr;
     std::vector<unsigned char> graph_key;
     EventClock clock;
+    // material tables and the spheres that may be emissive lights (for updates)
+    void* d_m32 = nullptr;
+    void* d_m64 = nullptr;
+    struct Sphere {
+        int prim, mat;
+        mh_vec3 c;
+        double r;
+    };
+    std::vector<Sphere> spheres;
+    // adjoint buffers
+    float* d_dimg = nullptr;
+    size_t dimg_n = 0;
+    double* d_grad = nullptr;
+    size_t grad_n = 0;
+    void* d_frozen = nullptr;     // frozen-sampling material table (fp32)
+    size_t frozen_bytes = 0;
 };
 
 namespace {
@@ -289,6 +305,27 @@ int upload(mh_scene* s, const void* src, size_t bytes, void** dst)
     return 0;
 }
 
+int validate_sky(const mh_sky& k)
+{
+    if (k.cloud_thickness > 1e-4) return fail(4, "volumetric clouds (cloud_thickness %g) are outside the GPU path", k.cloud_thickness);
+    if (k.star_intensity > 0.0 || k.nebula_intensity > 0.0 || k.galaxy_intensity > 0.0)
+        return fail(4, "stars / nebula / galaxy sky layers are outside the GPU path");
+    if (k.fog_density > 0.0) return fail(4, "fog is outside the GPU path");
+    return 0;
+}
+
+int validate_materials(const mh_material* mats, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        const mh_material& m = mats[i];
+        if (m.bump_strength > 1e-6) return fail(4, "material %d: bump mapping is outside the GPU path", i);
+        if (m.texture_kind != MH_TEXTURE_NONE && m.texture_kind != MH_TEXTURE_CHECKER &&
+            m.texture_kind != MH_TEXTURE_UV_CHECKER)
+            return fail(4, "material %d: texture kind %d is outside the GPU path", i, m.texture_kind);
+    }
+    return 0;
+}
+
 int validate(const mh_scene_soa* soa)
 {
     if (soa->abi_version != MH_ABI_VERSION) return fail(5, "abi_version %d != %d", soa->abi_version, MH_ABI_VERSION);
@@ -297,18 +334,10 @@ int validate(const mh_scene_soa* soa)
     if ((soa->n_prims && !soa->prims) || (soa->n_materials && !soa->materials) || (soa->n_nodes && !soa->nodes) ||
         (soa->n_leaf_prims && !soa->leaf_prims) || (soa->n_planes && !soa->planes))
         return fail(1, "NULL array with non-zero count");
-    const mh_sky& k = soa->sky;
-    if (k.cloud_thickness > 1e-4) return fail(4, "volumetric clouds (cloud_thickness %g) are outside the GPU path", k.cloud_thickness);
-    if (k.star_intensity > 0.0 || k.nebula_intensity > 0.0 || k.galaxy_intensity > 0.0)
-        return fail(4, "stars / nebula / galaxy sky layers are outside the GPU path");
-    if (k.fog_density > 0.0) return fail(4, "fog is outside the GPU path");
-    for (int i = 0; i < soa->n_materials; ++i) {
-        const mh_material& m = soa->materials[i];
-        if (m.bump_strength > 1e-6) return fail(4, "material %d: bump mapping is outside the GPU path", i);
-        if (m.texture_kind != MH_TEXTURE_NONE && m.texture_kind != MH_TEXTURE_CHECKER &&
-            m.texture_kind != MH_TEXTURE_UV_CHECKER)
-            return fail(4, "material %d: texture kind %d is outside the GPU path", i, m.texture_kind);
-    }
+    int rc = validate_sky(soa->sky);
+    if (rc) return rc;
+    rc = validate_materials(soa->materials, soa->n_materials);
+    if (rc) return rc;
     int n_planes = 0;
     for (int i = 0; i < soa->n_prims; ++i) {
         const mh_prim& p = soa->prims[i];
@@ -481,6 +510,36 @@ int ensure_wf(mh_scene* s, int pool, size_t npix)
     if (rc) return rc;
     s->wf_pool = pool;
     s->wf_npix = npix;
+    return 0;
+}
+
+// Emissive area lights: the first 8 PBR spheres with emission, file order
+// (reference scene_collect_emissive_lights, src/scene.c:1384-1426).
+int collect_lights(const mh_scene* s, const mh_material* mats, int n_mats, DLight<float>* l32, DLight<double>* l64)
+{
+    int nl = 0;
+    for (const auto& sp : s->spheres) {
+        if (nl >= 8) break;
+        if (sp.mat < 0 || sp.mat >= n_mats) continue;
+        const mh_material& m = mats[sp.mat];
+        if (!m.pbr) continue;
+        if (m.emissive.x == 0.0 && m.emissive.y == 0.0 && m.emissive.z == 0.0) continue;
+        if (!(sp.r > 0.0)) continue;
+        l32[nl] = {sp.prim, sp.mat, cv<float>(sp.c), (float)sp.r, cv<float>(m.emissive)};
+        l64[nl] = {sp.prim, sp.mat, cv<double>(sp.c), sp.r, cv<double>(m.emissive)};
+        ++nl;
+    }
+    for (int i = nl; i < 8; ++i) {
+        l32[i] = DLight<float>{};
+        l64[i] = DLight<double>{};
+    }
+    return nl;
+}
+
+int any_transmissive(const mh_material* mats, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (mats[i].is_water || mats[i].transparency > 0.8) return 1;
     return 0;
 }
 
@@ -781,25 +840,16 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         return bail(fail(102, "hipMalloc failed for counters"));
     if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(103, "upload failed"));
 
-    // emissive area lights: first 8 PBR spheres with emission, file order
-    int nl = 0;
+    s->d_m32 = d_m32;
+    s->d_m64 = d_m64;
+    for (int i = 0; i < soa->n_prims; ++i) {
+        const mh_prim& p = soa->prims[i];
+        if (p.kind == MH_PRIM_SPHERE) s->spheres.push_back({i, p.material, p.p0, p.r0});
+    }
     DLight<float> l32[8];
     DLight<double> l64[8];
-    for (int i = 0; i < soa->n_prims && nl < 8; ++i) {
-        const mh_prim& p = soa->prims[i];
-        if (p.kind != MH_PRIM_SPHERE || p.material < 0 || p.material >= soa->n_materials) continue;
-        const mh_material& m = soa->materials[p.material];
-        if (!m.pbr) continue;
-        if (m.emissive.x == 0.0 && m.emissive.y == 0.0 && m.emissive.z == 0.0) continue;
-        if (!(p.r0 > 0.0)) continue;
-        l32[nl] = {i, cv<float>(p.p0), (float)p.r0, cv<float>(m.emissive)};
-        l64[nl] = {i, cv<double>(p.p0), p.r0, cv<double>(m.emissive)};
-        ++nl;
-    }
-
-    int has_transmissive = 0;
-    for (int i = 0; i < soa->n_materials; ++i)
-        if (soa->materials[i].is_water || soa->materials[i].transparency > 0.8) has_transmissive = 1;
+    int nl = collect_lights(s, soa->materials, soa->n_materials, l32, l64);
+    int has_transmissive = any_transmissive(soa->materials, soa->n_materials);
 
     auto fill = [&](auto& v, const void* pr, const void* pl, const void* at, const void* mt, auto* lights, auto tag) {
         typedef decltype(tag) R;
@@ -840,10 +890,138 @@ void mh_scene_destroy(mh_scene* s)
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
     if (s->graph_def) (void)hipGraphDestroy(s->graph_def);
     if (s->wf_mem) (void)hipFree(s->wf_mem);
+    if (s->d_dimg) (void)hipFree(s->d_dimg);
+    if (s->d_grad) (void)hipFree(s->d_grad);
+    if (s->d_frozen) (void)hipFree(s->d_frozen);
     for (auto& e : s->ev)
         if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
+}
+
+static void drop_scene_graph(mh_scene* s)
+{
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    if (s->graph_def) (void)hipGraphDestroy(s->graph_def);
+    s->graph = nullptr;
+    s->graph_def = nullptr;
+    s->graph_key.clear();
+}
+
+int mh_scene_set_materials(mh_scene* s, const mh_material* mats, int n)
+{
+    if (!s || (!mats && n > 0)) return fail(1, "NULL argument");
+    if (n != s->v32.n_mats) return fail(5, "material count %d != the scene's %d", n, s->v32.n_mats);
+    int rc = validate_materials(mats, n);
+    if (rc) return rc;
+    std::vector<DMat<float>> m32((size_t)n);
+    std::vector<DMat<double>> m64((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        m32[i] = conv_mat<float>(mats[i]);
+        m64[i] = conv_mat<double>(mats[i]);
+    }
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(s->d_m32, m32.data(), m32.size() * sizeof(DMat<float>), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->d_m64, m64.data(), m64.size() * sizeof(DMat<double>), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    DLight<float> l32[8];
+    DLight<double> l64[8];
+    int nl = collect_lights(s, mats, n, l32, l64);
+    int tr = any_transmissive(mats, n);
+    s->v32.n_lights = s->v64.n_lights = nl;
+    for (int i = 0; i < 8; ++i) {
+        s->v32.lights[i] = l32[i];
+        s->v64.lights[i] = l64[i];
+    }
+    s->v32.has_transmissive = s->v64.has_transmissive = tr;
+    drop_scene_graph(s);  // captured launches hold the scene view by value
+    return 0;
+}
+
+int mh_scene_set_sky(mh_scene* s, const mh_sky* sky)
+{
+    if (!s || !sky) return fail(1, "NULL argument");
+    int rc = validate_sky(*sky);
+    if (rc) return rc;
+    s->v32.sky = conv_sky<float>(*sky);
+    s->v64.sky = conv_sky<double>(*sky);
+    drop_scene_graph(s);
+    return 0;
+}
+
+int mh_render_frozen(mh_scene* s, const mh_camera* cam, const mh_render_params* p, const mh_material* frozen, int n,
+                     unsigned char* rgb_out, float* linear_out)
+{
+    if (!s || !cam || !p || !frozen) return fail(1, "NULL argument");
+    if (p->width <= 0 || p->height <= 0) return fail(2, "non-positive image size %dx%d", p->width, p->height);
+    if (p->spp < 1 || p->max_depth < 0 || p->sample_offset < 0)
+        return fail(3, "bad spp %d / depth %d / offset %d", p->spp, p->max_depth, p->sample_offset);
+    if ((long long)p->width * p->height > (1LL << 28)) return fail(2, "image too large");
+    if (p->mode != MH_MODE_PATHTRACE || p->precision != MH_PREC_FP32)
+        return fail(4, "frozen sampling renders the fp32 path tracer only");
+    if (n != s->v32.n_mats) return fail(5, "frozen material count %d != the scene's %d", n, s->v32.n_mats);
+    int rc = validate_materials(frozen, n);
+    if (rc) return rc;
+    std::vector<DMat<float>> f((size_t)n);
+    for (int i = 0; i < n; ++i) f[i] = conv_mat<float>(frozen[i]);
+    size_t fb = f.size() * sizeof(DMat<float>);
+    if (fb > s->frozen_bytes) {
+        if (s->d_frozen) (void)hipFree(s->d_frozen);
+        s->d_frozen = nullptr;
+        s->frozen_bytes = 0;
+        HIP_TRY(hipMalloc(&s->d_frozen, fb > 0 ? fb : 1));
+        s->frozen_bytes = fb;
+    }
+    rc = ensure_fb(s, p->width, p->height);
+    if (rc) return rc;
+    if (fb) HIP_TRY(hipMemcpyAsync(s->d_frozen, f.data(), fb, hipMemcpyHostToDevice, s->stream));
+    LaunchOut lo{s->fb_linear, s->fb_rgb, s->d_rays, s->d_overflow};
+    HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+    HIP_TRY(launch_render_frozen(s->v32, conv_cam<float>(*cam), p->width, p->height, p->spp, p->sample_offset,
+                                 p->max_depth, s->d_frozen, lo, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+    size_t n3 = (size_t)p->width * p->height * 3;
+    if (rgb_out) HIP_TRY(hipMemcpyAsync(rgb_out, s->fb_rgb, n3, hipMemcpyDeviceToHost, s->stream));
+    if (linear_out) HIP_TRY(hipMemcpyAsync(linear_out, s->fb_linear, n3 * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    memset(&g_timing, 0, sizeof g_timing);
+    g_timing.kernel_ms = g_timing.total_ms = ms_between(s->ev[0], s->ev[1]);
+    g_timing.samples = (int64_t)p->width * p->height * p->spp;
+    g_seconds = g_timing.kernel_ms * 1e-3;
+    return 0;
+}
+
+int mh_adjoint(mh_scene* s, const mh_camera* cam, const mh_render_params* p, const float* d_image, float* grad)
+{
+    if (!s || !cam || !p || !d_image || !grad) return fail(1, "NULL argument");
+    if (p->width <= 0 || p->height <= 0) return fail(2, "non-positive image size %dx%d", p->width, p->height);
+    if (p->spp < 1 || p->max_depth < 0 || p->sample_offset < 0)
+        return fail(3, "bad spp %d / depth %d / offset %d", p->spp, p->max_depth, p->sample_offset);
+    if ((long long)p->width * p->height > (1LL << 28)) return fail(2, "image too large");
+    if (p->mode != MH_MODE_PATHTRACE) return fail(4, "the adjoint differentiates the path tracer only");
+    if (p->precision != MH_PREC_FP32) return fail(4, "the adjoint computes in fp32 only");
+    size_t n3 = (size_t)p->width * p->height * 3;
+    size_t ng = (size_t)s->v32.n_mats * 8 + 3;
+    int rc = grow(&s->d_dimg, &s->dimg_n, n3, sizeof(float));
+    if (rc) return rc;
+    rc = grow(&s->d_grad, &s->grad_n, ng, sizeof(double));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(s->d_dimg, d_image, n3 * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_grad, 0, ng * sizeof(double), s->stream));
+    HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+    HIP_TRY(launch_adjoint(s->v32, conv_cam<float>(*cam), p->width, p->height, p->spp, p->sample_offset, p->max_depth,
+                           s->d_dimg, s->d_grad, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+    std::vector<double> g(ng);
+    HIP_TRY(hipMemcpyAsync(g.data(), s->d_grad, ng * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < ng; ++i) grad[i] = (float)g[i];
+    memset(&g_timing, 0, sizeof g_timing);
+    g_timing.kernel_ms = g_timing.total_ms = ms_between(s->ev[0], s->ev[1]);
+    g_timing.samples = (int64_t)p->width * p->height * p->spp;
+    g_seconds = g_timing.kernel_ms * 1e-3;
+    return 0;
 }
 
 size_t mh_scene_device_bytes(const mh_scene* s) { return s ? s->device_bytes : 0; }

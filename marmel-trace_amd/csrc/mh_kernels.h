@@ -22,6 +22,17 @@ template <typename R>
 hipError_t launch_pathtrace(const SceneView<R>& sc, const CamView<R>& cam, int w, int h, int spp, int s0, int depth,
                             const LaunchOut& out, int block, hipStream_t st);
 
+// Path-replay adjoint (fp32): accumulates dLoss/dtheta into grad
+// (n_mats * 8 + 3 doubles: per material albedo rgb, roughness, metallic,
+// emissive rgb; then sun colour rgb) from dimg = dLoss/d(pixel mean).
+hipError_t launch_adjoint(const SceneView<float>& sc, const CamView<float>& cam, int w, int h, int spp, int s0,
+                          int depth, const float* dimg, double* grad, hipStream_t st);
+
+// fp32 path tracer whose sampling decisions use the device material table
+// `frozen` (DMat<float>[n_mats]) and BSDF values the scene's materials.
+hipError_t launch_render_frozen(const SceneView<float>& sc, const CamView<float>& cam, int w, int h, int spp, int s0,
+                                int depth, const void* frozen, const LaunchOut& out, hipStream_t st);
+
 // ---- wavefront engine (fp32) ---------------------------------------------
 
 // Device state of the wavefront renderer for `pool` paths in flight.

@@ -59,6 +59,53 @@ __device__ bool sample_pbr(const DMat<R>& m, V3<R> alb, V3<R> N, V3<R> d, V3<R> 
     return true;
 }
 
+// PBR lobe sampling with every decision (lobe, half vector, pdf, lobe
+// probability) from the frozen material f / albedo falb and the BSDF value
+// from m / alb: the weight f_m / p_f is then a smooth function of m.
+template <typename R>
+__device__ bool sample_pbr_frozen(const DMat<R>& m, V3<R> alb, const DMat<R>& f, V3<R> falb, V3<R> N, V3<R> d,
+                                  V3<R> V, R u1, R u2, R ul, Lobe<R>* out)
+{
+    out->spec = 0;
+    out->refr = 0;
+    R met = clamp01n(m.metallic), rough = clamp01n(m.roughness);
+    R alpha = rough_alpha(rough);
+    R fmet = clamp01n(f.metallic), frough = clamp01n(f.roughness);
+    R falpha = rough_alpha(frough);
+    V3<R> f0 = lerp(mk(R(0.04), R(0.04), R(0.04)), alb, met);
+    V3<R> ff0 = lerp(mk(R(0.04), R(0.04), R(0.04)), falb, fmet);
+    R lum = R(0.2126) * ff0.x + R(0.7152) * ff0.y + R(0.0722) * ff0.z;
+    lum = lum > R(0) ? lum : R(0);
+    R ps = p_specular(fmet, frough, lum);
+    R ndv = dot(N, V);
+    if (!(ndv > R(0))) return false;
+    if (ul < ps) {
+        V3<R> H = ggx_half(N, falpha, u1, u2);
+        V3<R> wi = reflect(d, H);
+        R ndl = dot(N, wi), ndh = dot(N, H), vdh = dot(V, H);
+        if (!(ndl > R(0)) || !(ndh > R(0)) || !(vdh > R(0))) return false;
+        R D = d_ggx(ndh, alpha);
+        R k = clamp01n((rough + R(1)) * (rough + R(1)) / R(8));
+        R G = (ndl / (ndl * (R(1) - k) + k)) * (ndv / (ndv * (R(1) - k) + k));
+        V3<R> F = schlick_s3(vdh, f0);
+        R pdf = ggx_pdf(ndh, vdh, falpha);
+        if (!(pdf > R(0)) || !(ps > R(0))) return false;
+        out->wi = wi;
+        out->w = F * (D * G / (R(4) * ndv * pdf * ps));
+        out->spec = 1;
+        return true;
+    }
+    R den = R(1) - ps;
+    if (!(den > R(0))) return false;
+    V3<R> wi = cosine_dir(N, u1, u2);
+    if (!(dot(N, wi) > R(0))) return false;
+    V3<R> F = schlick_s3(ndv, f0);
+    V3<R> kd = (mk(R(1), R(1), R(1)) - F) * (R(1) - met);
+    out->wi = wi;
+    out->w = vmul(kd, alb) * (R(1) / den);
+    return true;
+}
+
 template <typename R>
 __device__ bool sample_legacy(const DMat<R>& m, V3<R> alb, V3<R> N, V3<R> d, V3<R> V, int front, R u1, R u2, R ul,
                               R ug, Lobe<R>* out)
@@ -160,25 +207,50 @@ __device__ bool occluded_segment(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d
     return true;
 }
 
+// Returns the radiance it added to L. beta2 (optional) is attenuated alike.
 template <typename R>
-__device__ void apply_medium(const DMat<R>& m, R dist, V3<R>* beta, V3<R>* L)
+__device__ V3<R> apply_medium(const DMat<R>& m, R dist, V3<R>* beta, V3<R>* L, V3<R>* beta2 = nullptr)
 {
     V3<R> T, deep;
     medium_terms(m, dist, &T, &deep);
     V3<R> white = mk(R(1) * T.x + deep.x, R(1) * T.y + deep.y, R(1) * T.z + deep.z);
     V3<R> Tm = white - deep;
-    *L = *L + vmul(*beta, deep);
+    V3<R> add = vmul(*beta, deep);
+    *L = *L + add;
     *beta = vmul(*beta, Tm);
+    if (beta2) *beta2 = vmul(*beta2, Tm);
+    return add;
 }
 
-template <typename R, typename TS>
-__device__ V3<R> pt_radiance(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0, int max_depth, uint32_t seed,
-                             int* nq)
+// Observer of one path's radiance terms, for the path-replay adjoint. The
+// forward renderers use NoHook, which compiles away. Every term added to L
+// is reported once, in order, through exactly one of these calls.
+struct NoHook {
+    template <typename R> __device__ void add(V3<R>) {}
+    template <typename R> __device__ void sky(V3<R>, V3<R>, V3<R>) {}
+    template <typename R> __device__ void emission(V3<R>, int, V3<R>) {}
+    template <typename R>
+    __device__ void nee_light(const DMat<R>&, V3<R>, V3<R>, V3<R>, V3<R>, V3<R>, int, V3<R>, R, V3<R>)
+    {
+    }
+    template <typename R>
+    __device__ void nee_sun(const DMat<R>&, V3<R>, V3<R>, V3<R>, V3<R>, V3<R>, V3<R>, R, V3<R>)
+    {
+    }
+    template <typename R> __device__ void bsdf(int, const DMat<R>&, V3<R>, V3<R>, V3<R>, const Lobe<R>&) {}
+    // Frozen-sampling renders return the material whose parameters take the
+    // sampling decisions at this vertex; nullptr = the vertex's own.
+    template <typename R> __device__ const DMat<R>* frozen(int) const { return nullptr; }
+};
+
+template <typename R, typename TS, typename HK>
+__device__ V3<R> pt_radiance_h(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0, int max_depth, uint32_t seed,
+                               int* nq, HK& hk)
 {
     V3<R> L = zero3<R>();
     V3<R> o = ro, d = normalize(rd0);
     if (len2(d) <= R(0)) return L;
-    V3<R> beta = mk(R(1), R(1), R(1));
+    V3<R> beta = mk(R(1), R(1), R(1)), beta_f = beta;
     int medium = -1;
     if (sc.water_material >= 0 && ro.y < sc.water_level && sc.water_material < sc.n_mats) medium = sc.water_material;
     int last_spec = 1;
@@ -187,15 +259,19 @@ __device__ V3<R> pt_radiance(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0,
         ++*nq;
         if (!intersect(sc, ts, o, d, R(1e-4), R(1e30), &h)) {
             if (medium >= 0) {
-                apply_medium(sc.mats[medium], R(5), &beta, &L);
+                hk.add(apply_medium(sc.mats[medium], R(5), &beta, &L, &beta_f));
                 medium = -1;
             }
-            L = L + vmul(beta, sky_sample(d, sc.sky));
+            V3<R> t = vmul(beta, sky_sample(d, sc.sky));
+            L = L + t;
+            hk.sky(beta, d, t);
             break;
         }
-        if (medium >= 0) apply_medium(sc.mats[medium], h.t, &beta, &L);
+        if (medium >= 0) hk.add(apply_medium(sc.mats[medium], h.t, &beta, &L, &beta_f));
         if (h.mat < 0 || h.mat >= sc.n_mats) {
-            L = L + vmul(beta, sky_sample(d, sc.sky));
+            V3<R> t = vmul(beta, sky_sample(d, sc.sky));
+            L = L + t;
+            hk.sky(beta, d, t);
             break;
         }
         const DMat<R>& m = sc.mats[h.mat];
@@ -207,8 +283,16 @@ __device__ V3<R> pt_radiance(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0,
         if (len2(N) <= R(0)) break;
         V3<R> V = -d;
         V3<R> alb = texture_albedo(m, P, h.u, h.v);
-        if ((b == 0 || last_spec) && m.has_emission) L = L + vmul(beta, m.emissive);
-        if (b == 0 && m.has_glow) L = L + vmul(beta, limb_glow(m, N, V, sc.sky.sun_dir));
+        if (b == 0 || last_spec) {
+            V3<R> t = m.has_emission ? vmul(beta, m.emissive) : zero3<R>();
+            if (m.has_emission) L = L + t;
+            hk.emission(beta, h.mat, t);
+        }
+        if (b == 0 && m.has_glow) {
+            V3<R> t = vmul(beta, limb_glow(m, N, V, sc.sky.sun_dir));
+            L = L + t;
+            hk.add(t);
+        }
 
         // emissive-sphere NEE, one cone sample per light
         for (int li = 0; li < sc.n_lights; ++li) {
@@ -235,7 +319,9 @@ __device__ V3<R> pt_radiance(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0,
             V3<R> att;
             if (occluded_segment(sc, ts, so, wi, R(1e-3), lt_t - R(1e-3), &att, nq)) continue;
             V3<R> lit = shade(m, alb, N, wi, V, vmul(lt.emissive, att));
-            L = L + vmul(beta, lit * omega);
+            V3<R> t = vmul(beta, lit * omega);
+            L = L + t;
+            hk.nee_light(m, alb, N, wi, V, beta, li, att, omega, t);
         }
         // sun NEE at the primary hit (or the first hit inside a medium after refraction)
         if (b == 0 || (b == 1 && medium >= 0 && last_spec)) {
@@ -247,7 +333,9 @@ __device__ V3<R> pt_radiance(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0,
                 if (!occluded_segment(sc, ts, P + N * R(1e-3), Ls, R(1e-3), R(1e30), &att, nq)) {
                     if (!(att.x < R(1e-4) && att.y < R(1e-4) && att.z < R(1e-4))) {
                         V3<R> lit = shade(m, alb, N, Ls, V, vmul(sc.sky.sun_color, att));
-                        L = L + vmul(beta, lit * sc.sky.sun_omega);
+                        V3<R> t = vmul(beta, lit * sc.sky.sun_omega);
+                        L = L + t;
+                        hk.nee_sun(m, alb, N, Ls, V, beta, att, sc.sky.sun_omega, t);
                     }
                 }
             }
@@ -259,22 +347,43 @@ __device__ V3<R> pt_radiance(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0,
         R ul = rand01<R>(kb, (uint32_t)b, 0x1001u);
         R ug = rand01<R>(kb, (uint32_t)b, 0x1002u);
         Lobe<R> lb;
-        bool ok = m.pbr ? sample_pbr(m, alb, N, d, V, u1, u2, ul, &lb)
-                        : sample_legacy(m, alb, N, d, V, h.front, u1, u2, ul, ug, &lb);
+        bool ok;
+        const DMat<R>* fm = hk.template frozen<R>(h.mat);
+        V3<R> w0 = zero3<R>();
+        if (fm) {
+            // decisions (and the roulette's throughput) from the frozen material
+            V3<R> falb = texture_albedo(*fm, P, h.u, h.v);
+            Lobe<R> l0;
+            if (m.pbr) {
+                ok = sample_pbr_frozen(m, alb, *fm, falb, N, d, V, u1, u2, ul, &lb);
+                if (ok) ok = sample_pbr(*fm, falb, N, d, V, u1, u2, ul, &l0);
+            } else {
+                ok = sample_legacy(m, alb, N, d, V, h.front, u1, u2, ul, ug, &lb);
+                if (ok) ok = sample_legacy(*fm, falb, N, d, V, h.front, u1, u2, ul, ug, &l0);
+            }
+            w0 = l0.w;
+        } else {
+            ok = m.pbr ? sample_pbr(m, alb, N, d, V, u1, u2, ul, &lb)
+                       : sample_legacy(m, alb, N, d, V, h.front, u1, u2, ul, ug, &lb);
+        }
         if (!ok) break;
+        hk.bsdf(h.mat, m, alb, N, V, lb);
         last_spec = lb.spec;
         beta = vmul(beta, lb.w);
-        R mc = beta.x;
-        if (beta.y > mc) mc = beta.y;
-        if (beta.z > mc) mc = beta.z;
+        if (fm) beta_f = vmul(beta_f, w0);
+        else beta_f = beta;
+        R mc = beta_f.x;
+        if (beta_f.y > mc) mc = beta_f.y;
+        if (beta_f.z > mc) mc = beta_f.z;
         if (!(mc > R(0))) mc = R(0);
-        if (!finite3(beta) || !(mc > R(0))) break;
+        if (!finite3(beta_f) || !(mc > R(0))) break;
         if (lb.refr && (m.is_water || m.beer_lambert)) medium = h.front ? h.mat : -1;
         if (b >= 2) {
             R q = clamp01n(mc);
             if (!(q > R(0))) break;
             if (rand01<R>(kb, (uint32_t)b, 0x5201u) >= q) break;
             beta = beta * (R(1) / q);
+            if (fm) beta_f = beta_f * (R(1) / q);
         }
         V3<R> off = lb.refr ? lb.wi * R(1e-4) : N * (dot(lb.wi, N) >= R(0) ? R(1e-4) : R(-1e-4));
         o = P + off;
@@ -285,6 +394,14 @@ __device__ V3<R> pt_radiance(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0,
     if (L.y < R(0)) L.y = R(0);
     if (L.z < R(0)) L.z = R(0);
     return L;
+}
+
+template <typename R, typename TS>
+__device__ V3<R> pt_radiance(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0, int max_depth, uint32_t seed,
+                             int* nq)
+{
+    NoHook hk;
+    return pt_radiance_h(sc, ts, ro, rd0, max_depth, seed, nq, hk);
 }
 
 template <typename R>

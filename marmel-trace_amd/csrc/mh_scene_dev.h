@@ -73,6 +73,7 @@ template <typename R> struct DSky {
 
 template <typename R> struct DLight {
     int prim;
+    int mat;
     V3<R> center;
     R radius;
     V3<R> emissive;

@@ -11,6 +11,12 @@
  *                     PBR materials.
  *   MT_GEN_ICOSPHERE  a randomly displaced icosphere + spheres on a plane
  *                     (the scenes of a multi-scene batch).
+ *   MT_GEN_MATERIALS  the differentiable-rendering scene: a PBR ground plane
+ *                     and a 9 x 7 grid of spheres, one GGX PBR material each
+ *                     (64 materials: albedo U[0.1,0.9], roughness
+ *                     U[0.1,0.9], metallic U[0,1]); the first n_emissive
+ *                     spheres (default 6) also emit U[1,6] per channel and
+ *                     are the scene's area lights.
  *
  * Every draw comes from a splitmix64 stream seeded by mt_gen_params.seed, so a
  * scene is a pure function of its parameters.
@@ -380,6 +386,54 @@ static int gen_icosphere(mt_scene *s, const mt_gen_params *p, rng_t *r, mt_gen_i
     return rc;
 }
 
+static mh_material pbr_material(rng_t *r)
+{
+    mh_material m;
+    mt_material_default(&m);
+    m.pbr = 1;
+    m.albedo = V(uni(r, 0.1, 0.9), uni(r, 0.1, 0.9), uni(r, 0.1, 0.9));
+    m.roughness = uni(r, 0.1, 0.9);
+    m.metallic = uni(r, 0.0, 1.0);
+    m.specular = V(0.04, 0.04, 0.04);
+    m.shininess = 32.0;
+    return m;
+}
+
+static int gen_materials(mt_scene *s, const mt_gen_params *p, rng_t *r, mt_gen_info *info)
+{
+    int nem = p->n_emissive > 0 ? p->n_emissive : 6;
+    if (nem > 8) nem = 8;
+    mh_material g = pbr_material(r);
+    g.metallic = 0.1 * g.metallic; /* a mostly dielectric ground, inside (0, 1) */
+    int gm = mt_scene_add_material(s, &g);
+    mh_prim plane;
+    memset(&plane, 0, sizeof plane);
+    plane.kind = MH_PRIM_PLANE;
+    plane.material = gm;
+    plane.p1 = V(0.0, 1.0, 0.0);
+    if (gm < 0 || mt_scene_add_prims(s, &plane, NULL, 1) < 0) return -1;
+    int k = 0;
+    for (int row = 0; row < 7; ++row)
+        for (int col = 0; col < 9; ++col, ++k) {
+            mh_material m = pbr_material(r);
+            if (k < nem) m.emissive = V(uni(r, 1.0, 6.0), uni(r, 1.0, 6.0), uni(r, 1.0, 6.0));
+            int mi = mt_scene_add_material(s, &m);
+            double rad = uni(r, 0.45, 0.8);
+            double x = (col - 4) * 2.0 + uni(r, -0.3, 0.3), z = (row - 3) * 2.0 + uni(r, -0.3, 0.3);
+            mh_prim sp = sphere(x, rad, z, rad, mi);
+            if (mi < 0 || mt_scene_add_prims(s, &sp, NULL, 1) < 0) return -1;
+        }
+    if (info) {
+        info->eye[0] = 0.0; info->eye[1] = 9.0; info->eye[2] = 14.0;
+        info->at[0] = 0.0; info->at[1] = 0.0; info->at[2] = 0.5;
+        info->vfov = 50.0;
+        info->n_spheres = 63;
+        info->n_emissive = nem;
+        info->n_prims = 64;
+    }
+    return 0;
+}
+
 int mt_scene_generate(mt_scene *s, const mt_gen_params *p, mt_gen_info *info)
 {
     if (!s || !p) return 1;
@@ -395,6 +449,7 @@ int mt_scene_generate(mt_scene *s, const mt_gen_params *p, mt_gen_info *info)
     case MT_GEN_SPHERES: return gen_spheres(s, p, &r, info);
     case MT_GEN_TERRAIN: return gen_terrain(s, p, &r, info);
     case MT_GEN_ICOSPHERE: return gen_icosphere(s, p, &r, info);
+    case MT_GEN_MATERIALS: return gen_materials(s, p, &r, info);
     default: return 2;
     }
 }

@@ -87,6 +87,11 @@ def hip():
                                          C.c_int, P])
     _sig(L, "mh_render", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P, P])
     _sig(L, "mh_framebuffer_download", C.c_int, [P, P, P])
+    _sig(L, "mh_scene_set_materials", C.c_int, [P, P, C.c_int])
+    _sig(L, "mh_scene_set_sky", C.c_int, [P, C.POINTER(abi.Sky)])
+    _sig(L, "mh_adjoint", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P, P])
+    _sig(L, "mh_render_frozen", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P,
+                                          C.c_int, P, P])
     _sig(L, "mh_batch_create", C.c_int, [C.POINTER(P)])
     _sig(L, "mh_batch_destroy", None, [P])
     _sig(L, "mh_render_batch", C.c_int, [P, P, C.c_int, C.POINTER(abi.Job), C.c_int,
@@ -120,4 +125,4 @@ class GenInfo(C.Structure):
                 ("n_spheres", C.c_int), ("n_emissive", C.c_int)]
 
 
-GEN_SPHERES, GEN_TERRAIN, GEN_ICOSPHERE = 1, 2, 3
+GEN_SPHERES, GEN_TERRAIN, GEN_ICOSPHERE, GEN_MATERIALS = 1, 2, 3, 4

@@ -170,6 +170,7 @@ class GpuScene:
         if rc:
             raise MarmelError(rc, self._L.mh_last_error().decode())
         self._h = h
+        self.n_materials = soa.n_materials
 
     def close(self):
         if self._h:
@@ -233,6 +234,39 @@ class GpuScene:
         self._check(self._L.mh_trace_any(self._h, abi.ptr(rays), len(rays), tmin, tmax,
                                          precision, abi.ptr(out)))
         return out
+
+    def set_materials(self, mats):
+        """Replace every material (numpy MATERIAL_DTYPE array, same count)."""
+        mats = np.ascontiguousarray(mats, dtype=abi.MATERIAL_DTYPE)
+        self._check(self._L.mh_scene_set_materials(self._h, abi.ptr(mats), len(mats)))
+
+    def set_sky(self, sky):
+        self._check(self._L.mh_scene_set_sky(self._h, C.byref(sky)))
+
+    def render_frozen(self, cam, width, height, spp, depth, frozen_mats, sample_offset=0):
+        """fp32 path tracer with sampling decisions taken from frozen_mats; returns linear (h, w, 3)."""
+        f = np.ascontiguousarray(frozen_mats, dtype=abi.MATERIAL_DTYPE)
+        p = abi.RenderParams(width=width, height=height, spp=spp, max_depth=depth,
+                             mode=abi.MODE_PATHTRACE, precision=abi.PREC_FP32,
+                             sample_offset=sample_offset)
+        lin = np.zeros((height, width, 3), np.float32)
+        self._check(self._L.mh_render_frozen(self._h, C.byref(cam), C.byref(p), abi.ptr(f), len(f), None,
+                                             abi.ptr(lin)))
+        return lin
+
+    def adjoint(self, cam, width, height, spp, depth, d_image, sample_offset=0):
+        """dLoss/dtheta for dLoss/d(pixel mean) d_image (h, w, 3): returns
+        (per-material (n, 8) array [albedo rgb, roughness, metallic, emissive rgb],
+        sun colour gradient (3,))."""
+        d = np.ascontiguousarray(d_image, dtype=np.float32)
+        assert d.shape == (height, width, 3)
+        p = abi.RenderParams(width=width, height=height, spp=spp, max_depth=depth,
+                             mode=abi.MODE_PATHTRACE, precision=abi.PREC_FP32,
+                             sample_offset=sample_offset)
+        n = self.n_materials
+        g = np.zeros(n * 8 + 3, np.float32)
+        self._check(self._L.mh_adjoint(self._h, C.byref(cam), C.byref(p), abi.ptr(d), abi.ptr(g)))
+        return g[:n * 8].reshape(n, 8), g[n * 8:]
 
     @staticmethod
     def last_kernel_times():

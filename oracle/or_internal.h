@@ -81,6 +81,7 @@ typedef struct {
 
 typedef struct {
     int prim;
+    int mat;
     dv3 center;
     double radius;
     dv3 emissive;
@@ -95,6 +96,7 @@ typedef struct or_scene {
     double water_level;
     int water_material;
     int has_transmissive;   /* some material is water or transparency > 0.8 */
+    mh_material *freeze;    /* non-NULL: sampling decisions use these materials */
     int n_lights;
     olight lights[8];
     /* binary SAH BVH restated from the reference build */
@@ -153,6 +155,7 @@ int or_occluded1(const or_scene *s, oray r, double tmin, double tmax);
 double or_clamp01(double v);
 dv3 or_shade_local(const mh_material *m, dv3 N, dv3 L, dv3 V, dv3 light);
 dv3 or_shade_pbr(const mh_material *m, dv3 N, dv3 L, dv3 V, dv3 light);
+double or_sky_cloud_alpha(dv3 dir, const mh_sky *sky);
 double or_roughness_to_alpha(double r);
 dv3 or_sample_glossy_dir(const mh_material *m, dv3 N, dv3 incident, double r1, double r2);
 dv3 or_ambient(const mh_material *m, dv3 N, const mh_sky *sky);

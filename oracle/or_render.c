@@ -148,13 +148,14 @@ void or_camera_ray(const mh_camera *cam, double u, double v, double r1, double r
 #define CH_EM_A 0xA201u
 #define CH_EM_B 0xA202u
 
-static void apply_medium(const mh_material *m, double dist, dv3 *beta, dv3 *rad)
+static void apply_medium(const mh_material *m, double dist, dv3 *beta, dv3 *rad, dv3 *beta2)
 {
     dv3 white = or_water_attenuate(m, d3(1.0, 1.0, 1.0), dist);
     dv3 deep = or_water_attenuate(m, d3(0.0, 0.0, 0.0), dist);
     dv3 T = dsub(white, deep);
     *rad = dadd(*rad, dmul(*beta, deep));
     *beta = dmul(*beta, T);
+    if (beta2) *beta2 = dmul(*beta2, T); /* frozen-sampling throughput */
 }
 
 static double pt_smith(double ndl, double ndv, double rough)
@@ -304,8 +305,143 @@ static dv3 shade(const mh_material *m, dv3 N, dv3 L, dv3 V, dv3 c)
     return m->pbr ? or_shade_pbr(m, N, L, V, c) : or_shade_local(m, N, L, V, c);
 }
 
+/* ---- path-replay adjoint (FP64) ---------------------------------------
+ * Observer of one path's radiance terms: with adj == NULL the radiance code
+ * below is the plain estimator. Gradient slots: per material {albedo rgb,
+ * roughness, metallic, emissive rgb}, then sun colour rgb. */
+typedef struct or_adj {
+    dv3 Lt, delta, Lacc;  /* path radiance, dLoss/dL per channel, terms so far */
+    double *g;
+    int n_mats;
+    int mat;              /* material of the vertex being shaded */
+    dv3 texf;             /* d(shaded albedo)/d(material albedo) there */
+} or_adj;
+
+static double in01(double v) { return v > 0.0 && v < 1.0 ? 1.0 : 0.0; }
+
+static double powpos_r(double b, double e) { return b <= 0.0 ? 0.0 : pow(b, e); }
+
+static void gput(or_adj *a, int mat, int k, double v)
+{
+    if (v != 0.0 && v == v) a->g[(size_t)mat * 8 + k] += v;
+}
+
+static void gsun(or_adj *a, int c, double v)
+{
+    if (v != 0.0 && v == v) a->g[(size_t)a->n_mats * 8 + c] += v;
+}
+
+/* d ln D / d alpha and d ln G1 / d k (GGX, Smith-Schlick). */
+static double dlnD_da(double ndh, double a)
+{
+    double d = ndh * ndh * (a * a - 1.0) + 1.0;
+    return 2.0 / a - 4.0 * a * ndh * ndh / d;
+}
+
+static double dlnG1_dk(double x, double k) { return -(1.0 - x) / (x * (1.0 - k) + k); }
+
+static double da_dr(double r) /* alpha = max(clamp(r)^2, 1e-4) */
+{
+    if (!(r > 0.0) || r >= 1.0) return 0.0;
+    return r * r > 1e-4 ? 2.0 * r : 0.0;
+}
+
+/* d shade(m, N, L, V, light)/d(shaded albedo, roughness, metallic). */
+static void shade_grad(const mh_material *m, dv3 N, dv3 L, dv3 V, dv3 light, dv3 *ga, dv3 *gr, dv3 *gm)
+{
+    *ga = *gr = *gm = d3(0.0, 0.0, 0.0);
+    double ndl = ddot(N, L);
+    if (ndl <= 0.0) return;
+    if (!m->pbr) {
+        *ga = dscale(light, ndl);
+        return;
+    }
+    double ndv = ddot(N, V);
+    if (ndv <= 0.0) return;
+    double met = or_clamp01(m->metallic), rr = or_clamp01(m->roughness);
+    double alpha = or_roughness_to_alpha(m->roughness);
+    dv3 hv = dadd(L, V);
+    double ndh = 0.0;
+    if (dlen2(hv) > 1e-12) {
+        ndh = ddot(N, dnorm(hv));
+        if (ndh < 0.0) ndh = 0.0;
+    }
+    double c = or_clamp01(ndv), q = 1.0 - c, k5 = q * q * q * q * q;
+    double D = or_d_ggx(ndh, alpha);
+    double k = (rr + 1.0) * (rr + 1.0) / 8.0;
+    double G = (ndl / (ndl * (1.0 - k) + k)) * (ndv / (ndv * (1.0 - k) + k));
+    double ss = D * G / (4.0 * ndl * ndv);
+    double dls = dlnD_da(ndh, alpha) * da_dr(rr) + (dlnG1_dk(ndl, k) + dlnG1_dk(ndv, k)) * (rr + 1.0) / 4.0 *
+                 in01(m->roughness);
+    double dmt = in01(m->metallic);
+    double a[3] = { m->albedo.x, m->albedo.y, m->albedo.z }, lc[3] = { light.x, light.y, light.z };
+    double oa[3], orr[3], om[3];
+    for (int i = 0; i < 3; ++i) {
+        double f0 = 0.04 + (a[i] - 0.04) * met;
+        double F = f0 + (1.0 - f0) * k5;
+        double dFa = met * (1.0 - k5), dFm = (a[i] - 0.04) * (1.0 - k5) * dmt;
+        oa[i] = lc[i] * ndl * (dFa * ss + (1.0 - met) / OR_PI * ((1.0 - F) - a[i] * dFa));
+        om[i] = lc[i] * ndl * (dFm * ss + a[i] / OR_PI * (-dFm * (1.0 - met) - (1.0 - F) * dmt));
+        orr[i] = lc[i] * ndl * F * ss * dls;
+    }
+    *ga = d3(oa[0], oa[1], oa[2]);
+    *gr = d3(orr[0], orr[1], orr[2]);
+    *gm = d3(om[0], om[1], om[2]);
+}
+
+/* A NEE term beta * shade(mm, light_col * att) * omega: its light source
+ * (emissive of light material lmat, or the sun when lmat < 0) and the shaded
+ * surface's parameters. */
+static void adj_nee(or_adj *a, const mh_material *mm, dv3 N, dv3 L, dv3 V, dv3 beta, dv3 light_col, dv3 att,
+                    double omega, int lmat, dv3 t)
+{
+    dv3 K = shade(mm, N, L, V, att);
+    dv3 w = dscale(dmul(a->delta, beta), omega);
+    double kv[3] = { w.x * K.x, w.y * K.y, w.z * K.z };
+    for (int c = 0; c < 3; ++c) {
+        if (lmat >= 0) gput(a, lmat, 5 + c, kv[c]);
+        else gsun(a, c, kv[c]);
+    }
+    dv3 ga, gr, gm;
+    shade_grad(mm, N, L, V, dmul(light_col, att), &ga, &gr, &gm);
+    ga = dmul(ga, a->texf);
+    gput(a, a->mat, 0, w.x * ga.x);
+    gput(a, a->mat, 1, w.y * ga.y);
+    gput(a, a->mat, 2, w.z * ga.z);
+    gput(a, a->mat, 3, w.x * gr.x + w.y * gr.y + w.z * gr.z);
+    gput(a, a->mat, 4, w.x * gm.x + w.y * gm.y + w.z * gm.z);
+    a->Lacc = dadd(a->Lacc, t);
+}
+
+/* d(sky)/d(sun colour) per channel. */
+static dv3 sky_dsun(dv3 dir, const mh_sky *sky)
+{
+    double h = or_clamp01(dir.y);
+    double t = powpos_r(h, sky->gradient_gamma);
+    dv3 base = dlerp(fromm(sky->horizon_color), fromm(sky->zenith_color), t);
+    double cs = ddot(dir, fromm(sky->sun_dir));
+    double gl = 0.0;
+    if (cs > 0.0) {
+        gl = powpos_r(cs, sky->sun_glow_exponent) * sky->sun_glow_strength;
+        base = dadd(base, dscale(fromm(sky->sun_color), gl));
+    }
+    dv3 dv = d3(gl, gl, gl);
+    double al = or_sky_cloud_alpha(dir, sky);
+    if (al > 0.0) {
+        dv3 cc = dscale(d3(0.95, 0.95, 0.98), 0.6 + 0.4 * al);
+        double lit = cs > 0.0 ? cs : 0.0;
+        dv3 lc = dadd(cc, dscale(dmul(cc, fromm(sky->sun_color)), 0.35 * lit));
+        base = dlerp(base, lc, al);
+        dv = dadd(dscale(dv, 1.0 - al), dscale(cc, al * 0.35 * lit));
+    }
+    if (!(base.x >= 0.0)) dv.x = 0.0;
+    if (!(base.y >= 0.0)) dv.y = 0.0;
+    if (!(base.z >= 0.0)) dv.z = 0.0;
+    return dv;
+}
+
 static dv3 nee_sun(const or_scene *s, const mh_material *mm, dv3 P, dv3 N, dv3 V, dv3 beta,
-                   unsigned seed, int b)
+                   unsigned seed, int b, or_adj *adj)
 {
     double r1 = or_rand01(seed, (unsigned)b, CH_SUN_A);
     double r2 = or_rand01(seed, (unsigned)b, CH_SUN_B);
@@ -318,11 +454,13 @@ static dv3 nee_sun(const or_scene *s, const mh_material *mm, dv3 P, dv3 N, dv3 V
     double rad = s->sky.sun_radius * OR_PI / 180.0;
     double omega = s->sky.sun_radius > 0.0 ? 2.0 * OR_PI * (1.0 - cos(rad)) : 1.0;
     dv3 lit = shade(mm, N, L, V, dmul(fromm(s->sky.sun_color), att));
-    return dmul(beta, dscale(lit, omega));
+    dv3 t = dmul(beta, dscale(lit, omega));
+    if (adj) adj_nee(adj, mm, N, L, V, beta, fromm(s->sky.sun_color), att, omega, -1, t);
+    return t;
 }
 
 static dv3 nee_emissive(const or_scene *s, const mh_material *mm, dv3 P, dv3 N, dv3 V,
-                        dv3 beta, int self, unsigned seed, int b)
+                        dv3 beta, int self, unsigned seed, int b, or_adj *adj)
 {
     dv3 sum = d3(0.0, 0.0, 0.0);
     for (int li = 0; li < s->n_lights; ++li) {
@@ -348,7 +486,9 @@ static dv3 nee_emissive(const or_scene *s, const mh_material *mm, dv3 P, dv3 N, 
         dv3 att;
         if (occluded_segment(s, sr, 1e-3, lh.t - 1e-3, &att)) continue;
         dv3 lit = shade(mm, N, wi, V, dmul(lt->emissive, att));
-        sum = dadd(sum, dmul(beta, dscale(lit, omega)));
+        dv3 t = dmul(beta, dscale(lit, omega));
+        if (adj) adj_nee(adj, mm, N, wi, V, beta, lt->emissive, att, omega, lt->mat, t);
+        sum = dadd(sum, t);
     }
     return sum;
 }
@@ -371,13 +511,100 @@ static dv3 limb_glow(const mh_material *mm, dv3 N, dv3 V, dv3 sun)
     return dscale(g, limb * day * 2.5);
 }
 
-static dv3 pt_radiance(const or_scene *s, oray primary, int max_depth, unsigned seed)
+/* PBR lobe sampling with every decision (lobe, half vector, pdf, lobe
+ * probability) taken from the frozen material f / albedo falb, and the BSDF
+ * value from m / alb: the weight is f_m / (p_f), a smooth function of m. */
+static int sample_pbr_frozen(const mh_material *m, dv3 alb, const mh_material *f, dv3 falb, dv3 N, dv3 d, dv3 V,
+                             double u1, double u2, double ul, dv3 *wi_o, dv3 *w_o, int *spec)
+{
+    *spec = 0;
+    double met = cl01n(m->metallic), rough = cl01n(m->roughness), alpha = or_roughness_to_alpha(rough);
+    double fmet = cl01n(f->metallic), frough = cl01n(f->roughness), falpha = or_roughness_to_alpha(frough);
+    dv3 f0 = dlerp(d3(0.04, 0.04, 0.04), alb, met);
+    dv3 ff0 = dlerp(d3(0.04, 0.04, 0.04), falb, fmet);
+    double ps = or_specular_probability(fmet, frough, luma(ff0));
+    double ndv = ddot(N, V);
+    if (!(ndv > 0.0)) return 0;
+    if (ul < ps) {
+        dv3 H = or_ggx_half(N, falpha, u1, u2);
+        dv3 wi = dreflect(d, H);
+        double ndl = ddot(N, wi), ndh = ddot(N, H), vdh = ddot(V, H);
+        if (!(ndl > 0.0) || !(ndh > 0.0) || !(vdh > 0.0)) return 0;
+        double D = or_d_ggx(ndh, alpha);
+        double G = pt_smith(ndl, ndv, rough);
+        dv3 F = or_fresnel_schlick_rgb(vdh, f0);
+        double pdf = or_ggx_pdf(ndh, vdh, falpha);
+        if (!(pdf > 0.0) || !(ps > 0.0)) return 0;
+        *wi_o = wi;
+        *w_o = dscale(F, D * G / (4.0 * ndv * pdf * ps));
+        *spec = 1;
+        return 1;
+    }
+    double den = 1.0 - ps;
+    if (!(den > 0.0)) return 0;
+    dv3 wi = or_cosine_hemisphere(N, u1, u2);
+    if (!(ddot(N, wi) > 0.0)) return 0;
+    dv3 F = or_fresnel_schlick_rgb(ndv, f0);
+    dv3 kd = dscale(dsub(d3(1.0, 1.0, 1.0), F), 1.0 - met);
+    *wi_o = wi;
+    *w_o = dscale(dmul(kd, alb), 1.0 / den);
+    return 1;
+}
+
+/* The sampled lobe's weight w multiplies all later terms: gradient
+ * delta . L_after * (dw/dtheta) / w per channel (sampling held fixed). */
+static void adj_bsdf(or_adj *a, const mh_material *mm, dv3 alb, dv3 N, dv3 V, dv3 wi, dv3 w, int spec, int refr)
+{
+    dv3 la = dsub(a->Lt, a->Lacc);
+    double sv[3] = { a->delta.x * la.x, a->delta.y * la.y, a->delta.z * la.z };
+    double wv[3] = { w.x, w.y, w.z }, av[3] = { alb.x, alb.y, alb.z };
+    double tf[3] = { a->texf.x, a->texf.y, a->texf.z };
+    if (!mm->pbr) {
+        if (spec || refr) return;
+        for (int i = 0; i < 3; ++i)
+            if (wv[i] != 0.0 && av[i] != 0.0) gput(a, a->mat, i, sv[i] / av[i] * tf[i]);
+        return;
+    }
+    double met = cl01n(mm->metallic), rough = cl01n(mm->roughness), dmt = in01(mm->metallic);
+    double ndv = ddot(N, V);
+    double gmet = 0.0, grough = 0.0;
+    if (spec) {
+        dv3 H = dnorm(dadd(V, wi));
+        double vdh = ddot(V, H), ndh = ddot(N, H), ndl = ddot(N, wi);
+        double c = cl01n(vdh), q = 1.0 - c, k5 = q * q * q * q * q;
+        double alpha = or_roughness_to_alpha(rough);
+        double k = cl01n((rough + 1.0) * (rough + 1.0) / 8.0);
+        double dr = dlnD_da(ndh, alpha) * da_dr(rough) + (dlnG1_dk(ndl, k) + dlnG1_dk(ndv, k)) * (rough + 1.0) / 4.0;
+        for (int i = 0; i < 3; ++i) {
+            if (wv[i] == 0.0) continue;
+            double f0 = 0.04 + (av[i] - 0.04) * met, fin = in01(f0);
+            double f = cl01n(f0), F = f + (1.0 - f) * k5;
+            gput(a, a->mat, i, sv[i] * met * (1.0 - k5) * fin / F * tf[i]);
+            gmet += sv[i] * (av[i] - 0.04) * (1.0 - k5) * fin * dmt / F;
+            grough += sv[i] * dr;
+        }
+        gput(a, a->mat, 4, gmet);
+        gput(a, a->mat, 3, grough * in01(mm->roughness));
+    } else {
+        double c = cl01n(ndv), q = 1.0 - c, k5 = q * q * q * q * q;
+        for (int i = 0; i < 3; ++i) {
+            if (wv[i] == 0.0) continue;
+            double f0 = 0.04 + (av[i] - 0.04) * met, fin = in01(f0);
+            double f = cl01n(f0), F = f + (1.0 - f) * k5;
+            if (av[i] != 0.0) gput(a, a->mat, i, sv[i] * (1.0 / av[i] - met * (1.0 - k5) * fin / (1.0 - F)) * tf[i]);
+            gmet += sv[i] * (-(av[i] - 0.04) * (1.0 - k5) * fin * dmt / (1.0 - F) - dmt / (1.0 - met));
+        }
+        gput(a, a->mat, 4, gmet);
+    }
+}
+
+static dv3 pt_radiance(const or_scene *s, oray primary, int max_depth, unsigned seed, or_adj *adj)
 {
     dv3 L = d3(0.0, 0.0, 0.0);
     if (max_depth < 0) max_depth = 0;
     oray r = { primary.o, dnorm(primary.d) };
     if (dlen2(r.d) <= 0.0) return L;
-    dv3 beta = d3(1.0, 1.0, 1.0);
+    dv3 beta = d3(1.0, 1.0, 1.0), beta_f = beta;
     const mh_material *medium = NULL;
     if (s->water_material >= 0 && primary.o.y < s->water_level) medium = mat_of(s, s->water_material);
     int last_spec = 1;
@@ -386,16 +613,38 @@ static dv3 pt_radiance(const or_scene *s, oray primary, int max_depth, unsigned 
         or_tls_stats.ext_queries++;
         if (!or_intersect1(s, r, 1e-4, 1e30, &h)) {
             if (medium) {
-                apply_medium(medium, 5.0, &beta, &L);
+                dv3 L0 = L;
+                apply_medium(medium, 5.0, &beta, &L, &beta_f);
+                if (adj) adj->Lacc = dadd(adj->Lacc, dsub(L, L0));
                 medium = NULL;
             }
-            L = dadd(L, dmul(beta, or_sky_sample(r.d, &s->sky)));
+            dv3 t = dmul(beta, or_sky_sample(r.d, &s->sky));
+            if (adj) {
+                dv3 ds = dmul(dmul(adj->delta, beta), sky_dsun(r.d, &s->sky));
+                gsun(adj, 0, ds.x);
+                gsun(adj, 1, ds.y);
+                gsun(adj, 2, ds.z);
+                adj->Lacc = dadd(adj->Lacc, t);
+            }
+            L = dadd(L, t);
             break;
         }
-        if (medium) apply_medium(medium, h.t, &beta, &L);
+        if (medium) {
+            dv3 L0 = L;
+            apply_medium(medium, h.t, &beta, &L, &beta_f);
+            if (adj) adj->Lacc = dadd(adj->Lacc, dsub(L, L0));
+        }
         const mh_material *m = mat_of(s, h.material);
         if (!m) {
-            L = dadd(L, dmul(beta, or_sky_sample(r.d, &s->sky)));
+            dv3 t = dmul(beta, or_sky_sample(r.d, &s->sky));
+            if (adj) {
+                dv3 ds = dmul(dmul(adj->delta, beta), sky_dsun(r.d, &s->sky));
+                gsun(adj, 0, ds.x);
+                gsun(adj, 1, ds.y);
+                gsun(adj, 2, ds.z);
+                adj->Lacc = dadd(adj->Lacc, t);
+            }
+            L = dadd(L, t);
             break;
         }
         dv3 P = h.point, d = r.d, N = h.normal;
@@ -409,12 +658,30 @@ static dv3 pt_radiance(const or_scene *s, oray primary, int max_depth, unsigned 
         mh_material ml = *m;
         dv3 alb = or_texture_albedo(m, P, h.u, h.v);
         ml.albedo = (mh_vec3){ alb.x, alb.y, alb.z };
-        if (b == 0 || last_spec) L = dadd(L, dmul(beta, fromm(ml.emissive)));
-        if (b == 0 && dlen2(fromm(ml.atmosphere_glow)) > 1e-6)
-            L = dadd(L, dmul(beta, limb_glow(&ml, N, V, fromm(s->sky.sun_dir))));
-        L = dadd(L, nee_emissive(s, &ml, P, N, V, beta, h.prim, seed, b));
+        if (adj) {
+            adj->mat = h.material;
+            adj->texf = d3(m->texture_kind == 0 ? 1.0 : (m->albedo.x != 0.0 ? alb.x / m->albedo.x : 0.0),
+                           m->texture_kind == 0 ? 1.0 : (m->albedo.y != 0.0 ? alb.y / m->albedo.y : 0.0),
+                           m->texture_kind == 0 ? 1.0 : (m->albedo.z != 0.0 ? alb.z / m->albedo.z : 0.0));
+        }
+        if (b == 0 || last_spec) {
+            dv3 t = dmul(beta, fromm(ml.emissive));
+            L = dadd(L, t);
+            if (adj) {
+                gput(adj, h.material, 5, adj->delta.x * beta.x);
+                gput(adj, h.material, 6, adj->delta.y * beta.y);
+                gput(adj, h.material, 7, adj->delta.z * beta.z);
+                adj->Lacc = dadd(adj->Lacc, t);
+            }
+        }
+        if (b == 0 && dlen2(fromm(ml.atmosphere_glow)) > 1e-6) {
+            dv3 t = dmul(beta, limb_glow(&ml, N, V, fromm(s->sky.sun_dir)));
+            L = dadd(L, t);
+            if (adj) adj->Lacc = dadd(adj->Lacc, t);
+        }
+        L = dadd(L, nee_emissive(s, &ml, P, N, V, beta, h.prim, seed, b, adj));
         if (b == 0 || (b == 1 && medium != NULL && last_spec))
-            L = dadd(L, nee_sun(s, &ml, P, N, V, beta, seed, b));
+            L = dadd(L, nee_sun(s, &ml, P, N, V, beta, seed, b, adj));
         unsigned kb = or_hash3(seed ^ K_BOUNCE, (unsigned)b, 0u);
         double u1 = or_rand01(kb, (unsigned)b, CH_U1);
         double u2 = or_rand01(kb, (unsigned)b, CH_U2);
@@ -422,9 +689,43 @@ static dv3 pt_radiance(const or_scene *s, oray primary, int max_depth, unsigned 
         double ug = or_rand01(kb, (unsigned)b, CH_GLASS);
         dv3 wi, w;
         int ok, refr = 0, spec = 0;
+        if (s->freeze) {
+            /* frozen sampling: decisions and the roulette's throughput from
+             * the frozen materials, BSDF values from the current ones */
+            const mh_material *fm = &s->freeze[h.material];
+            dv3 falb = or_texture_albedo(fm, P, h.u, h.v);
+            mh_material fl = *fm;
+            fl.albedo = (mh_vec3){ falb.x, falb.y, falb.z };
+            dv3 w0;
+            if (ml.pbr) {
+                ok = sample_pbr_frozen(&ml, alb, &fl, falb, N, d, V, u1, u2, ul, &wi, &w, &spec);
+                if (ok) ok = sample_pbr(&fl, falb, N, d, V, u1, u2, ul, &wi, &w0, &spec);
+            } else {
+                ok = sample_legacy(&ml, alb, N, d, V, h.front, u1, u2, ul, ug, &wi, &w, &refr, &spec);
+                if (ok) ok = sample_legacy(&fl, falb, N, d, V, h.front, u1, u2, ul, ug, &wi, &w0, &refr, &spec);
+            }
+            if (!ok) break;
+            last_spec = spec;
+            beta = dmul(beta, w);
+            beta_f = dmul(beta_f, w0);
+            if (!dfinite(beta_f) || !(max_comp(beta_f) > 0.0)) break;
+            if (refr && (m->is_water || m->beer_lambert)) medium = h.front ? m : NULL;
+            if (b >= 2) {
+                double q = cl01n(max_comp(beta_f));
+                if (!(q > 0.0)) break;
+                if (or_rand01(kb, (unsigned)b, CH_RR) >= q) break;
+                beta = dscale(beta, 1.0 / q);
+                beta_f = dscale(beta_f, 1.0 / q);
+            }
+            dv3 off = refr ? dscale(wi, 1e-4) : dscale(N, ddot(wi, N) >= 0.0 ? 1e-4 : -1e-4);
+            r.o = dadd(P, off);
+            r.d = wi;
+            continue;
+        }
         if (ml.pbr) ok = sample_pbr(&ml, alb, N, d, V, u1, u2, ul, &wi, &w, &spec);
         else ok = sample_legacy(&ml, alb, N, d, V, h.front, u1, u2, ul, ug, &wi, &w, &refr, &spec);
         if (!ok) break;
+        if (adj) adj_bsdf(adj, &ml, alb, N, V, wi, w, spec, refr);
         last_spec = spec;
         beta = dmul(beta, w);
         if (!dfinite(beta) || !(max_comp(beta) > 0.0)) break;
@@ -637,7 +938,7 @@ static void render_pixel(job_t *j, int x, int y)
                 dv3 pd = dadd(ray.o, dscale(ray.d, j->cam->dome_radius));
                 if (pd.y < s->water_level) ray.o = dsub(pd, d3(0.0, 1e-3, 0.0));
             }
-            c = pt_radiance(s, ray, j->depth, or_hash3((unsigned)x, (unsigned)y, (unsigned)sidx));
+            c = pt_radiance(s, ray, j->depth, or_hash3((unsigned)x, (unsigned)y, (unsigned)sidx), NULL);
         } else {
             double du, dv;
             if (g > 0 && g * g == j->spp) {
@@ -756,6 +1057,148 @@ int or_render(const or_scene *s, const mh_camera *cam, int width, int height, in
 
 void or_render_stats_get(long long out[OR_NSTATS]) { memcpy(out, g_render_stats, sizeof g_render_stats); }
 
+/* ---- adjoint driver ----------------------------------------------------- */
+
+typedef struct {
+    const or_scene *s;
+    const mh_camera *cam;
+    int w, h, spp, depth, s0;
+    const double *dimg;
+    double *grad;        /* shared result, n_mats*8 + 3 */
+    int ng;
+    int tiles_x, tiles_y;
+    atomic_int next;
+    pthread_mutex_t lock;
+} adj_job_t;
+
+static void adj_pixel(adj_job_t *j, int x, int y, or_adj *a)
+{
+    const or_scene *s = j->s;
+    size_t o = ((size_t)y * (size_t)j->w + (size_t)x) * 3;
+    double inv = 1.0 / (double)j->spp;
+    dv3 dp = d3(j->dimg[o] * inv, j->dimg[o + 1] * inv, j->dimg[o + 2] * inv);
+    for (int si = 0; si < j->spp; ++si) {
+        int sidx = j->s0 + si;
+        double jx = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 0));
+        double jy = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 1));
+        double l1 = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 100));
+        double l2 = or_rand01((unsigned)x, (unsigned)y, (unsigned)(sidx * 2 + 101));
+        double uu = ((double)x + jx) / (double)j->w;
+        double vv = 1.0 - ((double)y + jy) / (double)j->h;
+        oray ray = cam_ray(j->cam, uu, vv, l1, l2);
+        if (j->cam->dome_radius > 1e-4 && s->water_material >= 0) {
+            dv3 pd = dadd(ray.o, dscale(ray.d, j->cam->dome_radius));
+            if (pd.y < s->water_level) ray.o = dsub(pd, d3(0.0, 1e-3, 0.0));
+        }
+        unsigned seed = or_hash3((unsigned)x, (unsigned)y, (unsigned)sidx);
+        /* pass 1: the raw (unclamped) sum of the path's terms; the final
+         * clamp of a negative channel to 0 has zero derivative */
+        a->delta = d3(0.0, 0.0, 0.0);
+        a->Lacc = d3(0.0, 0.0, 0.0);
+        (void)pt_radiance(s, ray, j->depth, seed, a);
+        dv3 L = a->Lacc;
+        if (!dfinite(L)) continue;
+        a->Lt = L;
+        a->Lacc = d3(0.0, 0.0, 0.0);
+        a->delta = d3(L.x >= 0.0 ? dp.x : 0.0, L.y >= 0.0 ? dp.y : 0.0, L.z >= 0.0 ? dp.z : 0.0);
+        (void)pt_radiance(s, ray, j->depth, seed, a);
+    }
+}
+
+static void *adj_worker(void *arg)
+{
+    adj_job_t *j = (adj_job_t *)arg;
+    or_adj a;
+    memset(&a, 0, sizeof a);
+    a.n_mats = j->s->n_mats;
+    a.g = (double *)calloc((size_t)j->ng, sizeof(double));
+    if (!a.g) return NULL;
+    int nt = j->tiles_x * j->tiles_y;
+    for (;;) {
+        int t = atomic_fetch_add(&j->next, 1);
+        if (t >= nt) break;
+        int ty = t / j->tiles_x, tx = t - ty * j->tiles_x;
+        int x0 = tx * TILE, y0 = ty * TILE;
+        int x1 = x0 + TILE < j->w ? x0 + TILE : j->w;
+        int y1 = y0 + TILE < j->h ? y0 + TILE : j->h;
+        for (int y = y0; y < y1; ++y)
+            for (int x = x0; x < x1; ++x) adj_pixel(j, x, y, &a);
+    }
+    pthread_mutex_lock(&j->lock);
+    for (int i = 0; i < j->ng; ++i) j->grad[i] += a.g[i];
+    pthread_mutex_unlock(&j->lock);
+    free(a.g);
+    return NULL;
+}
+
+int or_adjoint(const or_scene *s, const mh_camera *cam, int width, int height, int spp, int depth,
+               int sample_offset, const double *dimg, double *grad, int threads)
+{
+    if (!s || !cam || !dimg || !grad) return 1;
+    if (width <= 0 || height <= 0) return 2;
+    if (spp < 1 || depth < 0 || sample_offset < 0) return 3;
+    if (s->freeze) return 4;
+    adj_job_t *j = (adj_job_t *)calloc(1, sizeof(adj_job_t));
+    if (!j) return 4;
+    j->s = s; j->cam = cam; j->w = width; j->h = height; j->spp = spp; j->depth = depth;
+    j->s0 = sample_offset; j->dimg = dimg; j->grad = grad;
+    j->ng = s->n_mats * 8 + 3;
+    memset(grad, 0, sizeof(double) * (size_t)j->ng);
+    j->tiles_x = (width + TILE - 1) / TILE;
+    j->tiles_y = (height + TILE - 1) / TILE;
+    atomic_init(&j->next, 0);
+    pthread_mutex_init(&j->lock, NULL);
+    int nt = threads;
+    if (nt <= 0) {
+        long nc = sysconf(_SC_NPROCESSORS_ONLN);
+        nt = nc >= 1 ? (int)nc : 4;
+    }
+    if (nt > 64) nt = 64;
+    if (nt > j->tiles_x * j->tiles_y) nt = j->tiles_x * j->tiles_y;
+    if (nt < 1) nt = 1;
+    pthread_t tid[64];
+    int started = 0;
+    for (int i = 1; i < nt; ++i)
+        if (pthread_create(&tid[started], NULL, adj_worker, j) == 0) ++started;
+    adj_worker(j);
+    for (int i = 0; i < started; ++i) pthread_join(tid[i], NULL);
+    pthread_mutex_destroy(&j->lock);
+    free(j);
+    return 0;
+}
+
+/* Replace the materials (same count): the emissive-light SET stays the one
+ * built at creation, each light taking its material's new emission. */
+int or_scene_set_materials(or_scene *s, const mh_material *mats, int n)
+{
+    if (!s || !mats) return 1;
+    if (n != s->n_mats) return 5;
+    memcpy(s->mats, mats, sizeof(mh_material) * (size_t)n);
+    for (int i = 0; i < s->n_lights; ++i) s->lights[i].emissive = fromm(mats[s->lights[i].mat].emissive);
+    return 0;
+}
+
+int or_scene_set_sky(or_scene *s, const mh_sky *sky)
+{
+    if (!s || !sky) return 1;
+    s->sky = *sky;
+    return 0;
+}
+
+/* Freeze sampling decisions at these materials (NULL: unfreeze). */
+int or_set_freeze(or_scene *s, const mh_material *mats, int n)
+{
+    if (!s) return 1;
+    free(s->freeze);
+    s->freeze = NULL;
+    if (!mats) return 0;
+    if (n != s->n_mats) return 5;
+    s->freeze = (mh_material *)malloc(sizeof(mh_material) * (size_t)(n > 0 ? n : 1));
+    if (!s->freeze) return 4;
+    memcpy(s->freeze, mats, sizeof(mh_material) * (size_t)n);
+    return 0;
+}
+
 void or_set_anyhit_shadow(int on) { or_anyhit_shadow = on ? 1 : 0; }
 
 void or_stats_reset(void) { memset(&or_tls_stats, 0, sizeof or_tls_stats); }
@@ -824,6 +1267,7 @@ or_scene *or_scene_create(const mh_scene_soa *soa)
         if (!(p->r0 > 0.0)) continue;
         olight *l = &s->lights[s->n_lights++];
         l->prim = i;
+        l->mat = p->material;
         l->center = p->p0;
         l->radius = p->r0;
         l->emissive = fromm(m->emissive);
@@ -844,6 +1288,7 @@ void or_scene_destroy(or_scene *s)
     free(s->order);
     free(s->leaf_prims);
     free(s->planes);
+    free(s->freeze);
     free(s);
 }
 
@@ -894,7 +1339,7 @@ int or_occluded(const or_scene *s, const mh_ray *rays, int n, double tmin, doubl
 int or_radiance(const or_scene *s, const mh_ray *ray, int depth, unsigned seed, double out[3])
 {
     if (!s || !ray) return 1;
-    dv3 c = pt_radiance(s, from_mh(ray), depth, seed);
+    dv3 c = pt_radiance(s, from_mh(ray), depth, seed, NULL);
     out[0] = c.x; out[1] = c.y; out[2] = c.z;
     return 0;
 }

@@ -132,6 +132,22 @@ dv3 or_ambient(const mh_material *m, dv3 N, const mh_sky *sky)
     return dscale(dmul(fromm(m->albedo), hemi), 0.15);
 }
 
+/* Cloud-layer opacity of the planar fBm layer along dir (0 below 0.03). */
+double or_sky_cloud_alpha(dv3 dir, const mh_sky *sky)
+{
+    if (!(dir.y > 0.03)) return 0.0;
+    double tt = sky->cloud_height / dir.y;
+    double px = dir.x * tt, pz = dir.z * tt;
+    int oct = sky->cloud_octaves < 1 ? 1 : sky->cloud_octaves;
+    double raw = or_fbm2(px * sky->cloud_scale, pz * sky->cloud_scale, oct, 2.0, 0.5, sky->seed);
+    double dens = or_clamp01(0.5 * (raw + 1.0));
+    double a = sky->cloud_coverage - sky->cloud_softness;
+    double b = sky->cloud_coverage + sky->cloud_softness;
+    double al = powpos(smooth3(a, b, dens), sky->cloud_sharpness);
+    al *= smooth3(0.0, 0.15, dir.y);
+    return al;
+}
+
 dv3 or_sky_sample(dv3 dir, const mh_sky *sky)
 {
     double h = or_clamp01(dir.y);
@@ -145,15 +161,7 @@ dv3 or_sky_sample(dv3 dir, const mh_sky *sky)
     }
     /* planar cloud layer at cloud_height (volumetric clouds are off) */
     if (dir.y > 0.03) {
-        double tt = sky->cloud_height / dir.y;
-        double px = dir.x * tt, pz = dir.z * tt;
-        int oct = sky->cloud_octaves < 1 ? 1 : sky->cloud_octaves;
-        double raw = or_fbm2(px * sky->cloud_scale, pz * sky->cloud_scale, oct, 2.0, 0.5, sky->seed);
-        double dens = or_clamp01(0.5 * (raw + 1.0));
-        double a = sky->cloud_coverage - sky->cloud_softness;
-        double b = sky->cloud_coverage + sky->cloud_softness;
-        double al = powpos(smooth3(a, b, dens), sky->cloud_sharpness);
-        al *= smooth3(0.0, 0.15, dir.y);
+        double al = or_sky_cloud_alpha(dir, sky);
         if (al > 0.0) {
             double shade = 0.6 + 0.4 * al;
             dv3 cc = dscale(d3(0.95, 0.95, 0.98), shade);

@@ -65,6 +65,30 @@ void or_render_stats_get(long long out[10]);
  * closest-hit walk. */
 void or_set_anyhit_shadow(int on);
 
+/*
+ * Path-replay adjoint of the path tracer in FP64 (the algorithm of the GPU
+ * adjoint): grad[n_mats*8 + 3] = dLoss/dtheta for dimg = dLoss/d(pixel mean)
+ * (w*h*3): per material albedo rgb, roughness, metallic, emissive rgb; then
+ * sun colour rgb. Sampling decisions held fixed. Returns 4 while frozen.
+ */
+int or_adjoint(const or_scene *s, const mh_camera *cam, int width, int height, int spp,
+               int depth, int sample_offset, const double *dimg, double *grad, int threads);
+
+/* Parameter updates: materials (same count; the emissive-light set stays the
+ * one built at creation) and sky. */
+int or_scene_set_materials(or_scene *s, const mh_material *mats, int n);
+int or_scene_set_sky(or_scene *s, const mh_sky *sky);
+
+/*
+ * Frozen sampling: every sampling decision (lobe choice and probability,
+ * GGX half vector and pdf, Russian roulette on a throughput computed from
+ * these materials) uses `mats`, the BSDF values the scene's current
+ * materials. The path-traced image is then a smooth function of the
+ * current material parameters whose derivative at mats is what the adjoint
+ * computes (finite differences of it pin the adjoint exactly). NULL unfreezes.
+ */
+int or_set_freeze(or_scene *s, const mh_material *mats, int n);
+
 /* Pure-function vectors. */
 unsigned or_hash3(unsigned a, unsigned b, unsigned c);
 double or_rand01(unsigned a, unsigned b, unsigned c);

@@ -54,6 +54,11 @@ def oracle():
         _sig(L, "or_stats_get", None, [C.POINTER(C.c_longlong)])
         _sig(L, "or_render_stats_get", None, [C.POINTER(C.c_longlong)])
         _sig(L, "or_set_anyhit_shadow", None, [C.c_int])
+        _sig(L, "or_adjoint", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, P, P, C.c_int])
+        _sig(L, "or_scene_set_materials", C.c_int, [P, P, C.c_int])
+        _sig(L, "or_scene_set_sky", C.c_int, [P, C.POINTER(abi.Sky)])
+        _sig(L, "or_set_freeze", C.c_int, [P, P, C.c_int])
         _sig(L, "or_hash3", C.c_uint, [C.c_uint, C.c_uint, C.c_uint])
         _sig(L, "or_rand01", D, [C.c_uint, C.c_uint, C.c_uint])
         _sig(L, "or_camera_create", None, [DP, DP, DP, D, D, C.POINTER(abi.Camera)])
@@ -176,6 +181,32 @@ class OracleScene:
 
     def set_anyhit_shadow(self, on):
         self.L.or_set_anyhit_shadow(1 if on else 0)
+
+    def adjoint(self, cam, w, h, spp, depth, dimg, n_mats, sample_offset=0, threads=0):
+        """FP64 path-replay adjoint: (n_mats, 8) material gradients, (3,) sun colour."""
+        d = np.ascontiguousarray(dimg, dtype=np.float64)
+        g = np.zeros(n_mats * 8 + 3)
+        rc = self.L.or_adjoint(self.h, C.byref(cam), w, h, spp, depth, sample_offset, abi.ptr(d),
+                               abi.ptr(g), threads)
+        if rc:
+            raise RuntimeError(f"or_adjoint rc {rc}")
+        return g[:n_mats * 8].reshape(n_mats, 8), g[n_mats * 8:]
+
+    def set_materials(self, mats):
+        mats = np.ascontiguousarray(mats, dtype=abi.MATERIAL_DTYPE)
+        assert self.L.or_scene_set_materials(self.h, abi.ptr(mats), len(mats)) == 0
+        self._mats = mats
+
+    def set_sky(self, sky):
+        assert self.L.or_scene_set_sky(self.h, C.byref(sky)) == 0
+
+    def set_freeze(self, mats):
+        if mats is None:
+            assert self.L.or_set_freeze(self.h, None, 0) == 0
+            return
+        mats = np.ascontiguousarray(mats, dtype=abi.MATERIAL_DTYPE)
+        self._freeze = mats
+        assert self.L.or_set_freeze(self.h, abi.ptr(mats), len(mats)) == 0
 
 
 class RefScene:
