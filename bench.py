@@ -103,6 +103,79 @@ def cpu_baseline(o, cam, scale, threads):
     return samples / sec / 1e6, f"{w}x{h} downscaled frame of the same scene/camera, {SPP} spp, depth {DEPTH}", sec
 
 
+def adjoint_leg(reps):
+    """Differentiable workload: 512 x 512, 32 spp, depth 4, 64 materials, L2
+    loss vs the image of hidden seeded parameters; ms per gradient."""
+    import numpy as np
+    from marmel_amd import GpuScene
+    from marmel_amd.workloads import hidden_params, materials_scene
+    hs = materials_scene()
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    g = GpuScene(hs.soa())
+    w = h = 512
+    spp, depth = 32, 4
+    cam = hs.camera(1.0)
+    g.set_materials(hidden_params(mats))
+    _, target = g.render(cam, w, h, spp, depth, sample_offset=1 << 20)
+    g.set_materials(mats)
+    _, lin = g.render(cam, w, h, spp, depth)
+    fwd_ms = g.last_timing().kernel_ms
+    d = (2.0 * (lin.astype(np.float64) - target) / lin.size).astype(np.float32)
+    g.adjoint(cam, w, h, spp, depth, d)  # warm-up
+    ks, t0 = [], time.perf_counter()
+    for _ in range(reps):
+        g.adjoint(cam, w, h, spp, depth, d)
+        ks.append(g.last_timing().kernel_ms)
+    wall = (time.perf_counter() - t0) / reps * 1e3
+    g.close()
+    return {"workload": "512x512, 32 spp, depth 4, 64 PBR materials (513 parameters), L2 loss",
+            "ms_per_grad": round(wall, 2), "kernel_ms_per_grad": round(sum(ks) / len(ks), 2),
+            "forward_kernel_ms": round(fwd_ms, 2)}
+
+
+def batch_leg():
+    """Batched workload: 32 uneven scenes in one persistent wavefront launch
+    sequence (autotuned), against rendering each job alone with the plain
+    GPU kernel and with the single-scene wavefront engine."""
+    from marmel_amd import Batch, GpuScene, abi
+    from marmel_amd.workloads import batch32_scenes, batch32_spec
+    t0 = time.time()
+    spec = batch32_spec()
+    hss = batch32_scenes(spec)
+    gss = [GpuScene(hs.soa()) for hs in hss]
+    jobs = [dict(scene=i, cam=hs.camera(1.0), width=r, height=r, spp=spp, depth=dp)
+            for i, (hs, (_, _, _, r, spp, dp)) in enumerate(zip(hss, spec))]
+    samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    log(f"bench: batch32 scenes built in {time.time() - t0:.1f}s, {samples / 1e6:.1f} Msamples")
+    b = Batch()
+    b.render(gss, jobs, autotune=True, fetch=False)  # tunes once, cached on the handle
+    tune = b.last_tune()
+    t1 = time.perf_counter()
+    b.render(gss, jobs, autotune=True, fetch=False)
+    batched = samples / (time.perf_counter() - t1) / 1e6
+    log(f"bench: batch32 batched {batched:.1f} Msamples/s, tune {tune}")
+    for j in jobs[:2]:  # warm both single-job engines
+        gss[j["scene"]].render(j["cam"], j["width"], j["height"], 1, j["depth"], fetch=False)
+    t1 = time.perf_counter()
+    for j in jobs:
+        gss[j["scene"]].render(j["cam"], j["width"], j["height"], j["spp"], j["depth"], fetch=False)
+    plain = samples / (time.perf_counter() - t1) / 1e6
+    t1 = time.perf_counter()
+    for j in jobs:
+        gss[j["scene"]].render(j["cam"], j["width"], j["height"], j["spp"], j["depth"], fetch=False,
+                               engine=abi.ENGINE_WAVEFRONT, use_graph=True)
+    single_wf = samples / (time.perf_counter() - t1) / 1e6
+    log(f"bench: batch32 plain megakernel {plain:.1f}, single-scene wavefront {single_wf:.1f} Msamples/s")
+    b.close()
+    for gsc in gss:
+        gsc.close()
+    return {"workload": "32 seeded scenes: icospheres 1,280-327,680 triangles + 20-300 spheres, "
+                        "256^2-1024^2, 4-256 spp, depth 4-10",
+            "msamples": round(samples / 1e6, 1), "batched_msamples_per_s": round(batched, 1),
+            "plain_msamples_per_s": round(plain, 1), "single_scene_wavefront_msamples_per_s": round(single_wf, 1),
+            "batched_vs_plain": round(batched / plain, 3), "tune": tune}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +193,9 @@ def main():
                     help="frames of the other engine timed for the batched-vs-plain ratio")
     ap.add_argument("--pool", type=int, default=0, help="wavefront paths per batch (0 = default)")
     ap.add_argument("--grid-blocks", type=int, default=0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint ms/grad leg")
+    ap.add_argument("--adjoint-reps", type=int, default=3)
+    ap.add_argument("--no-batch32", action="store_true", help="skip the 32-scene batched leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -176,6 +252,11 @@ def main():
             log(f"bench: {other} step {i + 1}/{args.plain_steps} {g.last_timing().kernel_ms:.1f} ms")
         plain = samples * args.plain_steps / (time.perf_counter() - t0) / 1e6
 
+    adj = None if args.no_adjoint else adjoint_leg(args.adjoint_reps)
+    if adj:
+        log(f"bench: adjoint {adj}")
+    b32 = None if args.no_batch32 else batch_leg()
+
     # host threads: the GPU's share of the host (16 cores), as the reference
     # caps its pool at min(nproc, 64) (pathtrace.c:1210-1234)
     threads = min(os.cpu_count() or 1, 16)
@@ -217,6 +298,8 @@ def main():
         f"{other}_msamples_per_s": None if plain is None else round(plain, 2),
         "batched_vs_plain": None if plain is None else round(
             (value / plain) if args.engine == "wavefront" else (plain / value), 3),
+        "adjoint": adj,
+        "batch32": b32,
         "roofline": roof,
         "cpu_baseline": cpu,
         "device": name,

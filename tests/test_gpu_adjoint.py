@@ -17,6 +17,7 @@ import torch
 import checkers as ck
 from marmel_amd import GpuScene, HostScene, abi, native
 from marmel_amd.diff import DiffScene, mats_to_vector, vector_to_mats
+from marmel_amd.workloads import hidden_params
 
 pytestmark = pytest.mark.gpu
 
@@ -31,18 +32,6 @@ def mscene():
     hold = ck.SoaHolder(prims, attrs, mats, sky, wl, wm)
     yield hs, gs, ck.OracleScene(hold.soa), mats, sky, hold
     gs.close()
-
-
-def hidden_params(mats, seed=4):
-    """The seeded hidden parameters the target image is rendered from."""
-    rng = np.random.default_rng(seed)
-    v = mats_to_vector(mats).reshape(len(mats), 8)
-    v[:, 0:3] = np.clip(v[:, 0:3] + rng.uniform(-0.15, 0.15, (len(mats), 3)), 0.05, 0.95)
-    v[:, 3] = np.clip(v[:, 3] + rng.uniform(-0.2, 0.2, len(mats)), 0.05, 0.95)
-    v[:, 4] = np.clip(v[:, 4] + rng.uniform(-0.2, 0.2, len(mats)), 0.02, 0.98)
-    em = v[:, 5:8] > 0
-    v[:, 5:8] = np.where(em, v[:, 5:8] * rng.uniform(0.7, 1.3, (len(mats), 3)), 0.0)
-    return vector_to_mats(v.reshape(-1), mats)
 
 
 def flat(gm, gs):
