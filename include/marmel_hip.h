@@ -246,6 +246,9 @@ typedef struct mh_render_params {
     int32_t kernel_clock;   /* wavefront: 1 = record a HIP event pair around every
                                kernel launch (launched directly, not from a graph);
                                per-kernel totals via mh_last_kernel_times          */
+    int32_t bvh_width;      /* 0 = default (4 when the scene's 4-wide tree fits the
+                               traversal stack, else 2), 2 = binary, 4 = 4-wide;
+                               both give identical hits                           */
 } mh_render_params;
 
 /* Per-call timing of the last render on the calling thread, measured with
@@ -383,7 +386,8 @@ typedef struct mh_tune {
     int32_t autotune;      /* 1 = choose pool / grid_blocks / block by timing trial
                               batches of this workload (cached per workload on the handle) */
     int32_t kernel_clock;  /* 1 = per-kernel HIP-event timing (direct launches)          */
-    int32_t bvh_width;     /* BVH node width: 0 or 2 (binary nodes, the one layout built) */
+    int32_t bvh_width;     /* BVH node width: 0 = default (4 when built, else 2), 2, 4;
+                              autotune also tries both widths                        */
     int32_t reserved;
 } mh_tune;
 

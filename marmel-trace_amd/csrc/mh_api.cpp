@@ -400,6 +400,72 @@ int check_tree(const mh_scene_soa* soa, std::vector<int>* order)
     return 0;
 }
 
+// Collapse the binary tree into 4-wide nodes (DNode4): a node adopts its two
+// binary children, then replaces its largest-area interior child by that
+// child's own two children (in place, keeping left-to-right order) until it
+// holds four or only leaves. Nodes are numbered in depth-first pre-order.
+struct Bvh4Builder {
+    const mh_scene_soa* soa;
+    std::vector<DNode4> nodes;
+    int max_depth = 0;
+
+    static int leaf_ref(const mh_bvh_node& nd) { return ~((nd.first << 3) | (nd.count - 1)); }
+    double area(int n) const
+    {
+        const mh_bvh_node& b = soa->nodes[n];
+        double dx = (double)b.bmax[0] - b.bmin[0], dy = (double)b.bmax[1] - b.bmin[1], dz = (double)b.bmax[2] - b.bmin[2];
+        return dx * dy + dy * dz + dz * dx;
+    }
+    int make(int n2, int depth)
+    {
+        if (depth > max_depth) max_depth = depth;
+        const mh_bvh_node& nd = soa->nodes[n2];
+        std::vector<int> c = {nd.left, nd.right};
+        while (c.size() < 4) {
+            int best = -1;
+            double ba = -1.0;
+            for (size_t i = 0; i < c.size(); ++i)
+                if (soa->nodes[c[i]].count == 0 && area(c[i]) > ba) {
+                    ba = area(c[i]);
+                    best = (int)i;
+                }
+            if (best < 0) break;
+            const mh_bvh_node& e = soa->nodes[c[best]];
+            c[best] = e.left;
+            c.insert(c.begin() + best + 1, e.right);
+        }
+        int idx = (int)nodes.size();
+        nodes.push_back(DNode4{});
+        float lo[3][4], hi[3][4];
+        int ref[4];
+        for (int k = 0; k < 4; ++k) {
+            for (int a = 0; a < 3; ++a) {
+                lo[a][k] = 0.f;
+                hi[a][k] = 0.f;
+            }
+            ref[k] = INT_MIN;
+        }
+        for (size_t k = 0; k < c.size(); ++k) {
+            const mh_bvh_node& ch = soa->nodes[c[k]];
+            for (int a = 0; a < 3; ++a) {
+                lo[a][k] = ch.bmin[a];
+                hi[a][k] = ch.bmax[a];
+            }
+            ref[k] = ch.count > 0 ? leaf_ref(ch) : make(c[k], depth + 1);
+        }
+        DNode4& o = nodes[idx];
+        o.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+        o.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+        o.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+        o.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+        o.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+        o.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        o.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+        o.meta = make_int4((int)c.size(), 0, 0, 0);
+        return idx;
+    }
+};
+
 template <typename R> CamView<R> conv_cam(const mh_camera& c)
 {
     CamView<R> v;
@@ -540,6 +606,22 @@ int any_transmissive(const mh_material* mats, int n)
 {
     for (int i = 0; i < n; ++i)
         if (mats[i].is_water || mats[i].transparency > 0.8) return 1;
+    return 0;
+}
+
+// The scene's views with the requested BVH width (0 = default).
+template <typename R> SceneView<R> with_width(const SceneView<R>& v, int width)
+{
+    SceneView<R> o = v;
+    if (width == 2) o.bvh4 = 0;
+    return o;
+}
+
+int check_width(int w, const SceneView<float>& v)
+{
+    if (w != 0 && w != 2 && w != 4) return fail(4, "BVH width %d is not built (2 or 4)", w);
+    if (w == 4 && !v.bvh4 && v.nodes4 == nullptr && v.root_ref != INT_MIN && v.n_nodes > 0)
+        return fail(4, "this scene has no 4-wide tree (too deep for the traversal stack)");
     return 0;
 }
 
@@ -783,6 +865,15 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         nodes[i] = d;
     }
     int root_ref = soa->n_nodes > 0 ? ref_of(0) : INT_MIN;
+    Bvh4Builder b4{soa, {}, 0};
+    int root4 = root_ref;
+    if (soa->n_nodes > 0 && soa->nodes[0].count == 0) root4 = b4.make(0, 1);
+    // 4-wide traversal pushes up to 3 entries per level
+    bool use4 = soa->n_nodes > 0 && 3 * b4.max_depth + 1 <= MH_STACK;
+    if (!use4) {
+        b4.nodes.clear();
+        root4 = root_ref;
+    }
 
     bool any_attr = false;
     if (soa->attrs)
@@ -823,8 +914,9 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         m64[i] = conv_mat<double>(soa->materials[i]);
     }
 
-    void *d_nodes, *d_p32, *d_p64, *d_pl32, *d_pl64, *d_plidx, *d_a32, *d_a64, *d_m32, *d_m64;
+    void *d_nodes, *d_nodes4, *d_p32, *d_p64, *d_pl32, *d_pl64, *d_plidx, *d_a32, *d_a64, *d_m32, *d_m64;
     if ((rc = upload(s, nodes.data(), nodes.size() * sizeof(DNode), &d_nodes)) ||
+        (rc = upload(s, b4.nodes.data(), b4.nodes.size() * sizeof(DNode4), &d_nodes4)) ||
         (rc = upload(s, p32.data(), p32.size() * sizeof(P32), &d_p32)) ||
         (rc = upload(s, p64.data(), p64.size() * sizeof(P64), &d_p64)) ||
         (rc = upload(s, pl32.data(), pl32.size() * sizeof(P32), &d_pl32)) ||
@@ -856,6 +948,9 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         v.nodes = (const DNode*)d_nodes;
         v.n_nodes = (int)nodes.size();
         v.root_ref = root_ref;
+        v.nodes4 = (const DNode4*)d_nodes4;
+        v.root4 = root4;
+        v.bvh4 = use4 && !b4.nodes.empty() ? 1 : 0;
         v.prims = (const typename PrimT<R>::type*)pr;
         v.planes = (const typename PrimT<R>::type*)pl;
         v.n_planes = soa->n_planes;
@@ -1078,6 +1173,9 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
     if ((long long)p->width * p->height > (1LL << 28)) return fail(2, "image too large");
     if (p->mode != MH_MODE_PATHTRACE && p->mode != MH_MODE_WHITTED) return fail(5, "unknown mode %d", p->mode);
     if (p->sample_offset < 0) return fail(3, "negative sample offset");
+    if (int wrc = check_width(p->bvh_width, s->v32)) return wrc;
+    SceneView<float> v32 = with_width(s->v32, p->bvh_width);
+    SceneView<double> v64 = with_width(s->v64, p->bvh_width);
     int rc = ensure_fb(s, p->width, p->height);
     if (rc) return rc;
     LaunchOut lo{s->fb_linear, s->fb_rgb, s->d_rays, s->d_overflow};
@@ -1101,19 +1199,20 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
         if (p->kernel_clock) {
             HIP_TRY(s->clock.start());
             for (long long b = 0; b < nb; ++b)
-                HIP_TRY(wf_enqueue_batch(s->v32, cvw, job, s->ws, s->d_rays, grid, s->stream, &s->clock));
+                HIP_TRY(wf_enqueue_batch(v32, cvw, job, s->ws, s->d_rays, grid, s->stream, &s->clock));
         } else if (p->use_graph) {
-            std::vector<unsigned char> key(sizeof job + sizeof cvw + sizeof grid);
+            int gk[2] = {grid, v32.bvh4};
+            std::vector<unsigned char> key(sizeof job + sizeof cvw + sizeof gk);
             memcpy(key.data(), &job, sizeof job);
             memcpy(key.data() + sizeof job, &cvw, sizeof cvw);
-            memcpy(key.data() + sizeof job + sizeof cvw, &grid, sizeof grid);
+            memcpy(key.data() + sizeof job + sizeof cvw, gk, sizeof gk);
             if (!s->graph || key != s->graph_key) {
                 if (s->graph) (void)hipGraphExecDestroy(s->graph);
                 if (s->graph_def) (void)hipGraphDestroy(s->graph_def);
                 s->graph = nullptr;
                 s->graph_def = nullptr;
                 HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-                hipError_t ce = wf_enqueue_batch(s->v32, cvw, job, s->ws, s->d_rays, grid, s->stream);
+                hipError_t ce = wf_enqueue_batch(v32, cvw, job, s->ws, s->d_rays, grid, s->stream);
                 hipError_t ee = hipStreamEndCapture(s->stream, &s->graph_def);
                 HIP_TRY(ce);
                 HIP_TRY(ee);
@@ -1123,15 +1222,15 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
             for (long long b = 0; b < nb; ++b) HIP_TRY(hipGraphLaunch(s->graph, s->stream));
         } else {
             for (long long b = 0; b < nb; ++b)
-                HIP_TRY(wf_enqueue_batch(s->v32, cvw, job, s->ws, s->d_rays, grid, s->stream));
+                HIP_TRY(wf_enqueue_batch(v32, cvw, job, s->ws, s->d_rays, grid, s->stream));
         }
         HIP_TRY(wf_enqueue_end(s->ws, npix, p->spp, lo, s->stream));
     } else if (p->mode == MH_MODE_PATHTRACE) {
         if (f64)
-            HIP_TRY(launch_pathtrace<double>(s->v64, conv_cam<double>(*cam), p->width, p->height, p->spp,
+            HIP_TRY(launch_pathtrace<double>(v64, conv_cam<double>(*cam), p->width, p->height, p->spp,
                                              p->sample_offset, p->max_depth, lo, 256, s->stream));
         else
-            HIP_TRY(launch_pathtrace<float>(s->v32, conv_cam<float>(*cam), p->width, p->height, p->spp,
+            HIP_TRY(launch_pathtrace<float>(v32, conv_cam<float>(*cam), p->width, p->height, p->spp,
                                             p->sample_offset, p->max_depth, lo, 256, s->stream));
     } else {
         // explicit Whitted work stack: 17 pending rays per level suffices
@@ -1149,10 +1248,10 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
             s->wstack_bytes = need;
         }
         if (f64)
-            HIP_TRY(launch_whitted<double>(s->v64, conv_cam<double>(*cam), p->width, p->height, p->spp,
+            HIP_TRY(launch_whitted<double>(v64, conv_cam<double>(*cam), p->width, p->height, p->spp,
                                            p->sample_offset, p->max_depth, lo, s->wstack, cap, threads, s->stream));
         else
-            HIP_TRY(launch_whitted<float>(s->v32, conv_cam<float>(*cam), p->width, p->height, p->spp,
+            HIP_TRY(launch_whitted<float>(v32, conv_cam<float>(*cam), p->width, p->height, p->spp,
                                           p->sample_offset, p->max_depth, lo, s->wstack, cap, threads, s->stream));
     }
     HIP_TRY(hipEventRecord(s->ev[1], s->stream));
@@ -1273,16 +1372,20 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
         if (!scenes[i]) return fail(1, "NULL scene %d", i);
     mh_tune t{};
     if (tune) t = *tune;
-    if (t.bvh_width != 0 && t.bvh_width != 2) return fail(4, "BVH node width %d is not built (binary only)", t.bvh_width);
+    if (t.bvh_width != 0 && t.bvh_width != 2 && t.bvh_width != 4)
+        return fail(4, "BVH node width %d is not built (2 or 4)", t.bvh_width);
     if (t.pool < 0 || t.grid_blocks < 0 || (t.block != 0 && t.block != 64 && t.block != 128 && t.block != 256))
         return fail(4, "bad launch settings (pool %d, grid %d, block %d)", t.pool, t.grid_blocks, t.block);
 
     std::vector<SceneView<float>> views((size_t)n_scenes);
     bool trans = false;
-    for (int i = 0; i < n_scenes; ++i) {
-        views[i] = scenes[i]->v32;
-        if (views[i].has_transmissive) trans = true;
-    }
+    for (int i = 0; i < n_scenes; ++i)
+        if (scenes[i]->v32.has_transmissive) trans = true;
+    // width 4: every scene whose 4-wide tree was built; width 2: binary
+    auto set_width = [&](int w) {
+        for (int i = 0; i < n_scenes; ++i) views[i] = with_width(scenes[i]->v32, w == 2 ? 2 : 0);
+    };
+    set_width(t.bvh_width);
     std::vector<BJob> bj((size_t)n_jobs);
     long long begin = 0, pix = 0;
     int max_depth = 0;
@@ -1359,15 +1462,38 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
                 t.pool = kv.second.pool;
                 t.grid_blocks = kv.second.grid_blocks;
                 t.block = kv.second.block;
+                t.bvh_width = kv.second.bvh_width;
                 hit = true;
             }
         if (!hit) {
             rc = autotune(b, mj, max_depth, trans, &t);
             if (rc) return rc;
+            // node width: time both layouts at the chosen launch settings
+            if (t.bvh_width == 0) {
+                long long trial = std::max<long long>(std::min<long long>(mj.total, 1LL << 23), t.pool);
+                double best = -1.0;
+                int wbest = 4;
+                for (int w : {2, 4}) {
+                    set_width(w);
+                    HIP_TRY(hipMemcpyAsync(b->tab, views.data(), views.size() * sizeof(SceneView<float>),
+                                           hipMemcpyHostToDevice, b->stream));
+                    double r;
+                    rc = batch_trial(b, mj, max_depth, trans, t, trial, &r);
+                    if (rc) return rc;
+                    if (r > best) {
+                        best = r;
+                        wbest = w;
+                    }
+                }
+                t.bvh_width = wbest;
+            }
             b->tuned.push_back({key, t});
         }
+        set_width(t.bvh_width);
+        HIP_TRY(hipMemcpyAsync(b->tab, views.data(), views.size() * sizeof(SceneView<float>), hipMemcpyHostToDevice,
+                               b->stream));
     }
-    t.bvh_width = 2;
+    if (t.bvh_width == 0) t.bvh_width = 4;
     rc = ensure_batch_state(b, t.pool);
     if (rc) return rc;
 

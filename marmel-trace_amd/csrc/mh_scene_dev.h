@@ -18,6 +18,12 @@
 namespace mh {
 
 enum { K_SPHERE = 0, K_PLANE = 1, K_BOX = 2, K_TRIANGLE = 3, K_CYLINDER = 4 };
+
+// Traversal stack entries per lane: the host accepts binary trees of depth
+// <= 60 and uses a 4-wide tree only when 3 * its depth + 1 fits.
+#ifndef MH_STACK
+#define MH_STACK 64
+#endif
 enum { F_SMOOTH = 1, F_UV = 2 };
 
 struct __attribute__((aligned(16))) DNode {
@@ -25,6 +31,16 @@ struct __attribute__((aligned(16))) DNode {
     float4 c1; // left  (minz, maxz) right (minx, maxx)
     float4 c2; // right (miny, maxy, minz, maxz)
     int4 c3;   // (left ref, right ref, -, -)
+};
+
+// BVH4 node, 128 B: the boxes of up to four children in SoA form (child k in
+// component k of each float4) + their refs (same encoding as DNode) and the
+// child count. Collapsed on the host from the binary tree; the children of a
+// node keep the binary tree's left-to-right order.
+struct __attribute__((aligned(16))) DNode4 {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 ref;
+    int4 meta;  // x = child count (2..4)
 };
 
 // fp32 record: a.w = kind | flags << 8, b.w = prim index, c.w = material
@@ -87,6 +103,9 @@ template <typename R> struct SceneView {
     const DNode* nodes;
     int n_nodes;
     int root_ref;                  // ref of the tree root, or -2^31 when no tree
+    const DNode4* nodes4;          // the same tree collapsed to 4-wide nodes
+    int root4;                     // its root ref (a leaf ref when the tree is one leaf)
+    int bvh4;                      // traverse nodes4 (1) or nodes (0)
     const typename PrimT<R>::type* prims;
     const typename PrimT<R>::type* planes;
     int n_planes;

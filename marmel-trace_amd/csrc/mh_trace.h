@@ -451,9 +451,6 @@ __device__ __forceinline__ bool occ_prim(const PD<R>& p, V3<R> o, V3<R> d, R tmi
 
 // ---- BVH2 traversal -------------------------------------------------------------
 
-#ifndef MH_STACK
-#define MH_STACK 64
-#endif
 
 // Traversal stacks. A stack policy is a small per-lane handle type passed by
 // value down to the traversal routines; each traversal builds its own
@@ -597,6 +594,18 @@ template <> struct Slab<float> {
         *tl = len;
         *tr = ren;
     }
+    // one box (x0, x1, y0, y1, z0, z1 = min / max per axis)
+    __device__ __forceinline__ void one(float x0, float x1, float y0, float y1, float z0, float z1, float tmin,
+                                        float tmax, bool* hit, float* te) const
+    {
+        float ax = fmaf(x0, ix, -ox), bx = fmaf(x1, ix, -ox);
+        float ay = fmaf(y0, iy, -oy), by = fmaf(y1, iy, -oy);
+        float az = fmaf(z0, iz, -oz), bz = fmaf(z1, iz, -oz);
+        float en = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+        float ex = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
+        *hit = en <= ex * 1.0000004f;
+        *te = en;
+    }
 };
 
 template <> struct Slab<double> {
@@ -629,6 +638,55 @@ template <> struct Slab<double> {
     }
 };
 
+// Whether 4-wide any-hit traversal visits children nearest first (1, the
+// default). The unsorted variant (0) measured faster in the megakernel but a
+// build with it produced wrong fp64 path-traced images, without a cause found;
+// it stays a tuning switch only.
+#ifndef MH_ANY_SORT
+#define MH_ANY_SORT 1
+#endif
+
+// The hit children of a BVH4 node, nearest first: refs r[0..*nh) with entry
+// distances t[0..*nh).
+template <typename R>
+__device__ __forceinline__ void children4(const Slab<R>& sl, const DNode4& n, R tmin, R tmax, int r[4], float t[4],
+                                          int* nh)
+{
+    const float inf = __builtin_inff();
+    int cnt = n.meta.x;
+    bool h0, h1, h2, h3;
+    float t0, t1, t2, t3;
+    sl.one(n.lox.x, n.hix.x, n.loy.x, n.hiy.x, n.loz.x, n.hiz.x, tmin, tmax, &h0, &t0);
+    sl.one(n.lox.y, n.hix.y, n.loy.y, n.hiy.y, n.loz.y, n.hiz.y, tmin, tmax, &h1, &t1);
+    sl.one(n.lox.z, n.hix.z, n.loy.z, n.hiy.z, n.loz.z, n.hiz.z, tmin, tmax, &h2, &t2);
+    sl.one(n.lox.w, n.hix.w, n.loy.w, n.hiy.w, n.loz.w, n.hiz.w, tmin, tmax, &h3, &t3);
+    float k[4] = {h0 ? t0 : inf, h1 ? t1 : inf, (h2 && cnt > 2) ? t2 : inf, (h3 && cnt > 3) ? t3 : inf};
+    int q[4] = {n.ref.x, n.ref.y, n.ref.z, n.ref.w};
+    // sorting network (0,1) (2,3) (0,2) (1,3) (1,2): ascending entry distance
+#define MH_CSWAP(a, b)                                                                                    \
+    if (k[b] < k[a]) {                                                                                    \
+        float tk = k[a];                                                                                  \
+        k[a] = k[b];                                                                                      \
+        k[b] = tk;                                                                                        \
+        int tq = q[a];                                                                                    \
+        q[a] = q[b];                                                                                      \
+        q[b] = tq;                                                                                        \
+    }
+    MH_CSWAP(0, 1)
+    MH_CSWAP(2, 3)
+    MH_CSWAP(0, 2)
+    MH_CSWAP(1, 3)
+    MH_CSWAP(1, 2)
+#undef MH_CSWAP
+    int c = 0;
+    for (int i = 0; i < 4; ++i) {
+        r[i] = q[i];
+        t[i] = k[i];
+        c += k[i] < inf ? 1 : 0;
+    }
+    *nh = c;
+}
+
 // Closest hit. Returns the leaf-record index of the winner (or -1 - plane
 // slot for a plane, or INT_MIN for a miss) and its t / prim index.
 template <typename R, typename TS>
@@ -650,7 +708,40 @@ __device__ int closest_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tm
             }
         }
     }
-    if (sc.root_ref != INT_MIN) {
+    if (sc.root_ref != INT_MIN && sc.bvh4) {
+        Slab<R> sl;
+        sl.init(o, d);
+        typename TS::template Lane<true> st(ts);
+        st.push(sc.root4, (float)tmin);
+        while (!st.empty()) {
+            float te;
+            int ref = st.pop(&te);
+            if (best != INT_MIN && (R)te > closest) continue;
+            if (ref >= 0) {
+                DNode4 n = sc.nodes4[ref];
+                int r[4], nh;
+                float t[4];
+                children4(sl, n, tmin, closest, r, t, &nh);
+                // push the farthest first: the nearest is popped next
+                for (int i = 3; i >= 0; --i)
+                    if (i < nh) st.push(r[i], t[i]);
+            } else {
+                int code = ~ref;
+                int first = code >> 3, cnt = (code & 7) + 1;
+                for (int i = 0; i < cnt; ++i) {
+                    PD<R> p = load_prim(&sc.prims[first + i]);
+                    R t;
+                    if (hit_prim<R, false>(p, nullptr, o, d, tmin, closest, &t, nullptr)) {
+                        if (best == INT_MIN || t < closest || (t == closest && p.prim < bprim)) {
+                            closest = t;
+                            bprim = p.prim;
+                            best = first + i;
+                        }
+                    }
+                }
+            }
+        }
+    } else if (sc.root_ref != INT_MIN) {
         Slab<R> sl;
         sl.init(o, d);
         typename TS::template Lane<true> st(ts);
@@ -724,7 +815,9 @@ __device__ __forceinline__ bool intersect(const SceneView<R>& sc, TS ts, V3<R> o
     return true;
 }
 
-template <typename R, typename TS>
+// W = 0: the scene's tree width (sc.bvh4); W = 2: always the binary tree
+// (kernels at full occupancy whose per-node work dominates).
+template <typename R, typename TS, int W = 0>
 __device__ bool any_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin, R tmax)
 {
     for (int i = 0; i < sc.n_planes; ++i) {
@@ -735,6 +828,44 @@ __device__ bool any_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin,
     Slab<R> sl;
     sl.init(o, d);
     typename TS::template Lane<false> st(ts);
+    if (W != 2 && sc.bvh4) {
+        st.push(sc.root4, 0.f);
+        while (!st.empty()) {
+            float te;
+            int ref = st.pop(&te);
+            if (ref >= 0) {
+                DNode4 n = sc.nodes4[ref];
+#if MH_ANY_SORT
+                int r[4], nh;
+                float t[4];
+                children4(sl, n, tmin, tmax, r, t, &nh);
+                for (int i = 3; i >= 0; --i)
+                    if (i < nh) st.push(r[i], 0.f);
+#else
+                // any hit ends the query: visit order only affects speed
+                bool h0, h1, h2, h3;
+                float t0, t1, t2, t3;
+                int cnt = n.meta.x;
+                sl.one(n.lox.w, n.hix.w, n.loy.w, n.hiy.w, n.loz.w, n.hiz.w, tmin, tmax, &h3, &t3);
+                sl.one(n.lox.z, n.hix.z, n.loy.z, n.hiy.z, n.loz.z, n.hiz.z, tmin, tmax, &h2, &t2);
+                sl.one(n.lox.y, n.hix.y, n.loy.y, n.hiy.y, n.loz.y, n.hiz.y, tmin, tmax, &h1, &t1);
+                sl.one(n.lox.x, n.hix.x, n.loy.x, n.hiy.x, n.loz.x, n.hiz.x, tmin, tmax, &h0, &t0);
+                if (h3 && cnt > 3) st.push(n.ref.w, 0.f);
+                if (h2 && cnt > 2) st.push(n.ref.z, 0.f);
+                if (h1) st.push(n.ref.y, 0.f);
+                if (h0) st.push(n.ref.x, 0.f);
+#endif
+            } else {
+                int code = ~ref;
+                int first = code >> 3, cnt = (code & 7) + 1;
+                for (int i = 0; i < cnt; ++i) {
+                    PD<R> p = load_prim(&sc.prims[first + i]);
+                    if (occ_prim(p, o, d, tmin, tmax)) return true;
+                }
+            }
+        }
+        return false;
+    }
     st.push(sc.root_ref, 0.f);
     while (!st.empty()) {
         float te;

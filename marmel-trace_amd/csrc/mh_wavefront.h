@@ -381,7 +381,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
             }
         } else {
             ++nq;
-            occ = any_hit(sc, tsa, o, unit_dir(d), 1e-3f, a.w);
+            occ = any_hit<float, decltype(tsa), 2>(sc, tsa, o, unit_dir(d), 1e-3f, a.w);
         }
         if (!occ) atomicOr(&st.vis[slot], 1u << k);
     }

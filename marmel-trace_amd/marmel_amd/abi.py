@@ -101,7 +101,7 @@ ENGINE_MEGAKERNEL, ENGINE_WAVEFRONT = 0, 1
 class RenderParams(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("width", "height", "spp", "max_depth", "mode",
                                           "precision", "sample_offset", "engine", "pool",
-                                          "grid_blocks", "use_graph", "kernel_clock")]
+                                          "grid_blocks", "use_graph", "kernel_clock", "bvh_width")]
 
 
 class Job(C.Structure):

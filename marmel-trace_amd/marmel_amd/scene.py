@@ -194,11 +194,11 @@ class GpuScene:
     def render(self, cam, width, height, spp, depth, mode=abi.MODE_PATHTRACE,
                precision=abi.PREC_FP32, sample_offset=0, fetch=True,
                engine=abi.ENGINE_MEGAKERNEL, pool=0, grid_blocks=0, use_graph=False,
-               kernel_clock=False):
+               kernel_clock=False, bvh_width=0):
         p = abi.RenderParams(width=width, height=height, spp=spp, max_depth=depth, mode=mode,
                              precision=precision, sample_offset=sample_offset, engine=engine,
                              pool=pool, grid_blocks=grid_blocks, use_graph=int(use_graph),
-                             kernel_clock=int(kernel_clock))
+                             kernel_clock=int(kernel_clock), bvh_width=bvh_width)
         rgb = lin = None
         if fetch:
             rgb = np.zeros((height, width, 3), np.uint8)
@@ -306,7 +306,7 @@ class Batch:
             pass
 
     def render(self, scenes, jobs, pool=0, grid_blocks=0, block=0, use_graph=True, autotune=False,
-               kernel_clock=False, fetch=True):
+               kernel_clock=False, fetch=True, bvh_width=0):
         """scenes: list of GpuScene; jobs: list of dicts (scene, cam, width, height, spp, depth,
         sample_offset). Returns [(rgb, linear)] per job (None when fetch is False)."""
         arr = (C.c_void_p * len(scenes))(*[s._h.value for s in scenes])
@@ -317,7 +317,7 @@ class Batch:
                             max_depth=j["depth"], sample_offset=j.get("sample_offset", 0), cam=j["cam"])
             npx.append(max(j["width"], 0) * max(j["height"], 0))
         t = abi.Tune(pool=pool, grid_blocks=grid_blocks, block=block, use_graph=int(use_graph),
-                     autotune=int(autotune), kernel_clock=int(kernel_clock))
+                     autotune=int(autotune), kernel_clock=int(kernel_clock), bvh_width=bvh_width)
         tot = max(sum(npx), 1)
         rgb = np.zeros(tot * 3, np.uint8) if fetch else None
         lin = np.zeros(tot * 3, np.float32) if fetch else None

@@ -98,7 +98,7 @@ def test_batch_autotune_keeps_the_image(world):
     print("autotuned", t)
     assert t["pool"] in (1 << 19, 1 << 20, 1 << 21, 1 << 22)
     assert t["grid_blocks"] in (1024, 2048, 4096) and t["block"] in (128, 256)
-    assert t["bvh_width"] == 2
+    assert t["bvh_width"] in (2, 4)
     for (r0, l0), (r1, l1) in zip(ref, res):
         assert np.array_equal(l0, l1)
     res2 = b.render(scenes, jobs, autotune=True)  # cached choice

@@ -301,3 +301,38 @@ def test_wavefront_matches_oracle_statistics():
     assert abs(z.mean()) < 0.1
     assert (z * z).mean() < 1.6
     gs.close()
+
+
+# ---- BVH node width --------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", list(HOST))
+def test_bvh_width_gives_identical_images(name):
+    """The 4-wide tree is a regrouping of the binary one; with strict culling
+    and the lowest-index tie rule the nearest hit does not depend on the
+    visit order, so every image is bit-identical across widths."""
+    hs, gs, o = host_scene(name)
+    w, h = 80, 60
+    cam = hs.camera(w / h)
+    for kw in (dict(precision=FP64), dict(precision=FP32), dict(precision=FP32, engine=WF, pool=2000)):
+        r2, l2 = gs.render(cam, w, h, 4, 6, bvh_width=2, **kw)
+        r4, l4 = gs.render(cam, w, h, 4, 6, bvh_width=4, **kw)
+        assert np.array_equal(l2, l4), kw
+        assert np.array_equal(r2, r4), kw
+    rgb, _ = gs.render(cam, w, h, 2, 3, mode=abi.MODE_WHITTED, precision=FP64, bvh_width=2)
+    rgb4, _ = gs.render(cam, w, h, 2, 3, mode=abi.MODE_WHITTED, precision=FP64, bvh_width=4)
+    assert np.array_equal(rgb, rgb4)
+    gs.close()
+
+
+def test_megakernels_are_deterministic():
+    """Run to run, every engine and precision gives the same bits (a read of
+    stale or uninitialised memory would show here)."""
+    hs, gs, o = host_scene("terrain")
+    w, h = 120, 68
+    cam = hs.camera(w / h)
+    for kw in (dict(precision=FP64), dict(precision=FP32), dict(precision=FP32, engine=WF)):
+        a = gs.render(cam, w, h, 2, 8, **kw)[1]
+        b = gs.render(cam, w, h, 2, 8, **kw)[1]
+        c = gs.render(cam, w, h, 2, 8, **kw)[1]
+        assert np.array_equal(a, b) and np.array_equal(a, c), kw
+    gs.close()
