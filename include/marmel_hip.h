@@ -388,7 +388,9 @@ typedef struct mh_tune {
     int32_t kernel_clock;  /* 1 = per-kernel HIP-event timing (direct launches)          */
     int32_t bvh_width;     /* BVH node width: 0 = default (4 when built, else 2), 2, 4;
                               autotune also tries both widths                        */
-    int32_t reserved;
+    int32_t fixed_batches; /* 0 (default): path regeneration, every bounce step refills
+                              the slots of ended paths with new ones; 1: batches of
+                              `pool` paths run to completion one after another     */
 } mh_tune;
 
 struct mh_batch;

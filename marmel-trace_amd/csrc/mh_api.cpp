@@ -527,7 +527,7 @@ int alloc_wf(int pool, size_t npix, bool jid, void** mem, size_t* bytes, WfState
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t sz[] = {al(P * 16), al(P * 16), al(P * 16), al(P * 16), al(P * 8), al(P * 9 * 16), al(P * 4),
                    al(P * 4),  al(P * 4),  al(P * 4),  al(P * 4),  al(P * 9 * 16), al(P * 9 * 16), al(64),
-                   al(npix * 3 * 4), al(64), jid ? al(P * 4) : 0};
+                   al(npix * 3 * 4), al(64), jid ? al(P * 4) : 0, jid ? al(P * 4) : 0};
     const int nbuf = (int)(sizeof sz / sizeof sz[0]);
     size_t total = 0;
     for (size_t v : sz) total += v;
@@ -556,6 +556,7 @@ int alloc_wf(int pool, size_t npix, bool jid, void** mem, size_t* bytes, WfState
     w->acc = (float*)ptrs[14];
     w->batch = (unsigned long long*)ptrs[15];
     w->jid = (uint32_t*)ptrs[16];
+    w->freeq = (uint32_t*)ptrs[17];
     return 0;
 }
 
@@ -657,6 +658,10 @@ struct mh_batch {
     EventClock clock;
     std::vector<std::pair<std::vector<long long>, mh_tune>> tuned;  // autotune cache
     mh_tune last{};
+    unsigned int* poll = nullptr;   // pinned: queue length + cursor, read between graph replays
+    hipGraphExec_t sgraph = nullptr;  // two captured regeneration steps
+    hipGraph_t sgraph_def = nullptr;
+    std::vector<unsigned char> sgraph_key;
 };
 
 namespace {
@@ -668,6 +673,68 @@ void drop_graph(mh_batch* b)
     b->graph = nullptr;
     b->graph_def = nullptr;
     b->graph_key.clear();
+    if (b->sgraph) (void)hipGraphExecDestroy(b->sgraph);
+    if (b->sgraph_def) (void)hipGraphDestroy(b->sgraph_def);
+    b->sgraph = nullptr;
+    b->sgraph_def = nullptr;
+    b->sgraph_key.clear();
+}
+
+// Path regeneration: steps until every path of the workload (or the first
+// `limit` ones) has ended. The host replays pairs of steps (the queues
+// ping-pong) and between rounds reads the queue length and cursor back.
+int stream_run(mh_batch* b, WfMulti mj, bool trans, const mh_tune& t, long long limit)
+{
+    if (limit >= 0 && limit < mj.total) mj.total = limit;
+    mj.stream = 1;
+    HIP_TRY(wf_stream_init(b->ws, t.pool, b->stream));
+    long long min_pairs = (mj.total + 2LL * t.pool - 1) / (2LL * t.pool);
+    long long round = min_pairs < 1 ? 1 : min_pairs;
+    bool graph = t.use_graph && !t.kernel_clock;
+    if (graph) {
+        std::vector<unsigned char> key(sizeof mj + sizeof b->ws + 4 * sizeof(int));
+        int extra[4] = {t.grid_blocks, t.block, trans ? 1 : 0, t.pool};
+        memcpy(key.data(), &mj, sizeof mj);
+        memcpy(key.data() + sizeof mj, &b->ws, sizeof b->ws);
+        memcpy(key.data() + sizeof mj + sizeof b->ws, extra, sizeof extra);
+        if (!b->sgraph || key != b->sgraph_key) {
+            if (b->sgraph) (void)hipGraphExecDestroy(b->sgraph);
+            if (b->sgraph_def) (void)hipGraphDestroy(b->sgraph_def);
+            b->sgraph = nullptr;
+            b->sgraph_def = nullptr;
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = wf_stream_step(mj, b->ws, 0, b->d_rays, t.grid_blocks, t.block, trans, b->stream);
+            if (ce == hipSuccess)
+                ce = wf_stream_step(mj, b->ws, 1, b->d_rays, t.grid_blocks, t.block, trans, b->stream);
+            hipError_t ee = hipStreamEndCapture(b->stream, &b->sgraph_def);
+            HIP_TRY(ce);
+            HIP_TRY(ee);
+            HIP_TRY(hipGraphInstantiate(&b->sgraph, b->sgraph_def, nullptr, nullptr, 0));
+            b->sgraph_key = key;
+        }
+    }
+    if (t.kernel_clock) HIP_TRY(b->clock.start());
+    for (int guard = 0; guard < 1 << 20; ++guard) {
+        for (long long i = 0; i < round; ++i) {
+            if (graph) {
+                HIP_TRY(hipGraphLaunch(b->sgraph, b->stream));
+            } else {
+                KernelClock* kc = t.kernel_clock ? &b->clock : nullptr;
+                HIP_TRY(wf_stream_step(mj, b->ws, 0, b->d_rays, t.grid_blocks, t.block, trans, b->stream, kc));
+                HIP_TRY(wf_stream_step(mj, b->ws, 1, b->d_rays, t.grid_blocks, t.block, trans, b->stream, kc));
+            }
+        }
+        // paths queued for the next step (cnt[0]) and the cursor
+        HIP_TRY(hipMemcpyAsync(b->poll, b->ws.cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->poll + 2, b->ws.batch, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        unsigned long long cursor;
+        memcpy(&cursor, b->poll + 2, sizeof cursor);
+        if (b->poll[0] == 0 && (long long)cursor >= mj.total) return 0;
+        round = 4;
+    }
+    return fail(6, "path regeneration did not finish");
 }
 
 template <typename T> int grow(T** p, size_t* have, size_t need, size_t elem)
@@ -684,6 +751,7 @@ template <typename T> int grow(T** p, size_t* have, size_t need, size_t elem)
 // Run the batches of one render: `limit` paths (all when < 0) with settings t.
 int batch_run(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, const mh_tune& t, long long limit)
 {
+    if (!t.fixed_batches) return stream_run(b, mj, trans, t, limit);  // path regeneration (default)
     long long total = limit >= 0 && limit < mj.total ? limit : mj.total;
     long long nb = (total + t.pool - 1) / t.pool;
     if (t.kernel_clock) {
@@ -1338,6 +1406,11 @@ int mh_batch_create(mh_batch** out)
             mh_batch_destroy(b);
             return fail(101, "hipEventCreate failed");
         }
+    if (hipHostMalloc((void**)&b->poll, 16, hipHostMallocDefault) != hipSuccess) {
+        b->poll = nullptr;
+        mh_batch_destroy(b);
+        return fail(102, "hipHostMalloc failed");
+    }
     if (hipMalloc(&b->d_rays, sizeof(unsigned long long)) != hipSuccess) {
         mh_batch_destroy(b);
         return fail(102, "hipMalloc failed");
@@ -1357,6 +1430,7 @@ void mh_batch_destroy(mh_batch* b)
     if (b->fb_linear) (void)hipFree(b->fb_linear);
     if (b->fb_rgb) (void)hipFree(b->fb_rgb);
     if (b->d_rays) (void)hipFree(b->d_rays);
+    if (b->poll) (void)hipHostFree(b->poll);
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);

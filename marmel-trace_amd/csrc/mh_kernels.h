@@ -53,6 +53,7 @@ struct WfState {
     float* acc;       // per-pixel radiance sums (w*h*3)
     unsigned long long* batch;  // [0] first global path index of the current batch
     uint32_t* jid;    // batched engine: job of each path slot
+    uint32_t* freeq;  // batched engine: free path slots (count in cnt[4])
 };
 
 // One render job of the batched engine: a camera view of one scene.
@@ -73,6 +74,7 @@ struct WfMulti {
     long long total;             // paths over all jobs
     long long total_pix;         // pixels over all jobs
     unsigned long long* acc64;   // fixed-point (2^-24) per-pixel sums, 3 per pixel
+    int stream;                  // path regeneration: per-slot bounce, free-slot list
 };
 
 struct WfJob {
@@ -108,6 +110,12 @@ hipError_t wf_multi_batch(const WfMulti& mj, const WfState& ws, int pool, int ma
                           unsigned long long* rays, int grid, int block, bool trans, hipStream_t st,
                           KernelClock* kc = nullptr);
 hipError_t wf_multi_end(const WfMulti& mj, const LaunchOut& out, hipStream_t st);
+
+// Path regeneration: init (all slots free), and one step = refill + extend +
+// shade + shadow + finalize on extension queue `which`.
+hipError_t wf_stream_init(const WfState& ws, int pool, hipStream_t st);
+hipError_t wf_stream_step(const WfMulti& mj, const WfState& ws, int which, unsigned long long* rays, int grid,
+                          int block, bool trans, hipStream_t st, KernelClock* kc = nullptr);
 
 // Whitted: persistent grid of `threads` lanes, each with `cap` stack items
 // of `item_bytes` at stack + lane * cap * item_bytes.

@@ -84,6 +84,44 @@ __device__ __forceinline__ const SceneView<float>& slot_scene(const SceneView<fl
     return MULTI ? mj.scenes[mj.jobs[st.jid[slot]].scene] : sc;
 }
 
+// Start global path g (camera ray, seed, unit throughput) in path slot `slot`.
+template <bool MULTI>
+__device__ __forceinline__ void start_path(const SceneView<float>& sc0, const CamView<float>& cam0, const WfJob& job,
+                                           const WfState& st, const WfMulti& mj, long long g, uint32_t slot)
+{
+    int j = MULTI ? find_job(mj, g) : 0;
+    const BJob* bj = MULTI ? &mj.jobs[j] : nullptr;
+    const SceneView<float>& sc = MULTI ? mj.scenes[bj->scene] : sc0;
+    const CamView<float>& cam = MULTI ? bj->cam : cam0;
+    long long l = MULTI ? g - bj->begin : g;
+    int npix = MULTI ? bj->npix : job.npix;
+    int jw = MULTI ? bj->w : job.w, jh = MULTI ? bj->h : job.h;
+    uint32_t pix = (uint32_t)(l % npix);
+    uint32_t s = (uint32_t)(MULTI ? bj->s0 : job.s0) + (uint32_t)(l / npix);
+    uint32_t x = pix % (uint32_t)jw, y = pix / (uint32_t)jw;
+    float jx = rand01<float>(x, y, s * 2u + 0u), jy = rand01<float>(x, y, s * 2u + 1u);
+    float l1 = rand01<float>(x, y, s * 2u + 100u), l2 = rand01<float>(x, y, s * 2u + 101u);
+    float uu = ((float)x + jx) / (float)jw;
+    float vv = 1.0f - ((float)y + jy) / (float)jh;
+    V3<float> o, d;
+    camera_ray(cam, uu, vv, l1, l2, &o, &d);
+    if (cam.dome > 1e-4f && sc.water_material >= 0) {
+        V3<float> pd = o + d * cam.dome;
+        if (pd.y < sc.water_level) o = pd - mk(0.f, 1e-3f, 0.f);
+    }
+    int medium = -1;
+    if (sc.water_material >= 0 && o.y < sc.water_level && sc.water_material < sc.n_mats) medium = sc.water_material;
+    d = normalize(d);
+    uint32_t seed = hash3(x, y, s);
+    uint32_t meta = 0u | (1u << 8) | ((uint32_t)(medium + 1) << 9);
+    st.p0[slot] = make_float4(o.x, o.y, o.z, u2f(seed));
+    st.p1[slot] = make_float4(d.x, d.y, d.z, u2f(len2(d) > 0.f ? meta : 0xffffffffu));
+    st.p2[slot] = make_float4(1.f, 1.f, 1.f, u2f(pix));
+    st.p3[slot] = make_float4(0.f, 0.f, 0.f, u2f(s));
+    st.npend[slot] = 0;
+    if (MULTI) st.jid[slot] = (uint32_t)j;
+}
+
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_wf_gen(SceneView<float> sc0, CamView<float> cam0, WfJob job, WfState st,
                                                 WfMulti mj)
@@ -93,39 +131,68 @@ __global__ __launch_bounds__(256) void k_wf_gen(SceneView<float> sc0, CamView<fl
     int n = (int)(remain < job.pool ? remain : job.pool);
     if (blockIdx.x == 0 && threadIdx.x == 0) st.cnt[0] = (uint32_t)(n > 0 ? n : 0);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        long long g = (long long)g0 + i;
-        int j = MULTI ? find_job(mj, g) : 0;
-        const BJob* bj = MULTI ? &mj.jobs[j] : nullptr;
-        const SceneView<float>& sc = MULTI ? mj.scenes[bj->scene] : sc0;
-        const CamView<float>& cam = MULTI ? bj->cam : cam0;
-        long long l = MULTI ? g - bj->begin : g;
-        int npix = MULTI ? bj->npix : job.npix;
-        int jw = MULTI ? bj->w : job.w, jh = MULTI ? bj->h : job.h;
-        uint32_t pix = (uint32_t)(l % npix);
-        uint32_t s = (uint32_t)(MULTI ? bj->s0 : job.s0) + (uint32_t)(l / npix);
-        uint32_t x = pix % (uint32_t)jw, y = pix / (uint32_t)jw;
-        float jx = rand01<float>(x, y, s * 2u + 0u), jy = rand01<float>(x, y, s * 2u + 1u);
-        float l1 = rand01<float>(x, y, s * 2u + 100u), l2 = rand01<float>(x, y, s * 2u + 101u);
-        float uu = ((float)x + jx) / (float)jw;
-        float vv = 1.0f - ((float)y + jy) / (float)jh;
-        V3<float> o, d;
-        camera_ray(cam, uu, vv, l1, l2, &o, &d);
-        if (cam.dome > 1e-4f && sc.water_material >= 0) {
-            V3<float> pd = o + d * cam.dome;
-            if (pd.y < sc.water_level) o = pd - mk(0.f, 1e-3f, 0.f);
-        }
-        int medium = -1;
-        if (sc.water_material >= 0 && o.y < sc.water_level && sc.water_material < sc.n_mats) medium = sc.water_material;
-        d = normalize(d);
-        uint32_t seed = hash3(x, y, s);
-        uint32_t meta = 0u | (1u << 8) | ((uint32_t)(medium + 1) << 9);
-        st.p0[i] = make_float4(o.x, o.y, o.z, u2f(seed));
-        st.p1[i] = make_float4(d.x, d.y, d.z, u2f(len2(d) > 0.f ? meta : 0xffffffffu));
-        st.p2[i] = make_float4(1.f, 1.f, 1.f, u2f(pix));
-        st.p3[i] = make_float4(0.f, 0.f, 0.f, u2f(s));
-        st.npend[i] = 0;
-        if (MULTI) st.jid[i] = (uint32_t)j;
+        start_path<MULTI>(sc0, cam0, job, st, mj, (long long)g0 + i, (uint32_t)i);
         st.q[0][i] = (uint32_t)i;
+    }
+}
+
+// ---- path regeneration (batched engine) --------------------------------------
+//
+// Each bounce step first refills the slots freed by the previous step's ended
+// paths with new paths from the global cursor, appending them to the same
+// extension queue as the continuing paths: every step's kernels see a full
+// pool until the work runs out. cnt[4] counts free slots (listed in freeq).
+
+// Number of new paths the step starts: min(free slots, paths not started).
+__device__ __forceinline__ uint32_t refill_count(const WfState& st, const WfMulti& mj)
+{
+    uint32_t nfree = st.cnt[4];
+    long long remain = mj.total - (long long)st.batch[0];
+    if (remain <= 0) return 0u;
+    return remain < (long long)nfree ? (uint32_t)remain : nfree;
+}
+
+__global__ __launch_bounds__(256) void k_wf_refill(WfState st, WfMulti mj, int which)
+{
+    uint32_t nfree = st.cnt[4];
+    uint32_t k = refill_count(st, mj);
+    unsigned long long g0 = st.batch[0];
+    uint32_t n_round = (k + 63u) & ~63u;
+    SceneView<float> none{};
+    CamView<float> cnone{};
+    WfJob jnone{};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += gridDim.x * blockDim.x) {
+        bool active = i < k;
+        uint32_t slot = 0;
+        if (active) {
+            slot = st.freeq[nfree - 1u - i];
+            start_path<true>(none, cnone, jnone, st, mj, (long long)g0 + i, slot);
+        }
+        bool ok;
+        uint32_t at = wave_append(&st.cnt[which], active, &ok);
+        if (ok) st.q[which][at] = slot;
+    }
+}
+
+// Advance the cursor and pop the used free slots (after k_wf_refill).
+__global__ __launch_bounds__(64) void k_wf_refill_commit(WfState st, WfMulti mj)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        uint32_t k = refill_count(st, mj);
+        st.batch[0] += (unsigned long long)k;
+        st.cnt[4] -= k;
+    }
+}
+
+// Every slot free, queues empty, cursor at 0.
+__global__ __launch_bounds__(256) void k_wf_stream_init(WfState st, int pool)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < pool; i += gridDim.x * blockDim.x)
+        st.freeq[i] = (uint32_t)(pool - 1 - i);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st.cnt[0] = st.cnt[1] = st.cnt[2] = st.cnt[3] = 0u;
+        st.cnt[4] = (uint32_t)pool;
+        st.batch[0] = 0ull;
     }
 }
 
@@ -196,7 +263,8 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
             uint32_t seed = f2u(a.w), meta = f2u(b.w);
             int last_spec = (meta >> 8) & 1u;
             int medium = (int)(meta >> 9) - 1;
-            int b_idx = bounce;
+            // per-slot bounce under path regeneration (paths of every depth share a step)
+            int b_idx = (MULTI && mj.stream) ? (int)(f2u(b.w) & 0xffu) : bounce;
             int rec = __float_as_int(hh.y);
             L = resolve_pending(st, slot, L);
             st.npend[slot] = 0;
@@ -414,6 +482,12 @@ __global__ __launch_bounds__(256) void k_wf_finalize(WfState st, WfMulti mj)
             atomicAdd(&mj.acc64[p], to_fixed(L.x));
             atomicAdd(&mj.acc64[p + 1], to_fixed(L.y));
             atomicAdd(&mj.acc64[p + 2], to_fixed(L.z));
+            if (mj.stream) {
+                // the slot is free for the next step's new paths
+                bool ok;
+                uint32_t at = wave_append(&st.cnt[4], true, &ok);
+                st.freeq[at] = slot;
+            }
         } else {
             // one sample per pixel per batch: no other thread touches this pixel
             st.acc[(size_t)pix * 3] += L.x;

@@ -81,6 +81,31 @@ hipError_t wf_multi_batch(const WfMulti& mj, const WfState& ws, int pool, int ma
     return enqueue_bounces<true>(sc, cam, job, ws, mj, rays, grid, block, trans, st, kc);
 }
 
+hipError_t wf_stream_init(const WfState& ws, int pool, hipStream_t st)
+{
+    int blocks = (pool + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_wf_stream_init, dim3(blocks), dim3(256), 0, st, ws, pool);
+    return hipGetLastError();
+}
+
+hipError_t wf_stream_step(const WfMulti& mj, const WfState& ws, int which, unsigned long long* rays, int grid,
+                          int block, bool trans, hipStream_t st, KernelClock* kc)
+{
+    dim3 G(grid), B(block);
+    SceneView<float> sc{};
+    WfJob job{};
+    auto shadow = trans ? k_wf_shadow<true, true> : k_wf_shadow<true, false>;
+    MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_refill, G, B, 0, st, ws, mj, which));
+    MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_refill_commit, dim3(1), dim3(64), 0, st, ws, mj));
+    MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, ws.cnt, which ^ 1));
+    MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(k_wf_extend<true>, G, B, 0, st, sc, ws, which, mj));
+    MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(k_wf_shade<true>, G, B, 0, st, sc, ws, job, which, 0, rays, mj));
+    MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
+    MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<true>, G, B, 0, st, ws, mj));
+    return hipGetLastError();
+}
+
 hipError_t wf_multi_end(const WfMulti& mj, const LaunchOut& out, hipStream_t st)
 {
     long long n = mj.total_pix;

@@ -306,7 +306,7 @@ class Batch:
             pass
 
     def render(self, scenes, jobs, pool=0, grid_blocks=0, block=0, use_graph=True, autotune=False,
-               kernel_clock=False, fetch=True, bvh_width=0):
+               kernel_clock=False, fetch=True, bvh_width=0, fixed_batches=False):
         """scenes: list of GpuScene; jobs: list of dicts (scene, cam, width, height, spp, depth,
         sample_offset). Returns [(rgb, linear)] per job (None when fetch is False)."""
         arr = (C.c_void_p * len(scenes))(*[s._h.value for s in scenes])
@@ -317,7 +317,8 @@ class Batch:
                             max_depth=j["depth"], sample_offset=j.get("sample_offset", 0), cam=j["cam"])
             npx.append(max(j["width"], 0) * max(j["height"], 0))
         t = abi.Tune(pool=pool, grid_blocks=grid_blocks, block=block, use_graph=int(use_graph),
-                     autotune=int(autotune), kernel_clock=int(kernel_clock), bvh_width=bvh_width)
+                     autotune=int(autotune), kernel_clock=int(kernel_clock), bvh_width=bvh_width,
+                     fixed_batches=int(fixed_batches))
         tot = max(sum(npx), 1)
         rgb = np.zeros(tot * 3, np.uint8) if fetch else None
         lin = np.zeros(tot * 3, np.float32) if fetch else None
@@ -337,7 +338,7 @@ class Batch:
     def last_tune(self):
         t = abi.Tune()
         self._L.mh_batch_last_tune(self._h, C.byref(t))
-        return {n: getattr(t, n) for n, _ in abi.Tune._fields_ if n != "reserved"}
+        return {n: getattr(t, n) for n, _ in abi.Tune._fields_}
 
 
 def device_info():

@@ -34,7 +34,8 @@ def world():
 
 
 def jobs_of(world, names):
-    """Uneven jobs: sizes, spp, depths and sample offsets differ."""
+    """Uneven jobs: sizes, spp, depths and sample offsets differ (pools far
+    below and far above the path count; regeneration and fixed batches)."""
     spec = [("spheres", 64, 48, 8, 6, 0), ("terrain", 80, 45, 4, 8, 0), ("icosphere", 40, 40, 16, 5, 0),
             ("spheres", 33, 17, 12, 3, 100), ("terrain", 24, 56, 2, 10, 7)]
     jobs = []
@@ -76,7 +77,11 @@ def test_batch_is_independent_of_launch_settings(world):
     ref = b.render(scenes, jobs, pool=1 << 21, grid_blocks=2048, block=256, use_graph=True)
     for kw in (dict(pool=1000, grid_blocks=64, block=128, use_graph=False),
                dict(pool=777, grid_blocks=300, block=64, use_graph=True),
-               dict(pool=50000, grid_blocks=512, block=256, use_graph=True, kernel_clock=True)):
+               dict(pool=50000, grid_blocks=512, block=256, use_graph=True, kernel_clock=True),
+               dict(pool=4096, grid_blocks=64, fixed_batches=True),
+               dict(pool=3000, grid_blocks=128, fixed_batches=True, use_graph=False),
+               dict(pool=1 << 21, bvh_width=2),
+               dict(pool=999, grid_blocks=96, bvh_width=4, use_graph=False)):
         res = b.render(scenes, jobs, **kw)
         for (r0, l0), (r1, l1) in zip(ref, res):
             assert np.array_equal(l0, l1), kw
