@@ -74,7 +74,8 @@ enum {
     MT_GEN_SPHERES = 1,   /* plane + 64 spheres (Lambert / mirror / glass / GGX) */
     MT_GEN_TERRAIN = 2,   /* n x n fBm heightfield + 200 spheres, 8 emissive   */
     MT_GEN_ICOSPHERE = 3, /* displaced icosphere + spheres (batched scenes)     */
-    MT_GEN_MATERIALS = 4  /* plane + 63 spheres, 64 PBR materials (adjoint)    */
+    MT_GEN_MATERIALS = 4, /* plane + 63 spheres, 64 PBR materials (adjoint)    */
+    MT_GEN_FOREST = 5     /* `grid` trees of `subdiv` levels + pond (stress)   */
 };
 
 typedef struct mt_gen_params {

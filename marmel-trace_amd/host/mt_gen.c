@@ -11,6 +11,12 @@
  *                     PBR materials.
  *   MT_GEN_ICOSPHERE  a randomly displaced icosphere + spheres on a plane
  *                     (the scenes of a multi-scene batch).
+ *   MT_GEN_FOREST     the stress scene: `grid` seeded procedural trees (branch
+ *                     cylinders tessellated to 16-segment triangle tubes,
+ *                     `subdiv` branching levels, leaf quads at the tips) on a
+ *                     ground plane around a pond (a water box whose flat top
+ *                     takes the wave normal; refraction, Beer-Lambert), sun
+ *                     radius 0.5 degrees.
  *   MT_GEN_MATERIALS  the differentiable-rendering scene: a PBR ground plane
  *                     and a 9 x 7 grid of spheres, one GGX PBR material each
  *                     (64 materials: albedo U[0.1,0.9], roughness
@@ -386,6 +392,181 @@ static int gen_icosphere(mt_scene *s, const mt_gen_params *p, rng_t *r, mt_gen_i
     return rc;
 }
 
+/* ---- procedural trees ------------------------------------------------------ */
+
+typedef struct {
+    mh_prim *p;
+    long n, cap;
+} tribuf;
+
+static int tb_push(tribuf *b, mh_vec3 a, mh_vec3 c, mh_vec3 d, int mat)
+{
+    if (b->n == b->cap) {
+        long cap = b->cap ? 2 * b->cap : 4096;
+        mh_prim *q = (mh_prim *)realloc(b->p, sizeof(mh_prim) * (size_t)cap);
+        if (!q) return -1;
+        b->p = q;
+        b->cap = cap;
+    }
+    mh_prim *t = &b->p[b->n++];
+    memset(t, 0, sizeof *t);
+    t->kind = MH_PRIM_TRIANGLE;
+    t->material = mat;
+    t->p0 = V(f32(a.x), f32(a.y), f32(a.z));
+    t->p1 = V(f32(c.x), f32(c.y), f32(c.z));
+    t->p2 = V(f32(d.x), f32(d.y), f32(d.z));
+    return 0;
+}
+
+static mh_vec3 vadd(mh_vec3 a, mh_vec3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static mh_vec3 vsc(mh_vec3 a, double k) { return V(a.x * k, a.y * k, a.z * k); }
+static mh_vec3 vcross(mh_vec3 a, mh_vec3 b)
+{
+    return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static mh_vec3 vnorm(mh_vec3 a)
+{
+    double l = sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    return l > 0.0 ? vsc(a, 1.0 / l) : V(0.0, 1.0, 0.0);
+}
+
+/* Frame (u, v) perpendicular to unit axis w. */
+static void perp(mh_vec3 w, mh_vec3 *u, mh_vec3 *v)
+{
+    mh_vec3 a = fabs(w.y) < 0.9 ? V(0.0, 1.0, 0.0) : V(1.0, 0.0, 0.0);
+    *u = vnorm(vcross(a, w));
+    *v = vcross(w, *u);
+}
+
+/* Open tapered tube from a (radius ra) to b (radius rb), 16 segments. */
+static int tube(tribuf *tb, mh_vec3 a, mh_vec3 b, double ra, double rb, int mat)
+{
+    mh_vec3 w = vnorm(vadd(b, vsc(a, -1.0))), u, v;
+    perp(w, &u, &v);
+    for (int k = 0; k < 16; ++k) {
+        double t0 = 6.283185307179586 * k / 16.0, t1 = 6.283185307179586 * (k + 1) / 16.0;
+        mh_vec3 d0 = vadd(vsc(u, cos(t0)), vsc(v, sin(t0))), d1 = vadd(vsc(u, cos(t1)), vsc(v, sin(t1)));
+        mh_vec3 a0 = vadd(a, vsc(d0, ra)), a1 = vadd(a, vsc(d1, ra));
+        mh_vec3 b0 = vadd(b, vsc(d0, rb)), b1 = vadd(b, vsc(d1, rb));
+        if (tb_push(tb, a0, a1, b1, mat) || tb_push(tb, a0, b1, b0, mat)) return -1;
+    }
+    return 0;
+}
+
+static int branch(tribuf *tb, rng_t *r, mh_vec3 base, mh_vec3 dir, double len, double rad, int level, int bark,
+                  int leaf)
+{
+    mh_vec3 tip = vadd(base, vsc(dir, len));
+    if (tube(tb, base, tip, rad, rad * 0.7, bark)) return -1;
+    if (level <= 0) {
+        /* leaf quads around the tip */
+        for (int i = 0; i < 10; ++i) {
+            mh_vec3 c = vadd(tip, V(uni(r, -0.5, 0.5) * len, uni(r, -0.2, 0.5) * len, uni(r, -0.5, 0.5) * len));
+            mh_vec3 n = vnorm(V(uni(r, -1.0, 1.0), uni(r, -1.0, 1.0), uni(r, -1.0, 1.0))), u, v;
+            perp(n, &u, &v);
+            double s = 0.12 + 0.1 * uni(r, 0.0, 1.0);
+            mh_vec3 p0 = vadd(c, vadd(vsc(u, -s), vsc(v, -s))), p1 = vadd(c, vadd(vsc(u, s), vsc(v, -s)));
+            mh_vec3 p2 = vadd(c, vadd(vsc(u, s), vsc(v, s))), p3 = vadd(c, vadd(vsc(u, -s), vsc(v, s)));
+            if (tb_push(tb, p0, p1, p2, leaf) || tb_push(tb, p0, p2, p3, leaf)) return -1;
+        }
+        return 0;
+    }
+    int kids = 3;
+    mh_vec3 u, v;
+    perp(dir, &u, &v);
+    double phase = uni(r, 0.0, 6.283185307179586);
+    for (int k = 0; k < kids; ++k) {
+        double az = phase + 6.283185307179586 * k / kids + uni(r, -0.3, 0.3);
+        double tilt = uni(r, 0.45, 0.9);
+        mh_vec3 side = vadd(vsc(u, cos(az)), vsc(v, sin(az)));
+        mh_vec3 nd = vnorm(vadd(vsc(dir, cos(tilt)), vsc(side, sin(tilt))));
+        nd = vnorm(vadd(nd, V(0.0, 0.25, 0.0)));
+        if (branch(tb, r, tip, nd, len * uni(r, 0.62, 0.78), rad * 0.62, level - 1, bark, leaf)) return -1;
+    }
+    return 0;
+}
+
+static int gen_forest(mt_scene *s, const mt_gen_params *p, rng_t *r, mt_gen_info *info)
+{
+    int ntrees = p->grid > 0 ? p->grid : 8;
+    int levels = p->subdiv > 0 ? p->subdiv : 4;
+    if (levels > 9) levels = 9;
+    double radius = 6.0 + 2.0 * sqrt((double)ntrees); /* forest extent */
+    mh_material ground;
+    mt_material_default(&ground);
+    ground.albedo = V(0.32, 0.30, 0.22);
+    ground.specular = V(0.02, 0.02, 0.02);
+    ground.shininess = 8.0;
+    mh_material bark;
+    mt_material_default(&bark);
+    bark.albedo = V(0.28, 0.18, 0.10);
+    bark.specular = V(0.02, 0.02, 0.02);
+    bark.shininess = 8.0;
+    mh_material leaf;
+    mt_material_default(&leaf);
+    leaf.pbr = 1;
+    leaf.albedo = V(0.18, 0.42, 0.10);
+    leaf.roughness = 0.55;
+    leaf.metallic = 0.0;
+    mh_material water;
+    mt_material_preset("water", &water);
+    int gm = mt_scene_add_material(s, &ground), bm = mt_scene_add_material(s, &bark);
+    int lm = mt_scene_add_material(s, &leaf), wm = mt_scene_add_material(s, &water);
+    if (gm < 0 || bm < 0 || lm < 0 || wm < 0) return -1;
+    mh_prim plane;
+    memset(&plane, 0, sizeof plane);
+    plane.kind = MH_PRIM_PLANE;
+    plane.material = gm;
+    plane.p0 = V(0.0, -1.0, 0.0); /* pond bed and ground level */
+    plane.p1 = V(0.0, 1.0, 0.0);
+    /* the pond: a water box of depth 1 whose top (y = 0) is the water level */
+    mh_prim pond;
+    memset(&pond, 0, sizeof pond);
+    pond.kind = MH_PRIM_BOX;
+    pond.material = wm;
+    pond.p0 = V(0.0, -0.5, 0.0);
+    pond.p1 = V(4.0, 0.5, 3.0);
+    if (mt_scene_add_prims(s, &plane, NULL, 1) < 0 || mt_scene_add_prims(s, &pond, NULL, 1) < 0) return -1;
+    mt_scene_set_water(s, 0.0, wm);
+    /* banks around the pond: a ring of ground tiles at y = 0 */
+    tribuf tb = { NULL, 0, 0 };
+    double R = radius + 4.0;
+    mh_vec3 q[8][2] = { { V(-R, 0, -R), V(R, 0, -3) }, { V(-R, 0, 3), V(R, 0, R) },
+                        { V(-R, 0, -3), V(-4, 0, 3) }, { V(4, 0, -3), V(R, 0, 3) } };
+    for (int i = 0; i < 4; ++i) {
+        mh_vec3 a = q[i][0], b = q[i][1];
+        if (tb_push(&tb, V(a.x, 0.0, a.z), V(b.x, 0.0, b.z), V(b.x, 0.0, a.z), gm) ||
+            tb_push(&tb, V(a.x, 0.0, a.z), V(a.x, 0.0, b.z), V(b.x, 0.0, b.z), gm)) {
+            free(tb.p);
+            return -1;
+        }
+    }
+    for (int t = 0; t < ntrees; ++t) {
+        double ang = uni(r, 0.0, 6.283185307179586), dist = uni(r, 5.5, radius);
+        mh_vec3 base = V(dist * cos(ang), 0.0, dist * sin(ang));
+        double h = uni(r, 1.6, 2.6);
+        mh_vec3 dir = vnorm(V(uni(r, -0.1, 0.1), 1.0, uni(r, -0.1, 0.1)));
+        if (branch(&tb, r, base, dir, h, 0.16 + 0.06 * uni(r, 0.0, 1.0), levels, bm, lm)) {
+            free(tb.p);
+            return -1;
+        }
+    }
+    int rc = mt_scene_add_prims(s, tb.p, NULL, (int)tb.n) < 0 ? -1 : 0;
+    mh_sky sky;
+    mt_sky_default(&sky);
+    sky.sun_radius = 0.5;
+    mt_scene_set_sky(s, &sky);
+    if (info) {
+        info->eye[0] = 0.0; info->eye[1] = 4.5; info->eye[2] = radius + 6.0;
+        info->at[0] = 0.0; info->at[1] = 1.0; info->at[2] = 0.0;
+        info->vfov = 55.0;
+        info->n_triangles = (int)tb.n;
+        info->n_prims = (int)tb.n + 2;
+    }
+    free(tb.p);
+    return rc;
+}
+
 static mh_material pbr_material(rng_t *r)
 {
     mh_material m;
@@ -450,6 +631,7 @@ int mt_scene_generate(mt_scene *s, const mt_gen_params *p, mt_gen_info *info)
     case MT_GEN_TERRAIN: return gen_terrain(s, p, &r, info);
     case MT_GEN_ICOSPHERE: return gen_icosphere(s, p, &r, info);
     case MT_GEN_MATERIALS: return gen_materials(s, p, &r, info);
+    case MT_GEN_FOREST: return gen_forest(s, p, &r, info);
     default: return 2;
     }
 }

@@ -125,4 +125,4 @@ class GenInfo(C.Structure):
                 ("n_spheres", C.c_int), ("n_emissive", C.c_int)]
 
 
-GEN_SPHERES, GEN_TERRAIN, GEN_ICOSPHERE, GEN_MATERIALS = 1, 2, 3, 4
+GEN_SPHERES, GEN_TERRAIN, GEN_ICOSPHERE, GEN_MATERIALS, GEN_FOREST = 1, 2, 3, 4, 5

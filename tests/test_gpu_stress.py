@@ -1,0 +1,99 @@
+"""The stress workload: seeded procedural trees (16-segment branch tubes +
+leaf quads) around a wave-perturbed refractive pond (water box, Beer-Lambert),
+sun radius 0.5 degrees, depth 12.
+
+  * at test size the fp64 GPU path equals the oracle (which equals the
+    reference on this content, tests/test_oracle.py) within 1/255 on
+    >= 99.9 % of pixels for the path tracer and Whitted, nearest-hit ids
+    match on seeded rays, and fp32 is statistically consistent,
+  * at full size (200 trees of 6 levels: 9.9 M triangles, 3840 x 2160,
+    progressive chunks) the render is finite, progressive chunks average to
+    the single render, and the wavefront engine agrees with the megakernel.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import checkers as ck
+from marmel_amd import Batch, GpuScene, HostScene, abi, native
+
+pytestmark = pytest.mark.gpu
+
+FP32, FP64 = abi.PREC_FP32, abi.PREC_FP64
+
+
+def within(a, b, tol=1):
+    return (np.abs(a.astype(int) - b.astype(int)) <= tol).all(axis=-1).mean()
+
+
+def forest(ntrees, levels, seed=5):
+    hs = HostScene()
+    info = hs.generate(native.GEN_FOREST, seed=seed, grid=ntrees, subdiv=levels)
+    st = hs.build_bvh()
+    return hs, info, st
+
+
+def test_forest_pond_matches_oracle():
+    hs, info, st = forest(4, 3)
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    assert any(m["is_water"] for m in mats) and sky.sun_radius == 0.5
+    o = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm).soa)
+    gs = GpuScene(hs.soa())
+    w, h = 96, 64
+    cam = hs.camera(w / h)
+    for mode, depth in ((abi.MODE_PATHTRACE, 12), (abi.MODE_WHITTED, 6)):
+        ref, _, _, _ = o.render(cam, w, h, 4, depth, mode=0 if mode == abi.MODE_PATHTRACE else 1)
+        for bw in (2, 4):
+            rgb, _ = gs.render(cam, w, h, 4, depth, mode=mode, precision=FP64, bvh_width=bw)
+            frac = within(rgb, ref)
+            print("forest mode", mode, "width", bw, "fp64 within 1/255:", frac)
+            assert frac >= 0.999
+    rays = ck.seeded_rays(4000, 9, [-14, 0.05, -14], [14, 5, 14])
+    ho = o.intersect(rays)
+    for prec in (FP64, FP32):
+        hg = gs.trace_closest(rays, precision=prec)
+        same = hg["prim_index"] == ho["prim_index"]
+        tie = np.isclose(hg["t"], ho["t"], rtol=1e-5)
+        print("forest nearest ids, precision", prec, "equal", same.mean(), "equal or tied", (same | tie).mean())
+        assert (same | tie).all()
+    # fp32 path tracer vs a converged fp64 oracle render
+    w2, h2 = 40, 28
+    cam2 = hs.camera(w2 / h2)
+    _, ref_mean, ref_var, _ = o.render(cam2, w2, h2, 512, 12, want_var=True, sample_offset=3000)
+    _, lin = gs.render(cam2, w2, h2, 64, 12, precision=FP32)
+    z = (lin - ref_mean) / (np.sqrt(ref_var / 64 + ref_var / 512) + 1e-6)
+    print("forest fp32 within 3 sigma", (np.abs(z) <= 3).mean(), "mean z", z.mean())
+    assert (np.abs(z) <= 3).mean() >= 0.97 and abs(z.mean()) < 0.1
+    gs.close()
+
+
+def test_stress_full_size_progressive():
+    t0 = time.time()
+    hs, info, st = forest(200, 6)
+    assert info.n_triangles > 9_000_000
+    gs = GpuScene(hs.soa())
+    print("stress scene: %d triangles, BVH depth %d, host build %.1fs, device %.0f MiB"
+          % (info.n_triangles, st.max_depth, time.time() - t0, gs.device_bytes / 2**20))
+    w, h, depth = 3840, 2160, 12
+    cam = hs.camera(w / h)
+    _, full = gs.render(cam, w, h, 4, depth, precision=FP32)
+    t = gs.last_timing()
+    print("stress 4 spp megakernel %.1f ms, %.1f Msamples/s" % (t.kernel_ms, w * h * 4 / t.kernel_ms / 1e3))
+    assert np.isfinite(full).all() and (full >= 0).all() and 0.01 < full.mean() < 10.0
+    # progressive: two 2-spp chunks with sample offsets 0 and 2 average to the 4-spp image
+    _, c1 = gs.render(cam, w, h, 2, depth, precision=FP32, sample_offset=0)
+    _, c2 = gs.render(cam, w, h, 2, depth, precision=FP32, sample_offset=2)
+    avg = 0.5 * (c1.astype(np.float64) + c2)
+    assert np.allclose(avg, full, rtol=1e-4, atol=1e-5)
+    # the batched engine on the same frame: same estimator up to fp32 rounding
+    b = Batch()
+    (rgb_b, lin_b), = b.render([gs], [dict(scene=0, cam=cam, width=w, height=h, spp=4, depth=depth)])
+    t = gs.last_timing()
+    print("stress 4 spp batched wavefront %.1f ms, %.1f Msamples/s" % (t.kernel_ms, w * h * 4 / t.kernel_ms / 1e3))
+    rgb_m = np.clip(np.power(np.clip(full, 0, 1), 1 / 2.2) * 255 + 0.5, 0, 255).astype(np.uint8)
+    frac = within(rgb_b, rgb_m)
+    print("stress batched vs megakernel within 1/255:", frac)
+    assert frac >= 0.99
+    b.close()
+    gs.close()

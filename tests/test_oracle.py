@@ -258,7 +258,9 @@ def test_bvh_nearest_hit_equals_brute_force_spheres():
 @pytest.mark.skipif(not ck.have_ref(), reason="oracle/_ref not built (reference absent)")
 @pytest.mark.parametrize("kind,kw,seed", [(native.GEN_SPHERES, {}, 7),
                                           (native.GEN_TERRAIN, dict(grid=40, n_spheres=30), 3),
-                                          (native.GEN_ICOSPHERE, dict(subdiv=1), 5)])
+                                          (native.GEN_ICOSPHERE, dict(subdiv=1), 5),
+                                          (native.GEN_MATERIALS, {}, 4),
+                                          (native.GEN_FOREST, dict(grid=3, subdiv=2), 5)])
 def test_oracle_equals_live_reference(kind, kw, seed):
     hs = HostScene()
     hs.generate(kind, seed=seed, **kw)
