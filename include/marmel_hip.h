@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define MH_ABI_VERSION 1
+#define MH_ABI_VERSION 2
 
 /* Opaque device-resident scene handle. */
 struct mh_scene;
@@ -147,8 +147,25 @@ typedef struct mh_sky {
     double   star_intensity;    /* must be 0 */
     double   nebula_intensity;  /* must be 0 */
     double   galaxy_intensity;  /* must be 0 */
-    double   fog_density;       /* must be 0 */
+    double   fog_density;       /* must be 0: fog is mh_scene_soa.fog */
 } mh_sky;
+
+/* Atmospheric fog (reference FogParams, src/material.h:199-209). density 0
+ * disables it. shadow_steps > 0 marches camera-ray segments with a shadow
+ * ray to the sun per step (god rays) and marches underwater segments; 0 uses
+ * the closed form. */
+typedef struct mh_fog {
+    double  density;             /* extinction coefficient >= 0            */
+    mh_vec3 color;               /* ambient in-scatter colour (linear)     */
+    double  height;              /* reference height of the falloff        */
+    double  height_falloff;      /* exponential decay with height, >= 0    */
+    double  inscatter_strength;  /* sun in-scatter intensity >= 0          */
+    double  sun_anisotropy;      /* Henyey-Greenstein g, clamped to 0.95   */
+    double  noise_scale;         /* 3D fBm turbulence frequency (0 = none) */
+    double  noise_amount;        /* turbulence strength in [0, 1]          */
+    int32_t shadow_steps;        /* march steps (0 = closed form)          */
+    int32_t reserved;
+} mh_fog;
 
 /* Pinhole / thin-lens camera (reference Camera, src/camera.h:17-48). */
 typedef struct mh_camera {
@@ -190,6 +207,7 @@ typedef struct mh_scene_soa {
     const int32_t      *leaf_prims;      /* prim indices in leaf order       */
     int32_t             n_planes;
     const int32_t      *planes;          /* plane prim indices, file order   */
+    mh_fog              fog;             /* reference Scene.fog              */
 } mh_scene_soa;
 
 /* ------------------------------------------------------------------ */
@@ -328,13 +346,14 @@ int mh_render(struct mh_scene *s, const mh_camera *cam,
 int mh_framebuffer_download(struct mh_scene *s, unsigned char *rgb_out,
                             float *linear_out);
 
-/* Replace every material (same count as at creation) / the sky of an
- * uploaded scene, keeping geometry and BVH. The emissive-light list and the
- * transmissive flag are recomputed as mh_scene_create would. No reference
- * counterpart (the reference rebuilds its Scene); used by the differentiable
- * renderer and parameter sweeps. */
+/* Replace every material (same count as at creation) / the sky / the fog
+ * of an uploaded scene, keeping geometry and BVH. The emissive-light list
+ * and the transmissive flag are recomputed as mh_scene_create would. No
+ * reference counterpart (the reference rebuilds its Scene); used by the
+ * differentiable renderer and parameter sweeps. */
 int mh_scene_set_materials(struct mh_scene *s, const mh_material *mats, int n);
 int mh_scene_set_sky(struct mh_scene *s, const mh_sky *sky);
+int mh_scene_set_fog(struct mh_scene *s, const mh_fog *fog);
 
 /* Path-replay adjoint of the fp32 path tracer (no reference counterpart).
  * d_image: dLoss/d(linear pixel mean), w*h*3 floats, top-down like the

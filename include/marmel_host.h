@@ -43,6 +43,9 @@ int mt_scene_add_material(mt_scene *s, const mh_material *m);
 int mt_scene_add_prims(mt_scene *s, const mh_prim *prims, const mh_prim_attr *attrs, int n);
 void mt_scene_set_sky(mt_scene *s, const mh_sky *sky);
 void mt_scene_set_water(mt_scene *s, double level, int material);
+/* Fog (default: reference fog_default_params, disabled). */
+void mt_fog_default(mh_fog *fog);
+void mt_scene_set_fog(mt_scene *s, const mh_fog *fog);
 int mt_scene_counts(const mt_scene *s, int *n_prims, int *n_materials);
 
 /* BVH build options. */

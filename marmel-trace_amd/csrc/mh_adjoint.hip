@@ -196,7 +196,15 @@ struct AdjHook : NoHook {
         if (v == 0.f || !(v == v)) return;
         atomicAdd(&g[MH_ADJ_LDS_MATS * G_PER_MAT + c], v);
     }
+    static constexpr bool kGrad = true;
     __device__ void add(V3<float> t) { Lacc = Lacc + t; }
+    // fog / marched-water in-scatter: w = d(term)/d(sun colour), per channel
+    __device__ void sun_lin(V3<float> w)
+    {
+        put_sun(0, delta.x * w.x);
+        put_sun(1, delta.y * w.y);
+        put_sun(2, delta.z * w.z);
+    }
     __device__ void sky(V3<float> beta, V3<float> d, V3<float> t)
     {
         V3<float> ds = sky_dsun(d, sc->sky);
@@ -310,6 +318,7 @@ struct AdjHook : NoHook {
 
 // One thread per pixel (16x16 tiles), samples in order; each sample is
 // traced twice (primal, then replay with gradients).
+template <bool FOG>
 __global__ __launch_bounds__(256) void k_adjoint(SceneView<float> sc, CamView<float> cam, int w, int h, int spp,
                                                   int s0, int depth, int tiles_x, int ntiles, AdjParams ap)
 {
@@ -350,7 +359,7 @@ __global__ __launch_bounds__(256) void k_adjoint(SceneView<float> sc, CamView<fl
                 uint32_t seed = hash3((uint32_t)x, (uint32_t)y, s);
                 // pass 1: the raw (unclamped) sum of the path's terms
                 SumHook sum;
-                (void)pt_radiance_h(sc, ts, o, d, depth, seed, &nq, sum);
+                (void)pt_radiance_h<FOG>(sc, ts, o, d, depth, seed, &nq, sum);
                 V3<float> L = sum.L;
                 // the final clamp of a negative channel to 0 (and of a
                 // non-finite sum) has zero derivative
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(256) void k_adjoint(SceneView<float> sc, CamView<fl
                 hk.Lacc = zero3<float>();
                 hk.delta = mk(L.x >= 0.f ? dpix.x : 0.f, L.y >= 0.f ? dpix.y : 0.f, L.z >= 0.f ? dpix.z : 0.f);
                 // pass 2: replay with gradients
-                (void)pt_radiance_h(sc, ts, o, d, depth, seed, &nq, hk);
+                (void)pt_radiance_h<FOG>(sc, ts, o, d, depth, seed, &nq, hk);
             }
         }
     }
@@ -379,6 +388,7 @@ struct FreezeHook : NoHook {
     template <typename R> __device__ const DMat<R>* frozen(int m) const { return &f[m]; }
 };
 
+template <bool FOG>
 __global__ __launch_bounds__(256) void k_render_frozen(SceneView<float> sc, CamView<float> cam, int w, int h, int spp,
                                                         int s0, int depth, int tiles_x, int ntiles,
                                                         const DMat<float>* frozen, LaunchOut out)
@@ -408,7 +418,7 @@ __global__ __launch_bounds__(256) void k_render_frozen(SceneView<float> sc, CamV
                     V3<float> pd = o + d * cam.dome;
                     if (pd.y < sc.water_level) o = pd - mk(0.f, 1e-3f, 0.f);
                 }
-                acc = acc + pt_radiance_h(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq, fh);
+                acc = acc + pt_radiance_h<FOG>(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq, fh);
             }
             acc = acc * (1.f / (float)spp);
             size_t p = ((size_t)y * (size_t)w + (size_t)x) * 3;
@@ -427,7 +437,8 @@ hipError_t launch_render_frozen(const SceneView<float>& sc, const CamView<float>
 {
     int tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16;
     int ntiles = tiles_x * tiles_y;
-    hipLaunchKernelGGL(k_render_frozen, dim3(ntiles), dim3(256), 0, st, sc, cam, w, h, spp, s0, depth, tiles_x, ntiles,
+    auto kern = sc.fog_kernel ? k_render_frozen<true> : k_render_frozen<false>;
+    hipLaunchKernelGGL(kern, dim3(ntiles), dim3(256), 0, st, sc, cam, w, h, spp, s0, depth, tiles_x, ntiles,
                        (const DMat<float>*)frozen, out);
     return hipGetLastError();
 }
@@ -438,7 +449,8 @@ hipError_t launch_adjoint(const SceneView<float>& sc, const CamView<float>& cam,
     int tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16;
     int ntiles = tiles_x * tiles_y;
     AdjParams ap{dimg, grad, sc.n_mats, sc.n_mats < MH_ADJ_LDS_MATS ? sc.n_mats : MH_ADJ_LDS_MATS};
-    hipLaunchKernelGGL(k_adjoint, dim3(ntiles), dim3(256), 0, st, sc, cam, w, h, spp, s0, depth, tiles_x, ntiles, ap);
+    auto kern = sc.fog_kernel ? k_adjoint<true> : k_adjoint<false>;
+    hipLaunchKernelGGL(kern, dim3(ntiles), dim3(256), 0, st, sc, cam, w, h, spp, s0, depth, tiles_x, ntiles, ap);
     return hipGetLastError();
 }
 

@@ -288,6 +288,9 @@ struct mh_scene {
     size_t grad_n = 0;
     void* d_frozen = nullptr;     // frozen-sampling material table (fp32)
     size_t frozen_bytes = 0;
+    // fog as given, and whether a material is water (marched water segments)
+    mh_fog fog_host{};
+    int has_water = 0;
 };
 
 namespace {
@@ -310,9 +313,52 @@ int validate_sky(const mh_sky& k)
     if (k.cloud_thickness > 1e-4) return fail(4, "volumetric clouds (cloud_thickness %g) are outside the GPU path", k.cloud_thickness);
     if (k.star_intensity > 0.0 || k.nebula_intensity > 0.0 || k.galaxy_intensity > 0.0)
         return fail(4, "stars / nebula / galaxy sky layers are outside the GPU path");
-    if (k.fog_density > 0.0) return fail(4, "fog is outside the GPU path");
+    if (k.fog_density != 0.0) return fail(5, "mh_sky.fog_density must be 0: fog is mh_scene_soa.fog");
     return 0;
 }
+
+int validate_fog(const mh_fog& f)
+{
+    double v[10] = {f.density, f.color.x, f.color.y, f.color.z, f.height, f.height_falloff, f.inscatter_strength,
+                    f.sun_anisotropy, f.noise_scale, f.noise_amount};
+    for (double x : v)
+        if (!std::isfinite(x)) return fail(5, "fog parameter is not finite");
+    if (f.density < 0.0) return fail(5, "negative fog density %g", f.density);
+    return 0;
+}
+
+// Device fog parameters: the anisotropy clamped to [-0.95, 0.95] and the
+// turbulence switch evaluated once, as the reference evaluates them per call.
+template <typename R> DFog<R> conv_fog(const mh_fog& f)
+{
+    DFog<R> o;
+    memset(&o, 0, sizeof o);
+    o.density = (R)f.density;
+    o.color = cv<R>(f.color);
+    o.height = (R)f.height;
+    o.falloff = (R)f.height_falloff;
+    o.inscatter = (R)f.inscatter_strength;
+    double g = f.sun_anisotropy;
+    if (g < -0.95) g = -0.95;
+    if (g > 0.95) g = 0.95;
+    o.g = (R)g;
+    o.noise_scale = (R)f.noise_scale;
+    o.noise_amount = (R)f.noise_amount;
+    o.steps = f.shadow_steps;
+    o.noise = f.noise_amount > 1e-4 && f.noise_scale > 1e-4 ? 1 : 0;
+    return o;
+}
+
+int any_water(const mh_material* mats, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (mats[i].is_water) return 1;
+    return 0;
+}
+
+// Fog on, or marched underwater segments possible (they need fog steps > 0
+// and a water material, fog density or not).
+int needs_fog_kernel(const mh_fog& f, int has_water) { return f.density > 0.0 || (f.shadow_steps > 0 && has_water); }
 
 int validate_materials(const mh_material* mats, int n)
 {
@@ -335,6 +381,8 @@ int validate(const mh_scene_soa* soa)
         (soa->n_leaf_prims && !soa->leaf_prims) || (soa->n_planes && !soa->planes))
         return fail(1, "NULL array with non-zero count");
     int rc = validate_sky(soa->sky);
+    if (rc) return rc;
+    rc = validate_fog(soa->fog);
     if (rc) return rc;
     rc = validate_materials(soa->materials, soa->n_materials);
     if (rc) return rc;
@@ -683,7 +731,7 @@ void drop_graph(mh_batch* b)
 // Path regeneration: steps until every path of the workload (or the first
 // `limit` ones) has ended. The host replays pairs of steps (the queues
 // ping-pong) and between rounds reads the queue length and cursor back.
-int stream_run(mh_batch* b, WfMulti mj, bool trans, const mh_tune& t, long long limit)
+int stream_run(mh_batch* b, WfMulti mj, int feat, const mh_tune& t, long long limit)
 {
     if (limit >= 0 && limit < mj.total) mj.total = limit;
     mj.stream = 1;
@@ -693,7 +741,7 @@ int stream_run(mh_batch* b, WfMulti mj, bool trans, const mh_tune& t, long long 
     bool graph = t.use_graph && !t.kernel_clock;
     if (graph) {
         std::vector<unsigned char> key(sizeof mj + sizeof b->ws + 4 * sizeof(int));
-        int extra[4] = {t.grid_blocks, t.block, trans ? 1 : 0, t.pool};
+        int extra[4] = {t.grid_blocks, t.block, feat, t.pool};
         memcpy(key.data(), &mj, sizeof mj);
         memcpy(key.data() + sizeof mj, &b->ws, sizeof b->ws);
         memcpy(key.data() + sizeof mj + sizeof b->ws, extra, sizeof extra);
@@ -703,9 +751,9 @@ int stream_run(mh_batch* b, WfMulti mj, bool trans, const mh_tune& t, long long 
             b->sgraph = nullptr;
             b->sgraph_def = nullptr;
             HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t ce = wf_stream_step(mj, b->ws, 0, b->d_rays, t.grid_blocks, t.block, trans, b->stream);
+            hipError_t ce = wf_stream_step(mj, b->ws, 0, b->d_rays, t.grid_blocks, t.block, feat, b->stream);
             if (ce == hipSuccess)
-                ce = wf_stream_step(mj, b->ws, 1, b->d_rays, t.grid_blocks, t.block, trans, b->stream);
+                ce = wf_stream_step(mj, b->ws, 1, b->d_rays, t.grid_blocks, t.block, feat, b->stream);
             hipError_t ee = hipStreamEndCapture(b->stream, &b->sgraph_def);
             HIP_TRY(ce);
             HIP_TRY(ee);
@@ -720,8 +768,8 @@ int stream_run(mh_batch* b, WfMulti mj, bool trans, const mh_tune& t, long long 
                 HIP_TRY(hipGraphLaunch(b->sgraph, b->stream));
             } else {
                 KernelClock* kc = t.kernel_clock ? &b->clock : nullptr;
-                HIP_TRY(wf_stream_step(mj, b->ws, 0, b->d_rays, t.grid_blocks, t.block, trans, b->stream, kc));
-                HIP_TRY(wf_stream_step(mj, b->ws, 1, b->d_rays, t.grid_blocks, t.block, trans, b->stream, kc));
+                HIP_TRY(wf_stream_step(mj, b->ws, 0, b->d_rays, t.grid_blocks, t.block, feat, b->stream, kc));
+                HIP_TRY(wf_stream_step(mj, b->ws, 1, b->d_rays, t.grid_blocks, t.block, feat, b->stream, kc));
             }
         }
         // paths queued for the next step (cnt[0]) and the cursor
@@ -749,26 +797,26 @@ template <typename T> int grow(T** p, size_t* have, size_t need, size_t elem)
 }
 
 // Run the batches of one render: `limit` paths (all when < 0) with settings t.
-int batch_run(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, const mh_tune& t, long long limit)
+int batch_run(mh_batch* b, const WfMulti& mj, int max_depth, int feat, const mh_tune& t, long long limit)
 {
-    if (!t.fixed_batches) return stream_run(b, mj, trans, t, limit);  // path regeneration (default)
+    if (!t.fixed_batches) return stream_run(b, mj, feat, t, limit);  // path regeneration (default)
     long long total = limit >= 0 && limit < mj.total ? limit : mj.total;
     long long nb = (total + t.pool - 1) / t.pool;
     if (t.kernel_clock) {
         HIP_TRY(b->clock.start());
         for (long long i = 0; i < nb; ++i)
-            HIP_TRY(wf_multi_batch(mj, b->ws, t.pool, max_depth, b->d_rays, t.grid_blocks, t.block, trans,
+            HIP_TRY(wf_multi_batch(mj, b->ws, t.pool, max_depth, b->d_rays, t.grid_blocks, t.block, feat,
                                    b->stream, &b->clock));
     } else if (t.use_graph) {
         std::vector<unsigned char> key(sizeof mj + sizeof b->ws + 5 * sizeof(int));
-        int extra[5] = {t.pool, max_depth, t.grid_blocks, t.block, trans ? 1 : 0};
+        int extra[5] = {t.pool, max_depth, t.grid_blocks, t.block, feat};
         memcpy(key.data(), &mj, sizeof mj);
         memcpy(key.data() + sizeof mj, &b->ws, sizeof b->ws);
         memcpy(key.data() + sizeof mj + sizeof b->ws, extra, sizeof extra);
         if (!b->graph || key != b->graph_key) {
             drop_graph(b);
             HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t ce = wf_multi_batch(mj, b->ws, t.pool, max_depth, b->d_rays, t.grid_blocks, t.block, trans,
+            hipError_t ce = wf_multi_batch(mj, b->ws, t.pool, max_depth, b->d_rays, t.grid_blocks, t.block, feat,
                                            b->stream);
             hipError_t ee = hipStreamEndCapture(b->stream, &b->graph_def);
             HIP_TRY(ce);
@@ -779,7 +827,7 @@ int batch_run(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, const m
         for (long long i = 0; i < nb; ++i) HIP_TRY(hipGraphLaunch(b->graph, b->stream));
     } else {
         for (long long i = 0; i < nb; ++i)
-            HIP_TRY(wf_multi_batch(mj, b->ws, t.pool, max_depth, b->d_rays, t.grid_blocks, t.block, trans, b->stream));
+            HIP_TRY(wf_multi_batch(mj, b->ws, t.pool, max_depth, b->d_rays, t.grid_blocks, t.block, feat, b->stream));
     }
     return 0;
 }
@@ -799,7 +847,7 @@ int ensure_batch_state(mh_batch* b, int pool)
 }
 
 // Paths per ms of `trial` paths of the workload under settings t.
-int batch_trial(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, mh_tune t, long long trial, double* rate)
+int batch_trial(mh_batch* b, const WfMulti& mj, int max_depth, int feat, mh_tune t, long long trial, double* rate)
 {
     int rc = ensure_batch_state(b, t.pool);
     if (rc) return rc;
@@ -807,7 +855,7 @@ int batch_trial(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, mh_tu
     t.kernel_clock = 0;
     HIP_TRY(wf_multi_begin(b->ws, mj, b->stream));
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    rc = batch_run(b, mj, max_depth, trans, t, trial);
+    rc = batch_run(b, mj, max_depth, feat, t, trial);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
@@ -821,7 +869,7 @@ int batch_trial(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, mh_tu
 
 // Coordinate search: pool, then grid blocks, then block size, each by the
 // path rate of a fixed trial of the workload's first paths.
-int autotune(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, mh_tune* t)
+int autotune(mh_batch* b, const WfMulti& mj, int max_depth, int feat, mh_tune* t)
 {
     long long trial = std::min<long long>(mj.total, 1LL << 23);
     double best = -1.0;
@@ -833,7 +881,7 @@ int autotune(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, mh_tune*
         mh_tune c = cur;
         c.pool = p;
         double r;
-        int rc = batch_trial(b, mj, max_depth, trans, c, std::max<long long>(trial, p), &r);
+        int rc = batch_trial(b, mj, max_depth, feat, c, std::max<long long>(trial, p), &r);
         if (rc) return rc;
         if (r > best) {
             best = r;
@@ -848,7 +896,7 @@ int autotune(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, mh_tune*
         mh_tune c = cur;
         c.grid_blocks = g;
         double r;
-        int rc = batch_trial(b, mj, max_depth, trans, c, std::max<long long>(trial, cur.pool), &r);
+        int rc = batch_trial(b, mj, max_depth, feat, c, std::max<long long>(trial, cur.pool), &r);
         if (rc) return rc;
         if (r > best) {
             best = r;
@@ -863,7 +911,7 @@ int autotune(mh_batch* b, const WfMulti& mj, int max_depth, bool trans, mh_tune*
         mh_tune c = cur;
         c.block = bl;
         double r;
-        int rc = batch_trial(b, mj, max_depth, trans, c, std::max<long long>(trial, cur.pool), &r);
+        int rc = batch_trial(b, mj, max_depth, feat, c, std::max<long long>(trial, cur.pool), &r);
         if (rc) return rc;
         if (r > best) {
             best = r;
@@ -1027,12 +1075,16 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         v.mats = (const DMat<R>*)mt;
         v.n_mats = soa->n_materials;
         v.sky = conv_sky<R>(soa->sky);
+        v.fog = conv_fog<R>(soa->fog);
+        v.fog_kernel = needs_fog_kernel(soa->fog, any_water(soa->materials, soa->n_materials));
         v.water_level = (R)soa->water_level;
         v.water_material = soa->water_material;
         v.has_transmissive = has_transmissive;
         v.n_lights = nl;
         for (int i = 0; i < 8; ++i) v.lights[i] = lights[i];
     };
+    s->fog_host = soa->fog;
+    s->has_water = any_water(soa->materials, soa->n_materials);
     fill(s->v32, d_p32, d_pl32, d_a32, d_m32, l32, 0.0f);
     fill(s->v64, d_p64, d_pl64, d_a64, d_m64, l64, 0.0);
     *out = s;
@@ -1098,6 +1150,8 @@ int mh_scene_set_materials(mh_scene* s, const mh_material* mats, int n)
         s->v64.lights[i] = l64[i];
     }
     s->v32.has_transmissive = s->v64.has_transmissive = tr;
+    s->has_water = any_water(mats, n);
+    s->v32.fog_kernel = s->v64.fog_kernel = needs_fog_kernel(s->fog_host, s->has_water);
     drop_scene_graph(s);  // captured launches hold the scene view by value
     return 0;
 }
@@ -1109,6 +1163,19 @@ int mh_scene_set_sky(mh_scene* s, const mh_sky* sky)
     if (rc) return rc;
     s->v32.sky = conv_sky<float>(*sky);
     s->v64.sky = conv_sky<double>(*sky);
+    drop_scene_graph(s);
+    return 0;
+}
+
+int mh_scene_set_fog(mh_scene* s, const mh_fog* fog)
+{
+    if (!s || !fog) return fail(1, "NULL argument");
+    int rc = validate_fog(*fog);
+    if (rc) return rc;
+    s->fog_host = *fog;
+    s->v32.fog = conv_fog<float>(*fog);
+    s->v64.fog = conv_fog<double>(*fog);
+    s->v32.fog_kernel = s->v64.fog_kernel = needs_fog_kernel(*fog, s->has_water);
     drop_scene_graph(s);
     return 0;
 }
@@ -1452,9 +1519,11 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
         return fail(4, "bad launch settings (pool %d, grid %d, block %d)", t.pool, t.grid_blocks, t.block);
 
     std::vector<SceneView<float>> views((size_t)n_scenes);
-    bool trans = false;
-    for (int i = 0; i < n_scenes; ++i)
-        if (scenes[i]->v32.has_transmissive) trans = true;
+    int feat = 0;
+    for (int i = 0; i < n_scenes; ++i) {
+        if (scenes[i]->v32.has_transmissive) feat |= WF_TRANS;
+        if (scenes[i]->v32.fog_kernel) feat |= WF_FOG;
+    }
     // width 4: every scene whose 4-wide tree was built; width 2: binary
     auto set_width = [&](int w) {
         for (int i = 0; i < n_scenes; ++i) views[i] = with_width(scenes[i]->v32, w == 2 ? 2 : 0);
@@ -1540,7 +1609,7 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
                 hit = true;
             }
         if (!hit) {
-            rc = autotune(b, mj, max_depth, trans, &t);
+            rc = autotune(b, mj, max_depth, feat, &t);
             if (rc) return rc;
             // node width: time both layouts at the chosen launch settings
             if (t.bvh_width == 0) {
@@ -1552,7 +1621,7 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
                     HIP_TRY(hipMemcpyAsync(b->tab, views.data(), views.size() * sizeof(SceneView<float>),
                                            hipMemcpyHostToDevice, b->stream));
                     double r;
-                    rc = batch_trial(b, mj, max_depth, trans, t, trial, &r);
+                    rc = batch_trial(b, mj, max_depth, feat, t, trial, &r);
                     if (rc) return rc;
                     if (r > best) {
                         best = r;
@@ -1575,7 +1644,7 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     HIP_TRY(hipMemsetAsync(b->d_rays, 0, sizeof(unsigned long long), b->stream));
     HIP_TRY(wf_multi_begin(b->ws, mj, b->stream));
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    rc = batch_run(b, mj, max_depth, trans, t, -1);
+    rc = batch_run(b, mj, max_depth, feat, t, -1);
     if (rc) return rc;
     HIP_TRY(wf_multi_end(mj, lo, b->stream));
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));

@@ -102,12 +102,15 @@ hipError_t wf_enqueue_batch(const SceneView<float>& sc, const CamView<float>& ca
 hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st);
 hipError_t wf_enqueue_end(const WfState& ws, int npix, int spp, const LaunchOut& out, hipStream_t st);
 
+// Kernel variants a batch needs (bit set): some scene has a transmissive
+// material (shadow rays walk through it) / has fog or marched water.
+enum { WF_TRANS = 1, WF_FOG = 2 };
+
 // Batched engine: one batch of `pool` paths drawn from all jobs, bounce
-// sequence to the deepest job's depth. `trans`: some scene of the batch has a
-// transmissive material (shadow rays walk through it).
+// sequence to the deepest job's depth. `feat`: WF_* bits over the batch's scenes.
 hipError_t wf_multi_begin(const WfState& ws, const WfMulti& mj, hipStream_t st);
 hipError_t wf_multi_batch(const WfMulti& mj, const WfState& ws, int pool, int max_depth,
-                          unsigned long long* rays, int grid, int block, bool trans, hipStream_t st,
+                          unsigned long long* rays, int grid, int block, int feat, hipStream_t st,
                           KernelClock* kc = nullptr);
 hipError_t wf_multi_end(const WfMulti& mj, const LaunchOut& out, hipStream_t st);
 
@@ -115,7 +118,7 @@ hipError_t wf_multi_end(const WfMulti& mj, const LaunchOut& out, hipStream_t st)
 // shade + shadow + finalize on extension queue `which`.
 hipError_t wf_stream_init(const WfState& ws, int pool, hipStream_t st);
 hipError_t wf_stream_step(const WfMulti& mj, const WfState& ws, int which, unsigned long long* rays, int grid,
-                          int block, bool trans, hipStream_t st, KernelClock* kc = nullptr);
+                          int block, int feat, hipStream_t st, KernelClock* kc = nullptr);
 
 // Whitted: persistent grid of `threads` lanes, each with `cap` stack items
 // of `item_bytes` at stack + lane * cap * item_bytes.

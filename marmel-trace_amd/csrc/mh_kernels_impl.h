@@ -27,7 +27,7 @@ __device__ __forceinline__ void count_rays(unsigned long long* ctr, int nq)
 // One thread per pixel, 16x16-pixel tiles per 256-thread block; the thread
 // loops over the pixel's samples in order (same summation order as the
 // reference pixel loop), then writes the mean and its 8-bit tone map.
-template <typename R>
+template <typename R, bool FOG>
 __global__ __launch_bounds__(256) void k_pathtrace(SceneView<R> sc, CamView<R> cam, int w, int h, int spp, int s0,
                                                    int depth, int tiles_x, int ntiles, LaunchOut out)
 {
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_pathtrace(SceneView<R> sc, CamView<R> c
                     V3<R> pd = o + d * cam.dome;
                     if (pd.y < sc.water_level) o = pd - mk(R(0), R(1e-3), R(0));
                 }
-                acc = acc + pt_radiance(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq);
+                acc = acc + pt_radiance<FOG>(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq);
             }
             acc = acc * (R(1) / (R)spp);
             size_t p = ((size_t)y * (size_t)w + (size_t)x) * 3;
@@ -175,8 +175,10 @@ hipError_t launch_pathtrace(const SceneView<R>& sc, const CamView<R>& cam, int w
     int tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16;
     int ntiles = tiles_x * tiles_y;
     int nblocks = (ntiles + 7) / 8 * 8;
-    hipLaunchKernelGGL(k_pathtrace<R>, dim3(nblocks), dim3(256), 0, st, sc, cam, w, h, spp, s0, depth, tiles_x, ntiles,
-                       out);
+    // the fog variant only for scenes that need it: the fog code raises the
+    // kernel's register count
+    auto kern = sc.fog_kernel ? k_pathtrace<R, true> : k_pathtrace<R, false>;
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, st, sc, cam, w, h, spp, s0, depth, tiles_x, ntiles, out);
     return hipGetLastError();
 }
 

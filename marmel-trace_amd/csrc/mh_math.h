@@ -172,4 +172,67 @@ template <typename R> __device__ R fbm2(R x, R y, int octaves, uint32_t seed)
     return clampr(sum / norm, R(-1), R(1));
 }
 
+// ---- 3D gradient noise and fBm (reference src/noise.c:189-253, 296-317)
+
+__device__ __forceinline__ uint32_t lattice_hash3(int x, int y, int z, uint32_t seed)
+{
+    uint32_t h = seed + 0x9e3779b9u;
+    h ^= (uint32_t)x * 0x85ebca6bu;
+    h ^= (uint32_t)y * 0xc2b2ae35u;
+    h ^= (uint32_t)z * 0x27d4eb2fu;
+    return hash_u32(h);
+}
+
+template <typename R> __device__ __forceinline__ R lattice_signed3(int x, int y, int z, uint32_t seed)
+{
+    return (R)lattice_hash3(x, y, z, seed) / (R)4294967295.0 * R(2) - R(1);
+}
+
+// Per corner a gradient of three signed lattice values (seed, seed ^
+// 0x68bc21eb, seed ^ 0x02e5be93), normalised (the x axis when degenerate),
+// dotted with the corner offset; quintic trilinear blend, x then y then z;
+// scaled by 2/sqrt(3) and clamped.
+template <typename R> __device__ R perlin3(R x, R y, R z, uint32_t seed)
+{
+    R fx = mfloor(x), fy = mfloor(y), fz = mfloor(z);
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    R rx = x - fx, ry = y - fy, rz = z - fz;
+    R dv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+        R gx = lattice_signed3<R>(ix + ox, iy + oy, iz + oz, seed);
+        R gy = lattice_signed3<R>(ix + ox, iy + oy, iz + oz, seed ^ 0x68bc21ebu);
+        R gz = lattice_signed3<R>(ix + ox, iy + oy, iz + oz, seed ^ 0x02e5be93u);
+        R len = msqrt(gx * gx + gy * gy + gz * gz);
+        if (len < R(1e-12)) {
+            gx = R(1);
+            gy = R(0);
+            gz = R(0);
+        } else {
+            gx /= len;
+            gy /= len;
+            gz /= len;
+        }
+        dv[c] = gx * (rx - (R)ox) + gy * (ry - (R)oy) + gz * (rz - (R)oz);
+    }
+    R u = fade5(rx), v = fade5(ry), w = fade5(rz);
+    R a = mixs(mixs(dv[0], dv[1], u), mixs(dv[2], dv[3], u), v);
+    R b = mixs(mixs(dv[4], dv[5], u), mixs(dv[6], dv[7], u), v);
+    return clampr(mixs(a, b, w) * R(1.1547005383792515), R(-1), R(1));
+}
+
+// 3 octaves, lacunarity 2, gain 0.5 (the fog turbulence).
+template <typename R> __device__ R fbm3(R x, R y, R z, uint32_t seed)
+{
+    R amp = R(1), freq = R(1), sum = R(0), norm = R(0);
+    for (int i = 0; i < 3; ++i) {
+        sum += amp * perlin3<R>(x * freq, y * freq, z * freq, seed);
+        norm += amp;
+        amp *= R(0.5);
+        freq *= R(2);
+    }
+    return clampr(sum / norm, R(-1), R(1));
+}
+
 } // namespace mh

@@ -222,11 +222,189 @@ __device__ V3<R> apply_medium(const DMat<R>& m, R dist, V3<R>* beta, V3<R>* L, V
     return add;
 }
 
+// ---- fog (reference src/material.c:963-1039, src/pathtrace.c:614-794) --------
+
+// Henyey-Greenstein lobe (1 - g^2) / (1 + g^2 - 2 g cos)^1.5, denominator
+// floored at 1e-4.
+template <typename R> __device__ __forceinline__ R hg_lobe(R g, R c)
+{
+    R den = R(1) + g * g - R(2) * g * c;
+    if (den < R(1e-4)) den = R(1e-4);
+    return (R(1) - g * g) / (den * msqrt(den));
+}
+
+// Density multiplier of the fog turbulence at p: 1 - a + 2 a clamp01((fbm3 + 1) / 2).
+template <typename R> __device__ __forceinline__ R fog_turb(const DFog<R>& f, V3<R> p)
+{
+    if (!f.noise) return R(1);
+    R n = fbm3<R>(p.x * f.noise_scale, p.y * f.noise_scale, p.z * f.noise_scale, 1337u);
+    R t = R(0.5) * (n + R(1));
+    if (t < R(0)) t = R(0);
+    if (t > R(1)) t = R(1);
+    return (R(1) - f.noise_amount) + f.noise_amount * t * R(2);
+}
+
+// Closed-form segment [o, o + dist d] (dist >= 1e20: to infinity) through
+// exponential height fog: transmittance T and in-scatter
+// (colour + sun HG strength) (1 - T), the optical depth scaled by the
+// turbulence at one point. dsun (optional): d(in-scatter)/d(sun colour).
+template <typename R>
+__device__ void fog_segment(const DFog<R>& f, const DSky<R>& sky, V3<R> o, V3<R> d, R dist, R* T, V3<R>* ins,
+                            V3<R>* dsun)
+{
+    *T = R(1);
+    *ins = zero3<R>();
+    if (dsun) *dsun = zero3<R>();
+    if (f.density <= R(0) || dist <= R(1e-7)) return;
+    R lam = f.falloff > R(0) ? f.falloff : R(0);
+    R y0 = o.y - f.height, dy = d.y, tau;
+    bool to_sky = dist >= R(1e20);
+    if (lam <= R(1e-6)) {
+        tau = to_sky ? R(1e6) : f.density * dist;
+    } else {
+        R base = f.density * mexp(-lam * y0);
+        if (to_sky)
+            tau = dy > R(1e-5) ? base / (lam * dy) : R(1e6);
+        else if (mfabs(dy) < R(1e-5))
+            tau = base * dist;
+        else
+            tau = base * ((R(1) - mexp(-lam * dy * dist)) / (lam * dy));
+    }
+    if (tau < R(0)) tau = R(0);
+    R ev = to_sky ? R(20) : (dist > R(50) ? R(50) : dist * R(0.5));
+    tau *= fog_turb(f, o + d * ev);
+    R tr = mexp(-tau);
+    if (tr < R(0)) tr = R(0);
+    if (tr > R(1)) tr = R(1);
+    V3<R> col = f.color;
+    R ph = R(0);
+    if (f.inscatter > R(0)) {
+        ph = hg_lobe(f.g, dot(d, sky.sun_dir));
+        col = col + sky.sun_color * (ph * f.inscatter);
+    }
+    *T = tr;
+    *ins = col * (R(1) - tr);
+    if (dsun) {
+        R k = ph * f.inscatter * (R(1) - tr);
+        *dsun = mk(k, k, k);
+    }
+}
+
+// Camera-ray segment: shadow_steps (clamped to [4, 128]) jittered steps, each
+// lit by the sun unless its shadow ray (tmin 0.05, tmax 100) is blocked; HG
+// clamped at 6, mixed 0.7 / 0.3 with isotropic. The march covers at most 60
+// units (to infinity: up to the fog layer's top, at most 40); the rest is the
+// closed form.
+template <typename R, typename TS>
+__device__ void pt_fog_segment(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R dist, uint32_t seed, int b,
+                               R* T, V3<R>* ins, V3<R>* dsun, int* nq)
+{
+    const DFog<R>& f = sc.fog;
+    if (f.steps <= 0) {
+        fog_segment(f, sc.sky, o, d, dist, T, ins, dsun);
+        return;
+    }
+    int steps = f.steps < 4 ? 4 : (f.steps > 128 ? 128 : f.steps);
+    bool to_sky = dist >= R(1e20);
+    R lam = f.falloff, max_t;
+    if (to_sky) {
+        R top = lam > R(1e-4) ? f.height + R(3) / lam : R(25);
+        if (o.y >= top && d.y >= R(0)) {
+            *T = R(1);
+            *ins = zero3<R>();
+            if (dsun) *dsun = zero3<R>();
+            return;
+        }
+        if (d.y > R(0.02)) {
+            max_t = (top - o.y) / d.y;
+            if (max_t > R(40)) max_t = R(40);
+        } else {
+            max_t = R(40);
+        }
+    } else {
+        max_t = dist > R(60) ? R(60) : dist;
+    }
+    R dt = max_t / (R)steps;
+    R ph = hg_lobe(f.g, dot(d, sc.sky.sun_dir));
+    if (ph > R(6)) ph = R(6);
+    R sk = (R(0.70) * ph + R(0.30)) * f.inscatter;
+    V3<R> glow = sc.sky.sun_color * sk;
+    R Ta = R(1);
+    V3<R> acc = zero3<R>(), dacc = zero3<R>();
+    R jit = rand01<R>(seed, (uint32_t)b, 0xFE01u);
+    for (int i = 0; i < steps; ++i) {
+        V3<R> p = o + d * (((R)i + jit) * dt);
+        R rho = f.density;
+        if (lam > R(1e-6)) rho *= mexp(-lam * (p.y - f.height));
+        rho *= fog_turb(f, p);
+        if (rho < R(0)) rho = R(0);
+        R sT = mexp(-(rho * dt));
+        V3<R> att;
+        bool shadowed = occluded_segment(sc, ts, p, sc.sky.sun_dir, R(0.05), R(100), &att, nq);
+        V3<R> col = f.color;
+        if (!shadowed) col = col + vmul(glow, att);
+        acc = acc + (col * (R(1) - sT)) * Ta;
+        if (dsun && !shadowed) dacc = dacc + (att * (sk * (R(1) - sT))) * Ta;
+        Ta *= sT;
+        if (Ta < R(1e-4)) break;
+    }
+    if (dist > max_t) {
+        R rT;
+        V3<R> rins, rd;
+        fog_segment(f, sc.sky, o + d * max_t, d, to_sky ? R(1e30) : dist - max_t, &rT, &rins, dsun ? &rd : nullptr);
+        acc = acc + rins * Ta;
+        if (dsun) dacc = dacc + rd * Ta;
+        Ta *= rT;
+    }
+    *T = Ta;
+    *ins = acc;
+    if (dsun) *dsun = dacc;
+}
+
+// Segment inside medium m. Water with fog steps > 0 is marched in steps
+// (clamped to [4, 16]) jittered steps: Beer-Lambert exp(-absorption dt) per
+// step and in-scatter (0.4 deep colour + sun HG(0.72, clamped at 4.5) 0.18
+// where the shadow ray (tmin 0.02, tmax 100) is clear) (1 - T); other media
+// take the closed-form blend. Returns the radiance added to L; beta2 takes
+// the same transmittance; dsun (optional) accumulates d(added)/d(sun colour).
+template <typename R, typename TS>
+__device__ V3<R> pt_water_segment(const SceneView<R>& sc, TS ts, const DMat<R>& m, V3<R> o, V3<R> d, R dist,
+                                  uint32_t seed, int b, V3<R>* beta, V3<R>* L, V3<R>* beta2, V3<R>* dsun, int* nq)
+{
+    if (!m.is_water || sc.fog.steps <= 0 || dist <= R(1e-4)) return apply_medium(m, dist, beta, L, beta2);
+    int steps = sc.fog.steps < 4 ? 4 : (sc.fog.steps > 16 ? 16 : sc.fog.steps);
+    R dt = dist / (R)steps;
+    R ph = hg_lobe(R(0.72), dot(d, sc.sky.sun_dir));
+    if (ph > R(4.5)) ph = R(4.5);
+    R jit = rand01<R>(seed, (uint32_t)b, 0xAC31u);
+    V3<R> sT = mk(mexp(-m.absorption.x * dt), mexp(-m.absorption.y * dt), mexp(-m.absorption.z * dt));
+    V3<R> om = mk(R(1) - sT.x, R(1) - sT.y, R(1) - sT.z);
+    V3<R> amb = m.deep_color * R(0.40), added = zero3<R>();
+    for (int i = 0; i < steps; ++i) {
+        V3<R> p = o + d * (((R)i + jit) * dt);
+        V3<R> att;
+        bool shadowed = occluded_segment(sc, ts, p, sc.sky.sun_dir, R(0.02), R(100), &att, nq);
+        V3<R> col = amb;
+        if (!shadowed) col = col + vmul(sc.sky.sun_color, att) * (ph * R(0.18));
+        V3<R> t = vmul(*beta, vmul(col, om));
+        *L = *L + t;
+        added = added + t;
+        if (dsun && !shadowed) *dsun = *dsun + vmul(*beta, vmul(att * (ph * R(0.18)), om));
+        *beta = vmul(*beta, sT);
+        if (beta2) *beta2 = vmul(*beta2, sT);
+    }
+    return added;
+}
+
 // Observer of one path's radiance terms, for the path-replay adjoint. The
 // forward renderers use NoHook, which compiles away. Every term added to L
 // is reported once, in order, through exactly one of these calls.
 struct NoHook {
+    // kGrad: the hook wants sun-colour derivatives of fog in-scatter (sun_lin)
+    static constexpr bool kGrad = false;
     template <typename R> __device__ void add(V3<R>) {}
+    // a term of L that is linear in the sun colour: d(term)/d(sun colour)
+    template <typename R> __device__ void sun_lin(V3<R>) {}
     template <typename R> __device__ void sky(V3<R>, V3<R>, V3<R>) {}
     template <typename R> __device__ void emission(V3<R>, int, V3<R>) {}
     template <typename R>
@@ -243,7 +421,8 @@ struct NoHook {
     template <typename R> __device__ const DMat<R>* frozen(int) const { return nullptr; }
 };
 
-template <typename R, typename TS, typename HK>
+// FOG: compile the fog and marched-water code (scenes with fog_kernel set).
+template <bool FOG, typename R, typename TS, typename HK>
 __device__ V3<R> pt_radiance_h(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0, int max_depth, uint32_t seed,
                                int* nq, HK& hk)
 {
@@ -259,15 +438,56 @@ __device__ V3<R> pt_radiance_h(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd
         ++*nq;
         if (!intersect(sc, ts, o, d, R(1e-4), R(1e30), &h)) {
             if (medium >= 0) {
-                hk.add(apply_medium(sc.mats[medium], R(5), &beta, &L, &beta_f));
+                if (FOG) {
+                    V3<R> dw = zero3<R>();
+                    hk.add(pt_water_segment(sc, ts, sc.mats[medium], o, d, R(5), seed, b, &beta, &L, &beta_f,
+                                            HK::kGrad ? &dw : nullptr, nq));
+                    if (HK::kGrad) hk.sun_lin(dw);
+                } else {
+                    hk.add(apply_medium(sc.mats[medium], R(5), &beta, &L, &beta_f));
+                }
                 medium = -1;
             }
-            V3<R> t = vmul(beta, sky_sample(d, sc.sky));
+            V3<R> skyc = sky_sample(d, sc.sky), bs = beta;
+            if (FOG && sc.fog.density > R(0)) {
+                R T;
+                V3<R> ins, dfs;
+                if (b == 0)
+                    pt_fog_segment(sc, ts, o, d, R(1e30), seed, b, &T, &ins, HK::kGrad ? &dfs : nullptr, nq);
+                else
+                    fog_segment(sc.fog, sc.sky, o, d, R(1e30), &T, &ins, HK::kGrad ? &dfs : nullptr);
+                skyc = skyc * T + ins;
+                bs = beta * T;
+                if (HK::kGrad) hk.sun_lin(vmul(beta, dfs));
+            }
+            V3<R> t = vmul(beta, skyc);
             L = L + t;
-            hk.sky(beta, d, t);
+            hk.sky(bs, d, t);
             break;
         }
-        if (medium >= 0) hk.add(apply_medium(sc.mats[medium], h.t, &beta, &L, &beta_f));
+        if (medium >= 0) {
+            if (FOG) {
+                V3<R> dw = zero3<R>();
+                hk.add(pt_water_segment(sc, ts, sc.mats[medium], o, d, h.t, seed, b, &beta, &L, &beta_f,
+                                        HK::kGrad ? &dw : nullptr, nq));
+                if (HK::kGrad) hk.sun_lin(dw);
+            } else {
+                hk.add(apply_medium(sc.mats[medium], h.t, &beta, &L, &beta_f));
+            }
+        } else if (FOG && sc.fog.density > R(0)) {
+            R T;
+            V3<R> ins, dfs;
+            if (b == 0)
+                pt_fog_segment(sc, ts, o, d, h.t, seed, b, &T, &ins, HK::kGrad ? &dfs : nullptr, nq);
+            else
+                fog_segment(sc.fog, sc.sky, o, d, h.t, &T, &ins, HK::kGrad ? &dfs : nullptr);
+            V3<R> t = vmul(beta, ins);
+            L = L + t;
+            hk.add(t);
+            if (HK::kGrad) hk.sun_lin(vmul(beta, dfs));
+            beta = beta * T;
+            beta_f = beta_f * T;
+        }
         if (h.mat < 0 || h.mat >= sc.n_mats) {
             V3<R> t = vmul(beta, sky_sample(d, sc.sky));
             L = L + t;
@@ -396,12 +616,12 @@ __device__ V3<R> pt_radiance_h(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd
     return L;
 }
 
-template <typename R, typename TS>
+template <bool FOG, typename R, typename TS>
 __device__ V3<R> pt_radiance(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd0, int max_depth, uint32_t seed,
                              int* nq)
 {
     NoHook hk;
-    return pt_radiance_h(sc, ts, ro, rd0, max_depth, seed, nq, hk);
+    return pt_radiance_h<FOG>(sc, ts, ro, rd0, max_depth, seed, nq, hk);
 }
 
 template <typename R>
@@ -628,6 +848,15 @@ __device__ V3<R> whitted_sample(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d,
             medium_terms(mm, hit ? h.t : R(5), &T, &deep);
             acc = acc + vmul(it.w, deep);
             it.w = vmul(it.w, T);
+        }
+        if (it.medium == -1 && sc.fog.density > R(0)) {
+            // the root and reflected rays carry fog over their segment
+            // (refracted rays are shaded without it)
+            R T;
+            V3<R> ins;
+            fog_segment(sc.fog, sc.sky, it.o, it.d, hit ? h.t : R(1e30), &T, &ins, (V3<R>*)nullptr);
+            acc = acc + vmul(it.w, ins);
+            it.w = it.w * T;
         }
         if (!hit) {
             acc = acc + vmul(it.w, sky_sample(it.d, sc.sky));

@@ -87,6 +87,20 @@ template <typename R> struct DSky {
     R sun_omega;       // solid angle used by sun NEE (1 for a point sun)
 };
 
+// Atmospheric fog. density > 0 turns it on; steps > 0 marches camera-ray
+// segments (and underwater segments, fog or not) with sun shadow rays.
+template <typename R> struct DFog {
+    R density;
+    V3<R> color;
+    R height;
+    R falloff;        // as given (the closed form uses max(falloff, 0))
+    R inscatter;
+    R g;              // Henyey-Greenstein anisotropy clamped to [-0.95, 0.95]
+    R noise_scale, noise_amount;
+    int steps;        // shadow_steps as given
+    int noise;        // noise_amount > 1e-4 && noise_scale > 1e-4
+};
+
 template <typename R> struct DLight {
     int prim;
     int mat;
@@ -114,6 +128,10 @@ template <typename R> struct SceneView {
     const DMat<R>* mats;
     int n_mats;
     DSky<R> sky;
+    DFog<R> fog;
+    // 1 when fog is on or marched underwater segments can occur: renders
+    // launch the kernel variants that carry the fog code
+    int fog_kernel;
     R water_level;
     int water_material;
     // 0 when no material is water or has transparency > 0.8: a shadow

@@ -239,10 +239,13 @@ __device__ __forceinline__ V3<float> resolve_pending(const WfState& st, uint32_t
     return L;
 }
 
-template <bool MULTI>
+template <bool MULTI, bool FOG>
 __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState st, WfJob job, int which, int bounce,
                                                    unsigned long long* rays, WfMulti mj)
 {
+    // FOG: fog segments and marched water; the marches trace their sun
+    // shadow rays here, with this stack
+    MH_STACK_PRIV(ts)
     uint32_t n = st.cnt[which];
     const uint32_t* qin = st.q[which];
     uint32_t* qout = st.q[which ^ 1];
@@ -272,12 +275,43 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
             if (meta == 0xffffffffu) {
                 // degenerate primary ray: zero radiance
             } else if (rec == INT_MIN) {
-                if (medium >= 0) apply_medium(sc.mats[medium], 5.f, &beta, &L);
-                L = L + vmul(beta, sky_sample(d, sc.sky));
+                if (medium >= 0) {
+                    if (FOG)
+                        pt_water_segment(sc, ts, sc.mats[medium], o, d, 5.f, seed, b_idx, &beta, &L,
+                                         (V3<float>*)nullptr, (V3<float>*)nullptr, &nq);
+                    else
+                        apply_medium(sc.mats[medium], 5.f, &beta, &L);
+                }
+                V3<float> skyc = sky_sample(d, sc.sky);
+                if (FOG && sc.fog.density > 0.f) {
+                    float T;
+                    V3<float> ins;
+                    if (b_idx == 0)
+                        pt_fog_segment(sc, ts, o, d, 1e30f, seed, 0, &T, &ins, (V3<float>*)nullptr, &nq);
+                    else
+                        fog_segment(sc.fog, sc.sky, o, d, 1e30f, &T, &ins, (V3<float>*)nullptr);
+                    skyc = skyc * T + ins;
+                }
+                L = L + vmul(beta, skyc);
             } else {
                 DHit<float> h;
                 finish_hit(sc, rec, o, d, 1e-4f, hh.x, &h);
-                if (medium >= 0) apply_medium(sc.mats[medium], h.t, &beta, &L);
+                if (medium >= 0) {
+                    if (FOG)
+                        pt_water_segment(sc, ts, sc.mats[medium], o, d, h.t, seed, b_idx, &beta, &L,
+                                         (V3<float>*)nullptr, (V3<float>*)nullptr, &nq);
+                    else
+                        apply_medium(sc.mats[medium], h.t, &beta, &L);
+                } else if (FOG && sc.fog.density > 0.f) {
+                    float T;
+                    V3<float> ins;
+                    if (b_idx == 0)
+                        pt_fog_segment(sc, ts, o, d, h.t, seed, 0, &T, &ins, (V3<float>*)nullptr, &nq);
+                    else
+                        fog_segment(sc.fog, sc.sky, o, d, h.t, &T, &ins, (V3<float>*)nullptr);
+                    L = L + vmul(beta, ins);
+                    beta = beta * T;
+                }
                 if (h.mat < 0 || h.mat >= sc.n_mats) {
                     L = L + vmul(beta, sky_sample(d, sc.sky));
                 } else {

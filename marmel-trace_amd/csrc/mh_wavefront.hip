@@ -36,16 +36,17 @@ hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st)
 template <bool MULTI>
 static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<float>& cam, const WfJob& job,
                                   const WfState& ws, const WfMulti& mj, unsigned long long* rays, int grid, int block,
-                                  bool trans, hipStream_t st, KernelClock* kc)
+                                  int feat, hipStream_t st, KernelClock* kc)
 {
-    auto shadow = trans ? k_wf_shadow<MULTI, true> : k_wf_shadow<MULTI, false>;
+    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<MULTI, true> : k_wf_shadow<MULTI, false>;
+    auto shade = (feat & WF_FOG) ? k_wf_shade<MULTI, true> : k_wf_shade<MULTI, false>;
     dim3 G(grid), B(block);
     MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_gen<MULTI>, G, B, 0, st, sc, cam, job, ws, mj));
     for (int b = 0; b <= job.depth; ++b) {
         int which = b & 1;
         MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, ws.cnt, which ^ 1));
         MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(k_wf_extend<MULTI>, G, B, 0, st, sc, ws, which, mj));
-        MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(k_wf_shade<MULTI>, G, B, 0, st, sc, ws, job, which, b, rays, mj));
+        MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, 0, st, sc, ws, job, which, b, rays, mj));
         MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
         MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MULTI>, G, B, 0, st, ws, mj));
     }
@@ -57,7 +58,8 @@ hipError_t wf_enqueue_batch(const SceneView<float>& sc, const CamView<float>& ca
                             const WfState& ws, unsigned long long* rays, int grid, hipStream_t st, KernelClock* kc)
 {
     WfMulti none{};
-    return enqueue_bounces<false>(sc, cam, job, ws, none, rays, grid, 256, sc.has_transmissive != 0, st, kc);
+    int feat = (sc.has_transmissive ? WF_TRANS : 0) | (sc.fog_kernel ? WF_FOG : 0);
+    return enqueue_bounces<false>(sc, cam, job, ws, none, rays, grid, 256, feat, st, kc);
 }
 
 hipError_t wf_multi_begin(const WfState& ws, const WfMulti& mj, hipStream_t st)
@@ -71,14 +73,14 @@ hipError_t wf_multi_begin(const WfState& ws, const WfMulti& mj, hipStream_t st)
 }
 
 hipError_t wf_multi_batch(const WfMulti& mj, const WfState& ws, int pool, int max_depth, unsigned long long* rays,
-                          int grid, int block, bool trans, hipStream_t st, KernelClock* kc)
+                          int grid, int block, int feat, hipStream_t st, KernelClock* kc)
 {
     SceneView<float> sc{};
     CamView<float> cam{};
     WfJob job{};
     job.depth = max_depth;
     job.pool = pool;
-    return enqueue_bounces<true>(sc, cam, job, ws, mj, rays, grid, block, trans, st, kc);
+    return enqueue_bounces<true>(sc, cam, job, ws, mj, rays, grid, block, feat, st, kc);
 }
 
 hipError_t wf_stream_init(const WfState& ws, int pool, hipStream_t st)
@@ -90,17 +92,18 @@ hipError_t wf_stream_init(const WfState& ws, int pool, hipStream_t st)
 }
 
 hipError_t wf_stream_step(const WfMulti& mj, const WfState& ws, int which, unsigned long long* rays, int grid,
-                          int block, bool trans, hipStream_t st, KernelClock* kc)
+                          int block, int feat, hipStream_t st, KernelClock* kc)
 {
     dim3 G(grid), B(block);
     SceneView<float> sc{};
     WfJob job{};
-    auto shadow = trans ? k_wf_shadow<true, true> : k_wf_shadow<true, false>;
+    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<true, true> : k_wf_shadow<true, false>;
+    auto shade = (feat & WF_FOG) ? k_wf_shade<true, true> : k_wf_shade<true, false>;
     MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_refill, G, B, 0, st, ws, mj, which));
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_refill_commit, dim3(1), dim3(64), 0, st, ws, mj));
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, ws.cnt, which ^ 1));
     MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(k_wf_extend<true>, G, B, 0, st, sc, ws, which, mj));
-    MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(k_wf_shade<true>, G, B, 0, st, sc, ws, job, which, 0, rays, mj));
+    MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, 0, st, sc, ws, job, which, 0, rays, mj));
     MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
     MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<true>, G, B, 0, st, ws, mj));
     return hipGetLastError();

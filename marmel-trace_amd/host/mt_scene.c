@@ -41,6 +41,7 @@ struct mt_scene {
     mh_material *mats;
     int nm, capm;
     mh_sky sky;
+    mh_fog fog;
     double water_level;
     int water_material;
     /* BVH */
@@ -70,6 +71,7 @@ mt_scene *mt_scene_new(void)
     mt_scene *s = (mt_scene *)calloc(1, sizeof(mt_scene));
     if (!s) return NULL;
     mt_sky_default(&s->sky);
+    mt_fog_default(&s->fog);
     s->water_material = -1;
     return s;
 }
@@ -212,6 +214,29 @@ int mt_scene_add_prims(mt_scene *s, const mh_prim *prims, const mh_prim_attr *at
 void mt_scene_set_sky(mt_scene *s, const mh_sky *sky)
 {
     if (s && sky) s->sky = *sky;
+}
+
+/* Reference fog_default_params (src/material.h:189-197): disabled. */
+void mt_fog_default(mh_fog *f)
+{
+    if (!f) return;
+    memset(f, 0, sizeof *f);
+    f->density = 0.0;
+    f->color.x = 0.70;
+    f->color.y = 0.75;
+    f->color.z = 0.80;
+    f->height = 0.0;
+    f->height_falloff = 0.0;
+    f->inscatter_strength = 0.50;
+    f->sun_anisotropy = 0.70;
+    f->noise_scale = 0.0;
+    f->noise_amount = 0.0;
+    f->shadow_steps = 0;
+}
+
+void mt_scene_set_fog(mt_scene *s, const mh_fog *fog)
+{
+    if (s && fog) s->fog = *fog;
 }
 
 void mt_scene_set_water(mt_scene *s, double level, int material)
@@ -583,6 +608,7 @@ int mt_scene_soa(const mt_scene *s, mh_scene_soa *out)
     out->sky = s->sky;
     out->water_level = s->water_level;
     out->water_material = s->water_material;
+    out->fog = s->fog;
     out->n_nodes = s->n_nodes;
     out->nodes = s->nodes;
     out->n_leaf_prims = s->n_leaf;

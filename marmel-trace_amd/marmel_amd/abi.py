@@ -8,7 +8,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 PRIM_SPHERE, PRIM_PLANE, PRIM_BOX, PRIM_TRIANGLE, PRIM_CYLINDER = 0, 1, 2, 3, 4
 PRIM_SMOOTH, PRIM_UV = 1, 2
@@ -62,6 +62,20 @@ class Sky(C.Structure):
                 ("fog_density", C.c_double)]
 
 
+class Fog(C.Structure):
+    _fields_ = [("density", C.c_double), ("color", Vec3), ("height", C.c_double),
+                ("height_falloff", C.c_double), ("inscatter_strength", C.c_double),
+                ("sun_anisotropy", C.c_double), ("noise_scale", C.c_double),
+                ("noise_amount", C.c_double), ("shadow_steps", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+def fog_default():
+    """Reference fog_default_params: disabled."""
+    return Fog(density=0.0, color=Vec3(0.70, 0.75, 0.80), inscatter_strength=0.50,
+               sun_anisotropy=0.70)
+
+
 class Camera(C.Structure):
     _fields_ = [("position", Vec3), ("lower_left", Vec3), ("horizontal", Vec3),
                 ("vertical", Vec3), ("aperture", C.c_double),
@@ -81,7 +95,7 @@ class SceneSoa(C.Structure):
                 ("sky", Sky), ("water_level", C.c_double), ("water_material", C.c_int32),
                 ("n_nodes", C.c_int32), ("nodes", C.c_void_p),
                 ("n_leaf_prims", C.c_int32), ("leaf_prims", C.c_void_p),
-                ("n_planes", C.c_int32), ("planes", C.c_void_p)]
+                ("n_planes", C.c_int32), ("planes", C.c_void_p), ("fog", Fog)]
 
 
 class Ray(C.Structure):

@@ -53,6 +53,7 @@ def host():
     _sig(L, "mt_scene_add_prims", C.c_int, [P, P, P, C.c_int])
     _sig(L, "mt_scene_set_sky", None, [P, C.POINTER(abi.Sky)])
     _sig(L, "mt_scene_set_water", None, [P, C.c_double, C.c_int])
+    _sig(L, "mt_scene_set_fog", None, [P, C.POINTER(abi.Fog)])
     _sig(L, "mt_scene_counts", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)])
     _sig(L, "mt_scene_build_bvh", C.c_int, [P, C.POINTER(BvhOptions), C.POINTER(BvhStats)])
     _sig(L, "mt_scene_soa", C.c_int, [P, C.POINTER(abi.SceneSoa)])
@@ -89,6 +90,7 @@ def hip():
     _sig(L, "mh_framebuffer_download", C.c_int, [P, P, P])
     _sig(L, "mh_scene_set_materials", C.c_int, [P, P, C.c_int])
     _sig(L, "mh_scene_set_sky", C.c_int, [P, C.POINTER(abi.Sky)])
+    _sig(L, "mh_scene_set_fog", C.c_int, [P, C.POINTER(abi.Fog)])
     _sig(L, "mh_adjoint", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P, P])
     _sig(L, "mh_render_frozen", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P,
                                           C.c_int, P, P])

@@ -88,6 +88,10 @@ class HostScene:
     def set_water(self, level, material):
         self._L.mt_scene_set_water(self._h, level, material)
 
+    def set_fog(self, fog):
+        """Scene fog (abi.Fog; density 0 disables it)."""
+        self._L.mt_scene_set_fog(self._h, C.byref(fog))
+
     def counts(self):
         n, m = C.c_int(), C.c_int()
         self._L.mt_scene_counts(self._h, C.byref(n), C.byref(m))
@@ -242,6 +246,10 @@ class GpuScene:
 
     def set_sky(self, sky):
         self._check(self._L.mh_scene_set_sky(self._h, C.byref(sky)))
+
+    def set_fog(self, fog):
+        """Replace the uploaded scene's fog (abi.Fog)."""
+        self._check(self._L.mh_scene_set_fog(self._h, C.byref(fog)))
 
     def render_frozen(self, cam, width, height, spp, depth, frozen_mats, sample_offset=0):
         """fp32 path tracer with sampling decisions taken from frozen_mats; returns linear (h, w, 3)."""

@@ -93,6 +93,7 @@ typedef struct or_scene {
     int n_mats;
     mh_material *mats;
     mh_sky sky;
+    mh_fog fog;
     double water_level;
     int water_material;
     int has_transmissive;   /* some material is water or transparency > 0.8 */
@@ -132,6 +133,8 @@ unsigned or_hash3(unsigned a, unsigned b, unsigned c);
 double or_rand01(unsigned a, unsigned b, unsigned c);
 double or_perlin2(double x, double y, unsigned seed);
 double or_fbm2(double x, double y, int octaves, double lac, double gain, unsigned seed);
+double or_perlin3(double x, double y, double z, unsigned seed);
+double or_fbm3(double x, double y, double z, int octaves, double lac, double gain, unsigned seed);
 void or_basis(dv3 n, dv3 *t, dv3 *b);
 dv3 or_cosine_hemisphere(dv3 n, double u1, double u2);
 double or_fresnel_schlick(double c, double f0);

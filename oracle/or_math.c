@@ -5,7 +5,8 @@
  * Follows:
  *   PRNG      reference src/pathtrace.c:92-117 (== src/render.c:371-396)
  *   noise     reference src/noise.c:31-61 (hash), 68-76 (fade/lerp),
- *             158-187 (perlin2), 272-294 (fbm2)
+ *             158-187 (perlin2), 272-294 (fbm2), 189-253 (perlin3),
+ *             296-317 (fbm3)
  *   sampling  reference src/sampling.c:52-281
  */
 #include "or_internal.h"
@@ -91,6 +92,56 @@ double or_fbm2(double x, double y, int octaves, double lac, double gain, unsigne
     double amp = 1.0, freq = 1.0, sum = 0.0, norm = 0.0;
     for (int i = 0; i < n; ++i) {
         sum += amp * or_perlin2(x * freq, y * freq, seed);
+        norm += amp;
+        amp *= g;
+        freq *= l;
+    }
+    if (norm <= 0.0) return 0.0;
+    return clampd(sum / norm, -1.0, 1.0);
+}
+
+/* 3D gradient noise: per corner a gradient of three signed lattice values
+ * (the seed, and the seed xor 0x68bc21eb / 0x02e5be93 for y / z),
+ * normalised (the x axis when degenerate); dot with the corner offset,
+ * trilinear quintic blend x then y then z, scale by 2/sqrt(3), clamp. */
+static double lattice_signed3(int x, int y, int z, unsigned seed)
+{
+    return (double)lattice_hash(x, y, z, seed) / (double)UINT32_MAX * 2.0 - 1.0;
+}
+
+double or_perlin3(double x, double y, double z, unsigned seed)
+{
+    double fx = floor(x), fy = floor(y), fz = floor(z);
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    double r[3] = {x - fx, y - fy, z - fz};
+    double d[8];
+    for (int c = 0; c < 8; ++c) {
+        int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+        double gx = lattice_signed3(ix + ox, iy + oy, iz + oz, seed);
+        double gy = lattice_signed3(ix + ox, iy + oy, iz + oz, seed ^ 0x68bc21ebu);
+        double gz = lattice_signed3(ix + ox, iy + oy, iz + oz, seed ^ 0x02e5be93u);
+        double len = sqrt(gx * gx + gy * gy + gz * gz);
+        if (len < 1e-12) {
+            gx = 1.0; gy = 0.0; gz = 0.0;
+        } else {
+            gx /= len; gy /= len; gz /= len;
+        }
+        d[c] = gx * (r[0] - ox) + gy * (r[1] - oy) + gz * (r[2] - oz);
+    }
+    double u = fade5(r[0]), v = fade5(r[1]), w = fade5(r[2]);
+    double a = mixd(mixd(d[0], d[1], u), mixd(d[2], d[3], u), v);
+    double b = mixd(mixd(d[4], d[5], u), mixd(d[6], d[7], u), v);
+    return clampd(mixd(a, b, w) * 1.1547005383792515, -1.0, 1.0);
+}
+
+double or_fbm3(double x, double y, double z, int octaves, double lac, double gain, unsigned seed)
+{
+    int n = octaves <= 0 ? 1 : octaves;
+    double l = lac > 0.0 ? lac : 2.0;
+    double g = gain > 0.0 ? gain : 0.5;
+    double amp = 1.0, freq = 1.0, sum = 0.0, norm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        sum += amp * or_perlin3(x * freq, y * freq, z * freq, seed);
         norm += amp;
         amp *= g;
         freq *= l;

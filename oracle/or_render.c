@@ -7,13 +7,15 @@
  *   camera                 reference src/camera.c:24-134
  *   PT lobe sampling       reference src/pathtrace.c:141-200, 220-233, 251-453
  *   PT NEE + shadow        reference src/pathtrace.c:504-606
- *   PT radiance            reference src/pathtrace.c:800-1028 (fog and the
- *                          underwater in-scatter march are off: fog density 0)
+ *   PT radiance            reference src/pathtrace.c:800-1028
+ *   fog                    reference src/material.c:963-1039 (closed-form
+ *                          segment), src/pathtrace.c:614-729 (shadow-marched
+ *                          camera segment), 737-794 (marched underwater segment)
  *   PT pixel + pool        reference src/pathtrace.c:1047-1096, 1134-1234
  *   tone map               reference src/pathtrace.c:464-480, src/render.c:892-907
  *   Whitted                reference src/render.c:507-863 (trace/trace_hit,
  *                          emissive_direct), 874-889 (sample_offset),
- *                          924-970 (render_region)
+ *                          924-970 (render_region), 842-863 (fog)
  */
 #define _POSIX_C_SOURCE 200809L
 
@@ -298,6 +300,191 @@ static int occluded_segment(const or_scene *s, oray r, double tmin, double tmax,
         return 1;
     }
     return 1;
+}
+
+/* ---- fog ------------------------------------------------------------- */
+
+#define CH_FOG 0xFE01u
+#define CH_WATER 0xAC31u
+
+/* Henyey-Greenstein lobe (1 - g^2) / (1 + g^2 - 2 g cos)^1.5 with the
+ * denominator floored at 1e-4. */
+static double hg_lobe(double g, double cosv)
+{
+    double den = 1.0 + g * g - 2.0 * g * cosv;
+    if (den < 1e-4) den = 1e-4;
+    return (1.0 - g * g) / (den * sqrt(den));
+}
+
+static double clamp_g(double g) { return g < -0.95 ? -0.95 : (g > 0.95 ? 0.95 : g); }
+
+/* Turbulence multiplier of the density at p: 1 - a + 2 a clamp01((fbm3 + 1)/2)
+ * (3 octaves, lacunarity 2, gain 0.5, seed 1337); 1 when noise is off. */
+static double fog_turb(const mh_fog *f, dv3 p)
+{
+    if (!(f->noise_amount > 1e-4 && f->noise_scale > 1e-4)) return 1.0;
+    double n = or_fbm3(p.x * f->noise_scale, p.y * f->noise_scale, p.z * f->noise_scale,
+                       3, 2.0, 0.5, 1337u);
+    double t = 0.5 * (n + 1.0);
+    if (t < 0.0) t = 0.0;
+    if (t > 1.0) t = 1.0;
+    return (1.0 - f->noise_amount) + f->noise_amount * t * 2.0;
+}
+
+/* Closed-form segment of length dist (>= 1e20: to infinity) through
+ * exponential height fog: transmittance exp(-tau) and in-scatter
+ * (colour + sun * HG * strength) (1 - T), tau scaled by the turbulence at one
+ * point of the segment. dsun (optional): d(in-scatter)/d(sun colour), per
+ * channel. */
+static void fog_segment(const mh_fog *f, const mh_sky *sky, oray r, double dist, double *T,
+                        dv3 *ins, dv3 *dsun)
+{
+    *T = 1.0;
+    *ins = d3(0.0, 0.0, 0.0);
+    if (dsun) *dsun = d3(0.0, 0.0, 0.0);
+    if (f->density <= 0.0 || dist <= 1e-7) return;
+    double lam = f->height_falloff > 0.0 ? f->height_falloff : 0.0;
+    double y0 = r.o.y - f->height, dy = r.d.y, tau;
+    int to_sky = dist >= 1e20;
+    if (lam <= 1e-6) {
+        tau = to_sky ? 1e6 : f->density * dist;
+    } else {
+        double base = f->density * exp(-lam * y0);
+        if (to_sky)
+            tau = dy > 1e-5 ? base / (lam * dy) : 1e6;
+        else if (fabs(dy) < 1e-5)
+            tau = base * dist;
+        else
+            tau = base * ((1.0 - exp(-lam * dy * dist)) / (lam * dy));
+    }
+    if (tau < 0.0) tau = 0.0;
+    double ev = to_sky ? 20.0 : (dist > 50.0 ? 50.0 : dist * 0.5);
+    tau *= fog_turb(f, dat(r.o, r.d, ev));
+    double tr = exp(-tau);
+    if (tr < 0.0) tr = 0.0;
+    if (tr > 1.0) tr = 1.0;
+    dv3 col = fromm(f->color);
+    double ph = 0.0;
+    if (f->inscatter_strength > 0.0) {
+        ph = hg_lobe(clamp_g(f->sun_anisotropy), ddot(r.d, fromm(sky->sun_dir)));
+        col = dadd(col, dscale(fromm(sky->sun_color), ph * f->inscatter_strength));
+    }
+    *T = tr;
+    *ins = dscale(col, 1.0 - tr);
+    if (dsun) {
+        double k = ph * f->inscatter_strength * (1.0 - tr);
+        *dsun = d3(k, k, k);
+    }
+}
+
+/* Camera-ray segment marched in shadow_steps (clamped to [4, 128]) jittered
+ * steps, each lit by the sun unless a shadow ray (tmin 0.05, tmax 100)
+ * is blocked; HG phase clamped at 6 and mixed 0.7 / 0.3 with isotropic. The
+ * march covers at most 60 units (to infinity: up to the top of the fog
+ * layer, at most 40); the rest of the segment is closed form. */
+static void pt_fog_segment(const or_scene *s, oray r, double dist, unsigned seed, int b,
+                           double *T, dv3 *ins, dv3 *dsun)
+{
+    const mh_fog *f = &s->fog;
+    if (f->shadow_steps <= 0) {
+        fog_segment(f, &s->sky, r, dist, T, ins, dsun);
+        return;
+    }
+    int steps = f->shadow_steps < 4 ? 4 : (f->shadow_steps > 128 ? 128 : f->shadow_steps);
+    int to_sky = dist >= 1e20;
+    double lam = f->height_falloff, max_t;
+    if (to_sky) {
+        double top = lam > 1e-4 ? f->height + 3.0 / lam : 25.0;
+        if (r.o.y >= top && r.d.y >= 0.0) {
+            *T = 1.0;
+            *ins = d3(0.0, 0.0, 0.0);
+            if (dsun) *dsun = d3(0.0, 0.0, 0.0);
+            return;
+        }
+        if (r.d.y > 0.02) {
+            max_t = (top - r.o.y) / r.d.y;
+            if (max_t > 40.0) max_t = 40.0;
+        } else {
+            max_t = 40.0;
+        }
+    } else {
+        max_t = dist > 60.0 ? 60.0 : dist;
+    }
+    double dt = max_t / (double)steps;
+    double ph = hg_lobe(clamp_g(f->sun_anisotropy), ddot(r.d, fromm(s->sky.sun_dir)));
+    if (ph > 6.0) ph = 6.0;
+    double sk = (0.70 * ph + 0.30) * f->inscatter_strength;
+    dv3 glow = dscale(fromm(s->sky.sun_color), sk);
+    dv3 sun = fromm(s->sky.sun_dir);
+    double Ta = 1.0;
+    dv3 acc = d3(0.0, 0.0, 0.0), dacc = d3(0.0, 0.0, 0.0);
+    double jit = or_rand01(seed, (unsigned)b, CH_FOG);
+    for (int i = 0; i < steps; ++i) {
+        dv3 p = dat(r.o, r.d, ((double)i + jit) * dt);
+        double rho = f->density;
+        if (lam > 1e-6) rho *= exp(-lam * (p.y - f->height));
+        rho *= fog_turb(f, p);
+        if (rho < 0.0) rho = 0.0;
+        double sT = exp(-(rho * dt));
+        oray sr = { p, sun };
+        dv3 att;
+        int shadowed = occluded_segment(s, sr, 0.05, 100.0, &att);
+        dv3 col = fromm(f->color);
+        if (!shadowed) col = dadd(col, dmul(glow, att));
+        acc = dadd(acc, dscale(dscale(col, 1.0 - sT), Ta));
+        if (!shadowed) dacc = dadd(dacc, dscale(dscale(att, sk * (1.0 - sT)), Ta));
+        Ta *= sT;
+        if (Ta < 1e-4) break;
+    }
+    if (dist > max_t) {
+        oray rr = { dat(r.o, r.d, max_t), r.d };
+        double rT;
+        dv3 rins, rd;
+        fog_segment(f, &s->sky, rr, to_sky ? 1e30 : dist - max_t, &rT, &rins, &rd);
+        acc = dadd(acc, dscale(rins, Ta));
+        dacc = dadd(dacc, dscale(rd, Ta));
+        Ta *= rT;
+    }
+    *T = Ta;
+    *ins = acc;
+    if (dsun) *dsun = dacc;
+}
+
+/* Segment inside a medium. Water with fog shadow_steps > 0 is marched in
+ * shadow_steps (clamped to [4, 16]) jittered steps: per-step Beer-Lambert
+ * transmittance exp(-absorption dt) and in-scatter (0.4 deep colour + sun
+ * * HG(0.72, clamped at 4.5) * 0.18 where a shadow ray (tmin 0.02, tmax
+ * 100) is clear) (1 - T); other media use apply_medium. beta2 (optional)
+ * takes the same transmittance; dsun (optional) accumulates d(added
+ * radiance)/d(sun colour). */
+static void pt_water_segment(const or_scene *s, const mh_material *m, oray r, double dist,
+                             unsigned seed, int b, dv3 *beta, dv3 *rad, dv3 *beta2, dv3 *dsun)
+{
+    if (!m->is_water || s->fog.shadow_steps <= 0 || dist <= 1e-4) {
+        apply_medium(m, dist, beta, rad, beta2);
+        return;
+    }
+    int steps = s->fog.shadow_steps < 4 ? 4 : (s->fog.shadow_steps > 16 ? 16 : s->fog.shadow_steps);
+    double dt = dist / (double)steps;
+    double ph = hg_lobe(0.72, ddot(r.d, fromm(s->sky.sun_dir)));
+    if (ph > 4.5) ph = 4.5;
+    double jit = or_rand01(seed, (unsigned)b, CH_WATER);
+    dv3 sT = d3(exp(-m->absorption.x * dt), exp(-m->absorption.y * dt), exp(-m->absorption.z * dt));
+    dv3 sun = fromm(s->sky.sun_dir), amb = dscale(fromm(m->deep_color), 0.40);
+    for (int i = 0; i < steps; ++i) {
+        dv3 p = dat(r.o, r.d, ((double)i + jit) * dt);
+        oray sr = { p, sun };
+        dv3 att;
+        int shadowed = occluded_segment(s, sr, 0.02, 100.0, &att);
+        dv3 sc = d3(0.0, 0.0, 0.0);
+        if (!shadowed) sc = dscale(dmul(fromm(s->sky.sun_color), att), ph * 0.18);
+        dv3 col = dadd(amb, sc);
+        dv3 one_m = d3(1.0 - sT.x, 1.0 - sT.y, 1.0 - sT.z);
+        *rad = dadd(*rad, dmul(*beta, dmul(col, one_m)));
+        if (dsun && !shadowed) *dsun = dadd(*dsun, dmul(*beta, dmul(dscale(att, ph * 0.18), one_m)));
+        *beta = dmul(*beta, sT);
+        if (beta2) *beta2 = dmul(*beta2, sT);
+    }
 }
 
 static dv3 shade(const mh_material *m, dv3 N, dv3 L, dv3 V, dv3 c)
@@ -613,14 +800,30 @@ static dv3 pt_radiance(const or_scene *s, oray primary, int max_depth, unsigned 
         or_tls_stats.ext_queries++;
         if (!or_intersect1(s, r, 1e-4, 1e30, &h)) {
             if (medium) {
-                dv3 L0 = L;
-                apply_medium(medium, 5.0, &beta, &L, &beta_f);
-                if (adj) adj->Lacc = dadd(adj->Lacc, dsub(L, L0));
+                dv3 L0 = L, dw = d3(0.0, 0.0, 0.0);
+                pt_water_segment(s, medium, r, 5.0, seed, b, &beta, &L, &beta_f, adj ? &dw : NULL);
+                if (adj) {
+                    adj->Lacc = dadd(adj->Lacc, dsub(L, L0));
+                    dv3 ds = dmul(adj->delta, dw);
+                    gsun(adj, 0, ds.x);
+                    gsun(adj, 1, ds.y);
+                    gsun(adj, 2, ds.z);
+                }
                 medium = NULL;
             }
-            dv3 t = dmul(beta, or_sky_sample(r.d, &s->sky));
+            dv3 sky = or_sky_sample(r.d, &s->sky);
+            dv3 dsk = adj ? sky_dsun(r.d, &s->sky) : d3(0.0, 0.0, 0.0);
+            if (s->fog.density > 0.0) {
+                double T;
+                dv3 ins, dfs;
+                if (b == 0) pt_fog_segment(s, r, 1e30, seed, b, &T, &ins, &dfs);
+                else fog_segment(&s->fog, &s->sky, r, 1e30, &T, &ins, &dfs);
+                sky = dadd(dscale(sky, T), ins);
+                dsk = dadd(dscale(dsk, T), dfs);
+            }
+            dv3 t = dmul(beta, sky);
             if (adj) {
-                dv3 ds = dmul(dmul(adj->delta, beta), sky_dsun(r.d, &s->sky));
+                dv3 ds = dmul(dmul(adj->delta, beta), dsk);
                 gsun(adj, 0, ds.x);
                 gsun(adj, 1, ds.y);
                 gsun(adj, 2, ds.z);
@@ -630,9 +833,31 @@ static dv3 pt_radiance(const or_scene *s, oray primary, int max_depth, unsigned 
             break;
         }
         if (medium) {
-            dv3 L0 = L;
-            apply_medium(medium, h.t, &beta, &L, &beta_f);
-            if (adj) adj->Lacc = dadd(adj->Lacc, dsub(L, L0));
+            dv3 L0 = L, dw = d3(0.0, 0.0, 0.0);
+            pt_water_segment(s, medium, r, h.t, seed, b, &beta, &L, &beta_f, adj ? &dw : NULL);
+            if (adj) {
+                adj->Lacc = dadd(adj->Lacc, dsub(L, L0));
+                dv3 ds = dmul(adj->delta, dw);
+                gsun(adj, 0, ds.x);
+                gsun(adj, 1, ds.y);
+                gsun(adj, 2, ds.z);
+            }
+        } else if (s->fog.density > 0.0) {
+            double T;
+            dv3 ins, dfs;
+            if (b == 0) pt_fog_segment(s, r, h.t, seed, b, &T, &ins, &dfs);
+            else fog_segment(&s->fog, &s->sky, r, h.t, &T, &ins, &dfs);
+            dv3 t = dmul(beta, ins);
+            L = dadd(L, t);
+            if (adj) {
+                dv3 ds = dmul(dmul(adj->delta, beta), dfs);
+                gsun(adj, 0, ds.x);
+                gsun(adj, 1, ds.y);
+                gsun(adj, 2, ds.z);
+                adj->Lacc = dadd(adj->Lacc, t);
+            }
+            beta = dscale(beta, T);
+            beta_f = dscale(beta_f, T);
         }
         const mh_material *m = mat_of(s, h.material);
         if (!m) {
@@ -898,8 +1123,18 @@ static dv3 w_trace(const or_scene *s, oray r, int depth, int maxd, unsigned seed
 {
     if (depth > maxd) return d3(0.0, 0.0, 0.0);
     ohit h;
-    if (!or_intersect1(s, r, 1e-4, 1e30, &h)) return or_sky_sample(r.d, &s->sky);
-    return w_trace_hit(s, r, depth, maxd, &h, seed);
+    double T;
+    dv3 ins;
+    if (!or_intersect1(s, r, 1e-4, 1e30, &h)) {
+        dv3 sky = or_sky_sample(r.d, &s->sky);
+        if (s->fog.density <= 0.0) return sky;
+        fog_segment(&s->fog, &s->sky, r, 1e30, &T, &ins, NULL);
+        return dadd(dscale(sky, T), ins);
+    }
+    dv3 c = w_trace_hit(s, r, depth, maxd, &h, seed);
+    if (s->fog.density <= 0.0) return c;
+    fog_segment(&s->fog, &s->sky, r, h.t, &T, &ins, NULL);
+    return dadd(dscale(c, T), ins);
 }
 
 /* ---- pixel kernels ---------------------------------------------------- */
@@ -1234,6 +1469,7 @@ or_scene *or_scene_create(const mh_scene_soa *soa)
     }
     if (soa->n_materials > 0) memcpy(s->mats, soa->materials, sizeof(mh_material) * (size_t)soa->n_materials);
     s->sky = soa->sky;
+    s->fog = soa->fog;
     s->water_level = soa->water_level;
     s->water_material = soa->water_material;
     for (int i = 0; i < s->n_mats; ++i)
@@ -1365,6 +1601,14 @@ void or_noise(const double *xy, int n, int octaves, unsigned seed, double *perli
     for (int i = 0; i < n; ++i) {
         perlin[i] = or_perlin2(xy[2 * i], xy[2 * i + 1], seed);
         fbm[i] = or_fbm2(xy[2 * i], xy[2 * i + 1], octaves, 2.0, 0.5, seed);
+    }
+}
+
+void or_noise3(const double *xyz, int n, unsigned seed, double *perlin, double *fbm)
+{
+    for (int i = 0; i < n; ++i) {
+        perlin[i] = or_perlin3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], seed);
+        fbm[i] = or_fbm3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 3, 2.0, 0.5, seed);
     }
 }
 

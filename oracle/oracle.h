@@ -100,6 +100,8 @@ void or_sky_samples(const or_scene *s, const double *dirs, int n, double *out);
 void or_water_normals(const double *xz, int n, double *out);
 void or_noise(const double *xy, int n, int octaves, unsigned seed, double *perlin,
               double *fbm);
+/* 3D gradient noise and its 3-octave fBm (the fog turbulence) at n points. */
+void or_noise3(const double *xyz, int n, unsigned seed, double *perlin, double *fbm);
 void or_sun_disk_dirs(const double sun[3], double radius_deg, const double *u,
                       int n, double *out);
 void or_light_cone_dirs(const double w[3], double cos_max, const double *u, int n,

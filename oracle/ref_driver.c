@@ -127,7 +127,15 @@ void *mtref_from_soa(const mh_scene_soa *soa)
     s->sky.star_intensity = k->star_intensity;
     s->sky.nebula_intensity = k->nebula_intensity;
     s->sky.galaxy_intensity = k->galaxy_intensity;
-    s->fog.density = k->fog_density;
+    s->fog.density = soa->fog.density;
+    s->fog.color = v3(soa->fog.color);
+    s->fog.height = soa->fog.height;
+    s->fog.height_falloff = soa->fog.height_falloff;
+    s->fog.inscatter_strength = soa->fog.inscatter_strength;
+    s->fog.sun_anisotropy = soa->fog.sun_anisotropy;
+    s->fog.noise_scale = soa->fog.noise_scale;
+    s->fog.noise_amount = soa->fog.noise_amount;
+    s->fog.shadow_steps = soa->fog.shadow_steps;
 
     s->material_count = soa->n_materials;
     s->materials = (Material *)calloc((size_t)(soa->n_materials > 0 ? soa->n_materials : 1), sizeof(Material));
@@ -228,7 +236,7 @@ int mtref_counts(void *h, int *n_prims, int *n_materials, int *n_lights)
 /* Export the reference scene in the flattened ABI layout. Returns 0, or 4
  * when the scene holds a primitive kind outside the GPU path. */
 int mtref_export(void *h, mh_prim *prims, mh_prim_attr *attrs, mh_material *mats,
-                 mh_sky *sky, double *water_level, int *water_material)
+                 mh_sky *sky, double *water_level, int *water_material, mh_fog *fog)
 {
     RefScene *rs = (RefScene *)h;
     if (!rs) return 1;
@@ -292,7 +300,17 @@ int mtref_export(void *h, mh_prim *prims, mh_prim_attr *attrs, mh_material *mats
     sky->cloud_octaves = k->cloud_octaves; sky->seed = k->seed; sky->sun_radius = k->sun_radius;
     sky->cloud_thickness = k->cloud_thickness; sky->star_intensity = k->star_intensity;
     sky->nebula_intensity = k->nebula_intensity; sky->galaxy_intensity = k->galaxy_intensity;
-    sky->fog_density = s->fog.density;
+    sky->fog_density = 0.0;
+    if (fog) {
+        const FogParams *f = &s->fog;
+        memset(fog, 0, sizeof *fog);
+        fog->density = f->density; fog->color = m3(f->color);
+        fog->height = f->height; fog->height_falloff = f->height_falloff;
+        fog->inscatter_strength = f->inscatter_strength;
+        fog->sun_anisotropy = f->sun_anisotropy;
+        fog->noise_scale = f->noise_scale; fog->noise_amount = f->noise_amount;
+        fog->shadow_steps = f->shadow_steps;
+    }
     *water_level = s->water_level;
     *water_material = s->water_material;
     if (s->has_ocean) rc = 4;
@@ -447,6 +465,14 @@ void mtref_noise(const double *xy, int n, int octaves, unsigned seed, double *pe
     for (int i = 0; i < n; ++i) {
         perlin[i] = noise_perlin2(xy[2 * i], xy[2 * i + 1], seed);
         fbm[i] = noise_fbm2(xy[2 * i], xy[2 * i + 1], octaves, 2.0, 0.5, seed);
+    }
+}
+
+void mtref_noise3(const double *xyz, int n, unsigned seed, double *perlin, double *fbm)
+{
+    for (int i = 0; i < n; ++i) {
+        perlin[i] = noise_perlin3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], seed);
+        fbm[i] = noise_fbm3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 3, 2.0, 0.5, seed);
     }
 }
 

@@ -66,6 +66,7 @@ def oracle():
         _sig(L, "or_sky_samples", None, [P, P, C.c_int, P])
         _sig(L, "or_water_normals", None, [P, C.c_int, P])
         _sig(L, "or_noise", None, [P, C.c_int, C.c_int, C.c_uint, P, P])
+        _sig(L, "or_noise3", None, [P, C.c_int, C.c_uint, P, P])
         _sig(L, "or_sun_disk_dirs", None, [DP, D, P, C.c_int, P])
         _sig(L, "or_light_cone_dirs", None, [DP, D, P, C.c_int, P])
         _sig(L, "or_cosine_dirs", None, [DP, P, C.c_int, P])
@@ -92,7 +93,7 @@ def ref():
         _sig(L, "mtref_counts", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int)])
         _sig(L, "mtref_export", C.c_int, [P, P, P, P, C.POINTER(abi.Sky), DP,
-                                          C.POINTER(C.c_int)])
+                                          C.POINTER(C.c_int), C.POINTER(abi.Fog)])
         _sig(L, "mtref_camera_create", None, [DP, DP, DP, D, D, C.POINTER(abi.Camera)])
         _sig(L, "mtref_scene_camera", C.c_int, [P, D, C.POINTER(abi.Camera)])
         _sig(L, "mtref_render", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int, C.c_int,
@@ -103,6 +104,7 @@ def ref():
         _sig(L, "mtref_sky_sample", None, [P, P, C.c_int, P])
         _sig(L, "mtref_water_normal", None, [P, C.c_int, P])
         _sig(L, "mtref_noise", None, [P, C.c_int, C.c_int, C.c_uint, P, P])
+        _sig(L, "mtref_noise3", None, [P, C.c_int, C.c_uint, P, P])
         _sig(L, "mtref_sun_disk_dir", None, [DP, D, P, C.c_int, P])
         _sig(L, "mtref_light_cone_dir", None, [DP, D, P, C.c_int, P])
         _sig(L, "mtref_cosine_dir", None, [DP, P, C.c_int, P])
@@ -122,7 +124,7 @@ def d3(v):
 class SoaHolder:
     """Keeps numpy arrays alive behind an mh_scene_soa (no BVH)."""
 
-    def __init__(self, prims, attrs, mats, sky, water_level=0.0, water_material=-1):
+    def __init__(self, prims, attrs, mats, sky, water_level=0.0, water_material=-1, fog=None):
         self.prims = np.ascontiguousarray(prims, dtype=abi.PRIM_DTYPE)
         self.attrs = None if attrs is None else np.ascontiguousarray(attrs, dtype=abi.ATTR_DTYPE)
         self.mats = np.ascontiguousarray(mats, dtype=abi.MATERIAL_DTYPE)
@@ -130,7 +132,8 @@ class SoaHolder:
         self.soa = abi.SceneSoa(abi_version=abi.ABI_VERSION, n_prims=len(self.prims),
                                 prims=abi.ptr(self.prims), attrs=abi.ptr(self.attrs),
                                 n_materials=len(self.mats), materials=abi.ptr(self.mats),
-                                sky=sky, water_level=water_level, water_material=water_material)
+                                sky=sky, water_level=water_level, water_material=water_material,
+                                fog=abi.fog_default() if fog is None else fog)
 
 
 class OracleScene:
@@ -157,6 +160,10 @@ class OracleScene:
         if rc:
             raise RuntimeError(f"or_render rc {rc}")
         return rgb, mean, var, sec.value
+
+    def radiance(self, rays, depth, seed0):
+        """FP64 path radiance of each ray, seed seed0 + i for ray i."""
+        return _radiance(self.L.or_radiance, self.h, rays, depth, seed0)
 
     def intersect(self, rays, tmin=1e-4, tmax=1e30):
         rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
@@ -239,9 +246,10 @@ class RefScene:
         mats = np.zeros(n_m.value, abi.MATERIAL_DTYPE)
         sky = abi.Sky()
         wl, wm = C.c_double(), C.c_int()
+        fog = abi.Fog()
         rc = self.L.mtref_export(self.h, abi.ptr(prims), abi.ptr(attrs), abi.ptr(mats),
-                                 C.byref(sky), C.byref(wl), C.byref(wm))
-        return rc, prims, attrs, mats, sky, wl.value, wm.value
+                                 C.byref(sky), C.byref(wl), C.byref(wm), C.byref(fog))
+        return rc, prims, attrs, mats, sky, wl.value, wm.value, fog
 
     def camera(self, aspect):
         cam = abi.Camera()
@@ -257,11 +265,24 @@ class RefScene:
             raise RuntimeError(f"mtref_render rc {rc}")
         return rgb, sec.value
 
+    def radiance(self, rays, depth, seed0):
+        return _radiance(self.L.mtref_radiance, self.h, rays, depth, seed0)
+
     def intersect(self, rays, tmin=1e-4, tmax=1e30):
         rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
         out = np.zeros(len(rays), abi.HIT_DTYPE)
         self.L.mtref_intersect(self.h, abi.ptr(rays), len(rays), tmin, tmax, abi.ptr(out))
         return out
+
+
+def _radiance(fn, h, rays, depth, seed0):
+    rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
+    out = np.zeros((len(rays), 3))
+    base = rays.ctypes.data
+    for i in range(len(rays)):
+        fn(h, C.c_void_p(base + i * abi.RAY_DTYPE.itemsize), depth, (seed0 + i) & 0xFFFFFFFF,
+           out[i].ctypes.data_as(C.POINTER(C.c_double)))
+    return out
 
 
 def seeded_rays(n, seed, lo, hi, towards=None):

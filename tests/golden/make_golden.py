@@ -77,6 +77,44 @@ def host_scene_images():
     np.savez_compressed(os.path.join(HERE, "host_scenes.npz"), **out)
 
 
+FOG_SCENE = (native.GEN_FOREST, dict(grid=3, subdiv=2), 5)
+FOG_IMG = (64, 48, 2, 6)
+
+
+def fog_scenes():
+    """Reference radiance (fp64, seed 500 + i for ray i, depth 8) and images
+    of the forest-and-pond workload under each fog of scenes.FOG_SETS, and
+    3D gradient noise / fBm vectors."""
+    import scenes
+    kind, kw, seed = FOG_SCENE
+    hs = HostScene()
+    hs.generate(kind, seed=seed, **kw)
+    hs.build_bvh()
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    out = {"prims_sha": np.frombuffer(hashlib.sha256(prims.tobytes() + mats.tobytes()).hexdigest()
+                                      .encode(), np.uint8).copy()}
+    rays = scenes.fog_rays()
+    out["rays"] = rays
+    w, h, spp, depth = FOG_IMG
+    cam = hs.camera(w / h)
+    out["cam"] = sbytes(cam)
+    for name, f in scenes.FOG_SETS.items():
+        hold = ck.SoaHolder(prims, attrs, mats, sky, wl, wm, fog=f)
+        r = ck.RefScene(soa=hold.soa)
+        out[f"{name}_fog"] = sbytes(f)
+        out[f"{name}_rad"] = r.radiance(rays, 8, 500)
+        for mode, tag in [(0, "pt"), (1, "whitted")]:
+            rgb, _ = r.render(cam, w, h, spp, depth, mode, threads=8)
+            out[f"{name}_{tag}"] = rgb
+    rng = np.random.default_rng(31)
+    xyz = np.ascontiguousarray(np.concatenate([rng.uniform(-40, 40, (400, 3)),
+                                               np.round(rng.uniform(-5, 5, (100, 3)))]))
+    per, fbm = np.zeros(len(xyz)), np.zeros(len(xyz))
+    ck.ref().mtref_noise3(abi.ptr(xyz), len(xyz), 1337, abi.ptr(per), abi.ptr(fbm))
+    out["noise_xyz"], out["perlin3"], out["fbm3"] = xyz, per, fbm
+    np.savez_compressed(os.path.join(HERE, "fog.npz"), **out)
+
+
 def vectors(r, sky):
     """Reference outputs of the per-function routines on seeded inputs."""
     L = ck.ref()
@@ -130,6 +168,7 @@ def main():
     r, prims, attrs, mats, sky, wl, wm = default_scene()
     vectors(r, sky)
     host_scene_images()
+    fog_scenes()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

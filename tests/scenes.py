@@ -92,3 +92,36 @@ def down_ray(x=0.0, z=0.0, y=3.0, tilt=(0.0, 0.0)):
     r["ox"], r["oy"], r["oz"] = x, y, z
     r["dx"], r["dy"], r["dz"] = d
     return r
+
+
+def fog(density, color, height=0.0, falloff=0.0, inscatter=0.5, g=0.7, noise_scale=0.0,
+        noise_amount=0.0, steps=0):
+    f = abi.Fog()
+    f.density, f.color, f.height, f.height_falloff = density, abi.vec3(color), height, falloff
+    f.inscatter_strength, f.sun_anisotropy = inscatter, g
+    f.noise_scale, f.noise_amount, f.shadow_steps = noise_scale, noise_amount, steps
+    return f
+
+
+# The fog blocks of the reference's own scene files (scenes/ashenvale_forest,
+# nordic_trees, cyberpunk_anamorphic, ocean_reef_abyss .scene), densities x4 so
+# a small test scene shows them, plus the step clamps (3 -> 4, 200 -> 128 / 16).
+FOG_SETS = {
+    "ashenvale": fog(0.064, (0.36, 0.15, 0.40), 5.2, 0.18, 0.45, 0.82, 0.16, 0.25, 10),
+    "nordic": fog(0.024, (0.84, 0.88, 0.94), 5.5, 0.22, 0.35, 0.74, 0.16, 0.35, 0),
+    "cyberpunk": fog(0.060, (0.02, 0.025, 0.05), 10.0),
+    "reef": fog(0.012, (0.65, 0.78, 0.90), 1.8, 0.50, 0.60, 0.75, 0.20, 0.10, 16),
+    "steps3": fog(0.05, (0.5, 0.55, 0.6), 2.0, 0.3, 0.8, -0.99, 0.3, 0.9, 3),
+    "steps200": fog(0.03, (0.5, 0.55, 0.6), 0.0, 0.0, 0.6, 0.99, 0.0, 0.0, 200),
+}
+
+
+def fog_rays():
+    """Rays for the fog checks on the forest-and-pond workload: from the air
+    (some hit the pond and refract in), from under the water (the path starts
+    in the water medium) and from above the fog layer's top."""
+    from checkers import seeded_rays
+    a = seeded_rays(400, 21, [-10, 0.2, -10], [10, 6, 10])
+    b = seeded_rays(150, 22, [-3.5, -0.9, -2.5], [3.5, -0.1, 2.5])
+    c = seeded_rays(50, 23, [-10, 26, -10], [10, 30, 10])
+    return np.concatenate([a, b, c])

@@ -43,6 +43,9 @@ def test_gpu_entry_points_reject_null_arguments():
     assert L.mh_render_batch(None, None, 0, None, 0, None, None, None) == 1
     assert L.mh_batch_last_tune(None, None) == 1
     assert L.mh_last_kernel_times(None) == 1
+    assert L.mh_scene_set_materials(None, None, 0) == 1
+    assert L.mh_scene_set_sky(None, None) == 1
+    assert L.mh_scene_set_fog(None, None) == 1
 
 
 def bvh_arrays(hs):

@@ -278,3 +278,93 @@ def test_oracle_equals_live_reference(kind, kw, seed):
     ha, hb = r.intersect(rays), o.intersect(rays)
     assert np.array_equal(ha["prim_index"], hb["prim_index"])
     assert np.array_equal(ha["t"], hb["t"])
+
+
+# ---- fog ------------------------------------------------------------------------
+
+def _fog_workload(golden):
+    g = golden["fog"]
+    hs = HostScene()
+    hs.generate(native.GEN_FOREST, seed=5, grid=3, subdiv=2)
+    hs.build_bvh()
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    sha = hashlib.sha256(prims.tobytes() + mats.tobytes()).hexdigest()
+    assert sha == g["prims_sha"].tobytes().decode(), "generator output changed"
+    return g, hs, (prims, attrs, mats, sky, wl, wm)
+
+
+def test_oracle_noise3_bit_exact(golden):
+    g = golden["fog"]
+    xyz = np.ascontiguousarray(g["noise_xyz"])
+    per, fbm = np.zeros(len(xyz)), np.zeros(len(xyz))
+    ck.oracle().or_noise3(abi.ptr(xyz), len(xyz), 1337, abi.ptr(per), abi.ptr(fbm))
+    assert np.array_equal(per, g["perlin3"]) and np.array_equal(fbm, g["fbm3"])
+    # gradient noise vanishes on the lattice
+    assert np.all(per[400:] == 0.0)
+
+
+@pytest.mark.parametrize("name", list(scenes.FOG_SETS))
+def test_oracle_fog_equals_reference_golden(golden, name):
+    """Fog (closed form, shadow-marched camera segment, marched underwater
+    segment): FP64 radiance bit-exact with the reference's on 600 rays and the
+    reference's PT and Whitted images equal."""
+    g, hs, arrs = _fog_workload(golden)
+    f = abi.Fog.from_buffer_copy(g[f"{name}_fog"].tobytes())
+    assert bytes(f) == bytes(scenes.FOG_SETS[name])
+    hold = ck.SoaHolder(*arrs, fog=f)
+    o = ck.OracleScene(hold.soa)
+    rad = o.radiance(g["rays"], 8, 500)
+    assert np.array_equal(rad, g[f"{name}_rad"])
+    cam = cam_of(g["cam"])
+    for tag, mode in (("pt", 0), ("whitted", 1)):
+        rgb, _, _, _ = o.render(cam, 64, 48, 2, 6, mode=mode)
+        assert np.array_equal(rgb, g[f"{name}_{tag}"]), tag
+
+
+def test_fog_closed_forms():
+    """Uniform fog (no falloff, no noise, no in-scatter) over a black plane
+    under a black sky: a camera ray straight down from height z returns
+    exactly color * (1 - exp(-density z)); with shadow_steps the march of a
+    constant medium telescopes to the same value; zero density is no fog."""
+    m = scenes.material(albedo=(0.0, 0.0, 0.0))
+    p = scenes.prim(abi.PRIM_PLANE, 0, p0=(0.0, 0.0, 0.0), p1=(0.0, 1.0, 0.0))
+    sk = scenes.sky(sun_color=(0.0, 0.0, 0.0))
+    hs = scenes.build([p], [m], sk)
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    ray = np.zeros(1, abi.RAY_DTYPE)
+    ray["oy"], ray["dy"] = 3.0, -1.0
+    col = np.array([0.3, 0.5, 0.7])
+    for steps in (0, 8):
+        f = scenes.fog(0.2, col, inscatter=0.0, steps=steps)
+        o = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm, fog=f).soa)
+        got = o.radiance(ray, 4, 1)[0]
+        assert np.allclose(got, col * (1.0 - np.exp(-0.2 * 3.0)), rtol=1e-9, atol=0), steps
+    f = scenes.fog(0.0, col)
+    o = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm, fog=f).soa)
+    assert np.all(o.radiance(ray, 4, 1)[0] == 0.0)
+    # to the sky: exponential height fog has optical depth base / (lambda dy)
+    up = np.zeros(1, abi.RAY_DTYPE)
+    up["oy"], up["dy"] = 1.0, 1.0
+    f = scenes.fog(0.1, col, height=0.5, falloff=0.4, inscatter=0.0)
+    o = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm, fog=f).soa)
+    tau = 0.1 * np.exp(-0.4 * 0.5) / 0.4
+    assert np.allclose(o.radiance(up, 4, 1)[0], col * (1.0 - np.exp(-tau)), rtol=1e-12)
+
+
+@pytest.mark.skipif(not ck.have_ref(), reason="oracle/_ref not built (reference absent)")
+@pytest.mark.parametrize("kind,kw,seed", [(native.GEN_TERRAIN, dict(grid=40, n_spheres=30), 3),
+                                          (native.GEN_FOREST, dict(grid=3, subdiv=2), 7)])
+def test_oracle_fog_equals_live_reference(kind, kw, seed):
+    hs = HostScene()
+    hs.generate(kind, seed=seed, **kw)
+    hs.build_bvh()
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    rays = ck.seeded_rays(300, seed, [-12, 0.1, -12], [12, 8, 12])
+    for name in ("ashenvale", "nordic", "reef", "steps200"):
+        hold = ck.SoaHolder(prims, attrs, mats, sky, wl, wm, fog=scenes.FOG_SETS[name])
+        o, r = ck.OracleScene(hold.soa), ck.RefScene(soa=hold.soa)
+        assert np.array_equal(o.radiance(rays, 8, 77), r.radiance(rays, 8, 77)), name
+        cam = hs.camera(4 / 3)
+        a, _ = r.render(cam, 48, 36, 2, 6, 0)
+        b, _, _, _ = o.render(cam, 48, 36, 2, 6, mode=0)
+        assert np.array_equal(a, b), name
