@@ -308,6 +308,33 @@ int mh_device_info(char *name, size_t name_len, int *cu_count, size_t *hbm_bytes
 int mh_scene_create(const mh_scene_soa *soa, struct mh_scene **out);
 void mh_scene_destroy(struct mh_scene *s);
 
+/* Tree statistics of an uploaded scene. */
+typedef struct mh_bvh_info {
+    int32_t n_nodes;       /* binary nodes                                  */
+    int32_t n_leaf;        /* leaf-order primitive slots                    */
+    int32_t max_depth;     /* levels (device build; 0 for a host tree)      */
+    int32_t width4;        /* 1: the 4-wide tree was built too              */
+    int32_t device_built;  /* 1: built by mh_scene_create_gpu_bvh           */
+    int32_t reserved;
+    double  build_ms;      /* device build time (tree + device layout)      */
+} mh_bvh_info;
+
+/* Upload a scene WITHOUT a host tree and build its BVH on the device:
+ * binned SAH over `bins` bins (0: 16) on all three axes with leaves of at
+ * most max_leaf prims (0: 4) -- the split decisions mt_scene_build_bvh makes
+ * for the same options, so the tree has the host tree's nodes (prim order
+ * inside a leaf may differ). soa->nodes / leaf_prims / planes are ignored.
+ * Replaces the reference's host build before a render (bvh_build,
+ * src/bvh.c:180-422, called from scene setup); the reference has no device
+ * build. Returns 5 for a tree deeper than 60 levels. */
+int mh_scene_create_gpu_bvh(const mh_scene_soa *soa, int max_leaf, int bins,
+                            struct mh_scene **out);
+int mh_scene_bvh_info(struct mh_scene *s, mh_bvh_info *info);
+/* The device-built binary tree (mh_bvh_node, node 0 = root; leaf ranges
+ * index leaf_prims) into host arrays of info.n_nodes / info.n_leaf entries.
+ * Returns 4 for a scene created from a host tree. */
+int mh_scene_bvh_export(struct mh_scene *s, mh_bvh_node *nodes, int32_t *leaf_prims);
+
 /* Device bytes held by the scene (nodes + primitive records + tables). */
 size_t mh_scene_device_bytes(const struct mh_scene *s);
 

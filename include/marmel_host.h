@@ -67,6 +67,9 @@ int mt_scene_build_bvh(mt_scene *s, const mt_bvh_options *opts, mt_bvh_stats *st
 
 /* Flattened view for mh_scene_create (requires a built BVH). */
 int mt_scene_soa(const mt_scene *s, mh_scene_soa *out);
+/* The same without the tree (n_nodes = n_leaf_prims = n_planes = 0), for
+ * mh_scene_create_gpu_bvh, which builds it on the device. */
+int mt_scene_soa_no_bvh(const mt_scene *s, mh_scene_soa *out);
 
 /* Look-at camera with vertical fov in degrees (reference camera_create). */
 void mt_camera_create(const double eye[3], const double at[3], const double up[3],

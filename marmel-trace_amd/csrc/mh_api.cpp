@@ -5,7 +5,9 @@
 // the kernels on the scene's own stream, time them with HIP events recorded
 // on that stream, and copy results back only when the caller asks.
 #include "marmel_hip.h"
+#include "mh_build.h"
 #include "mh_kernels.h"
+#include "mh_pack.h"
 
 #include <hip/hip_runtime.h>
 
@@ -44,8 +46,6 @@ int fail(int code, const char* fmt, ...)
         if (e_ != hipSuccess)                                                                         \
             return fail(100 + (int)e_, "%s failed: %s", #expr, hipGetErrorString(e_));               \
     } while (0)
-
-template <typename R> V3<R> cv(const mh_vec3& v) { return V3<R>{(R)v.x, (R)v.y, (R)v.z}; }
 
 template <typename R> DMat<R> conv_mat(const mh_material& a)
 {
@@ -100,63 +100,6 @@ template <typename R> DSky<R> conv_sky(const mh_sky& k)
     double rad = k.sun_radius * 3.14159265358979323846 / 180.0;
     s.sun_omega = (R)(k.sun_radius > 0.0 ? 2.0 * 3.14159265358979323846 * (1.0 - cos(rad)) : 1.0);
     return s;
-}
-
-P32 pack32(const mh_prim& q)
-{
-    P32 r;
-    memset(&r, 0, sizeof r);
-    float kf;
-    int kv = q.kind | (q.flags << 8);
-    memcpy(&kf, &kv, 4);
-    float pf, mf;
-    memcpy(&mf, &q.material, 4);
-    r.a = make_float4((float)q.p0.x, (float)q.p0.y, (float)q.p0.z, kf);
-    r.c.w = mf;
-    switch (q.kind) {
-    case MH_PRIM_TRIANGLE:
-        // edges from the double vertices, then rounded
-        r.b = make_float4((float)(q.p1.x - q.p0.x), (float)(q.p1.y - q.p0.y), (float)(q.p1.z - q.p0.z), 0.f);
-        r.c = make_float4((float)(q.p2.x - q.p0.x), (float)(q.p2.y - q.p0.y), (float)(q.p2.z - q.p0.z), mf);
-        break;
-    case MH_PRIM_SPHERE:
-        r.b = make_float4((float)q.r0, 0.f, 0.f, 0.f);
-        break;
-    case MH_PRIM_CYLINDER:
-        r.b = make_float4((float)q.p1.x, (float)q.p1.y, (float)q.p1.z, 0.f);
-        r.c = make_float4((float)q.r0, (float)q.r1, 0.f, mf);
-        break;
-    default: // box, plane
-        r.b = make_float4((float)q.p1.x, (float)q.p1.y, (float)q.p1.z, 0.f);
-        break;
-    }
-    (void)pf;
-    return r;
-}
-
-void set_prim_index32(P32& r, int idx) { memcpy(&r.b.w, &idx, 4); }
-
-P64 pack64(const mh_prim& q, int idx)
-{
-    P64 r;
-    r.p0[0] = q.p0.x; r.p0[1] = q.p0.y; r.p0[2] = q.p0.z;
-    r.p1[0] = q.p1.x; r.p1[1] = q.p1.y; r.p1[2] = q.p1.z;
-    r.p2[0] = q.p2.x; r.p2[1] = q.p2.y; r.p2[2] = q.p2.z;
-    r.r0 = q.r0;
-    r.r1 = q.r1;
-    r.kind = q.kind;
-    r.prim = idx;
-    r.material = q.material;
-    r.flags = q.flags;
-    return r;
-}
-
-template <typename R> DAttr<R> conv_attr(const mh_prim_attr& a)
-{
-    DAttr<R> d;
-    d.n0 = cv<R>(a.n0); d.n1 = cv<R>(a.n1); d.n2 = cv<R>(a.n2);
-    d.uv0 = cv<R>(a.uv0); d.uv1 = cv<R>(a.uv1); d.uv2 = cv<R>(a.uv2);
-    return d;
 }
 
 struct DevBuf {
@@ -291,6 +234,11 @@ struct mh_scene {
     // fog as given, and whether a material is water (marched water segments)
     mh_fog fog_host{};
     int has_water = 0;
+    // the tree: its statistics, and for a device build the binary nodes and
+    // leaf order (device, owned through bufs)
+    mh_bvh_info bvh{};
+    const mh_bvh_node* tree_nodes = nullptr;
+    const int32_t* tree_leaf = nullptr;
 };
 
 namespace {
@@ -372,7 +320,8 @@ int validate_materials(const mh_material* mats, int n)
     return 0;
 }
 
-int validate(const mh_scene_soa* soa)
+// with_tree false: the tree fields are not part of the scene (device build)
+int validate(const mh_scene_soa* soa, bool with_tree = true)
 {
     if (soa->abi_version != MH_ABI_VERSION) return fail(5, "abi_version %d != %d", soa->abi_version, MH_ABI_VERSION);
     if (soa->n_prims < 0 || soa->n_materials < 0 || soa->n_nodes < 0 || soa->n_leaf_prims < 0 || soa->n_planes < 0)
@@ -393,6 +342,10 @@ int validate(const mh_scene_soa* soa)
             return fail(4, "prim %d: kind %d is outside the GPU path", i, p.kind);
         if (p.kind == MH_PRIM_PLANE) ++n_planes;
         if ((p.flags & ~(MH_PRIM_SMOOTH | MH_PRIM_UV)) != 0) return fail(5, "prim %d: bad flags", i);
+    }
+    if (!with_tree) {
+        if (soa->n_prims - n_planes > (1 << 27)) return fail(5, "too many primitives (%d)", soa->n_prims - n_planes);
+        return 0;
     }
     if (n_planes != soa->n_planes) return fail(5, "plane list has %d entries, scene has %d planes", soa->n_planes, n_planes);
     for (int i = 0; i < soa->n_planes; ++i) {
@@ -941,6 +894,107 @@ int mh_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes
     return 0;
 }
 
+namespace {
+
+// Device arrays of an uploaded scene and the roots the kernels start from.
+struct Layout {
+    const void *nodes = nullptr, *nodes4 = nullptr;
+    int n_nodes = 0, root_ref = INT_MIN, root4 = INT_MIN;
+    bool bvh4 = false;
+    const void *p32 = nullptr, *p64 = nullptr, *pl32 = nullptr, *pl64 = nullptr, *plidx = nullptr;
+    const void *a32 = nullptr, *a64 = nullptr;
+    int n_planes = 0;
+};
+
+mh_scene* new_scene()
+{
+    mh_scene* s = new mh_scene();
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        fail(101, "hipStreamCreate failed");
+        mh_scene_destroy(s);
+        return nullptr;
+    }
+    for (auto& e : s->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            fail(101, "hipEventCreate failed");
+            mh_scene_destroy(s);
+            return nullptr;
+        }
+    return s;
+}
+
+// Materials, lights, counters and the two precision views: the part of scene
+// creation that does not depend on where the tree was built.
+int finish_scene(mh_scene* s, const mh_scene_soa* soa, const Layout& lay)
+{
+    std::vector<DMat<float>> m32(soa->n_materials);
+    std::vector<DMat<double>> m64(soa->n_materials);
+    for (int i = 0; i < soa->n_materials; ++i) {
+        m32[i] = conv_mat<float>(soa->materials[i]);
+        m64[i] = conv_mat<double>(soa->materials[i]);
+    }
+    void *d_m32, *d_m64;
+    int rc;
+    if ((rc = upload(s, m32.data(), m32.size() * sizeof(DMat<float>), &d_m32)) ||
+        (rc = upload(s, m64.data(), m64.size() * sizeof(DMat<double>), &d_m64)))
+        return rc;
+    if (hipMalloc(&s->d_rays, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&s->d_overflow, sizeof(int)) != hipSuccess)
+        return fail(102, "hipMalloc failed for counters");
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(103, "upload failed");
+
+    s->d_m32 = d_m32;
+    s->d_m64 = d_m64;
+    for (int i = 0; i < soa->n_prims; ++i) {
+        const mh_prim& p = soa->prims[i];
+        if (p.kind == MH_PRIM_SPHERE) s->spheres.push_back({i, p.material, p.p0, p.r0});
+    }
+    DLight<float> l32[8];
+    DLight<double> l64[8];
+    int nl = collect_lights(s, soa->materials, soa->n_materials, l32, l64);
+    int has_transmissive = any_transmissive(soa->materials, soa->n_materials);
+    s->fog_host = soa->fog;
+    s->has_water = any_water(soa->materials, soa->n_materials);
+
+    auto fill = [&](auto& v, const void* pr, const void* pl, const void* at, const void* mt, auto* lights, auto tag) {
+        typedef decltype(tag) R;
+        v.nodes = (const DNode*)lay.nodes;
+        v.n_nodes = lay.n_nodes;
+        v.root_ref = lay.root_ref;
+        v.nodes4 = (const DNode4*)lay.nodes4;
+        v.root4 = lay.root4;
+        v.bvh4 = lay.bvh4 ? 1 : 0;
+        v.prims = (const typename PrimT<R>::type*)pr;
+        v.planes = (const typename PrimT<R>::type*)pl;
+        v.n_planes = lay.n_planes;
+        v.plane_index = (const int*)lay.plidx;
+        v.attrs = (const DAttr<R>*)at;
+        v.mats = (const DMat<R>*)mt;
+        v.n_mats = soa->n_materials;
+        v.sky = conv_sky<R>(soa->sky);
+        v.fog = conv_fog<R>(soa->fog);
+        v.fog_kernel = needs_fog_kernel(soa->fog, s->has_water);
+        v.water_level = (R)soa->water_level;
+        v.water_material = soa->water_material;
+        v.has_transmissive = has_transmissive;
+        v.n_lights = nl;
+        for (int i = 0; i < 8; ++i) v.lights[i] = lights[i];
+    };
+    fill(s->v32, lay.p32, lay.pl32, lay.a32, d_m32, l32, 0.0f);
+    fill(s->v64, lay.p64, lay.pl64, lay.a64, d_m64, l64, 0.0);
+    return 0;
+}
+
+bool has_attrs(const mh_scene_soa* soa)
+{
+    if (!soa->attrs) return false;
+    for (int i = 0; i < soa->n_prims; ++i)
+        if (soa->prims[i].flags != 0) return true;
+    return false;
+}
+
+} // namespace
+
 int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
 {
     if (!soa || !out) return fail(1, "NULL argument");
@@ -951,14 +1005,12 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
     rc = check_tree(soa, &inner);
     if (rc) return rc;
 
-    mh_scene* s = new mh_scene();
+    mh_scene* s = new_scene();
+    if (!s) return 101;
     auto bail = [&](int code) {
         mh_scene_destroy(s);
         return code;
     };
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(101, "hipStreamCreate failed"));
-    for (auto& e : s->ev)
-        if (hipEventCreate(&e) != hipSuccess) return bail(fail(101, "hipEventCreate failed"));
 
     // device node numbering: interior nodes in DFS order
     std::vector<int> dev_index(soa->n_nodes, -1);
@@ -980,21 +1032,22 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         d.c3 = make_int4(ref_of(nd.left), ref_of(nd.right), 0, 0);
         nodes[i] = d;
     }
-    int root_ref = soa->n_nodes > 0 ? ref_of(0) : INT_MIN;
+    Layout lay;
+    lay.root_ref = soa->n_nodes > 0 ? ref_of(0) : INT_MIN;
     Bvh4Builder b4{soa, {}, 0};
-    int root4 = root_ref;
-    if (soa->n_nodes > 0 && soa->nodes[0].count == 0) root4 = b4.make(0, 1);
+    lay.root4 = lay.root_ref;
+    if (soa->n_nodes > 0 && soa->nodes[0].count == 0) lay.root4 = b4.make(0, 1);
     // 4-wide traversal pushes up to 3 entries per level
     bool use4 = soa->n_nodes > 0 && 3 * b4.max_depth + 1 <= MH_STACK;
     if (!use4) {
         b4.nodes.clear();
-        root4 = root_ref;
+        lay.root4 = lay.root_ref;
     }
+    lay.bvh4 = use4 && !b4.nodes.empty();
+    lay.n_nodes = (int)nodes.size();
+    lay.n_planes = soa->n_planes;
 
-    bool any_attr = false;
-    if (soa->attrs)
-        for (int i = 0; i < soa->n_prims && !any_attr; ++i) any_attr = soa->prims[i].flags != 0;
-
+    bool any_attr = has_attrs(soa);
     std::vector<P32> p32(soa->n_leaf_prims);
     std::vector<P64> p64(soa->n_leaf_prims);
     std::vector<DAttr<float>> a32;
@@ -1023,14 +1076,8 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         set_prim_index32(pl32[i], pi);
         pl64[i] = pack64(soa->prims[pi], pi);
     }
-    std::vector<DMat<float>> m32(soa->n_materials);
-    std::vector<DMat<double>> m64(soa->n_materials);
-    for (int i = 0; i < soa->n_materials; ++i) {
-        m32[i] = conv_mat<float>(soa->materials[i]);
-        m64[i] = conv_mat<double>(soa->materials[i]);
-    }
 
-    void *d_nodes, *d_nodes4, *d_p32, *d_p64, *d_pl32, *d_pl64, *d_plidx, *d_a32, *d_a64, *d_m32, *d_m64;
+    void *d_nodes, *d_nodes4, *d_p32, *d_p64, *d_pl32, *d_pl64, *d_plidx, *d_a32, *d_a64;
     if ((rc = upload(s, nodes.data(), nodes.size() * sizeof(DNode), &d_nodes)) ||
         (rc = upload(s, b4.nodes.data(), b4.nodes.size() * sizeof(DNode4), &d_nodes4)) ||
         (rc = upload(s, p32.data(), p32.size() * sizeof(P32), &d_p32)) ||
@@ -1039,55 +1086,131 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         (rc = upload(s, pl64.data(), pl64.size() * sizeof(P64), &d_pl64)) ||
         (rc = upload(s, soa->planes, (size_t)soa->n_planes * sizeof(int), &d_plidx)) ||
         (rc = upload(s, a32.data(), a32.size() * sizeof(DAttr<float>), &d_a32)) ||
-        (rc = upload(s, a64.data(), a64.size() * sizeof(DAttr<double>), &d_a64)) ||
-        (rc = upload(s, m32.data(), m32.size() * sizeof(DMat<float>), &d_m32)) ||
-        (rc = upload(s, m64.data(), m64.size() * sizeof(DMat<double>), &d_m64)))
+        (rc = upload(s, a64.data(), a64.size() * sizeof(DAttr<double>), &d_a64)))
         return bail(rc);
-    if (hipMalloc(&s->d_rays, sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&s->d_overflow, sizeof(int)) != hipSuccess)
-        return bail(fail(102, "hipMalloc failed for counters"));
-    if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(103, "upload failed"));
-
-    s->d_m32 = d_m32;
-    s->d_m64 = d_m64;
-    for (int i = 0; i < soa->n_prims; ++i) {
-        const mh_prim& p = soa->prims[i];
-        if (p.kind == MH_PRIM_SPHERE) s->spheres.push_back({i, p.material, p.p0, p.r0});
-    }
-    DLight<float> l32[8];
-    DLight<double> l64[8];
-    int nl = collect_lights(s, soa->materials, soa->n_materials, l32, l64);
-    int has_transmissive = any_transmissive(soa->materials, soa->n_materials);
-
-    auto fill = [&](auto& v, const void* pr, const void* pl, const void* at, const void* mt, auto* lights, auto tag) {
-        typedef decltype(tag) R;
-        v.nodes = (const DNode*)d_nodes;
-        v.n_nodes = (int)nodes.size();
-        v.root_ref = root_ref;
-        v.nodes4 = (const DNode4*)d_nodes4;
-        v.root4 = root4;
-        v.bvh4 = use4 && !b4.nodes.empty() ? 1 : 0;
-        v.prims = (const typename PrimT<R>::type*)pr;
-        v.planes = (const typename PrimT<R>::type*)pl;
-        v.n_planes = soa->n_planes;
-        v.plane_index = (const int*)d_plidx;
-        v.attrs = (const DAttr<R>*)at;
-        v.mats = (const DMat<R>*)mt;
-        v.n_mats = soa->n_materials;
-        v.sky = conv_sky<R>(soa->sky);
-        v.fog = conv_fog<R>(soa->fog);
-        v.fog_kernel = needs_fog_kernel(soa->fog, any_water(soa->materials, soa->n_materials));
-        v.water_level = (R)soa->water_level;
-        v.water_material = soa->water_material;
-        v.has_transmissive = has_transmissive;
-        v.n_lights = nl;
-        for (int i = 0; i < 8; ++i) v.lights[i] = lights[i];
-    };
-    s->fog_host = soa->fog;
-    s->has_water = any_water(soa->materials, soa->n_materials);
-    fill(s->v32, d_p32, d_pl32, d_a32, d_m32, l32, 0.0f);
-    fill(s->v64, d_p64, d_pl64, d_a64, d_m64, l64, 0.0);
+    lay.nodes = d_nodes;
+    lay.nodes4 = d_nodes4;
+    lay.p32 = d_p32;
+    lay.p64 = d_p64;
+    lay.pl32 = d_pl32;
+    lay.pl64 = d_pl64;
+    lay.plidx = d_plidx;
+    lay.a32 = d_a32;
+    lay.a64 = d_a64;
+    s->bvh.n_nodes = soa->n_nodes;
+    s->bvh.n_leaf = soa->n_leaf_prims;
+    s->bvh.max_depth = 0;
+    s->bvh.width4 = lay.bvh4 ? 1 : 0;
+    s->bvh.build_ms = 0.0;
+    s->bvh.device_built = 0;
+    if ((rc = finish_scene(s, soa, lay))) return bail(rc);
     *out = s;
+    return 0;
+}
+
+int mh_scene_create_gpu_bvh(const mh_scene_soa* soa, int max_leaf, int bins, mh_scene** out)
+{
+    if (!soa || !out) return fail(1, "NULL argument");
+    *out = nullptr;
+    // the tree fields are not read: validate the rest
+    mh_scene_soa v = *soa;
+    v.n_nodes = 0;
+    v.nodes = nullptr;
+    v.n_leaf_prims = 0;
+    v.leaf_prims = nullptr;
+    v.n_planes = 0;
+    v.planes = nullptr;
+    int rc = validate(&v, false);
+    if (rc) return rc;
+    if (max_leaf < 0 || max_leaf > 8 || bins < 0 || bins > 64)
+        return fail(5, "BVH build options out of range (max_leaf %d, bins %d)", max_leaf, bins);
+
+    mh_scene* s = new_scene();
+    if (!s) return 101;
+    auto bail = [&](int code) {
+        mh_scene_destroy(s);
+        return code;
+    };
+    bool any_attr = has_attrs(soa);
+    void *d_prims = nullptr, *d_attrs = nullptr;
+    auto free_in = [&]() {
+        if (d_prims) (void)hipFree(d_prims);
+        if (d_attrs) (void)hipFree(d_attrs);
+    };
+    size_t pbytes = sizeof(mh_prim) * (size_t)soa->n_prims;
+    if (pbytes && hipMalloc(&d_prims, pbytes) != hipSuccess) return bail(fail(102, "hipMalloc failed for prims"));
+    if (soa->attrs && soa->n_prims > 0 &&
+        hipMalloc(&d_attrs, sizeof(mh_prim_attr) * (size_t)soa->n_prims) != hipSuccess) {
+        free_in();
+        return bail(fail(102, "hipMalloc failed for attributes"));
+    }
+    GpuBvh g;
+    int depth_err = 0;
+    hipError_t he = hipSuccess;
+    if (pbytes) he = hipMemcpyAsync(d_prims, soa->prims, pbytes, hipMemcpyHostToDevice, s->stream);
+    if (he == hipSuccess && d_attrs)
+        he = hipMemcpyAsync(d_attrs, soa->attrs, sizeof(mh_prim_attr) * (size_t)soa->n_prims, hipMemcpyHostToDevice,
+                            s->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(s->stream);
+    if (he == hipSuccess) he = hipEventRecord(s->ev[0], s->stream);
+    int brc = he == hipSuccess ? gpu_build_scene((const mh_prim*)d_prims, (const mh_prim_attr*)d_attrs, soa->n_prims,
+                                                 any_attr, max_leaf, bins, s->stream, &g, &depth_err)
+                               : (int)he;
+    if (brc == 0) brc = (int)hipEventRecord(s->ev[1], s->stream);
+    if (brc == 0) brc = (int)hipEventSynchronize(s->ev[1]);
+    float ms = 0.f;
+    if (brc == 0) (void)hipEventElapsedTime(&ms, s->ev[0], s->ev[1]);
+    free_in();
+    for (void* p : g.owned) s->bufs.push_back({p, 0});
+    s->device_bytes += g.bytes;
+    if (brc != 0) return bail(fail(100 + brc, "device BVH build failed: %s", hipGetErrorString((hipError_t)brc)));
+    if (depth_err) return bail(fail(5, "BVH depth %d exceeds the GPU traversal bound (60)", depth_err));
+
+    Layout lay;
+    lay.nodes = g.dnodes;
+    lay.n_nodes = g.n_inner;
+    lay.root_ref = g.root_ref;
+    bool use4 = g.n_nodes > 0 && 3 * g.depth4 + 1 <= MH_STACK;
+    lay.bvh4 = use4 && g.n_nodes4 > 0;
+    lay.nodes4 = g.dnodes4;
+    lay.root4 = lay.bvh4 ? g.root4 : g.root_ref;
+    lay.p32 = g.p32;
+    lay.p64 = g.p64;
+    lay.pl32 = g.pl32;
+    lay.pl64 = g.pl64;
+    lay.plidx = g.planes;
+    lay.a32 = g.a32;
+    lay.a64 = g.a64;
+    lay.n_planes = g.n_planes;
+    s->bvh.n_nodes = g.n_nodes;
+    s->bvh.n_leaf = g.n_leaf;
+    s->bvh.max_depth = g.max_depth;
+    s->bvh.width4 = lay.bvh4 ? 1 : 0;
+    s->bvh.build_ms = ms;
+    s->bvh.device_built = 1;
+    s->tree_nodes = g.nodes;
+    s->tree_leaf = g.leaf_prims;
+    if ((rc = finish_scene(s, soa, lay))) return bail(rc);
+    *out = s;
+    return 0;
+}
+
+int mh_scene_bvh_info(mh_scene* s, mh_bvh_info* info)
+{
+    if (!s || !info) return fail(1, "NULL argument");
+    *info = s->bvh;
+    return 0;
+}
+
+int mh_scene_bvh_export(mh_scene* s, mh_bvh_node* nodes, int32_t* leaf_prims)
+{
+    if (!s || !nodes || !leaf_prims) return fail(1, "NULL argument");
+    if (!s->bvh.device_built) return fail(4, "the scene's tree was built on the host (it holds the tree)");
+    HIP_TRY(hipMemcpyAsync(nodes, s->tree_nodes, sizeof(mh_bvh_node) * (size_t)s->bvh.n_nodes, hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipMemcpyAsync(leaf_prims, s->tree_leaf, sizeof(int32_t) * (size_t)s->bvh.n_leaf, hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return 0;
 }
 

@@ -594,6 +594,23 @@ oom:
     return set_err(2, "out of memory");
 }
 
+int mt_scene_soa_no_bvh(const mt_scene *s, mh_scene_soa *out)
+{
+    if (!s || !out) return set_err(1, "NULL argument");
+    memset(out, 0, sizeof *out);
+    out->abi_version = MH_ABI_VERSION;
+    out->n_prims = s->n;
+    out->prims = s->prims;
+    out->attrs = s->attrs;
+    out->n_materials = s->nm;
+    out->materials = s->mats;
+    out->sky = s->sky;
+    out->water_level = s->water_level;
+    out->water_material = s->water_material;
+    out->fog = s->fog;
+    return 0;
+}
+
 int mt_scene_soa(const mt_scene *s, mh_scene_soa *out)
 {
     if (!s || !out) return set_err(1, "NULL argument");

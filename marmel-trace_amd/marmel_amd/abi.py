@@ -98,6 +98,12 @@ class SceneSoa(C.Structure):
                 ("n_planes", C.c_int32), ("planes", C.c_void_p), ("fog", Fog)]
 
 
+class BvhInfo(C.Structure):
+    _fields_ = [("n_nodes", C.c_int32), ("n_leaf", C.c_int32), ("max_depth", C.c_int32),
+                ("width4", C.c_int32), ("device_built", C.c_int32), ("reserved", C.c_int32),
+                ("build_ms", C.c_double)]
+
+
 class Ray(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("ox", "oy", "oz", "dx", "dy", "dz")]
 

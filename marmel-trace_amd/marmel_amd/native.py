@@ -57,6 +57,7 @@ def host():
     _sig(L, "mt_scene_counts", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)])
     _sig(L, "mt_scene_build_bvh", C.c_int, [P, C.POINTER(BvhOptions), C.POINTER(BvhStats)])
     _sig(L, "mt_scene_soa", C.c_int, [P, C.POINTER(abi.SceneSoa)])
+    _sig(L, "mt_scene_soa_no_bvh", C.c_int, [P, C.POINTER(abi.SceneSoa)])
     _sig(L, "mt_camera_create", None, [C.POINTER(C.c_double)] * 3 + [C.c_double, C.c_double,
                                                                       C.POINTER(abi.Camera)])
     _sig(L, "mt_scene_generate", C.c_int, [P, C.POINTER(GenParams), C.POINTER(GenInfo)])
@@ -78,6 +79,9 @@ def hip():
     _sig(L, "mh_device_info", C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
                                         C.POINTER(C.c_size_t)])
     _sig(L, "mh_scene_create", C.c_int, [C.POINTER(abi.SceneSoa), C.POINTER(P)])
+    _sig(L, "mh_scene_create_gpu_bvh", C.c_int, [C.POINTER(abi.SceneSoa), C.c_int, C.c_int, C.POINTER(P)])
+    _sig(L, "mh_scene_bvh_info", C.c_int, [P, C.POINTER(abi.BvhInfo)])
+    _sig(L, "mh_scene_bvh_export", C.c_int, [P, P, P])
     _sig(L, "mh_scene_destroy", None, [P])
     _sig(L, "mh_scene_device_bytes", C.c_size_t, [P])
     _sig(L, "mh_trace_closest", C.c_int, [P, P, C.c_int, C.c_double, C.c_double, C.c_int, P])

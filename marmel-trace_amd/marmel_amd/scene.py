@@ -106,9 +106,10 @@ class HostScene:
         self.bvh_stats = st
         return st
 
-    def soa(self):
+    def soa(self, bvh=True):
+        """Flattened scene; bvh=False leaves the tree out (for a device build)."""
         s = abi.SceneSoa()
-        rc = self._L.mt_scene_soa(self._h, C.byref(s))
+        rc = (self._L.mt_scene_soa if bvh else self._L.mt_scene_soa_no_bvh)(self._h, C.byref(s))
         if rc:
             self._err(rc)
         return s
@@ -165,16 +166,34 @@ def encode_bmp(rgb):
 
 
 class GpuScene:
-    """A scene resident in GPU memory (mh_scene_create)."""
+    """A scene resident in GPU memory (mh_scene_create), or, with
+    gpu_bvh=True, uploaded without a tree and its BVH built on the device
+    (mh_scene_create_gpu_bvh with max_leaf / bins)."""
 
-    def __init__(self, soa):
+    def __init__(self, soa, gpu_bvh=False, max_leaf=0, bins=0):
         self._L = native.hip()
         h = C.c_void_p()
-        rc = self._L.mh_scene_create(C.byref(soa), C.byref(h))
+        if gpu_bvh:
+            rc = self._L.mh_scene_create_gpu_bvh(C.byref(soa), max_leaf, bins, C.byref(h))
+        else:
+            rc = self._L.mh_scene_create(C.byref(soa), C.byref(h))
         if rc:
             raise MarmelError(rc, self._L.mh_last_error().decode())
         self._h = h
         self.n_materials = soa.n_materials
+
+    def bvh_info(self):
+        info = abi.BvhInfo()
+        self._check(self._L.mh_scene_bvh_info(self._h, C.byref(info)))
+        return info
+
+    def bvh_export(self):
+        """(nodes, leaf_prims) of a device-built tree (numpy)."""
+        info = self.bvh_info()
+        nodes = np.zeros(info.n_nodes, abi.NODE_DTYPE)
+        leaf = np.zeros(info.n_leaf, np.int32)
+        self._check(self._L.mh_scene_bvh_export(self._h, abi.ptr(nodes), abi.ptr(leaf)))
+        return nodes, leaf
 
     def close(self):
         if self._h:

@@ -75,6 +75,19 @@ def test_stress_full_size_progressive():
     gs = GpuScene(hs.soa())
     print("stress scene: %d triangles, BVH depth %d, host build %.1fs, device %.0f MiB"
           % (info.n_triangles, st.max_depth, time.time() - t0, gs.device_bytes / 2**20))
+    # the same tree built on the device
+    t1 = time.time()
+    gd = GpuScene(hs.soa(bvh=False), gpu_bvh=True)
+    bi = gd.bvh_info()
+    print("stress device BVH build: %.1f ms (create incl. upload %.0f ms), host build %.1f s, nodes %d"
+          % (bi.build_ms, (time.time() - t1) * 1e3, st.build_seconds, bi.n_nodes))
+    from test_gpu_bvh_build import assert_same_tree, host_tree
+    assert_same_tree(*host_tree(hs), *gd.bvh_export())
+    cam_s = hs.camera(4 / 3)
+    _, la = gs.render(cam_s, 160, 120, 1, 12, precision=FP64)
+    _, lb = gd.render(cam_s, 160, 120, 1, 12, precision=FP64)
+    assert np.array_equal(la, lb)
+    gd.close()
     w, h, depth = 3840, 2160, 12
     cam = hs.camera(w / h)
     _, full = gs.render(cam, w, h, 4, depth, precision=FP32)

@@ -46,6 +46,9 @@ def test_gpu_entry_points_reject_null_arguments():
     assert L.mh_scene_set_materials(None, None, 0) == 1
     assert L.mh_scene_set_sky(None, None) == 1
     assert L.mh_scene_set_fog(None, None) == 1
+    assert L.mh_scene_create_gpu_bvh(None, 0, 0, None) == 1
+    assert L.mh_scene_bvh_info(None, None) == 1
+    assert L.mh_scene_bvh_export(None, None, None) == 1
 
 
 def bvh_arrays(hs):
