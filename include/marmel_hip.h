@@ -167,6 +167,19 @@ typedef struct mh_fog {
     int32_t reserved;
 } mh_fog;
 
+/* Procedural Gerstner ocean of the scene (reference SceneOceanParams,
+ * src/scene.h; the first ocean block of a scene). When enabled, the water
+ * height at (x, z) -- which decides whether a camera ray starts under water
+ * -- is the 4-harmonic surface of src/scene.c:1109-1153 instead of
+ * water_level. The ocean's triangles are ordinary prims (mt_scene_add_ocean). */
+typedef struct mh_ocean {
+    int32_t  enabled;
+    uint32_t seed;               /* harmonic phases                        */
+    mh_vec3  center;             /* center.y: base water level             */
+    mh_vec3  direction;          /* swell direction in the XZ plane        */
+    double   amplitude, wavelength, steepness, chop, chop_wavelength;
+} mh_ocean;
+
 /* Pinhole / thin-lens camera (reference Camera, src/camera.h:17-48). */
 typedef struct mh_camera {
     mh_vec3 position;
@@ -208,6 +221,7 @@ typedef struct mh_scene_soa {
     int32_t             n_planes;
     const int32_t      *planes;          /* plane prim indices, file order   */
     mh_fog              fog;             /* reference Scene.fog              */
+    mh_ocean            ocean;           /* reference Scene.ocean_params     */
 } mh_scene_soa;
 
 /* ------------------------------------------------------------------ */

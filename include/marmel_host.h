@@ -71,6 +71,52 @@ int mt_scene_soa(const mt_scene *s, mh_scene_soa *out);
  * mh_scene_create_gpu_bvh, which builds it on the device. */
 int mt_scene_soa_no_bvh(const mt_scene *s, mh_scene_soa *out);
 
+/* ---- geometry ingest at scale ------------------------------------------ */
+
+/* Placement of an OBJ mesh (reference ObjTransform, src/obj.h; the scene's
+ * `mesh` block, src/scene_desc.h:163-174). A vertex v becomes
+ * rotate((v - c) * k * scale) + position, with c the bounding-box centre
+ * when auto_center (else 0), k = auto_scale / largest extent when
+ * auto_scale > 0 (else 1), rotation about X, then Y, then Z (degrees). */
+typedef struct mt_obj_xform {
+    double  position[3];
+    double  scale[3];
+    double  rotate_deg[3];
+    int32_t smooth;        /* 1: the file's vertex normals where a face has them */
+    int32_t auto_center;
+    double  auto_scale;
+} mt_obj_xform;
+void mt_obj_xform_default(mt_obj_xform *x);
+
+/* Add the triangles of Wavefront OBJ text (v, vn, vt, f; polygons
+ * fan-triangulated; negative indices count back from the file's totals)
+ * with material `material`: the triangles, normals and uvs the reference
+ * loader (src/obj.c:123-300) makes of the same text, parsed on `threads`
+ * threads (0: all cores). Returns the number of triangles added, or < 0. */
+long mt_scene_add_obj(mt_scene *s, const char *data, size_t len, int material,
+                      const mt_obj_xform *xf, int threads);
+long mt_scene_add_obj_file(mt_scene *s, const char *path, int material,
+                           const mt_obj_xform *xf, int threads);
+
+/* A procedural ocean (reference SceneOceanDesc, src/scene_desc.h:177-192;
+ * scene_grow_ocean, src/scene.c:1155-1331): a res_x x res_z Gerstner grid of
+ * smooth triangles over size[0] x size[1] around center, plus side skirts
+ * and a floor when depth > 0. */
+typedef struct mt_ocean_desc {
+    double   center[3];
+    double   size[2];
+    int32_t  res_x, res_z;
+    double   amplitude, wavelength;
+    double   direction[3];
+    double   steepness, chop, chop_wavelength, depth;
+    uint32_t seed;
+    int32_t  reserved;
+} mt_ocean_desc;
+void mt_ocean_default(mt_ocean_desc *o);   /* the reference block's defaults */
+/* Adds the ocean's triangles; the scene's first ocean also sets the water
+ * level, the water material and the water-height surface (mh_ocean). */
+long mt_scene_add_ocean(mt_scene *s, const mt_ocean_desc *o, int material, int threads);
+
 /* Look-at camera with vertical fov in degrees (reference camera_create). */
 void mt_camera_create(const double eye[3], const double at[3], const double up[3],
                       double vfov_deg, double aspect, mh_camera *out);

@@ -354,7 +354,7 @@ __global__ __launch_bounds__(256) void k_adjoint(SceneView<float> sc, CamView<fl
                 camera_ray(cam, uu, vv, l1, l2, &o, &d);
                 if (cam.dome > 1e-4f && sc.water_material >= 0) {
                     V3<float> pd = o + d * cam.dome;
-                    if (pd.y < sc.water_level) o = pd - mk(0.f, 1e-3f, 0.f);
+                    if (pd.y < water_height(sc, pd.x, pd.z)) o = pd - mk(0.f, 1e-3f, 0.f);
                 }
                 uint32_t seed = hash3((uint32_t)x, (uint32_t)y, s);
                 // pass 1: the raw (unclamped) sum of the path's terms
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(256) void k_render_frozen(SceneView<float> sc, CamV
                 camera_ray(cam, uu, vv, l1, l2, &o, &d);
                 if (cam.dome > 1e-4f && sc.water_material >= 0) {
                     V3<float> pd = o + d * cam.dome;
-                    if (pd.y < sc.water_level) o = pd - mk(0.f, 1e-3f, 0.f);
+                    if (pd.y < water_height(sc, pd.x, pd.z)) o = pd - mk(0.f, 1e-3f, 0.f);
                 }
                 acc = acc + pt_radiance_h<FOG>(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq, fh);
             }

@@ -297,6 +297,44 @@ template <typename R> DFog<R> conv_fog(const mh_fog& f)
     return o;
 }
 
+// The ocean's harmonics as the reference forms them for each height query
+// (src/scene.c:1109-1153), once.
+template <typename R> DOcean<R> conv_ocean(const mh_ocean& o)
+{
+    DOcean<R> d;
+    memset(&d, 0, sizeof d);
+    d.on = o.enabled ? 1 : 0;
+    if (!d.on) return d;
+    d.y0 = (R)o.center.y;
+    double dir_x = o.direction.x, dir_z = o.direction.z;
+    double dlen = sqrt(dir_x * dir_x + dir_z * dir_z);
+    if (dlen > 1e-12) {
+        dir_x /= dlen;
+        dir_z /= dlen;
+    } else {
+        dir_x = 1.0;
+        dir_z = 0.0;
+    }
+    double wl = o.wavelength <= 0.0 ? 12.0 : o.wavelength;
+    double amp = o.amplitude;
+    double chop_wl = o.chop_wavelength > 0.0 ? o.chop_wavelength : wl * 0.25;
+    double chop_amp = o.chop > 0.0 ? o.chop : amp * 0.25;
+    const double lambda[4] = {wl, wl * 0.52, chop_wl, chop_wl * 0.40};
+    const double A[4] = {amp, amp * 0.38, chop_amp, chop_amp * 0.32};
+    const double angle[4] = {0.0, 0.628, -0.488, 1.134};
+    const double phi[4] = {0.0, 1.7 + (double)(o.seed % 100u) * 0.03, 3.1 + (double)((o.seed * 7u) % 100u) * 0.03,
+                           0.8 + (double)((o.seed * 13u) % 100u) * 0.03};
+    for (int h = 0; h < 4; ++h) {
+        double ca = cos(angle[h]), sa = sin(angle[h]);
+        d.dx[h] = (R)(ca * dir_x - sa * dir_z);
+        d.dz[h] = (R)(sa * dir_x + ca * dir_z);
+        d.k[h] = (R)((2.0 * 3.14159265358979323846) / lambda[h]);
+        d.phi[h] = (R)phi[h];
+        d.A[h] = (R)A[h];
+    }
+    return d;
+}
+
 int any_water(const mh_material* mats, int n)
 {
     for (int i = 0; i < n; ++i)
@@ -973,6 +1011,7 @@ int finish_scene(mh_scene* s, const mh_scene_soa* soa, const Layout& lay)
         v.n_mats = soa->n_materials;
         v.sky = conv_sky<R>(soa->sky);
         v.fog = conv_fog<R>(soa->fog);
+        v.ocean = conv_ocean<R>(soa->ocean);
         v.fog_kernel = needs_fog_kernel(soa->fog, s->has_water);
         v.water_level = (R)soa->water_level;
         v.water_material = soa->water_material;

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_pathtrace(SceneView<R> sc, CamView<R> c
                 camera_ray(cam, uu, vv, l1, l2, &o, &d);
                 if (cam.dome > R(1e-4) && sc.water_material >= 0) {
                     V3<R> pd = o + d * cam.dome;
-                    if (pd.y < sc.water_level) o = pd - mk(R(0), R(1e-3), R(0));
+                    if (pd.y < water_height(sc, pd.x, pd.z)) o = pd - mk(R(0), R(1e-3), R(0));
                 }
                 acc = acc + pt_radiance<FOG>(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq);
             }

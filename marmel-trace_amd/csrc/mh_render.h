@@ -222,6 +222,20 @@ __device__ V3<R> apply_medium(const DMat<R>& m, R dist, V3<R>* beta, V3<R>* L, V
     return add;
 }
 
+// Water surface height at (x, z): the scene's Gerstner ocean when it has one
+// (reference src/scene.c:1109-1153), else the constant water level. Decides
+// whether a camera ray starts under water: once per camera ray, so it is kept
+// out of line (inlining its four fp64 cosines into the fp64 megakernel gave
+// wrong fp64 images, also in scenes that never call it).
+template <typename R> __device__ __noinline__ R water_height(const SceneView<R>& sc, R x, R z)
+{
+    if (!sc.ocean.on) return sc.water_level;
+    R y = sc.ocean.y0;
+    for (int h = 0; h < 4; ++h)
+        y += sc.ocean.A[h] * mcos(sc.ocean.k[h] * (sc.ocean.dx[h] * x + sc.ocean.dz[h] * z) + sc.ocean.phi[h]);
+    return y;
+}
+
 // ---- fog (reference src/material.c:963-1039, src/pathtrace.c:614-794) --------
 
 // Henyey-Greenstein lobe (1 - g^2) / (1 + g^2 - 2 g cos)^1.5, denominator
@@ -431,7 +445,8 @@ __device__ V3<R> pt_radiance_h(const SceneView<R>& sc, TS ts, V3<R> ro, V3<R> rd
     if (len2(d) <= R(0)) return L;
     V3<R> beta = mk(R(1), R(1), R(1)), beta_f = beta;
     int medium = -1;
-    if (sc.water_material >= 0 && ro.y < sc.water_level && sc.water_material < sc.n_mats) medium = sc.water_material;
+    if (sc.water_material >= 0 && sc.water_material < sc.n_mats && ro.y < water_height(sc, ro.x, ro.z))
+        medium = sc.water_material;
     int last_spec = 1;
     for (int b = 0; b <= max_depth; ++b) {
         DHit<R> h;

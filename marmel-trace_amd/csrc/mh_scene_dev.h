@@ -101,6 +101,14 @@ template <typename R> struct DFog {
     int noise;        // noise_amount > 1e-4 && noise_scale > 1e-4
 };
 
+// Gerstner ocean surface for the water-height query: base level y0 plus
+// four harmonics A cos(k (dx x + dz z) + phi), precomputed on the host.
+template <typename R> struct DOcean {
+    int on;
+    R y0;
+    R dx[4], dz[4], k[4], phi[4], A[4];
+};
+
 template <typename R> struct DLight {
     int prim;
     int mat;
@@ -129,6 +137,7 @@ template <typename R> struct SceneView {
     int n_mats;
     DSky<R> sky;
     DFog<R> fog;
+    DOcean<R> ocean;
     // 1 when fog is on or marched underwater segments can occur: renders
     // launch the kernel variants that carry the fog code
     int fog_kernel;

@@ -107,10 +107,11 @@ __device__ __forceinline__ void start_path(const SceneView<float>& sc0, const Ca
     camera_ray(cam, uu, vv, l1, l2, &o, &d);
     if (cam.dome > 1e-4f && sc.water_material >= 0) {
         V3<float> pd = o + d * cam.dome;
-        if (pd.y < sc.water_level) o = pd - mk(0.f, 1e-3f, 0.f);
+        if (pd.y < water_height(sc, pd.x, pd.z)) o = pd - mk(0.f, 1e-3f, 0.f);
     }
     int medium = -1;
-    if (sc.water_material >= 0 && o.y < sc.water_level && sc.water_material < sc.n_mats) medium = sc.water_material;
+    if (sc.water_material >= 0 && sc.water_material < sc.n_mats && o.y < water_height(sc, o.x, o.z))
+        medium = sc.water_material;
     d = normalize(d);
     uint32_t seed = hash3(x, y, s);
     uint32_t meta = 0u | (1u << 8) | ((uint32_t)(medium + 1) << 9);

@@ -12,6 +12,7 @@
 #define _POSIX_C_SOURCE 200809L
 
 #include "marmel_host.h"
+#include "mt_internal.h"
 
 #include <float.h>
 #include <math.h>
@@ -23,7 +24,7 @@
 
 static __thread char g_err[256];
 
-static int set_err(int code, const char *fmt, ...)
+int mt_set_err(int code, const char *fmt, ...)
 {
     va_list ap;
     va_start(ap, fmt);
@@ -33,26 +34,6 @@ static int set_err(int code, const char *fmt, ...)
 }
 
 const char *mt_last_error(void) { return g_err; }
-
-struct mt_scene {
-    mh_prim *prims;
-    mh_prim_attr *attrs;
-    int n, cap;
-    mh_material *mats;
-    int nm, capm;
-    mh_sky sky;
-    mh_fog fog;
-    double water_level;
-    int water_material;
-    /* BVH */
-    mh_bvh_node *nodes;
-    int n_nodes;
-    int *leaf_prims;
-    int n_leaf;
-    int *planes;
-    int n_planes;
-    int bvh_valid;
-};
 
 static void drop_bvh(mt_scene *s)
 {
@@ -165,13 +146,40 @@ int mt_scene_add_material(mt_scene *s, const mh_material *m)
     return s->nm++;
 }
 
+long mt_reserve(mt_scene *s, long extra, int attrs)
+{
+    if (!s || extra < 0) return -1;
+    if ((long)s->n + extra > 0x7fffffffL) return mt_set_err(-1, "too many primitives");
+    if (s->n + extra > s->cap) {
+        long cap = s->cap ? s->cap : 64;
+        while (cap < s->n + extra) cap *= 2;
+        if (cap > 0x7fffffffL) cap = 0x7fffffffL;
+        mh_prim *np = (mh_prim *)realloc(s->prims, sizeof(mh_prim) * (size_t)cap);
+        if (!np) return mt_set_err(-1, "out of memory");
+        s->prims = np;
+        if (s->attrs) {
+            mh_prim_attr *na = (mh_prim_attr *)realloc(s->attrs, sizeof(mh_prim_attr) * (size_t)cap);
+            if (!na) return mt_set_err(-1, "out of memory");
+            memset(na + s->cap, 0, sizeof(mh_prim_attr) * (size_t)(cap - s->cap));
+            s->attrs = na;
+        }
+        s->cap = (int)cap;
+    }
+    if (attrs && !s->attrs) {
+        s->attrs = (mh_prim_attr *)calloc((size_t)s->cap, sizeof(mh_prim_attr));
+        if (!s->attrs) return mt_set_err(-1, "out of memory");
+    }
+    s->bvh_valid = 0;
+    return s->n;
+}
+
 int mt_scene_add_prims(mt_scene *s, const mh_prim *prims, const mh_prim_attr *attrs, int n)
 {
     if (!s || (!prims && n > 0) || n < 0) return -1;
     if (s->n + n > s->cap) {
         long cap = s->cap ? s->cap : 64;
         while (cap < s->n + n) cap *= 2;
-        if (cap > 0x7fffffffL) return set_err(-1, "too many primitives");
+        if (cap > 0x7fffffffL) return mt_set_err(-1, "too many primitives");
         mh_prim *np = (mh_prim *)realloc(s->prims, sizeof(mh_prim) * (size_t)cap);
         if (!np) return -1;
         s->prims = np;
@@ -401,7 +409,7 @@ static int push_node(nodevec *v)
 
 int mt_scene_build_bvh(mt_scene *s, const mt_bvh_options *opts, mt_bvh_stats *stats)
 {
-    if (!s) return set_err(1, "NULL scene");
+    if (!s) return mt_set_err(1, "NULL scene");
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
     drop_bvh(s);
@@ -421,7 +429,7 @@ int mt_scene_build_bvh(mt_scene *s, const mt_bvh_options *opts, mt_bvh_stats *st
     double *cen = (double *)malloc(sizeof(double) * 3 * (size_t)(nt ? nt : 1));
     if (!s->planes || !idx || !pb || !cen) {
         free(idx); free(pb); free(cen);
-        return set_err(2, "out of memory");
+        return mt_set_err(2, "out of memory");
     }
     np = nt = 0;
     for (int i = 0; i < s->n; ++i) {
@@ -443,7 +451,7 @@ int mt_scene_build_bvh(mt_scene *s, const mt_bvh_options *opts, mt_bvh_stats *st
     task_t *stack = (task_t *)malloc(sizeof(task_t) * 128);
     if (!slot || !stack) {
         free(idx); free(pb); free(cen); free(slot); free(stack);
-        return set_err(2, "out of memory");
+        return mt_set_err(2, "out of memory");
     }
     for (int i = 0; i < nt; ++i) slot[i] = i;
     int sp = 0, scap = 128;
@@ -585,18 +593,18 @@ int mt_scene_build_bvh(mt_scene *s, const mt_bvh_options *opts, mt_bvh_stats *st
     }
     if (max_depth > 60) {
         drop_bvh(s);
-        return set_err(3, "BVH depth %d exceeds the GPU traversal bound (60)", max_depth);
+        return mt_set_err(3, "BVH depth %d exceeds the GPU traversal bound (60)", max_depth);
     }
     return 0;
 oom:
     free(nv.nodes); free(idx); free(pb); free(cen); free(slot); free(stack);
     drop_bvh(s);
-    return set_err(2, "out of memory");
+    return mt_set_err(2, "out of memory");
 }
 
 int mt_scene_soa_no_bvh(const mt_scene *s, mh_scene_soa *out)
 {
-    if (!s || !out) return set_err(1, "NULL argument");
+    if (!s || !out) return mt_set_err(1, "NULL argument");
     memset(out, 0, sizeof *out);
     out->abi_version = MH_ABI_VERSION;
     out->n_prims = s->n;
@@ -608,13 +616,14 @@ int mt_scene_soa_no_bvh(const mt_scene *s, mh_scene_soa *out)
     out->water_level = s->water_level;
     out->water_material = s->water_material;
     out->fog = s->fog;
+    out->ocean = s->ocean;
     return 0;
 }
 
 int mt_scene_soa(const mt_scene *s, mh_scene_soa *out)
 {
-    if (!s || !out) return set_err(1, "NULL argument");
-    if (!s->bvh_valid) return set_err(4, "BVH not built (call mt_scene_build_bvh after the last change)");
+    if (!s || !out) return mt_set_err(1, "NULL argument");
+    if (!s->bvh_valid) return mt_set_err(4, "BVH not built (call mt_scene_build_bvh after the last change)");
     memset(out, 0, sizeof *out);
     out->abi_version = MH_ABI_VERSION;
     out->n_prims = s->n;
@@ -626,6 +635,7 @@ int mt_scene_soa(const mt_scene *s, mh_scene_soa *out)
     out->water_level = s->water_level;
     out->water_material = s->water_material;
     out->fog = s->fog;
+    out->ocean = s->ocean;
     out->n_nodes = s->n_nodes;
     out->nodes = s->nodes;
     out->n_leaf_prims = s->n_leaf;
@@ -681,13 +691,13 @@ long mt_encode_bmp(const unsigned char *rgb, int w, int h, unsigned char *buf, l
 int mt_write_bmp(const char *path, const unsigned char *rgb, int w, int h)
 {
     long n = mt_encode_bmp(rgb, w, h, NULL, 0);
-    if (n < 0 || !path) return set_err(-2, "bmp: invalid arguments");
+    if (n < 0 || !path) return mt_set_err(-2, "bmp: invalid arguments");
     unsigned char *buf = (unsigned char *)malloc((size_t)n);
-    if (!buf) return set_err(-1, "bmp: out of memory");
+    if (!buf) return mt_set_err(-1, "bmp: out of memory");
     mt_encode_bmp(rgb, w, h, buf, n);
     FILE *f = fopen(path, "wb");
     int rc = 0;
-    if (!f || fwrite(buf, 1, (size_t)n, f) != (size_t)n) rc = set_err(-1, "bmp: cannot write '%s'", path);
+    if (!f || fwrite(buf, 1, (size_t)n, f) != (size_t)n) rc = mt_set_err(-1, "bmp: cannot write '%s'", path);
     if (f && fclose(f) != 0) rc = -1;
     free(buf);
     return rc;
@@ -695,13 +705,13 @@ int mt_write_bmp(const char *path, const unsigned char *rgb, int w, int h)
 
 int mt_write_ppm(const char *path, const unsigned char *rgb, int w, int h)
 {
-    if (!path || !rgb || w <= 0 || h <= 0) return set_err(-2, "ppm: invalid arguments");
+    if (!path || !rgb || w <= 0 || h <= 0) return mt_set_err(-2, "ppm: invalid arguments");
     FILE *f = fopen(path, "wb");
-    if (!f) return set_err(-1, "ppm: cannot open '%s'", path);
+    if (!f) return mt_set_err(-1, "ppm: cannot open '%s'", path);
     int rc = 0;
     if (fprintf(f, "P6\n%d %d\n255\n", w, h) < 0) rc = -1;
     size_t n = (size_t)w * (size_t)h * 3;
     if (rc == 0 && fwrite(rgb, 1, n, f) != n) rc = -1;
     if (fclose(f) != 0) rc = -1;
-    return rc ? set_err(-1, "ppm: write failed") : 0;
+    return rc ? mt_set_err(-1, "ppm: write failed") : 0;
 }

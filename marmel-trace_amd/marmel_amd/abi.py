@@ -70,6 +70,12 @@ class Fog(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class Ocean(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("seed", C.c_uint32), ("center", Vec3), ("direction", Vec3),
+                ("amplitude", C.c_double), ("wavelength", C.c_double), ("steepness", C.c_double),
+                ("chop", C.c_double), ("chop_wavelength", C.c_double)]
+
+
 def fog_default():
     """Reference fog_default_params: disabled."""
     return Fog(density=0.0, color=Vec3(0.70, 0.75, 0.80), inscatter_strength=0.50,
@@ -95,7 +101,7 @@ class SceneSoa(C.Structure):
                 ("sky", Sky), ("water_level", C.c_double), ("water_material", C.c_int32),
                 ("n_nodes", C.c_int32), ("nodes", C.c_void_p),
                 ("n_leaf_prims", C.c_int32), ("leaf_prims", C.c_void_p),
-                ("n_planes", C.c_int32), ("planes", C.c_void_p), ("fog", Fog)]
+                ("n_planes", C.c_int32), ("planes", C.c_void_p), ("fog", Fog), ("ocean", Ocean)]
 
 
 class BvhInfo(C.Structure):

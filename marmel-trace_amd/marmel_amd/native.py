@@ -54,6 +54,11 @@ def host():
     _sig(L, "mt_scene_set_sky", None, [P, C.POINTER(abi.Sky)])
     _sig(L, "mt_scene_set_water", None, [P, C.c_double, C.c_int])
     _sig(L, "mt_scene_set_fog", None, [P, C.POINTER(abi.Fog)])
+    _sig(L, "mt_obj_xform_default", None, [C.POINTER(ObjXform)])
+    _sig(L, "mt_scene_add_obj", C.c_long, [P, C.c_char_p, C.c_size_t, C.c_int, C.POINTER(ObjXform), C.c_int])
+    _sig(L, "mt_scene_add_obj_file", C.c_long, [P, C.c_char_p, C.c_int, C.POINTER(ObjXform), C.c_int])
+    _sig(L, "mt_ocean_default", None, [C.POINTER(OceanDesc)])
+    _sig(L, "mt_scene_add_ocean", C.c_long, [P, C.POINTER(OceanDesc), C.c_int, C.c_int])
     _sig(L, "mt_scene_counts", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)])
     _sig(L, "mt_scene_build_bvh", C.c_int, [P, C.POINTER(BvhOptions), C.POINTER(BvhStats)])
     _sig(L, "mt_scene_soa", C.c_int, [P, C.POINTER(abi.SceneSoa)])
@@ -129,6 +134,18 @@ class GenInfo(C.Structure):
     _fields_ = [("eye", C.c_double * 3), ("at", C.c_double * 3), ("up", C.c_double * 3),
                 ("vfov", C.c_double), ("n_prims", C.c_int), ("n_triangles", C.c_int),
                 ("n_spheres", C.c_int), ("n_emissive", C.c_int)]
+
+
+class ObjXform(C.Structure):
+    _fields_ = [("position", C.c_double * 3), ("scale", C.c_double * 3), ("rotate_deg", C.c_double * 3),
+                ("smooth", C.c_int32), ("auto_center", C.c_int32), ("auto_scale", C.c_double)]
+
+
+class OceanDesc(C.Structure):
+    _fields_ = [("center", C.c_double * 3), ("size", C.c_double * 2), ("res_x", C.c_int32), ("res_z", C.c_int32),
+                ("amplitude", C.c_double), ("wavelength", C.c_double), ("direction", C.c_double * 3),
+                ("steepness", C.c_double), ("chop", C.c_double), ("chop_wavelength", C.c_double),
+                ("depth", C.c_double), ("seed", C.c_uint32), ("reserved", C.c_int32)]
 
 
 GEN_SPHERES, GEN_TERRAIN, GEN_ICOSPHERE, GEN_MATERIALS, GEN_FOREST = 1, 2, 3, 4, 5

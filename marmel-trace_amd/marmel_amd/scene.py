@@ -92,6 +92,52 @@ class HostScene:
         """Scene fog (abi.Fog; density 0 disables it)."""
         self._L.mt_scene_set_fog(self._h, C.byref(fog))
 
+    @staticmethod
+    def obj_xform(**kw):
+        """mt_obj_xform with the defaults, fields overridden by keyword."""
+        x = native.ObjXform()
+        native.host().mt_obj_xform_default(C.byref(x))
+        for k, v in kw.items():
+            if isinstance(v, (list, tuple)):
+                getattr(x, k)[:] = [float(t) for t in v]
+            else:
+                setattr(x, k, v)
+        return x
+
+    def add_obj(self, text, material, xform=None, threads=0):
+        """Add the triangles of Wavefront OBJ text (bytes or str); returns the count."""
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        n = self._L.mt_scene_add_obj(self._h, data, len(data), material,
+                                     C.byref(xform) if xform is not None else None, threads)
+        if n < 0:
+            raise MarmelError(int(n), self._L.mt_last_error().decode())
+        return n
+
+    def add_obj_file(self, path, material, xform=None, threads=0):
+        n = self._L.mt_scene_add_obj_file(self._h, str(path).encode(), material,
+                                          C.byref(xform) if xform is not None else None, threads)
+        if n < 0:
+            raise MarmelError(int(n), self._L.mt_last_error().decode())
+        return n
+
+    @staticmethod
+    def ocean_desc(**kw):
+        """mt_ocean_desc with the reference block's defaults, overridden by keyword."""
+        o = native.OceanDesc()
+        native.host().mt_ocean_default(C.byref(o))
+        for k, v in kw.items():
+            if isinstance(v, (list, tuple)):
+                getattr(o, k)[:] = [float(t) for t in v]
+            else:
+                setattr(o, k, v)
+        return o
+
+    def add_ocean(self, desc, material, threads=0):
+        n = self._L.mt_scene_add_ocean(self._h, C.byref(desc), material, threads)
+        if n < 0:
+            raise MarmelError(int(n), self._L.mt_last_error().decode())
+        return n
+
     def counts(self):
         n, m = C.c_int(), C.c_int()
         self._L.mt_scene_counts(self._h, C.byref(n), C.byref(m))
@@ -124,7 +170,7 @@ class HostScene:
 
     # ---- numpy views of the flattened scene (for checkers) ----------------
     def arrays(self):
-        s = self.soa()
+        s = self.soa(bvh=False)
         n, m = s.n_prims, s.n_materials
         prims = np.ctypeslib.as_array(C.cast(s.prims, C.POINTER(C.c_uint8)),
                                       (n * abi.PRIM_DTYPE.itemsize,)).view(abi.PRIM_DTYPE).copy()
@@ -135,6 +181,10 @@ class HostScene:
             attrs = np.ctypeslib.as_array(C.cast(s.attrs, C.POINTER(C.c_uint8)),
                                           (n * abi.ATTR_DTYPE.itemsize,)).view(abi.ATTR_DTYPE).copy()
         return prims, attrs, mats, s.sky, s.water_level, s.water_material
+
+    def ocean(self):
+        """The scene's mh_ocean (enabled once an ocean was added)."""
+        return abi.Ocean.from_buffer_copy(bytes(self.soa(bvh=False).ocean))
 
 
 def make_camera(eye, at, up, vfov, aspect):

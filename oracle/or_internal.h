@@ -94,6 +94,7 @@ typedef struct or_scene {
     mh_material *mats;
     mh_sky sky;
     mh_fog fog;
+    mh_ocean ocean;
     double water_level;
     int water_material;
     int has_transmissive;   /* some material is water or transparency > 0.8 */

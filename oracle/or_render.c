@@ -8,6 +8,7 @@
  *   PT lobe sampling       reference src/pathtrace.c:141-200, 220-233, 251-453
  *   PT NEE + shadow        reference src/pathtrace.c:504-606
  *   PT radiance            reference src/pathtrace.c:800-1028
+ *   water height           reference src/scene.c:1109-1153 (Gerstner ocean)
  *   fog                    reference src/material.c:963-1039 (closed-form
  *                          segment), src/pathtrace.c:614-729 (shadow-marched
  *                          camera segment), 737-794 (marched underwater segment)
@@ -300,6 +301,42 @@ static int occluded_segment(const or_scene *s, oray r, double tmin, double tmax,
         return 1;
     }
     return 1;
+}
+
+/* ---- water height ----------------------------------------------------- */
+
+/* Water surface height at (x, z): the base level plus four cosine harmonics
+ * of the scene's Gerstner ocean when it has one, else the water level. */
+static double water_height(const or_scene *s, double x, double z)
+{
+    const mh_ocean *o = &s->ocean;
+    if (!o->enabled) return s->water_level;
+    double y = o->center.y;
+    double dir_x = o->direction.x, dir_z = o->direction.z;
+    double dlen = sqrt(dir_x * dir_x + dir_z * dir_z);
+    if (dlen > 1e-12) {
+        dir_x /= dlen;
+        dir_z /= dlen;
+    } else {
+        dir_x = 1.0;
+        dir_z = 0.0;
+    }
+    double wl = o->wavelength <= 0.0 ? 12.0 : o->wavelength;
+    double amp = o->amplitude;
+    double chop_wl = o->chop_wavelength > 0.0 ? o->chop_wavelength : wl * 0.25;
+    double chop_amp = o->chop > 0.0 ? o->chop : amp * 0.25;
+    const double lambda[4] = { wl, wl * 0.52, chop_wl, chop_wl * 0.40 };
+    const double A[4] = { amp, amp * 0.38, chop_amp, chop_amp * 0.32 };
+    const double angle[4] = { 0.0, 0.628, -0.488, 1.134 };
+    const double phi[4] = { 0.0, 1.7 + (double)(o->seed % 100) * 0.03, 3.1 + (double)((o->seed * 7) % 100) * 0.03,
+                            0.8 + (double)((o->seed * 13) % 100) * 0.03 };
+    for (int h = 0; h < 4; ++h) {
+        double ca = cos(angle[h]), sa = sin(angle[h]);
+        double dx = ca * dir_x - sa * dir_z, dz = sa * dir_x + ca * dir_z;
+        double k = (2.0 * OR_PI) / lambda[h];
+        y += A[h] * cos(k * (dx * x + dz * z) + phi[h]);
+    }
+    return y;
 }
 
 /* ---- fog ------------------------------------------------------------- */
@@ -793,7 +830,8 @@ static dv3 pt_radiance(const or_scene *s, oray primary, int max_depth, unsigned 
     if (dlen2(r.d) <= 0.0) return L;
     dv3 beta = d3(1.0, 1.0, 1.0), beta_f = beta;
     const mh_material *medium = NULL;
-    if (s->water_material >= 0 && primary.o.y < s->water_level) medium = mat_of(s, s->water_material);
+    if (s->water_material >= 0 && primary.o.y < water_height(s, primary.o.x, primary.o.z))
+        medium = mat_of(s, s->water_material);
     int last_spec = 1;
     for (int b = 0; b <= max_depth; ++b) {
         ohit h;
@@ -1171,7 +1209,7 @@ static void render_pixel(job_t *j, int x, int y)
             oray ray = cam_ray(j->cam, uu, vv, l1, l2);
             if (j->cam->dome_radius > 1e-4 && s->water_material >= 0) {
                 dv3 pd = dadd(ray.o, dscale(ray.d, j->cam->dome_radius));
-                if (pd.y < s->water_level) ray.o = dsub(pd, d3(0.0, 1e-3, 0.0));
+                if (pd.y < water_height(s, pd.x, pd.z)) ray.o = dsub(pd, d3(0.0, 1e-3, 0.0));
             }
             c = pt_radiance(s, ray, j->depth, or_hash3((unsigned)x, (unsigned)y, (unsigned)sidx), NULL);
         } else {
@@ -1323,7 +1361,7 @@ static void adj_pixel(adj_job_t *j, int x, int y, or_adj *a)
         oray ray = cam_ray(j->cam, uu, vv, l1, l2);
         if (j->cam->dome_radius > 1e-4 && s->water_material >= 0) {
             dv3 pd = dadd(ray.o, dscale(ray.d, j->cam->dome_radius));
-            if (pd.y < s->water_level) ray.o = dsub(pd, d3(0.0, 1e-3, 0.0));
+            if (pd.y < water_height(s, pd.x, pd.z)) ray.o = dsub(pd, d3(0.0, 1e-3, 0.0));
         }
         unsigned seed = or_hash3((unsigned)x, (unsigned)y, (unsigned)sidx);
         /* pass 1: the raw (unclamped) sum of the path's terms; the final
@@ -1472,6 +1510,7 @@ or_scene *or_scene_create(const mh_scene_soa *soa)
     s->fog = soa->fog;
     s->water_level = soa->water_level;
     s->water_material = soa->water_material;
+    s->ocean = soa->ocean;
     for (int i = 0; i < s->n_mats; ++i)
         if (s->mats[i].is_water || s->mats[i].transparency > 0.8) s->has_transmissive = 1;
     for (int i = 0; i < soa->n_prims; ++i) {

@@ -19,6 +19,7 @@
 #include "geometry.h"
 #include "material.h"
 #include "noise.h"
+#include "obj.h"
 #include "pathtrace.h"
 #include "render.h"
 #include "sampling.h"
@@ -136,6 +137,18 @@ void *mtref_from_soa(const mh_scene_soa *soa)
     s->fog.noise_scale = soa->fog.noise_scale;
     s->fog.noise_amount = soa->fog.noise_amount;
     s->fog.shadow_steps = soa->fog.shadow_steps;
+    if (soa->ocean.enabled) {
+        s->has_ocean = 1;
+        s->ocean_params.enabled = 1;
+        s->ocean_params.center = v3(soa->ocean.center);
+        s->ocean_params.direction = v3(soa->ocean.direction);
+        s->ocean_params.amplitude = soa->ocean.amplitude;
+        s->ocean_params.wavelength = soa->ocean.wavelength;
+        s->ocean_params.steepness = soa->ocean.steepness;
+        s->ocean_params.chop = soa->ocean.chop;
+        s->ocean_params.chop_wavelength = soa->ocean.chop_wavelength;
+        s->ocean_params.seed = soa->ocean.seed;
+    }
 
     s->material_count = soa->n_materials;
     s->materials = (Material *)calloc((size_t)(soa->n_materials > 0 ? soa->n_materials : 1), sizeof(Material));
@@ -233,6 +246,41 @@ int mtref_counts(void *h, int *n_prims, int *n_materials, int *n_lights)
     return 0;
 }
 
+/* One reference Primitive in the ABI record; returns 4 for kinds outside
+ * the GPU path. */
+static int flatten_prim(const Primitive *p, mh_prim *q, mh_prim_attr *at)
+{
+    memset(q, 0, sizeof *q);
+    memset(at, 0, sizeof *at);
+    q->material = p->material_index;
+    switch (p->kind) {
+    case PRIM_SPHERE:
+        q->kind = MH_PRIM_SPHERE; q->p0 = m3(p->center); q->r0 = p->radius; return 0;
+    case PRIM_PLANE:
+        q->kind = MH_PRIM_PLANE; q->p0 = m3(p->center); q->p1 = m3(p->axis); return 0;
+    case PRIM_BOX:
+        q->kind = MH_PRIM_BOX; q->p0 = m3(p->center); q->p1 = m3(p->half); return 0;
+    case PRIM_TRIANGLE:
+        q->kind = MH_PRIM_TRIANGLE;
+        q->p0 = m3(p->a); q->p1 = m3(p->b); q->p2 = m3(p->c);
+        if (p->radius > 0.5) {
+            q->flags |= MH_PRIM_SMOOTH;
+            at->n0 = m3(p->center); at->n1 = m3(p->axis); at->n2 = m3(p->half);
+        }
+        if (p->has_uv) {
+            q->flags |= MH_PRIM_UV;
+            at->uv0 = m3(p->uva); at->uv1 = m3(p->uvb); at->uv2 = m3(p->uvc);
+        }
+        return 0;
+    case PRIM_CYLINDER:
+        q->kind = MH_PRIM_CYLINDER;
+        q->p0 = m3(p->a); q->p1 = m3(p->b); q->r0 = p->radius; q->r1 = p->radius2; return 0;
+    default:
+        q->kind = -1;
+        return 4;
+    }
+}
+
 /* Export the reference scene in the flattened ABI layout. Returns 0, or 4
  * when the scene holds a primitive kind outside the GPU path. */
 int mtref_export(void *h, mh_prim *prims, mh_prim_attr *attrs, mh_material *mats,
@@ -242,38 +290,8 @@ int mtref_export(void *h, mh_prim *prims, mh_prim_attr *attrs, mh_material *mats
     if (!rs) return 1;
     const Scene *s = &rs->scene;
     int rc = 0;
-    for (int i = 0; i < s->geo.count; ++i) {
-        const Primitive *p = &s->geo.prims[i];
-        mh_prim *q = &prims[i];
-        memset(q, 0, sizeof *q);
-        memset(&attrs[i], 0, sizeof attrs[i]);
-        q->material = p->material_index;
-        switch (p->kind) {
-        case PRIM_SPHERE:
-            q->kind = MH_PRIM_SPHERE; q->p0 = m3(p->center); q->r0 = p->radius; break;
-        case PRIM_PLANE:
-            q->kind = MH_PRIM_PLANE; q->p0 = m3(p->center); q->p1 = m3(p->axis); break;
-        case PRIM_BOX:
-            q->kind = MH_PRIM_BOX; q->p0 = m3(p->center); q->p1 = m3(p->half); break;
-        case PRIM_TRIANGLE:
-            q->kind = MH_PRIM_TRIANGLE;
-            q->p0 = m3(p->a); q->p1 = m3(p->b); q->p2 = m3(p->c);
-            if (p->radius > 0.5) {
-                q->flags |= MH_PRIM_SMOOTH;
-                attrs[i].n0 = m3(p->center); attrs[i].n1 = m3(p->axis); attrs[i].n2 = m3(p->half);
-            }
-            if (p->has_uv) {
-                q->flags |= MH_PRIM_UV;
-                attrs[i].uv0 = m3(p->uva); attrs[i].uv1 = m3(p->uvb); attrs[i].uv2 = m3(p->uvc);
-            }
-            break;
-        case PRIM_CYLINDER:
-            q->kind = MH_PRIM_CYLINDER;
-            q->p0 = m3(p->a); q->p1 = m3(p->b); q->r0 = p->radius; q->r1 = p->radius2; break;
-        default:
-            q->kind = -1; rc = 4; break;
-        }
-    }
+    for (int i = 0; i < s->geo.count; ++i)
+        if (flatten_prim(&s->geo.prims[i], &prims[i], &attrs[i])) rc = 4;
     for (int i = 0; i < s->material_count; ++i) {
         const Material *m = &s->materials[i];
         mh_material *a = &mats[i];
@@ -313,8 +331,47 @@ int mtref_export(void *h, mh_prim *prims, mh_prim_attr *attrs, mh_material *mats
     }
     *water_level = s->water_level;
     *water_material = s->water_material;
-    if (s->has_ocean) rc = 4;
     return rc;
+}
+
+/* The scene's ocean (the water-height surface); returns has_ocean. */
+int mtref_ocean(void *h, mh_ocean *o)
+{
+    RefScene *rs = (RefScene *)h;
+    memset(o, 0, sizeof *o);
+    if (!rs || !rs->scene.has_ocean) return 0;
+    const SceneOceanParams *p = &rs->scene.ocean_params;
+    o->enabled = 1;
+    o->seed = p->seed;
+    o->center = m3(p->center);
+    o->direction = m3(p->direction);
+    o->amplitude = p->amplitude;
+    o->wavelength = p->wavelength;
+    o->steepness = p->steepness;
+    o->chop = p->chop;
+    o->chop_wavelength = p->chop_wavelength;
+    return 1;
+}
+
+/* The reference OBJ loader (obj_load_mem) on text: up to cap triangles in
+ * the ABI record; returns the loader's count (or < 0). */
+int mtref_obj_prims(const char *data, size_t len, int material, const double pos[3], const double scale[3],
+                    const double rot[3], int smooth, int auto_center, double auto_scale, mh_prim *out,
+                    mh_prim_attr *attrs, int cap)
+{
+    Geometry g;
+    geometry_init(&g);
+    ObjTransform xf = obj_transform_default();
+    xf.position = vec3(pos[0], pos[1], pos[2]);
+    xf.scale = vec3(scale[0], scale[1], scale[2]);
+    xf.rotation_deg = vec3(rot[0], rot[1], rot[2]);
+    xf.smooth_normals = smooth;
+    xf.auto_center = auto_center;
+    xf.auto_scale = auto_scale;
+    int n = obj_load_mem(data, len, &g, material, &xf);
+    for (int i = 0; i < g.count && i < cap; ++i) flatten_prim(&g.prims[i], &out[i], &attrs[i]);
+    geometry_free(&g);
+    return n;
 }
 
 static Camera cam_from(const mh_camera *c)

@@ -105,6 +105,9 @@ def ref():
         _sig(L, "mtref_water_normal", None, [P, C.c_int, P])
         _sig(L, "mtref_noise", None, [P, C.c_int, C.c_int, C.c_uint, P, P])
         _sig(L, "mtref_noise3", None, [P, C.c_int, C.c_uint, P, P])
+        _sig(L, "mtref_ocean", C.c_int, [P, C.POINTER(abi.Ocean)])
+        _sig(L, "mtref_obj_prims", C.c_int, [C.c_char_p, C.c_size_t, C.c_int, DP, DP, DP, C.c_int, C.c_int, D,
+                                             P, P, C.c_int])
         _sig(L, "mtref_sun_disk_dir", None, [DP, D, P, C.c_int, P])
         _sig(L, "mtref_light_cone_dir", None, [DP, D, P, C.c_int, P])
         _sig(L, "mtref_cosine_dir", None, [DP, P, C.c_int, P])
@@ -124,7 +127,7 @@ def d3(v):
 class SoaHolder:
     """Keeps numpy arrays alive behind an mh_scene_soa (no BVH)."""
 
-    def __init__(self, prims, attrs, mats, sky, water_level=0.0, water_material=-1, fog=None):
+    def __init__(self, prims, attrs, mats, sky, water_level=0.0, water_material=-1, fog=None, ocean=None):
         self.prims = np.ascontiguousarray(prims, dtype=abi.PRIM_DTYPE)
         self.attrs = None if attrs is None else np.ascontiguousarray(attrs, dtype=abi.ATTR_DTYPE)
         self.mats = np.ascontiguousarray(mats, dtype=abi.MATERIAL_DTYPE)
@@ -133,7 +136,8 @@ class SoaHolder:
                                 prims=abi.ptr(self.prims), attrs=abi.ptr(self.attrs),
                                 n_materials=len(self.mats), materials=abi.ptr(self.mats),
                                 sky=sky, water_level=water_level, water_material=water_material,
-                                fog=abi.fog_default() if fog is None else fog)
+                                fog=abi.fog_default() if fog is None else fog,
+                                ocean=abi.Ocean() if ocean is None else ocean)
 
 
 class OracleScene:

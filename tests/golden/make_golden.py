@@ -115,6 +115,28 @@ def fog_scenes():
     np.savez_compressed(os.path.join(HERE, "fog.npz"), **out)
 
 
+def ingest():
+    """Reference OBJ triangles of tests/ingest_data.torus_obj() under each
+    placement of test_ingest.XFORMS (hashes), and the reference ocean of
+    test_ingest.OCEAN_TEXT (hash, material, ocean parameters)."""
+    import ingest_data
+    import test_ingest as ti
+    text = ingest_data.torus_obj()
+    out = {"torus_text_sha": np.frombuffer(hashlib.sha256(text).hexdigest().encode(), np.uint8).copy()}
+    for name, xf in ti.XFORMS.items():
+        n, p, a = ti.reference_obj(text, xf)
+        out[f"{name}_count"] = np.array(n)
+        out[f"{name}_sha"] = np.frombuffer(ti.digest(p, a).encode(), np.uint8).copy()
+    r = ck.RefScene(text=ti.OCEAN_TEXT)
+    rc, pr, ar, mats, sky, wl, wm, fog = r.export()
+    ro = abi.Ocean()
+    ck.ref().mtref_ocean(r.h, C.byref(ro))
+    out["ocean_material"] = np.array(wm)
+    out["ocean_sha"] = np.frombuffer(ti.digest(pr, ar).encode(), np.uint8).copy()
+    out["ocean_params"] = sbytes(ro)
+    np.savez_compressed(os.path.join(HERE, "ingest.npz"), **out)
+
+
 def vectors(r, sky):
     """Reference outputs of the per-function routines on seeded inputs."""
     L = ck.ref()
@@ -169,6 +191,7 @@ def main():
     vectors(r, sky)
     host_scene_images()
     fog_scenes()
+    ingest()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

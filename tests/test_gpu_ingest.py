@@ -1,0 +1,134 @@
+"""Ingested geometry on the GPU: OBJ meshes and Gerstner oceans.
+
+  * an ocean world (Gerstner surface with skirts and floor, spheres above
+    and below the waves) seen through a dome camera, so that camera rays
+    start above or below the moving surface pixel by pixel: fp64 Whitted
+    equals the oracle (which equals the reference on this content,
+    tests/test_ingest.py) within 1/255 on >= 99.9 % of pixels, the fp64 path
+    tracer on >= 99.8 %, and the fp32 path tracer (megakernel and wavefront)
+    is statistically consistent with a converged oracle render,
+  * the seeded OBJ torus (smooth and flat, uv) renders as the oracle does,
+  * at scale: a 2,000,000-triangle OBJ height field ingested on the host
+    threads and its tree built on the device renders identically to the
+    host-built tree (timings printed).
+"""
+import time
+
+import numpy as np
+import pytest
+
+import checkers as ck
+import ingest_data
+import scenes
+from marmel_amd import GpuScene, HostScene, abi
+from test_ingest import ocean_world
+
+pytestmark = pytest.mark.gpu
+
+FP32, FP64 = abi.PREC_FP32, abi.PREC_FP64
+
+
+def within(a, b, tol=1):
+    return (np.abs(a.astype(int) - b.astype(int)) <= tol).all(axis=-1).mean()
+
+
+def dome_camera(hs, aspect):
+    cam = hs.camera(aspect, eye=[0.0, 0.25, -9.0], at=[0.0, -0.2, 4.0], up=[0, 1, 0], vfov=55)
+    cam.dome_radius = 1.5
+    return cam
+
+
+def test_ocean_world_fp64_matches_oracle():
+    hs = ocean_world()
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    o = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm, ocean=hs.ocean()).soa)
+    gs = GpuScene(hs.soa())
+    try:
+        w, h = 96, 72
+        cam = dome_camera(hs, w / h)
+        fr = {}
+        for mode, depth in ((0, 8), (1, 5)):
+            ref, m, _, _ = o.render(cam, w, h, 2, depth, mode=mode)
+            rgb, lin = gs.render(cam, w, h, 2, depth, mode=mode, precision=FP64)
+            fr[mode] = within(rgb, ref)
+            off = np.abs(rgb.astype(int) - ref.astype(int)).max(-1) > 1
+            print("ocean world mode", mode, "fp64 within 1/255 of the oracle:", fr[mode], "pixels off",
+                  int(off.sum()), "linear rel. diff there", np.abs(lin - m)[off].max(initial=0.0))
+            if mode == 0:
+                mean = m
+        # Whitted: the 1/255 bar. Path tracing: the bar is statistical (below);
+        # through transparent water every path takes many refraction and
+        # roulette decisions, where last-bit differences of the device math
+        # (exp, sqrt) flip a few paths, so the fp64 image check is 99.8 %.
+        assert fr[1] >= 0.999
+        assert fr[0] >= 0.998
+        # the dome point decides the starting medium: both kinds occur
+        flat = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm).soa)
+        _, mflat, _, _ = flat.render(cam, w, h, 2, 8)
+        print("pixels whose start medium depends on the waves:", (np.abs(mflat - mean) > 1e-9).any(-1).mean())
+        # fp32 engines against a converged oracle render
+        ws, hs2 = 40, 30
+        cam2 = dome_camera(hs, ws / hs2)
+        _, rm, rv, _ = o.render(cam2, ws, hs2, 1024, 8, want_var=True, sample_offset=5000)
+        se = np.sqrt(rv / 64 + rv / 1024) + 1e-6
+        for eng in (abi.ENGINE_MEGAKERNEL, abi.ENGINE_WAVEFRONT):
+            _, l32 = gs.render(cam2, ws, hs2, 64, 8, precision=FP32, engine=eng)
+            z = (l32 - rm) / se
+            print("ocean world fp32 engine", eng, "within 3 sigma", (np.abs(z) <= 3).mean(), "mean z", z.mean())
+            assert (np.abs(z) <= 3).mean() >= 0.98 and abs(z.mean()) < 0.15
+    finally:
+        gs.close()
+
+
+def test_obj_torus_fp64_matches_oracle():
+    for smooth in (1, 0):
+        hs = HostScene()
+        hs.add_materials(np.array([scenes.material(albedo=(0.6, 0.4, 0.3), pbr=1, roughness=0.4, metallic=0.2),
+                                   scenes.material(albedo=(0.5, 0.5, 0.5))], dtype=abi.MATERIAL_DTYPE))
+        hs.add_obj(ingest_data.torus_obj(), 0, HostScene.obj_xform(smooth=smooth, position=[0, 1.2, 0]))
+        hs.add_prims(np.array([scenes.prim(abi.PRIM_PLANE, 1, p0=(0, -0.5, 0), p1=(0, 1, 0))], dtype=abi.PRIM_DTYPE))
+        hs.set_sky(scenes.sky(horizon=(0.7, 0.8, 0.9), zenith=(0.2, 0.35, 0.8), sun_color=(3, 2.8, 2.5),
+                              sun_dir=(0.3, 0.8, -0.5)))
+        hs.build_bvh()
+        prims, attrs, mats, sky, wl, wm = hs.arrays()
+        o = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm).soa)
+        gs = GpuScene(hs.soa())
+        try:
+            cam = hs.camera(4 / 3, eye=[0, 4, -6], at=[0, 1, 0], up=[0, 1, 0], vfov=45)
+            for mode in (0, 1):
+                ref, _, _, _ = o.render(cam, 80, 60, 2, 6, mode=mode)
+                rgb, _ = gs.render(cam, 80, 60, 2, 6, mode=mode, precision=FP64)
+                frac = within(rgb, ref)
+                print("torus smooth", smooth, "mode", mode, "fp64 within 1/255:", frac)
+                assert frac >= 0.999
+        finally:
+            gs.close()
+
+
+def test_obj_at_scale_ingest_build_render():
+    text = ingest_data.big_obj(1000)
+    hs = HostScene()
+    hs.add_materials(np.array([scenes.material(albedo=(0.5, 0.6, 0.4))], dtype=abi.MATERIAL_DTYPE))
+    t0 = time.time()
+    n = hs.add_obj(text, 0, threads=16)
+    t_ingest = time.time() - t0
+    hs.set_sky(scenes.sky(horizon=(0.7, 0.8, 0.9), zenith=(0.2, 0.35, 0.8), sun_color=(3, 2.8, 2.5),
+                          sun_dir=(0.3, 0.8, -0.5)))
+    gd = GpuScene(hs.soa(bvh=False), gpu_bvh=True)
+    info = gd.bvh_info()
+    t0 = time.time()
+    st = hs.build_bvh()
+    t_host = time.time() - t0
+    gh = GpuScene(hs.soa())
+    try:
+        print("OBJ %.0f MB, %d triangles: host ingest %.0f ms (16 threads), device BVH %.1f ms, host BVH %.0f ms"
+              % (len(text) / 1e6, n, t_ingest * 1e3, info.build_ms, t_host * 1e3))
+        assert n == 2 * 999 * 999
+        cam = hs.camera(16 / 9, eye=[0, 30, -60], at=[0, 0, 0], up=[0, 1, 0], vfov=50)
+        a, la = gd.render(cam, 320, 180, 2, 6, precision=FP64)
+        b, lb = gh.render(cam, 320, 180, 2, 6, precision=FP64)
+        assert np.array_equal(la, lb)
+        assert np.isfinite(la).all() and la.mean() > 0.01
+    finally:
+        gd.close()
+        gh.close()
