@@ -17,6 +17,9 @@
  *   Whitted                reference src/render.c:507-863 (trace/trace_hit,
  *                          emissive_direct), 874-889 (sample_offset),
  *                          924-970 (render_region), 842-863 (fog)
+ *   adaptive Whitted       reference src/render.c:1128-1496 (per-pixel relative
+ *                          standard error of the mean luminance, refinement
+ *                          passes of n0 samples), 1627-1665 (defaults)
  */
 #define _POSIX_C_SOURCE 200809L
 
@@ -1329,6 +1332,164 @@ int or_render(const or_scene *s, const mh_camera *cam, int width, int height, in
 }
 
 void or_render_stats_get(long long out[OR_NSTATS]) { memcpy(out, g_render_stats, sizeof g_render_stats); }
+
+/* ---- adaptive Whitted ----------------------------------------------------- */
+
+typedef struct {
+    const or_scene *s;
+    const mh_camera *cam;
+    int w, h, n0, nmax, depth;
+    double *sum;            /* R, G, B, Y, Y^2 per pixel */
+    int *ns;
+    const unsigned char *active;
+    atomic_int next_row;
+} adapt_job;
+
+/* One refinement pass of one pixel: up to n0 more samples, continuing the
+ * pixel's global sample index; the fixed render's jitter for s < n0, uniform
+ * jitter after. The pass's sums are added to the running sums at the end. */
+static void adapt_pixel(adapt_job *j, int x, int y)
+{
+    size_t p = (size_t)y * (size_t)j->w + (size_t)x;
+    int n = j->ns[p], b = j->n0;
+    if (b > j->nmax - n) b = j->nmax - n;
+    if (b <= 0) return;
+    int g = (int)(sqrt((double)j->n0) + 0.5);
+    double sr = 0.0, sg = 0.0, sb = 0.0, sy = 0.0, sy2 = 0.0;
+    for (int k = 0; k < b; ++k) {
+        int s = n + k;
+        double du, dv;
+        if (s < j->n0 && g > 0 && g * g == j->n0) {
+            double jx = or_rand01((unsigned)x, (unsigned)y, (unsigned)(s * 2 + 0));
+            double jy = or_rand01((unsigned)x, (unsigned)y, (unsigned)(s * 2 + 1));
+            du = ((double)(s % g) + jx) / (double)g;
+            dv = ((double)(s / g) + jy) / (double)g;
+        } else {
+            du = or_rand01((unsigned)x, (unsigned)y, (unsigned)(s * 2 + 0));
+            dv = or_rand01((unsigned)x, (unsigned)y, (unsigned)(s * 2 + 1));
+        }
+        double uu = ((double)x + du) / (double)j->w;
+        double vv = 1.0 - ((double)y + dv) / (double)j->h;
+        double l1 = or_rand01((unsigned)x, (unsigned)y, (unsigned)(s * 2 + 2));
+        double l2 = or_rand01((unsigned)x, (unsigned)y, (unsigned)(s * 2 + 3));
+        oray ray = cam_ray(j->cam, uu, vv, l1, l2);
+        dv3 c = w_trace(j->s, ray, 0, j->depth, or_hash3((unsigned)x, (unsigned)y, (unsigned)s));
+        sr += c.x;
+        sg += c.y;
+        sb += c.z;
+        double Y = 0.2126 * c.x + 0.7152 * c.y + 0.0722 * c.z;
+        sy += Y;
+        sy2 += Y * Y;
+    }
+    double *q = &j->sum[5 * p];
+    q[0] += sr;
+    q[1] += sg;
+    q[2] += sb;
+    q[3] += sy;
+    q[4] += sy2;
+    j->ns[p] = n + b;
+}
+
+static void *adapt_worker(void *arg)
+{
+    adapt_job *j = (adapt_job *)arg;
+    for (;;) {
+        int y = atomic_fetch_add(&j->next_row, 1);
+        if (y >= j->h) break;
+        for (int x = 0; x < j->w; ++x)
+            if (j->active[(size_t)y * (size_t)j->w + (size_t)x]) adapt_pixel(j, x, y);
+    }
+    return NULL;
+}
+
+static void adapt_pass(adapt_job *j, int threads)
+{
+    atomic_store(&j->next_row, 0);
+    pthread_t tid[64];
+    int started = 0;
+    for (int i = 1; i < threads; ++i)
+        if (pthread_create(&tid[started], NULL, adapt_worker, j) == 0) ++started;
+    adapt_worker(j);
+    for (int i = 0; i < started; ++i) pthread_join(tid[i], NULL);
+}
+
+int or_render_adaptive(const or_scene *s, const mh_camera *cam, int width, int height, int n0, int depth,
+                       int nmax, double tau, int threads, unsigned char *rgb, double *mean, int *nsamp,
+                       long long *total, int *max_n)
+{
+    if (!s || !cam) return 1;
+    if (width <= 0 || height <= 0) return 2;
+    if (n0 < 1 || depth < 0) return 3;
+    if (nmax <= 0) nmax = 4 * n0;
+    if (!(tau > 0.0)) tau = 0.02;
+    if (nmax < n0) nmax = n0;
+    size_t np = (size_t)width * (size_t)height;
+    adapt_job j;
+    j.s = s; j.cam = cam; j.w = width; j.h = height; j.n0 = n0; j.nmax = nmax; j.depth = depth;
+    j.sum = (double *)calloc(5 * np, sizeof(double));
+    j.ns = (int *)calloc(np, sizeof(int));
+    unsigned char *act = (unsigned char *)malloc(np);
+    if (!j.sum || !j.ns || !act) {
+        free(j.sum); free(j.ns); free(act);
+        return 4;
+    }
+    j.active = act;
+    int nt = threads;
+    if (nt <= 0) {
+        long nc = sysconf(_SC_NPROCESSORS_ONLN);
+        nt = nc >= 1 ? (int)nc : 4;
+    }
+    if (nt > 64) nt = 64;
+    int max_passes = 1;
+    if (nmax > n0) max_passes += (nmax - n0 + n0 - 1) / n0;
+    memset(act, 1, np);
+    adapt_pass(&j, nt);
+    for (int pass = 1;; ++pass) {
+        int count = 0;
+        for (size_t p = 0; p < np; ++p) {
+            int n = j.ns[p], flag = 0;
+            if (n < nmax) {
+                if (n < 2) {
+                    flag = 1;
+                } else {
+                    double nf = (double)n, *q = &j.sum[5 * p];
+                    double ybar = q[3] / nf;
+                    double var = (q[4] - q[3] * ybar) / (nf - 1.0);
+                    if (var < 0.0) var = 0.0;
+                    double e = sqrt(var / nf) / (ybar + 1e-4);
+                    flag = e > tau;
+                }
+            }
+            act[p] = (unsigned char)flag;
+            count += flag;
+        }
+        if (count == 0 || pass >= max_passes) break;
+        adapt_pass(&j, nt);
+    }
+    long long tot = 0;
+    int mx = 0;
+    for (size_t p = 0; p < np; ++p) {
+        double inv = 1.0 / (double)j.ns[p];
+        double *q = &j.sum[5 * p];
+        if (rgb) {
+            rgb[3 * p] = or_to_byte(q[0] * inv);
+            rgb[3 * p + 1] = or_to_byte(q[1] * inv);
+            rgb[3 * p + 2] = or_to_byte(q[2] * inv);
+        }
+        if (mean) {
+            mean[3 * p] = q[0] * inv;
+            mean[3 * p + 1] = q[1] * inv;
+            mean[3 * p + 2] = q[2] * inv;
+        }
+        if (nsamp) nsamp[p] = j.ns[p];
+        tot += j.ns[p];
+        if (j.ns[p] > mx) mx = j.ns[p];
+    }
+    if (total) *total = tot;
+    if (max_n) *max_n = mx;
+    free(j.sum); free(j.ns); free(act);
+    return 0;
+}
 
 /* ---- adjoint driver ----------------------------------------------------- */
 

@@ -51,6 +51,13 @@ int or_render(const or_scene *s, const mh_camera *cam, int width, int height,
               int threads, unsigned char *rgb, double *mean, double *var,
               double *seconds);
 
+/* Adaptive fixed-batch Whitted (reference render_image_ex with adaptive on):
+ * per-pixel linear means, 8-bit image and sample counts; total / max samples.
+ * nmax <= 0: 4 n0; tau <= 0: 0.02. */
+int or_render_adaptive(const or_scene *s, const mh_camera *cam, int width, int height, int n0, int depth,
+                       int nmax, double tau, int threads, unsigned char *rgb, double *mean, int *nsamp,
+                       long long *total, int *max_n);
+
 /* Traversal counts accumulated by the calling thread's queries since the
  * last reset (used to price algorithmic bytes per sample). */
 void or_stats_reset(void);

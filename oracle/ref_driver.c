@@ -591,3 +591,24 @@ unsigned char mtref_to_byte(double c)
 {
     return pathtrace_to_byte(c);
 }
+
+/* render_image_ex (adaptive or fixed Whitted) with the last render's sample
+ * statistics. */
+int mtref_render_ex(void *h, const mh_camera *c, int w, int hgt, int n0, int depth, int adaptive, int nmax,
+                    double tau, unsigned char *rgb, unsigned long long *total, int *max_n)
+{
+    RefScene *rs = (RefScene *)h;
+    if (!rs) return 1;
+    Camera cam = cam_from(c);
+    RenderParams p;
+    memset(&p, 0, sizeof p);
+    p.samples_per_pixel = n0;
+    p.max_depth = depth;
+    p.adaptive = adaptive;
+    p.adaptive_max_spp = nmax;
+    p.adaptive_tau = tau;
+    int rc = render_image_ex(&rs->scene, &cam, w, hgt, &p, rgb);
+    if (total) *total = render_last_total_samples();
+    if (max_n) *max_n = render_last_max_samples();
+    return rc;
+}

@@ -67,6 +67,8 @@ def oracle():
         _sig(L, "or_water_normals", None, [P, C.c_int, P])
         _sig(L, "or_noise", None, [P, C.c_int, C.c_int, C.c_uint, P, P])
         _sig(L, "or_noise3", None, [P, C.c_int, C.c_uint, P, P])
+        _sig(L, "or_render_adaptive", C.c_int, [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, C.c_int,
+                                                P, P, P, C.POINTER(C.c_longlong), C.POINTER(C.c_int)])
         _sig(L, "or_sun_disk_dirs", None, [DP, D, P, C.c_int, P])
         _sig(L, "or_light_cone_dirs", None, [DP, D, P, C.c_int, P])
         _sig(L, "or_cosine_dirs", None, [DP, P, C.c_int, P])
@@ -106,6 +108,8 @@ def ref():
         _sig(L, "mtref_noise", None, [P, C.c_int, C.c_int, C.c_uint, P, P])
         _sig(L, "mtref_noise3", None, [P, C.c_int, C.c_uint, P, P])
         _sig(L, "mtref_ocean", C.c_int, [P, C.POINTER(abi.Ocean)])
+        _sig(L, "mtref_render_ex", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, D, P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_int)])
         _sig(L, "mtref_obj_prims", C.c_int, [C.c_char_p, C.c_size_t, C.c_int, DP, DP, DP, C.c_int, C.c_int, D,
                                              P, P, C.c_int])
         _sig(L, "mtref_sun_disk_dir", None, [DP, D, P, C.c_int, P])
@@ -164,6 +168,18 @@ class OracleScene:
         if rc:
             raise RuntimeError(f"or_render rc {rc}")
         return rgb, mean, var, sec.value
+
+    def render_adaptive(self, cam, w, h, n0, depth, nmax=0, tau=0.0, threads=0):
+        """Adaptive Whitted: (rgb, linear mean, samples per pixel, total, max)."""
+        rgb = np.zeros((h, w, 3), np.uint8)
+        mean = np.zeros((h, w, 3), np.float64)
+        ns = np.zeros((h, w), np.int32)
+        tot, mx = C.c_longlong(), C.c_int()
+        rc = self.L.or_render_adaptive(self.h, C.byref(cam), w, h, n0, depth, nmax, tau, threads, abi.ptr(rgb),
+                                       abi.ptr(mean), abi.ptr(ns), C.byref(tot), C.byref(mx))
+        if rc:
+            raise RuntimeError(f"or_render_adaptive rc {rc}")
+        return rgb, mean, ns, tot.value, mx.value
 
     def radiance(self, rays, depth, seed0):
         """FP64 path radiance of each ray, seed seed0 + i for ray i."""
@@ -271,6 +287,16 @@ class RefScene:
 
     def radiance(self, rays, depth, seed0):
         return _radiance(self.L.mtref_radiance, self.h, rays, depth, seed0)
+
+    def render_ex(self, cam, w, h, n0, depth, adaptive=1, nmax=0, tau=0.0):
+        """render_image_ex: (rgb, total samples, max samples per pixel)."""
+        rgb = np.zeros((h, w, 3), np.uint8)
+        tot, mx = C.c_ulonglong(), C.c_int()
+        rc = self.L.mtref_render_ex(self.h, C.byref(cam), w, h, n0, depth, adaptive, nmax, tau, abi.ptr(rgb),
+                                    C.byref(tot), C.byref(mx))
+        if rc:
+            raise RuntimeError(f"mtref_render_ex rc {rc}")
+        return rgb, tot.value, mx.value
 
     def intersect(self, rays, tmin=1e-4, tmax=1e30):
         rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)

@@ -137,6 +137,21 @@ def ingest():
     np.savez_compressed(os.path.join(HERE, "ingest.npz"), **out)
 
 
+def adaptive():
+    """Reference render_image_ex (adaptive Whitted) of the default scene at
+    96x54, depth 5, under test_oracle.ADAPTIVE_CASES: images, total and
+    maximum per-pixel sample counts."""
+    import test_oracle as to
+    text = open(REF_SCENE, "rb").read()
+    r = ck.RefScene(text=text)
+    cam = r.camera(96 / 54)
+    out = {"cam": sbytes(cam)}
+    for i, (n0, nmax, tau) in enumerate(to.ADAPTIVE_CASES):
+        rgb, tot, mx = r.render_ex(cam, 96, 54, n0, 5, 1, nmax, tau)
+        out[f"rgb_{i}"], out[f"total_{i}"], out[f"max_{i}"] = rgb, np.array(tot), np.array(mx)
+    np.savez_compressed(os.path.join(HERE, "adaptive.npz"), **out)
+
+
 def vectors(r, sky):
     """Reference outputs of the per-function routines on seeded inputs."""
     L = ck.ref()
@@ -187,6 +202,10 @@ def vectors(r, sky):
 def main():
     if not ck.have_ref():
         raise SystemExit("oracle/_ref/libmtref.so missing: make -C oracle ref")
+    if len(sys.argv) > 1:       # regenerate the named fixtures only
+        for name in sys.argv[1:]:
+            globals()[name]()
+        return
     r, prims, attrs, mats, sky, wl, wm = default_scene()
     vectors(r, sky)
     host_scene_images()

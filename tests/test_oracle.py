@@ -368,3 +368,71 @@ def test_oracle_fog_equals_live_reference(kind, kw, seed):
         a, _ = r.render(cam, 48, 36, 2, 6, 0)
         b, _, _, _ = o.render(cam, 48, 36, 2, 6, mode=0)
         assert np.array_equal(a, b), name
+
+
+# ---- adaptive Whitted -----------------------------------------------------------
+
+ADAPTIVE_CASES = [(4, 0, 0.0), (4, 16, 0.05), (3, 10, 0.01), (1, 5, 0.03)]
+
+
+def test_oracle_adaptive_matches_golden(golden):
+    g, hold = default_scene(golden)
+    o = ck.OracleScene(hold.soa)
+    a = golden["adaptive"]
+    for i, (n0, nmax, tau) in enumerate(ADAPTIVE_CASES):
+        cam = cam_of(a["cam"])
+        rgb, mean, ns, tot, mx = o.render_adaptive(cam, 96, 54, n0, 5, nmax, tau)
+        assert np.array_equal(rgb, a[f"rgb_{i}"]), (n0, nmax, tau)
+        assert (tot, mx) == (int(a[f"total_{i}"]), int(a[f"max_{i}"]))
+        assert int(ns.sum()) == tot and ns.max() == mx and ns.min() >= n0
+
+
+def test_oracle_adaptive_closed_forms():
+    """What the sampler's definition fixes, on a white sphere under a uniform
+    sky: a cap of n0 samples is the fixed render; sky pixels (one constant
+    colour, zero luminance variance) stop at n0 while silhouette pixels refine
+    to the cap; a larger tau never takes more samples; the first pass uses the
+    fixed render's stratified offsets (pixels that stop at n0 equal it)."""
+    hs = scenes.white_furnace()
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    o = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm).soa)
+    cam = hs.camera(1.0, eye=[0, 0, 5], at=[0, 0, 0], up=[0, 1, 0], vfov=40)
+    w = h = 32
+    fixed, fm, _, _ = o.render(cam, w, h, 4, 3, mode=1)
+    rgb, mean, ns, tot, mx = o.render_adaptive(cam, w, h, 4, 3, 4, 0.02)
+    assert np.array_equal(rgb, fixed) and tot == w * h * 4 and mx == 4 and (ns == 4).all()
+    rgb, mean, ns, tot, mx = o.render_adaptive(cam, w, h, 4, 3, 64, 1e-3)
+    yy, xx = np.mgrid[0:h, 0:w]
+    assert (ns[[0, 0, -1, -1], [0, -1, 0, -1]] == 4).all()
+    assert mx == 64 and tot == int(ns.sum())
+    stopped = ns == 4
+    assert np.array_equal(rgb[stopped], fixed[stopped])
+    # refined pixels hold both sky and sphere among their first samples
+    assert (ns > 4).sum() > 0
+    prev = None
+    for tau in (1e-3, 0.01, 0.05, 0.3):
+        _, _, n2, t2, _ = o.render_adaptive(cam, w, h, 4, 3, 64, tau)
+        if prev is not None:
+            assert (n2 <= prev).all()
+        prev = n2
+
+
+@pytest.mark.skipif(not ck.have_ref(), reason="oracle/_ref not built (reference absent)")
+@pytest.mark.parametrize("kind,kw,seed", [(native.GEN_SPHERES, {}, 7), (native.GEN_FOREST, dict(grid=3, subdiv=2), 5)])
+def test_oracle_adaptive_equals_live_reference(kind, kw, seed):
+    hs = HostScene()
+    hs.generate(kind, seed=seed, **kw)
+    hs.build_bvh()
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    hold = ck.SoaHolder(prims, attrs, mats, sky, wl, wm)
+    o, r = ck.OracleScene(hold.soa), ck.RefScene(soa=hold.soa)
+    cam = hs.camera(4 / 3)
+    for n0, nmax, tau in ADAPTIVE_CASES:
+        a, tot_r, mx_r = r.render_ex(cam, 64, 48, n0, 5, 1, nmax, tau)
+        b, mean, ns, tot, mx = o.render_adaptive(cam, 64, 48, n0, 5, nmax, tau)
+        assert np.array_equal(a, b), (n0, nmax, tau)
+        assert (tot, mx) == (tot_r, mx_r)
+    # adaptive off is the fixed render
+    a, _, _ = r.render_ex(cam, 64, 48, 4, 5, 0)
+    b, _, _, _ = o.render(cam, 64, 48, 4, 5, mode=1)
+    assert np.array_equal(a, b)
