@@ -375,6 +375,71 @@ int mh_render_image(struct mh_scene *s, const mh_camera *cam, int width,
                     int height, int samples_per_pixel, int max_depth,
                     unsigned char *rgb_out);
 
+/* Parameters of render_image_ex, field for field the reference's
+ * RenderParams (src/render.h:49-55): base samples n0 (>= 1), depth (>= 0),
+ * adaptive (0 = the fixed render), the per-pixel cap (<= 0: 4 * n0; below
+ * n0: n0) and the relative-error tolerance (<= 0: 0.02). */
+typedef struct mh_render_ex_params {
+    int32_t samples_per_pixel;
+    int32_t max_depth;
+    int32_t adaptive;
+    int32_t adaptive_max_spp;
+    double adaptive_tau;
+} mh_render_ex_params;
+
+/* Drop-in for reference render_image_ex (src/render.h:65-66,
+ * src/render.c:1627-1680): with adaptive on, every pixel takes n0 Whitted
+ * samples, then passes of up to n0 more refine the pixels whose relative
+ * standard error of mean luminance exceeds tau, until none does or the cap
+ * is met (at most 1 + ceil((cap - n0) / n0) passes); with adaptive off, the
+ * fixed render (mh_render_image). fp64, the same return codes. */
+int mh_render_image_ex(struct mh_scene *s, const mh_camera *cam, int width,
+                       int height, const mh_render_ex_params *params,
+                       unsigned char *rgb_out);
+
+/* The same with a choice of precision (MH_PREC_*) and optional outputs:
+ * linear_out (w*h*3 floats, per-pixel means) and nsamp_out (w*h samples
+ * taken per pixel). Any output pointer may be NULL. */
+int mh_render_adaptive(struct mh_scene *s, const mh_camera *cam, int width,
+                       int height, const mh_render_ex_params *params,
+                       int precision, unsigned char *rgb_out,
+                       float *linear_out, int32_t *nsamp_out);
+
+/* Primary samples of the last render on the calling thread: the total and
+ * the largest per pixel (reference render_last_total_samples and
+ * render_last_max_samples, src/render.h:79-85; w*h*spp and spp for the
+ * fixed renders). */
+int mh_last_sample_stats(unsigned long long *total, int *max_per_pixel);
+
+/* Anamorphic streak flare, field for field the reference's FlareParams
+ * (src/flare.h:30-36; the C layout puts 4 bytes of padding after `enabled`):
+ * intensity (<= 0: no flare), highlight threshold (clamped to [0, 0.999]),
+ * streak half-width as a fraction of the image width (clamped to
+ * [0.005, 1]), tint (|tint|^2 < 1e-6: (0.15, 0.45, 1.0)). */
+typedef struct mh_flare_params {
+    int32_t enabled;
+    int32_t reserved;
+    double intensity;
+    double threshold;
+    double streak_length;
+    mh_vec3 tint;
+} mh_flare_params;
+
+/* Drop-in for reference flare_apply (src/flare.h:55, src/flare.c:60-173):
+ * horizontal streaks from the highlights of each scanline of the top-down
+ * RGB8 image, composited in place. Runs on the GPU (host buffer in and
+ * out); bytes equal the reference's. Returns 1 on invalid arguments, 0 when
+ * disabled (nothing changes). */
+int mh_flare_apply(unsigned char *rgb, int width, int height,
+                   const mh_flare_params *params);
+
+/* The same on the scene's device framebuffer (the 8-bit image of the last
+ * render on `s`), on the scene's stream: mh_framebuffer_download then
+ * returns the composited bytes. No reference counterpart (the reference
+ * flares its host image); saves the host round trip between render and
+ * post-pass. */
+int mh_scene_flare(struct mh_scene *s, const mh_flare_params *params);
+
 /* General render. rgb_out (w*h*3 bytes) and linear_out (w*h*3 floats, the
  * per-pixel mean radiance) are host buffers and may each be NULL: the result
  * then stays in the scene's device framebuffer (fetch it with
@@ -439,12 +504,14 @@ typedef struct mh_job {
 /* Launch settings of the batched engine. Zeros pick the defaults. On
  * return from mh_render_batch, mh_batch_last_tune gives what was used. */
 typedef struct mh_tune {
-    int32_t pool;          /* paths in flight per batch (queue capacity); 0 = 2^21       */
+    int32_t pool;          /* paths in flight per batch (queue capacity); 0 = 2^21;
+                              at most 2^26                                           */
     int32_t grid_blocks;   /* workgroups of each persistent kernel; 0 = 2048             */
     int32_t block;         /* lanes per workgroup: 64, 128 or 256; 0 = 256               */
     int32_t use_graph;     /* replay each batch's bounce sequence as a hipGraph          */
-    int32_t autotune;      /* 1 = choose pool / grid_blocks / block by timing trial
-                              batches of this workload (cached per workload on the handle) */
+    int32_t autotune;      /* 1 = choose pool / grid_blocks / block / fixed_batches by
+                              timing trial batches of this workload (cached per
+                              workload on the handle)                                */
     int32_t kernel_clock;  /* 1 = per-kernel HIP-event timing (direct launches)          */
     int32_t bvh_width;     /* BVH node width: 0 = default (4 when built, else 2), 2, 4;
                               autotune also tries both widths                        */

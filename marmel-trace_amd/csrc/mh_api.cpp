@@ -6,6 +6,7 @@
 // on that stream, and copy results back only when the caller asks.
 #include "marmel_hip.h"
 #include "mh_build.h"
+#include "mh_flare.h"
 #include "mh_kernels.h"
 #include "mh_pack.h"
 
@@ -16,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <algorithm>
 #include <vector>
@@ -28,6 +30,9 @@ thread_local std::string g_err;
 thread_local mh_timing g_timing;
 thread_local double g_seconds = 0.0;
 thread_local mh_kernel_times g_ktimes;
+// primary samples of the last render: total and the largest per pixel
+thread_local unsigned long long g_total_samples = 0;
+thread_local int g_max_samples = 0;
 
 int fail(int code, const char* fmt, ...)
 {
@@ -239,6 +244,13 @@ struct mh_scene {
     mh_bvh_info bvh{};
     const mh_bvh_node* tree_nodes = nullptr;
     const int32_t* tree_leaf = nullptr;
+    // adaptive Whitted state (grown on demand): sums, counts, active list,
+    // per-pass list lengths and the resolve statistics in one allocation
+    void* ad_mem = nullptr;
+    size_t ad_bytes = 0;
+    // flare scratch for rows too wide for LDS
+    void* flare_mem = nullptr;
+    size_t flare_bytes = 0;
 };
 
 namespace {
@@ -858,28 +870,34 @@ int batch_trial(mh_batch* b, const WfMulti& mj, int max_depth, int feat, mh_tune
     return 0;
 }
 
-// Coordinate search: pool, then grid blocks, then block size, each by the
-// path rate of a fixed trial of the workload's first paths.
+// Coordinate search by the path rate of a fixed trial of the workload's first
+// paths: path regeneration or fixed batches jointly with the pool size (the
+// better mode depends on how the workload's depths and scene sizes mix, and
+// each mode has its own best pool), then grid blocks, then block size.
 int autotune(mh_batch* b, const WfMulti& mj, int max_depth, int feat, mh_tune* t)
 {
     long long trial = std::min<long long>(mj.total, 1LL << 23);
     double best = -1.0;
     mh_tune cur = *t;
-    const int pools[] = {1 << 19, 1 << 20, 1 << 21, 1 << 22};
-    int pool_best = cur.pool;
-    for (int p : pools) {
-        if (p > 2 * trial && p != pools[0]) break;  // larger pools cannot fill up
-        mh_tune c = cur;
-        c.pool = p;
-        double r;
-        int rc = batch_trial(b, mj, max_depth, feat, c, std::max<long long>(trial, p), &r);
-        if (rc) return rc;
-        if (r > best) {
-            best = r;
-            pool_best = p;
+    const int pools[] = {1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23};
+    int pool_best = cur.pool, mode_best = 0;
+    for (int mode = 0; mode < 2; ++mode)
+        for (int p : pools) {
+            if (p > 2 * trial && p != pools[0]) break;  // larger pools cannot fill up
+            mh_tune c = cur;
+            c.pool = p;
+            c.fixed_batches = mode;
+            double r;
+            int rc = batch_trial(b, mj, max_depth, feat, c, std::max<long long>(trial, p), &r);
+            if (rc) return rc;
+            if (r > best) {
+                best = r;
+                pool_best = p;
+                mode_best = mode;
+            }
         }
-    }
     cur.pool = pool_best;
+    cur.fixed_batches = mode_best;
     best = -1.0;
     const int grids[] = {1024, 2048, 4096};
     int grid_best = cur.grid_blocks;
@@ -1270,6 +1288,8 @@ void mh_scene_destroy(mh_scene* s)
     if (s->d_dimg) (void)hipFree(s->d_dimg);
     if (s->d_grad) (void)hipFree(s->d_grad);
     if (s->d_frozen) (void)hipFree(s->d_frozen);
+    if (s->ad_mem) (void)hipFree(s->ad_mem);
+    if (s->flare_mem) (void)hipFree(s->flare_mem);
     for (auto& e : s->ev)
         if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1462,6 +1482,28 @@ int mh_trace_any(mh_scene* s, const mh_ray* rays, int n, double tmin, double tma
     return 0;
 }
 
+// Explicit Whitted work stacks of a persistent grid of `threads` lanes
+// (at most 256 Ki, at most one per pixel): 17 pending rays per level suffice.
+static int ensure_wstack(mh_scene* s, int w, int h, int depth, bool f64, int* cap_out, int* threads_out)
+{
+    int cap = 17 * (depth + 2) + 4;
+    int threads = 256 * 1024;
+    size_t px = (size_t)w * h;
+    if ((size_t)threads > px) threads = (int)((px + 255) / 256 * 256);
+    size_t item = f64 ? whitted_item_bytes<double>() : whitted_item_bytes<float>();
+    size_t need = item * (size_t)cap * (size_t)threads;
+    if (need > s->wstack_bytes) {
+        if (s->wstack) (void)hipFree(s->wstack);
+        s->wstack = nullptr;
+        s->wstack_bytes = 0;
+        HIP_TRY(hipMalloc(&s->wstack, need));
+        s->wstack_bytes = need;
+    }
+    *cap_out = cap;
+    *threads_out = threads;
+    return 0;
+}
+
 int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsigned char* rgb_out, float* linear_out)
 {
     if (!s || !cam || !p) return fail(1, "NULL argument");
@@ -1530,20 +1572,9 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
             HIP_TRY(launch_pathtrace<float>(v32, conv_cam<float>(*cam), p->width, p->height, p->spp,
                                             p->sample_offset, p->max_depth, lo, 256, s->stream));
     } else {
-        // explicit Whitted work stack: 17 pending rays per level suffices
-        int cap = 17 * (p->max_depth + 2) + 4;
-        int threads = 256 * 1024;
-        size_t px = (size_t)p->width * p->height;
-        if ((size_t)threads > px) threads = (int)((px + 255) / 256 * 256);
-        size_t item = f64 ? whitted_item_bytes<double>() : whitted_item_bytes<float>();
-        size_t need = item * (size_t)cap * (size_t)threads;
-        if (need > s->wstack_bytes) {
-            if (s->wstack) (void)hipFree(s->wstack);
-            s->wstack = nullptr;
-            s->wstack_bytes = 0;
-            HIP_TRY(hipMalloc(&s->wstack, need));
-            s->wstack_bytes = need;
-        }
+        int cap = 0, threads = 0;
+        rc = ensure_wstack(s, p->width, p->height, p->max_depth, f64, &cap, &threads);
+        if (rc) return rc;
         if (f64)
             HIP_TRY(launch_whitted<double>(v64, conv_cam<double>(*cam), p->width, p->height, p->spp,
                                            p->sample_offset, p->max_depth, lo, s->wstack, cap, threads, s->stream));
@@ -1567,6 +1598,8 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
     g_timing.samples = (int64_t)p->width * p->height * p->spp;
     g_timing.rays = (int64_t)rays;
     g_seconds = g_timing.kernel_ms * 1e-3;
+    g_total_samples = (unsigned long long)p->width * (unsigned long long)p->height * (unsigned long long)p->spp;
+    g_max_samples = p->spp;
     memset(&g_ktimes, 0, sizeof g_ktimes);
     if (p->kernel_clock && p->mode == MH_MODE_PATHTRACE && p->engine == MH_ENGINE_WAVEFRONT) {
         HIP_TRY(s->clock.drain());
@@ -1619,6 +1652,240 @@ int mh_render_image(mh_scene* s, const mh_camera* cam, int width, int height, in
                     unsigned char* rgb_out)
 {
     return render_drop_in(s, cam, width, height, spp, max_depth, rgb_out, MH_MODE_WHITTED);
+}
+
+// Adaptive Whitted (reference src/render.c:1337-1496 and render_image_ex,
+// src/render.c:1627-1680): validation and defaults as the reference's, then
+// the whole pass sequence on the scene's stream without host round trips.
+static int render_adaptive_impl(mh_scene* s, const mh_camera* cam, int w, int h, const mh_render_ex_params* p,
+                                int precision, unsigned char* rgb_out, float* linear_out, int32_t* nsamp_out)
+{
+    if (!p || !s || !cam) return fail(1, "NULL argument");
+    if (w <= 0 || h <= 0) return fail(2, "non-positive image size %dx%d", w, h);
+    if (p->samples_per_pixel < 1 || p->max_depth < 0)
+        return fail(3, "bad spp %d / depth %d", p->samples_per_pixel, p->max_depth);
+    if ((long long)w * h > (1LL << 28)) return fail(2, "image too large");
+    if (precision != MH_PREC_FP32 && precision != MH_PREC_FP64) return fail(5, "unknown precision %d", precision);
+    int n0 = p->samples_per_pixel;
+    if (!p->adaptive) {
+        mh_render_params rp;
+        memset(&rp, 0, sizeof rp);
+        rp.width = w;
+        rp.height = h;
+        rp.spp = n0;
+        rp.max_depth = p->max_depth;
+        rp.mode = MH_MODE_WHITTED;
+        rp.precision = precision;
+        int rc = mh_render(s, cam, &rp, rgb_out, linear_out);
+        if (rc == 0 && nsamp_out)
+            for (size_t i = 0; i < (size_t)w * h; ++i) nsamp_out[i] = n0;
+        return rc;
+    }
+    long long nmax = p->adaptive_max_spp;
+    if (nmax <= 0) nmax = 4LL * n0;
+    if (nmax < n0) nmax = n0;
+    if (nmax > INT_MAX) nmax = INT_MAX;
+    double tau = p->adaptive_tau;
+    if (!(tau > 0.0)) tau = 0.02;
+    long long max_passes = 1;
+    if (nmax > n0) max_passes += (nmax - n0 + n0 - 1) / n0;
+    if (max_passes > (1 << 20)) return fail(3, "adaptive cap %lld over base %d needs too many passes", nmax, n0);
+    bool f64 = precision == MH_PREC_FP64;
+    int rc = ensure_fb(s, w, h);
+    if (rc) return rc;
+    int cap = 0, threads = 0;
+    rc = ensure_wstack(s, w, h, p->max_depth, f64, &cap, &threads);
+    if (rc) return rc;
+    size_t npix = (size_t)w * h;
+    size_t o_ns = 5 * npix * sizeof(double);
+    size_t o_list = o_ns + npix * sizeof(int);
+    size_t o_cnt = o_list + npix * sizeof(int);
+    size_t o_tot = (o_cnt + (size_t)max_passes * sizeof(int) + 7) & ~(size_t)7;
+    size_t need = o_tot + 16;
+    if (need > s->ad_bytes) {
+        if (s->ad_mem) (void)hipFree(s->ad_mem);
+        s->ad_mem = nullptr;
+        s->ad_bytes = 0;
+        HIP_TRY(hipMalloc(&s->ad_mem, need));
+        s->ad_bytes = need;
+    }
+    char* base = (char*)s->ad_mem;
+    AdaptBuf ab{(double*)base, (int*)(base + o_ns), (int*)(base + o_list), (int*)(base + o_cnt),
+                (unsigned long long*)(base + o_tot), (int*)(base + o_tot + 8)};
+    LaunchOut lo{s->fb_linear, s->fb_rgb, s->d_rays, s->d_overflow};
+    HIP_TRY(hipMemsetAsync(s->d_rays, 0, sizeof(unsigned long long), s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_overflow, 0, sizeof(int), s->stream));
+    HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+    // sums and counts start at zero; the list needs no clearing
+    HIP_TRY(hipMemsetAsync(base, 0, o_list, s->stream));
+    HIP_TRY(hipMemsetAsync(base + o_cnt, 0, need - o_cnt, s->stream));
+    SceneView<float> v32 = with_width(s->v32, 0);
+    SceneView<double> v64 = with_width(s->v64, 0);
+    if (f64)
+        HIP_TRY(launch_whitted_adaptive<double>(v64, conv_cam<double>(*cam), w, h, n0, (int)nmax, tau, p->max_depth,
+                                                ab, (int)max_passes, lo, s->wstack, cap, threads, s->stream));
+    else
+        HIP_TRY(launch_whitted_adaptive<float>(v32, conv_cam<float>(*cam), w, h, n0, (int)nmax, tau, p->max_depth,
+                                               ab, (int)max_passes, lo, s->wstack, cap, threads, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+    size_t n3 = npix * 3;
+    if (rgb_out) HIP_TRY(hipMemcpyAsync(rgb_out, s->fb_rgb, n3, hipMemcpyDeviceToHost, s->stream));
+    if (linear_out) HIP_TRY(hipMemcpyAsync(linear_out, s->fb_linear, n3 * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    if (nsamp_out) HIP_TRY(hipMemcpyAsync(nsamp_out, ab.ns, npix * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    unsigned long long stats[2] = {0, 0}, rays = 0;
+    int ovf = 0;
+    HIP_TRY(hipMemcpyAsync(stats, ab.total, 16, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(&rays, s->d_rays, sizeof rays, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(&ovf, s->d_overflow, sizeof ovf, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[2], s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    int mx = 0;
+    memcpy(&mx, &stats[1], sizeof mx);
+    memset(&g_timing, 0, sizeof g_timing);
+    g_timing.kernel_ms = ms_between(s->ev[0], s->ev[1]);
+    g_timing.total_ms = ms_between(s->ev[0], s->ev[2]);
+    g_timing.samples = (int64_t)stats[0];
+    g_timing.rays = (int64_t)rays;
+    g_seconds = g_timing.kernel_ms * 1e-3;
+    memset(&g_ktimes, 0, sizeof g_ktimes);
+    g_total_samples = stats[0];
+    g_max_samples = mx;
+    if (ovf) return fail(6, "Whitted work stack overflow");
+    return 0;
+}
+
+int mh_render_image_ex(mh_scene* s, const mh_camera* cam, int width, int height, const mh_render_ex_params* params,
+                       unsigned char* rgb_out)
+{
+    if (!params) return fail(1, "NULL argument");
+    if (!s || !cam || !rgb_out) return fail(1, "NULL argument");
+    return render_adaptive_impl(s, cam, width, height, params, MH_PREC_FP64, rgb_out, nullptr, nullptr);
+}
+
+int mh_render_adaptive(mh_scene* s, const mh_camera* cam, int width, int height, const mh_render_ex_params* params,
+                       int precision, unsigned char* rgb_out, float* linear_out, int32_t* nsamp_out)
+{
+    return render_adaptive_impl(s, cam, width, height, params, precision, rgb_out, linear_out, nsamp_out);
+}
+
+int mh_last_sample_stats(unsigned long long* total, int* max_per_pixel)
+{
+    if (total) *total = g_total_samples;
+    if (max_per_pixel) *max_per_pixel = g_max_samples;
+    return 0;
+}
+
+// ---- flare post-pass (reference src/flare.c:60-173) ------------------------
+
+// Per-call constants as the reference derives them; 1 = disabled (nothing
+// to do). alpha = expf(-1 / radius) is the host's expf, like the reference's.
+static int flare_prepare(const mh_flare_params* p, int w, FlareK* k)
+{
+    if (!p->enabled || p->intensity <= 0.0) return 1;
+    double th = p->threshold < 0.0 ? 0.0 : (p->threshold > 0.999 ? 0.999 : p->threshold);
+    double len = p->streak_length < 0.005 ? 0.005 : (p->streak_length > 1.0 ? 1.0 : p->streak_length);
+    mh_vec3 t = p->tint;
+    if (t.x * t.x + t.y * t.y + t.z * t.z < 1e-6) t = mh_vec3{0.15, 0.45, 1.0};
+    float r1 = (float)(len * (double)w * 0.12);
+    if (r1 < 2.0f) r1 = 2.0f;
+    float r2 = (float)(len * (double)w);
+    if (r2 < 6.0f) r2 = 6.0f;
+    k->intensity = p->intensity;
+    k->thresh = th;
+    k->tint[0] = t.x;
+    k->tint[1] = t.y;
+    k->tint[2] = t.z;
+    k->a1 = expf(-1.0f / r1);
+    k->a2 = expf(-1.0f / r2);
+    k->n1 = 0.45f * 0.045f;
+    k->n2 = 0.55f * 0.045f;
+    return 0;
+}
+
+// Rows in LDS when 9 float planes fit the device's per-workgroup limit, else
+// in a scratch slice per workgroup (grown on demand in *mem).
+static int flare_run(unsigned char* d_rgb, int w, int h, const FlareK& k, void** mem, size_t* bytes, hipStream_t st)
+{
+    int dev = 0, lim = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    float* scr = nullptr;
+    int grid = h < 4096 ? h : 4096;
+    if (flare_lds_bytes(w) > (size_t)lim) {
+        grid = h < 512 ? h : 512;
+        size_t need = (size_t)grid * flare_lds_bytes(w);
+        if (need > *bytes) {
+            if (*mem) (void)hipFree(*mem);
+            *mem = nullptr;
+            *bytes = 0;
+            HIP_TRY(hipMalloc(mem, need));
+            *bytes = need;
+        }
+        scr = (float*)*mem;
+    }
+    HIP_TRY(launch_flare(d_rgb, w, h, k, scr, grid, st));
+    return 0;
+}
+
+namespace {
+// Device image and scratch of mh_flare_apply (host buffers), one caller at a time.
+struct FlareCtx {
+    std::mutex mu;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    void* img = nullptr;
+    size_t img_bytes = 0;
+    void* scr = nullptr;
+    size_t scr_bytes = 0;
+};
+FlareCtx g_flare;
+} // namespace
+
+int mh_flare_apply(unsigned char* rgb, int width, int height, const mh_flare_params* params)
+{
+    if (!rgb || width <= 0 || height <= 0 || !params) return fail(1, "invalid flare arguments");
+    FlareK k;
+    if (flare_prepare(params, width, &k)) return 0;
+    if ((long long)width * height > (1LL << 28)) return fail(2, "image too large");
+    std::lock_guard<std::mutex> lock(g_flare.mu);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (g_flare.device != dev) {
+        // a context made on another device is left as it is (its memory
+        // belongs to that device); this device gets its own
+        g_flare.stream = nullptr;
+        g_flare.img = g_flare.scr = nullptr;
+        g_flare.img_bytes = g_flare.scr_bytes = 0;
+        HIP_TRY(hipStreamCreateWithFlags(&g_flare.stream, hipStreamNonBlocking));
+        g_flare.device = dev;
+    }
+    size_t n = (size_t)width * height * 3;
+    if (n > g_flare.img_bytes) {
+        if (g_flare.img) (void)hipFree(g_flare.img);
+        g_flare.img = nullptr;
+        g_flare.img_bytes = 0;
+        HIP_TRY(hipMalloc(&g_flare.img, n));
+        g_flare.img_bytes = n;
+    }
+    unsigned char* d = (unsigned char*)g_flare.img;
+    HIP_TRY(hipMemcpyAsync(d, rgb, n, hipMemcpyHostToDevice, g_flare.stream));
+    int rc = flare_run(d, width, height, k, &g_flare.scr, &g_flare.scr_bytes, g_flare.stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(rgb, d, n, hipMemcpyDeviceToHost, g_flare.stream));
+    HIP_TRY(hipStreamSynchronize(g_flare.stream));
+    return 0;
+}
+
+int mh_scene_flare(mh_scene* s, const mh_flare_params* params)
+{
+    if (!s || !params) return fail(1, "NULL argument");
+    if (!s->fb_rgb || s->fb_w <= 0 || s->fb_h <= 0) return fail(5, "no framebuffer yet");
+    FlareK k;
+    if (flare_prepare(params, s->fb_w, &k)) return 0;
+    int rc = flare_run(s->fb_rgb, s->fb_w, s->fb_h, k, &s->flare_mem, &s->flare_bytes, s->stream);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return 0;
 }
 
 int mh_batch_create(mh_batch** out)
@@ -1677,7 +1944,9 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     if (tune) t = *tune;
     if (t.bvh_width != 0 && t.bvh_width != 2 && t.bvh_width != 4)
         return fail(4, "BVH node width %d is not built (2 or 4)", t.bvh_width);
-    if (t.pool < 0 || t.grid_blocks < 0 || (t.block != 0 && t.block != 64 && t.block != 128 && t.block != 256))
+    // shadow-ray tags hold the slot in 28 bits
+    if (t.pool < 0 || t.pool > (1 << 26) || t.grid_blocks < 0 ||
+        (t.block != 0 && t.block != 64 && t.block != 128 && t.block != 256))
         return fail(4, "bad launch settings (pool %d, grid %d, block %d)", t.pool, t.grid_blocks, t.block);
 
     std::vector<SceneView<float>> views((size_t)n_scenes);
@@ -1768,6 +2037,7 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
                 t.grid_blocks = kv.second.grid_blocks;
                 t.block = kv.second.block;
                 t.bvh_width = kv.second.bvh_width;
+                t.fixed_batches = kv.second.fixed_batches;
                 hit = true;
             }
         if (!hit) {

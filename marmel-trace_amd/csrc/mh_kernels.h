@@ -127,6 +127,24 @@ hipError_t launch_whitted(const SceneView<R>& sc, const CamView<R>& cam, int w, 
                           const LaunchOut& out, void* stack, int cap, int threads, hipStream_t st);
 
 template <typename R> size_t whitted_item_bytes();
+// Adaptive Whitted state: per-pixel sums in five planes of w*h doubles
+// (R, G, B, luminance, luminance^2), sample counts, the active list.
+struct AdaptBuf {
+    double* sum;
+    int* ns;
+    int* list;
+    int* count;                 // [pass] length of that pass's active list (zeroed)
+    unsigned long long* total;  // sum of the per-pixel sample counts (zeroed)
+    int* max_n;                 // largest per-pixel sample count (zeroed)
+};
+// Pass 0 (n0 samples everywhere), then max_passes - 1 refinement passes of
+// flag + sample, then the resolve; no host synchronisation between passes
+// (a pass whose list is empty does nothing).
+template <typename R>
+hipError_t launch_whitted_adaptive(const SceneView<R>& sc, const CamView<R>& cam, int w, int h, int n0, int nmax,
+                                   double tau, int depth, const AdaptBuf& ab, int max_passes, const LaunchOut& out,
+                                   void* stack, int cap, int threads, hipStream_t st);
+
 
 template <typename R>
 hipError_t launch_closest(const SceneView<R>& sc, const mh_ray* rays, int n, double tmin, double tmax, mh_hit* out,

@@ -121,6 +121,138 @@ __global__ __launch_bounds__(256) void k_whitted(SceneView<R> sc, CamView<R> cam
     count_rays(out.rays, nq);
 }
 
+// ---- adaptive Whitted ------------------------------------------------------
+// One refinement pass: each listed pixel (every pixel when `count` is null)
+// takes up to n0 more samples, continuing its global sample index. Sample s
+// uses the fixed render's offset (stratified when n0 is a perfect square)
+// for s < n0 and uniform jitter after; the pass's sums are formed serially in
+// sample order and then added to the pixel's running sums, so the result is
+// independent of the launch shape. Follows reference src/render.c:1151-1233.
+template <typename R>
+__global__ __launch_bounds__(256) void k_adapt_sample(SceneView<R> sc, CamView<R> cam, int w, int h, int n0, int nmax,
+                                                      int depth, AdaptBuf ab, const int* count, LaunchOut out,
+                                                      WItem<R>* stack, int cap)
+{
+    MH_STACK_PRIV(ts)
+    int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    int nlanes = gridDim.x * blockDim.x;
+    WItem<R>* stk = stack + (size_t)lane * (size_t)cap;
+    int nq = 0, ovf = 0;
+    int g = (int)(msqrt((double)n0) + 0.5);
+    bool strat = g > 0 && g * g == n0;
+    size_t npix = (size_t)w * (size_t)h;
+    int items = count ? *count : (int)npix;
+    for (int i = lane; i < items; i += nlanes) {
+        int p = count ? ab.list[i] : i;
+        int x = p % w, y = p / w;
+        int n = ab.ns[p];
+        int b = nmax - n < n0 ? nmax - n : n0;
+        if (b <= 0) continue;
+        double sr = 0.0, sg = 0.0, sb = 0.0, sy = 0.0, sy2 = 0.0;
+        for (int k = 0; k < b; ++k) {
+            int s = n + k;
+            R jx = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 0));
+            R jy = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 1));
+            R du = jx, dv = jy;
+            if (strat && s < n0) {
+                du = ((R)(s % g) + jx) / (R)g;
+                dv = ((R)(s / g) + jy) / (R)g;
+            }
+            R uu = ((R)x + du) / (R)w;
+            R vv = R(1) - ((R)y + dv) / (R)h;
+            R l1 = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 2));
+            R l2 = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 3));
+            V3<R> o, d;
+            camera_ray(cam, uu, vv, l1, l2, &o, &d);
+            V3<R> c = whitted_sample(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, (uint32_t)s), stk, cap,
+                                     &ovf, &nq);
+            double cr = (double)c.x, cg = (double)c.y, cb = (double)c.z;
+            sr += cr;
+            sg += cg;
+            sb += cb;
+            double Y = 0.2126 * cr + 0.7152 * cg + 0.0722 * cb;
+            sy += Y;
+            sy2 += Y * Y;
+        }
+        ab.sum[p] += sr;
+        ab.sum[npix + p] += sg;
+        ab.sum[2 * npix + p] += sb;
+        ab.sum[3 * npix + p] += sy;
+        ab.sum[4 * npix + p] += sy2;
+        ab.ns[p] = n + b;
+    }
+    if (ovf) atomicOr(out.overflow, 1);
+    count_rays(out.rays, nq);
+}
+
+// Active set of the next pass from each pixel's own running statistics:
+// refine while n < nmax and (n < 2 or the relative standard error of the
+// mean luminance exceeds tau). Appends the pixels to ab.list (one atomic per
+// wave) and their number to *count. Follows reference src/render.c:1330-1361.
+template <typename R>
+__global__ __launch_bounds__(256) void k_adapt_flag(AdaptBuf ab, int npix, int nmax, double tau, int* count)
+{
+    int lane = threadIdx.x & 63;
+    for (int base = blockIdx.x * blockDim.x; base < npix; base += gridDim.x * blockDim.x) {
+        int p = base + threadIdx.x;
+        int flag = 0;
+        if (p < npix) {
+            int n = ab.ns[p];
+            if (n < nmax) {
+                if (n < 2) {
+                    flag = 1;
+                } else {
+                    double nf = (double)n;
+                    double sy = ab.sum[3 * (size_t)npix + p], sy2 = ab.sum[4 * (size_t)npix + p];
+                    double ybar = sy / nf;
+                    double var = (sy2 - sy * ybar) / (nf - 1.0);
+                    if (var < 0.0) var = 0.0;
+                    double e = sqrt(var / nf) / (ybar + 1e-4);
+                    flag = e > tau;
+                }
+            }
+        }
+        unsigned long long m = __ballot(flag);
+        if (m) {
+            int leader = __ffsll((long long)m) - 1;
+            int at = 0;
+            if (lane == leader) at = atomicAdd(count, __popcll(m));
+            at = __shfl(at, leader);
+            if (flag) ab.list[at + __popcll(m & ((1ull << lane) - 1ull))] = p;
+        }
+    }
+}
+
+// Mean of each pixel's samples -> linear float and 8-bit output; the sum and
+// maximum of the per-pixel sample counts (reference src/render.c:1474-1494).
+template <typename R>
+__global__ __launch_bounds__(256) void k_adapt_resolve(AdaptBuf ab, int npix, LaunchOut out)
+{
+    long long tot = 0;
+    int mx = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+        int n = ab.ns[p];
+        double inv = 1.0 / (double)n;
+        size_t o = (size_t)p * 3;
+        for (int c = 0; c < 3; ++c) {
+            double m = ab.sum[(size_t)c * (size_t)npix + p] * inv;
+            out.linear[o + c] = (float)m;
+            out.rgb[o + c] = to_byte(m);
+        }
+        tot += n;
+        mx = n > mx ? n : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        tot += __shfl_xor(tot, off);
+        int o = __shfl_xor(mx, off);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0 && tot) {
+        atomicAdd(ab.total, (unsigned long long)tot);
+        atomicMax(ab.max_n, mx);
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_closest(SceneView<R> sc, const mh_ray* rays, int n, double tmin, double tmax,
                                                  mh_hit* out)
@@ -195,6 +327,26 @@ hipError_t launch_whitted(const SceneView<R>& sc, const CamView<R>& cam, int w, 
 }
 
 template <typename R>
+hipError_t launch_whitted_adaptive(const SceneView<R>& sc, const CamView<R>& cam, int w, int h, int n0, int nmax,
+                                   double tau, int depth, const AdaptBuf& ab, int max_passes, const LaunchOut& out,
+                                   void* stack, int cap, int threads, hipStream_t st)
+{
+    int npix = w * h;
+    int nblocks = threads / 256;
+    int fblocks = (npix + 255) / 256;
+    if (fblocks > 4096) fblocks = 4096;
+    hipLaunchKernelGGL(k_adapt_sample<R>, dim3(nblocks), dim3(256), 0, st, sc, cam, w, h, n0, nmax, depth, ab,
+                       (const int*)nullptr, out, (WItem<R>*)stack, cap);
+    for (int pass = 1; pass < max_passes; ++pass) {
+        hipLaunchKernelGGL(k_adapt_flag<R>, dim3(fblocks), dim3(256), 0, st, ab, npix, nmax, tau, ab.count + pass);
+        hipLaunchKernelGGL(k_adapt_sample<R>, dim3(nblocks), dim3(256), 0, st, sc, cam, w, h, n0, nmax, depth, ab,
+                           (const int*)(ab.count + pass), out, (WItem<R>*)stack, cap);
+    }
+    hipLaunchKernelGGL(k_adapt_resolve<R>, dim3(fblocks), dim3(256), 0, st, ab, npix, out);
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_closest(const SceneView<R>& sc, const mh_ray* rays, int n, double tmin, double tmax, mh_hit* out,
                           hipStream_t st)
 {
@@ -218,6 +370,9 @@ hipError_t launch_any(const SceneView<R>& sc, const mh_ray* rays, int n, double 
     template hipError_t mh::launch_whitted<R>(const SceneView<R>&, const CamView<R>&, int, int, int, int, int,    \
                                               const LaunchOut&, void*, int, int, hipStream_t);                     \
     template size_t mh::whitted_item_bytes<R>();                                                                   \
+    template hipError_t mh::launch_whitted_adaptive<R>(const SceneView<R>&, const CamView<R>&, int, int, int, int, \
+                                                       double, int, const AdaptBuf&, int, const LaunchOut&, void*, \
+                                                       int, int, hipStream_t);                                     \
     template hipError_t mh::launch_closest<R>(const SceneView<R>&, const mh_ray*, int, double, double, mh_hit*,    \
                                               hipStream_t);                                                        \
     template hipError_t mh::launch_any<R>(const SceneView<R>&, const mh_ray*, int, double, double, uint8_t*,       \

@@ -519,13 +519,16 @@ static int gen_forest(mt_scene *s, const mt_gen_params *p, rng_t *r, mt_gen_info
     plane.material = gm;
     plane.p0 = V(0.0, -1.0, 0.0); /* pond bed and ground level */
     plane.p1 = V(0.0, 1.0, 0.0);
-    /* the pond: a water box of depth 1 whose top (y = 0) is the water level */
+    /* the pond: a water box of depth 0.98 whose top (y = 0) is the water
+     * level. Its bottom stays clear of the bed: a face coincident with the
+     * plane would tie with it on every ray through the water, and which of
+     * the two is nearer would then turn on the last bit of the ray. */
     mh_prim pond;
     memset(&pond, 0, sizeof pond);
     pond.kind = MH_PRIM_BOX;
     pond.material = wm;
-    pond.p0 = V(0.0, -0.5, 0.0);
-    pond.p1 = V(4.0, 0.5, 3.0);
+    pond.p0 = V(0.0, -0.49, 0.0);
+    pond.p1 = V(4.0, 0.49, 3.0);
     if (mt_scene_add_prims(s, &plane, NULL, 1) < 0 || mt_scene_add_prims(s, &pond, NULL, 1) < 0) return -1;
     mt_scene_set_water(s, 0.0, wm);
     /* banks around the pond: a ring of ground tiles at y = 0 */

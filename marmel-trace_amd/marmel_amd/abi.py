@@ -130,6 +130,24 @@ class RenderParams(C.Structure):
                                           "grid_blocks", "use_graph", "kernel_clock", "bvh_width")]
 
 
+class RenderExParams(C.Structure):
+    """render_image_ex parameters (reference RenderParams)."""
+    _fields_ = [("samples_per_pixel", C.c_int32), ("max_depth", C.c_int32), ("adaptive", C.c_int32),
+                ("adaptive_max_spp", C.c_int32), ("adaptive_tau", C.c_double)]
+
+
+class FlareParams(C.Structure):
+    """Anamorphic streak flare parameters (reference FlareParams layout)."""
+    _fields_ = [("enabled", C.c_int32), ("reserved", C.c_int32), ("intensity", C.c_double),
+                ("threshold", C.c_double), ("streak_length", C.c_double), ("tint", Vec3)]
+
+
+def flare_default(enabled=1):
+    """The reference's defaults (src/flare.h:38-41, flare.c:10-19), enabled."""
+    return FlareParams(enabled=enabled, intensity=0.6, threshold=0.85, streak_length=0.25,
+                       tint=Vec3(0.15, 0.45, 1.0))
+
+
 class Job(C.Structure):
     _fields_ = [("scene", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
                 ("max_depth", C.c_int32), ("sample_offset", C.c_int32), ("cam", Camera)]

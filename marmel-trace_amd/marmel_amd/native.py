@@ -96,6 +96,13 @@ def hip():
     _sig(L, "mh_render_image", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int, C.c_int,
                                          C.c_int, P])
     _sig(L, "mh_render", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P, P])
+    _sig(L, "mh_render_image_ex", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int,
+                                            C.POINTER(abi.RenderExParams), P])
+    _sig(L, "mh_render_adaptive", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int,
+                                            C.POINTER(abi.RenderExParams), C.c_int, P, P, P])
+    _sig(L, "mh_last_sample_stats", C.c_int, [C.POINTER(C.c_ulonglong), C.POINTER(C.c_int)])
+    _sig(L, "mh_flare_apply", C.c_int, [P, C.c_int, C.c_int, C.POINTER(abi.FlareParams)])
+    _sig(L, "mh_scene_flare", C.c_int, [P, C.POINTER(abi.FlareParams)])
     _sig(L, "mh_framebuffer_download", C.c_int, [P, P, P])
     _sig(L, "mh_scene_set_materials", C.c_int, [P, P, C.c_int])
     _sig(L, "mh_scene_set_sky", C.c_int, [P, C.POINTER(abi.Sky)])

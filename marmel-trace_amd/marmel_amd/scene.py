@@ -294,6 +294,38 @@ class GpuScene:
                                      abi.ptr(rgb))
         return rc, rgb
 
+    def render_image_ex(self, cam, width, height, params):
+        """Drop-in form of reference render_image_ex: returns (code, rgb)."""
+        rgb = np.zeros((max(height, 1), max(width, 1), 3), np.uint8)
+        rc = self._L.mh_render_image_ex(self._h, C.byref(cam), width, height,
+                                        C.byref(params) if params is not None else None, abi.ptr(rgb))
+        return rc, rgb
+
+    def render_adaptive(self, cam, width, height, spp, depth, max_spp=0, tau=0.0,
+                        precision=abi.PREC_FP64, adaptive=True):
+        """Adaptive Whitted: (rgb, linear means, samples per pixel, total, max)."""
+        p = abi.RenderExParams(samples_per_pixel=spp, max_depth=depth, adaptive=int(adaptive),
+                               adaptive_max_spp=max_spp, adaptive_tau=tau)
+        rgb = np.zeros((height, width, 3), np.uint8)
+        lin = np.zeros((height, width, 3), np.float32)
+        ns = np.zeros((height, width), np.int32)
+        self._check(self._L.mh_render_adaptive(self._h, C.byref(cam), width, height, C.byref(p), precision,
+                                               abi.ptr(rgb), abi.ptr(lin), abi.ptr(ns)))
+        tot, mx = last_sample_stats()
+        return rgb, lin, ns, tot, mx
+
+    def flare(self, params):
+        """Streak flare on the device framebuffer of the last render (then
+        download() returns the composited bytes)."""
+        self._check(self._L.mh_scene_flare(self._h, C.byref(params)))
+
+    def download(self, width, height):
+        """(rgb, linear) of the device framebuffer of the last render (its size)."""
+        rgb = np.zeros((height, width, 3), np.uint8)
+        lin = np.zeros((height, width, 3), np.float32)
+        self._check(self._L.mh_framebuffer_download(self._h, abi.ptr(rgb), abi.ptr(lin)))
+        return rgb, lin
+
     def trace_closest(self, rays, tmin=1e-4, tmax=1e30, precision=abi.PREC_FP32):
         rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
         out = np.zeros(len(rays), abi.HIT_DTYPE)
@@ -358,6 +390,10 @@ class GpuScene:
         native.hip().mh_last_timing(C.byref(t))
         return t
 
+    @staticmethod
+    def last_sample_stats():
+        return last_sample_stats()
+
 
 class Batch:
     """Batched renderer (mh_batch): many jobs over many GPU scenes in one
@@ -416,6 +452,23 @@ class Batch:
         t = abi.Tune()
         self._L.mh_batch_last_tune(self._h, C.byref(t))
         return {n: getattr(t, n) for n, _ in abi.Tune._fields_}
+
+
+def flare_apply(rgb, params):
+    """Drop-in form of reference flare_apply on a (h, w, 3) uint8 image, in
+    place on the GPU; returns the code."""
+    if rgb is None:
+        return native.hip().mh_flare_apply(None, 1, 1, C.byref(params))
+    assert rgb.dtype == np.uint8 and rgb.flags["C_CONTIGUOUS"] and rgb.ndim == 3
+    return native.hip().mh_flare_apply(abi.ptr(rgb), rgb.shape[1], rgb.shape[0],
+                                       C.byref(params) if params is not None else None)
+
+
+def last_sample_stats():
+    """(total primary samples, largest per-pixel count) of the last render on this thread."""
+    tot, mx = C.c_ulonglong(), C.c_int()
+    native.hip().mh_last_sample_stats(C.byref(tot), C.byref(mx))
+    return tot.value, mx.value
 
 
 def device_info():

@@ -109,6 +109,10 @@ void or_noise(const double *xy, int n, int octaves, unsigned seed, double *perli
               double *fbm);
 /* 3D gradient noise and its 3-octave fBm (the fog turbulence) at n points. */
 void or_noise3(const double *xyz, int n, unsigned seed, double *perlin, double *fbm);
+
+/* Anamorphic streak flare on a top-down RGB8 image, in place (reference
+ * flare_apply). 1: invalid arguments, 2: out of memory. */
+int or_flare_apply(unsigned char *rgb, int width, int height, const mh_flare_params *p);
 void or_sun_disk_dirs(const double sun[3], double radius_deg, const double *u,
                       int n, double *out);
 void or_light_cone_dirs(const double w[3], double cos_max, const double *u, int n,

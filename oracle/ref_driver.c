@@ -16,6 +16,7 @@
 
 #include "bvh.h"
 #include "camera.h"
+#include "flare.h"
 #include "geometry.h"
 #include "material.h"
 #include "noise.h"
@@ -611,4 +612,17 @@ int mtref_render_ex(void *h, const mh_camera *c, int w, int hgt, int n0, int dep
     if (total) *total = render_last_total_samples();
     if (max_n) *max_n = render_last_max_samples();
     return rc;
+}
+
+/* flare_apply with FlareParams built field by field from mh_flare_params. */
+int mtref_flare(unsigned char *rgb, int w, int hgt, const mh_flare_params *p)
+{
+    if (!p) return flare_apply(rgb, w, hgt, NULL);
+    FlareParams f;
+    f.enabled = p->enabled;
+    f.intensity = p->intensity;
+    f.threshold = p->threshold;
+    f.streak_length = p->streak_length;
+    f.tint = vec3(p->tint.x, p->tint.y, p->tint.z);
+    return flare_apply(rgb, w, hgt, &f);
 }

@@ -67,6 +67,7 @@ def oracle():
         _sig(L, "or_water_normals", None, [P, C.c_int, P])
         _sig(L, "or_noise", None, [P, C.c_int, C.c_int, C.c_uint, P, P])
         _sig(L, "or_noise3", None, [P, C.c_int, C.c_uint, P, P])
+        _sig(L, "or_flare_apply", C.c_int, [P, C.c_int, C.c_int, C.POINTER(abi.FlareParams)])
         _sig(L, "or_render_adaptive", C.c_int, [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, C.c_int,
                                                 P, P, P, C.POINTER(C.c_longlong), C.POINTER(C.c_int)])
         _sig(L, "or_sun_disk_dirs", None, [DP, D, P, C.c_int, P])
@@ -108,6 +109,7 @@ def ref():
         _sig(L, "mtref_noise", None, [P, C.c_int, C.c_int, C.c_uint, P, P])
         _sig(L, "mtref_noise3", None, [P, C.c_int, C.c_uint, P, P])
         _sig(L, "mtref_ocean", C.c_int, [P, C.POINTER(abi.Ocean)])
+        _sig(L, "mtref_flare", C.c_int, [P, C.c_int, C.c_int, C.POINTER(abi.FlareParams)])
         _sig(L, "mtref_render_ex", C.c_int, [P, C.POINTER(abi.Camera), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                              C.c_int, D, P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_int)])
         _sig(L, "mtref_obj_prims", C.c_int, [C.c_char_p, C.c_size_t, C.c_int, DP, DP, DP, C.c_int, C.c_int, D,

@@ -152,6 +152,28 @@ def adaptive():
     np.savez_compressed(os.path.join(HERE, "adaptive.npz"), **out)
 
 
+def flare():
+    """Reference flare_apply on tests/flare_cases.py's images (stored with
+    the outputs) and on the default scene's 320x180 Whitted image, under
+    each parameter set."""
+    import flare_cases as fc
+    out = {}
+    imgs = {k: f() for k, f in fc.IMAGES.items()}
+    d = np.load(os.path.join(HERE, "default_scene.npz"))
+    imgs["whitted_320x180"] = d["whitted_320x180_s4_d4"]
+    for name, img in imgs.items():
+        if name != "whitted_320x180":
+            out[f"in_{name}"] = img
+        for pn, p in fc.PARAMS.items():
+            if name == "whitted_320x180" and pn not in ("default", "strong", "thresh_low"):
+                continue
+            o = np.ascontiguousarray(img.copy())
+            rc = ck.ref().mtref_flare(abi.ptr(o), img.shape[1], img.shape[0], C.byref(p))
+            assert rc == 0
+            out[f"out_{name}_{pn}"] = o
+    np.savez_compressed(os.path.join(HERE, "flare.npz"), **out)
+
+
 def vectors(r, sky):
     """Reference outputs of the per-function routines on seeded inputs."""
     L = ck.ref()
