@@ -177,6 +177,35 @@ def batch_leg():
             "batched_vs_plain": round(batched / plain, 3), "tune": tune}
 
 
+def post_leg(g, cam, reps):
+    """Steps after the framebuffer on the same scene and camera: the adaptive
+    Whitted sampler (fp64, n0 = 4, cap 16, tau 0.02, depth 5; the drop-in's
+    precision) against the fixed 4-spp Whitted render, then the streak flare
+    (defaults, enabled) on the device framebuffer of that frame."""
+    from marmel_amd import GpuScene, abi
+    from marmel_amd.abi import flare_default
+    g.render_adaptive(cam, W, H, 4, 5, 16, 0.02)  # warm-up (allocates the state)
+    ks, tots = [], []
+    for _ in range(reps):
+        _, _, _, tot, mx = g.render_adaptive(cam, W, H, 4, 5, 16, 0.02)
+        ks.append(GpuScene.last_timing().kernel_ms)
+        tots.append(tot)
+    ad_ms = sum(ks) / len(ks)
+    g.render(cam, W, H, 4, 5, mode=abi.MODE_WHITTED, precision=abi.PREC_FP64, fetch=False)
+    fixed_ms = GpuScene.last_timing().kernel_ms
+    fl = flare_default()
+    g.flare(fl)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.flare(fl)
+    flare_ms = (time.perf_counter() - t0) / reps * 1e3
+    return {"workload": f"{W}x{H} Whitted fp64 depth 5, adaptive n0=4 cap=16 tau=0.02; flare defaults",
+            "adaptive_kernel_ms": round(ad_ms, 2), "samples": tots[-1],
+            "samples_per_pixel": round(tots[-1] / (W * H), 3),
+            "adaptive_msamples_per_s": round(tots[-1] / ad_ms * 1e-3, 1),
+            "fixed_4spp_kernel_ms": round(fixed_ms, 2), "flare_ms": round(flare_ms, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +226,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint ms/grad leg")
     ap.add_argument("--adjoint-reps", type=int, default=3)
     ap.add_argument("--no-batch32", action="store_true", help="skip the 32-scene batched leg")
+    ap.add_argument("--no-post", action="store_true",
+                    help="skip the adaptive Whitted + flare leg (steps after the framebuffer)")
+    ap.add_argument("--only-post", action="store_true",
+                    help="run only the adaptive Whitted + flare leg (for profiling its kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -214,6 +247,10 @@ def main():
     g = GpuScene(hs.soa())
     log(f"bench: device {name}, scene resident: {g.device_bytes / 2**20:.1f} MiB")
     cam = hs.camera(W / H)
+
+    if args.only_post:
+        log(f"bench: post {post_leg(g, cam, 3)}")
+        return
 
     engines = {"wavefront": dict(engine=abi.ENGINE_WAVEFRONT, pool=args.pool, grid_blocks=args.grid_blocks,
                                  use_graph=args.graph, kernel_clock=not args.graph),
@@ -257,6 +294,9 @@ def main():
     if adj:
         log(f"bench: adjoint {adj}")
     b32 = None if args.no_batch32 else batch_leg()
+    post = None if args.no_post else post_leg(g, cam, 3)
+    if post:
+        log(f"bench: post {post}")
 
     # host threads: the GPU's share of the host (16 cores), as the reference
     # caps its pool at min(nproc, 64) (pathtrace.c:1210-1234)
@@ -301,6 +341,7 @@ def main():
             (value / plain) if args.engine == "wavefront" else (plain / value), 3),
         "adjoint": adj,
         "batch32": b32,
+        "after_framebuffer": post,
         "roofline": roof,
         "cpu_baseline": cpu,
         "device": name,

@@ -1802,7 +1802,7 @@ static int flare_prepare(const mh_flare_params* p, int w, FlareK* k)
     return 0;
 }
 
-// Rows in LDS when 9 float planes fit the device's per-workgroup limit, else
+// Rows in LDS when 15 float planes fit the device's per-workgroup limit, else
 // in a scratch slice per workgroup (grown on demand in *mem).
 static int flare_run(unsigned char* d_rgb, int w, int h, const FlareK& k, void** mem, size_t* bytes, hipStream_t st)
 {

@@ -4,33 +4,44 @@
 // equal the reference's.
 //
 // One wavefront per scanline. Phase 1 (all lanes, over x): highlight energy
-// of each pixel in double -> three float source planes. Phase 2 (lanes 0-2,
-// one colour channel each): the forward recurrence acc = src + alpha * acc
-// into a buffer, then the backward one combined on the fly into the streak
-// accumulator, first at the core radius, then at the tail radius; these are
-// first-order recurrences whose rounding fixes the result, so they run in
-// order, one lane per channel. Phase 3 (all lanes): composite and clamp.
-// The 9 planes of a row live in LDS when they fit (36 bytes per pixel), else
-// in a global scratch slice per workgroup.
+// of each pixel in double -> three float source planes. Phase 2 (12 lanes):
+// the first-order recurrences acc = src + alpha * acc, one lane each per
+// colour channel, scale (core and tail radius) and direction (forward and
+// backward); their rounding fixes the result, so each runs in order, with
+// its loads and stores blocked through registers so LDS latency is paid
+// once per block of 16. Phase 3 (all lanes): per pixel and channel the two
+// scales' (forward + backward - src) * weight * 0.045 are added to zero in
+// the reference's order, composited and clamped.
+// The 15 planes of a row (60 bytes per pixel) live in LDS when they fit,
+// else in a global scratch slice per workgroup.
 #include "mh_flare.h"
 
 #include <hip/hip_runtime.h>
 
 namespace mh {
 
-__device__ __forceinline__ void streak(const float* src, float* dst, float* fwd, int n, float alpha, float norm)
+// n steps of acc = src + alpha * acc along src[0], src[step], ... (step = 1
+// forward, -1 backward), writing each acc to the same position of out.
+__device__ __forceinline__ void recur(const float* src, float* out, int n, int step, float alpha)
 {
+    constexpr int B = 16;
     float acc = 0.0f;
-#pragma unroll 8
-    for (int x = 0; x < n; ++x) {
-        acc = src[x] + alpha * acc;
-        fwd[x] = acc;
+    int i = 0;
+    for (; i + B <= n; i += B) {
+        float v[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) v[k] = src[(i + k) * step];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            acc = v[k] + alpha * acc;
+            v[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k) out[(i + k) * step] = v[k];
     }
-    acc = 0.0f;
-#pragma unroll 8
-    for (int x = n - 1; x >= 0; --x) {
-        acc = src[x] + alpha * acc;
-        dst[x] += (fwd[x] + acc - src[x]) * norm;
+    for (; i < n; ++i) {
+        acc = src[i * step] + alpha * acc;
+        out[i * step] = acc;
     }
 }
 
@@ -40,7 +51,8 @@ __global__ __launch_bounds__(64) void k_flare(unsigned char* rgb, int w, int h, 
     extern __shared__ float lds[];
     int lane = threadIdx.x;
     size_t wz = (size_t)w;
-    float* buf = LDS ? lds : scratch + (size_t)blockIdx.x * 9 * wz;
+    // planes: src[3], then per (channel, scale, direction) the recurrence
+    float* buf = LDS ? lds : scratch + (size_t)blockIdx.x * 15 * wz;
     for (int y = blockIdx.x; y < h; y += gridDim.x) {
         unsigned char* row = rgb + (size_t)y * wz * 3;
         int any = 0;
@@ -62,23 +74,27 @@ __global__ __launch_bounds__(64) void k_flare(unsigned char* rgb, int w, int h, 
             buf[x] = s0;
             buf[wz + x] = s1;
             buf[2 * wz + x] = s2;
-            buf[3 * wz + x] = 0.0f;
-            buf[4 * wz + x] = 0.0f;
-            buf[5 * wz + x] = 0.0f;
         }
         if (__ballot(any) == 0) continue;   // the row has no highlight: unchanged
         __syncthreads();
-        if (lane < 3) {
-            const float* src = buf + (size_t)lane * wz;
-            float* dst = buf + (size_t)(3 + lane) * wz;
-            float* fwd = buf + (size_t)(6 + lane) * wz;
-            streak(src, dst, fwd, w, k.a1, k.n1);
-            streak(src, dst, fwd, w, k.a2, k.n2);
+        if (lane < 12) {
+            int c = lane >> 2, scale = (lane >> 1) & 1, bwd = lane & 1;
+            const float* src = buf + (size_t)c * wz;
+            float* out = buf + (size_t)(3 + lane) * wz;
+            int o = bwd ? w - 1 : 0;
+            recur(src + o, out + o, w, bwd ? -1 : 1, scale ? k.a2 : k.a1);
         }
         __syncthreads();
         for (int x = lane; x < w; x += 64) {
             for (int c = 0; c < 3; ++c) {
-                int v = (int)((float)row[3 * x + c] + buf[(size_t)(3 + c) * wz + x] * 255.0f + 0.5f);
+                const float* q = buf + (size_t)(3 + 4 * c) * wz + x;
+                float sv = buf[(size_t)c * wz + x];
+                float v1 = (q[0] + q[wz] - sv) * k.n1;
+                float v2 = (q[2 * wz] + q[3 * wz] - sv) * k.n2;
+                float d = 0.0f;
+                d += v1;
+                d += v2;
+                int v = (int)((float)row[3 * x + c] + d * 255.0f + 0.5f);
                 row[3 * x + c] = (unsigned char)(v > 255 ? 255 : (v < 0 ? 0 : v));
             }
         }

@@ -187,47 +187,72 @@ __global__ __launch_bounds__(256) void k_adapt_sample(SceneView<R> sc, CamView<R
 
 // Active set of the next pass from each pixel's own running statistics:
 // refine while n < nmax and (n < 2 or the relative standard error of the
-// mean luminance exceeds tau). Appends the pixels to ab.list (one atomic per
-// wave) and their number to *count. Follows reference src/render.c:1330-1361.
+// mean luminance exceeds tau). Appends the pixels to ab.list and their number
+// to *count: one atomic per workgroup and round of 1024 pixels (an offset
+// shared through LDS), as a counter hit by every wave serialises.
+// Follows reference src/render.c:1330-1361.
 template <typename R>
 __global__ __launch_bounds__(256) void k_adapt_flag(AdaptBuf ab, int npix, int nmax, double tau, int* count)
 {
-    int lane = threadIdx.x & 63;
-    for (int base = blockIdx.x * blockDim.x; base < npix; base += gridDim.x * blockDim.x) {
-        int p = base + threadIdx.x;
-        int flag = 0;
-        if (p < npix) {
-            int n = ab.ns[p];
-            if (n < nmax) {
-                if (n < 2) {
-                    flag = 1;
-                } else {
-                    double nf = (double)n;
-                    double sy = ab.sum[3 * (size_t)npix + p], sy2 = ab.sum[4 * (size_t)npix + p];
-                    double ybar = sy / nf;
-                    double var = (sy2 - sy * ybar) / (nf - 1.0);
-                    if (var < 0.0) var = 0.0;
-                    double e = sqrt(var / nf) / (ybar + 1e-4);
-                    flag = e > tau;
+    __shared__ int wave_n[4][4];
+    __shared__ int base_at;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int base = blockIdx.x * 1024; base < npix; base += gridDim.x * 1024) {
+        int flag[4];
+        unsigned long long m[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int p = base + r * 256 + threadIdx.x;
+            int f = 0;
+            if (p < npix) {
+                int n = ab.ns[p];
+                if (n < nmax) {
+                    if (n < 2) {
+                        f = 1;
+                    } else {
+                        double nf = (double)n;
+                        double sy = ab.sum[3 * (size_t)npix + p], sy2 = ab.sum[4 * (size_t)npix + p];
+                        double ybar = sy / nf;
+                        double var = (sy2 - sy * ybar) / (nf - 1.0);
+                        if (var < 0.0) var = 0.0;
+                        double e = sqrt(var / nf) / (ybar + 1e-4);
+                        f = e > tau;
+                    }
                 }
             }
+            flag[r] = f;
+            m[r] = __ballot(f);
+            if (lane == 0) wave_n[r][wave] = __popcll(m[r]);
         }
-        unsigned long long m = __ballot(flag);
-        if (m) {
-            int leader = __ffsll((long long)m) - 1;
-            int at = 0;
-            if (lane == leader) at = atomicAdd(count, __popcll(m));
-            at = __shfl(at, leader);
-            if (flag) ab.list[at + __popcll(m & ((1ull << lane) - 1ull))] = p;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int r = 0; r < 4; ++r)
+                for (int q = 0; q < 4; ++q) tot += wave_n[r][q];
+            base_at = tot ? atomicAdd(count, tot) : 0;
         }
+        __syncthreads();
+        int at = base_at;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int before = 0;
+            for (int rr = 0; rr < r; ++rr)
+                for (int q = 0; q < 4; ++q) before += wave_n[rr][q];
+            for (int q = 0; q < wave; ++q) before += wave_n[r][q];
+            if (flag[r]) ab.list[at + before + __popcll(m[r] & ((1ull << lane) - 1ull))] = base + r * 256 + threadIdx.x;
+        }
+        __syncthreads();  // wave_n and base_at are reused by the next round
     }
 }
 
 // Mean of each pixel's samples -> linear float and 8-bit output; the sum and
-// maximum of the per-pixel sample counts (reference src/render.c:1474-1494).
+// maximum of the per-pixel sample counts, reduced per workgroup before one
+// atomic each (reference src/render.c:1474-1494).
 template <typename R>
 __global__ __launch_bounds__(256) void k_adapt_resolve(AdaptBuf ab, int npix, LaunchOut out)
 {
+    __shared__ long long s_tot[4];
+    __shared__ int s_max[4];
     long long tot = 0;
     int mx = 0;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
@@ -247,9 +272,23 @@ __global__ __launch_bounds__(256) void k_adapt_resolve(AdaptBuf ab, int npix, La
         int o = __shfl_xor(mx, off);
         mx = o > mx ? o : mx;
     }
-    if ((threadIdx.x & 63) == 0 && tot) {
-        atomicAdd(ab.total, (unsigned long long)tot);
-        atomicMax(ab.max_n, mx);
+    int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_tot[wave] = tot;
+        s_max[wave] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        int m = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+            t += s_tot[q];
+            m = s_max[q] > m ? s_max[q] : m;
+        }
+        if (t) {
+            atomicAdd(ab.total, (unsigned long long)t);
+            atomicMax(ab.max_n, m);
+        }
     }
 }
 
@@ -333,8 +372,10 @@ hipError_t launch_whitted_adaptive(const SceneView<R>& sc, const CamView<R>& cam
 {
     int npix = w * h;
     int nblocks = threads / 256;
-    int fblocks = (npix + 255) / 256;
-    if (fblocks > 4096) fblocks = 4096;
+    int fblocks = (npix + 1023) / 1024;  // flag: 1024 pixels per workgroup and round
+    if (fblocks > 1024) fblocks = 1024;
+    int rblocks = (npix + 255) / 256;    // resolve: grid-strided, one atomic pair per workgroup
+    if (rblocks > 1024) rblocks = 1024;
     hipLaunchKernelGGL(k_adapt_sample<R>, dim3(nblocks), dim3(256), 0, st, sc, cam, w, h, n0, nmax, depth, ab,
                        (const int*)nullptr, out, (WItem<R>*)stack, cap);
     for (int pass = 1; pass < max_passes; ++pass) {
@@ -342,7 +383,7 @@ hipError_t launch_whitted_adaptive(const SceneView<R>& sc, const CamView<R>& cam
         hipLaunchKernelGGL(k_adapt_sample<R>, dim3(nblocks), dim3(256), 0, st, sc, cam, w, h, n0, nmax, depth, ab,
                            (const int*)(ab.count + pass), out, (WItem<R>*)stack, cap);
     }
-    hipLaunchKernelGGL(k_adapt_resolve<R>, dim3(fblocks), dim3(256), 0, st, ab, npix, out);
+    hipLaunchKernelGGL(k_adapt_resolve<R>, dim3(rblocks), dim3(256), 0, st, ab, npix, out);
     return hipGetLastError();
 }
 
