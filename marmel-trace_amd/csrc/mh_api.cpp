@@ -2023,6 +2023,8 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     mj.total = begin;
     mj.total_pix = pix;
     mj.acc64 = b->acc64;
+    mj.sh_stride = 1;
+    for (const auto& v : views) mj.sh_stride = std::max(mj.sh_stride, v.n_lights + 1);
 
     if (t.pool == 0) t.pool = 1 << 21;
     if (t.grid_blocks == 0) t.grid_blocks = 2048;

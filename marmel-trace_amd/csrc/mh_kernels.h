@@ -75,6 +75,7 @@ struct WfMulti {
     long long total_pix;         // pixels over all jobs
     unsigned long long* acc64;   // fixed-point (2^-24) per-pixel sums, 3 per pixel
     int stream;                  // path regeneration: per-slot bounce, free-slot list
+    int sh_stride;               // shadow-queue blocks per bounce: max lights over the scenes + 1 (sun)
 };
 
 struct WfJob {

@@ -40,19 +40,58 @@ namespace mh {
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 
-// Append `want` lanes of the wave to a queue: one atomic per wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want, bool* ok)
+// Block-wide append to Q queues: lane wants want[q] (0/1) entries in queue q;
+// returns each wanted entry's position in at[q]. One atomic per queue per
+// block and call: a counter that every wave hits serialises in L2 (~7 ns per
+// same-address atomic measured). Every thread of the block must call it
+// the same number of times (it synchronises); blockDim <= 256.
+template <int Q>
+__device__ __forceinline__ void block_append(uint32_t* const cnt[Q], const bool want[Q], uint32_t at[Q])
 {
-    uint64_t mask = __ballot(want);
-    *ok = want;
-    if (!mask) return 0;
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-    return base + rank;
+    __shared__ uint32_t wn[Q][4];
+    __shared__ uint32_t base[Q];
+    int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+    __syncthreads();  // the previous call's readers are done
+    uint64_t m[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        m[q] = __ballot(want[q]);
+        if (lane == 0) wn[q][wave] = (uint32_t)__popcll(m[q]);
+    }
+    __syncthreads();
+    int nw = (int)(blockDim.x >> 6);
+    if ((int)threadIdx.x < Q) {
+        int q = (int)threadIdx.x;
+        uint32_t tot = 0;
+        for (int w = 0; w < nw; ++w) tot += wn[q][w];
+        base[q] = tot ? atomicAdd(cnt[q], tot) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        uint32_t b = base[q];
+        for (int w = 0; w < wave; ++w) b += wn[q][w];
+        at[q] = b + __builtin_amdgcn_mbcnt_hi((uint32_t)(m[q] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[q], 0u));
+    }
 }
+
+// Ray-query count of a block: one atomic per block (every thread calls it).
+__device__ __forceinline__ void count_rays_block(unsigned long long* ctr, int nq)
+{
+    __shared__ unsigned long long part[4];
+    unsigned long long v = (unsigned long long)nq;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (__lane_id() == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        if (t) atomicAdd(ctr, t);
+    }
+}
+
+// Shadow-queue tag of an empty entry (no ray for this light and path).
+constexpr uint32_t SH_NONE = 0xffffffffu;
 
 __global__ __launch_bounds__(64) void k_wf_reset(uint32_t* cnt, int which)
 {
@@ -158,10 +197,11 @@ __global__ __launch_bounds__(256) void k_wf_refill(WfState st, WfMulti mj, int w
     uint32_t nfree = st.cnt[4];
     uint32_t k = refill_count(st, mj);
     unsigned long long g0 = st.batch[0];
-    uint32_t n_round = (k + 63u) & ~63u;
+    uint32_t n_round = (k + blockDim.x - 1u) / blockDim.x * blockDim.x;  // block-uniform trip count
     SceneView<float> none{};
     CamView<float> cnone{};
     WfJob jnone{};
+    uint32_t* const cq[1] = {&st.cnt[which]};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += gridDim.x * blockDim.x) {
         bool active = i < k;
         uint32_t slot = 0;
@@ -169,9 +209,10 @@ __global__ __launch_bounds__(256) void k_wf_refill(WfState st, WfMulti mj, int w
             slot = st.freeq[nfree - 1u - i];
             start_path<true>(none, cnone, jnone, st, mj, (long long)g0 + i, slot);
         }
-        bool ok;
-        uint32_t at = wave_append(&st.cnt[which], active, &ok);
-        if (ok) st.q[which][at] = slot;
+        const bool want[1] = {active};
+        uint32_t at[1];
+        block_append<1>(cq, want, at);
+        if (active) st.q[which][at[0]] = slot;
     }
 }
 
@@ -251,11 +292,19 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
     const uint32_t* qin = st.q[which];
     uint32_t* qout = st.q[which ^ 1];
     int nq = 0;
-    uint32_t n_round = (n + 63u) & ~63u;
+    // Shadow queue: fixed entries, light-major (entry li * n + i for queue
+    // position i), so a shadow wave holds one light's rays from neighbouring
+    // paths and no counter is touched; entries without a ray are tagged
+    // SH_NONE. S blocks: the lights (at most 8) and the sun, last.
+    const uint32_t S = MULTI ? (uint32_t)mj.sh_stride : (uint32_t)sc0.n_lights + 1u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.cnt[2] = S * n;
+    uint32_t* const cq[2] = {&st.cnt[which ^ 1], &st.cnt[3]};
+    uint32_t n_round = (n + blockDim.x - 1u) / blockDim.x * blockDim.x;  // block-uniform trip count
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += gridDim.x * blockDim.x) {
         bool active = i < n;
         uint32_t slot = active ? qin[i] : 0u;
         bool alive = false, ended = false;
+        uint32_t emitted = 0u;  // shadow entries written (bit = block)
         V3<float> nray_o = zero3<float>(), nray_d = zero3<float>();
         if (active) {
             const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
@@ -361,12 +410,12 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
                                     }
                                 }
                             }
-                            bool ok;
-                            uint32_t at = wave_append(&st.cnt[2], emit, &ok);
-                            if (ok) {
+                            if (emit) {
+                                size_t at = (size_t)li * n + i;
                                 st.sh0[at] = make_float4(so.x, so.y, so.z, tmax);
                                 st.sh1[at] = make_float4(wi.x, wi.y, wi.z, u2f((slot << 4) | (uint32_t)li));
                                 st.pend[(size_t)slot * 9 + li] = make_float4(cc.x, cc.y, cc.z, 0.f);
+                                emitted |= 1u << li;
                             }
                         }
                         npend = (uint32_t)sc.n_lights;
@@ -382,12 +431,12 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
                                     cc = vmul(beta, shade(m, alb, N, Ls, V, sc.sky.sun_color) * sc.sky.sun_omega);
                                 }
                             }
-                            bool ok;
-                            uint32_t at = wave_append(&st.cnt[2], emit, &ok);
-                            if (ok) {
+                            if (emit) {
+                                size_t at = (size_t)(S - 1u) * n + i;
                                 st.sh0[at] = make_float4(so.x, so.y, so.z, 1e30f);
                                 st.sh1[at] = make_float4(Ls.x, Ls.y, Ls.z, u2f((slot << 4) | 8u));
                                 st.pend[(size_t)slot * 9 + 8] = make_float4(cc.x, cc.y, cc.z, 0.f);
+                                emitted |= 1u << (S - 1u);
                                 npend |= 16u;
                             }
                         }
@@ -436,14 +485,16 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
             st.p2[slot] = make_float4(beta.x, beta.y, beta.z, c.w);
             st.p3[slot] = make_float4(L.x, L.y, L.z, e.w);
             nq += 1;
+            for (uint32_t k = 0; k < S; ++k)
+                if (!(emitted & (1u << k))) st.sh1[(size_t)k * n + i] = make_float4(0.f, 0.f, 0.f, u2f(SH_NONE));
         }
-        bool ok;
-        uint32_t at = wave_append(&st.cnt[which ^ 1], alive, &ok);
-        if (ok) qout[at] = slot;
-        uint32_t at2 = wave_append(&st.cnt[3], ended, &ok);
-        if (ok) st.term[at2] = slot;
+        const bool want[2] = {alive, ended};
+        uint32_t at[2];
+        block_append<2>(cq, want, at);
+        if (alive) qout[at[0]] = slot;
+        if (ended) st.term[at[1]] = slot;
     }
-    count_rays(rays, nq);
+    count_rays_block(rays, nq);
 }
 
 // TRANS: some scene of the launch has a transmissive material, so a shadow
@@ -465,30 +516,65 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 #else
     MH_STACK_PRIV(tsa)
 #endif
+    // The queue has an entry per (light, path), SH_NONE where no ray was
+    // cast. Each wave scans its own 64-entry chunks and packs the live
+    // entries into a per-wave LDS queue, and traverses only when it holds 64
+    // (or at the end): every traversal runs on a full wave, with no atomics.
+    __shared__ uint32_t wq[4][128];
     uint32_t n = st.cnt[2];
     int nq = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float4 a = st.sh0[i], b = st.sh1[i];
-        uint32_t tag = f2u(b.w);
-        uint32_t slot = tag >> 4, k = tag & 15u;
-        const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
-        V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
-        bool occ;
-        if (TRANS) {
-            V3<float> att;
-            occ = occluded_segment(sc, tsw, o, d, 1e-3f, a.w, &att, &nq);
-            if (!occ && k == 8 && att.x < 1e-4f && att.y < 1e-4f && att.z < 1e-4f) occ = true;
-            if (!occ && sc.has_transmissive) {
-                float4 c = st.pend[(size_t)slot * 9 + k];
-                st.pend[(size_t)slot * 9 + k] = make_float4(c.x * att.x, c.y * att.y, c.z * att.z, 0.f);
-            }
-        } else {
-            ++nq;
-            occ = any_hit<float, decltype(tsa), 2>(sc, tsa, o, unit_dir(d), 1e-3f, a.w);
+    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+    uint32_t* q = wq[wave];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t held = 0;  // live entries in q (wave-uniform)
+    for (uint32_t base = blockIdx.x * blockDim.x + (uint32_t)wave * 64u;; base += stride) {
+        bool more = base < n;
+        if (more) {
+            uint32_t i = base + (uint32_t)lane;
+            bool live = i < n && f2u(st.sh1[i].w) != SH_NONE;
+            uint64_t m = __ballot(live);
+            uint32_t at = held + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (live) q[at] = i;
+            held += (uint32_t)__popcll(m);
+            __builtin_amdgcn_wave_barrier();
         }
-        if (!occ) atomicOr(&st.vis[slot], 1u << k);
+        if (held >= 64u || (!more && held > 0u)) {
+            uint32_t take = held < 64u ? held : 64u;
+            if ((uint32_t)lane < take) {
+                uint32_t i = q[lane];
+                float4 b = st.sh1[i], a = st.sh0[i];
+                uint32_t tag = f2u(b.w);
+                uint32_t slot = tag >> 4, k = tag & 15u;
+                const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
+                V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+                bool occ;
+                if (TRANS) {
+                    V3<float> att;
+                    occ = occluded_segment(sc, tsw, o, d, 1e-3f, a.w, &att, &nq);
+                    if (!occ && k == 8 && att.x < 1e-4f && att.y < 1e-4f && att.z < 1e-4f) occ = true;
+                    if (!occ && sc.has_transmissive) {
+                        float4 c = st.pend[(size_t)slot * 9 + k];
+                        st.pend[(size_t)slot * 9 + k] = make_float4(c.x * att.x, c.y * att.y, c.z * att.z, 0.f);
+                    }
+                } else {
+                    ++nq;
+                    occ = any_hit<float, decltype(tsa), 2>(sc, tsa, o, unit_dir(d), 1e-3f, a.w);
+                }
+                if (!occ) atomicOr(&st.vis[slot], 1u << k);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // move the rest (fewer than 64) to the front
+            uint32_t rest = held - take;
+            uint32_t v = (uint32_t)lane < rest ? q[64 + lane] : 0u;
+            __builtin_amdgcn_wave_barrier();
+            if ((uint32_t)lane < rest) q[lane] = v;
+            __builtin_amdgcn_wave_barrier();
+            held = rest;
+        }
+        if (!more && held == 0u) break;
     }
-    count_rays(rays, nq);
+    count_rays_block(rays, nq);
 }
 
 // Radiance in 2^-24 fixed point. A sample is capped at 2^30 (no physical
@@ -502,32 +588,40 @@ template <bool MULTI>
 __global__ __launch_bounds__(256) void k_wf_finalize(WfState st, WfMulti mj)
 {
     uint32_t n = st.cnt[3];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t slot = st.term[i];
-        float4 c = st.p2[slot], e = st.p3[slot];
-        V3<float> L = resolve_pending(st, slot, mk(e.x, e.y, e.z));
-        st.npend[slot] = 0;
-        if (!finite3(L)) L = zero3<float>();
-        L.x = L.x < 0.f ? 0.f : L.x;
-        L.y = L.y < 0.f ? 0.f : L.y;
-        L.z = L.z < 0.f ? 0.f : L.z;
-        uint32_t pix = f2u(c.w);
-        if (MULTI) {
-            size_t p = ((size_t)mj.jobs[st.jid[slot]].acc_off + pix) * 3;
-            atomicAdd(&mj.acc64[p], to_fixed(L.x));
-            atomicAdd(&mj.acc64[p + 1], to_fixed(L.y));
-            atomicAdd(&mj.acc64[p + 2], to_fixed(L.z));
-            if (mj.stream) {
-                // the slot is free for the next step's new paths
-                bool ok;
-                uint32_t at = wave_append(&st.cnt[4], true, &ok);
-                st.freeq[at] = slot;
+    // path regeneration appends the freed slots: block-uniform trip count
+    uint32_t n_round = (MULTI && mj.stream) ? (n + blockDim.x - 1u) / blockDim.x * blockDim.x : n;
+    uint32_t* const cq[1] = {&st.cnt[4]};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += gridDim.x * blockDim.x) {
+        bool active = i < n;
+        uint32_t slot = 0;
+        if (active) {
+            slot = st.term[i];
+            float4 c = st.p2[slot], e = st.p3[slot];
+            V3<float> L = resolve_pending(st, slot, mk(e.x, e.y, e.z));
+            st.npend[slot] = 0;
+            if (!finite3(L)) L = zero3<float>();
+            L.x = L.x < 0.f ? 0.f : L.x;
+            L.y = L.y < 0.f ? 0.f : L.y;
+            L.z = L.z < 0.f ? 0.f : L.z;
+            uint32_t pix = f2u(c.w);
+            if (MULTI) {
+                size_t p = ((size_t)mj.jobs[st.jid[slot]].acc_off + pix) * 3;
+                atomicAdd(&mj.acc64[p], to_fixed(L.x));
+                atomicAdd(&mj.acc64[p + 1], to_fixed(L.y));
+                atomicAdd(&mj.acc64[p + 2], to_fixed(L.z));
+            } else {
+                // one sample per pixel per batch: no other thread touches this pixel
+                st.acc[(size_t)pix * 3] += L.x;
+                st.acc[(size_t)pix * 3 + 1] += L.y;
+                st.acc[(size_t)pix * 3 + 2] += L.z;
             }
-        } else {
-            // one sample per pixel per batch: no other thread touches this pixel
-            st.acc[(size_t)pix * 3] += L.x;
-            st.acc[(size_t)pix * 3 + 1] += L.y;
-            st.acc[(size_t)pix * 3 + 2] += L.z;
+        }
+        if (MULTI && mj.stream) {
+            // the slot is free for the next step's new paths
+            const bool want[1] = {active};
+            uint32_t at[1];
+            block_append<1>(cq, want, at);
+            if (active) st.freeq[at[0]] = slot;
         }
     }
 }
