@@ -948,14 +948,14 @@ int gpu_build_scene(const mh_prim* d_prims, const mh_prim_attr* d_attrs, int n, 
     int *flag, *pos;
     BTRY(tmp_int(&flag, (size_t)n + 1));
     BTRY(tmp_int(&pos, (size_t)n + 1));
-    hipLaunchKernelGGL(k_not_plane, grid_of(n), dim3(256), 0, st, d_prims, n, flag);
+    if (n > 0) hipLaunchKernelGGL(k_not_plane, grid_of(n), dim3(256), 0, st, d_prims, n, flag);
     BTRY(scan_excl(flag, pos, n, ctr + C_N, scan_tmp, st));
     BTRY(read_ctr());
     int nt = hc[C_N], np = n - nt;
     int *idx;
     BTRY(tmp_int(&idx, (size_t)nt + 1));
     BTRY(dalloc((void**)&out->planes, sizeof(int32_t) * (size_t)np, true));
-    hipLaunchKernelGGL(k_split_lists, grid_of(n), dim3(256), 0, st, d_prims, n, pos, idx, out->planes);
+    if (n > 0) hipLaunchKernelGGL(k_split_lists, grid_of(n), dim3(256), 0, st, d_prims, n, pos, idx, out->planes);
     out->n_planes = np;
 
     // per-prim boxes / centroids (the host's arithmetic) and the slot order

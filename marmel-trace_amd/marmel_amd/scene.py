@@ -172,12 +172,16 @@ class HostScene:
     def arrays(self):
         s = self.soa(bvh=False)
         n, m = s.n_prims, s.n_materials
-        prims = np.ctypeslib.as_array(C.cast(s.prims, C.POINTER(C.c_uint8)),
-                                      (n * abi.PRIM_DTYPE.itemsize,)).view(abi.PRIM_DTYPE).copy()
-        mats = np.ctypeslib.as_array(C.cast(s.materials, C.POINTER(C.c_uint8)),
-                                     (m * abi.MATERIAL_DTYPE.itemsize,)).view(abi.MATERIAL_DTYPE).copy()
+
+        def view(ptr, count, dtype):
+            if count == 0 or not ptr:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_uint8)),
+                                         (count * dtype.itemsize,)).view(dtype).copy()
+        prims = view(s.prims, n, abi.PRIM_DTYPE)
+        mats = view(s.materials, m, abi.MATERIAL_DTYPE)
         attrs = None
-        if s.attrs:
+        if s.attrs and n:
             attrs = np.ctypeslib.as_array(C.cast(s.attrs, C.POINTER(C.c_uint8)),
                                           (n * abi.ATTR_DTYPE.itemsize,)).view(abi.ATTR_DTYPE).copy()
         return prims, attrs, mats, s.sky, s.water_level, s.water_material
