@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <algorithm>
@@ -1828,17 +1829,17 @@ static int flare_run(unsigned char* d_rgb, int w, int h, const FlareK& k, void**
 }
 
 namespace {
-// Device image and scratch of mh_flare_apply (host buffers), one caller at a time.
+// Device image, scratch and stream of mh_flare_apply, one set per device
+// (kept until the process ends); one caller at a time.
 struct FlareCtx {
-    std::mutex mu;
-    int device = -1;
     hipStream_t stream = nullptr;
     void* img = nullptr;
     size_t img_bytes = 0;
     void* scr = nullptr;
     size_t scr_bytes = 0;
 };
-FlareCtx g_flare;
+std::mutex g_flare_mu;
+std::map<int, FlareCtx> g_flare;
 } // namespace
 
 int mh_flare_apply(unsigned char* rgb, int width, int height, const mh_flare_params* params)
@@ -1847,32 +1848,25 @@ int mh_flare_apply(unsigned char* rgb, int width, int height, const mh_flare_par
     FlareK k;
     if (flare_prepare(params, width, &k)) return 0;
     if ((long long)width * height > (1LL << 28)) return fail(2, "image too large");
-    std::lock_guard<std::mutex> lock(g_flare.mu);
+    std::lock_guard<std::mutex> lock(g_flare_mu);
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
-    if (g_flare.device != dev) {
-        // a context made on another device is left as it is (its memory
-        // belongs to that device); this device gets its own
-        g_flare.stream = nullptr;
-        g_flare.img = g_flare.scr = nullptr;
-        g_flare.img_bytes = g_flare.scr_bytes = 0;
-        HIP_TRY(hipStreamCreateWithFlags(&g_flare.stream, hipStreamNonBlocking));
-        g_flare.device = dev;
-    }
+    FlareCtx& c = g_flare[dev];
+    if (!c.stream) HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     size_t n = (size_t)width * height * 3;
-    if (n > g_flare.img_bytes) {
-        if (g_flare.img) (void)hipFree(g_flare.img);
-        g_flare.img = nullptr;
-        g_flare.img_bytes = 0;
-        HIP_TRY(hipMalloc(&g_flare.img, n));
-        g_flare.img_bytes = n;
+    if (n > c.img_bytes) {
+        if (c.img) (void)hipFree(c.img);
+        c.img = nullptr;
+        c.img_bytes = 0;
+        HIP_TRY(hipMalloc(&c.img, n));
+        c.img_bytes = n;
     }
-    unsigned char* d = (unsigned char*)g_flare.img;
-    HIP_TRY(hipMemcpyAsync(d, rgb, n, hipMemcpyHostToDevice, g_flare.stream));
-    int rc = flare_run(d, width, height, k, &g_flare.scr, &g_flare.scr_bytes, g_flare.stream);
+    unsigned char* d = (unsigned char*)c.img;
+    HIP_TRY(hipMemcpyAsync(d, rgb, n, hipMemcpyHostToDevice, c.stream));
+    int rc = flare_run(d, width, height, k, &c.scr, &c.scr_bytes, c.stream);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(rgb, d, n, hipMemcpyDeviceToHost, g_flare.stream));
-    HIP_TRY(hipStreamSynchronize(g_flare.stream));
+    HIP_TRY(hipMemcpyAsync(rgb, d, n, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
     return 0;
 }
 

@@ -49,6 +49,34 @@ def test_gpu_entry_points_reject_null_arguments():
     assert L.mh_scene_create_gpu_bvh(None, 0, 0, None) == 1
     assert L.mh_scene_bvh_info(None, None) == 1
     assert L.mh_scene_bvh_export(None, None, None) == 1
+    p = abi.RenderExParams(samples_per_pixel=4, max_depth=5, adaptive=1)
+    assert L.mh_render_image_ex(None, None, 10, 10, C.byref(p), None) == 1
+    assert L.mh_render_image_ex(None, None, 10, 10, None, None) == 1
+    assert L.mh_render_adaptive(None, None, 10, 10, C.byref(p), abi.PREC_FP64, None, None, None) == 1
+    f = abi.flare_default()
+    assert L.mh_flare_apply(None, 10, 10, C.byref(f)) == 1
+    assert L.mh_scene_flare(None, C.byref(f)) == 1
+
+
+def test_flare_entry_validates_and_skips_disabled_on_the_host():
+    """The reference's argument checks (src/flare.c:62-68) come first; a
+    disabled flare or intensity <= 0 changes nothing and needs no device."""
+    L = native.hip()
+    img = np.full((4, 6, 3), 200, np.uint8)
+    f = abi.flare_default()
+    assert L.mh_flare_apply(abi.ptr(img), 0, 4, C.byref(f)) == 1
+    assert L.mh_flare_apply(abi.ptr(img), 6, -1, C.byref(f)) == 1
+    assert L.mh_flare_apply(abi.ptr(img), 6, 4, None) == 1
+    for q in (abi.flare_default(enabled=0), abi.FlareParams(enabled=1, intensity=0.0),
+              abi.FlareParams(enabled=1, intensity=-2.0)):
+        assert L.mh_flare_apply(abi.ptr(img), 6, 4, C.byref(q)) == 0
+        assert (img == 200).all()
+
+
+def test_sample_stats_need_no_device():
+    tot, mx = C.c_ulonglong(7), C.c_int(7)
+    assert native.hip().mh_last_sample_stats(C.byref(tot), C.byref(mx)) == 0
+    assert native.hip().mh_last_sample_stats(None, None) == 0
 
 
 def bvh_arrays(hs):
