@@ -216,8 +216,9 @@ def main():
     ap.add_argument("--cpu-scale", type=int, default=3)
     ap.add_argument("--stats-scale", type=int, default=8,
                     help="downscale of the oracle render that counts traversal work")
-    ap.add_argument("--graph", action="store_true",
-                    help="time hipGraph replays instead of clocked direct launches (no per-kernel times)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time clocked direct launches instead of hipGraph replays (per-kernel times from "
+                         "the timed frames themselves)")
     ap.add_argument("--engine", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="frames of the other engine timed for the batched-vs-plain ratio")
@@ -252,8 +253,12 @@ def main():
         log(f"bench: post {post_leg(g, cam, 3)}")
         return
 
-    engines = {"wavefront": dict(engine=abi.ENGINE_WAVEFRONT, pool=args.pool, grid_blocks=args.grid_blocks,
-                                 use_graph=args.graph, kernel_clock=not args.graph),
+    # timed frames replay each batch's bounce sequence as a hipGraph (the
+    # production path); per-kernel times come from one more frame launched
+    # directly with a HIP-event pair around every kernel on the render stream
+    wf = dict(engine=abi.ENGINE_WAVEFRONT, pool=args.pool, grid_blocks=args.grid_blocks)
+    clocked_kw = dict(wf, use_graph=False, kernel_clock=True)
+    engines = {"wavefront": clocked_kw if args.no_graph else dict(wf, use_graph=True),
                "megakernel": dict(engine=abi.ENGINE_MEGAKERNEL)}
     main_kw = engines[args.engine]
     for i in range(args.warmup):
@@ -267,16 +272,25 @@ def main():
         t = g.last_timing()
         kernel_ms.append(t.kernel_ms)
         rays += t.rays
-        for k, (ms, n) in g.last_kernel_times().items():
-            a = ktimes.setdefault(k, [0.0, 0])
-            a[0] += ms
-            a[1] += n
+        if args.no_graph:
+            for k, (ms, n) in g.last_kernel_times().items():
+                a = ktimes.setdefault(k, [0.0, 0])
+                a[0] += ms
+                a[1] += n
         log(f"bench: step {i + 1}/{args.steps} {args.engine} {t.kernel_ms:.1f} ms")
     wall = time.perf_counter() - t_start
     ms_step = wall / args.steps * 1e3
     samples = W * H * SPP
     value = samples * args.steps / wall / 1e6
     mean_kernel_ms = sum(kernel_ms) / len(kernel_ms)
+    k_frames = args.steps
+    if args.engine == "wavefront" and not args.no_graph:
+        # the same kernels launched directly, clocked (not part of value)
+        g.render(cam, W, H, SPP, DEPTH, fetch=False, sample_offset=args.steps * SPP, **clocked_kw)
+        for k, (ms, n) in g.last_kernel_times().items():
+            ktimes[k] = [ms, n]
+        k_frames = 1
+        log(f"bench: clocked frame {g.last_timing().kernel_ms:.1f} ms (per-kernel times)")
 
     # the plain GPU kernel on the same frame and sample streams (equal RMSE:
     # both engines evaluate the identical estimator sample for sample)
@@ -316,7 +330,7 @@ def main():
     frame_gbs = frame_bytes(st, st_samples) * scale / (mean_kernel_ms * 1e-3) / 1e9
     sh_ms, sh_n = ktimes.get("shadow", (0.0, 0))
     if sh_n > 0:
-        per_launch = shadow_bytes(st) * scale * args.steps / sh_n
+        per_launch = shadow_bytes(st) * scale * k_frames / sh_n
         avg_s = sh_ms / sh_n * 1e-3
         achieved = per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": "k_wf_shadow", "achieved": round(achieved, 1),
@@ -335,7 +349,7 @@ def main():
                    "grid": args.grid},
         "mrays_per_s": round(rays / (sum(kernel_ms) * 1e-3) / 1e6, 1),
         "frame_algorithmic_gbs": None if frame_gbs is None else round(frame_gbs, 1),
-        "kernel_ms_per_frame": {k: round(v[0] / args.steps, 2) for k, v in ktimes.items() if v[1]},
+        "kernel_ms_per_frame": {k: round(v[0] / k_frames, 2) for k, v in ktimes.items() if v[1]},
         f"{other}_msamples_per_s": None if plain is None else round(plain, 2),
         "batched_vs_plain": None if plain is None else round(
             (value / plain) if args.engine == "wavefront" else (plain / value), 3),
