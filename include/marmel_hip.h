@@ -506,7 +506,7 @@ typedef struct mh_job {
 typedef struct mh_tune {
     int32_t pool;          /* paths in flight per batch (queue capacity); 0 = 2^21;
                               at most 2^26                                           */
-    int32_t grid_blocks;   /* workgroups of each persistent kernel; 0 = 2048             */
+    int32_t grid_blocks;   /* workgroups of each persistent kernel; 0 = 4096             */
     int32_t block;         /* lanes per workgroup: 64, 128 or 256; 0 = 256               */
     int32_t use_graph;     /* replay each batch's bounce sequence as a hipGraph          */
     int32_t autotune;      /* 1 = choose pool / grid_blocks / block / fixed_batches by

@@ -1530,7 +1530,7 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
         if (pool > npix) pool = npix; // one sample per pixel per batch
         rc = ensure_wf(s, pool, (size_t)npix);
         if (rc) return rc;
-        int grid = p->grid_blocks > 0 ? p->grid_blocks : 2048;
+        int grid = p->grid_blocks > 0 ? p->grid_blocks : 4096;  // 1080p terrain: 2048 -> 9 % slower, 8192 -> 0.5 %
         CamView<float> cvw = conv_cam<float>(*cam);
         WfJob job{p->width, p->height, p->spp, p->sample_offset, p->max_depth, npix,
                   (long long)npix * p->spp, pool};
@@ -2021,7 +2021,7 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     for (const auto& v : views) mj.sh_stride = std::max(mj.sh_stride, v.n_lights + 1);
 
     if (t.pool == 0) t.pool = 1 << 21;
-    if (t.grid_blocks == 0) t.grid_blocks = 2048;
+    if (t.grid_blocks == 0) t.grid_blocks = 4096;
     if (t.block == 0) t.block = 256;
     if (t.autotune) {
         std::vector<long long> key = {n_jobs, begin, pix, max_depth};
