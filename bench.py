@@ -38,10 +38,10 @@ sys.path.insert(0, os.path.join(REPO, "marmel-trace_amd"))
 
 HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per k_wf_shadow launch on this workload: rocprofv3 --pmc
-# FETCH_SIZE (419,267.8 KiB, x 2.0) + WRITE_SIZE (73,671.3 KiB, x 1.0) per
+# FETCH_SIZE (298,506.9 KiB, x 2.0) + WRITE_SIZE (47,730.6 KiB, x 1.0) per
 # dispatch, the factors being what the same counters read for a 1 GiB copy.
-# Re-measure when the shadow kernel changes.
-SHADOW_TRAFFIC_PER_LAUNCH = 934.1e6
+# Re-measure when the shadow kernel or its launch changes.
+SHADOW_TRAFFIC_PER_LAUNCH = 660.2e6
 W, H, SPP, DEPTH = 1920, 1080, 64, 8
 GRID = 1024
 
