@@ -142,12 +142,17 @@ def batch_leg():
     from marmel_amd.workloads import batch32_scenes, batch32_spec
     t0 = time.time()
     spec = batch32_spec()
-    hss = batch32_scenes(spec)
-    gss = [GpuScene(hs.soa()) for hs in hss]
+    hss = batch32_scenes(spec, host_bvh=False)
+    t_gen = time.time() - t0
+    t1 = time.time()
+    gss = [GpuScene(hs.soa(bvh=False), gpu_bvh=True) for hs in hss]  # trees built on the device
+    t_up = time.time() - t1
+    dev_build_ms = sum(g.bvh_info().build_ms for g in gss)
     jobs = [dict(scene=i, cam=hs.camera(1.0), width=r, height=r, spp=spp, depth=dp)
             for i, (hs, (_, _, _, r, spp, dp)) in enumerate(zip(hss, spec))]
     samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
-    log(f"bench: batch32 scenes built in {time.time() - t0:.1f}s, {samples / 1e6:.1f} Msamples")
+    log(f"bench: batch32 scenes generated in {t_gen:.1f}s, uploaded with device-built trees in {t_up:.2f}s "
+        f"(device builds {dev_build_ms:.0f} ms), {samples / 1e6:.1f} Msamples")
     b = Batch()
     b.render(gss, jobs, autotune=True, fetch=False)  # tunes once, cached on the handle
     tune = b.last_tune()
@@ -174,7 +179,8 @@ def batch_leg():
                         "256^2-1024^2, 4-256 spp, depth 4-10",
             "msamples": round(samples / 1e6, 1), "batched_msamples_per_s": round(batched, 1),
             "plain_msamples_per_s": round(plain, 1), "single_scene_wavefront_msamples_per_s": round(single_wf, 1),
-            "batched_vs_plain": round(batched / plain, 3), "tune": tune}
+            "batched_vs_plain": round(batched / plain, 3), "tune": tune,
+            "device_bvh_build_ms": round(dev_build_ms, 1)}
 
 
 def post_leg(g, cam, reps):

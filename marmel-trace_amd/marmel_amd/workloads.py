@@ -38,12 +38,15 @@ def batch32_spec(seed=3, n=32):
     return out
 
 
-def batch32_scenes(spec):
+def batch32_scenes(spec, host_bvh=True):
+    """The 32 scenes; host_bvh=False leaves the trees to the device build
+    (GpuScene(hs.soa(bvh=False), gpu_bvh=True))."""
     scenes = []
     for sseed, sub, ns, res, spp, depth in spec:
         hs = HostScene()
         hs.generate(native.GEN_ICOSPHERE, seed=sseed, subdiv=sub, n_spheres=ns)
-        hs.build_bvh()
+        if host_bvh:
+            hs.build_bvh()
         scenes.append(hs)
     return scenes
 
