@@ -7,6 +7,29 @@
 
 namespace mh {
 
+// Path index within a sample -> pixel (y * w + x), visiting 8x8 tiles row by
+// row and each tile row by row (edge tiles narrower / shorter). A bijection
+// on [0, w*h): the estimator per (pixel, sample) is unchanged, but a wave of
+// 64 consecutive paths covers one tile, so its camera rays and their first
+// shadow rays stay coherent in the tree.
+__device__ __forceinline__ uint32_t tile_pixel(uint32_t i, uint32_t w, uint32_t h)
+{
+    uint32_t ty = i / (8u * w), off = i - ty * 8u * w;
+    uint32_t th = h - ty * 8u < 8u ? h - ty * 8u : 8u;
+    uint32_t nfull = w / 8u, tx = off / (8u * th);
+    uint32_t px, py;
+    if (tx < nfull) {
+        uint32_t loc = off - tx * 8u * th;
+        px = tx * 8u + (loc & 7u);
+        py = ty * 8u + (loc >> 3);
+    } else {
+        uint32_t pw = w - nfull * 8u, loc = off - nfull * 8u * th;
+        px = nfull * 8u + loc % pw;
+        py = ty * 8u + loc / pw;
+    }
+    return py * w + px;
+}
+
 
 __device__ __forceinline__ void count_rays(unsigned long long* ctr, int nq)
 {
@@ -80,7 +103,8 @@ __global__ __launch_bounds__(256) void k_whitted(SceneView<R> sc, CamView<R> cam
     int nq = 0, ovf = 0;
     int g = (int)(msqrt((double)spp) + 0.5);
     bool strat = g > 0 && g * g == spp;
-    for (int pix = lane; pix < w * h; pix += nlanes) {
+    for (int i = lane; i < w * h; i += nlanes) {
+        int pix = (int)tile_pixel((uint32_t)i, (uint32_t)w, (uint32_t)h);
         int x = pix % w, y = pix / w;
         V3<R> acc = zero3<R>();
         for (int si = 0; si < spp; ++si) {
@@ -139,7 +163,7 @@ __global__ __launch_bounds__(256) void k_adapt_sample(SceneView<R> sc, CamView<R
     size_t npix = (size_t)w * (size_t)h;
     int items = count ? *count : (int)npix;
     for (int i = lane; i < items; i += nlanes) {
-        int p = count ? ab.list[i] : i;
+        int p = count ? ab.list[i] : (int)tile_pixel((uint32_t)i, (uint32_t)w, (uint32_t)h);
         int x = p % w, y = p / w;
         int n = ab.ns[p];
         int b = nmax - n < n0 ? nmax - n : n0;
@@ -188,19 +212,23 @@ __global__ __launch_bounds__(256) void k_adapt_sample(SceneView<R> sc, CamView<R
 // shared through LDS), as a counter hit by every wave serialises.
 // Follows reference src/render.c:1330-1361.
 template <typename R>
-__global__ __launch_bounds__(256) void k_adapt_flag(AdaptBuf ab, int npix, int nmax, double tau, int* count)
+__global__ __launch_bounds__(256) void k_adapt_flag(AdaptBuf ab, int w, int h, int nmax, double tau, int* count)
 {
     __shared__ int wave_n[4][4];
     __shared__ int base_at;
+    int npix = w * h;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int base = blockIdx.x * 1024; base < npix; base += gridDim.x * 1024) {
         int flag[4];
         unsigned long long m[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            int p = base + r * 256 + threadIdx.x;
+            // pixels in 8x8-tile order: the list, and so each refinement
+            // pass's waves, keep neighbouring pixels together
+            int li = base + r * 256 + threadIdx.x;
+            int p = li < npix ? (int)tile_pixel((uint32_t)li, (uint32_t)w, (uint32_t)h) : npix;
             int f = 0;
-            if (p < npix) {
+            if (li < npix) {
                 int n = ab.ns[p];
                 if (n < nmax) {
                     if (n < 2) {
@@ -235,7 +263,11 @@ __global__ __launch_bounds__(256) void k_adapt_flag(AdaptBuf ab, int npix, int n
             for (int rr = 0; rr < r; ++rr)
                 for (int q = 0; q < 4; ++q) before += wave_n[rr][q];
             for (int q = 0; q < wave; ++q) before += wave_n[r][q];
-            if (flag[r]) ab.list[at + before + __popcll(m[r] & ((1ull << lane) - 1ull))] = base + r * 256 + threadIdx.x;
+            if (flag[r]) {
+                int li = base + r * 256 + threadIdx.x;
+                ab.list[at + before + __popcll(m[r] & ((1ull << lane) - 1ull))] =
+                    (int)tile_pixel((uint32_t)li, (uint32_t)w, (uint32_t)h);
+            }
         }
         __syncthreads();  // wave_n and base_at are reused by the next round
     }
@@ -375,7 +407,7 @@ hipError_t launch_whitted_adaptive(const SceneView<R>& sc, const CamView<R>& cam
     hipLaunchKernelGGL(k_adapt_sample<R>, dim3(nblocks), dim3(256), 0, st, sc, cam, w, h, n0, nmax, depth, ab,
                        (const int*)nullptr, out, (WItem<R>*)stack, cap);
     for (int pass = 1; pass < max_passes; ++pass) {
-        hipLaunchKernelGGL(k_adapt_flag<R>, dim3(fblocks), dim3(256), 0, st, ab, npix, nmax, tau, ab.count + pass);
+        hipLaunchKernelGGL(k_adapt_flag<R>, dim3(fblocks), dim3(256), 0, st, ab, w, h, nmax, tau, ab.count + pass);
         hipLaunchKernelGGL(k_adapt_sample<R>, dim3(nblocks), dim3(256), 0, st, sc, cam, w, h, n0, nmax, depth, ab,
                            (const int*)(ab.count + pass), out, (WItem<R>*)stack, cap);
     }

@@ -123,28 +123,6 @@ __device__ __forceinline__ const SceneView<float>& slot_scene(const SceneView<fl
     return MULTI ? mj.scenes[mj.jobs[st.jid[slot]].scene] : sc;
 }
 
-// Path index within a sample -> pixel (y * w + x), visiting 8x8 tiles row by
-// row and each tile row by row (edge tiles narrower / shorter). A bijection
-// on [0, w*h): the estimator per (pixel, sample) is unchanged, but a wave of
-// 64 consecutive paths covers one tile, so its camera rays and their first
-// shadow rays stay coherent in the tree.
-__device__ __forceinline__ uint32_t tile_pixel(uint32_t i, uint32_t w, uint32_t h)
-{
-    uint32_t ty = i / (8u * w), off = i - ty * 8u * w;
-    uint32_t th = h - ty * 8u < 8u ? h - ty * 8u : 8u;
-    uint32_t nfull = w / 8u, tx = off / (8u * th);
-    uint32_t px, py;
-    if (tx < nfull) {
-        uint32_t loc = off - tx * 8u * th;
-        px = tx * 8u + (loc & 7u);
-        py = ty * 8u + (loc >> 3);
-    } else {
-        uint32_t pw = w - nfull * 8u, loc = off - nfull * 8u * th;
-        px = nfull * 8u + loc % pw;
-        py = ty * 8u + loc / pw;
-    }
-    return py * w + px;
-}
 
 // Start global path g (camera ray, seed, unit throughput) in path slot `slot`.
 template <bool MULTI>
