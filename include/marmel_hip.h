@@ -461,6 +461,16 @@ int mh_scene_set_materials(struct mh_scene *s, const mh_material *mats, int n);
 int mh_scene_set_sky(struct mh_scene *s, const mh_sky *sky);
 int mh_scene_set_fog(struct mh_scene *s, const mh_fog *fog);
 
+/* The fp32 path tracer as the adjoint computes it (same code and rounding),
+ * also recording every sample's radiance sum on the scene (16 bytes per
+ * sample, at most 16 GiB). A following mh_adjoint with the same camera and
+ * params (size, spp, sample_offset, max_depth) then replays each path once
+ * instead of twice. Any material / sky / fog update drops the record; the
+ * next recording render replaces it. No reference counterpart. */
+int mh_render_record(struct mh_scene *s, const mh_camera *cam,
+                     const mh_render_params *p, unsigned char *rgb_out,
+                     float *linear_out);
+
 /* Path-replay adjoint of the fp32 path tracer (no reference counterpart).
  * d_image: dLoss/d(linear pixel mean), w*h*3 floats, top-down like the
  * render. grad receives n_materials*8 + 3 floats: for each material

@@ -174,6 +174,7 @@ struct AdjParams {
     double* grad;        // n_mats*8 + 3 (device, accumulated)
     int n_mats;
     int lds_mats;        // materials whose slots live in LDS (the rest go straight to grad)
+    const float4* rec;   // recorded per-sample sums (k_render_record), or null
 };
 
 #define MH_ADJ_LDS_MATS 256
@@ -317,8 +318,10 @@ struct AdjHook : NoHook {
 };
 
 // One thread per pixel (16x16 tiles), samples in order; each sample is
-// traced twice (primal, then replay with gradients).
-template <bool FOG>
+// traced twice (primal, then replay with gradients), or, with REC, once: the
+// primal sum comes from the record of a recording render (k_render_record,
+// the same code, so the same bits).
+template <bool FOG, bool REC>
 __global__ __launch_bounds__(256) void k_adjoint(SceneView<float> sc, CamView<float> cam, int w, int h, int spp,
                                                   int s0, int depth, int tiles_x, int ntiles, AdjParams ap)
 {
@@ -358,9 +361,15 @@ __global__ __launch_bounds__(256) void k_adjoint(SceneView<float> sc, CamView<fl
                 }
                 uint32_t seed = hash3((uint32_t)x, (uint32_t)y, s);
                 // pass 1: the raw (unclamped) sum of the path's terms
-                SumHook sum;
-                (void)pt_radiance_h<FOG>(sc, ts, o, d, depth, seed, &nq, sum);
-                V3<float> L = sum.L;
+                V3<float> L;
+                if (REC) {
+                    float4 r = ap.rec[(size_t)si * (size_t)w * (size_t)h + (size_t)y * (size_t)w + (size_t)x];
+                    L = mk(r.x, r.y, r.z);
+                } else {
+                    SumHook sum;
+                    (void)pt_radiance_h<FOG>(sc, ts, o, d, depth, seed, &nq, sum);
+                    L = sum.L;
+                }
                 // the final clamp of a negative channel to 0 (and of a
                 // non-finite sum) has zero derivative
                 if (!finite3(L)) continue;
@@ -380,6 +389,65 @@ __global__ __launch_bounds__(256) void k_adjoint(SceneView<float> sc, CamView<fl
         float v = gl[MH_ADJ_LDS_MATS * G_PER_MAT + threadIdx.x];
         if (v != 0.f) atomicAdd(&ap.grad[(size_t)ap.n_mats * G_PER_MAT + threadIdx.x], (double)v);
     }
+}
+
+// Recording render: the fp32 path tracer in the adjoint's code (same
+// flags, same hooks as its first pass), writing each sample's raw term sum
+// to rec (one float4 per sample and pixel, sample-major) and the pixel
+// means to out.
+template <bool FOG>
+__global__ __launch_bounds__(256) void k_render_record(SceneView<float> sc, CamView<float> cam, int w, int h, int spp,
+                                                        int s0, int depth, int tiles_x, int ntiles, float4* rec,
+                                                        LaunchOut out)
+{
+    MH_STACK_PRIV(ts)
+    int tile = blockIdx.x;
+    int nq = 0;
+    if (tile < ntiles) {
+        int tx = tile % tiles_x, ty = tile / tiles_x;
+        int x = tx * 16 + (threadIdx.x & 15);
+        int y = ty * 16 + (threadIdx.x >> 4);
+        if (x < w && y < h) {
+            V3<float> acc = zero3<float>();
+            for (int si = 0; si < spp; ++si) {
+                uint32_t s = (uint32_t)(s0 + si);
+                float jx = rand01<float>((uint32_t)x, (uint32_t)y, s * 2u + 0u);
+                float jy = rand01<float>((uint32_t)x, (uint32_t)y, s * 2u + 1u);
+                float l1 = rand01<float>((uint32_t)x, (uint32_t)y, s * 2u + 100u);
+                float l2 = rand01<float>((uint32_t)x, (uint32_t)y, s * 2u + 101u);
+                float uu = ((float)x + jx) / (float)w;
+                float vv = 1.f - ((float)y + jy) / (float)h;
+                V3<float> o, d;
+                camera_ray(cam, uu, vv, l1, l2, &o, &d);
+                if (cam.dome > 1e-4f && sc.water_material >= 0) {
+                    V3<float> pd = o + d * cam.dome;
+                    if (pd.y < water_height(sc, pd.x, pd.z)) o = pd - mk(0.f, 1e-3f, 0.f);
+                }
+                SumHook sum;
+                acc = acc + pt_radiance_h<FOG>(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq, sum);
+                rec[(size_t)si * (size_t)w * (size_t)h + (size_t)y * (size_t)w + (size_t)x] =
+                    make_float4(sum.L.x, sum.L.y, sum.L.z, 0.f);
+            }
+            acc = acc * (1.f / (float)spp);
+            size_t p = ((size_t)y * (size_t)w + (size_t)x) * 3;
+            out.linear[p] = acc.x;
+            out.linear[p + 1] = acc.y;
+            out.linear[p + 2] = acc.z;
+            out.rgb[p] = to_byte(acc.x);
+            out.rgb[p + 1] = to_byte(acc.y);
+            out.rgb[p + 2] = to_byte(acc.z);
+        }
+    }
+}
+
+hipError_t launch_render_record(const SceneView<float>& sc, const CamView<float>& cam, int w, int h, int spp, int s0,
+                                int depth, float4* rec, const LaunchOut& out, hipStream_t st)
+{
+    int tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16;
+    int ntiles = tiles_x * tiles_y;
+    auto kern = sc.fog_kernel ? k_render_record<true> : k_render_record<false>;
+    hipLaunchKernelGGL(kern, dim3(ntiles), dim3(256), 0, st, sc, cam, w, h, spp, s0, depth, tiles_x, ntiles, rec, out);
+    return hipGetLastError();
 }
 
 // Frozen-sampling render: sampling decisions from the material table `f`.
@@ -444,12 +512,13 @@ hipError_t launch_render_frozen(const SceneView<float>& sc, const CamView<float>
 }
 
 hipError_t launch_adjoint(const SceneView<float>& sc, const CamView<float>& cam, int w, int h, int spp, int s0,
-                          int depth, const float* dimg, double* grad, hipStream_t st)
+                          int depth, const float* dimg, double* grad, const float4* rec, hipStream_t st)
 {
     int tiles_x = (w + 15) / 16, tiles_y = (h + 15) / 16;
     int ntiles = tiles_x * tiles_y;
-    AdjParams ap{dimg, grad, sc.n_mats, sc.n_mats < MH_ADJ_LDS_MATS ? sc.n_mats : MH_ADJ_LDS_MATS};
-    auto kern = sc.fog_kernel ? k_adjoint<true> : k_adjoint<false>;
+    AdjParams ap{dimg, grad, sc.n_mats, sc.n_mats < MH_ADJ_LDS_MATS ? sc.n_mats : MH_ADJ_LDS_MATS, rec};
+    auto kern = rec ? (sc.fog_kernel ? k_adjoint<true, true> : k_adjoint<false, true>)
+                    : (sc.fog_kernel ? k_adjoint<true, false> : k_adjoint<false, false>);
     hipLaunchKernelGGL(kern, dim3(ntiles), dim3(256), 0, st, sc, cam, w, h, spp, s0, depth, tiles_x, ntiles, ap);
     return hipGetLastError();
 }

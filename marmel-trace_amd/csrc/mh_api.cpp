@@ -252,6 +252,13 @@ struct mh_scene {
     // flare scratch for rows too wide for LDS
     void* flare_mem = nullptr;
     size_t flare_bytes = 0;
+    // per-sample sums of the last recording render and the view they belong
+    // to (dropped by any material / sky / fog update)
+    float4* rec = nullptr;
+    size_t rec_n = 0;
+    bool rec_valid = false;
+    mh_camera rec_cam{};
+    int rec_key[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -1291,14 +1298,18 @@ void mh_scene_destroy(mh_scene* s)
     if (s->d_frozen) (void)hipFree(s->d_frozen);
     if (s->ad_mem) (void)hipFree(s->ad_mem);
     if (s->flare_mem) (void)hipFree(s->flare_mem);
+    if (s->rec) (void)hipFree(s->rec);
     for (auto& e : s->ev)
         if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
+// Scene parameters changed: the captured wavefront graph and the recorded
+// per-sample sums no longer describe the scene.
 static void drop_scene_graph(mh_scene* s)
 {
+    s->rec_valid = false;
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
     if (s->graph_def) (void)hipGraphDestroy(s->graph_def);
     s->graph = nullptr;
@@ -1423,8 +1434,10 @@ int mh_adjoint(mh_scene* s, const mh_camera* cam, const mh_render_params* p, con
     HIP_TRY(hipMemcpyAsync(s->d_dimg, d_image, n3 * sizeof(float), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_grad, 0, ng * sizeof(double), s->stream));
     HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+    int key[5] = {p->width, p->height, p->spp, p->sample_offset, p->max_depth};
+    bool use_rec = s->rec_valid && !memcmp(key, s->rec_key, sizeof key) && !memcmp(cam, &s->rec_cam, sizeof *cam);
     HIP_TRY(launch_adjoint(s->v32, conv_cam<float>(*cam), p->width, p->height, p->spp, p->sample_offset, p->max_depth,
-                           s->d_dimg, s->d_grad, s->stream));
+                           s->d_dimg, s->d_grad, use_rec ? s->rec : nullptr, s->stream));
     HIP_TRY(hipEventRecord(s->ev[1], s->stream));
     std::vector<double> g(ng);
     HIP_TRY(hipMemcpyAsync(g.data(), s->d_grad, ng * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -1434,6 +1447,46 @@ int mh_adjoint(mh_scene* s, const mh_camera* cam, const mh_render_params* p, con
     g_timing.kernel_ms = g_timing.total_ms = ms_between(s->ev[0], s->ev[1]);
     g_timing.samples = (int64_t)p->width * p->height * p->spp;
     g_seconds = g_timing.kernel_ms * 1e-3;
+    return 0;
+}
+
+int mh_render_record(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsigned char* rgb_out,
+                     float* linear_out)
+{
+    if (!s || !cam || !p) return fail(1, "NULL argument");
+    if (p->width <= 0 || p->height <= 0) return fail(2, "non-positive image size %dx%d", p->width, p->height);
+    if (p->spp < 1 || p->max_depth < 0 || p->sample_offset < 0)
+        return fail(3, "bad spp %d / depth %d / offset %d", p->spp, p->max_depth, p->sample_offset);
+    if ((long long)p->width * p->height > (1LL << 28)) return fail(2, "image too large");
+    if (p->mode != MH_MODE_PATHTRACE) return fail(4, "the recording render is the path tracer");
+    if (p->precision != MH_PREC_FP32) return fail(4, "the recording render computes in fp32 only");
+    size_t nrec = (size_t)p->width * p->height * (size_t)p->spp;
+    if (nrec * sizeof(float4) > ((size_t)16 << 30)) return fail(5, "record of %zu samples exceeds 16 GiB", nrec);
+    s->rec_valid = false;
+    int rc = grow(&s->rec, &s->rec_n, nrec, sizeof(float4));
+    if (rc) return rc;
+    rc = ensure_fb(s, p->width, p->height);
+    if (rc) return rc;
+    LaunchOut lo{s->fb_linear, s->fb_rgb, s->d_rays, s->d_overflow};
+    HIP_TRY(hipMemsetAsync(s->d_rays, 0, sizeof(unsigned long long), s->stream));
+    HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+    HIP_TRY(launch_render_record(s->v32, conv_cam<float>(*cam), p->width, p->height, p->spp, p->sample_offset,
+                                 p->max_depth, s->rec, lo, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+    size_t n3 = (size_t)p->width * p->height * 3;
+    if (rgb_out) HIP_TRY(hipMemcpyAsync(rgb_out, s->fb_rgb, n3, hipMemcpyDeviceToHost, s->stream));
+    if (linear_out) HIP_TRY(hipMemcpyAsync(linear_out, s->fb_linear, n3 * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    int key[5] = {p->width, p->height, p->spp, p->sample_offset, p->max_depth};
+    memcpy(s->rec_key, key, sizeof key);
+    s->rec_cam = *cam;
+    s->rec_valid = true;
+    memset(&g_timing, 0, sizeof g_timing);
+    g_timing.kernel_ms = g_timing.total_ms = ms_between(s->ev[0], s->ev[1]);
+    g_timing.samples = (int64_t)p->width * p->height * p->spp;
+    g_seconds = g_timing.kernel_ms * 1e-3;
+    g_total_samples = (unsigned long long)g_timing.samples;
+    g_max_samples = p->spp;
     return 0;
 }
 

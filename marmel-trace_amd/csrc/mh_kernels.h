@@ -25,8 +25,14 @@ hipError_t launch_pathtrace(const SceneView<R>& sc, const CamView<R>& cam, int w
 // Path-replay adjoint (fp32): accumulates dLoss/dtheta into grad
 // (n_mats * 8 + 3 doubles: per material albedo rgb, roughness, metallic,
 // emissive rgb; then sun colour rgb) from dimg = dLoss/d(pixel mean).
+// rec: the per-sample sums of a recording render of the same view and
+// parameters (each path is then replayed once), or null (traced twice).
 hipError_t launch_adjoint(const SceneView<float>& sc, const CamView<float>& cam, int w, int h, int spp, int s0,
-                          int depth, const float* dimg, double* grad, hipStream_t st);
+                          int depth, const float* dimg, double* grad, const float4* rec, hipStream_t st);
+// fp32 path tracer in the adjoint's code that records each sample's raw sum
+// (one float4 per sample and pixel, sample-major) for launch_adjoint.
+hipError_t launch_render_record(const SceneView<float>& sc, const CamView<float>& cam, int w, int h, int spp, int s0,
+                                int depth, float4* rec, const LaunchOut& out, hipStream_t st);
 
 // fp32 path tracer whose sampling decisions use the device material table
 // `frozen` (DMat<float>[n_mats]) and BSDF values the scene's materials.

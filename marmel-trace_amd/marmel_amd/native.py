@@ -108,6 +108,7 @@ def hip():
     _sig(L, "mh_scene_set_sky", C.c_int, [P, C.POINTER(abi.Sky)])
     _sig(L, "mh_scene_set_fog", C.c_int, [P, C.POINTER(abi.Fog)])
     _sig(L, "mh_adjoint", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P, P])
+    _sig(L, "mh_render_record", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P, P])
     _sig(L, "mh_render_frozen", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P,
                                           C.c_int, P, P])
     _sig(L, "mh_batch_create", C.c_int, [C.POINTER(P)])

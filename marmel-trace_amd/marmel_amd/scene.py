@@ -347,13 +347,16 @@ class GpuScene:
     def set_materials(self, mats):
         """Replace every material (numpy MATERIAL_DTYPE array, same count)."""
         mats = np.ascontiguousarray(mats, dtype=abi.MATERIAL_DTYPE)
+        self.updates = getattr(self, "updates", 0) + 1
         self._check(self._L.mh_scene_set_materials(self._h, abi.ptr(mats), len(mats)))
 
     def set_sky(self, sky):
+        self.updates = getattr(self, "updates", 0) + 1
         self._check(self._L.mh_scene_set_sky(self._h, C.byref(sky)))
 
     def set_fog(self, fog):
         """Replace the uploaded scene's fog (abi.Fog)."""
+        self.updates = getattr(self, "updates", 0) + 1
         self._check(self._L.mh_scene_set_fog(self._h, C.byref(fog)))
 
     def render_frozen(self, cam, width, height, spp, depth, frozen_mats, sample_offset=0):
@@ -366,6 +369,17 @@ class GpuScene:
         self._check(self._L.mh_render_frozen(self._h, C.byref(cam), C.byref(p), abi.ptr(f), len(f), None,
                                              abi.ptr(lin)))
         return lin
+
+    def render_record(self, cam, width, height, spp, depth, sample_offset=0):
+        """fp32 path-traced render that records each sample for the next
+        adjoint of the same view: returns (rgb, linear)."""
+        p = abi.RenderParams(width=width, height=height, spp=spp, max_depth=depth,
+                             mode=abi.MODE_PATHTRACE, precision=abi.PREC_FP32,
+                             sample_offset=sample_offset)
+        rgb = np.zeros((height, width, 3), np.uint8)
+        lin = np.zeros((height, width, 3), np.float32)
+        self._check(self._L.mh_render_record(self._h, C.byref(cam), C.byref(p), abi.ptr(rgb), abi.ptr(lin)))
+        return rgb, lin
 
     def adjoint(self, cam, width, height, spp, depth, d_image, sample_offset=0):
         """dLoss/dtheta for dLoss/d(pixel mean) d_image (h, w, 3): returns

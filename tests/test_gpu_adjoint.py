@@ -74,6 +74,58 @@ def test_gpu_adjoint_is_linear_and_deterministic(mscene):
     assert np.allclose(g3, 2.0 * g1, rtol=1e-4, atol=1e-8)
 
 
+def test_recorded_forward_gives_the_same_adjoint(mscene):
+    """mh_render_record stores each sample's sum; the adjoint that follows
+    replays each path once and returns the two-pass adjoint (up to the order
+    of float atomic sums), also under fog; an update drops the record (the
+    next adjoint traces twice again, same result), and a record of another
+    view is not used."""
+    hs, gs, o, mats, sky, hold = mscene
+    gs.set_materials(mats)
+    gs.set_sky(sky)
+    w, h, spp, depth = 96, 64, 4, 4
+    cam = hs.camera(w / h)
+    rng = np.random.default_rng(8)
+    a = rng.normal(size=(h, w, 3)).astype(np.float32)
+    ref = flat(*gs.adjoint(cam, w, h, spp, depth, a))
+    t2 = GpuScene.last_timing().kernel_ms
+    rgb, lin = gs.render_record(cam, w, h, spp, depth)
+    # the adjoint's code: the frozen-sampling render at the current point,
+    # bit for bit; the megakernel (built with FMA contraction) agrees up to
+    # fp32 rounding flipping an occasional path decision
+    assert np.array_equal(lin, gs.render_frozen(cam, w, h, spp, depth, mats))
+    _, lin_m = gs.render(cam, w, h, spp, depth)
+    print("record vs megakernel: mean |d|", np.abs(lin - lin_m).mean())
+    assert np.abs(lin - lin_m).mean() < 1e-3
+    g1 = flat(*gs.adjoint(cam, w, h, spp, depth, a))
+    t1 = GpuScene.last_timing().kernel_ms
+    print("adjoint with record %.2f ms, without %.2f ms" % (t1, t2))
+
+    def same(g, r):
+        # float atomic sums in a different order: equal to float precision
+        rel = np.linalg.norm(g - r) / np.linalg.norm(r)
+        big = np.abs(r) > 1e-3 * np.abs(r).max()
+        worst = np.max(np.abs(g - r)[big] / np.abs(r)[big])
+        print("  relative L2", rel, "worst entry above 1e-3 of max", worst)
+        return rel < 1e-5 and worst < 1e-4
+    assert same(g1, ref)
+    assert t1 < t2
+    gs.set_sky(sky)  # any update drops the record
+    g2 = flat(*gs.adjoint(cam, w, h, spp, depth, a))
+    assert same(g2, ref)
+    gs.render_record(cam, w, h, spp, depth, sample_offset=4)  # another sample range
+    g3 = flat(*gs.adjoint(cam, w, h, spp, depth, a))
+    assert same(g3, ref)
+    import scenes
+    gs.set_fog(scenes.FOG_SETS["ashenvale"])
+    ref_f = flat(*gs.adjoint(cam, w, h, spp, depth, a))
+    gs.render_record(cam, w, h, spp, depth)
+    g4 = flat(*gs.adjoint(cam, w, h, spp, depth, a))
+    assert same(g4, ref_f)
+    assert not same(ref_f, ref)
+    gs.set_fog(abi.fog_default())
+
+
 def c4_setup(mscene):
     hs, gs, o, mats, sky, hold = mscene
     w, h, spp, depth = 512, 512, 32, 4
