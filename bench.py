@@ -174,6 +174,16 @@ def batch_leg():
     b.render(gss, jobs, autotune=True, fetch=False)
     batched = samples / (time.perf_counter() - t1) / 1e6
     log(f"bench: batch32 batched {batched:.1f} Msamples/s, tune {tune}")
+    # each batched engine on its own (the tuner keeps the faster)
+    eng = {}
+    for name, e in (("wavefront", abi.BATCH_WAVEFRONT), ("megakernel", abi.BATCH_MEGAKERNEL)):
+        kw = dict(pool=tune["pool"], grid_blocks=tune["grid_blocks"], block=tune["block"],
+                  bvh_width=tune["bvh_width"], fixed_batches=bool(tune["fixed_batches"]), engine=e)
+        b.render(gss, jobs, fetch=False, **kw)
+        t1 = time.perf_counter()
+        b.render(gss, jobs, fetch=False, **kw)
+        eng[name] = round(samples / (time.perf_counter() - t1) / 1e6, 1)
+    log(f"bench: batch32 batched engines {eng} Msamples/s")
     for j in jobs[:2]:  # warm both single-job engines
         gss[j["scene"]].render(j["cam"], j["width"], j["height"], 1, j["depth"], fetch=False)
     t1 = time.perf_counter()
@@ -194,6 +204,7 @@ def batch_leg():
             "msamples": round(samples / 1e6, 1), "batched_msamples_per_s": round(batched, 1),
             "plain_msamples_per_s": round(plain, 1), "single_scene_wavefront_msamples_per_s": round(single_wf, 1),
             "batched_vs_plain": round(batched / plain, 3), "tune": tune,
+            "batched_engines_msamples_per_s": eng,
             "device_bvh_build_ms": round(dev_build_ms, 1)}
 
 

@@ -528,7 +528,12 @@ typedef struct mh_tune {
     int32_t fixed_batches; /* 0 (default): path regeneration, every bounce step refills
                               the slots of ended paths with new ones; 1: batches of
                               `pool` paths run to completion one after another     */
+    int32_t engine;        /* MH_BATCH_WAVEFRONT (0, default) or MH_BATCH_MEGAKERNEL:
+                              one launch of (job, 16x16 tile, 4 samples) units, each
+                              lane tracing whole paths; autotune times both       */
 } mh_tune;
+
+enum { MH_BATCH_WAVEFRONT = 0, MH_BATCH_MEGAKERNEL = 1 };
 
 struct mh_batch;
 typedef struct mh_batch mh_batch;

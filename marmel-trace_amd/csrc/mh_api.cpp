@@ -721,6 +721,8 @@ struct mh_batch {
     hipGraphExec_t sgraph = nullptr;  // two captured regeneration steps
     hipGraph_t sgraph_def = nullptr;
     std::vector<unsigned char> sgraph_key;
+    BUnit* units = nullptr;         // batched megakernel work units (device)
+    size_t units_n = 0;
 };
 
 namespace {
@@ -1977,7 +1979,34 @@ void mh_batch_destroy(mh_batch* b)
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->units) (void)hipFree(b->units);
     delete b;
+}
+
+// Batched megakernel units, chunk-major (the first samples of every job
+// first), so a prefix of the list is a sample of the whole workload.
+static void make_units(const std::vector<BJob>& bj, int chunk, std::vector<BUnit>* out)
+{
+    out->clear();
+    int max_chunks = 0;
+    for (const auto& j : bj) max_chunks = std::max(max_chunks, (j.spp + chunk - 1) / chunk);
+    for (int c = 0; c < max_chunks; ++c)
+        for (size_t ji = 0; ji < bj.size(); ++ji) {
+            const BJob& j = bj[ji];
+            if (c * chunk >= j.spp) continue;
+            int tiles = ((j.w + 15) / 16) * ((j.h + 15) / 16);
+            int se = std::min(j.spp, (c + 1) * chunk);
+            for (int t = 0; t < tiles; ++t) out->push_back(BUnit{(int)ji, t, c * chunk, se});
+        }
+}
+
+// The first `n` units (all when n < 0) on the batched megakernel.
+static int mega_run(mh_batch* b, const WfMulti& mj, int feat, int n)
+{
+    int total = (int)b->units_n;
+    if (n < 0 || n > total) n = total;
+    HIP_TRY(batch_mega(mj, b->units, n, (feat & WF_FOG) != 0, b->d_rays, b->stream));
+    return 0;
 }
 
 int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh_job* jobs, int n_jobs,
@@ -1995,6 +2024,8 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     if (t.pool < 0 || t.pool > (1 << 26) || t.grid_blocks < 0 ||
         (t.block != 0 && t.block != 64 && t.block != 128 && t.block != 256))
         return fail(4, "bad launch settings (pool %d, grid %d, block %d)", t.pool, t.grid_blocks, t.block);
+    if (t.engine != MH_BATCH_WAVEFRONT && t.engine != MH_BATCH_MEGAKERNEL)
+        return fail(4, "unknown batch engine %d", t.engine);
 
     std::vector<SceneView<float>> views((size_t)n_scenes);
     int feat = 0;
@@ -2076,6 +2107,23 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     if (t.pool == 0) t.pool = 1 << 21;
     if (t.grid_blocks == 0) t.grid_blocks = 4096;
     if (t.block == 0) t.block = 256;
+    // batched megakernel units (4 samples of one 16x16 tile of one job)
+    bool want_mega = t.engine == MH_BATCH_MEGAKERNEL || t.autotune;
+    if (want_mega) {
+        std::vector<BUnit> units;
+        make_units(bj, 4, &units);
+        if (units.size() > (size_t)INT_MAX) return fail(2, "batch has too many work units");
+        if (units.size() > b->units_n) {
+            if (b->units) (void)hipFree(b->units);
+            b->units = nullptr;
+            b->units_n = 0;
+            HIP_TRY(hipMalloc(&b->units, units.size() * sizeof(BUnit)));
+        }
+        b->units_n = units.size();
+        HIP_TRY(hipMemcpyAsync(b->units, units.data(), units.size() * sizeof(BUnit), hipMemcpyHostToDevice,
+                               b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+    }
     if (t.autotune) {
         std::vector<long long> key = {n_jobs, begin, pix, max_depth};
         for (int i = 0; i < n_scenes; ++i) key.push_back((long long)(uintptr_t)scenes[i]);
@@ -2087,6 +2135,7 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
                 t.block = kv.second.block;
                 t.bvh_width = kv.second.bvh_width;
                 t.fixed_batches = kv.second.fixed_batches;
+                t.engine = kv.second.engine;
                 hit = true;
             }
         if (!hit) {
@@ -2111,6 +2160,25 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
                 }
                 t.bvh_width = wbest;
             }
+            // engine: the tuned wavefront batches against the batched
+            // megakernel, each over the whole workload once
+            set_width(t.bvh_width);
+            HIP_TRY(hipMemcpyAsync(b->tab, views.data(), views.size() * sizeof(SceneView<float>),
+                                   hipMemcpyHostToDevice, b->stream));
+            double r_wf;
+            mh_tune tw = t;
+            tw.engine = MH_BATCH_WAVEFRONT;
+            rc = batch_trial(b, mj, max_depth, feat, tw, mj.total, &r_wf);
+            if (rc) return rc;
+            HIP_TRY(wf_multi_begin(b->ws, mj, b->stream));
+            HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+            rc = mega_run(b, mj, feat, -1);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+            HIP_TRY(hipStreamSynchronize(b->stream));
+            float ms = ms_between(b->ev[0], b->ev[1]);
+            double r_mk = ms > 0.f ? (double)mj.total / ms : 0.0;
+            t.engine = r_mk > r_wf ? MH_BATCH_MEGAKERNEL : MH_BATCH_WAVEFRONT;
             b->tuned.push_back({key, t});
         }
         set_width(t.bvh_width);
@@ -2118,14 +2186,16 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
                                b->stream));
     }
     if (t.bvh_width == 0) t.bvh_width = 4;
-    rc = ensure_batch_state(b, t.pool);
-    if (rc) return rc;
+    if (t.engine == MH_BATCH_WAVEFRONT) {
+        rc = ensure_batch_state(b, t.pool);
+        if (rc) return rc;
+    }
 
     LaunchOut lo{b->fb_linear, b->fb_rgb, b->d_rays, nullptr};
     HIP_TRY(hipMemsetAsync(b->d_rays, 0, sizeof(unsigned long long), b->stream));
     HIP_TRY(wf_multi_begin(b->ws, mj, b->stream));
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    rc = batch_run(b, mj, max_depth, feat, t, -1);
+    rc = t.engine == MH_BATCH_MEGAKERNEL ? mega_run(b, mj, feat, -1) : batch_run(b, mj, max_depth, feat, t, -1);
     if (rc) return rc;
     HIP_TRY(wf_multi_end(mj, lo, b->stream));
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));

@@ -84,6 +84,16 @@ struct WfMulti {
     int sh_stride;               // shadow-queue blocks per bounce: max lights over the scenes + 1 (sun)
 };
 
+// Batched megakernel work unit: one 16x16 tile of a job, samples
+// [s_begin, s_end) of it.
+struct BUnit {
+    int job, tile, s_begin, s_end;
+};
+// Batched megakernel: every unit in one launch (sums into mj.acc64;
+// resolve with wf_multi_end). fog: some scene has fog or marched water.
+hipError_t batch_mega(const WfMulti& mj, const BUnit* units, int n_units, bool fog, unsigned long long* rays,
+                      hipStream_t st);
+
 struct WfJob {
     int w, h, spp, s0, depth;
     int npix;

@@ -658,11 +658,66 @@ __global__ __launch_bounds__(256) void k_wf_multi_clear(WfState st, WfMulti mj)
     size_t n = (size_t)mj.total_pix * 3;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         mj.acc64[i] = 0ull;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st.batch[0] = 0;
+    // the batched megakernel runs without the wavefront state
+    if (st.batch && blockIdx.x == 0 && threadIdx.x == 0) st.batch[0] = 0;
 }
 
 // Batched engine: per-pixel means and tone map of every job, written to the
 // concatenated output (job order, each job top-down RGB).
+// ---- batched megakernel ------------------------------------------------------
+//
+// The same jobs as the batched wavefront engine, as work units of one
+// 16x16-pixel tile and a range of samples of one job (BUnit), one
+// workgroup per unit in a single launch: every lane runs whole paths
+// (pt_radiance, the megakernel's estimator) and the units of all jobs fill
+// the machine together. Each sample's radiance goes to 2^-24 fixed point
+// before it is summed, so the result does not depend on how samples are cut
+// into units or in which order units run.
+template <bool FOG>
+__global__ __launch_bounds__(256) void k_batch_mega(WfMulti mj, const BUnit* units, int n_units,
+                                                    unsigned long long* rays)
+{
+    MH_STACK_PRIV(ts)
+    int nq = 0;
+    if ((int)blockIdx.x < n_units) {
+        BUnit u = units[blockIdx.x];
+        const BJob& job = mj.jobs[u.job];
+        const SceneView<float>& sc = mj.scenes[job.scene];
+        const CamView<float>& cam = job.cam;
+        int w = job.w, h = job.h;
+        int tiles_x = (w + 15) / 16;
+        int x = (u.tile % tiles_x) * 16 + (int)(threadIdx.x & 15);
+        int y = (u.tile / tiles_x) * 16 + (int)(threadIdx.x >> 4);
+        if (x < w && y < h) {
+            unsigned long long f[3] = {0ull, 0ull, 0ull};
+            for (int si = u.s_begin; si < u.s_end; ++si) {
+                uint32_t s = (uint32_t)(job.s0 + si);
+                float jx = rand01<float>((uint32_t)x, (uint32_t)y, s * 2u + 0u);
+                float jy = rand01<float>((uint32_t)x, (uint32_t)y, s * 2u + 1u);
+                float l1 = rand01<float>((uint32_t)x, (uint32_t)y, s * 2u + 100u);
+                float l2 = rand01<float>((uint32_t)x, (uint32_t)y, s * 2u + 101u);
+                float uu = ((float)x + jx) / (float)w;
+                float vv = 1.f - ((float)y + jy) / (float)h;
+                V3<float> o, d;
+                camera_ray(cam, uu, vv, l1, l2, &o, &d);
+                if (cam.dome > 1e-4f && sc.water_material >= 0) {
+                    V3<float> pd = o + d * cam.dome;
+                    if (pd.y < water_height(sc, pd.x, pd.z)) o = pd - mk(0.f, 1e-3f, 0.f);
+                }
+                V3<float> L = pt_radiance<FOG>(sc, ts, o, d, job.depth, hash3((uint32_t)x, (uint32_t)y, s), &nq);
+                f[0] += to_fixed(L.x);
+                f[1] += to_fixed(L.y);
+                f[2] += to_fixed(L.z);
+            }
+            size_t p = ((size_t)job.acc_off + (size_t)y * (size_t)w + (size_t)x) * 3;
+            atomicAdd(&mj.acc64[p], f[0]);
+            atomicAdd(&mj.acc64[p + 1], f[1]);
+            atomicAdd(&mj.acc64[p + 2], f[2]);
+        }
+    }
+    count_rays(rays, nq);
+}
+
 __global__ __launch_bounds__(256) void k_wf_multi_resolve(WfMulti mj, LaunchOut out)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < mj.total_pix;

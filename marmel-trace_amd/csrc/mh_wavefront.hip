@@ -109,6 +109,15 @@ hipError_t wf_stream_step(const WfMulti& mj, const WfState& ws, int which, unsig
     return hipGetLastError();
 }
 
+hipError_t batch_mega(const WfMulti& mj, const BUnit* units, int n_units, bool fog, unsigned long long* rays,
+                      hipStream_t st)
+{
+    if (n_units <= 0) return hipSuccess;
+    auto kern = fog ? k_batch_mega<true> : k_batch_mega<false>;
+    hipLaunchKernelGGL(kern, dim3(n_units), dim3(256), 0, st, mj, units, n_units, rays);
+    return hipGetLastError();
+}
+
 hipError_t wf_multi_end(const WfMulti& mj, const LaunchOut& out, hipStream_t st)
 {
     long long n = mj.total_pix;

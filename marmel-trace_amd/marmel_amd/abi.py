@@ -155,7 +155,10 @@ class Job(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("pool", "grid_blocks", "block", "use_graph", "autotune",
-                                          "kernel_clock", "bvh_width", "fixed_batches")]
+                                          "kernel_clock", "bvh_width", "fixed_batches", "engine")]
+
+
+BATCH_WAVEFRONT, BATCH_MEGAKERNEL = 0, 1
 
 
 K_GEN, K_EXTEND, K_SHADE, K_SHADOW, K_FINALIZE, K_OTHER = range(6)

@@ -437,7 +437,7 @@ class Batch:
             pass
 
     def render(self, scenes, jobs, pool=0, grid_blocks=0, block=0, use_graph=True, autotune=False,
-               kernel_clock=False, fetch=True, bvh_width=0, fixed_batches=False):
+               kernel_clock=False, fetch=True, bvh_width=0, fixed_batches=False, engine=abi.BATCH_WAVEFRONT):
         """scenes: list of GpuScene; jobs: list of dicts (scene, cam, width, height, spp, depth,
         sample_offset). Returns [(rgb, linear)] per job (None when fetch is False)."""
         arr = (C.c_void_p * len(scenes))(*[s._h.value for s in scenes])
@@ -449,7 +449,7 @@ class Batch:
             npx.append(max(j["width"], 0) * max(j["height"], 0))
         t = abi.Tune(pool=pool, grid_blocks=grid_blocks, block=block, use_graph=int(use_graph),
                      autotune=int(autotune), kernel_clock=int(kernel_clock), bvh_width=bvh_width,
-                     fixed_batches=int(fixed_batches))
+                     fixed_batches=int(fixed_batches), engine=int(engine))
         tot = max(sum(npx), 1)
         rgb = np.zeros(tot * 3, np.uint8) if fetch else None
         lin = np.zeros(tot * 3, np.float32) if fetch else None

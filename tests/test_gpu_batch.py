@@ -70,6 +70,39 @@ def test_batch_jobs_equal_single_renders(world):
     b.close()
 
 
+def test_batched_megakernel_equals_single_megakernel_renders(world):
+    """The batched megakernel (one launch of tile x 4-sample units over all
+    jobs) runs the megakernel's estimator: each job equals its single
+    megakernel render up to fp32 rounding (and the 2^-24 fixed-point sums);
+    deterministic; the
+    autotuner's engine choice keeps the image of that engine."""
+    b = Batch()
+    scenes = [world[n][1] for n in NAMES]
+    jobs = jobs_of(world, NAMES)
+    res = b.render(scenes, jobs, engine=abi.BATCH_MEGAKERNEL)
+    res2 = b.render(scenes, jobs, engine=abi.BATCH_MEGAKERNEL)
+    for j, (rgb, lin), (rgb2, lin2) in zip(jobs, res, res2):
+        assert np.array_equal(lin, lin2)
+        gs = scenes[j["scene"]]
+        rgb_m, lin_m = gs.render(j["cam"], j["width"], j["height"], j["spp"], j["depth"],
+                                 sample_offset=j["sample_offset"])
+        d = np.abs(lin - lin_m)
+        print(NAMES[j["scene"]], j["width"], j["height"], "batched megakernel max |d|", d.max(),
+              "within 1/255", within(rgb, rgb_m))
+        # same code in another kernel: last-bit differences of contracted
+        # arithmetic can flip an occasional path decision
+        assert d.mean() < 1e-4
+        assert within(rgb, rgb_m) >= 0.995
+    res3 = b.render(scenes, jobs, autotune=True)
+    t = b.last_tune()
+    print("autotuned engine", t["engine"])
+    ref = res if t["engine"] == abi.BATCH_MEGAKERNEL else b.render(scenes, jobs, pool=t["pool"],
+                                                                   fixed_batches=bool(t["fixed_batches"]))
+    for (r3, l3), (r, l) in zip(res3, ref):
+        assert np.array_equal(l3, l)
+    b.close()
+
+
 def test_batch_is_independent_of_launch_settings(world):
     b = Batch()
     scenes = [world[n][1] for n in NAMES]
