@@ -178,7 +178,8 @@ def batch_leg():
     eng = {}
     for name, e in (("wavefront", abi.BATCH_WAVEFRONT), ("megakernel", abi.BATCH_MEGAKERNEL)):
         kw = dict(pool=tune["pool"], grid_blocks=tune["grid_blocks"], block=tune["block"],
-                  bvh_width=tune["bvh_width"], fixed_batches=bool(tune["fixed_batches"]), engine=e)
+                  bvh_width=tune["bvh_width"] if e == abi.BATCH_WAVEFRONT else 0,
+                  fixed_batches=bool(tune["fixed_batches"]), engine=e)
         b.render(gss, jobs, fetch=False, **kw)
         t1 = time.perf_counter()
         b.render(gss, jobs, fetch=False, **kw)

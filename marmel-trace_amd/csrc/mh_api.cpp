@@ -2170,6 +2170,10 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
             tw.engine = MH_BATCH_WAVEFRONT;
             rc = batch_trial(b, mj, max_depth, feat, tw, mj.total, &r_wf);
             if (rc) return rc;
+            // the megakernel on its own best layout: the 4-wide tree where built
+            set_width(4);
+            HIP_TRY(hipMemcpyAsync(b->tab, views.data(), views.size() * sizeof(SceneView<float>),
+                                   hipMemcpyHostToDevice, b->stream));
             HIP_TRY(wf_multi_begin(b->ws, mj, b->stream));
             HIP_TRY(hipEventRecord(b->ev[0], b->stream));
             rc = mega_run(b, mj, feat, -1);
@@ -2178,7 +2182,12 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
             HIP_TRY(hipStreamSynchronize(b->stream));
             float ms = ms_between(b->ev[0], b->ev[1]);
             double r_mk = ms > 0.f ? (double)mj.total / ms : 0.0;
-            t.engine = r_mk > r_wf ? MH_BATCH_MEGAKERNEL : MH_BATCH_WAVEFRONT;
+            if (r_mk > r_wf) {
+                t.engine = MH_BATCH_MEGAKERNEL;
+                t.bvh_width = 4;
+            } else {
+                t.engine = MH_BATCH_WAVEFRONT;
+            }
             b->tuned.push_back({key, t});
         }
         set_width(t.bvh_width);
