@@ -529,7 +529,7 @@ typedef struct mh_tune {
                               the slots of ended paths with new ones; 1: batches of
                               `pool` paths run to completion one after another     */
     int32_t engine;        /* MH_BATCH_WAVEFRONT (0, default) or MH_BATCH_MEGAKERNEL:
-                              one launch of (job, 16x16 tile, 4 samples) units, each
+                              one launch of (job, 16x16 tile, 16 samples) units, each
                               lane tracing whole paths; autotune times both       */
 } mh_tune;
 

@@ -1983,6 +1983,11 @@ void mh_batch_destroy(mh_batch* b)
     delete b;
 }
 
+// Samples per batched-megakernel unit.
+#ifndef MH_BATCH_CHUNK
+#define MH_BATCH_CHUNK 16  // 32-scene batch: 2 -> 941, 4 -> 973, 16 -> 998, 64 -> 1000 Msamples/s
+#endif
+
 // Batched megakernel units, chunk-major (the first samples of every job
 // first), so a prefix of the list is a sample of the whole workload.
 static void make_units(const std::vector<BJob>& bj, int chunk, std::vector<BUnit>* out)
@@ -2107,11 +2112,11 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     if (t.pool == 0) t.pool = 1 << 21;
     if (t.grid_blocks == 0) t.grid_blocks = 4096;
     if (t.block == 0) t.block = 256;
-    // batched megakernel units (4 samples of one 16x16 tile of one job)
+    // batched megakernel units (MH_BATCH_CHUNK samples of one 16x16 tile of one job)
     bool want_mega = t.engine == MH_BATCH_MEGAKERNEL || t.autotune;
     if (want_mega) {
         std::vector<BUnit> units;
-        make_units(bj, 4, &units);
+        make_units(bj, MH_BATCH_CHUNK, &units);
         if (units.size() > (size_t)INT_MAX) return fail(2, "batch has too many work units");
         if (units.size() > b->units_n) {
             if (b->units) (void)hipFree(b->units);
