@@ -37,11 +37,13 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "marmel-trace_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Measured HBM bytes per k_wf_shadow launch on this workload: rocprofv3 --pmc
-# FETCH_SIZE (298,506.9 KiB, x 2.0) + WRITE_SIZE (47,730.6 KiB, x 1.0) per
-# dispatch, the factors being what the same counters read for a 1 GiB copy.
-# Re-measure when the shadow kernel or its launch changes.
-SHADOW_TRAFFIC_PER_LAUNCH = 660.2e6
+# Measured HBM bytes per k_wf_shadow launch of the batched engine (the
+# default main engine) on this workload: rocprofv3 --pmc FETCH_SIZE
+# (1,304,730.9 KiB, x 2.0) + WRITE_SIZE (197,214.8 KiB, x 1.0) per dispatch,
+# averaged over the clocked frame's 144 dispatches, the factors being what
+# the same counters read for a 1 GiB copy. Re-measure when the shadow kernel
+# or its launch changes.
+SHADOW_TRAFFIC_PER_LAUNCH = 2874.0e6
 W, H, SPP, DEPTH = 1920, 1080, 64, 8
 GRID = 1024
 
@@ -251,7 +253,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="time clocked direct launches instead of hipGraph replays (per-kernel times from "
                          "the timed frames themselves)")
-    ap.add_argument("--engine", choices=["wavefront", "megakernel"], default="wavefront")
+    ap.add_argument("--engine", choices=["batched", "wavefront", "megakernel"], default="batched",
+                    help="batched: the frame as one job of the batched engine (autotuned, cached); "
+                         "wavefront: the single-scene wavefront render; megakernel: the plain kernel")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="frames of the other engine timed for the batched-vs-plain ratio")
     ap.add_argument("--pool", type=int, default=0, help="wavefront paths per batch (0 = default)")
@@ -269,7 +273,7 @@ def main():
               "the path would shard by image tiles or sample ranges with one reduction", file=sys.stderr)
         sys.exit(2)
 
-    from marmel_amd import GpuScene, abi, device_info
+    from marmel_amd import Batch, GpuScene, abi, device_info
 
     t0 = time.time()
     hs, info, st = build_scene(args.grid)
@@ -285,26 +289,45 @@ def main():
         log(f"bench: post {post_leg(g, cam, 3)}")
         return
 
-    # timed frames replay each batch's bounce sequence as a hipGraph (the
+    # Timed frames replay each batch's bounce sequence as a hipGraph (the
     # production path); per-kernel times come from one more frame launched
-    # directly with a HIP-event pair around every kernel on the render stream
+    # directly with a HIP-event pair around every kernel on the render stream.
+    # The batched engine renders the frame as one job (autotuned once, the
+    # choice cached on the handle; paths of several samples of a pixel in
+    # flight, fixed-point pixel sums).
     wf = dict(engine=abi.ENGINE_WAVEFRONT, pool=args.pool, grid_blocks=args.grid_blocks)
-    clocked_kw = dict(wf, use_graph=False, kernel_clock=True)
-    engines = {"wavefront": clocked_kw if args.no_graph else dict(wf, use_graph=True),
-               "megakernel": dict(engine=abi.ENGINE_MEGAKERNEL)}
-    main_kw = engines[args.engine]
+    batch = Batch()
+    job = dict(scene=0, cam=cam, width=W, height=H, spp=SPP, depth=DEPTH)
+
+    def frame(engine, off, clocked=False):
+        if engine == "batched":
+            if clocked:
+                t = batch.last_tune()
+                batch.render([g], [dict(job, sample_offset=off)], fetch=False, pool=t["pool"],
+                             grid_blocks=t["grid_blocks"], block=t["block"], bvh_width=t["bvh_width"],
+                             fixed_batches=bool(t["fixed_batches"]), engine=t["engine"], use_graph=False,
+                             kernel_clock=True)
+            else:
+                batch.render([g], [dict(job, sample_offset=off)], fetch=False, autotune=True,
+                             use_graph=not args.no_graph, kernel_clock=args.no_graph)
+        elif engine == "wavefront":
+            kw = dict(wf, use_graph=False, kernel_clock=True) if clocked or args.no_graph else dict(wf, use_graph=True)
+            g.render(cam, W, H, SPP, DEPTH, fetch=False, sample_offset=off, **kw)
+        else:
+            g.render(cam, W, H, SPP, DEPTH, fetch=False, sample_offset=off, engine=abi.ENGINE_MEGAKERNEL)
+        return g.last_timing()
+
     for i in range(args.warmup):
-        g.render(cam, W, H, SPP, DEPTH, fetch=False, **main_kw)
+        frame(args.engine, 0)
         log(f"bench: warmup {i + 1}/{args.warmup} done")
     kernel_ms, rays = [], 0
     ktimes = {}
     t_start = time.perf_counter()
     for i in range(args.steps):
-        g.render(cam, W, H, SPP, DEPTH, fetch=False, sample_offset=i * SPP, **main_kw)
-        t = g.last_timing()
+        t = frame(args.engine, i * SPP)
         kernel_ms.append(t.kernel_ms)
         rays += t.rays
-        if args.no_graph:
+        if args.no_graph and args.engine != "megakernel":
             for k, (ms, n) in g.last_kernel_times().items():
                 a = ktimes.setdefault(k, [0.0, 0])
                 a[0] += ms
@@ -316,24 +339,33 @@ def main():
     value = samples * args.steps / wall / 1e6
     mean_kernel_ms = sum(kernel_ms) / len(kernel_ms)
     k_frames = args.steps
-    if args.engine == "wavefront" and not args.no_graph:
+    tune = batch.last_tune() if args.engine == "batched" else None
+    if args.engine != "megakernel" and not args.no_graph:
         # the same kernels launched directly, clocked (not part of value)
-        g.render(cam, W, H, SPP, DEPTH, fetch=False, sample_offset=args.steps * SPP, **clocked_kw)
+        tc = frame(args.engine, args.steps * SPP, clocked=True)
         for k, (ms, n) in g.last_kernel_times().items():
             ktimes[k] = [ms, n]
         k_frames = 1
-        log(f"bench: clocked frame {g.last_timing().kernel_ms:.1f} ms (per-kernel times)")
+        log(f"bench: clocked frame {tc.kernel_ms:.1f} ms (per-kernel times)")
+    single_wf = None
+    if args.engine == "batched" and args.plain_steps > 0:
+        frame("wavefront", 0)
+        t0 = time.perf_counter()
+        for i in range(args.plain_steps):
+            frame("wavefront", i * SPP)
+        single_wf = samples * args.plain_steps / (time.perf_counter() - t0) / 1e6
+        log(f"bench: single-scene wavefront {single_wf:.1f} Msamples/s")
 
     # the plain GPU kernel on the same frame and sample streams (equal RMSE:
     # both engines evaluate the identical estimator sample for sample)
     plain = None
-    other = "megakernel" if args.engine == "wavefront" else "wavefront"
+    other = "megakernel" if args.engine != "megakernel" else "batched"
     if args.plain_steps > 0:
-        g.render(cam, W, H, SPP, DEPTH, fetch=False, **engines[other])
+        frame(other, 0)
         t0 = time.perf_counter()
         for i in range(args.plain_steps):
-            g.render(cam, W, H, SPP, DEPTH, fetch=False, sample_offset=i * SPP, **engines[other])
-            log(f"bench: {other} step {i + 1}/{args.plain_steps} {g.last_timing().kernel_ms:.1f} ms")
+            tt = frame(other, i * SPP)
+            log(f"bench: {other} step {i + 1}/{args.plain_steps} {tt.kernel_ms:.1f} ms")
         plain = samples * args.plain_steps / (time.perf_counter() - t0) / 1e6
 
     adj = None if args.no_adjoint else adjoint_leg(args.adjoint_reps)
@@ -365,9 +397,12 @@ def main():
         per_launch = shadow_bytes(st) * scale * k_frames / sh_n
         avg_s = sh_ms / sh_n * 1e-3
         achieved = per_launch / avg_s / 1e9
-        roof = {"bound": "hbm", "kernel": "k_wf_shadow", "achieved": round(achieved, 1),
+        roof = {"bound": "hbm", "kernel": "k_wf_shadow" + ("<true, false> (batched engine)" if args.engine == "batched"
+                                                           else "<false, false>"),
+                "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": SHADOW_TRAFFIC_PER_LAUNCH, "bytes_per_launch": round(per_launch),
+                "traffic": SHADOW_TRAFFIC_PER_LAUNCH if args.engine == "batched" else None,
+                "bytes_per_launch": round(per_launch),
                 "avg_launch_us": round(avg_s * 1e6, 1), "launches": sh_n,
                 "kernel_share": round(sh_ms / sum(v[0] for v in ktimes.values()), 3)}
     line = {
@@ -384,7 +419,9 @@ def main():
         "kernel_ms_per_frame": {k: round(v[0] / k_frames, 2) for k, v in ktimes.items() if v[1]},
         f"{other}_msamples_per_s": None if plain is None else round(plain, 2),
         "batched_vs_plain": None if plain is None else round(
-            (value / plain) if args.engine == "wavefront" else (plain / value), 3),
+            (value / plain) if args.engine != "megakernel" else (plain / value), 3),
+        "single_scene_wavefront_msamples_per_s": None if single_wf is None else round(single_wf, 2),
+        "tune": tune,
         "adjoint": adj,
         "batch32": b32,
         "after_framebuffer": post,
