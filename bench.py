@@ -1,18 +1,21 @@
 #!/usr/bin/env python3
-"""Benchmark of the plain GPU path tracer on the 1080p terrain workload.
+"""Benchmark of the GPU path tracer on the 1080p terrain workload: batched
+engine vs the plain GPU kernel at equal RMSE.
 
 Workload (one GPU): fBm heightfield terrain on a 1024 x 1024 vertex grid
 (2,093,058 fp32 triangles) + 200 seeded spheres (8 emissive, radiance
 U[2,20]), GGX PBR materials, 1920 x 1080, 64 spp, depth 8, NEE to the sun and
-the 8 emitters, Russian roulette. One step = one full frame rendered into the
-scene's device framebuffer (scene already resident in HBM).
+the 8 emitters, Russian roulette. One step = one full frame (scene already
+resident in HBM), by default rendered as one job of the batched engine
+(autotuned once, cached); the plain megakernel renders the same frame and
+sample streams for the ratio.
 
 Prints ONE JSON line. Also reports:
   roofline      for the dominant kernel, k_wf_shadow (the any-hit shadow
                 queries): algorithmic bytes per launch over its average launch
                 duration, both from this run. The duration comes from HIP
                 event pairs the library records around every launch on its
-                render stream during the timed frames (kernel_clock). The
+                stream during one clocked frame (kernel_clock). The
                 bytes are the shadow queries' traversal counts from the CPU
                 oracle (run with shadow segments as one any-hit query, which
                 is what the GPU does and gives the identical image) on a
