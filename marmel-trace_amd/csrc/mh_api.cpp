@@ -889,11 +889,13 @@ int autotune(mh_batch* b, const WfMulti& mj, int max_depth, int feat, mh_tune* t
     long long trial = std::min<long long>(mj.total, 1LL << 23);
     double best = -1.0;
     mh_tune cur = *t;
-    const int pools[] = {1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23};
+    // pools up to the 2^26 slots the shadow tags address (35 GB of state:
+    // HBM holds it); a pool's trial covers at least the pool
+    const int pools[] = {1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26};
     int pool_best = cur.pool, mode_best = 0;
     for (int mode = 0; mode < 2; ++mode)
         for (int p : pools) {
-            if (p > 2 * trial && p != pools[0]) break;  // larger pools cannot fill up
+            if (p > 2 * mj.total && p != pools[0]) break;  // larger pools cannot fill up
             mh_tune c = cur;
             c.pool = p;
             c.fixed_batches = mode;
@@ -909,7 +911,7 @@ int autotune(mh_batch* b, const WfMulti& mj, int max_depth, int feat, mh_tune* t
     cur.pool = pool_best;
     cur.fixed_batches = mode_best;
     best = -1.0;
-    const int grids[] = {1024, 2048, 4096};
+    const int grids[] = {1024, 2048, 4096, 8192, 16384};
     int grid_best = cur.grid_blocks;
     for (int g : grids) {
         mh_tune c = cur;

@@ -134,9 +134,9 @@ def test_batch_autotune_keeps_the_image(world):
     res = b.render(scenes, jobs, autotune=True)
     t = b.last_tune()
     print("autotuned", t)
-    assert t["pool"] in (1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23)
+    assert t["pool"] in tuple(1 << k for k in range(19, 27))
     assert t["fixed_batches"] in (0, 1)
-    assert t["grid_blocks"] in (1024, 2048, 4096) and t["block"] in (128, 256)
+    assert t["grid_blocks"] in (1024, 2048, 4096, 8192, 16384) and t["block"] in (128, 256)
     assert t["bvh_width"] in (2, 4)
     for (r0, l0), (r1, l1) in zip(ref, res):
         assert np.array_equal(l0, l1)
