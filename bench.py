@@ -42,11 +42,11 @@ sys.path.insert(0, os.path.join(REPO, "marmel-trace_amd"))
 HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per k_wf_shadow launch of the batched engine (the
 # default main engine) on this workload: rocprofv3 --pmc FETCH_SIZE
-# (1,304,730.9 KiB, x 2.0) + WRITE_SIZE (197,214.8 KiB, x 1.0) per dispatch,
-# averaged over the clocked frame's 144 dispatches, the factors being what
-# the same counters read for a 1 GiB copy. Re-measure when the shadow kernel
-# or its launch changes.
-SHADOW_TRAFFIC_PER_LAUNCH = 2874.0e6
+# (11,014,738.3 KiB, x 2.0) + WRITE_SIZE (1,821,232.9 KiB, x 1.0) per
+# dispatch, averaged over the clocked frame's 18 dispatches (pool 2^26 paths,
+# 16384 workgroups), the factors being what the same counters read for a
+# 1 GiB copy. Re-measure when the shadow kernel or its launch changes.
+SHADOW_TRAFFIC_PER_LAUNCH = 24423.1e6
 W, H, SPP, DEPTH = 1920, 1080, 64, 8
 GRID = 1024
 
