@@ -901,6 +901,13 @@ int autotune(mh_batch* b, const WfMulti& mj, int max_depth, int feat, mh_tune* t
             c.fixed_batches = mode;
             double r;
             int rc = batch_trial(b, mj, max_depth, feat, c, std::max<long long>(trial, p), &r);
+            if (rc == 100 + (int)hipErrorOutOfMemory && p != pools[0]) {
+                // the device cannot hold this pool next to what else it holds:
+                // the largest pool that fitted is the ceiling
+                (void)hipGetLastError();
+                g_err.clear();
+                break;
+            }
             if (rc) return rc;
             if (r > best) {
                 best = r;

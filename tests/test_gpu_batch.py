@@ -180,3 +180,28 @@ def test_batch_error_codes(world, patch, code):
         b.render([gs], [job])
     assert e.value.code == code
     b.close()
+
+
+def test_autotune_stops_at_the_largest_pool_that_fits(world):
+    """With most of the device's memory taken, a pool whose path state does
+    not fit (2^26 paths are tried: the job has 66 M) ends the pool search,
+    allocation failure is not an error, and the image is the same bits as
+    a render on the chosen engine without the autotuner."""
+    import torch
+    hs, gs = world["spheres"]
+    jobs = [dict(scene=0, cam=hs.camera(16 / 9), width=1920, height=1080, spp=32, depth=2)]
+    b = Batch()
+    free, _ = torch.cuda.mem_get_info()
+    keep = 24 << 30  # room for pools up to 2^25 paths (~17.5 GB), not 2^26 (~35 GB)
+    blocker = torch.empty(max(free - keep, 0), dtype=torch.uint8, device="cuda") if free > keep else None
+    try:
+        res = b.render([gs], jobs, autotune=True)
+        t = b.last_tune()
+    finally:
+        del blocker
+        torch.cuda.empty_cache()
+    print("free before the blocker %.1f GiB, autotuned %s" % (free / 2**30, t))
+    assert t["pool"] <= 1 << 25
+    ref = b.render([gs], jobs, engine=t["engine"])
+    assert np.array_equal(ref[0][1], res[0][1])
+    b.close()
