@@ -1842,6 +1842,12 @@ double or_fresnel_dielectric_x(double cos_i, double eta) { return or_fresnel_die
 
 double or_specular_probability_x(double m, double r, double f0) { return or_specular_probability(m, r, f0); }
 
+void or_texture_albedos(const mh_material *m, const double *p, const double *uv, int n, double *out)
+{
+    for (int i = 0; i < n; ++i)
+        put3(out + 3 * i, or_texture_albedo(m, d3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), uv[2 * i], uv[2 * i + 1]));
+}
+
 void or_shade(const or_scene *s, int material, const double N[3], const double L[3],
               const double V[3], const double light[3], double out[3])
 {

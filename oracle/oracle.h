@@ -125,6 +125,11 @@ double or_specular_probability_x(double metallic, double roughness, double f0);
 void or_shade(const or_scene *s, int material, const double N[3], const double L[3],
               const double V[3], const double light[3], double out[3]);
 unsigned char or_to_byte(double c);
+/* Albedo of material m at n points p (3 doubles each) with surface uvs uv
+ * (2 doubles each): textures NONE, CHECKER, UV_CHECKER (reference
+ * texture_albedo_uv, src/texture.c:269-327). out: 3 doubles per point. */
+void or_texture_albedos(const mh_material *m, const double *p, const double *uv, int n,
+                        double *out);
 
 #ifdef __cplusplus
 }

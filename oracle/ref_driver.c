@@ -26,6 +26,7 @@
 #include "sampling.h"
 #include "scene.h"
 #include "scene_desc.h"
+#include "texture.h"
 
 #include <stdio.h>
 #include <stdlib.h>
@@ -99,6 +100,33 @@ static void collect_lights(Scene *s)
 }
 
 /* Build a reference Scene from flattened arrays (same data the GPU gets). */
+/* One flattened material as the reference's Material. */
+static void mat_from(const mh_material *a, Material *m)
+{
+    material_texture_defaults(m);
+    m->albedo = v3(a->albedo);
+    m->specular = v3(a->specular);
+    m->shininess = a->shininess;
+    m->reflectivity = a->reflectivity;
+    m->transparency = a->transparency;
+    m->ior = a->ior;
+    m->is_water = a->is_water;
+    m->beer_lambert = a->beer_lambert;
+    m->pbr = a->pbr;
+    m->texture_kind = a->texture_kind;
+    m->absorption = v3(a->absorption);
+    m->deep_color = v3(a->deep_color);
+    m->metallic = a->metallic;
+    m->roughness = a->roughness;
+    m->emissive = v3(a->emissive);
+    m->texture_scale = a->texture_scale;
+    m->texture_color_a = v3(a->texture_color_a);
+    m->texture_color_b = v3(a->texture_color_b);
+    m->atmosphere_glow = v3(a->atmosphere_glow);
+    m->bump_strength = a->bump_strength;
+    m->bump_scale = 1.0;
+}
+
 void *mtref_from_soa(const mh_scene_soa *soa)
 {
     RefScene *rs = (RefScene *)calloc(1, sizeof(RefScene));
@@ -156,28 +184,7 @@ void *mtref_from_soa(const mh_scene_soa *soa)
     for (int i = 0; i < soa->n_materials; ++i) {
         const mh_material *a = &soa->materials[i];
         Material *m = &s->materials[i];
-        material_texture_defaults(m);
-        m->albedo = v3(a->albedo);
-        m->specular = v3(a->specular);
-        m->shininess = a->shininess;
-        m->reflectivity = a->reflectivity;
-        m->transparency = a->transparency;
-        m->ior = a->ior;
-        m->is_water = a->is_water;
-        m->beer_lambert = a->beer_lambert;
-        m->pbr = a->pbr;
-        m->texture_kind = a->texture_kind;
-        m->absorption = v3(a->absorption);
-        m->deep_color = v3(a->deep_color);
-        m->metallic = a->metallic;
-        m->roughness = a->roughness;
-        m->emissive = v3(a->emissive);
-        m->texture_scale = a->texture_scale;
-        m->texture_color_a = v3(a->texture_color_a);
-        m->texture_color_b = v3(a->texture_color_b);
-        m->atmosphere_glow = v3(a->atmosphere_glow);
-        m->bump_strength = a->bump_strength;
-        m->bump_scale = 1.0;
+        mat_from(a, m);
     }
 
     for (int i = 0; i < soa->n_prims; ++i) {
@@ -591,6 +598,31 @@ void mtref_shade(void *h, int mi, const double N[3], const double L[3], const do
 unsigned char mtref_to_byte(double c)
 {
     return pathtrace_to_byte(c);
+}
+
+/* camera_ray_dof (reference src/camera.c:83-134) for n (u, v, r1, r2)
+ * quadruples: out holds origin and direction, 6 doubles per ray. */
+void mtref_camera_ray_dof(const mh_camera *c, const double *uvr, int n, double *out)
+{
+    Camera cam = cam_from(c);
+    for (int i = 0; i < n; ++i) {
+        Ray r = camera_ray_dof(&cam, uvr[4 * i], uvr[4 * i + 1], uvr[4 * i + 2], uvr[4 * i + 3]);
+        out[6 * i] = r.origin.x; out[6 * i + 1] = r.origin.y; out[6 * i + 2] = r.origin.z;
+        out[6 * i + 3] = r.dir.x; out[6 * i + 4] = r.dir.y; out[6 * i + 5] = r.dir.z;
+    }
+}
+
+/* texture_albedo_uv (reference src/texture.c:269-327) of one material at n
+ * points (3 doubles each) with uvs (2 doubles each). */
+void mtref_texture_albedo(const mh_material *a, const double *p, const double *uv, int n, double *out)
+{
+    Material m;
+    memset(&m, 0, sizeof m);
+    mat_from(a, &m);
+    for (int i = 0; i < n; ++i) {
+        Vec3 c = texture_albedo_uv(&m, vec3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), uv[2 * i], uv[2 * i + 1]);
+        out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
 }
 
 /* render_image_ex (adaptive or fixed Whitted) with the last render's sample

@@ -20,4 +20,5 @@ def golden():
     import numpy as np
     g = os.path.join(HERE, "golden")
     return {name: np.load(os.path.join(g, name + ".npz"))
-            for name in ("default_scene", "host_scenes", "vectors", "fog", "ingest", "adaptive", "flare")}
+            for name in ("default_scene", "host_scenes", "vectors", "fog", "ingest", "adaptive", "flare",
+                         "lens_texture")}

@@ -174,6 +174,77 @@ def flare():
     np.savez_compressed(os.path.join(HERE, "flare.npz"), **out)
 
 
+LENS_IMG = {"pt": (96, 72, 8, 5), "whitted": (96, 72, 4, 4)}
+
+# Cameras of the per-function lens vectors: (aperture, focus distance,
+# squeeze). Pinhole and negative aperture (the exact pinhole ray), the round
+# lens, squeezes above and below 1, squeeze <= 0 and within 1e-12 of 1 (both
+# the round lens), focus distance 0 (the unit fallback).
+LENS_VEC_CAMS = [(0.0, 8.4, 1.0), (-3.0, 8.4, 1.0), (0.15, 8.4, 1.0), (0.25, 6.0, 1.8),
+                 (0.2, 10.0, 0.55), (0.3, 5.0, -1.0), (0.3, 5.0, 1.0 + 1e-13), (0.1, 0.0, 1.3)]
+# Texture materials of the per-function texture vectors: (kind, scale).
+TEX_VEC_MATS = [(0, 1.0), (1, 1.0), (1, 0.5), (1, 0.0), (1, -2.0), (1, float("nan")), (1, 2.3),
+                (7, 1.0), (7, 4.0), (7, 0.0), (7, -1.0), (7, 0.37)]
+
+
+def lens_inputs():
+    """(u, v, r1, r2) quadruples and texture points / uvs of the lens and
+    texture vectors (seeded; edges r1 = 0, r1 = 1 and cell boundaries)."""
+    rng = np.random.default_rng(20261204)
+    uvr = rng.random((400, 4))
+    uvr[:20, 2] = 0.0
+    uvr[20:40, 2] = 1.0
+    uvr[40:60, 3] = 0.0
+    p = rng.uniform(-20, 20, (600, 3))
+    p[:100] = np.round(p[:100] * 2) / 2          # on cell faces of scales 1 and 0.5
+    p[100:150] *= 5e4                             # large coordinates
+    uv = rng.uniform(-3, 3, (600, 2))
+    uv[:100] = np.round(uv[:100] * 4) / 4
+    return np.ascontiguousarray(uvr), np.ascontiguousarray(p), np.ascontiguousarray(uv)
+
+
+def lens_texture():
+    """Reference camera_ray_dof and texture_albedo_uv vectors, and PT / Whitted
+    images of tests/scenes.lens_texture_scene under each thin-lens camera of
+    scenes.LENS_CAMS."""
+    import scenes
+    L = ck.ref()
+    out = {}
+    uvr, p, uv = lens_inputs()
+    from marmel_amd import make_camera
+    for i, (ap, fd, sq) in enumerate(LENS_VEC_CAMS):
+        cam = make_camera(scenes.LENS_EYE, scenes.LENS_AT, [0, 1, 0], scenes.LENS_VFOV, 4 / 3)
+        cam.aperture, cam.focus_distance, cam.anamorphic_squeeze = ap, fd, sq
+        o = np.zeros((len(uvr), 6))
+        L.mtref_camera_ray_dof(C.byref(cam), abi.ptr(uvr), len(uvr), abi.ptr(o))
+        out[f"cam{i}"] = sbytes(cam)
+        out[f"cam{i}_rays"] = o
+    out["uvr"], out["tex_p"], out["tex_uv"] = uvr, p, uv
+    for i, (kind, scale) in enumerate(TEX_VEC_MATS):
+        m = abi.Material()
+        m.albedo = abi.vec3((0.5, 1.0, 0.25))
+        m.texture_kind, m.texture_scale = kind, scale
+        m.texture_color_a = abi.vec3((0.9, 0.1, 0.3))
+        m.texture_color_b = abi.vec3((0.2, 0.7, 0.8))
+        o = np.zeros((len(p), 3))
+        L.mtref_texture_albedo(C.byref(m), abi.ptr(p), abi.ptr(uv), len(p), abi.ptr(o))
+        out[f"tex{i}_mat"] = sbytes(m)
+        out[f"tex{i}_out"] = o
+    hs = scenes.lens_texture_scene()
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    out["prims_sha"] = np.frombuffer(hashlib.sha256(prims.tobytes() + attrs.tobytes() + mats.tobytes())
+                                     .hexdigest().encode(), np.uint8).copy()
+    r = ck.RefScene(soa=ck.SoaHolder(prims, attrs, mats, sky, wl, wm).soa)
+    for name in scenes.LENS_CAMS:
+        for tag, mode in (("pt", 0), ("whitted", 1)):
+            w, h, spp, depth = LENS_IMG[tag]
+            cam = scenes.lens_camera(hs, name, w / h)
+            out[f"{name}_cam"] = sbytes(cam)
+            rgb, _ = r.render(cam, w, h, spp, depth, mode, threads=8)
+            out[f"{name}_{tag}"] = rgb
+    np.savez_compressed(os.path.join(HERE, "lens_texture.npz"), **out)
+
+
 def vectors(r, sky):
     """Reference outputs of the per-function routines on seeded inputs."""
     L = ck.ref()
@@ -233,6 +304,7 @@ def main():
     host_scene_images()
     fog_scenes()
     ingest()
+    lens_texture()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

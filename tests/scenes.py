@@ -15,9 +15,9 @@ def material(**kw):
     return m
 
 
-def prim(kind, mat, p0=(0, 0, 0), p1=(0, 0, 0), p2=(0, 0, 0), r0=0.0, r1=0.0):
+def prim(kind, mat, p0=(0, 0, 0), p1=(0, 0, 0), p2=(0, 0, 0), r0=0.0, r1=0.0, flags=0):
     p = np.zeros(1, abi.PRIM_DTYPE)[0]
-    p["kind"], p["material"] = kind, mat
+    p["kind"], p["material"], p["flags"] = kind, mat, flags
     p["p0"], p["p1"], p["p2"] = p0, p1, p2
     p["r0"], p["r1"] = r0, r1
     return p
@@ -45,10 +45,11 @@ def sky(horizon=(0, 0, 0), zenith=(0, 0, 0), sun_color=(0, 0, 0), sun_dir=(0.0, 
     return s
 
 
-def build(prims, mats, sk, water=(0.0, -1)):
+def build(prims, mats, sk, water=(0.0, -1), attrs=None):
     hs = HostScene()
     hs.add_materials(np.array(mats, dtype=abi.MATERIAL_DTYPE))
-    hs.add_prims(np.array(prims, dtype=abi.PRIM_DTYPE))
+    hs.add_prims(np.array(prims, dtype=abi.PRIM_DTYPE),
+                 None if attrs is None else np.array(attrs, dtype=abi.ATTR_DTYPE))
     hs.set_sky(sk)
     hs.set_water(*water)
     hs.build_bvh()
@@ -125,3 +126,98 @@ def fog_rays():
     b = seeded_rays(150, 22, [-3.5, -0.9, -2.5], [3.5, -0.1, 2.5])
     c = seeded_rays(50, 23, [-10, 26, -10], [10, 30, 10])
     return np.concatenate([a, b, c])
+
+
+# ---- thin lens, anamorphic squeeze and albedo textures ---------------------
+
+def _uv_wall(mat, x0, x1, y0, y1, z, nx, ny, smooth_rows):
+    """A vertical grid of nx x ny quads at depth z, two uv triangles each
+    (uv = position in the grid, spanning [0, 1.5] x [0, 1]); the quads of the
+    rows in smooth_rows also carry tilted vertex normals."""
+    prims, attrs = [], []
+    xs, ys = np.linspace(x0, x1, nx + 1), np.linspace(y0, y1, ny + 1)
+    for j in range(ny):
+        for i in range(nx):
+            c = [(xs[i], ys[j]), (xs[i + 1], ys[j]), (xs[i + 1], ys[j + 1]), (xs[i], ys[j + 1])]
+            for tri in ((0, 1, 2), (0, 2, 3)):
+                v = [(c[k][0], c[k][1], z) for k in tri]
+                uv = [(1.5 * (c[k][0] - x0) / (x1 - x0), (c[k][1] - y0) / (y1 - y0), 0.0) for k in tri]
+                a = np.zeros(1, abi.ATTR_DTYPE)[0]
+                flags = abi.PRIM_UV
+                if j in smooth_rows:
+                    flags |= abi.PRIM_SMOOTH
+                    for k, key in enumerate(("n0", "n1", "n2")):
+                        nv = np.array([0.25 * np.sin(3.0 * v[k][0]), 0.15 * np.cos(2.0 * v[k][1]), 1.0])
+                        a[key] = tuple(nv / np.linalg.norm(nv))
+                for k, key in enumerate(("uv0", "uv1", "uv2")):
+                    a[key] = uv[k]
+                prims.append(prim(abi.PRIM_TRIANGLE, mat, v[0], v[1], v[2], flags=flags))
+                attrs.append(a)
+    return prims, attrs
+
+
+def lens_texture_scene():
+    """Every albedo texture kind of the GPU path on every primitive kind that
+    carries surface coordinates: a CHECKER ground plane and box, a PBR sphere
+    with a 3D checker, UV_CHECKER on a sphere (spherical uv), a tapered
+    cylinder, a uv-mapped triangle wall (half of it with smooth normals) and a
+    quad whose scale falls back to 1; plus a mirror, a glass sphere and a
+    small emissive lamp, under a sun with a disk, glow and clouds."""
+    T = abi
+    mats = [
+        material(albedo=(0.85, 0.85, 0.85), texture_kind=T.TEXTURE_CHECKER, texture_scale=1.0,
+                 texture_color_a=(0.9, 0.85, 0.8), texture_color_b=(0.1, 0.15, 0.25)),
+        material(pbr=1, albedo=(1, 1, 1), roughness=0.35, metallic=0.3,
+                 texture_kind=T.TEXTURE_CHECKER, texture_scale=0.35,
+                 texture_color_a=(0.9, 0.3, 0.2), texture_color_b=(0.2, 0.3, 0.9)),
+        material(albedo=(0.8, 0.7, 0.6), texture_kind=T.TEXTURE_UV_CHECKER, texture_scale=8.0,
+                 texture_color_a=(1.0, 1.0, 1.0), texture_color_b=(0.05, 0.05, 0.05)),
+        material(albedo=(0.9, 0.9, 0.9), texture_kind=T.TEXTURE_UV_CHECKER, texture_scale=4.0,
+                 texture_color_a=(0.9, 0.2, 0.2), texture_color_b=(0.9, 0.9, 0.2)),
+        material(albedo=(0.7, 0.7, 0.7), specular=(0.3, 0.3, 0.3), shininess=40.0,
+                 texture_kind=T.TEXTURE_CHECKER, texture_scale=0.25,
+                 texture_color_a=(0.2, 0.8, 0.3), texture_color_b=(0.9, 0.9, 0.9)),
+        material(pbr=1, albedo=(0.95, 0.9, 0.85), roughness=0.6, metallic=0.0,
+                 texture_kind=T.TEXTURE_UV_CHECKER, texture_scale=6.0,
+                 texture_color_a=(0.3, 0.3, 0.9), texture_color_b=(0.9, 0.6, 0.1)),
+        material(albedo=(0.9, 0.9, 0.9), reflectivity=1.0),
+        material(albedo=(1, 1, 1), transparency=1.0, ior=1.5),
+        material(pbr=1, albedo=(1, 1, 1), emissive=(8.0, 7.0, 6.0), roughness=1.0),
+        material(albedo=(0.8, 0.8, 0.8), texture_kind=T.TEXTURE_UV_CHECKER, texture_scale=-2.0,
+                 texture_color_a=(0.1, 0.9, 0.9), texture_color_b=(0.9, 0.1, 0.9)),
+    ]
+    prims = [
+        prim(T.PRIM_PLANE, 0, p0=(0, 0, 0), p1=(0, 1, 0)),
+        prim(T.PRIM_SPHERE, 8, p0=(-0.5, 3.2, -1.5), r0=0.3),
+        prim(T.PRIM_SPHERE, 1, p0=(-2.2, 1.0, 0.0), r0=1.0),
+        prim(T.PRIM_SPHERE, 2, p0=(0.3, 0.8, 1.2), r0=0.8),
+        prim(T.PRIM_SPHERE, 6, p0=(2.4, 0.7, -0.5), r0=0.7),
+        prim(T.PRIM_SPHERE, 7, p0=(1.2, 0.5, 2.6), r0=0.5),
+        prim(T.PRIM_BOX, 4, p0=(-0.6, 0.5, -2.0), p1=(0.5, 0.5, 0.5)),
+        prim(T.PRIM_CYLINDER, 5, p0=(3.2, 0.0, 1.8), p1=(3.2, 1.6, 1.8), r0=0.45, r1=0.3),
+    ]
+    attrs = [np.zeros(1, abi.ATTR_DTYPE)[0] for _ in prims]
+    wp, wa = _uv_wall(3, -4.5, 4.5, 0.0, 3.0, -4.0, 6, 4, smooth_rows=(2, 3))
+    qp, qa = _uv_wall(9, 1.8, 3.0, 1.6, 2.6, -2.5, 1, 1, smooth_rows=())
+    sk = sky(horizon=(0.45, 0.5, 0.6), zenith=(0.15, 0.25, 0.5), sun_color=(1.4, 1.3, 1.2),
+             sun_dir=tuple(np.array([0.4, 0.7, 0.3]) / np.linalg.norm([0.4, 0.7, 0.3])),
+             glow=0.5, clouds=True, sun_radius=0.5)
+    return build(prims + wp + qp, mats, sk, attrs=attrs + wa + qa)
+
+
+# Thin-lens cameras of the lens-and-texture scene: (aperture, focus distance,
+# anamorphic squeeze). A squeeze of 1 is the round lens; above 1 the lens
+# disc is compressed horizontally, below 1 widened.
+LENS_EYE, LENS_AT, LENS_VFOV = (0.5, 2.4, 8.5), (0.0, 0.9, 0.0), 45.0
+LENS_CAMS = {
+    "round": (0.15, 8.4, 1.0),
+    "anamorphic": (0.25, 6.0, 1.8),
+    "wide": (0.2, 10.0, 0.55),
+}
+
+
+def lens_camera(hs, name, aspect):
+    ap, fd, sq = LENS_CAMS[name]
+    cam = hs.camera(aspect, eye=list(LENS_EYE), at=list(LENS_AT), up=[0, 1, 0], vfov=LENS_VFOV)
+    cam.aperture, cam.focus_distance, cam.anamorphic_squeeze = ap, fd, sq
+    return cam

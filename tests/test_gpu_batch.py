@@ -159,12 +159,7 @@ def test_batch_job_statistics_vs_oracle(world):
     jobs = [dict(scene=1, cam=hs2.camera(1.0), width=16, height=16, spp=4, depth=4),
             dict(scene=0, cam=cam, width=w, height=h, spp=64, depth=8)]
     (_, _), (_, lin) = b.render([gs, gs2], jobs)
-    se = np.sqrt(ref_var / 64 + ref_var / 512) + 1e-6
-    z = (lin - ref_mean) / se
-    print("batched terrain within 3 sigma", (np.abs(z) <= 3).mean(), "mean z", z.mean())
-    assert (np.abs(z) <= 3).mean() >= 0.97
-    assert abs(z.mean()) < 0.1
-    assert (z * z).mean() < 1.6
+    ck.assert_mc_consistent(lin, 64, ref_mean, ref_var, 512, "batched terrain")
     b.close()
 
 

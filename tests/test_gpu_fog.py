@@ -51,6 +51,13 @@ def dump(name, a):
         np.save(os.path.join(d, name + ".npy"), a)
 
 
+# Bars of the marched fogs, calibrated with the oracle in place of the GPU
+# (64 vs 1024 samples): few march steps through turbulence correlate
+# neighbouring pixels (4x4 block z^2 1.65-1.68 with the oracle against
+# itself), 128 steps give heavier single-pixel tails (frac3 0.969-0.973).
+FOG_BARS = {"steps3": dict(bfrac3=0.95, bz2=2.0), "steps200": dict(frac3=0.955, bfrac3=0.96, bz2=1.5)}
+
+
 def oracle_for(arrs, fog):
     return ck.OracleScene(ck.SoaHolder(*arrs, fog=fog).soa)
 
@@ -100,7 +107,6 @@ def test_fog_fp32_engines_statistically_consistent(world, name):
     cam = hs.camera(w / h)
     _, ref_mean, ref_var, _ = o.render(cam, w, h, 1024, 8, want_var=True, sample_offset=7000)
     n = 64
-    se = np.sqrt(ref_var / n + ref_var / 1024) + 1e-6
     outs = {
         "megakernel": gs.render(cam, w, h, n, 8, precision=FP32)[1],
         "wavefront": gs.render(cam, w, h, n, 8, precision=FP32, engine=abi.ENGINE_WAVEFRONT)[1],
@@ -112,12 +118,8 @@ def test_fog_fp32_engines_statistically_consistent(world, name):
         b.close()
     for eng, lin in outs.items():
         lin = np.asarray(lin, np.float64).reshape(h, w, 3)
-        z = (lin - ref_mean) / se
-        frac3 = (np.abs(z) <= 3).mean()
-        print(name, eng, "within 3 sigma", frac3, "mean z", z.mean())
         assert np.isfinite(lin).all()
-        assert frac3 >= 0.98
-        assert abs(z.mean()) < 0.15
+        ck.assert_mc_consistent(lin, n, ref_mean, ref_var, 1024, f"{name} {eng}", **FOG_BARS.get(name, {}))
 
 
 def test_fog_host_and_device_setters_agree(world):

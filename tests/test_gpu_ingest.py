@@ -70,12 +70,12 @@ def test_ocean_world_fp64_matches_oracle():
         ws, hs2 = 40, 30
         cam2 = dome_camera(hs, ws / hs2)
         _, rm, rv, _ = o.render(cam2, ws, hs2, 1024, 8, want_var=True, sample_offset=5000)
-        se = np.sqrt(rv / 64 + rv / 1024) + 1e-6
         for eng in (abi.ENGINE_MEGAKERNEL, abi.ENGINE_WAVEFRONT):
             _, l32 = gs.render(cam2, ws, hs2, 64, 8, precision=FP32, engine=eng)
-            z = (l32 - rm) / se
-            print("ocean world fp32 engine", eng, "within 3 sigma", (np.abs(z) <= 3).mean(), "mean z", z.mean())
-            assert (np.abs(z) <= 3).mean() >= 0.98 and abs(z.mean()) < 0.15
+            # caustics through the water: heavy tails also at block level
+            # (the oracle against itself: bfrac3 0.962, bz2 up to 1.2)
+            ck.assert_mc_consistent(l32, 64, rm, rv, 1024, f"ocean world fp32 engine {eng}", bfrac3=0.94,
+                                    bz2=1.5)
     finally:
         gs.close()
 

@@ -1,0 +1,112 @@
+"""GPU tests of the thin-lens camera (depth of field and anamorphic squeeze,
+reference src/camera.c:83-134 with the lens channels of src/pathtrace.c:
+1060-1073 and src/render.c:874-889) and of the CHECKER / UV_CHECKER albedo
+textures (reference src/texture.c:269-327) on every primitive kind that
+carries surface coordinates.
+
+  * fp64 path-traced and Whitted images within 1/255 of the reference's own
+    images (tests/golden/lens_texture.npz) on >= 99.9 % of pixels, for the
+    round, anamorphic (squeeze 1.8) and widened (squeeze 0.55) lenses; the
+    drop-in Whitted entry too,
+  * fp32 engines (megakernel, wavefront, batched, batched megakernel) against
+    a converged FP64 oracle render of the anamorphic view: Monte Carlo
+    consistency (pixels, 4x4 blocks, image sums),
+  * the batched engine's job with a lens camera equals its single wavefront
+    render up to fp32 rounding.
+"""
+import numpy as np
+import pytest
+
+import checkers as ck
+import scenes
+from marmel_amd import Batch, GpuScene, abi
+
+pytestmark = pytest.mark.gpu
+
+FP32, FP64 = abi.PREC_FP32, abi.PREC_FP64
+W, H = 96, 72
+
+
+def cam_of(arr):
+    return abi.Camera.from_buffer_copy(arr.tobytes())
+
+
+def within(a, b, tol=1):
+    return (np.abs(a.astype(int) - b.astype(int)) <= tol).all(axis=-1).mean()
+
+
+@pytest.fixture(scope="module")
+def world(golden):
+    g = golden["lens_texture"]
+    hs = scenes.lens_texture_scene()
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    gs = GpuScene(hs.soa())
+    o = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm).soa)
+    yield g, hs, gs, o
+    gs.close()
+
+
+@pytest.mark.parametrize("name", list(scenes.LENS_CAMS))
+def test_lens_texture_fp64_matches_reference_images(world, name):
+    g, hs, gs, o = world
+    cam = cam_of(g[f"{name}_cam"])
+    for tag, mode, spp, depth in (("pt", abi.MODE_PATHTRACE, 8, 5), ("whitted", abi.MODE_WHITTED, 4, 4)):
+        for bw in (2, 4):
+            rgb, _ = gs.render(cam, W, H, spp, depth, mode=mode, precision=FP64, bvh_width=bw)
+            frac = within(rgb, g[f"{name}_{tag}"])
+            print(name, tag, "width", bw, "fp64 within 1/255 of the reference:", frac)
+            assert frac >= 0.999
+        if mode == abi.MODE_WHITTED:
+            rgb32, _ = gs.render(cam, W, H, spp, depth, mode=mode, precision=FP32)
+            f32 = within(rgb32, g[f"{name}_{tag}"])
+            print(name, tag, "fp32 within 1/255:", f32)
+            assert f32 >= 0.995
+
+
+def test_lens_texture_drop_in_whitted(world):
+    g, hs, gs, o = world
+    rc, rgb = gs.render_image(cam_of(g["anamorphic_cam"]), W, H, 4, 4)
+    assert rc == 0
+    assert within(rgb, g["anamorphic_whitted"]) >= 0.999
+
+
+def test_lens_texture_fp32_engines_vs_converged_oracle(world):
+    g, hs, gs, o = world
+    w, h, n, nref, depth = 48, 36, 64, 1024, 5
+    cam = scenes.lens_camera(hs, "anamorphic", w / h)
+    _, ref_mean, ref_var, _ = o.render(cam, w, h, nref, depth, want_var=True, sample_offset=7000)
+    outs = {"megakernel": gs.render(cam, w, h, n, depth, precision=FP32)[1],
+            "wavefront": gs.render(cam, w, h, n, depth, precision=FP32, engine=abi.ENGINE_WAVEFRONT)[1]}
+    rc, rgb = gs.pathtrace_render(cam, w, h, n, depth)
+    assert rc == 0
+    b = Batch()
+    try:
+        job = [dict(scene=0, cam=cam, width=w, height=h, spp=n, depth=depth)]
+        outs["batched"] = b.render([gs], job)[0][1]
+        outs["batched megakernel"] = b.render([gs], job, engine=abi.BATCH_MEGAKERNEL)[0][1]
+    finally:
+        b.close()
+    for eng, lin in outs.items():
+        assert np.isfinite(lin).all()
+        ck.assert_mc_consistent(lin, n, ref_mean, ref_var, nref, f"lens {eng}")
+    # the drop-in path tracer is the fp32 megakernel
+    assert within(rgb, gs.render(cam, w, h, n, depth, precision=FP32)[0], 0) == 1.0
+
+
+@pytest.mark.parametrize("name", list(scenes.LENS_CAMS))
+def test_lens_wavefront_and_batch_equal_megakernel(world, name):
+    g, hs, gs, o = world
+    cam = cam_of(g[f"{name}_cam"])
+    rgb_m, lin_m = gs.render(cam, W, H, 4, 5, precision=FP32)
+    rgb_w, lin_w = gs.render(cam, W, H, 4, 5, precision=FP32, engine=abi.ENGINE_WAVEFRONT, pool=3000)
+    b = Batch()
+    try:
+        (rgb_b, lin_b), = b.render([gs], [dict(scene=0, cam=cam, width=W, height=H, spp=4, depth=5)],
+                                   pool=2000, grid_blocks=64)
+    finally:
+        b.close()
+    for tag, rgb, lin in (("wavefront", rgb_w, lin_w), ("batched", rgb_b, lin_b)):
+        frac = within(rgb, rgb_m)
+        print(name, tag, "vs megakernel within 1/255:", frac, "mean |d|", np.abs(lin - lin_m).mean())
+        assert frac >= 0.995
+        assert np.abs(lin - lin_m).mean() < 2e-3

@@ -436,3 +436,172 @@ def test_oracle_adaptive_equals_live_reference(kind, kw, seed):
     a, _, _ = r.render_ex(cam, 64, 48, 4, 5, 0)
     b, _, _, _ = o.render(cam, 64, 48, 4, 5, mode=1)
     assert np.array_equal(a, b)
+
+
+# ---- thin lens, anamorphic squeeze, albedo textures ------------------------------
+
+def lens_rays(cam, uvr):
+    """The oracle's camera_ray_dof for each (u, v, r1, r2): (n, 6) origin, direction."""
+    L = ck.oracle()
+    out = np.zeros((len(uvr), 6))
+    row = (C.c_double * 6)()
+    for i, (u, v, r1, r2) in enumerate(uvr):
+        L.or_camera_ray(C.byref(cam), u, v, r1, r2, row)
+        out[i] = list(row)
+    return out
+
+
+def test_oracle_lens_vectors_bit_exact(golden):
+    """camera_ray_dof on seeded (u, v, r1, r2) under the pinhole, a negative
+    aperture, the round lens, squeezes 1.8 / 0.55 / -1 / 1+1e-13 and focus
+    distance 0: equal to the reference's bit for bit."""
+    g = golden["lens_texture"]
+    uvr = g["uvr"]
+    i = 0
+    while f"cam{i}" in g:
+        cam = cam_of(g[f"cam{i}"])
+        assert np.array_equal(lens_rays(cam, uvr), g[f"cam{i}_rays"]), i
+        i += 1
+    assert i == 8
+
+
+def test_oracle_texture_vectors_bit_exact(golden):
+    """texture_albedo_uv of NONE / CHECKER / UV_CHECKER materials over scales
+    1, 0.5, 0, -2, NaN, 2.3 (checker) and 1, 4, 0, -1, 0.37 (uv checker) at
+    seeded points (cell faces and coordinates up to 1e6 included)."""
+    g = golden["lens_texture"]
+    L = ck.oracle()
+    p, uv = np.ascontiguousarray(g["tex_p"]), np.ascontiguousarray(g["tex_uv"])
+    i = 0
+    while f"tex{i}_mat" in g:
+        m = np.frombuffer(g[f"tex{i}_mat"].tobytes(), abi.MATERIAL_DTYPE).copy()
+        out = np.zeros((len(p), 3))
+        L.or_texture_albedos(abi.ptr(m), abi.ptr(p), abi.ptr(uv), len(p), abi.ptr(out))
+        assert np.array_equal(out, g[f"tex{i}_out"]), i
+        i += 1
+    assert i == 12
+
+
+def test_oracle_lens_texture_images(golden):
+    """PT and Whitted images of the lens-and-texture scene under the round,
+    anamorphic (1.8) and widened (0.55) thin lenses: equal to the reference's
+    byte for byte."""
+    import hashlib
+    g = golden["lens_texture"]
+    hs = scenes.lens_texture_scene()
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    sha = hashlib.sha256(prims.tobytes() + attrs.tobytes() + mats.tobytes()).hexdigest()
+    assert sha == g["prims_sha"].tobytes().decode(), "scene builder output changed"
+    o = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm).soa)
+    for name in scenes.LENS_CAMS:
+        cam = cam_of(g[f"{name}_cam"])
+        assert (cam.aperture, cam.focus_distance, cam.anamorphic_squeeze) == scenes.LENS_CAMS[name]
+        for tag, mode, spp, depth in (("pt", 0, 8, 5), ("whitted", 1, 4, 4)):
+            rgb, _, _, _ = o.render(cam, 96, 72, spp, depth, mode=mode)
+            assert np.array_equal(rgb, g[f"{name}_{tag}"]), (name, tag)
+
+
+def _basis(cam):
+    r = np.array(list(cam.horizontal))
+    u = np.array(list(cam.vertical))
+    return r / np.linalg.norm(r), u / np.linalg.norm(u)
+
+
+def _lens_cam(ap, fd, sq):
+    cam = HostScene().camera(1.0, eye=[-18.0, 6.0, 22.0], at=[0.5, 1.0, -0.5], up=[0, 1, 0], vfov=35.0)
+    cam.aperture, cam.focus_distance, cam.anamorphic_squeeze = ap, fd, sq
+    return cam
+
+
+def test_oracle_lens_closed_forms():
+    """The reference's own thin-lens expectations (tests/test_dof.c:87-260,
+    tests/test_anamorphic.c:32-80), on the oracle."""
+    grid = np.array([((i + 0.5) / 16, (j + 0.5) / 16, (k + 0.5) / 8, 1 - (k + 0.5) / 8)
+                     for i in range(16) for j in range(16) for k in range(8)])
+    pin = _lens_cam(0.0, 20.0, 1.0)
+    pos = np.array(list(pin.position))
+    fwd = np.cross(*[np.array(list(x)) for x in (pin.vertical, pin.horizontal)])
+    fwd /= np.linalg.norm(fwd)
+    base = lens_rays(pin, grid)
+    # aperture 0 and negative apertures are the pinhole ray, bit for bit; the
+    # pinhole ray leaves the eye through the viewport point
+    assert np.array_equal(lens_rays(_lens_cam(-3.0, 20.0, 1.0), grid), base)
+    assert (base[:, :3] == pos).all()
+    ll, hor, ver = (np.array(list(x)) for x in (pin.lower_left, pin.horizontal, pin.vertical))
+    tgt = ll + grid[:, :1] * hor + grid[:, 1:2] * ver - pos
+    assert np.abs(base[:, 3:] - tgt / np.linalg.norm(tgt, axis=1, keepdims=True)).max() < 1e-14
+    # lens disc: within the aperture radius, in the camera plane, reaching its rim
+    lens = _lens_cam(0.4, 30.0, 1.0)
+    disc = np.array([(0.5, 0.5, (i + 0.5) / 64, (j + 0.5) / 64) for i in range(64) for j in range(64)])
+    off = lens_rays(lens, disc)[:, :3] - pos
+    rad = np.linalg.norm(off, axis=1)
+    assert (rad <= 0.4 + 1e-12).all() and np.abs(off @ fwd).max() <= 1e-12
+    assert rad.max() > 0.95 * 0.4
+    # r1 = 0 is the lens centre: the eye itself, and the pinhole direction
+    c0 = lens_rays(lens, np.array([(0.6, 0.4, 0.0, 0.0), (0.2, 0.8, 0.0, 0.5)]))
+    assert (c0[:, :3] == pos).all()
+    assert np.abs(c0[0, 3:] - lens_rays(pin, np.array([(0.6, 0.4, 0.0, 0.0)]))[0, 3:]).max() < 1e-12
+    # every lens ray passes through the pinhole ray's point at the focus distance
+    for fd, fd_eff in ((17.5, 17.5), (0.0, 1.0), (-4.0, 1.0)):
+        cam = _lens_cam(0.3, fd, 1.0)
+        q = np.array([(i / 8, j / 8, 0.37, 0.61) for i in range(1, 8) for j in range(1, 8)])
+        pr, lr = lens_rays(pin, q), lens_rays(cam, q)
+        focal = pos + pr[:, 3:] * fd_eff
+        to_f = focal - lr[:, :3]
+        cr = np.linalg.norm(np.cross(to_f, lr[:, 3:]), axis=1)
+        assert (cr <= 1e-9 * (1 + np.linalg.norm(to_f, axis=1))).all(), fd
+        assert ((to_f * lr[:, 3:]).sum(1) > 0).all()
+    # anamorphic squeeze: the horizontal lens offset divided by the squeeze,
+    # the vertical one unchanged; squeeze <= 0 or within 1e-12 of 1 is round
+    rt, up = _basis(pin)
+    sph = lens_rays(_lens_cam(0.5, 5.0, 1.0), grid)
+    for sq in (2.0, 0.55, 1.8):
+        ana = lens_rays(_lens_cam(0.5, 5.0, sq), grid)
+        os_, oa = sph[:, :3] - pos, ana[:, :3] - pos
+        assert np.abs(oa @ up - os_ @ up).max() < 1e-12
+        assert np.abs((oa @ rt) * sq - os_ @ rt).max() < 1e-12
+        assert (np.abs(oa @ rt) <= 0.5 / sq + 1e-12).all()
+    for sq in (0.0, -1.0, 1.0 + 1e-13):
+        assert np.array_equal(lens_rays(_lens_cam(0.5, 5.0, sq), grid), sph), sq
+
+
+def _tex(kind, scale, a, b, albedo=(1.0, 1.0, 1.0)):
+    m = scenes.material(albedo=albedo, texture_kind=kind, texture_scale=scale, texture_color_a=a,
+                        texture_color_b=b)
+    return np.array([m], abi.MATERIAL_DTYPE)
+
+
+def tex_eval(m, pts, uvs=None):
+    L = ck.oracle()
+    p = np.ascontiguousarray(pts, np.float64)
+    uv = np.ascontiguousarray(np.zeros((len(p), 2)) if uvs is None else uvs, np.float64)
+    out = np.zeros((len(p), 3))
+    L.or_texture_albedos(abi.ptr(m), abi.ptr(p), abi.ptr(uv), len(p), abi.ptr(out))
+    return out
+
+
+def test_oracle_texture_closed_forms():
+    """The reference's own checker expectations (tests/test_texture.c:95-235)
+    and the uv checker's cell parity, on the oracle."""
+    A, B = (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    ck_ = _tex(abi.TEXTURE_CHECKER, 1.0, A, B)
+    cells = {(0.5, 0.5, 0.5): B, (1.5, 0.5, 0.5): A, (0.5, 1.5, 0.5): A, (0.5, 0.5, 1.5): A,
+             (1.5, 1.5, 1.5): A, (1.5, 1.5, 0.5): B, (-0.5, 0.5, 0.5): A}
+    out = tex_eval(ck_, list(cells))
+    assert np.array_equal(out, np.array(list(cells.values())))
+    # scale 0.5 halves the cell; scale 0 (and negative) falls back to 1
+    assert np.array_equal(tex_eval(_tex(abi.TEXTURE_CHECKER, 0.5, A, B), [(0.5, 0.5, 0.5)])[0], A)
+    for s in (0.0, -2.0):
+        assert np.array_equal(tex_eval(_tex(abi.TEXTURE_CHECKER, s, A, B), [(0.5, 0.5, 0.5), (1.5, 0.5, 0.5)]),
+                              np.array([B, A]))
+    # the base albedo tints the pattern
+    tint = _tex(abi.TEXTURE_CHECKER, 1.0, (0.5, 0.5, 0.5), (1.0, 1.0, 1.0), albedo=(0.5, 1.0, 0.25))
+    assert np.array_equal(tex_eval(tint, [(1.5, 0.5, 0.5)])[0], (0.25, 0.5, 0.125))
+    # uv checker: colour a on cells of odd floor(u s) + floor(v s), else b;
+    # independent of the hit point; NONE returns the albedo unchanged
+    uvc = _tex(abi.TEXTURE_UV_CHECKER, 4.0, A, B)
+    uv = np.array([(0.1, 0.1), (0.3, 0.1), (0.3, 0.3), (-0.1, 0.1), (0.9, 0.6)])
+    pts = np.random.default_rng(3).uniform(-9, 9, (len(uv), 3))
+    assert np.array_equal(tex_eval(uvc, pts, uv), np.array([B, A, B, A, A]))
+    none = _tex(abi.TEXTURE_NONE, 1.0, A, B, albedo=(0.3, 0.6, 0.9))
+    assert np.array_equal(tex_eval(none, pts, uv), np.tile((0.3, 0.6, 0.9), (len(uv), 1)))
