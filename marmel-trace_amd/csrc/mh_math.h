@@ -37,8 +37,15 @@ template <typename R> __device__ __forceinline__ V3<R> reflect(V3<R> i, V3<R> n)
     R d = dot(i, n);
     return mk(i.x - R(2) * d * n.x, i.y - R(2) * d * n.y, i.z - R(2) * d * n.z);
 }
+// Hit point o + t d, each product rounded before the sum (never a fused
+// multiply-add), as the reference's double code rounds it. On a surface
+// that lies on a coordinate plane (a ground plane y = 0, a box face) the
+// rounded sum is then exactly the plane's coordinate for almost every ray,
+// as in the reference; a fused o.y + t d.y lands on either side of it half
+// the time, which flips floor() in the 3D checker texture.
 template <typename R> __device__ __forceinline__ V3<R> at(V3<R> o, V3<R> d, R t)
 {
+#pragma clang fp contract(off)
     return mk(o.x + t * d.x, o.y + t * d.y, o.z + t * d.z);
 }
 template <typename R> __device__ __forceinline__ bool finite3(V3<R> v)
