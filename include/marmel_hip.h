@@ -448,6 +448,26 @@ int mh_render(struct mh_scene *s, const mh_camera *cam,
               const mh_render_params *p, unsigned char *rgb_out,
               float *linear_out);
 
+/* Largest image (width * height) mh_render_noalloc accepts. */
+#define MH_NOALLOC_MAX_PIXELS (1 << 21)
+
+/* mh_render without any device allocation. No reference counterpart: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). It runs the same
+ * path-tracing megakernel as mh_render (p->mode MH_MODE_PATHTRACE, p->engine
+ * MH_ENGINE_MEGAKERNEL, fp32 or fp64) and writes the image into the caller's
+ * DEVICE buffers: d_rgb_out (w*h*3 bytes, required) and d_linear_out (w*h*3
+ * floats, may be NULL). Its scratch -- the ray counter and, when
+ * d_linear_out is NULL, the linear framebuffer -- is one fixed array per GPU
+ * (24 MiB + 256 B, held until the process ends), so w*h must not exceed
+ * MH_NOALLOC_MAX_PIXELS (else code 2). Calls that overlap on one GPU share
+ * that array and race: issue them one at a time per GPU. Synchronous on the
+ * scene's stream; the scene's own framebuffer is left untouched. Returns 4
+ * for Whitted or wavefront params, other codes as mh_render. */
+int mh_render_noalloc(struct mh_scene *s, const mh_camera *cam,
+                      const mh_render_params *p, unsigned char *d_rgb_out,
+                      float *d_linear_out);
+
 /* Copy the device framebuffer of the last mh_render on `s` to the host. */
 int mh_framebuffer_download(struct mh_scene *s, unsigned char *rgb_out,
                             float *linear_out);

@@ -194,6 +194,7 @@ private:
 } // namespace
 
 struct mh_scene {
+    int device = 0;                // the device current at creation (owns stream and memory)
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<DevBuf> bufs;
@@ -984,6 +985,11 @@ struct Layout {
 mh_scene* new_scene()
 {
     mh_scene* s = new mh_scene();
+    if (hipGetDevice(&s->device) != hipSuccess) {
+        fail(101, "hipGetDevice failed");
+        mh_scene_destroy(s);
+        return nullptr;
+    }
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         fail(101, "hipStreamCreate failed");
         mh_scene_destroy(s);
@@ -1674,6 +1680,101 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
         }
     }
     if (ovf) return fail(6, "Whitted work stack overflow");
+    return 0;
+}
+
+} // extern "C"
+
+// ---- render without allocation ---------------------------------------------
+//
+// The scratch of mh_render_noalloc is one file-scope device array: the ray
+// counter and overflow flag in its first 256 bytes, then a linear
+// framebuffer of MH_NOALLOC_MAX_PIXELS pixels. Each GPU holds its own copy;
+// host code reaches it only through the address hipGetSymbolAddress gives
+// with that GPU current (the array's name is a host-side stand-in).
+namespace {
+
+constexpr size_t NOALLOC_HEAD = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_HEAD + (size_t)MH_NOALLOC_MAX_PIXELS * 3 * sizeof(float);
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Clears the head of the scratch (a fill kernel rather than a memset, so a
+// caller's stream capture records no memset node).
+__global__ void k_noalloc_clear(unsigned long long* head)
+{
+    if (threadIdx.x < NOALLOC_HEAD / sizeof(unsigned long long)) head[threadIdx.x] = 0ull;
+}
+
+// Device address of `dev`'s scratch, looked up once per device with that
+// device current.
+int noalloc_scratch(int dev, void** out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(5, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        HIP_TRY(hipGetDevice(&cur));
+        if (cur != dev) HIP_TRY(hipSetDevice(dev));
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) HIP_TRY(hipSetDevice(cur));
+        HIP_TRY(e);
+        if (!ptr) return fail(100, "hipGetSymbolAddress gave no address for the scratch");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int mh_render_noalloc(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsigned char* d_rgb_out,
+                      float* d_linear_out)
+{
+    if (!s || !cam || !p || !d_rgb_out) return fail(1, "NULL argument");
+    if (p->width <= 0 || p->height <= 0) return fail(2, "non-positive image size %dx%d", p->width, p->height);
+    if (p->spp < 1 || p->max_depth < 0) return fail(3, "bad spp %d / depth %d", p->spp, p->max_depth);
+    if ((long long)p->width * p->height > (long long)MH_NOALLOC_MAX_PIXELS)
+        return fail(2, "image of %lld pixels exceeds the no-allocation limit of %d pixels",
+                    (long long)p->width * p->height, MH_NOALLOC_MAX_PIXELS);
+    if (p->sample_offset < 0) return fail(3, "negative sample offset");
+    if (p->mode != MH_MODE_PATHTRACE || p->engine != MH_ENGINE_MEGAKERNEL)
+        return fail(4, "mh_render_noalloc runs the path-tracing megakernel only");
+    if (int wrc = check_width(p->bvh_width, s->v32)) return wrc;
+    void* scr = nullptr;
+    if (int rc = noalloc_scratch(s->device, &scr)) return rc;
+    unsigned long long* d_rays = (unsigned long long*)scr;
+    int* d_ovf = (int*)((char*)scr + sizeof(unsigned long long));
+    float* lin = d_linear_out ? d_linear_out : (float*)((char*)scr + NOALLOC_HEAD);
+    LaunchOut lo{lin, d_rgb_out, d_rays, d_ovf};
+    hipLaunchKernelGGL(k_noalloc_clear, dim3(1), dim3(64), 0, s->stream, (unsigned long long*)scr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+    if (p->precision == MH_PREC_FP64)
+        HIP_TRY(launch_pathtrace<double>(with_width(s->v64, p->bvh_width), conv_cam<double>(*cam), p->width,
+                                         p->height, p->spp, p->sample_offset, p->max_depth, lo, 256, s->stream));
+    else
+        HIP_TRY(launch_pathtrace<float>(with_width(s->v32, p->bvh_width), conv_cam<float>(*cam), p->width,
+                                        p->height, p->spp, p->sample_offset, p->max_depth, lo, 256, s->stream));
+    HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+    unsigned long long rays = 0;
+    HIP_TRY(hipMemcpyAsync(&rays, d_rays, sizeof rays, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    memset(&g_timing, 0, sizeof g_timing);
+    g_timing.kernel_ms = g_timing.total_ms = ms_between(s->ev[0], s->ev[1]);
+    g_timing.samples = (int64_t)p->width * p->height * p->spp;
+    g_timing.rays = (int64_t)rays;
+    g_seconds = g_timing.kernel_ms * 1e-3;
+    g_total_samples = (unsigned long long)p->width * (unsigned long long)p->height * (unsigned long long)p->spp;
+    g_max_samples = p->spp;
+    memset(&g_ktimes, 0, sizeof g_ktimes);
     return 0;
 }
 

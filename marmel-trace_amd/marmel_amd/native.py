@@ -104,6 +104,7 @@ def hip():
     _sig(L, "mh_flare_apply", C.c_int, [P, C.c_int, C.c_int, C.POINTER(abi.FlareParams)])
     _sig(L, "mh_scene_flare", C.c_int, [P, C.POINTER(abi.FlareParams)])
     _sig(L, "mh_framebuffer_download", C.c_int, [P, P, P])
+    _sig(L, "mh_render_noalloc", C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P, P])
     _sig(L, "mh_scene_set_materials", C.c_int, [P, P, C.c_int])
     _sig(L, "mh_scene_set_sky", C.c_int, [P, C.POINTER(abi.Sky)])
     _sig(L, "mh_scene_set_fog", C.c_int, [P, C.POINTER(abi.Fog)])

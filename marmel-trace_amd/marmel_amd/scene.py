@@ -284,6 +284,18 @@ class GpuScene:
         self._check(rc)
         return rgb, lin
 
+    def render_noalloc(self, cam, width, height, spp, depth, d_rgb, d_linear=None,
+                       precision=abi.PREC_FP32, sample_offset=0, bvh_width=0):
+        """mh_render_noalloc: the path-tracing megakernel into the caller's
+        device buffers (integer addresses; d_linear may be None). Returns the
+        code; the library allocates no device memory for the call."""
+        p = abi.RenderParams(width=width, height=height, spp=spp, max_depth=depth,
+                             mode=abi.MODE_PATHTRACE, precision=precision,
+                             sample_offset=sample_offset, engine=abi.ENGINE_MEGAKERNEL,
+                             bvh_width=bvh_width)
+        return self._L.mh_render_noalloc(self._h, C.byref(cam), C.byref(p), C.c_void_p(d_rgb),
+                                         C.c_void_p(d_linear) if d_linear else None)
+
     def pathtrace_render(self, cam, width, height, spp, depth):
         """Drop-in form of reference pathtrace_render: returns (code, rgb)."""
         rgb = np.zeros((max(height, 1), max(width, 1), 3), np.uint8)

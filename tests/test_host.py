@@ -37,6 +37,7 @@ def test_gpu_entry_points_reject_null_arguments():
     L = native.hip()
     assert L.mh_scene_create(None, None) == 1
     assert L.mh_render(None, None, None, None, None) == 1
+    assert L.mh_render_noalloc(None, None, None, None, None) == 1
     assert L.mh_pathtrace_render(None, None, 10, 10, 1, 1, None) == 1
     assert L.mh_render_image(None, None, 10, 10, 1, 1, None) == 1
     assert L.mh_batch_create(None) == 1
