@@ -25,9 +25,32 @@ template <typename R> struct PD {
     R r0, r1;
 };
 
+// Load a 16-byte-aligned record through a global-address-space pointer, in
+// 16-byte vectors. Scene arrays are device allocations, but a pointer read
+// from a device table (the batched engine's scenes) is generic to the
+// compiler, which then fetches with flat instructions that also wait on LDS
+// traffic (the traversal stack).
+typedef unsigned int mh_u4v __attribute__((ext_vector_type(4)));
+template <typename T> __device__ __forceinline__ T ldg(const T* p)
+{
+    static_assert(sizeof(T) % 16 == 0, "16-byte records");
+    T r;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __attribute__((address_space(1))) mh_u4v* s = (const __attribute__((address_space(1))) mh_u4v*)p;
+    mh_u4v v[sizeof(T) / 16];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 16); ++i) v[i] = s[i];
+    __builtin_memcpy(&r, v, sizeof(T));
+#else
+    r = *p;
+#endif
+    return r;
+}
+
 __device__ __forceinline__ PD<float> load_prim(const P32* p)
 {
-    float4 a = p->a, b = p->b, c = p->c;
+    P32 r = ldg(p);
+    float4 a = r.a, b = r.b, c = r.c;
     PD<float> d;
     int kf = __float_as_int(a.w);
     d.kind = kf & 0xff;
@@ -732,7 +755,7 @@ __device__ int closest_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tm
                 }
                 held = false;
                 if (ref >= 0) {
-                    DNode4 n = sc.nodes4[ref];
+                    DNode4 n = ldg(&sc.nodes4[ref]);
                     int r[4], nh;
                     float t[4];
                     children4(sl, n, tmin, closest, r, t, &nh);
@@ -764,7 +787,7 @@ __device__ int closest_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tm
                 int ref = st.pop(&te);
                 if (best != INT_MIN && (R)te > closest) continue;
                 if (ref >= 0) {
-                    DNode4 n = sc.nodes4[ref];
+                    DNode4 n = ldg(&sc.nodes4[ref]);
                     int r[4], nh;
                     float t[4];
                     children4(sl, n, tmin, closest, r, t, &nh);
@@ -805,7 +828,7 @@ __device__ int closest_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tm
                 }
                 held = false;
                 if (ref >= 0) {
-                    DNode n = sc.nodes[ref];
+                    DNode n = ldg(&sc.nodes[ref]);
                     bool hl, hr;
                     float tl, tr;
                     sl.test(n, tmin, closest, &hl, &hr, &tl, &tr);
@@ -836,7 +859,7 @@ __device__ int closest_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tm
                 int ref = st.pop(&te);
                 if (best != INT_MIN && (R)te > closest) continue;
                 if (ref >= 0) {
-                    DNode n = sc.nodes[ref];
+                    DNode n = ldg(&sc.nodes[ref]);
                     bool hl, hr;
                     float tl, tr;
                     sl.test(n, tmin, closest, &hl, &hr, &tl, &tr);
@@ -920,7 +943,7 @@ __device__ bool any_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin,
             float te;
             int ref = st.pop(&te);
             if (ref >= 0) {
-                DNode4 n = sc.nodes4[ref];
+                DNode4 n = ldg(&sc.nodes4[ref]);
 #if MH_ANY_SORT
                 int r[4], nh;
                 float t[4];
@@ -957,7 +980,7 @@ __device__ bool any_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin,
         float te;
         int ref = st.pop(&te);
         if (ref >= 0) {
-            DNode n = sc.nodes[ref];
+            DNode n = ldg(&sc.nodes[ref]);
             bool hl, hr;
             float tl, tr;
             sl.test(n, tmin, tmax, &hl, &hr, &tl, &tr);

@@ -578,6 +578,170 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
     count_rays_block(rays, nq);
 }
 
+// Idle lanes of a wave at which the any-hit shadow kernel stops traversing to
+// hand its idle lanes new rays (dynamic fetch); 0 selects k_wf_shadow's
+// packed batches of 64 instead. Swept on the 1080p terrain frame (shadow ms
+// per frame): 0 -> 278, 8 -> 259, 16 -> 242, 24 -> 236, 32 -> 238,
+// 40 -> 240, 48 -> 233, 56 -> 230. A refill round stalls the wave's busy
+// lanes (queue scan, ray loads), so it pays to refill late.
+#ifndef MH_SHADOW_REFILL
+#define MH_SHADOW_REFILL 56
+#endif
+
+// Any-hit shadow queries (no transmissive material) with dynamic fetch. Each
+// lane owns one ray at a time and walks the binary tree one node (or one
+// leaf) per step. A wave packs the live entries of its 64-entry chunks into
+// an LDS queue (ballot + mbcnt, as k_wf_shadow does); whenever
+// MH_SHADOW_REFILL of its lanes have finished and queued or unscanned work
+// remains, it leaves the step loop and gives each idle lane the next queued
+// ray. Lanes that finish early therefore take new rays instead of idling
+// until the wave's longest traversal ends. Same visibility bits as
+// k_wf_shadow<., false> (the query result does not depend on the order).
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, WfState st, unsigned long long* rays,
+                                                           WfMulti mj)
+{
+    // per-lane stack of node refs: the top K in LDS (lane-interleaved, slot
+    // k of a lane at my[k * blockDim.x]), older ones in a private overflow
+    // array; held in scalars so that restarting it for a new ray is three
+    // register writes
+    constexpr int K = MH_SHADOW_ANY_LDS;
+    static_assert(K > 0 && (K & (K - 1)) == 0 && K < MH_STACK, "LDS stack depth must be a power of two");
+    __shared__ int lds_ref[K * 256];
+    __shared__ uint32_t wq[4][128];
+    int* const my = lds_ref + threadIdx.x;
+    const int bs = (int)blockDim.x;
+    int ovf[MH_STACK - K];
+    int sp = 0, base = 0, psp = 0;
+    auto push = [&](int r) {
+        if (sp == K) {
+            ovf[psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1] = my[base * bs];
+            ++psp;
+            base = (base + 1) & (K - 1);
+            --sp;
+        }
+        my[((base + sp) & (K - 1)) * bs] = r;
+        ++sp;
+    };
+    auto pop = [&]() -> int {
+        if (sp > 0) {
+            --sp;
+            return my[((base + sp) & (K - 1)) * bs];
+        }
+        --psp;
+        return ovf[psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1];
+    };
+
+    const uint32_t n = st.cnt[2];
+    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+    uint32_t* q = wq[wave];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t next = blockIdx.x * blockDim.x + (uint32_t)wave * 64u;  // next chunk to scan (wave-uniform)
+    uint32_t held = 0;                                                // queued entries (wave-uniform)
+    int nq = 0;
+    bool busy = false;
+    Slab<float> sl;
+    V3<float> o = zero3<float>(), d = zero3<float>();
+    float tmax = 0.f;
+    uint32_t slot = 0, bit = 0;
+    const DNode* nodes = nullptr;
+    const P32* prims = nullptr;
+    for (;;) {
+        const uint64_t idle = __ballot(!busy);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        // top up the queue to at least one entry per idle lane (q holds < 128:
+        // held < nidle <= 64 before each chunk adds at most 64)
+        while (held < nidle && next < n) {
+            uint32_t i = next + (uint32_t)lane;
+            bool live = i < n && f2u(st.sh1[i].w) != SH_NONE;
+            uint64_t m = __ballot(live);
+            uint32_t at = held + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (live) q[at] = i;
+            held += (uint32_t)__popcll(m);
+            next += stride;
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (nidle == 64u && held == 0u) break;  // the loop above ran out of chunks
+        // the next queued rays to the idle lanes, in lane order
+        const uint32_t take = held < nidle ? held : nidle;
+        if (take > 0u) {
+            uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            if (!busy && rank < take) {
+                uint32_t i = q[rank];
+                float4 b = st.sh1[i], a = st.sh0[i];
+                uint32_t tag = f2u(b.w);
+                slot = tag >> 4;
+                bit = 1u << (tag & 15u);
+                const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
+                o = mk(a.x, a.y, a.z);
+                d = unit_dir(mk(b.x, b.y, b.z));
+                tmax = a.w;
+                ++nq;
+                // planes first (never in the tree), as any_hit does
+                bool occ = false;
+                for (int p = 0; p < sc.n_planes && !occ; ++p)
+                    occ = occ_prim(load_prim(&sc.planes[p]), o, d, 1e-3f, tmax);
+                if (!occ && sc.root_ref != INT_MIN) {
+                    nodes = sc.nodes;
+                    prims = sc.prims;
+                    sl.init(o, d);
+                    sp = base = psp = 0;
+                    push(sc.root_ref);
+                    busy = true;
+                } else if (!occ) {
+                    atomicOr(&st.vis[slot], bit);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            // move the rest (fewer than 64) to the front
+            uint32_t rest = held - take;
+            uint32_t v = (uint32_t)lane < rest ? q[take + (uint32_t)lane] : 0u;
+            __builtin_amdgcn_wave_barrier();
+            if ((uint32_t)lane < rest) q[lane] = v;
+            __builtin_amdgcn_wave_barrier();
+            held = rest;
+        }
+        const bool more = held > 0u || next < n;  // wave-uniform
+        // traversal steps until the wave is done, or enough lanes are idle
+        // while work remains
+        for (;;) {
+            if (busy) {
+                int ref = pop();
+                bool occ = false;
+                if (ref >= 0) {
+                    DNode nd = ldg(&nodes[ref]);
+                    bool hl, hr;
+                    float tl, tr;
+                    sl.test(nd, 1e-3f, tmax, &hl, &hr, &tl, &tr);
+                    if (hl && hr) {
+                        bool lfirst = tl <= tr;
+                        push(lfirst ? nd.c3.y : nd.c3.x);
+                        push(lfirst ? nd.c3.x : nd.c3.y);
+                    } else if (hl) {
+                        push(nd.c3.x);
+                    } else if (hr) {
+                        push(nd.c3.y);
+                    }
+                } else {
+                    int code = ~ref;
+                    int first = code >> 3, cnt = (code & 7) + 1;
+                    for (int j = 0; j < cnt && !occ; ++j) occ = occ_prim(load_prim(&prims[first + j]), o, d, 1e-3f, tmax);
+                }
+                if (occ || (sp == 0 && psp == 0)) {
+                    if (!occ) atomicOr(&st.vis[slot], bit);
+                    busy = false;
+                }
+            }
+            const uint64_t b = __ballot(busy);
+            if (b == 0ull) break;
+            if (more && 64u - (uint32_t)__popcll(b) >= (uint32_t)MH_SHADOW_REFILL) break;
+        }
+    }
+    count_rays_block(rays, nq);
+}
+
 // Radiance in 2^-24 fixed point. A sample is capped at 2^30 (no physical
 // estimate here comes near it), so 2^10 samples cannot overflow a pixel sum.
 __device__ __forceinline__ unsigned long long to_fixed(float v)
