@@ -2149,8 +2149,13 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
         if (scenes[i]->v32.fog_kernel) feat |= WF_FOG;
     }
     // width 4: every scene whose 4-wide tree was built; width 2: binary
+    // (the kernels a launch runs depend on it: WF_WIDE)
     auto set_width = [&](int w) {
-        for (int i = 0; i < n_scenes; ++i) views[i] = with_width(scenes[i]->v32, w == 2 ? 2 : 0);
+        feat &= ~WF_WIDE;
+        for (int i = 0; i < n_scenes; ++i) {
+            views[i] = with_width(scenes[i]->v32, w == 2 ? 2 : 0);
+            if (views[i].bvh4) feat |= WF_WIDE;
+        }
     };
     set_width(t.bvh_width);
     std::vector<BJob> bj((size_t)n_jobs);

@@ -120,8 +120,9 @@ hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st);
 hipError_t wf_enqueue_end(const WfState& ws, int npix, int spp, const LaunchOut& out, hipStream_t st);
 
 // Kernel variants a batch needs (bit set): some scene has a transmissive
-// material (shadow rays walk through it) / has fog or marched water.
-enum { WF_TRANS = 1, WF_FOG = 2 };
+// material (shadow rays walk through it) / has fog or marched water / is
+// traversed on its 4-wide tree.
+enum { WF_TRANS = 1, WF_FOG = 2, WF_WIDE = 4 };
 
 // Batched engine: one batch of `pool` paths drawn from all jobs, bounce
 // sequence to the deepest job's depth. `feat`: WF_* bits over the batch's scenes.

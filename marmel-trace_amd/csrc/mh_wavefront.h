@@ -260,6 +260,172 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState
     }
 }
 
+// Idle lanes of a wave at which the closest-hit extend kernel hands them new
+// rays (dynamic fetch); 0 selects k_wf_extend. Binary trees only (a launch
+// whose scenes use the 4-wide tree runs k_wf_extend). Measured on the 1080p
+// terrain frame (extend ms per frame): 0 -> 77.2, 32 -> 61.7, 48 -> 59.9,
+// 56 -> 61.0.
+#ifndef MH_EXTEND_REFILL
+#define MH_EXTEND_REFILL 48
+#endif
+
+// Nearest hit of every queued path with dynamic fetch, on the binary tree:
+// the search of closest_hit<float> (planes first, then the tree with the
+// nearer hit child held in a register and the farther one stacked with its
+// entry distance, strict > culling, lowest prim index on equal t), one node
+// or leaf per step per lane. The queue is compacted, so a wave's entries are
+// its 64-entry chunks (wave w, w + waves, ...) in order, addressed by a
+// wave-uniform position; whenever MH_EXTEND_REFILL lanes have finished and
+// entries remain, the idle lanes take the next ones. Same hits as
+// k_wf_extend.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, WfState st, int which, WfMulti mj)
+{
+    constexpr int K = MH_EXTEND_LDS;
+    static_assert(K > 0 && (K & (K - 1)) == 0 && K < MH_STACK, "LDS stack depth must be a power of two");
+    __shared__ int lds_ref[K * 256];
+    __shared__ float lds_t[K * 256];
+    int* const myr = lds_ref + threadIdx.x;
+    float* const myt = lds_t + threadIdx.x;
+    const int bs = (int)blockDim.x;
+    int ovr[MH_STACK - K];
+    float ovt[MH_STACK - K];
+    int sp = 0, base = 0, psp = 0;
+    auto push = [&](int r, float t) {
+        if (sp == K) {
+            int j = psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1;
+            ovr[j] = myr[base * bs];
+            ovt[j] = myt[base * bs];
+            ++psp;
+            base = (base + 1) & (K - 1);
+            --sp;
+        }
+        int k = ((base + sp) & (K - 1)) * bs;
+        myr[k] = r;
+        myt[k] = t;
+        ++sp;
+    };
+    auto pop = [&](float* t) -> int {
+        if (sp > 0) {
+            --sp;
+            int k = ((base + sp) & (K - 1)) * bs;
+            *t = myt[k];
+            return myr[k];
+        }
+        --psp;
+        int j = psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1;
+        *t = ovt[j];
+        return ovr[j];
+    };
+
+    const uint32_t n = st.cnt[which];
+    const uint32_t* q = st.q[which];
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t wg = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = (int)__lane_id();
+    // queue index of the wave's p-th entry: chunks wg, wg + nwaves, ...
+    auto entry = [&](uint32_t p) -> uint32_t { return (wg + (p >> 6) * nwaves) * 64u + (p & 63u); };
+    uint32_t pos = 0;  // wave-uniform
+    const float tmin = 1e-4f;
+    bool busy = false, held = false;
+    Slab<float> sl;
+    V3<float> o = zero3<float>(), d = zero3<float>();
+    float closest = 0.f;
+    int best = INT_MIN, bprim = 0x7fffffff, ref = 0;
+    uint32_t slot = 0;
+    const DNode* nodes = nullptr;
+    const P32* prims = nullptr;
+    for (;;) {
+        const uint64_t idle = __ballot(!busy);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (nidle == 64u && entry(pos) >= n) break;
+        if (entry(pos) < n) {
+            uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            uint32_t i = entry(pos + rank);
+            if (!busy && i < n) {
+                slot = q[i];
+                const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
+                float4 a = st.p0[slot], b = st.p1[slot];
+                o = mk(a.x, a.y, a.z);
+                d = mk(b.x, b.y, b.z);
+                closest = 1e30f;
+                best = INT_MIN;
+                bprim = 0x7fffffff;
+                for (int k = 0; k < sc.n_planes; ++k) {
+                    PD<float> p = load_prim(&sc.planes[k]);
+                    float t;
+                    if (hit_plane<float, false>(p, o, d, tmin, closest, &t, nullptr)) {
+                        int pi = sc.plane_index[k];
+                        if (best == INT_MIN || t < closest || (t == closest && pi < bprim)) {
+                            closest = t;
+                            bprim = pi;
+                            best = -1 - k;
+                        }
+                    }
+                }
+                if (sc.root_ref != INT_MIN) {
+                    nodes = sc.nodes;
+                    prims = sc.prims;
+                    sl.init(o, d);
+                    sp = base = psp = 0;
+                    ref = sc.root_ref;
+                    held = true;
+                    busy = true;
+                } else {
+                    st.hit[slot] = make_float2(closest, __int_as_float(best));
+                }
+            }
+            pos += nidle;
+        }
+        const bool more = entry(pos) < n;  // wave-uniform
+        for (;;) {
+            if (busy) {
+                bool work = true;
+                if (!held) {
+                    float te;
+                    ref = pop(&te);
+                    work = !(best != INT_MIN && te > closest);
+                }
+                held = false;
+                if (work) {
+                    if (ref >= 0) {
+                        DNode nd = ldg(&nodes[ref]);
+                        bool hl, hr;
+                        float tl, tr;
+                        sl.test(nd, tmin, closest, &hl, &hr, &tl, &tr);
+                        bool lfirst = hl && (!hr || tl <= tr);
+                        if (hl && hr) push(lfirst ? nd.c3.y : nd.c3.x, lfirst ? tr : tl);
+                        ref = lfirst ? nd.c3.x : nd.c3.y;
+                        held = hl || hr;
+                    } else {
+                        int code = ~ref;
+                        int first = code >> 3, cnt = (code & 7) + 1;
+                        for (int j = 0; j < cnt; ++j) {
+                            PD<float> p = load_prim(&prims[first + j]);
+                            float t;
+                            if (hit_prim<float, false>(p, nullptr, o, d, tmin, closest, &t, nullptr)) {
+                                if (best == INT_MIN || t < closest || (t == closest && p.prim < bprim)) {
+                                    closest = t;
+                                    bprim = p.prim;
+                                    best = first + j;
+                                }
+                            }
+                        }
+                    }
+                }
+                if (!held && sp == 0 && psp == 0) {
+                    st.hit[slot] = make_float2(closest, __int_as_float(best));
+                    busy = false;
+                }
+            }
+            const uint64_t b = __ballot(busy);
+            if (b == 0ull) break;
+            if (more && 64u - (uint32_t)__popcll(b) >= (uint32_t)MH_EXTEND_REFILL) break;
+        }
+    }
+}
+
 // add the visible pending estimates of the previous bounce, in estimate order
 __device__ __forceinline__ V3<float> resolve_pending(const WfState& st, uint32_t slot, V3<float> L)
 {

@@ -18,6 +18,12 @@ hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st)
     return hipGetLastError();
 }
 
+// The extension kernel: dynamic fetch on binary trees.
+template <bool MULTI> static auto extend_kernel(int feat)
+{
+    return (MH_EXTEND_REFILL > 0 && !(feat & WF_WIDE)) ? k_wf_extend_refill<MULTI> : k_wf_extend<MULTI>;
+}
+
 // The any-hit shadow kernel (no transmissive material in the launch).
 template <bool MULTI> static auto shadow_any()
 {
@@ -51,7 +57,7 @@ static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<floa
     for (int b = 0; b <= job.depth; ++b) {
         int which = b & 1;
         MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, ws.cnt, which ^ 1));
-        MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(k_wf_extend<MULTI>, G, B, 0, st, sc, ws, which, mj));
+        MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(extend_kernel<MULTI>(feat), G, B, 0, st, sc, ws, which, mj));
         MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, 0, st, sc, ws, job, which, b, rays, mj));
         MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
         MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MULTI>, G, B, 0, st, ws, mj));
@@ -64,7 +70,7 @@ hipError_t wf_enqueue_batch(const SceneView<float>& sc, const CamView<float>& ca
                             const WfState& ws, unsigned long long* rays, int grid, hipStream_t st, KernelClock* kc)
 {
     WfMulti none{};
-    int feat = (sc.has_transmissive ? WF_TRANS : 0) | (sc.fog_kernel ? WF_FOG : 0);
+    int feat = (sc.has_transmissive ? WF_TRANS : 0) | (sc.fog_kernel ? WF_FOG : 0) | (sc.bvh4 ? WF_WIDE : 0);
     return enqueue_bounces<false>(sc, cam, job, ws, none, rays, grid, 256, feat, st, kc);
 }
 
@@ -108,7 +114,7 @@ hipError_t wf_stream_step(const WfMulti& mj, const WfState& ws, int which, unsig
     MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_refill, G, B, 0, st, ws, mj, which));
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_refill_commit, dim3(1), dim3(64), 0, st, ws, mj));
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, ws.cnt, which ^ 1));
-    MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(k_wf_extend<true>, G, B, 0, st, sc, ws, which, mj));
+    MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(extend_kernel<true>(feat), G, B, 0, st, sc, ws, which, mj));
     MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, 0, st, sc, ws, job, which, 0, rays, mj));
     MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
     MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<true>, G, B, 0, st, ws, mj));
