@@ -109,6 +109,43 @@ def cpu_baseline(o, cam, scale, threads):
     return samples / sec / 1e6, f"{w}x{h} downscaled frame of the same scene/camera, {SPP} spp, depth {DEPTH}", sec
 
 
+def reference_baseline(hold, cam, scale, threads):
+    """The reference renderer itself (its pathtrace_render compiled from its
+    own sources into oracle/_ref by oracle/Makefile, pthread tile pool) on the
+    same downscaled frame; None when that library was not built. Its render
+    clock (CLOCK_MONOTONIC around the tile pool, reference
+    src/pathtrace.c:1270-1340) excludes the scene and BVH build."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import checkers as ck
+    if not ck.have_ref():
+        return None
+    t0 = time.time()
+    r = ck.RefScene(soa=hold.soa)
+    log(f"bench: reference scene + BVH built in {time.time() - t0:.1f}s")
+    w, h = W // scale, H // scale
+    rgb, sec = r.render(cam, w, h, SPP, DEPTH, 0, threads=threads)
+    del r
+    return w * h * SPP / sec / 1e6, sec
+
+
+def host_cpus():
+    """nproc (what the reference sizes its pool by), the CPUs this process may
+    run on, and the cgroup CPU quota (cores) when one is set."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = nproc
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        pass
+    return nproc, aff, quota
+
+
 def adjoint_leg(reps):
     """Differentiable workload: 512 x 512, 32 spp, depth 4, 64 materials, L2
     loss vs the image of hidden seeded parameters; ms per gradient."""
@@ -243,6 +280,22 @@ def post_leg(g, cam, reps):
             "fixed_4spp_kernel_ms": round(fixed_ms, 2), "flare_ms": round(flare_ms, 3)}
 
 
+def hbm_calibration(mib):
+    """Two plain kernels of known traffic for scaling rocprofv3 FETCH_SIZE /
+    WRITE_SIZE readings: a device copy (torch copy_) and an elementwise
+    multiply (y = 2x), each reading and writing exactly `mib` MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    log(f"bench: calibration copy and multiply, each read {n * 4} bytes and wrote {n * 4} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,7 +323,12 @@ def main():
                     help="skip the adaptive Whitted + flare leg (steps after the framebuffer)")
     ap.add_argument("--only-post", action="store_true",
                     help="run only the adaptive Whitted + flare leg (for profiling its kernels)")
+    ap.add_argument("--hbm-calib", type=int, default=0, metavar="MIB",
+                    help="only run the copy / multiply kernels of MIB MiB that calibrate the HBM counters")
     args = ap.parse_args()
+    if args.hbm_calib > 0:
+        hbm_calibration(args.hbm_calib)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the path would shard by image tiles or sample ranges with one reduction", file=sys.stderr)
@@ -379,9 +437,10 @@ def main():
     if post:
         log(f"bench: post {post}")
 
-    # host threads: the GPU's share of the host (16 cores), as the reference
-    # caps its pool at min(nproc, 64) (pathtrace.c:1210-1234)
-    threads = min(os.cpu_count() or 1, 16)
+    # host threads as the reference sizes its pool: min(nproc, 64)
+    # (reference src/pathtrace.c:1210-1234)
+    nproc, affinity, quota = host_cpus()
+    threads = min(nproc, 64)
     cpu = None
     roof = None
     frame_gbs = None
@@ -389,10 +448,19 @@ def main():
     st, st_samples, st_size = traversal_stats(o, cam, args.stats_scale, threads)
     log(f"bench: traversal counts from a {st_size} oracle render: {st}")
     if not args.no_cpu:
-        cpu_msps, sample, sec = cpu_baseline(o, cam, args.cpu_scale, threads)
-        log(f"bench: cpu baseline {cpu_msps:.3f} Msamples/s on {threads} threads ({sec:.1f}s)")
-        cpu = {"value": round(cpu_msps, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
-               "sample": sample}
+        port_msps, sample, sec = cpu_baseline(o, cam, args.cpu_scale, threads)
+        log(f"bench: oracle (port) {port_msps:.3f} Msamples/s on {threads} threads ({sec:.1f}s); "
+            f"nproc {nproc}, affinity {affinity}, cgroup quota {quota}")
+        ref = reference_baseline(hold, cam, args.cpu_scale, threads)
+        cpu = {"value": round(port_msps, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+               "sample": sample, "nproc": nproc, "affinity_cpus": affinity, "cgroup_cpu_quota": quota,
+               "port_msamples_per_s": round(port_msps, 4)}
+        if ref is not None:
+            ref_msps, rsec = ref
+            log(f"bench: reference renderer {ref_msps:.3f} Msamples/s on {threads} threads ({rsec:.1f}s)")
+            cpu.update(value=round(ref_msps, 4), kind="reference",
+                       reference_msamples_per_s=round(ref_msps, 4),
+                       port_over_reference=round(port_msps / ref_msps, 3))
     scale = samples / st_samples
     frame_gbs = frame_bytes(st, st_samples) * scale / (mean_kernel_ms * 1e-3) / 1e9
     sh_ms, sh_n = ktimes.get("shadow", (0.0, 0))

@@ -563,11 +563,15 @@ typedef struct mh_batch mh_batch;
 int  mh_batch_create(mh_batch **out);
 void mh_batch_destroy(mh_batch *b);
 
+/* Samples per pixel of one batched job (pixel sums are 64-bit fixed point
+ * with each sample capped at 2^26). */
+#define MH_BATCH_MAX_SPP 16383
+
 /* Render n_jobs images of n_scenes uploaded scenes in one persistent
  * wavefront launch sequence (fp32). Outputs are concatenated in job order,
  * each job top-down RGB: rgb_out holds sum(w*h*3) bytes, linear_out the same
  * count of floats; either may be NULL. Return codes as mh_render; 4 when a
- * job's max_depth exceeds 254. */
+ * job's max_depth exceeds 254 or its spp exceeds MH_BATCH_MAX_SPP. */
 int mh_render_batch(mh_batch *b, struct mh_scene *const *scenes, int n_scenes,
                     const mh_job *jobs, int n_jobs, const mh_tune *tune,
                     unsigned char *rgb_out, float *linear_out);

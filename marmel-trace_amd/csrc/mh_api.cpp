@@ -2169,6 +2169,9 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
         if (j.spp < 1 || j.max_depth < 0 || j.sample_offset < 0)
             return fail(3, "job %d: bad spp %d / depth %d / offset %d", i, j.spp, j.max_depth, j.sample_offset);
         if (j.max_depth > 254) return fail(4, "job %d: depth %d above the batched engine's 254", i, j.max_depth);
+        if (j.spp > MH_BATCH_MAX_SPP)
+            return fail(4, "job %d: %d samples per pixel above the batched engine's %d per call (render more "
+                           "samples as calls with increasing sample_offset)", i, j.spp, MH_BATCH_MAX_SPP);
         BJob& d = bj[i];
         d.scene = j.scene;
         d.w = j.width;

@@ -908,11 +908,13 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     count_rays_block(rays, nq);
 }
 
-// Radiance in 2^-24 fixed point. A sample is capped at 2^30 (no physical
-// estimate here comes near it), so 2^10 samples cannot overflow a pixel sum.
+// Radiance in 2^-24 fixed point. A sample is capped at 2^26 (no physical
+// estimate here comes near it), i.e. 2^50 units, so a pixel sum of up to
+// MH_BATCH_MAX_SPP = 2^14 - 1 samples stays below 2^64 (mh_render_batch
+// rejects jobs of more samples).
 __device__ __forceinline__ unsigned long long to_fixed(float v)
 {
-    return __float2ull_rn(fminf(v, 1073741824.0f) * 16777216.0f);
+    return __float2ull_rn(fminf(v, 67108864.0f) * 16777216.0f);
 }
 
 template <bool MULTI>

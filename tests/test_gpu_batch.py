@@ -165,6 +165,7 @@ def test_batch_job_statistics_vs_oracle(world):
 
 @pytest.mark.parametrize("patch,code", [(dict(scene=5), 5), (dict(width=0), 2), (dict(height=-3), 2),
                                         (dict(spp=0), 3), (dict(depth=-1), 3), (dict(depth=300), 4),
+                                        (dict(spp=16384), 4),
                                         (dict(sample_offset=-1), 3)])
 def test_batch_error_codes(world, patch, code):
     hs, gs = world["spheres"]

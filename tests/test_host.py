@@ -190,3 +190,44 @@ def test_ppm_writer_layout(tmp_path):
     p = str(tmp_path / "x.ppm")
     assert nat.host().mt_write_ppm(p.encode(), abi.ptr(np.ascontiguousarray(rgb)), 3, 2) == 0
     assert open(p, "rb").read() == b"P6\n3 2\n255\n" + rgb.tobytes()
+
+
+def test_tree_deeper_than_the_traversal_stack_is_refused():
+    """Every traversal stack holds 64 entries and the 4-wide tree is used only
+    when 3 * its depth + 1 fits, so mh_scene_create refuses a binary tree of
+    more than 60 levels (code 5) before any device call; a tree of exactly 60
+    levels passes the check (it fails later only for want of a device)."""
+    L = native.hip()
+    for levels, refused in ((61, True), (60, False)):
+        n = levels  # a chain: interior node k has leaf k and interior k + 1
+        prims = np.zeros(n, abi.PRIM_DTYPE)
+        for i in range(n):
+            prims[i]["kind"] = abi.PRIM_SPHERE
+            prims[i]["p0"] = (float(i), 0.0, 0.0)
+            prims[i]["r0"] = 0.4
+        mats = np.zeros(1, abi.MATERIAL_DTYPE)
+        mats[0]["albedo"] = (0.5, 0.5, 0.5)
+        nodes = np.zeros(2 * n - 1, abi.NODE_DTYPE)
+        for k in range(n - 1):
+            nodes[k]["left"], nodes[k]["right"] = n - 1 + k, k + 1
+            nodes[n - 1 + k]["first"], nodes[n - 1 + k]["count"] = k, 1
+        nodes[n - 2]["right"] = 2 * n - 2   # the deepest interior node ends in two leaves
+        nodes[2 * n - 2]["first"], nodes[2 * n - 2]["count"] = n - 1, 1
+        for nd in nodes:
+            nd["bmin"] = (-1.0, -1.0, -1.0)
+            nd["bmax"] = (float(n), 1.0, 1.0)
+        leaf = np.arange(n, dtype=np.int32)
+        s = abi.SceneSoa(abi_version=abi.ABI_VERSION, n_prims=n, prims=abi.ptr(prims), attrs=None,
+                         n_materials=1, materials=abi.ptr(mats), sky=abi.Sky(), water_level=0.0,
+                         water_material=-1, n_nodes=len(nodes), nodes=abi.ptr(nodes), n_leaf_prims=n,
+                         leaf_prims=abi.ptr(leaf), n_planes=0, planes=None, fog=abi.fog_default(),
+                         ocean=abi.Ocean())
+        h = C.c_void_p()
+        rc = L.mh_scene_create(C.byref(s), C.byref(h))
+        msg = L.mh_last_error().decode()
+        if refused:
+            assert rc == 5 and "deeper than 60" in msg, (rc, msg)
+        else:
+            assert rc != 5, msg
+            if rc == 0:
+                L.mh_scene_destroy(h)
