@@ -24,6 +24,9 @@
 //
 // Built without FMA contraction so that both passes round identically and
 // take identical decisions.
+// box hits at o + t d in fp32 here (see mh_trace.h): the adjoint kernels sit at
+// the register ceiling, where the snap halved the two-pass kernel's occupancy
+#define MH_SNAP_BOX 0
 #include "mh_kernels.h"
 #include "mh_render.h"
 

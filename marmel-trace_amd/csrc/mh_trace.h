@@ -19,6 +19,13 @@
 
 namespace mh {
 
+// fp32 box hits put on the face's plane (1), or left at o + t d (0: the
+// adjoint kernels, whose register budget sits at the 256-VGPR ceiling,
+// where it costs half their occupancy).
+#ifndef MH_SNAP_BOX
+#define MH_SNAP_BOX 1
+#endif
+
 template <typename R> struct PD {
     int kind, flags, prim, mat;
     V3<R> p0, p1, p2; // fp32: triangles carry (v0, e1, e2)
@@ -130,6 +137,15 @@ __device__ __forceinline__ bool hit_sphere(const PD<R>& p, V3<R> o, V3<R> d, R t
     if (FULL) {
         V3<R> pt = at(o, d, t);
         V3<R> n = (pt - p.p0) * (R(1) / p.r0);
+        if constexpr (sizeof(R) == 4) {
+            // fp32: o + t d is off the sphere by about |o| t ulp, which at a
+            // hundred units from the ray origin reaches the 1e-4 offset of the
+            // next ray; project the point back onto the sphere
+            // |n| = 1 + e with e of a few ulp * |o| t / r: one Newton step of
+            // 1 / |n| (1.5 - 0.5 |n|^2) removes e to second order
+            n = n * (R(1.5) - R(0.5) * len2(n));
+            pt = p.p0 + n * p.r0;
+        }
         R phi = matan2(n.z, n.x);
         R th = masin(mfmax(R(-1), mfmin(R(1), n.y)));
         R u = R(0.5) + phi / (R(2) * Consts<R>::pi);
@@ -209,14 +225,23 @@ __device__ bool hit_box(const PD<R>& p, V3<R> o, V3<R> d, R tmin, R tmax, R* tou
     *tout = th;
     if (FULL) {
         V3<R> pt = at(o, d, th);
+        // fp32: the hit point put on the face's own plane (see hit_sphere)
+        constexpr bool snap = sizeof(R) == 4 && MH_SNAP_BOX;
         V3<R> n;
         if (axis < 0) {
             n = -d;
         } else {
             n = zero3<R>();
-            if (axis == 0) n.x = (R)sgn;
-            else if (axis == 1) n.y = (R)sgn;
-            else n.z = (R)sgn;
+            if (axis == 0) {
+                n.x = (R)sgn;
+                if (snap) pt.x = sgn > 0 ? hi.x : lo.x;
+            } else if (axis == 1) {
+                n.y = (R)sgn;
+                if (snap) pt.y = sgn > 0 ? hi.y : lo.y;
+            } else {
+                n.z = (R)sgn;
+                if (snap) pt.z = sgn > 0 ? hi.z : lo.z;
+            }
         }
         R u = R(0), v = R(0);
         R dx = his[0] - los[0], dy = his[1] - los[1], dz = his[2] - los[2];
@@ -290,7 +315,15 @@ __device__ __forceinline__ bool hit_tri(const PD<R>& p, const DAttr<R>* attr, V3
     if (t < tmin || t > tmax) return false;
     *tout = t;
     if (FULL) {
-        V3<R> pt = at(o, d, t);
+        // fp64: o + t d as the reference forms it. fp32: from the barycentric
+        // coordinates, accurate to the triangle's own coordinates; o + t d is
+        // off the plane by about |o| t ulp, which at a hundred units from the
+        // ray origin reaches the 1e-4 offset of the next ray (paths then
+        // re-hit their own surface: the far terrain of the 1080p frame came
+        // out 1.6 % too dark)
+        V3<R> pt;
+        if constexpr (sizeof(R) == 4) pt = p.p0 + (e1 * u + e2 * v);
+        else pt = at(o, d, t);
         V3<R> n;
         bool have = false;
         if ((p.flags & F_SMOOTH) && attr) {
