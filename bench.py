@@ -190,10 +190,23 @@ def adjoint_leg(reps):
             "forward_kernel_ms": round(fwd_ms, 2)}
 
 
-def batch_leg():
-    """Batched workload: 32 uneven scenes in one persistent wavefront launch
-    sequence (autotuned), against rendering each job alone with the plain
-    GPU kernel and with the single-scene wavefront engine."""
+def timed_reps(fn, reps):
+    """Wall seconds of `reps` calls of fn after one untimed call."""
+    fn()
+    out = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        out.append(time.perf_counter() - t)
+    return out
+
+
+def batch_leg(reps):
+    """Batched workload: 32 uneven scenes (1,280-409,600 triangles) in one
+    persistent wavefront launch sequence (autotuned), against rendering each
+    job alone with the plain GPU kernel and with the single-scene wavefront
+    engine; every rate is the mean over `reps` timed repetitions (min / max
+    reported)."""
     from marmel_amd import Batch, GpuScene, abi
     from marmel_amd.workloads import batch32_scenes, batch32_spec
     t0 = time.time()
@@ -205,50 +218,148 @@ def batch_leg():
     t_up = time.time() - t1
     dev_build_ms = sum(g.bvh_info().build_ms for g in gss)
     jobs = [dict(scene=i, cam=hs.camera(1.0), width=r, height=r, spp=spp, depth=dp)
-            for i, (hs, (_, _, _, r, spp, dp)) in enumerate(zip(hss, spec))]
+            for i, (hs, (_, _, _, _, r, spp, dp)) in enumerate(zip(hss, spec))]
+    tris = [hs.info.n_triangles for hs in hss]
     samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
-    log(f"bench: batch32 scenes generated in {t_gen:.1f}s, uploaded with device-built trees in {t_up:.2f}s "
-        f"(device builds {dev_build_ms:.0f} ms), {samples / 1e6:.1f} Msamples")
+    log(f"bench: batch32 scenes ({min(tris)}-{max(tris)} triangles, {sum(tris)} in all) generated in "
+        f"{t_gen:.1f}s, uploaded with device-built trees in {t_up:.2f}s (device builds {dev_build_ms:.0f} ms), "
+        f"{samples / 1e6:.1f} Msamples")
     b = Batch()
     b.render(gss, jobs, autotune=True, fetch=False)  # tunes once, cached on the handle
     tune = b.last_tune()
-    t1 = time.perf_counter()
-    b.render(gss, jobs, autotune=True, fetch=False)
-    batched = samples / (time.perf_counter() - t1) / 1e6
-    log(f"bench: batch32 batched {batched:.1f} Msamples/s, tune {tune}")
+
+    def rate(secs):
+        r = [samples / s / 1e6 for s in secs]
+        return round(sum(r) / len(r), 1), round(min(r), 1), round(max(r), 1)
+
+    batched = rate(timed_reps(lambda: b.render(gss, jobs, autotune=True, fetch=False), reps))
+    log(f"bench: batch32 batched {batched} Msamples/s (mean, min, max of {reps}), tune {tune}")
     # each batched engine on its own (the tuner keeps the faster)
     eng = {}
     for name, e in (("wavefront", abi.BATCH_WAVEFRONT), ("megakernel", abi.BATCH_MEGAKERNEL)):
         kw = dict(pool=tune["pool"], grid_blocks=tune["grid_blocks"], block=tune["block"],
                   bvh_width=tune["bvh_width"] if e == abi.BATCH_WAVEFRONT else 0,
                   fixed_batches=bool(tune["fixed_batches"]), engine=e)
-        b.render(gss, jobs, fetch=False, **kw)
-        t1 = time.perf_counter()
-        b.render(gss, jobs, fetch=False, **kw)
-        eng[name] = round(samples / (time.perf_counter() - t1) / 1e6, 1)
+        eng[name] = rate(timed_reps(lambda: b.render(gss, jobs, fetch=False, **kw), reps))[0]
     log(f"bench: batch32 batched engines {eng} Msamples/s")
-    for j in jobs[:2]:  # warm both single-job engines
-        gss[j["scene"]].render(j["cam"], j["width"], j["height"], 1, j["depth"], fetch=False)
-    t1 = time.perf_counter()
-    for j in jobs:
-        gss[j["scene"]].render(j["cam"], j["width"], j["height"], j["spp"], j["depth"], fetch=False)
-    plain = samples / (time.perf_counter() - t1) / 1e6
-    t1 = time.perf_counter()
-    for j in jobs:
-        gss[j["scene"]].render(j["cam"], j["width"], j["height"], j["spp"], j["depth"], fetch=False,
-                               engine=abi.ENGINE_WAVEFRONT, use_graph=True)
-    single_wf = samples / (time.perf_counter() - t1) / 1e6
-    log(f"bench: batch32 plain megakernel {plain:.1f}, single-scene wavefront {single_wf:.1f} Msamples/s")
+
+    def each(**kw):
+        for j in jobs:
+            gss[j["scene"]].render(j["cam"], j["width"], j["height"], j["spp"], j["depth"], fetch=False, **kw)
+
+    plain = rate(timed_reps(each, reps))
+    single_wf = rate(timed_reps(lambda: each(engine=abi.ENGINE_WAVEFRONT, use_graph=True), reps))
+    log(f"bench: batch32 plain megakernel {plain}, single-scene wavefront {single_wf} Msamples/s")
     b.close()
     for gsc in gss:
         gsc.close()
-    return {"workload": "32 seeded scenes: icospheres 1,280-327,680 triangles + 20-300 spheres, "
-                        "256^2-1024^2, 4-256 spp, depth 4-10",
-            "msamples": round(samples / 1e6, 1), "batched_msamples_per_s": round(batched, 1),
-            "plain_msamples_per_s": round(plain, 1), "single_scene_wavefront_msamples_per_s": round(single_wf, 1),
-            "batched_vs_plain": round(batched / plain, 3), "tune": tune,
+    return {"workload": f"32 seeded scenes: displaced icospheres {min(tris):,}-{max(tris):,} triangles "
+                        "(stratified over 1k-500k) + 20-300 spheres, 256^2-1024^2, 4-256 spp, depth 4-10",
+            "msamples": round(samples / 1e6, 1), "triangles_total": sum(tris), "reps": reps,
+            "batched_msamples_per_s": batched[0], "batched_min_max": batched[1:],
+            "plain_msamples_per_s": plain[0], "plain_min_max": plain[1:],
+            "single_scene_wavefront_msamples_per_s": single_wf[0],
+            "batched_vs_plain": round(batched[0] / plain[0], 3),
+            "batched_vs_plain_worst_case": round(batched[1] / plain[2], 3), "tune": tune,
             "batched_engines_msamples_per_s": eng,
             "device_bvh_build_ms": round(dev_build_ms, 1)}
+
+
+def rmse_leg(g, cam, o, batch, tune, ref_spp, rows, oracle_spp, threads):
+    """Equal RMSE on the 1080p frame: a converged reference rendered on the
+    GPU (batched engine, ref_spp samples on streams disjoint from the frames'),
+    cross-checked against the FP64 oracle on `rows` full-width rows of the
+    frame (Monte Carlo consistency), then the RMSE of the batched engine's and
+    the megakernel's 64-spp frames (the same sample streams 0..63) against it.
+    Both engines evaluate the same estimator, so their RMSEs must agree to
+    within the fp32 rounding of individual paths."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import checkers as ck
+    from marmel_amd import abi
+    kw = {}
+    if tune:
+        kw = dict(pool=tune["pool"], grid_blocks=tune["grid_blocks"], block=tune["block"],
+                  bvh_width=tune["bvh_width"], fixed_batches=bool(tune["fixed_batches"]))
+    t0 = time.perf_counter()
+    (_, ref), = batch.render([g], [dict(scene=0, cam=cam, width=W, height=H, spp=ref_spp, depth=DEPTH,
+                                        sample_offset=1 << 20)], **kw)
+    t_ref = time.perf_counter() - t0
+    (_, lin_b), = batch.render([g], [dict(scene=0, cam=cam, width=W, height=H, spp=SPP, depth=DEPTH)], **kw)
+    _, lin_m = g.render(cam, W, H, SPP, DEPTH, engine=abi.ENGINE_MEGAKERNEL)
+    ref = ref.astype(np.float64)
+    rmse_b = float(np.sqrt(np.mean((lin_b - ref) ** 2)))
+    rmse_m = float(np.sqrt(np.mean((lin_m - ref) ** 2)))
+    # the same paths in both frames: their difference is rounding, far below the noise
+    engines_rms = float(np.sqrt(np.mean((lin_b.astype(np.float64) - lin_m) ** 2)))
+    y0 = H // 2 - rows // 2
+    t1 = time.perf_counter()
+    _, om, ov, _ = o.render(cam, W, H, oracle_spp, DEPTH, sample_offset=1 << 24, y0=y0, y1=y0 + rows,
+                            threads=threads, want_var=True)
+    t_or = time.perf_counter() - t1
+    s = ck.mc_consistency(ref[y0:y0 + rows], ref_spp, om[y0:y0 + rows], ov[y0:y0 + rows], oracle_spp)
+    # 256 oracle samples of heavy-tailed NEE terms: single pixels are held to
+    # 95 %, 4x4 blocks and the row sums to the normal law
+    ok = s["frac3"] >= 0.95 and s["bfrac3"] >= 0.97 and s["bz2"] < 1.35 and bool((np.abs(s["Z"]) < 4).all())
+    log(f"bench: rmse reference {ref_spp} spp in {t_ref:.1f}s; oracle rows {y0}..{y0 + rows} at {oracle_spp} spp "
+        f"in {t_or:.1f}s: {s}; rmse batched {rmse_b:.5f} megakernel {rmse_m:.5f} (engines differ by "
+        f"{engines_rms:.2e} rms)")
+    return {"reference": f"GPU batched engine, {ref_spp} spp on disjoint sample streams",
+            "reference_vs_oracle_rows": {"rows": [y0, y0 + rows], "oracle_spp": oracle_spp,
+                                         "frac3": round(s["frac3"], 4), "block_z2": round(s["bz2"], 3),
+                                         "image_z": [round(float(z), 2) for z in s["Z"]], "consistent": ok},
+            "batched_rmse": round(rmse_b, 6), "megakernel_rmse": round(rmse_m, 6),
+            "rmse_ratio": round(rmse_b / rmse_m, 5), "engines_rms_difference": engines_rms,
+            "equal_within_noise": bool(abs(rmse_b / rmse_m - 1.0) < 0.01)}
+
+
+def stress_leg(tune, seconds, chunk=64, target=1024):
+    """Stress workload (3840 x 2160, ~9.9 M triangles of seeded trees and a
+    refractive pond with Beer-Lambert water, sun disk 0.5 deg, depth 12),
+    progressive: chunks of `chunk` spp with increasing sample offsets through
+    the batched engine, accumulated into the running mean, until `target` spp
+    or `seconds` of rendering; the time for `target` spp is extrapolated
+    linearly from the chunks' mean when the budget ends first."""
+    import numpy as np
+    from marmel_amd import Batch, GpuScene, HostScene, native
+    t0 = time.time()
+    hs = HostScene()
+    info = hs.generate(native.GEN_FOREST, seed=5, grid=200, subdiv=6)
+    gs = GpuScene(hs.soa(bvh=False), gpu_bvh=True)
+    bi = gs.bvh_info()
+    w, h, depth = 3840, 2160, 12
+    cam = hs.camera(w / h)
+    log(f"bench: stress scene {info.n_triangles} triangles, generated and uploaded in {time.time() - t0:.1f}s "
+        f"(device BVH {bi.build_ms:.0f} ms)")
+    kw = {}
+    if tune:
+        kw = dict(pool=tune["pool"], grid_blocks=tune["grid_blocks"], block=tune["block"],
+                  bvh_width=2, fixed_batches=bool(tune["fixed_batches"]))
+    b = Batch()
+    acc = np.zeros((h, w, 3), np.float64)
+    secs = []
+    done = 0
+    t_start = time.perf_counter()
+    while done < target and (not secs or time.perf_counter() - t_start < seconds):
+        t1 = time.perf_counter()
+        (_, lin), = b.render([gs], [dict(scene=0, cam=cam, width=w, height=h, spp=chunk, depth=depth,
+                                         sample_offset=done)], **kw)
+        secs.append(time.perf_counter() - t1)
+        acc += lin
+        done += chunk
+        log(f"bench: stress chunk {done // chunk}: {done} spp, {secs[-1]:.2f} s")
+    b.close()
+    gs.close()
+    mean = acc / (done // chunk)
+    per_chunk = sum(secs[1:]) / max(len(secs) - 1, 1) if len(secs) > 1 else secs[0]
+    msps = w * h * chunk / per_chunk / 1e6
+    return {"workload": f"3840x2160, {info.n_triangles:,} triangles of seeded trees + refractive pond "
+                        f"(Beer-Lambert), sun radius 0.5 deg, depth {depth}, progressive {chunk}-spp chunks",
+            "spp_rendered": done, "chunks": len(secs), "msamples_per_s": round(msps, 1),
+            "seconds_per_chunk": round(per_chunk, 3),
+            "seconds_for_%d_spp" % target: round(per_chunk * target / chunk, 1),
+            "extrapolated": done < target, "image_mean": round(float(mean.mean()), 5),
+            "device_bvh_build_ms": round(bi.build_ms, 1)}
 
 
 def post_leg(g, cam, reps):
@@ -319,6 +430,12 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint ms/grad leg")
     ap.add_argument("--adjoint-reps", type=int, default=3)
     ap.add_argument("--no-batch32", action="store_true", help="skip the 32-scene batched leg")
+    ap.add_argument("--batch-reps", type=int, default=3, help="timed repetitions of each 32-scene rate")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the equal-RMSE leg (converged reference)")
+    ap.add_argument("--no-stress", action="store_true", help="skip the 4K / 9.9M-triangle progressive leg")
+    ap.add_argument("--stress-seconds", type=float, default=40.0,
+                    help="render budget of the stress leg (1024 spp are extrapolated past it)")
+    ap.add_argument("--rmse-ref-spp", type=int, default=2048, help="samples of the converged reference")
     ap.add_argument("--no-post", action="store_true",
                     help="skip the adaptive Whitted + flare leg (steps after the framebuffer)")
     ap.add_argument("--only-post", action="store_true",
@@ -432,7 +549,14 @@ def main():
     adj = None if args.no_adjoint else adjoint_leg(args.adjoint_reps)
     if adj:
         log(f"bench: adjoint {adj}")
-    b32 = None if args.no_batch32 else batch_leg()
+    b32 = None if args.no_batch32 else batch_leg(args.batch_reps)
+    c5 = None if args.no_stress else stress_leg(tune, args.stress_seconds)
+    if c5:
+        log(f"bench: stress {c5}")
+    eq = None
+    if not args.no_rmse:
+        hold_r, o_r = oracle_scene(hs)
+        eq = rmse_leg(g, cam, o_r, batch, tune, args.rmse_ref_spp, 8, 256, min(os.cpu_count() or 1, 64))
     post = None if args.no_post else post_leg(g, cam, 3)
     if post:
         log(f"bench: post {post}")
@@ -495,6 +619,8 @@ def main():
         "tune": tune,
         "adjoint": adj,
         "batch32": b32,
+        "equal_rmse": eq,
+        "stress": c5,
         "after_framebuffer": post,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -137,6 +137,7 @@ typedef struct mt_gen_params {
     int n_spheres;        /* sphere count (defaults per kind)              */
     int n_emissive;       /* emissive spheres (terrain default 8)          */
     int subdiv;           /* icosphere subdivision level                   */
+    int count;            /* icospheres per scene (0 or 1: one)            */
 } mt_gen_params;
 
 typedef struct mt_gen_info {

@@ -379,6 +379,38 @@ static int gen_icosphere(mt_scene *s, const mt_gen_params *p, rng_t *r, mt_gen_i
         pp[f].p2 = V(f32(cx + v[2][0]), f32(cy + v[2][1]), f32(cz + v[2][2]));
     }
     int rc = mt_scene_add_prims(s, pp, NULL, (int)cur) < 0 ? -1 : 0;
+    /* further icospheres (count > 1): rows of four behind the first, each
+     * with its own radius, displacement and material; drawn after everything
+     * above, so a scene of one icosphere is unchanged */
+    int count = p->count > 1 ? p->count : 1;
+    for (int k = 1; k < count && rc == 0; ++k) {
+        uint32_t ks = (uint32_t)next64(r);
+        double Rk = uni(r, 1.0, 2.5), ak = uni(r, 0.1, 0.35);
+        int col = (k - 1) % 4, rowk = (k - 1) / 4;
+        double kx = cx + (col - 1.5) * 6.0 + uni(r, -0.5, 0.5);
+        double kz = cz - R - 4.0 - rowk * 6.0 + uni(r, -0.5, 0.5);
+        double ky = Rk + 0.3;
+        mh_material km = random_material(r, (int)(next64(r) % 4u));
+        int kmi = mt_scene_add_material(s, &km);
+        if (kmi < 0) {
+            rc = -1;
+            break;
+        }
+        for (long f = 0; f < cur; ++f) {
+            double v[3][3];
+            for (int q = 0; q < 3; ++q) {
+                double *u = &tri[9 * f + 3 * q];
+                double disp = 1.0 + ak * gnoise(u[0] * 3.0 + u[2] * 1.7, u[1] * 3.0 - u[2] * 2.3, ks);
+                for (int c = 0; c < 3; ++c) v[q][c] = u[c] * Rk * disp;
+            }
+            pp[f].kind = MH_PRIM_TRIANGLE;
+            pp[f].material = kmi;
+            pp[f].p0 = V(f32(kx + v[0][0]), f32(ky + v[0][1]), f32(kz + v[0][2]));
+            pp[f].p1 = V(f32(kx + v[1][0]), f32(ky + v[1][1]), f32(kz + v[1][2]));
+            pp[f].p2 = V(f32(kx + v[2][0]), f32(ky + v[2][1]), f32(kz + v[2][2]));
+        }
+        rc = mt_scene_add_prims(s, pp, NULL, (int)cur) < 0 ? -1 : 0;
+    }
     free(pp);
     free(tri);
     if (info) {
@@ -386,8 +418,8 @@ static int gen_icosphere(mt_scene *s, const mt_gen_params *p, rng_t *r, mt_gen_i
         info->at[0] = cx; info->at[1] = cy * 0.6; info->at[2] = cz;
         info->vfov = 45.0;
         info->n_spheres = nsph;
-        info->n_triangles = (int)cur;
-        info->n_prims = nsph + 1 + (int)cur;
+        info->n_triangles = (int)(cur * count);
+        info->n_prims = nsph + 1 + (int)(cur * count);
     }
     return rc;
 }

@@ -136,7 +136,8 @@ class BvhStats(C.Structure):
 
 class GenParams(C.Structure):
     _fields_ = [("kind", C.c_int), ("seed", C.c_uint), ("grid", C.c_int),
-                ("n_spheres", C.c_int), ("n_emissive", C.c_int), ("subdiv", C.c_int)]
+                ("n_spheres", C.c_int), ("n_emissive", C.c_int), ("subdiv", C.c_int),
+                ("count", C.c_int)]
 
 
 class GenInfo(C.Structure):

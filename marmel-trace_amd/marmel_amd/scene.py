@@ -47,9 +47,9 @@ class HostScene:
         raise MarmelError(rc, self._L.mt_last_error().decode())
 
     # ---- construction -------------------------------------------------
-    def generate(self, kind, seed=1, grid=0, n_spheres=0, n_emissive=0, subdiv=0):
+    def generate(self, kind, seed=1, grid=0, n_spheres=0, n_emissive=0, subdiv=0, count=0):
         p = native.GenParams(kind=kind, seed=seed, grid=grid, n_spheres=n_spheres,
-                             n_emissive=n_emissive, subdiv=subdiv)
+                             n_emissive=n_emissive, subdiv=subdiv, count=count)
         info = native.GenInfo()
         rc = self._L.mt_scene_generate(self._h, C.byref(p), C.byref(info))
         if rc:
