@@ -47,6 +47,7 @@ HBM_PEAK_GBS = 8000.0
 # 16384 workgroups), the factors being what the same counters read for a
 # 1 GiB copy. Re-measure when the shadow kernel or its launch changes.
 SHADOW_TRAFFIC_PER_LAUNCH = 24423.1e6
+SHADOW_LIMITER = "dependent node-fetch latency and lane divergence, not HBM bandwidth"
 W, H, SPP, DEPTH = 1920, 1080, 64, 8
 GRID = 1024
 
@@ -525,6 +526,18 @@ def main():
             ktimes[k] = [ms, n]
         k_frames = 1
         log(f"bench: clocked frame {tc.kernel_ms:.1f} ms (per-kernel times)")
+    gpu_counts = None
+    if args.engine == "batched":
+        # the same frame once more with the counting kernels: the traversal
+        # work on the tree the GPU actually walks (not timed)
+        t = batch.last_tune()
+        batch.set_counting(True)
+        batch.render([g], [dict(job, sample_offset=args.steps * SPP)], fetch=False, pool=t["pool"],
+                     grid_blocks=t["grid_blocks"], block=t["block"], bvh_width=t["bvh_width"],
+                     fixed_batches=bool(t["fixed_batches"]), engine=t["engine"], use_graph=False)
+        gpu_counts = batch.last_counts()
+        batch.set_counting(False)
+        log(f"bench: GPU traversal counts of one frame: {gpu_counts}")
     single_wf = None
     if args.engine == "batched" and args.plain_steps > 0:
         frame("wavefront", 0)
@@ -592,14 +605,30 @@ def main():
         per_launch = shadow_bytes(st) * scale * k_frames / sh_n
         avg_s = sh_ms / sh_n * 1e-3
         achieved = per_launch / avg_s / 1e9
-        roof = {"bound": "hbm", "kernel": "k_wf_shadow" + ("<true, false> (batched engine)" if args.engine == "batched"
-                                                           else "<false, false>"),
+        batched = args.engine == "batched"
+        roof = {"bound": "hbm", "kernel": "k_wf_shadow_refill<true> (batched engine)" if batched
+                else "k_wf_shadow_refill<false>",
                 "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": SHADOW_TRAFFIC_PER_LAUNCH if args.engine == "batched" else None,
+                "traffic": SHADOW_TRAFFIC_PER_LAUNCH if batched else None,
                 "bytes_per_launch": round(per_launch),
                 "avg_launch_us": round(avg_s * 1e6, 1), "launches": sh_n,
-                "kernel_share": round(sh_ms / sum(v[0] for v in ktimes.values()), 3)}
+                "kernel_share": round(sh_ms / sum(v[0] for v in ktimes.values()), 3),
+                "achieved_basis": "oracle traversal counts on the reference-shaped tree (12-bin SAH, widest axis)"}
+        if batched:
+            roof["traffic_frac"] = round(SHADOW_TRAFFIC_PER_LAUNCH / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+            # what actually limits the kernel: dependent node fetches and lane
+            # divergence (rocprofv3 counters of this kernel, see profiles/)
+            roof["limiter"] = SHADOW_LIMITER
+        if gpu_counts:
+            c = gpu_counts
+            gb = (c["shadow_queries"] * B_ANY_IO + c["shadow_nodes"] * B_NODE + c["shadow_triangles"] * B_TRI
+                  + c["shadow_other"] * B_SPH) / sh_n
+            roof["bytes_per_launch_gpu_tree"] = round(gb)
+            roof["achieved_gpu_tree"] = round(gb / avg_s / 1e9, 1)
+            roof["frac_gpu_tree"] = round(gb / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+            roof["gpu_nodes_per_query"] = round(c["shadow_nodes"] / max(c["shadow_queries"], 1), 2)
+            roof["oracle_nodes_per_query"] = round(st["a_inner"] / max(st["any"], 1), 2)
     line = {
         "metric": "Msamples/s at 1080p, batched vs plain GPU at equal RMSE; adjoint ms/grad",
         "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
@@ -624,6 +653,7 @@ def main():
         "after_framebuffer": post,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "gpu_traversal_counts_per_frame": gpu_counts,
         "device": name,
     }
     print(json.dumps(line), flush=True)

@@ -579,6 +579,23 @@ int mh_render_batch(mh_batch *b, struct mh_scene *const *scenes, int n_scenes,
 /* Settings used by the last mh_render_batch on this handle. */
 int mh_batch_last_tune(const mh_batch *b, mh_tune *out);
 
+/* Traversal work of the batched engine's extension (closest-hit) and any-hit
+ * shadow queries in the last mh_render_batch made with counting on: queries,
+ * interior BVH nodes fetched, triangles and other primitives (spheres,
+ * boxes, cylinders, planes) tested. Counted on the binary-tree wavefront
+ * kernels only (zeros for 4-wide launches, transmissive shadow walks and
+ * the batched megakernel). No reference counterpart: it prices the
+ * algorithmic bytes of the tree the GPU actually traverses. */
+typedef struct mh_traversal_counts {
+    int64_t ext_queries, ext_nodes, ext_triangles, ext_other;
+    int64_t shadow_queries, shadow_nodes, shadow_triangles, shadow_other;
+} mh_traversal_counts;
+
+/* Counting on (1) or off (0) for the following renders on `b`; counting
+ * launches run separate instantiations of the two kernels. */
+int mh_batch_set_counting(mh_batch *b, int on);
+int mh_batch_last_counts(const mh_batch *b, mh_traversal_counts *out);
+
 /* Timing and error state of the last call on the calling thread. */
 int mh_last_timing(mh_timing *out);
 int mh_last_kernel_times(mh_kernel_times *out);  /* no reference counterpart */

@@ -724,6 +724,9 @@ struct mh_batch {
     std::vector<unsigned char> sgraph_key;
     BUnit* units = nullptr;         // batched megakernel work units (device)
     size_t units_n = 0;
+    int count_on = 0;                       // count traversal work (WF_COUNT launches)
+    unsigned long long* visits = nullptr;   // device counters, mh_traversal_counts order
+    mh_traversal_counts counts{};           // of the last render
 };
 
 namespace {
@@ -2090,6 +2093,7 @@ void mh_batch_destroy(mh_batch* b)
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->units) (void)hipFree(b->units);
+    if (b->visits) (void)hipFree(b->visits);
     delete b;
 }
 
@@ -2143,7 +2147,7 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
         return fail(4, "unknown batch engine %d", t.engine);
 
     std::vector<SceneView<float>> views((size_t)n_scenes);
-    int feat = 0;
+    int feat = b->count_on ? WF_COUNT : 0;
     for (int i = 0; i < n_scenes; ++i) {
         if (scenes[i]->v32.has_transmissive) feat |= WF_TRANS;
         if (scenes[i]->v32.fog_kernel) feat |= WF_FOG;
@@ -2224,6 +2228,7 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     mj.total = begin;
     mj.total_pix = pix;
     mj.acc64 = b->acc64;
+    mj.visits = b->visits;
     mj.sh_stride = 1;
     for (const auto& v : views) mj.sh_stride = std::max(mj.sh_stride, v.n_lights + 1);
 
@@ -2325,6 +2330,7 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
 
     LaunchOut lo{b->fb_linear, b->fb_rgb, b->d_rays, nullptr};
     HIP_TRY(hipMemsetAsync(b->d_rays, 0, sizeof(unsigned long long), b->stream));
+    if (b->count_on) HIP_TRY(hipMemsetAsync(b->visits, 0, sizeof(mh_traversal_counts), b->stream));
     HIP_TRY(wf_multi_begin(b->ws, mj, b->stream));
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     rc = t.engine == MH_BATCH_MEGAKERNEL ? mega_run(b, mj, feat, -1) : batch_run(b, mj, max_depth, feat, t, -1);
@@ -2336,8 +2342,12 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     if (linear_out) HIP_TRY(hipMemcpyAsync(linear_out, b->fb_linear, n3 * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     unsigned long long rays = 0;
     HIP_TRY(hipMemcpyAsync(&rays, b->d_rays, sizeof rays, hipMemcpyDeviceToHost, b->stream));
+    mh_traversal_counts counts{};
+    if (b->count_on)
+        HIP_TRY(hipMemcpyAsync(&counts, b->visits, sizeof counts, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipEventRecord(b->ev[2], b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    b->counts = counts;
     memset(&g_timing, 0, sizeof g_timing);
     g_timing.kernel_ms = ms_between(b->ev[0], b->ev[1]);
     g_timing.total_ms = ms_between(b->ev[0], b->ev[2]);
@@ -2360,6 +2370,21 @@ int mh_batch_last_tune(const mh_batch* b, mh_tune* out)
 {
     if (!b || !out) return fail(1, "NULL argument");
     *out = b->last;
+    return 0;
+}
+
+int mh_batch_set_counting(mh_batch* b, int on)
+{
+    if (!b) return fail(1, "NULL argument");
+    if (on && !b->visits) HIP_TRY(hipMalloc(&b->visits, sizeof(mh_traversal_counts)));
+    b->count_on = on ? 1 : 0;
+    return 0;
+}
+
+int mh_batch_last_counts(const mh_batch* b, mh_traversal_counts* out)
+{
+    if (!b || !out) return fail(1, "NULL argument");
+    *out = b->counts;
     return 0;
 }
 

@@ -82,6 +82,7 @@ struct WfMulti {
     unsigned long long* acc64;   // fixed-point (2^-24) per-pixel sums, 3 per pixel
     int stream;                  // path regeneration: per-slot bounce, free-slot list
     int sh_stride;               // shadow-queue blocks per bounce: max lights over the scenes + 1 (sun)
+    unsigned long long* visits;  // WF_COUNT launches: traversal counts, MH_V_* (device)
 };
 
 // Batched megakernel work unit: one 16x16 tile of a job, samples
@@ -122,7 +123,9 @@ hipError_t wf_enqueue_end(const WfState& ws, int npix, int spp, const LaunchOut&
 // Kernel variants a batch needs (bit set): some scene has a transmissive
 // material (shadow rays walk through it) / has fog or marched water / is
 // traversed on its 4-wide tree.
-enum { WF_TRANS = 1, WF_FOG = 2, WF_WIDE = 4 };
+enum { WF_TRANS = 1, WF_FOG = 2, WF_WIDE = 4, WF_COUNT = 8 };
+// WF_COUNT: the extension and any-hit shadow kernels count their traversal
+// work into WfMulti::visits (order of mh_traversal_counts; batched engine).
 
 // Batched engine: one batch of `pool` paths drawn from all jobs, bounce
 // sequence to the deepest job's depth. `feat`: WF_* bits over the batch's scenes.

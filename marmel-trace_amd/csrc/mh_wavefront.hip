@@ -21,13 +21,23 @@ hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st)
 // The extension kernel: dynamic fetch on binary trees.
 template <bool MULTI> static auto extend_kernel(int feat)
 {
-    return (MH_EXTEND_REFILL > 0 && !(feat & WF_WIDE)) ? k_wf_extend_refill<MULTI> : k_wf_extend<MULTI>;
+    if (MH_EXTEND_REFILL > 0 && !(feat & WF_WIDE)) {
+        if constexpr (MULTI)
+            if (feat & WF_COUNT) return k_wf_extend_refill<true, true>;
+        return k_wf_extend_refill<MULTI, false>;
+    }
+    return k_wf_extend<MULTI>;
 }
 
 // The any-hit shadow kernel (no transmissive material in the launch).
-template <bool MULTI> static auto shadow_any()
+template <bool MULTI> static auto shadow_any(int feat)
 {
-    return MH_SHADOW_REFILL > 0 ? k_wf_shadow_refill<MULTI> : k_wf_shadow<MULTI, false>;
+    if (MH_SHADOW_REFILL > 0) {
+        if constexpr (MULTI)
+            if (feat & WF_COUNT) return k_wf_shadow_refill<true, true>;
+        return k_wf_shadow_refill<MULTI, false>;
+    }
+    return k_wf_shadow<MULTI, false>;
 }
 
 // Launch wrapped in the optional clock.
@@ -50,7 +60,7 @@ static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<floa
                                   const WfState& ws, const WfMulti& mj, unsigned long long* rays, int grid, int block,
                                   int feat, hipStream_t st, KernelClock* kc)
 {
-    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<MULTI, true> : shadow_any<MULTI>();
+    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<MULTI, true> : shadow_any<MULTI>(feat);
     auto shade = (feat & WF_FOG) ? k_wf_shade<MULTI, true> : k_wf_shade<MULTI, false>;
     dim3 G(grid), B(block);
     MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_gen<MULTI>, G, B, 0, st, sc, cam, job, ws, mj));
@@ -109,7 +119,7 @@ hipError_t wf_stream_step(const WfMulti& mj, const WfState& ws, int which, unsig
     dim3 G(grid), B(block);
     SceneView<float> sc{};
     WfJob job{};
-    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<true, true> : shadow_any<true>();
+    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<true, true> : shadow_any<true>(feat);
     auto shade = (feat & WF_FOG) ? k_wf_shade<true, true> : k_wf_shade<true, false>;
     MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_refill, G, B, 0, st, ws, mj, which));
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_refill_commit, dim3(1), dim3(64), 0, st, ws, mj));

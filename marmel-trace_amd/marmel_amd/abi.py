@@ -165,6 +165,11 @@ K_GEN, K_EXTEND, K_SHADE, K_SHADOW, K_FINALIZE, K_OTHER = range(6)
 K_NAMES = ("gen", "extend", "shade", "shadow", "finalize", "other")
 
 
+class TraversalCounts(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("ext_queries", "ext_nodes", "ext_triangles", "ext_other",
+                                        "shadow_queries", "shadow_nodes", "shadow_triangles", "shadow_other")]
+
+
 class KernelTimes(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_int64 * 8)]
 

@@ -117,6 +117,8 @@ def hip():
     _sig(L, "mh_render_batch", C.c_int, [P, P, C.c_int, C.POINTER(abi.Job), C.c_int,
                                          C.POINTER(abi.Tune), P, P])
     _sig(L, "mh_batch_last_tune", C.c_int, [P, C.POINTER(abi.Tune)])
+    _sig(L, "mh_batch_set_counting", C.c_int, [P, C.c_int])
+    _sig(L, "mh_batch_last_counts", C.c_int, [P, C.POINTER(abi.TraversalCounts)])
     _sig(L, "mh_last_timing", C.c_int, [C.POINTER(abi.Timing)])
     _sig(L, "mh_last_kernel_times", C.c_int, [C.POINTER(abi.KernelTimes)])
     _sig(L, "mh_last_seconds", C.c_double, [])

@@ -478,6 +478,20 @@ class Batch:
             off += n
         return out
 
+    def set_counting(self, on):
+        """Count the traversal work of the following renders (mh_batch_set_counting)."""
+        rc = self._L.mh_batch_set_counting(self._h, 1 if on else 0)
+        if rc:
+            raise MarmelError(rc, self._L.mh_last_error().decode())
+
+    def last_counts(self):
+        """Traversal counts of the last render made with counting on (dict)."""
+        c = abi.TraversalCounts()
+        rc = self._L.mh_batch_last_counts(self._h, C.byref(c))
+        if rc:
+            raise MarmelError(rc, self._L.mh_last_error().decode())
+        return abi.struct_to_dict(c)
+
     def last_tune(self):
         t = abi.Tune()
         self._L.mh_batch_last_tune(self._h, C.byref(t))
