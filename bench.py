@@ -66,6 +66,7 @@ def build_scene(grid):
 
 # Per-unit algorithmic bytes (fixed fp32 record sizes of the device layout).
 B_NODE, B_TRI, B_SPH, B_PLANE = 64, 48, 32, 16
+B_NODE4 = 128                       # 4-wide node (SoA child boxes, refs, count): the shadow kernel's tree
 B_CLOSEST_IO, B_ANY_IO = 48, 36     # ray in (32 B) + hit (16 B) / visibility (4 B)
 B_BOUNCE, B_SAMPLE = 192, 12        # path state + material per bounce; accumulator per sample
 
@@ -622,7 +623,7 @@ def main():
             roof["limiter"] = SHADOW_LIMITER
         if gpu_counts:
             c = gpu_counts
-            gb = (c["shadow_queries"] * B_ANY_IO + c["shadow_nodes"] * B_NODE + c["shadow_triangles"] * B_TRI
+            gb = (c["shadow_queries"] * B_ANY_IO + c["shadow_nodes"] * B_NODE4 + c["shadow_triangles"] * B_TRI
                   + c["shadow_other"] * B_SPH) / sh_n
             roof["bytes_per_launch_gpu_tree"] = round(gb)
             roof["achieved_gpu_tree"] = round(gb / avg_s / 1e9, 1)

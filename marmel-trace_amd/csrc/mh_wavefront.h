@@ -272,12 +272,18 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState
 }
 
 // Idle lanes of a wave at which the closest-hit extend kernel hands them new
-// rays (dynamic fetch); 0 selects k_wf_extend. Binary trees only (a launch
-// whose scenes use the 4-wide tree runs k_wf_extend). Measured on the 1080p
-// terrain frame (extend ms per frame): 0 -> 77.2, 32 -> 61.7, 48 -> 59.9,
-// 56 -> 61.0.
+// rays (dynamic fetch); 0 selects k_wf_extend. Measured on the 1080p terrain
+// frame (extend ms per frame, binary tree): 0 -> 77.2, 32 -> 61.7,
+// 48 -> 59.9, 56 -> 61.0.
 #ifndef MH_EXTEND_REFILL
 #define MH_EXTEND_REFILL 48
+#endif
+// On the 4-wide tree (scenes whose view selects it; the autotuner picks it
+// for the 1080p terrain frame): the nearest hit child is visited next from a
+// register; the others are stacked 0 = in index order, 1 = nearest last
+// (sorted). Extend ms per frame: binary 60.5, 0 -> 59.0, 1 -> 57.1.
+#ifndef MH_EXTEND_W4_SORT
+#define MH_EXTEND_W4_SORT 1
 #endif
 
 // Nearest hit of every queued path with dynamic fetch, on the binary tree:
@@ -345,6 +351,7 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
     int best = INT_MIN, bprim = 0x7fffffff, ref = 0;
     uint32_t slot = 0;
     const DNode* nodes = nullptr;
+    const DNode4* nodes4 = nullptr;  // the 4-wide tree when the lane's scene view selects it
     const P32* prims = nullptr;
     uint32_t cnt_q = 0, cnt_n = 0, cnt_t = 0, cnt_o = 0;  // COUNT: queries, nodes, triangles, other prims
     for (;;) {
@@ -382,10 +389,11 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                 }
                 if (sc.root_ref != INT_MIN) {
                     nodes = sc.nodes;
+                    nodes4 = sc.bvh4 ? sc.nodes4 : nullptr;
                     prims = sc.prims;
                     sl.init(o, d);
                     sp = base = psp = 0;
-                    ref = sc.root_ref;
+                    ref = nodes4 ? sc.root4 : sc.root_ref;
                     held = true;
                     busy = true;
                 } else {
@@ -405,7 +413,42 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                 }
                 held = false;
                 if (work) {
-                    if (ref >= 0) {
+                    if (ref >= 0 && nodes4) {
+                        DNode4 nd = ldg(&nodes4[ref]);
+                        if (COUNT) ++cnt_n;
+                        if (MH_EXTEND_W4_SORT) {
+                            int r[4], nh;
+                            float t[4];
+                            children4(sl, nd, tmin, closest, r, t, &nh);
+                            for (int i = 3; i >= 1; --i)
+                                if (i < nh) push(r[i], t[i]);
+                            ref = r[0];
+                            held = nh > 0;
+                        } else {
+                            int cnt = nd.meta.x;
+                            bool h0, h1, h2, h3;
+                            float t0, t1, t2, t3;
+                            sl.one(nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, tmin, closest, &h0, &t0);
+                            sl.one(nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, tmin, closest, &h1, &t1);
+                            sl.one(nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, tmin, closest, &h2, &t2);
+                            sl.one(nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, tmin, closest, &h3, &t3);
+                            h2 = h2 && cnt > 2;
+                            h3 = h3 && cnt > 3;
+                            // the nearest hit child goes on from a register
+                            int k = -1;
+                            float tk = 0.f;
+                            if (h0) { k = 0; tk = t0; }
+                            if (h1 && (k < 0 || t1 < tk)) { k = 1; tk = t1; }
+                            if (h2 && (k < 0 || t2 < tk)) { k = 2; tk = t2; }
+                            if (h3 && (k < 0 || t3 < tk)) { k = 3; tk = t3; }
+                            if (h3 && k != 3) push(nd.ref.w, t3);
+                            if (h2 && k != 2) push(nd.ref.z, t2);
+                            if (h1 && k != 1) push(nd.ref.y, t1);
+                            if (h0 && k != 0) push(nd.ref.x, t0);
+                            ref = k == 0 ? nd.ref.x : k == 1 ? nd.ref.y : k == 2 ? nd.ref.z : nd.ref.w;
+                            held = k >= 0;
+                        }
+                    } else if (ref >= 0) {
                         DNode nd = ldg(&nodes[ref]);
                         if (COUNT) ++cnt_n;
                         bool hl, hr;

@@ -18,10 +18,11 @@ hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st)
     return hipGetLastError();
 }
 
-// The extension kernel: dynamic fetch on binary trees.
+// The extension kernel: dynamic fetch (on the tree width each scene view
+// selects).
 template <bool MULTI> static auto extend_kernel(int feat)
 {
-    if (MH_EXTEND_REFILL > 0 && !(feat & WF_WIDE)) {
+    if (MH_EXTEND_REFILL > 0) {
         if constexpr (MULTI)
             if (feat & WF_COUNT) return k_wf_extend_refill<true, true>;
         return k_wf_extend_refill<MULTI, false>;
