@@ -8,6 +8,7 @@
 #include "mh_build.h"
 #include "mh_flare.h"
 #include "mh_kernels.h"
+#include "mh_layout.h"
 #include "mh_pack.h"
 
 #include <hip/hip_runtime.h>
@@ -46,67 +47,27 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
+} // namespace
+
+int mh::layout_fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+namespace {
+
 #define HIP_TRY(expr)                                                                                 \
     do {                                                                                              \
         hipError_t e_ = (expr);                                                                       \
         if (e_ != hipSuccess)                                                                         \
             return fail(100 + (int)e_, "%s failed: %s", #expr, hipGetErrorString(e_));               \
     } while (0)
-
-template <typename R> DMat<R> conv_mat(const mh_material& a)
-{
-    DMat<R> m;
-    memset(&m, 0, sizeof m);
-    m.albedo = cv<R>(a.albedo);
-    m.specular = cv<R>(a.specular);
-    m.shininess = (R)a.shininess;
-    m.reflectivity = (R)a.reflectivity;
-    m.transparency = (R)a.transparency;
-    m.ior = (R)a.ior;
-    m.is_water = a.is_water;
-    m.beer_lambert = a.beer_lambert;
-    m.pbr = a.pbr;
-    m.texture_kind = a.texture_kind;
-    m.absorption = cv<R>(a.absorption);
-    m.deep_color = cv<R>(a.deep_color);
-    m.metallic = (R)a.metallic;
-    m.roughness = (R)a.roughness;
-    m.emissive = cv<R>(a.emissive);
-    m.texture_scale = (R)a.texture_scale;
-    m.tex_a = cv<R>(a.texture_color_a);
-    m.tex_b = cv<R>(a.texture_color_b);
-    m.glow = cv<R>(a.atmosphere_glow);
-    m.has_emission = (a.emissive.x != 0.0 || a.emissive.y != 0.0 || a.emissive.z != 0.0) ? 1 : 0;
-    double g2 = a.atmosphere_glow.x * a.atmosphere_glow.x + a.atmosphere_glow.y * a.atmosphere_glow.y +
-                a.atmosphere_glow.z * a.atmosphere_glow.z;
-    m.has_glow = g2 > 1e-6 ? 1 : 0;
-    return m;
-}
-
-template <typename R> DSky<R> conv_sky(const mh_sky& k)
-{
-    DSky<R> s;
-    memset(&s, 0, sizeof s);
-    s.sun_dir = cv<R>(k.sun_dir);
-    s.sun_color = cv<R>(k.sun_color);
-    s.horizon = cv<R>(k.horizon_color);
-    s.zenith = cv<R>(k.zenith_color);
-    s.gamma = (R)k.gradient_gamma;
-    s.glow_exp = (R)k.sun_glow_exponent;
-    s.glow_strength = (R)k.sun_glow_strength;
-    s.cloud_height = (R)k.cloud_height;
-    s.cloud_scale = (R)k.cloud_scale;
-    s.coverage = (R)k.cloud_coverage;
-    s.softness = (R)k.cloud_softness;
-    s.sharpness = (R)k.cloud_sharpness;
-    s.octaves = k.cloud_octaves < 1 ? 1 : k.cloud_octaves;
-    s.seed = k.seed;
-    s.sun_radius = (R)k.sun_radius;
-    // solid angle of the sun disk as the reference NEE uses it (1 for a point sun)
-    double rad = k.sun_radius * 3.14159265358979323846 / 180.0;
-    s.sun_omega = (R)(k.sun_radius > 0.0 ? 2.0 * 3.14159265358979323846 * (1.0 - cos(rad)) : 1.0);
-    return s;
-}
 
 struct DevBuf {
     void* p = nullptr;
@@ -225,12 +186,7 @@ struct mh_scene {
     // material tables and the spheres that may be emissive lights (for updates)
     void* d_m32 = nullptr;
     void* d_m64 = nullptr;
-    struct Sphere {
-        int prim, mat;
-        mh_vec3 c;
-        double r;
-    };
-    std::vector<Sphere> spheres;
+    std::vector<LightSphere> spheres;
     // adjoint buffers
     float* d_dimg = nullptr;
     size_t dimg_n = 0;
@@ -296,77 +252,6 @@ int validate_fog(const mh_fog& f)
     return 0;
 }
 
-// Device fog parameters: the anisotropy clamped to [-0.95, 0.95] and the
-// turbulence switch evaluated once, as the reference evaluates them per call.
-template <typename R> DFog<R> conv_fog(const mh_fog& f)
-{
-    DFog<R> o;
-    memset(&o, 0, sizeof o);
-    o.density = (R)f.density;
-    o.color = cv<R>(f.color);
-    o.height = (R)f.height;
-    o.falloff = (R)f.height_falloff;
-    o.inscatter = (R)f.inscatter_strength;
-    double g = f.sun_anisotropy;
-    if (g < -0.95) g = -0.95;
-    if (g > 0.95) g = 0.95;
-    o.g = (R)g;
-    o.noise_scale = (R)f.noise_scale;
-    o.noise_amount = (R)f.noise_amount;
-    o.steps = f.shadow_steps;
-    o.noise = f.noise_amount > 1e-4 && f.noise_scale > 1e-4 ? 1 : 0;
-    return o;
-}
-
-// The ocean's harmonics as the reference forms them for each height query
-// (src/scene.c:1109-1153), once.
-template <typename R> DOcean<R> conv_ocean(const mh_ocean& o)
-{
-    DOcean<R> d;
-    memset(&d, 0, sizeof d);
-    d.on = o.enabled ? 1 : 0;
-    if (!d.on) return d;
-    d.y0 = (R)o.center.y;
-    double dir_x = o.direction.x, dir_z = o.direction.z;
-    double dlen = sqrt(dir_x * dir_x + dir_z * dir_z);
-    if (dlen > 1e-12) {
-        dir_x /= dlen;
-        dir_z /= dlen;
-    } else {
-        dir_x = 1.0;
-        dir_z = 0.0;
-    }
-    double wl = o.wavelength <= 0.0 ? 12.0 : o.wavelength;
-    double amp = o.amplitude;
-    double chop_wl = o.chop_wavelength > 0.0 ? o.chop_wavelength : wl * 0.25;
-    double chop_amp = o.chop > 0.0 ? o.chop : amp * 0.25;
-    const double lambda[4] = {wl, wl * 0.52, chop_wl, chop_wl * 0.40};
-    const double A[4] = {amp, amp * 0.38, chop_amp, chop_amp * 0.32};
-    const double angle[4] = {0.0, 0.628, -0.488, 1.134};
-    const double phi[4] = {0.0, 1.7 + (double)(o.seed % 100u) * 0.03, 3.1 + (double)((o.seed * 7u) % 100u) * 0.03,
-                           0.8 + (double)((o.seed * 13u) % 100u) * 0.03};
-    for (int h = 0; h < 4; ++h) {
-        double ca = cos(angle[h]), sa = sin(angle[h]);
-        d.dx[h] = (R)(ca * dir_x - sa * dir_z);
-        d.dz[h] = (R)(sa * dir_x + ca * dir_z);
-        d.k[h] = (R)((2.0 * 3.14159265358979323846) / lambda[h]);
-        d.phi[h] = (R)phi[h];
-        d.A[h] = (R)A[h];
-    }
-    return d;
-}
-
-int any_water(const mh_material* mats, int n)
-{
-    for (int i = 0; i < n; ++i)
-        if (mats[i].is_water) return 1;
-    return 0;
-}
-
-// Fog on, or marched underwater segments possible (they need fog steps > 0
-// and a water material, fog density or not).
-int needs_fog_kernel(const mh_fog& f, int has_water) { return f.density > 0.0 || (f.shadow_steps > 0 && has_water); }
-
 int validate_materials(const mh_material* mats, int n)
 {
     for (int i = 0; i < n; ++i) {
@@ -422,131 +307,6 @@ int validate(const mh_scene_soa* soa, bool with_tree = true)
             return fail(5, "leaf list entry %d is invalid", i);
     }
     return 0;
-}
-
-// Walk the host tree from the root: checks child indices, leaf ranges, that
-// every leaf slot is covered once, and the depth bound of the GPU stack.
-int check_tree(const mh_scene_soa* soa, std::vector<int>* order)
-{
-    if (soa->n_nodes == 0) return 0;
-    std::vector<char> seen(soa->n_nodes, 0);
-    std::vector<char> slot(soa->n_leaf_prims, 0);
-    std::vector<std::pair<int, int>> st;
-    st.push_back({0, 1});
-    int covered = 0;
-    while (!st.empty()) {
-        auto [n, depth] = st.back();
-        st.pop_back();
-        if (n < 0 || n >= soa->n_nodes) return fail(5, "BVH child index %d out of range", n);
-        if (seen[n]) return fail(5, "BVH node %d reached twice", n);
-        seen[n] = 1;
-        if (depth > 60) return fail(5, "BVH deeper than 60 levels");
-        const mh_bvh_node& nd = soa->nodes[n];
-        if (nd.count > 0) {
-            if (nd.count > 8) return fail(5, "BVH leaf %d holds %d > 8 primitives", n, nd.count);
-            if (nd.first < 0 || nd.first + nd.count > soa->n_leaf_prims) return fail(5, "BVH leaf %d range invalid", n);
-            for (int i = 0; i < nd.count; ++i) {
-                if (slot[nd.first + i]) return fail(5, "leaf slot %d referenced twice", nd.first + i);
-                slot[nd.first + i] = 1;
-            }
-            covered += nd.count;
-        } else {
-            order->push_back(n);
-            st.push_back({nd.right, depth + 1});
-            st.push_back({nd.left, depth + 1});
-        }
-    }
-    if (covered != soa->n_leaf_prims) return fail(5, "BVH leaves cover %d of %d primitives", covered, soa->n_leaf_prims);
-    return 0;
-}
-
-// Collapse the binary tree into 4-wide nodes (DNode4): a node adopts its two
-// binary children, then replaces its largest-area interior child by that
-// child's own two children (in place, keeping left-to-right order) until it
-// holds four or only leaves. Nodes are numbered in depth-first pre-order.
-struct Bvh4Builder {
-    const mh_scene_soa* soa;
-    std::vector<DNode4> nodes;
-    int max_depth = 0;
-
-    static int leaf_ref(const mh_bvh_node& nd) { return ~((nd.first << 3) | (nd.count - 1)); }
-    double area(int n) const
-    {
-        const mh_bvh_node& b = soa->nodes[n];
-        double dx = (double)b.bmax[0] - b.bmin[0], dy = (double)b.bmax[1] - b.bmin[1], dz = (double)b.bmax[2] - b.bmin[2];
-        return dx * dy + dy * dz + dz * dx;
-    }
-    int make(int n2, int depth)
-    {
-        if (depth > max_depth) max_depth = depth;
-        const mh_bvh_node& nd = soa->nodes[n2];
-        std::vector<int> c = {nd.left, nd.right};
-        while (c.size() < 4) {
-            int best = -1;
-            double ba = -1.0;
-            for (size_t i = 0; i < c.size(); ++i)
-                if (soa->nodes[c[i]].count == 0 && area(c[i]) > ba) {
-                    ba = area(c[i]);
-                    best = (int)i;
-                }
-            if (best < 0) break;
-            const mh_bvh_node& e = soa->nodes[c[best]];
-            c[best] = e.left;
-            c.insert(c.begin() + best + 1, e.right);
-        }
-        int idx = (int)nodes.size();
-        nodes.push_back(DNode4{});
-        float lo[3][4], hi[3][4];
-        int ref[4];
-        for (int k = 0; k < 4; ++k) {
-            for (int a = 0; a < 3; ++a) {
-                lo[a][k] = 0.f;
-                hi[a][k] = 0.f;
-            }
-            ref[k] = INT_MIN;
-        }
-        for (size_t k = 0; k < c.size(); ++k) {
-            const mh_bvh_node& ch = soa->nodes[c[k]];
-            for (int a = 0; a < 3; ++a) {
-                lo[a][k] = ch.bmin[a];
-                hi[a][k] = ch.bmax[a];
-            }
-            ref[k] = ch.count > 0 ? leaf_ref(ch) : make(c[k], depth + 1);
-        }
-        DNode4& o = nodes[idx];
-        o.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
-        o.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
-        o.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
-        o.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
-        o.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
-        o.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
-        o.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
-        o.meta = make_int4((int)c.size(), 0, 0, 0);
-        return idx;
-    }
-};
-
-template <typename R> CamView<R> conv_cam(const mh_camera& c)
-{
-    CamView<R> v;
-    v.pos = cv<R>(c.position);
-    v.ll = cv<R>(c.lower_left);
-    v.hor = cv<R>(c.horizontal);
-    v.ver = cv<R>(c.vertical);
-    v.aperture = (R)c.aperture;
-    v.focus = (R)c.focus_distance;
-    v.dome = (R)c.dome_radius;
-    v.squeeze = (R)c.anamorphic_squeeze;
-    // normalised viewport axes, as the thin-lens sampler forms them (in double)
-    auto nrm = [](mh_vec3 a) {
-        double l2 = a.x * a.x + a.y * a.y + a.z * a.z;
-        if (l2 <= 1e-12) return mh_vec3{0, 0, 0};
-        double k = 1.0 / sqrt(l2);
-        return mh_vec3{a.x * k, a.y * k, a.z * k};
-    };
-    v.right_n = cv<R>(nrm(c.horizontal));
-    v.up_n = cv<R>(nrm(c.vertical));
-    return v;
 }
 
 int ensure_fb(mh_scene* s, int w, int h)
@@ -637,36 +397,6 @@ int ensure_wf(mh_scene* s, int pool, size_t npix)
     if (rc) return rc;
     s->wf_pool = pool;
     s->wf_npix = npix;
-    return 0;
-}
-
-// Emissive area lights: the first 8 PBR spheres with emission, file order
-// (reference scene_collect_emissive_lights, src/scene.c:1384-1426).
-int collect_lights(const mh_scene* s, const mh_material* mats, int n_mats, DLight<float>* l32, DLight<double>* l64)
-{
-    int nl = 0;
-    for (const auto& sp : s->spheres) {
-        if (nl >= 8) break;
-        if (sp.mat < 0 || sp.mat >= n_mats) continue;
-        const mh_material& m = mats[sp.mat];
-        if (!m.pbr) continue;
-        if (m.emissive.x == 0.0 && m.emissive.y == 0.0 && m.emissive.z == 0.0) continue;
-        if (!(sp.r > 0.0)) continue;
-        l32[nl] = {sp.prim, sp.mat, cv<float>(sp.c), (float)sp.r, cv<float>(m.emissive)};
-        l64[nl] = {sp.prim, sp.mat, cv<double>(sp.c), sp.r, cv<double>(m.emissive)};
-        ++nl;
-    }
-    for (int i = nl; i < 8; ++i) {
-        l32[i] = DLight<float>{};
-        l64[i] = DLight<double>{};
-    }
-    return nl;
-}
-
-int any_transmissive(const mh_material* mats, int n)
-{
-    for (int i = 0; i < n; ++i)
-        if (mats[i].is_water || mats[i].transparency > 0.8) return 1;
     return 0;
 }
 
@@ -975,16 +705,6 @@ int mh_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes
 
 namespace {
 
-// Device arrays of an uploaded scene and the roots the kernels start from.
-struct Layout {
-    const void *nodes = nullptr, *nodes4 = nullptr;
-    int n_nodes = 0, root_ref = INT_MIN, root4 = INT_MIN;
-    bool bvh4 = false;
-    const void *p32 = nullptr, *p64 = nullptr, *pl32 = nullptr, *pl64 = nullptr, *plidx = nullptr;
-    const void *a32 = nullptr, *a64 = nullptr;
-    int n_planes = 0;
-};
-
 mh_scene* new_scene()
 {
     mh_scene* s = new mh_scene();
@@ -1009,14 +729,9 @@ mh_scene* new_scene()
 
 // Materials, lights, counters and the two precision views: the part of scene
 // creation that does not depend on where the tree was built.
-int finish_scene(mh_scene* s, const mh_scene_soa* soa, const Layout& lay)
+int finish_scene(mh_scene* s, const mh_scene_soa* soa, Layout lay, const std::vector<DMat<float>>& m32,
+                 const std::vector<DMat<double>>& m64)
 {
-    std::vector<DMat<float>> m32(soa->n_materials);
-    std::vector<DMat<double>> m64(soa->n_materials);
-    for (int i = 0; i < soa->n_materials; ++i) {
-        m32[i] = conv_mat<float>(soa->materials[i]);
-        m64[i] = conv_mat<double>(soa->materials[i]);
-    }
     void *d_m32, *d_m64;
     int rc;
     if ((rc = upload(s, m32.data(), m32.size() * sizeof(DMat<float>), &d_m32)) ||
@@ -1029,53 +744,18 @@ int finish_scene(mh_scene* s, const mh_scene_soa* soa, const Layout& lay)
 
     s->d_m32 = d_m32;
     s->d_m64 = d_m64;
-    for (int i = 0; i < soa->n_prims; ++i) {
-        const mh_prim& p = soa->prims[i];
-        if (p.kind == MH_PRIM_SPHERE) s->spheres.push_back({i, p.material, p.p0, p.r0});
-    }
+    lay.m32 = d_m32;
+    lay.m64 = d_m64;
+    s->spheres = light_spheres(soa);
     DLight<float> l32[8];
     DLight<double> l64[8];
-    int nl = collect_lights(s, soa->materials, soa->n_materials, l32, l64);
+    int nl = collect_lights(s->spheres, soa->materials, soa->n_materials, l32, l64);
     int has_transmissive = any_transmissive(soa->materials, soa->n_materials);
     s->fog_host = soa->fog;
     s->has_water = any_water(soa->materials, soa->n_materials);
-
-    auto fill = [&](auto& v, const void* pr, const void* pl, const void* at, const void* mt, auto* lights, auto tag) {
-        typedef decltype(tag) R;
-        v.nodes = (const DNode*)lay.nodes;
-        v.n_nodes = lay.n_nodes;
-        v.root_ref = lay.root_ref;
-        v.nodes4 = (const DNode4*)lay.nodes4;
-        v.root4 = lay.root4;
-        v.bvh4 = lay.bvh4 ? 1 : 0;
-        v.prims = (const typename PrimT<R>::type*)pr;
-        v.planes = (const typename PrimT<R>::type*)pl;
-        v.n_planes = lay.n_planes;
-        v.plane_index = (const int*)lay.plidx;
-        v.attrs = (const DAttr<R>*)at;
-        v.mats = (const DMat<R>*)mt;
-        v.n_mats = soa->n_materials;
-        v.sky = conv_sky<R>(soa->sky);
-        v.fog = conv_fog<R>(soa->fog);
-        v.ocean = conv_ocean<R>(soa->ocean);
-        v.fog_kernel = needs_fog_kernel(soa->fog, s->has_water);
-        v.water_level = (R)soa->water_level;
-        v.water_material = soa->water_material;
-        v.has_transmissive = has_transmissive;
-        v.n_lights = nl;
-        for (int i = 0; i < 8; ++i) v.lights[i] = lights[i];
-    };
-    fill(s->v32, lay.p32, lay.pl32, lay.a32, d_m32, l32, 0.0f);
-    fill(s->v64, lay.p64, lay.pl64, lay.a64, d_m64, l64, 0.0);
+    fill_view(s->v32, lay, soa, s->has_water, l32, nl, has_transmissive);
+    fill_view(s->v64, lay, soa, s->has_water, l64, nl, has_transmissive);
     return 0;
-}
-
-bool has_attrs(const mh_scene_soa* soa)
-{
-    if (!soa->attrs) return false;
-    for (int i = 0; i < soa->n_prims; ++i)
-        if (soa->prims[i].flags != 0) return true;
-    return false;
 }
 
 } // namespace
@@ -1086,8 +766,8 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
     *out = nullptr;
     int rc = validate(soa);
     if (rc) return rc;
-    std::vector<int> inner;
-    rc = check_tree(soa, &inner);
+    HostLayout hl;
+    rc = build_host_layout(soa, &hl);
     if (rc) return rc;
 
     mh_scene* s = new_scene();
@@ -1096,82 +776,22 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
         mh_scene_destroy(s);
         return code;
     };
-
-    // device node numbering: interior nodes in DFS order
-    std::vector<int> dev_index(soa->n_nodes, -1);
-    for (size_t i = 0; i < inner.size(); ++i) dev_index[inner[i]] = (int)i;
-    auto ref_of = [&](int n) {
-        const mh_bvh_node& nd = soa->nodes[n];
-        if (nd.count > 0) return ~((nd.first << 3) | (nd.count - 1));
-        return dev_index[n];
-    };
-    std::vector<DNode> nodes(inner.size());
-    for (size_t i = 0; i < inner.size(); ++i) {
-        const mh_bvh_node& nd = soa->nodes[inner[i]];
-        const mh_bvh_node& L = soa->nodes[nd.left];
-        const mh_bvh_node& Rn = soa->nodes[nd.right];
-        DNode d;
-        d.c0 = make_float4(L.bmin[0], L.bmax[0], L.bmin[1], L.bmax[1]);
-        d.c1 = make_float4(L.bmin[2], L.bmax[2], Rn.bmin[0], Rn.bmax[0]);
-        d.c2 = make_float4(Rn.bmin[1], Rn.bmax[1], Rn.bmin[2], Rn.bmax[2]);
-        d.c3 = make_int4(ref_of(nd.left), ref_of(nd.right), 0, 0);
-        nodes[i] = d;
-    }
     Layout lay;
-    lay.root_ref = soa->n_nodes > 0 ? ref_of(0) : INT_MIN;
-    Bvh4Builder b4{soa, {}, 0};
-    lay.root4 = lay.root_ref;
-    if (soa->n_nodes > 0 && soa->nodes[0].count == 0) lay.root4 = b4.make(0, 1);
-    // 4-wide traversal pushes up to 3 entries per level
-    bool use4 = soa->n_nodes > 0 && 3 * b4.max_depth + 1 <= MH_STACK;
-    if (!use4) {
-        b4.nodes.clear();
-        lay.root4 = lay.root_ref;
-    }
-    lay.bvh4 = use4 && !b4.nodes.empty();
-    lay.n_nodes = (int)nodes.size();
+    lay.root_ref = hl.root_ref;
+    lay.root4 = hl.root4;
+    lay.bvh4 = hl.bvh4;
+    lay.n_nodes = (int)hl.nodes.size();
     lay.n_planes = soa->n_planes;
-
-    bool any_attr = has_attrs(soa);
-    std::vector<P32> p32(soa->n_leaf_prims);
-    std::vector<P64> p64(soa->n_leaf_prims);
-    std::vector<DAttr<float>> a32;
-    std::vector<DAttr<double>> a64;
-    if (any_attr) {
-        a32.resize(soa->n_leaf_prims);
-        a64.resize(soa->n_leaf_prims);
-    }
-    for (int i = 0; i < soa->n_leaf_prims; ++i) {
-        int pi = soa->leaf_prims[i];
-        mh_prim q = soa->prims[pi];
-        if (!soa->attrs) q.flags = 0;
-        p32[i] = pack32(q);
-        set_prim_index32(p32[i], pi);
-        p64[i] = pack64(q, pi);
-        if (any_attr) {
-            a32[i] = conv_attr<float>(soa->attrs[pi]);
-            a64[i] = conv_attr<double>(soa->attrs[pi]);
-        }
-    }
-    std::vector<P32> pl32(soa->n_planes);
-    std::vector<P64> pl64(soa->n_planes);
-    for (int i = 0; i < soa->n_planes; ++i) {
-        int pi = soa->planes[i];
-        pl32[i] = pack32(soa->prims[pi]);
-        set_prim_index32(pl32[i], pi);
-        pl64[i] = pack64(soa->prims[pi], pi);
-    }
-
     void *d_nodes, *d_nodes4, *d_p32, *d_p64, *d_pl32, *d_pl64, *d_plidx, *d_a32, *d_a64;
-    if ((rc = upload(s, nodes.data(), nodes.size() * sizeof(DNode), &d_nodes)) ||
-        (rc = upload(s, b4.nodes.data(), b4.nodes.size() * sizeof(DNode4), &d_nodes4)) ||
-        (rc = upload(s, p32.data(), p32.size() * sizeof(P32), &d_p32)) ||
-        (rc = upload(s, p64.data(), p64.size() * sizeof(P64), &d_p64)) ||
-        (rc = upload(s, pl32.data(), pl32.size() * sizeof(P32), &d_pl32)) ||
-        (rc = upload(s, pl64.data(), pl64.size() * sizeof(P64), &d_pl64)) ||
+    if ((rc = upload(s, hl.nodes.data(), hl.nodes.size() * sizeof(DNode), &d_nodes)) ||
+        (rc = upload(s, hl.nodes4.data(), hl.nodes4.size() * sizeof(DNode4), &d_nodes4)) ||
+        (rc = upload(s, hl.p32.data(), hl.p32.size() * sizeof(P32), &d_p32)) ||
+        (rc = upload(s, hl.p64.data(), hl.p64.size() * sizeof(P64), &d_p64)) ||
+        (rc = upload(s, hl.pl32.data(), hl.pl32.size() * sizeof(P32), &d_pl32)) ||
+        (rc = upload(s, hl.pl64.data(), hl.pl64.size() * sizeof(P64), &d_pl64)) ||
         (rc = upload(s, soa->planes, (size_t)soa->n_planes * sizeof(int), &d_plidx)) ||
-        (rc = upload(s, a32.data(), a32.size() * sizeof(DAttr<float>), &d_a32)) ||
-        (rc = upload(s, a64.data(), a64.size() * sizeof(DAttr<double>), &d_a64)))
+        (rc = upload(s, hl.a32.data(), hl.a32.size() * sizeof(DAttr<float>), &d_a32)) ||
+        (rc = upload(s, hl.a64.data(), hl.a64.size() * sizeof(DAttr<double>), &d_a64)))
         return bail(rc);
     lay.nodes = d_nodes;
     lay.nodes4 = d_nodes4;
@@ -1188,7 +808,7 @@ int mh_scene_create(const mh_scene_soa* soa, mh_scene** out)
     s->bvh.width4 = lay.bvh4 ? 1 : 0;
     s->bvh.build_ms = 0.0;
     s->bvh.device_built = 0;
-    if ((rc = finish_scene(s, soa, lay))) return bail(rc);
+    if ((rc = finish_scene(s, soa, lay, hl.m32, hl.m64))) return bail(rc);
     *out = s;
     return 0;
 }
@@ -1275,7 +895,13 @@ int mh_scene_create_gpu_bvh(const mh_scene_soa* soa, int max_leaf, int bins, mh_
     s->bvh.device_built = 1;
     s->tree_nodes = g.nodes;
     s->tree_leaf = g.leaf_prims;
-    if ((rc = finish_scene(s, soa, lay))) return bail(rc);
+    std::vector<DMat<float>> m32(soa->n_materials);
+    std::vector<DMat<double>> m64(soa->n_materials);
+    for (int i = 0; i < soa->n_materials; ++i) {
+        m32[i] = conv_mat<float>(soa->materials[i]);
+        m64[i] = conv_mat<double>(soa->materials[i]);
+    }
+    if ((rc = finish_scene(s, soa, lay, m32, m64))) return bail(rc);
     *out = s;
     return 0;
 }
@@ -1356,7 +982,7 @@ int mh_scene_set_materials(mh_scene* s, const mh_material* mats, int n)
     }
     DLight<float> l32[8];
     DLight<double> l64[8];
-    int nl = collect_lights(s, mats, n, l32, l64);
+    int nl = collect_lights(s->spheres, mats, n, l32, l64);
     int tr = any_transmissive(mats, n);
     s->v32.n_lights = s->v64.n_lights = nl;
     for (int i = 0; i < 8; ++i) {

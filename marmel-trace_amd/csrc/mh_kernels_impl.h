@@ -62,24 +62,7 @@ __global__ __launch_bounds__(256) void k_pathtrace(SceneView<R> sc, CamView<R> c
         int x = tx * 16 + (threadIdx.x & 15);
         int y = ty * 16 + (threadIdx.x >> 4);
         if (x < w && y < h) {
-            V3<R> acc = zero3<R>();
-            for (int si = 0; si < spp; ++si) {
-                uint32_t s = (uint32_t)(s0 + si);
-                R jx = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 0u);
-                R jy = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 1u);
-                R l1 = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 100u);
-                R l2 = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 101u);
-                R uu = ((R)x + jx) / (R)w;
-                R vv = R(1) - ((R)y + jy) / (R)h;
-                V3<R> o, d;
-                camera_ray(cam, uu, vv, l1, l2, &o, &d);
-                if (cam.dome > R(1e-4) && sc.water_material >= 0) {
-                    V3<R> pd = o + d * cam.dome;
-                    if (pd.y < water_height(sc, pd.x, pd.z)) o = pd - mk(R(0), R(1e-3), R(0));
-                }
-                acc = acc + pt_radiance<FOG>(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), &nq);
-            }
-            acc = acc * (R(1) / (R)spp);
+            V3<R> acc = pt_pixel<FOG>(sc, ts, cam, x, y, w, h, spp, s0, depth, &nq);
             size_t p = ((size_t)y * (size_t)w + (size_t)x) * 3;
             out.linear[p] = (float)acc.x;
             out.linear[p + 1] = (float)acc.y;
@@ -106,29 +89,7 @@ __global__ __launch_bounds__(256) void k_whitted(SceneView<R> sc, CamView<R> cam
     for (int i = lane; i < w * h; i += nlanes) {
         int pix = (int)tile_pixel((uint32_t)i, (uint32_t)w, (uint32_t)h);
         int x = pix % w, y = pix / w;
-        V3<R> acc = zero3<R>();
-        for (int si = 0; si < spp; ++si) {
-            int s = s0 + si;
-            R du, dv;
-            R jx = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 0));
-            R jy = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 1));
-            if (strat) {
-                du = ((R)(s % g) + jx) / (R)g;
-                dv = ((R)(s / g) + jy) / (R)g;
-            } else {
-                du = jx;
-                dv = jy;
-            }
-            R uu = ((R)x + du) / (R)w;
-            R vv = R(1) - ((R)y + dv) / (R)h;
-            R l1 = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 2));
-            R l2 = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 3));
-            V3<R> o, d;
-            camera_ray(cam, uu, vv, l1, l2, &o, &d);
-            acc = acc + whitted_sample(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, (uint32_t)s), stk, cap, &ovf,
-                                       &nq);
-        }
-        acc = acc * (R(1) / (R)spp);
+        V3<R> acc = whitted_pixel(sc, ts, cam, x, y, w, h, spp, s0, depth, g, strat, stk, cap, &ovf, &nq);
         size_t p = ((size_t)y * (size_t)w + (size_t)x) * 3;
         out.linear[p] = (float)acc.x;
         out.linear[p + 1] = (float)acc.y;

@@ -7,7 +7,16 @@
 // performs the same IEEE operations as the reference.
 #pragma once
 
+#if defined(MH_HOST_BUILD)
+#include "hd_compat.h"  // CPU build of the render code for sanitizer runs (tests/hostdev)
+#include <cassert>
+// bounds checks that only the CPU build compiles (the GPU code masks these
+// indices and relies on the bounds proved at upload)
+#define MH_HOST_ASSERT(c) assert(c)
+#else
 #include <hip/hip_runtime.h>
+#define MH_HOST_ASSERT(c) ((void)0)
+#endif
 #include <stdint.h>
 
 namespace mh {

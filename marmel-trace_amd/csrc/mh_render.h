@@ -186,10 +186,15 @@ __device__ bool occluded_segment(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d
         return any_hit(sc, ts, o, unit_dir(d), tmin, tmax);
     }
     R rem = tmax;
+    // The queries run on the unit direction, so hit distances (and the
+    // absorption over them) are in scene units and a sphere's unit-direction
+    // quadratic holds even for an unnormalised sun direction; the step past
+    // each transmissive hit goes along d as given.
+    const V3<R> u = unit_dir(d);
     for (int step = 0; step < 6; ++step) {
         DHit<R> h;
         ++*nq;
-        if (!intersect(sc, ts, o, d, tmin, rem, &h)) return false;
+        if (!intersect(sc, ts, o, u, tmin, rem, &h)) return false;
         if (h.mat < 0 || h.mat >= sc.n_mats) return true;
         const DMat<R>& m = sc.mats[h.mat];
         if (m.is_water || m.transparency > R(0.8)) {
@@ -884,6 +889,67 @@ __device__ V3<R> whitted_sample(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d,
         }
     }
     return acc;
+}
+
+// ---- per-pixel estimates (the megakernels' bodies) ----------------------------
+
+// Path-traced pixel (x, y) of a w x h image: samples s0 .. s0 + spp - 1 in
+// order, uniform jitter (channels 2s, 2s+1), thin-lens channels 2s+100 /
+// 2s+101, the dome's start-under-water test, then the mean (reference
+// src/pathtrace.c:1047-1096).
+template <bool FOG, typename R, typename TS>
+__device__ V3<R> pt_pixel(const SceneView<R>& sc, TS ts, const CamView<R>& cam, int x, int y, int w, int h, int spp,
+                          int s0, int depth, int* nq)
+{
+    V3<R> acc = zero3<R>();
+    for (int si = 0; si < spp; ++si) {
+        uint32_t s = (uint32_t)(s0 + si);
+        R jx = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 0u);
+        R jy = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 1u);
+        R l1 = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 100u);
+        R l2 = rand01<R>((uint32_t)x, (uint32_t)y, s * 2u + 101u);
+        R uu = ((R)x + jx) / (R)w;
+        R vv = R(1) - ((R)y + jy) / (R)h;
+        V3<R> o, d;
+        camera_ray(cam, uu, vv, l1, l2, &o, &d);
+        if (cam.dome > R(1e-4) && sc.water_material >= 0) {
+            V3<R> pd = o + d * cam.dome;
+            if (pd.y < water_height(sc, pd.x, pd.z)) o = pd - mk(R(0), R(1e-3), R(0));
+        }
+        acc = acc + pt_radiance<FOG>(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, s), nq);
+    }
+    return acc * (R(1) / (R)spp);
+}
+
+// Whitted pixel (x, y): stratified g x g offsets when spp = g^2 (strat), else
+// uniform; lens channels 2s+2 / 2s+3 (reference src/render.c:874-889).
+template <typename R, typename TS>
+__device__ V3<R> whitted_pixel(const SceneView<R>& sc, TS ts, const CamView<R>& cam, int x, int y, int w, int h,
+                               int spp, int s0, int depth, int g, bool strat, WItem<R>* stk, int cap, int* ovf,
+                               int* nq)
+{
+    V3<R> acc = zero3<R>();
+    for (int si = 0; si < spp; ++si) {
+        int s = s0 + si;
+        R du, dv;
+        R jx = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 0));
+        R jy = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 1));
+        if (strat) {
+            du = ((R)(s % g) + jx) / (R)g;
+            dv = ((R)(s / g) + jy) / (R)g;
+        } else {
+            du = jx;
+            dv = jy;
+        }
+        R uu = ((R)x + du) / (R)w;
+        R vv = R(1) - ((R)y + dv) / (R)h;
+        R l1 = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 2));
+        R l2 = rand01<R>((uint32_t)x, (uint32_t)y, (uint32_t)(s * 2 + 3));
+        V3<R> o, d;
+        camera_ray(cam, uu, vv, l1, l2, &o, &d);
+        acc = acc + whitted_sample(sc, ts, o, d, depth, hash3((uint32_t)x, (uint32_t)y, (uint32_t)s), stk, cap, ovf, nq);
+    }
+    return acc * (R(1) / (R)spp);
 }
 
 } // namespace mh

@@ -529,12 +529,14 @@ struct PrivStack {
         __device__ __forceinline__ bool empty() const { return sp == 0; }
         __device__ __forceinline__ void push(int r, float t)
         {
+            MH_HOST_ASSERT(sp >= 0 && sp < MH_STACK);
             pref[sp & (MH_STACK - 1)] = r;
             if (KeepT) pt[sp & (MH_STACK - 1)] = t;
             ++sp;
         }
         __device__ __forceinline__ int pop(float* t)
         {
+            MH_HOST_ASSERT(sp > 0);
             --sp;
             *t = KeepT ? pt[sp & (MH_STACK - 1)] : 0.f;
             return pref[sp & (MH_STACK - 1)];
@@ -562,6 +564,7 @@ template <int K> struct LdsStack {
         __device__ __forceinline__ void push(int r, float t)
         {
             if (sp == K) {
+                MH_HOST_ASSERT(psp < MH_STACK - K);
                 int j = psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1;
                 pref[j] = ref[base * stride];
                 if (KeepT) pt[j] = tt[base * stride];
@@ -583,6 +586,7 @@ template <int K> struct LdsStack {
                 return ref[k * stride];
             }
             --psp;
+            MH_HOST_ASSERT(psp >= 0);
             int j = psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1;
             *t = KeepT ? pt[j] : 0.f;
             return pref[j];
@@ -698,6 +702,13 @@ template <> struct Slab<double> {
 // default). The unsorted variant (0) measured faster in the megakernel but a
 // build with it produced wrong fp64 path-traced images, without a cause found;
 // it stays a tuning switch only.
+// Whether closest_hit<double> also visits the nearest hit child straight from
+// a register, as the fp32 path does (0, the default: push-all in fp64; the
+// register-held variant is a test switch, see closest_hit).
+#ifndef MH_HELD_FP64
+#define MH_HELD_FP64 0
+#endif
+
 #ifndef MH_ANY_SORT
 #define MH_ANY_SORT 1
 #endif
@@ -775,7 +786,7 @@ __device__ int closest_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tm
         Slab<R> sl;
         sl.init(o, d);
         typename TS::template Lane<true> st(ts);
-        if constexpr (sizeof(R) == 4) {
+        if constexpr (sizeof(R) == 4 || MH_HELD_FP64) {
             st.push(sc.root4, (float)tmin);
             int ref = 0;
             bool held = false;  // the nearest hit child, visited next without a stack round trip
@@ -848,7 +859,7 @@ __device__ int closest_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tm
         Slab<R> sl;
         sl.init(o, d);
         typename TS::template Lane<true> st(ts);
-        if constexpr (sizeof(R) == 4) {
+        if constexpr (sizeof(R) == 4 || MH_HELD_FP64) {
             st.push(sc.root_ref, (float)tmin);
             int ref = 0;
             bool held = false;  // the nearer hit child, visited next without a stack round trip

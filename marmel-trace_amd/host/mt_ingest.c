@@ -479,7 +479,7 @@ static double *concat(obj_chunk *c, int nt, size_t field, long *total)
     long k = 0;
     for (int t = 0; t < nt; ++t) {
         dvec *v = (dvec *)((char *)&c[t] + field);
-        memcpy(out + 3 * k, v->a, sizeof(double) * 3 * (size_t)v->n);
+        if (v->n) memcpy(out + 3 * k, v->a, sizeof(double) * 3 * (size_t)v->n); /* v->a is NULL when empty */
         k += v->n;
     }
     *total = n;

@@ -56,12 +56,13 @@ def test_ocean_world_fp64_matches_oracle():
                   int(off.sum()), "linear rel. diff there", np.abs(lin - m)[off].max(initial=0.0))
             if mode == 0:
                 mean = m
-        # Whitted: the 1/255 bar. Path tracing: the bar is statistical (below);
-        # through transparent water every path takes many refraction and
-        # roulette decisions, where last-bit differences of the device math
-        # (exp, sqrt) flip a few paths, so the fp64 image check is 99.8 %.
+        # the 1/255 bar for both modes. The sun's shadow rays leave the sea
+        # floor along the scene's unnormalised sun direction and cross the
+        # submerged sphere and the water surface: distances and absorption
+        # must be taken along the unit direction, as the reference's
+        # scene_intersect does (missing that left 0.2 % of pixels off)
         assert fr[1] >= 0.999
-        assert fr[0] >= 0.998
+        assert fr[0] >= 0.999
         # the dome point decides the starting medium: both kinds occur
         flat = ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm).soa)
         _, mflat, _, _ = flat.render(cam, w, h, 2, 8)

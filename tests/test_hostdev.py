@@ -1,0 +1,162 @@
+"""The library's render code built for the CPU: the per-pixel functions of
+the fp32 and fp64 megakernels (mh_render.h, mh_trace.h, mh_shade.h and the
+upload layout mh_layout.h) compiled by clang++ with the HIP qualifiers
+defined away (tests/hostdev), rendering through the same traversal stack the
+GPU megakernel uses. Built plain, under AddressSanitizer +
+UndefinedBehaviorSanitizer (bounds asserts on the traversal stacks), with
+every local pre-filled by a pattern and by zeros, and with the two traversal
+variants the library keeps off (unsorted 4-wide any-hit, the register-held
+nearest child in fp64).
+
+  * fp64 images equal the reference's own (golden default-scene images, made
+    by src/pathtrace.c's render loop) and the FP64 oracle's byte for byte —
+    the GPU's code with the CPU's libm,
+  * every variant renders the same bits as the plain build, in both
+    precisions, both modes and on both tree widths: no out-of-bounds access,
+    no undefined behaviour, no read of an uninitialised value, and the two
+    switched-off traversal variants are correct as written.
+"""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import checkers as ck
+import scenes
+from marmel_amd import HostScene, abi, native
+
+HD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hostdev")
+CXX = "/opt/rocm/llvm/bin/clang++"
+
+pytestmark = pytest.mark.skipif(not (os.path.exists(CXX) and shutil.which("make")),
+                                reason="clang++ from the ROCm toolchain not present")
+
+FP32, FP64 = abi.PREC_FP32, abi.PREC_FP64
+PT, WH = abi.MODE_PATHTRACE, abi.MODE_WHITTED
+
+
+@pytest.fixture(scope="module")
+def hd():
+    import sys
+    if HD not in sys.path:
+        sys.path.insert(0, HD)
+    import hostdev
+    hostdev.build()
+    return hostdev
+
+
+def cam_of(arr):
+    return abi.Camera.from_buffer_copy(arr.tobytes())
+
+
+def _generated(kind, seed=1, **kw):
+    hs = HostScene()
+    hs.generate(kind, seed=seed, **kw)
+    hs.build_bvh()
+    return hs, hs.camera(1.0)
+
+
+def _default(golden):
+    g = golden["default_scene"]
+    sky = abi.Sky.from_buffer_copy(g["sky"].tobytes())
+    hs = HostScene()
+    hs.add_materials(g["mats"])
+    hs.add_prims(g["prims"], g["attrs"])
+    hs.set_sky(sky)
+    hs.set_water(float(g["water_level"]), int(g["water_material"]))
+    hs.build_bvh()
+    return hs, cam_of(g["cam_160x90"])
+
+
+def _fog_forest():
+    hs, cam = _generated(native.GEN_FOREST, seed=5, grid=3, subdiv=2)
+    hs.set_fog(scenes.FOG_SETS["ashenvale"])
+    return hs, cam
+
+
+def _lens():
+    hs = scenes.lens_texture_scene()
+    return hs, scenes.lens_camera(hs, "anamorphic", 1.0)
+
+
+def _ocean():
+    from test_ingest import ocean_world
+    hs = ocean_world()
+    cam = hs.camera(1.0, eye=[0.0, 0.25, -9.0], at=[0.0, -0.2, 4.0], up=[0, 1, 0], vfov=55)
+    cam.dome_radius = 1.5
+    return hs, cam
+
+
+SCENES = {
+    "spheres": lambda g: _generated(native.GEN_SPHERES),
+    "terrain": lambda g: _generated(native.GEN_TERRAIN, grid=48),
+    "icosphere": lambda g: _generated(native.GEN_ICOSPHERE, subdiv=2),
+    "materials": lambda g: _generated(native.GEN_MATERIALS),
+    "forest_fog": lambda g: _fog_forest(),
+    "lens_texture": lambda g: _lens(),
+    "ocean": lambda g: _ocean(),
+    "default": _default,
+}
+
+
+@pytest.fixture(scope="module")
+def worlds(golden):
+    return {name: make(golden) for name, make in SCENES.items()}
+
+
+def oracle_of(hs):
+    prims, attrs, mats, sky, wl, wm = hs.arrays()
+    soa = hs.soa()
+    return ck.OracleScene(ck.SoaHolder(prims, attrs, mats, sky, wl, wm, fog=soa.fog, ocean=soa.ocean).soa)
+
+
+def test_host_build_reproduces_reference_golden_images(hd, golden):
+    """The default scene's reference images (src/pathtrace.c render loop),
+    fp64 path tracing and Whitted, 160x90, 16 spp, depth 6."""
+    g = golden["default_scene"]
+    hs, cam = _default(golden)
+    for mode, key in ((PT, "pt_160x90_s16_d6"), (WH, "whitted_160x90_s16_d6")):
+        rgb, _, _ = hd.render("hd_render", hs.soa(), cam, 160, 90, 16, 6, mode=mode, precision=FP64)
+        assert np.array_equal(rgb, g[key].reshape(90, 160, 3)), key
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_host_build_fp64_equals_oracle(hd, worlds, name):
+    hs, cam = worlds[name]
+    o = oracle_of(hs)
+    for mode in (PT, WH):
+        rgb, lin, _ = hd.render("hd_render", hs.soa(), cam, 40, 40, 4, 5, mode=mode, precision=FP64)
+        orgb, omean, _, _ = o.render(cam, 40, 40, 4, 5, mode=mode)
+        assert np.array_equal(rgb, orgb), (name, mode, (rgb != orgb).any(axis=-1).mean())
+        assert np.array_equal(lin, omean.astype(np.float32)), (name, mode)
+
+
+def _bits(hd, variant, hs, cam, prec, mode, width, size=32):
+    return hd.render(variant, hs.soa(), cam, size, size, 4, 5, mode=mode, precision=prec, width=width)
+
+
+@pytest.mark.parametrize("variant", ["hd_render_asan", "hd_render_pattern", "hd_render_zero",
+                                     "hd_render_anysort0", "hd_render_heldfp64"])
+def test_variant_renders_the_same_bits(hd, worlds, variant):
+    """Every scene, both precisions, both modes, binary and 4-wide tree."""
+    for name, (hs, cam) in worlds.items():
+        for prec in (FP64, FP32):
+            for mode in (PT, WH):
+                for width in (2, 4):
+                    a = _bits(hd, "hd_render", hs, cam, prec, mode, width)
+                    b = _bits(hd, variant, hs, cam, prec, mode, width)
+                    tag = (variant, name, prec, mode, width)
+                    assert np.array_equal(a[1], b[1]), tag
+                    assert np.array_equal(a[0], b[0]), tag
+                    assert a[2] == b[2], tag
+
+
+def test_host_library_and_oracle_under_sanitizers(hd):
+    """libmarmel_host's sources (generators, BVH build, OBJ ingest with
+    malformed faces, oceans, BMP encoding) and the oracle (renders, queries,
+    adaptive, adjoint, flare) run clean under ASan + UBSan."""
+    import subprocess
+    r = subprocess.run([os.path.join(HD, "out", "hd_host_asan")], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert r.stdout.strip().endswith("ok")
