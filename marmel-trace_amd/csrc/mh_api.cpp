@@ -347,7 +347,7 @@ int alloc_wf(int pool, size_t npix, bool jid, void** mem, size_t* bytes, WfState
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t sz[] = {al(P * 16), al(P * 16), al(P * 16), al(P * 16), al(P * 8), al(P * 9 * 16), al(P * 4),
                    al(P * 4),  al(P * 4),  al(P * 4),  al(P * 4),  al(P * 9 * 16), al(P * 9 * 16), al(64),
-                   al(npix * 3 * 4), al(64), jid ? al(P * 4) : 0, jid ? al(P * 4) : 0};
+                   al(npix * 3 * 4), al(64), jid ? al(P * 4) : 0, jid ? al(P * 4) : 0, al(P * 9 * 4)};
     const int nbuf = (int)(sizeof sz / sizeof sz[0]);
     size_t total = 0;
     for (size_t v : sz) total += v;
@@ -377,6 +377,7 @@ int alloc_wf(int pool, size_t npix, bool jid, void** mem, size_t* bytes, WfState
     w->batch = (unsigned long long*)ptrs[15];
     w->jid = (uint32_t*)ptrs[16];
     w->freeq = (uint32_t*)ptrs[17];
+    w->shq = (uint32_t*)ptrs[18];
     return 0;
 }
 

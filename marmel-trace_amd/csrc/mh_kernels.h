@@ -53,8 +53,9 @@ struct WfState {
     uint32_t* npend;  // pending estimates: lights in bits 0..3, sun in bit 4
     uint32_t* q[2];   // extension queues (ping-pong)
     uint32_t* term;   // paths that ended this bounce
-    float4* sh0;      // shadow rays: origin.xyz, tmax
+    float4* sh0;      // shadow rays: origin.xyz, tmax (entry k * n + i: light block k, queue position i)
     float4* sh1;      // direction.xyz, (slot << 4 | estimate) bits
+    uint32_t* shq;    // dense queue of the shadow entries cast this bounce (count in cnt[2])
     uint32_t* cnt;    // [0..1] ext queue lengths, [2] shadow, [3] term
     float* acc;       // per-pixel radiance sums (w*h*3)
     unsigned long long* batch;  // [0] first global path index of the current batch

@@ -75,6 +75,68 @@ __device__ __forceinline__ void block_append(uint32_t* const cnt[Q], const bool 
     }
 }
 
+// block_append<2> for the shade kernel's two queues, plus this thread's
+// shadow entries (bit k of `emitted`: entry k * n + i) appended to the dense
+// shadow queue `shq` (counter shc), in one pass with three barriers (every
+// thread calls it). Within the block's range the shadow entries are grouped
+// by light block, so 64 consecutive queue entries mostly hold one light's
+// rays from neighbouring paths. S <= 9.
+__device__ __forceinline__ void shade_append(uint32_t* const cnt[2], const bool want[2], uint32_t at[2],
+                                             uint32_t* shc, uint32_t* shq, uint32_t emitted, uint32_t S,
+                                             uint32_t n, uint32_t i)
+{
+    __shared__ uint32_t wn[2][4];
+    __shared__ uint32_t base[2];
+    __shared__ uint32_t sn[9][4];
+    __shared__ uint32_t off[9][4];
+    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+    const int nw = (int)(blockDim.x >> 6);
+    __syncthreads();  // the previous call's readers are done
+    uint64_t m[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        m[q] = __ballot(want[q]);
+        if (lane == 0) wn[q][wave] = (uint32_t)__popcll(m[q]);
+    }
+    for (uint32_t k = 0; k < S; ++k) {
+        uint64_t mk_ = __ballot((emitted >> k) & 1u);
+        if (lane == 0) sn[k][wave] = (uint32_t)__popcll(mk_);
+    }
+    __syncthreads();
+    // the queue atomics in wave 0, the shadow atomic in another wave when the
+    // block has one, so that their latencies overlap
+    const int tsh = blockDim.x > 64 ? 64 : 2;
+    if ((int)threadIdx.x < 2) {
+        int q = (int)threadIdx.x;
+        uint32_t tot = 0;
+        for (int w = 0; w < nw; ++w) tot += wn[q][w];
+        base[q] = tot ? atomicAdd(cnt[q], tot) : 0u;
+    } else if ((int)threadIdx.x == tsh) {
+        uint32_t tot = 0;
+        for (uint32_t k = 0; k < S; ++k)
+            for (int w = 0; w < nw; ++w) tot += sn[k][w];
+        uint32_t run = tot ? atomicAdd(shc, tot) : 0u;
+        for (uint32_t k = 0; k < S; ++k)
+            for (int w = 0; w < nw; ++w) {
+                off[k][w] = run;
+                run += sn[k][w];
+            }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint32_t b = base[q];
+        for (int w = 0; w < wave; ++w) b += wn[q][w];
+        at[q] = b + __builtin_amdgcn_mbcnt_hi((uint32_t)(m[q] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[q], 0u));
+    }
+    for (uint32_t k = 0; k < S; ++k) {
+        uint64_t mk_ = __ballot((emitted >> k) & 1u);
+        if ((emitted >> k) & 1u)
+            shq[off[k][wave] + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk_ >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mk_, 0u))] = k * n + i;
+    }
+}
+
 // Ray-query count of a block: one atomic per block (every thread calls it).
 __device__ __forceinline__ void count_rays_block(unsigned long long* ctr, int nq)
 {
@@ -101,8 +163,6 @@ template <int N> __device__ __forceinline__ void add_visits(unsigned long long* 
     }
 }
 
-// Shadow-queue tag of an empty entry (no ray for this light and path).
-constexpr uint32_t SH_NONE = 0xffffffffu;
 
 __global__ __launch_bounds__(64) void k_wf_reset(uint32_t* cnt, int which)
 {
@@ -527,12 +587,10 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
     const uint32_t* qin = st.q[which];
     uint32_t* qout = st.q[which ^ 1];
     int nq = 0;
-    // Shadow queue: fixed entries, light-major (entry li * n + i for queue
-    // position i), so a shadow wave holds one light's rays from neighbouring
-    // paths and no counter is touched; entries without a ray are tagged
-    // SH_NONE. S blocks: the lights (at most 8) and the sun, last.
+    // Shadow rays: fixed entries, light-major (entry li * n + i for queue
+    // position i); the entries actually cast go to the dense shadow queue
+    // (shadow_append). S blocks: the lights (at most 8) and the sun, last.
     const uint32_t S = MULTI ? (uint32_t)mj.sh_stride : (uint32_t)sc0.n_lights + 1u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st.cnt[2] = S * n;
     uint32_t* const cq[2] = {&st.cnt[which ^ 1], &st.cnt[3]};
     uint32_t n_round = (n + blockDim.x - 1u) / blockDim.x * blockDim.x;  // block-uniform trip count
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += gridDim.x * blockDim.x) {
@@ -720,12 +778,10 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
             st.p2[slot] = make_float4(beta.x, beta.y, beta.z, c.w);
             st.p3[slot] = make_float4(L.x, L.y, L.z, e.w);
             nq += 1;
-            for (uint32_t k = 0; k < S; ++k)
-                if (!(emitted & (1u << k))) st.sh1[(size_t)k * n + i] = make_float4(0.f, 0.f, 0.f, u2f(SH_NONE));
         }
         const bool want[2] = {alive, ended};
         uint32_t at[2];
-        block_append<2>(cq, want, at);
+        shade_append(cq, want, at, &st.cnt[2], st.shq, emitted, S, n, i);
         if (alive) qout[at[0]] = slot;
         if (ended) st.term[at[1]] = slot;
     }
@@ -751,73 +807,43 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 #else
     MH_STACK_PRIV(tsa)
 #endif
-    // The queue has an entry per (light, path), SH_NONE where no ray was
-    // cast. Each wave scans its own 64-entry chunks and packs the live
-    // entries into a per-wave LDS queue, and traverses only when it holds 64
-    // (or at the end): every traversal runs on a full wave, with no atomics.
-    __shared__ uint32_t wq[4][128];
-    uint32_t n = st.cnt[2];
+    // One lane per entry of the dense shadow queue (every entry holds a ray).
+    const uint32_t n = st.cnt[2];
     int nq = 0;
-    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
-    uint32_t* q = wq[wave];
     const uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t held = 0;  // live entries in q (wave-uniform)
-    for (uint32_t base = blockIdx.x * blockDim.x + (uint32_t)wave * 64u;; base += stride) {
-        bool more = base < n;
-        if (more) {
-            uint32_t i = base + (uint32_t)lane;
-            bool live = i < n && f2u(st.sh1[i].w) != SH_NONE;
-            uint64_t m = __ballot(live);
-            uint32_t at = held + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (live) q[at] = i;
-            held += (uint32_t)__popcll(m);
-            __builtin_amdgcn_wave_barrier();
-        }
-        if (held >= 64u || (!more && held > 0u)) {
-            uint32_t take = held < 64u ? held : 64u;
-            if ((uint32_t)lane < take) {
-                uint32_t i = q[lane];
-                float4 b = st.sh1[i], a = st.sh0[i];
-                uint32_t tag = f2u(b.w);
-                uint32_t slot = tag >> 4, k = tag & 15u;
-                const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
-                V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
-                bool occ;
-                if (TRANS) {
-                    V3<float> att;
-                    occ = occluded_segment(sc, tsw, o, d, 1e-3f, a.w, &att, &nq);
-                    if (!occ && k == 8 && att.x < 1e-4f && att.y < 1e-4f && att.z < 1e-4f) occ = true;
-                    if (!occ && sc.has_transmissive) {
-                        float4 c = st.pend[(size_t)slot * 9 + k];
-                        st.pend[(size_t)slot * 9 + k] = make_float4(c.x * att.x, c.y * att.y, c.z * att.z, 0.f);
-                    }
-                } else {
-                    ++nq;
-                    occ = any_hit<float, decltype(tsa), 2>(sc, tsa, o, unit_dir(d), 1e-3f, a.w);
-                }
-                if (!occ) atomicOr(&st.vis[slot], 1u << k);
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        uint32_t i = st.shq[v];
+        float4 b = st.sh1[i], a = st.sh0[i];
+        uint32_t tag = f2u(b.w);
+        uint32_t slot = tag >> 4, k = tag & 15u;
+        const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
+        V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+        bool occ;
+        if (TRANS) {
+            V3<float> att;
+            occ = occluded_segment(sc, tsw, o, d, 1e-3f, a.w, &att, &nq);
+            if (!occ && k == 8 && att.x < 1e-4f && att.y < 1e-4f && att.z < 1e-4f) occ = true;
+            if (!occ && sc.has_transmissive) {
+                float4 c = st.pend[(size_t)slot * 9 + k];
+                st.pend[(size_t)slot * 9 + k] = make_float4(c.x * att.x, c.y * att.y, c.z * att.z, 0.f);
             }
-            __builtin_amdgcn_wave_barrier();
-            // move the rest (fewer than 64) to the front
-            uint32_t rest = held - take;
-            uint32_t v = (uint32_t)lane < rest ? q[64 + lane] : 0u;
-            __builtin_amdgcn_wave_barrier();
-            if ((uint32_t)lane < rest) q[lane] = v;
-            __builtin_amdgcn_wave_barrier();
-            held = rest;
+        } else {
+            ++nq;
+            occ = any_hit<float, decltype(tsa), 2>(sc, tsa, o, unit_dir(d), 1e-3f, a.w);
         }
-        if (!more && held == 0u) break;
+        if (!occ) atomicOr(&st.vis[slot], 1u << k);
     }
     count_rays_block(rays, nq);
 }
 
 // Idle lanes of a wave at which the any-hit shadow kernel stops traversing to
 // hand its idle lanes new rays (dynamic fetch); 0 selects k_wf_shadow's
-// packed batches of 64 instead. Swept on the 1080p terrain frame (shadow ms
-// per frame): 0 -> 278, 8 -> 259, 16 -> 242, 24 -> 236, 32 -> 238,
-// 40 -> 240, 48 -> 233, 56 -> 230. A refill round stalls the wave's busy
-// lanes (queue scan, ray loads), so it pays to refill late.
+// one-ray-per-lane loop instead. Swept on the 1080p terrain frame (shadow ms
+// per frame), binary tree, tag-scanned queue: 0 -> 278, 8 -> 259, 16 -> 242,
+// 24 -> 236, 32 -> 238, 40 -> 240, 48 -> 233, 56 -> 230; 4-wide tree, dense
+// queue with prefetched entries (a refill round is about one memory round
+// trip): 16 -> 209, 32 -> 194, 48 -> 185.6, 56 -> 186. Frequent rounds still
+// cost more than the idle lanes they fill.
 #ifndef MH_SHADOW_REFILL
 #define MH_SHADOW_REFILL 56
 #endif
@@ -831,14 +857,17 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 #endif
 
 // Any-hit shadow queries (no transmissive material) with dynamic fetch. Each
-// lane owns one ray at a time and walks the binary tree one node (or one
-// leaf) per step. A wave packs the live entries of its 64-entry chunks into
-// an LDS queue (ballot + mbcnt, as k_wf_shadow does); whenever
-// MH_SHADOW_REFILL of its lanes have finished and queued or unscanned work
-// remains, it leaves the step loop and gives each idle lane the next queued
-// ray. Lanes that finish early therefore take new rays instead of idling
-// until the wave's longest traversal ends. Same visibility bits as
-// k_wf_shadow<., false> (the query result does not depend on the order).
+// lane owns one ray at a time and walks the tree one node (or one leaf) per
+// step. Each wave reads its own 64-entry chunks of the dense shadow queue
+// (chunk j at first + j * grid threads) and keeps the next entries, up to
+// 64, prefetched in LDS; whenever MH_SHADOW_REFILL of its lanes have
+// finished and work remains, it leaves the step loop, hands the prefetched
+// entries to its idle lanes and tops the prefetch up again. The new lanes'
+// ray loads and the prefetch's queue loads are in flight together, so a
+// refill round costs about one memory round trip. Lanes that finish early
+// therefore take new rays instead of idling until the wave's longest
+// traversal ends. Same visibility bits as k_wf_shadow<., false> (the query
+// result does not depend on the order).
 template <bool MULTI, bool COUNT = false>
 __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, WfState st, unsigned long long* rays,
                                                            WfMulti mj)
@@ -850,7 +879,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     constexpr int K = MH_SHADOW_ANY_LDS;
     static_assert(K > 0 && (K & (K - 1)) == 0 && K < MH_STACK, "LDS stack depth must be a power of two");
     __shared__ int lds_ref[K * 256];
-    __shared__ uint32_t wq[4][128];
+    __shared__ uint32_t wq[4][64];
     int* const my = lds_ref + threadIdx.x;
     const int bs = (int)blockDim.x;
     int ovf[MH_STACK - K];
@@ -876,10 +905,15 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
 
     const uint32_t n = st.cnt[2];
     const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
-    uint32_t* q = wq[wave];
+    uint32_t* q = wq[wave];  // prefetched entries (shadow entry indices), in hand-out order
+    // queue positions in 32 bits: n <= 9 * pool <= 9 * 2^26 and the grid's
+    // thread count is far below 2^31, so first + j * stride stays below 2^32
+    // for every chunk j the sequence reaches (the first beyond n ends it)
     const uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t next = blockIdx.x * blockDim.x + (uint32_t)wave * 64u;  // next chunk to scan (wave-uniform)
-    uint32_t held = 0;                                                // queued entries (wave-uniform)
+    const uint32_t first = blockIdx.x * blockDim.x + (uint32_t)wave * 64u;  // this wave's first chunk
+    uint32_t pos = 0;                // entries of the wave's chunk sequence fetched so far (wave-uniform)
+    uint32_t held = 0;               // prefetched entries in q (wave-uniform)
+    bool seq_more = first < n;       // entries of the sequence remain unfetched (wave-uniform)
     int nq = 0;
     bool busy = false;
     Slab<float> sl;
@@ -893,59 +927,17 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     for (;;) {
         const uint64_t idle = __ballot(!busy);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        // top up the queue to at least one entry per idle lane (q holds < 128:
-        // held < nidle <= 64 before each chunk adds at most 64)
-        while (held < nidle && next < n) {
-            uint32_t i = next + (uint32_t)lane;
-            bool live = i < n && f2u(st.sh1[i].w) != SH_NONE;
-            uint64_t m = __ballot(live);
-            uint32_t at = held + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (live) q[at] = i;
-            held += (uint32_t)__popcll(m);
-            next += stride;
-            __builtin_amdgcn_wave_barrier();
-        }
-        if (nidle == 64u && held == 0u) break;  // the loop above ran out of chunks
-        // the next queued rays to the idle lanes, in lane order
+        if (nidle == 64u && held == 0u && !seq_more) break;
+        // the prefetched entries to the idle lanes, in lane order
         const uint32_t take = held < nidle ? held : nidle;
+        bool got = false;
+        uint32_t mine = 0u;
         if (take > 0u) {
             uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
             if (!busy && rank < take) {
-                uint32_t i = q[rank];
-                float4 b = st.sh1[i], a = st.sh0[i];
-                uint32_t tag = f2u(b.w);
-                slot = tag >> 4;
-                bit = 1u << (tag & 15u);
-                const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
-                o = mk(a.x, a.y, a.z);
-                d = unit_dir(mk(b.x, b.y, b.z));
-                tmax = a.w;
-                ++nq;
-                // planes first (never in the tree), as any_hit does
-                bool occ = false;
-                for (int p = 0; p < sc.n_planes && !occ; ++p) {
-                    occ = occ_prim(load_prim(&sc.planes[p]), o, d, 1e-3f, tmax);
-                    if (COUNT) ++cnt_o;
-                }
-                if (COUNT) ++cnt_q;
-                if (!occ && sc.root_ref != INT_MIN) {
-                    nodes = sc.nodes;
-                    prims = sc.prims;
-                    sl.init(o, d);
-                    sp = base = psp = 0;
-                    if (MH_SHADOW_W4 && sc.nodes4) {
-                        nodes4 = sc.nodes4;
-                        push(sc.root4);
-                    } else {
-                        nodes4 = nullptr;
-                        push(sc.root_ref);
-                    }
-                    busy = true;
-                } else if (!occ) {
-                    atomicOr(&st.vis[slot], bit);
-                }
+                mine = q[rank];
+                got = true;
             }
             __builtin_amdgcn_wave_barrier();
             // move the rest (fewer than 64) to the front
@@ -956,7 +948,64 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
             __builtin_amdgcn_wave_barrier();
             held = rest;
         }
-        const bool more = held > 0u || next < n;  // wave-uniform
+        // top the prefetch up to 64 entries, its queue loads in flight with
+        // the new lanes' ray loads; the sequence's queue positions rise with
+        // the lane, so the lanes that find one form a prefix
+        uint32_t e = 0u, c = 0u;
+        bool ok = false;
+        if (held < 64u && seq_more) {
+            const uint32_t p = pos + (uint32_t)lane;
+            const uint32_t v = first + (p >> 6) * stride + (p & 63u);
+            ok = (uint32_t)lane < 64u - held && v < n;
+            if (ok) e = st.shq[v];
+            c = (uint32_t)__popcll(__ballot(ok));
+        }
+        if (got) {
+            // straight into the lane's ray registers (an idle lane's are free)
+            float4 b = st.sh1[mine], a = st.sh0[mine];
+            o = mk(a.x, a.y, a.z);
+            tmax = a.w;
+            d = mk(b.x, b.y, b.z);
+            slot = f2u(b.w);  // the tag, decoded below
+        }
+        if (held < 64u && seq_more) {
+            if (ok) q[held + (uint32_t)lane] = e;
+            __builtin_amdgcn_wave_barrier();
+            if (c < 64u - held) seq_more = false;
+            held += c;
+            pos += c;
+        }
+        if (got) {
+            bit = 1u << (slot & 15u);
+            slot >>= 4;
+            const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
+            d = unit_dir(d);
+            ++nq;
+            // planes first (never in the tree), as any_hit does
+            bool occ = false;
+            for (int pl = 0; pl < sc.n_planes && !occ; ++pl) {
+                occ = occ_prim(load_prim(&sc.planes[pl]), o, d, 1e-3f, tmax);
+                if (COUNT) ++cnt_o;
+            }
+            if (COUNT) ++cnt_q;
+            if (!occ && sc.root_ref != INT_MIN) {
+                nodes = sc.nodes;
+                prims = sc.prims;
+                sl.init(o, d);
+                sp = base = psp = 0;
+                if (MH_SHADOW_W4 && sc.nodes4) {
+                    nodes4 = sc.nodes4;
+                    push(sc.root4);
+                } else {
+                    nodes4 = nullptr;
+                    push(sc.root_ref);
+                }
+                busy = true;
+            } else if (!occ) {
+                atomicOr(&st.vis[slot], bit);
+            }
+        }
+        const bool more = held > 0u || seq_more;  // wave-uniform
         // traversal steps until the wave is done, or enough lanes are idle
         // while work remains
         for (;;) {

@@ -5,9 +5,9 @@ namespace mh {
 
 size_t wf_bytes_per_path()
 {
-    // p0..p3 + hit + 9 pending + vis + npend + 2 queues + term + 9 shadow rays
+    // p0..p3 + hit + 9 pending + vis + npend + 2 queues + term + 9 shadow rays + 9 shadow queue entries
     return 4 * sizeof(float4) + sizeof(float2) + 9 * sizeof(float4) + 2 * sizeof(uint32_t) +
-           3 * sizeof(uint32_t) + 9 * 2 * sizeof(float4);
+           3 * sizeof(uint32_t) + 9 * 2 * sizeof(float4) + 9 * sizeof(uint32_t);
 }
 
 hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st)
