@@ -39,8 +39,15 @@ struct __attribute__((aligned(16))) DNode {
 // node keep the binary tree's left-to-right order.
 struct __attribute__((aligned(16))) DNode4 {
     float4 lox, hix, loy, hiy, loz, hiz;
-    int4 ref;
+    int4 ref;   // INT_MIN in the slots past the child count
     int4 meta;  // x = child count (2..4)
+};
+
+// The first 112 B of a DNode4 (boxes and refs), for walks that tell the
+// unused slots by their INT_MIN ref and need no count.
+struct __attribute__((aligned(16))) DNode4Box {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 ref;
 };
 
 // fp32 record: a.w = kind | flags << 8, b.w = prim index, c.w = material

@@ -345,6 +345,12 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState
 #ifndef MH_EXTEND_W4_SORT
 #define MH_EXTEND_W4_SORT 1
 #endif
+// 1: a leaf's primitives are tested one per step, like nodes (one load per
+// lane per step); 0: the whole leaf in one step. Extend ms per frame:
+// 0 -> 57.1, 1 -> 55.5.
+#ifndef MH_EXTEND_PRIM_STEP
+#define MH_EXTEND_PRIM_STEP 1
+#endif
 
 // Nearest hit of every queued path with dynamic fetch, on the binary tree:
 // the search of closest_hit<float> (planes first, then the tree with the
@@ -413,6 +419,10 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
     const DNode* nodes = nullptr;
     const DNode4* nodes4 = nullptr;  // the 4-wide tree when the lane's scene view selects it
     const P32* prims = nullptr;
+    // MH_EXTEND_PRIM_STEP: the current leaf's untested primitives, the next
+    // one's index in bits 0..27 (tree primitives number at most 2^27) and
+    // how many remain in bits 28..31
+    uint32_t leaf = 0;
     uint32_t cnt_q = 0, cnt_n = 0, cnt_t = 0, cnt_o = 0;  // COUNT: queries, nodes, triangles, other prims
     for (;;) {
         const uint64_t idle = __ballot(!busy);
@@ -453,6 +463,7 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                     prims = sc.prims;
                     sl.init(o, d);
                     sp = base = psp = 0;
+                    leaf = 0;
                     ref = nodes4 ? sc.root4 : sc.root_ref;
                     held = true;
                     busy = true;
@@ -465,6 +476,11 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
         const bool more = entry(pos) < n;  // wave-uniform
         for (;;) {
             if (busy) {
+                int pj = -1;  // MH_EXTEND_PRIM_STEP: the primitive this lane tests in this step
+                if (MH_EXTEND_PRIM_STEP && (leaf >> 28)) {
+                    pj = (int)(leaf & 0x0fffffffu);
+                    leaf += 1u - (1u << 28);
+                } else {
                 bool work = true;
                 if (!held) {
                     float te;
@@ -521,24 +537,45 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                     } else {
                         int code = ~ref;
                         int first = code >> 3, cnt = (code & 7) + 1;
-                        for (int j = 0; j < cnt; ++j) {
-                            PD<float> p = load_prim(&prims[first + j]);
-                            if (COUNT) {
-                                if (p.kind == K_TRIANGLE) ++cnt_t;
-                                else ++cnt_o;
-                            }
-                            float t;
-                            if (hit_prim<float, false>(p, nullptr, o, d, tmin, closest, &t, nullptr)) {
-                                if (best == INT_MIN || t < closest || (t == closest && p.prim < bprim)) {
-                                    closest = t;
-                                    bprim = p.prim;
-                                    best = first + j;
+                        if (MH_EXTEND_PRIM_STEP) {
+                            pj = first;
+                            leaf = (uint32_t)(first + 1) | ((uint32_t)(cnt - 1) << 28);
+                        } else {
+                            for (int j = 0; j < cnt; ++j) {
+                                PD<float> p = load_prim(&prims[first + j]);
+                                if (COUNT) {
+                                    if (p.kind == K_TRIANGLE) ++cnt_t;
+                                    else ++cnt_o;
+                                }
+                                float t;
+                                if (hit_prim<float, false>(p, nullptr, o, d, tmin, closest, &t, nullptr)) {
+                                    if (best == INT_MIN || t < closest || (t == closest && p.prim < bprim)) {
+                                        closest = t;
+                                        bprim = p.prim;
+                                        best = first + j;
+                                    }
                                 }
                             }
                         }
                     }
                 }
-                if (!held && sp == 0 && psp == 0) {
+                }
+                if (MH_EXTEND_PRIM_STEP && pj >= 0) {
+                    PD<float> p = load_prim(&prims[pj]);
+                    if (COUNT) {
+                        if (p.kind == K_TRIANGLE) ++cnt_t;
+                        else ++cnt_o;
+                    }
+                    float t;
+                    if (hit_prim<float, false>(p, nullptr, o, d, tmin, closest, &t, nullptr)) {
+                        if (best == INT_MIN || t < closest || (t == closest && p.prim < bprim)) {
+                            closest = t;
+                            bprim = p.prim;
+                            best = pj;
+                        }
+                    }
+                }
+                if (!held && (leaf >> 28) == 0u && sp == 0 && psp == 0) {
                     st.hit[slot] = make_float2(closest, __int_as_float(best));
                     busy = false;
                 }
@@ -855,6 +892,13 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 #ifndef MH_SHADOW_W4
 #define MH_SHADOW_W4 1
 #endif
+// 1: a leaf's primitives are tested one per step, like nodes, so every step
+// of the wave waits for one load per lane; 0: the whole leaf in one step
+// (a load per primitive, one after another). 1080p terrain frame, shadow ms
+// per frame: 0 -> 185.6, 1 -> 163.6 (both at 80 VGPRs, 6 waves per SIMD).
+#ifndef MH_SHADOW_PRIM_STEP
+#define MH_SHADOW_PRIM_STEP 1
+#endif
 
 // Any-hit shadow queries (no transmissive material) with dynamic fetch. Each
 // lane owns one ray at a time and walks the tree one node (or one leaf) per
@@ -919,10 +963,15 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     Slab<float> sl;
     V3<float> o = zero3<float>(), d = zero3<float>();
     float tmax = 0.f;
-    uint32_t slot = 0, bit = 0;
-    const DNode* nodes = nullptr;
-    const DNode4* nodes4 = nullptr;  // MH_SHADOW_W4: the 4-wide tree of the lane's scene (or null)
+    uint32_t tag = 0;  // slot << 4 | estimate of the lane's ray
+    // the lane's tree: the 4-wide one when MH_SHADOW_W4 and the scene has
+    // one (flagged in bit 0: node arrays are 256-byte aligned), else the binary one
+    uintptr_t tree = 0;
     const P32* prims = nullptr;
+    // MH_SHADOW_PRIM_STEP: the current leaf's untested primitives, the next
+    // one's index in bits 0..27 (tree primitives number at most 2^27) and
+    // how many remain in bits 28..31
+    uint32_t leaf = 0;
     uint32_t cnt_q = 0, cnt_n = 0, cnt_t = 0, cnt_o = 0;  // COUNT: queries, nodes, triangles, other prims
     for (;;) {
         const uint64_t idle = __ballot(!busy);
@@ -966,7 +1015,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
             o = mk(a.x, a.y, a.z);
             tmax = a.w;
             d = mk(b.x, b.y, b.z);
-            slot = f2u(b.w);  // the tag, decoded below
+            tag = f2u(b.w);
         }
         if (held < 64u && seq_more) {
             if (ok) q[held + (uint32_t)lane] = e;
@@ -976,9 +1025,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
             pos += c;
         }
         if (got) {
-            bit = 1u << (slot & 15u);
-            slot >>= 4;
-            const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
+            const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, tag >> 4);
             d = unit_dir(d);
             ++nq;
             // planes first (never in the tree), as any_hit does
@@ -989,20 +1036,20 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
             }
             if (COUNT) ++cnt_q;
             if (!occ && sc.root_ref != INT_MIN) {
-                nodes = sc.nodes;
                 prims = sc.prims;
                 sl.init(o, d);
                 sp = base = psp = 0;
+                leaf = 0;
                 if (MH_SHADOW_W4 && sc.nodes4) {
-                    nodes4 = sc.nodes4;
+                    tree = (uintptr_t)sc.nodes4 | 1u;
                     push(sc.root4);
                 } else {
-                    nodes4 = nullptr;
+                    tree = (uintptr_t)sc.nodes;
                     push(sc.root_ref);
                 }
                 busy = true;
             } else if (!occ) {
-                atomicOr(&st.vis[slot], bit);
+                atomicOr(&st.vis[tag >> 4], 1u << (tag & 15u));
             }
         }
         const bool more = held > 0u || seq_more;  // wave-uniform
@@ -1010,32 +1057,38 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
         // while work remains
         for (;;) {
             if (busy) {
-                int ref = pop();
                 bool occ = false;
-                if (MH_SHADOW_W4 && ref >= 0 && nodes4) {
-                    DNode4 nd = ldg(&nodes4[ref]);
+                int pj = -1;  // MH_SHADOW_PRIM_STEP: the primitive this lane tests in this step
+                if (MH_SHADOW_PRIM_STEP && (leaf >> 28)) {
+                    pj = (int)(leaf & 0x0fffffffu);
+                    leaf += 1u - (1u << 28);
+                } else {
+                int ref = pop();
+                if (MH_SHADOW_W4 && ref >= 0 && (tree & 1u)) {
                     if (COUNT) ++cnt_n;
-                    int cnt = nd.meta.x;
                     if (MH_SHADOW_W4 == 2) {
+                        DNode4 nd = ldg(&((const DNode4*)(tree & ~(uintptr_t)1))[ref]);
                         int r[4], nh;
                         float t[4];
                         children4(sl, nd, 1e-3f, tmax, r, t, &nh);
                         for (int i = 3; i >= 0; --i)
                             if (i < nh) push(r[i]);
                     } else {
+                        // boxes and refs only: a slot past the child count has ref INT_MIN
+                        DNode4Box nd = ldg((const DNode4Box*)&((const DNode4*)(tree & ~(uintptr_t)1))[ref]);
                         bool h0, h1, h2, h3;
                         float t0, t1, t2, t3;
                         sl.one(nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, 1e-3f, tmax, &h3, &t3);
                         sl.one(nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, 1e-3f, tmax, &h2, &t2);
                         sl.one(nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, 1e-3f, tmax, &h1, &t1);
                         sl.one(nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, 1e-3f, tmax, &h0, &t0);
-                        if (h3 && cnt > 3) push(nd.ref.w);
-                        if (h2 && cnt > 2) push(nd.ref.z);
+                        if (h3 && nd.ref.w != INT_MIN) push(nd.ref.w);
+                        if (h2 && nd.ref.z != INT_MIN) push(nd.ref.z);
                         if (h1) push(nd.ref.y);
                         if (h0) push(nd.ref.x);
                     }
                 } else if (ref >= 0) {
-                    DNode nd = ldg(&nodes[ref]);
+                    DNode nd = ldg(&((const DNode*)tree)[ref]);
                     if (COUNT) ++cnt_n;
                     bool hl, hr;
                     float tl, tr;
@@ -1052,17 +1105,31 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
                 } else {
                     int code = ~ref;
                     int first = code >> 3, cnt = (code & 7) + 1;
-                    for (int j = 0; j < cnt && !occ; ++j) {
-                        PD<float> pr = load_prim(&prims[first + j]);
-                        occ = occ_prim(pr, o, d, 1e-3f, tmax);
-                        if (COUNT) {
-                            if (pr.kind == K_TRIANGLE) ++cnt_t;
-                            else ++cnt_o;
+                    if (MH_SHADOW_PRIM_STEP) {
+                        pj = first;
+                        leaf = (uint32_t)(first + 1) | ((uint32_t)(cnt - 1) << 28);
+                    } else {
+                        for (int j = 0; j < cnt && !occ; ++j) {
+                            PD<float> pr = load_prim(&prims[first + j]);
+                            occ = occ_prim(pr, o, d, 1e-3f, tmax);
+                            if (COUNT) {
+                                if (pr.kind == K_TRIANGLE) ++cnt_t;
+                                else ++cnt_o;
+                            }
                         }
                     }
                 }
-                if (occ || (sp == 0 && psp == 0)) {
-                    if (!occ) atomicOr(&st.vis[slot], bit);
+                }
+                if (MH_SHADOW_PRIM_STEP && pj >= 0) {
+                    PD<float> pr = load_prim(&prims[pj]);
+                    occ = occ_prim(pr, o, d, 1e-3f, tmax);
+                    if (COUNT) {
+                        if (pr.kind == K_TRIANGLE) ++cnt_t;
+                        else ++cnt_o;
+                    }
+                }
+                if (occ || ((leaf >> 28) == 0u && sp == 0 && psp == 0)) {
+                    if (!occ) atomicOr(&st.vis[tag >> 4], 1u << (tag & 15u));
                     busy = false;
                 }
             }
