@@ -37,6 +37,19 @@ template <typename R> struct PD {
 // from a device table (the batched engine's scenes) is generic to the
 // compiler, which then fetches with flat instructions that also wait on LDS
 // traffic (the traversal stack).
+// Read of an LDS (__shared__) slot as an LDS access that the compiler may
+// neither drop nor merge: a read of one of two places (an LDS slot or a
+// private overflow array) would otherwise become a single flat load after
+// the branch, which also waits on vector memory.
+template <typename T> __device__ __forceinline__ T lds_read(const T* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const volatile __attribute__((address_space(3))) T*)p;
+#else
+    return *p;
+#endif
+}
+
 typedef unsigned int mh_u4v __attribute__((ext_vector_type(4)));
 template <typename T> __device__ __forceinline__ T ldg(const T* p)
 {

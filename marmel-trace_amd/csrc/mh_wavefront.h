@@ -389,16 +389,20 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
         ++sp;
     };
     auto pop = [&](float* t) -> int {
+        // unconditional LDS reads (a slot of this lane either way)
+        int k = ((base + sp - 1) & (K - 1)) * bs;
+        float tv = lds_read(&myt[k]);
+        int v = lds_read(&myr[k]);
         if (sp > 0) {
             --sp;
-            int k = ((base + sp) & (K - 1)) * bs;
-            *t = myt[k];
-            return myr[k];
+        } else {
+            --psp;
+            int j = psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1;
+            tv = ovt[j];
+            v = ovr[j];
         }
-        --psp;
-        int j = psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1;
-        *t = ovt[j];
-        return ovr[j];
+        *t = tv;
+        return v;
     };
 
     const uint32_t n = st.cnt[which];
@@ -939,12 +943,15 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
         ++sp;
     };
     auto pop = [&]() -> int {
+        // the LDS read is unconditional (a slot of this lane either way)
+        int v = lds_read(&my[((base + sp - 1) & (K - 1)) * bs]);
         if (sp > 0) {
             --sp;
-            return my[((base + sp) & (K - 1)) * bs];
+        } else {
+            --psp;
+            v = ovf[psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1];
         }
-        --psp;
-        return ovf[psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1];
+        return v;
     };
 
     const uint32_t n = st.cnt[2];
