@@ -930,17 +930,32 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     __shared__ uint32_t wq[4][64];
     int* const my = lds_ref + threadIdx.x;
     const int bs = (int)blockDim.x;
-    int ovf[MH_STACK - K];
+    // capacity MH_STACK: K - 1 ring slots (push4 keeps one free) + the overflow
+    constexpr int NO = MH_STACK - K + 1;
+    int ovf[NO];
     int sp = 0, base = 0, psp = 0;
+    auto spill = [&]() {
+        ovf[psp < NO - 1 ? psp : NO - 1] = my[base * bs];
+        ++psp;
+        base = (base + 1) & (K - 1);
+        --sp;
+    };
     auto push = [&](int r) {
-        if (sp == K) {
-            ovf[psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1] = my[base * bs];
-            ++psp;
-            base = (base + 1) & (K - 1);
-            --sp;
-        }
+        if (sp == K) spill();
         my[((base + sp) & (K - 1)) * bs] = r;
         ++sp;
+    };
+    // Pushes of a 4-wide node's hit children without a branch per child:
+    // room4() makes the ring hold four more entries plus one free slot
+    // (oldest entries to the overflow, rare); then put() writes each child's
+    // ref to the next free slot and advances past it only if the child was
+    // hit, so a missed child's write lands in a free slot.
+    auto room4 = [&]() {
+        while (sp > K - 5) spill();
+    };
+    auto put = [&](bool h, int r) {
+        my[((base + sp) & (K - 1)) * bs] = r;
+        sp += (int)h;
     };
     auto pop = [&]() -> int {
         // the LDS read is unconditional (a slot of this lane either way)
@@ -949,7 +964,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
             --sp;
         } else {
             --psp;
-            v = ovf[psp < MH_STACK - K - 1 ? psp : MH_STACK - K - 1];
+            v = ovf[psp < NO - 1 ? psp : NO - 1];
         }
         return v;
     };
@@ -1083,16 +1098,17 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
                     } else {
                         // boxes and refs only: a slot past the child count has ref INT_MIN
                         DNode4Box nd = ldg((const DNode4Box*)&((const DNode4*)(tree & ~(uintptr_t)1))[ref]);
-                        bool h0, h1, h2, h3;
-                        float t0, t1, t2, t3;
-                        sl.one(nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, 1e-3f, tmax, &h3, &t3);
-                        sl.one(nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, 1e-3f, tmax, &h2, &t2);
-                        sl.one(nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, 1e-3f, tmax, &h1, &t1);
-                        sl.one(nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, 1e-3f, tmax, &h0, &t0);
-                        if (h3 && nd.ref.w != INT_MIN) push(nd.ref.w);
-                        if (h2 && nd.ref.z != INT_MIN) push(nd.ref.z);
-                        if (h1) push(nd.ref.y);
-                        if (h0) push(nd.ref.x);
+                        bool h;
+                        float t;
+                        room4();
+                        sl.one(nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, 1e-3f, tmax, &h, &t);
+                        put(h && nd.ref.w != INT_MIN, nd.ref.w);
+                        sl.one(nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, 1e-3f, tmax, &h, &t);
+                        put(h && nd.ref.z != INT_MIN, nd.ref.z);
+                        sl.one(nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, 1e-3f, tmax, &h, &t);
+                        put(h, nd.ref.y);
+                        sl.one(nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, 1e-3f, tmax, &h, &t);
+                        put(h, nd.ref.x);
                     }
                 } else if (ref >= 0) {
                     DNode nd = ldg(&((const DNode*)tree)[ref]);
