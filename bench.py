@@ -40,14 +40,19 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "marmel-trace_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Measured HBM bytes per k_wf_shadow launch of the batched engine (the
-# default main engine) on this workload: rocprofv3 --pmc FETCH_SIZE
-# (11,014,738.3 KiB, x 2.0) + WRITE_SIZE (1,821,232.9 KiB, x 1.0) per
+# Measured HBM bytes per k_wf_shadow_refill<true> launch of the batched
+# engine (the default main engine) on this workload: rocprofv3 --pmc
+# FETCH_SIZE (6,799,617.6 KiB, x 2.0) + WRITE_SIZE (414,309.3 KiB, x 1.0) per
 # dispatch, averaged over the clocked frame's 18 dispatches (pool 2^26 paths,
 # 16384 workgroups), the factors being what the same counters read for a
-# 1 GiB copy. Re-measure when the shadow kernel or its launch changes.
-SHADOW_TRAFFIC_PER_LAUNCH = 24423.1e6
-SHADOW_LIMITER = "dependent node-fetch latency and lane divergence, not HBM bandwidth"
+# 1 GiB copy (bench.py --hbm-calib 1024). Re-measure when the shadow kernel or
+# its launch changes.
+SHADOW_TRAFFIC_PER_LAUNCH = 14349.9e6
+# What the same kernel's counters show limits it (8.76 ms per launch):
+# VALUBusy 72.9 %, VALUUtilization 43.6 %, occupancy 70.2 % (6 of 8 waves per
+# SIMD at 80 VGPRs), L2 hit 62.1 %, HBM 1.64 TB/s.
+SHADOW_LIMITER = ("VALU issue (73 % busy) at 44 % lane utilisation: divergent traversal steps; "
+                  "HBM at 1.6 TB/s, 20 % of peak")
 W, H, SPP, DEPTH = 1920, 1080, 64, 8
 GRID = 1024
 
@@ -66,7 +71,7 @@ def build_scene(grid):
 
 # Per-unit algorithmic bytes (fixed fp32 record sizes of the device layout).
 B_NODE, B_TRI, B_SPH, B_PLANE = 64, 48, 32, 16
-B_NODE4 = 128                       # 4-wide node (SoA child boxes, refs, count): the shadow kernel's tree
+B_NODE4 = 112                       # 4-wide node as the shadow kernel reads it (SoA child boxes + refs)
 B_CLOSEST_IO, B_ANY_IO = 48, 36     # ray in (32 B) + hit (16 B) / visibility (4 B)
 B_BOUNCE, B_SAMPLE = 192, 12        # path state + material per bounce; accumulator per sample
 
