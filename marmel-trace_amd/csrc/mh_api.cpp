@@ -621,7 +621,10 @@ int batch_trial(mh_batch* b, const WfMulti& mj, int max_depth, int feat, mh_tune
 // each mode has its own best pool), then grid blocks, then block size.
 int autotune(mh_batch* b, const WfMulti& mj, int max_depth, int feat, mh_tune* t)
 {
-    long long trial = std::min<long long>(mj.total, 1LL << 23);
+    // every candidate runs the same first paths (as many as the largest pool
+    // holds), so that rates compare the same work: the workload's first
+    // jobs can be much cheaper or dearer than the rest
+    long long trial = std::min<long long>(mj.total, 1LL << 26);
     double best = -1.0;
     mh_tune cur = *t;
     // pools up to the 2^26 slots the shadow tags address (35 GB of state:
