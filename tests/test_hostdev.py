@@ -160,3 +160,38 @@ def test_host_library_and_oracle_under_sanitizers(hd):
     r = subprocess.run([os.path.join(HD, "out", "hd_host_asan")], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert r.stdout.strip().endswith("ok")
+
+
+def _wider_cases():
+    out = []
+    gens = (("spheres", native.GEN_SPHERES, {}), ("terrain", native.GEN_TERRAIN, dict(grid=32)),
+            ("icosphere", native.GEN_ICOSPHERE, dict(subdiv=2)), ("materials", native.GEN_MATERIALS, {}),
+            ("forest", native.GEN_FOREST, dict(grid=3, subdiv=2)))
+    for seed in (1, 2, 3):
+        for name, kind, kw in gens:
+            out.append((f"{name}{seed}", lambda kind=kind, seed=seed, kw=kw: _generated(kind, seed, **kw)))
+    for name, fog in scenes.FOG_SETS.items():
+        def make(fog=fog):
+            hs, cam = _generated(native.GEN_FOREST, seed=5, grid=3, subdiv=2)
+            hs.set_fog(fog)
+            return hs, cam
+        out.append((f"fog_{name}", make))
+    for lc in scenes.LENS_CAMS:
+        def make(lc=lc):
+            hs = scenes.lens_texture_scene()
+            return hs, scenes.lens_camera(hs, lc, 1.0)
+        out.append((f"lens_{lc}", make))
+    return out
+
+
+def test_host_build_fp64_equals_oracle_on_more_scenes(hd):
+    """Five generators x three seeds, the six fog settings, the three lens
+    cameras: 64x64, 4 spp, path tracing depth 8 and Whitted depth 6."""
+    for name, make in _wider_cases():
+        hs, cam = make()
+        o = oracle_of(hs)
+        for mode, depth in ((PT, 8), (WH, 6)):
+            rgb, lin, _ = hd.render("hd_render", hs.soa(), cam, 64, 64, 4, depth, mode=mode, precision=FP64)
+            orgb, omean, _, _ = o.render(cam, 64, 64, 4, depth, mode=mode)
+            assert np.array_equal(rgb, orgb), (name, mode)
+            assert np.array_equal(lin, omean.astype(np.float32)), (name, mode)
