@@ -1861,6 +1861,11 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     mj.visits = b->visits;
     mj.sh_stride = 1;
     for (const auto& v : views) mj.sh_stride = std::max(mj.sh_stride, v.n_lights + 1);
+    // every job on one scene: the kernels read its view at a launch-uniform
+    // index (scalar loads) instead of looking up each path's job
+    mj.one_scene = bj[0].scene;
+    for (const BJob& d : bj)
+        if (d.scene != mj.one_scene) mj.one_scene = -1;
 
     if (t.pool == 0) t.pool = 1 << 21;
     if (t.grid_blocks == 0) t.grid_blocks = 4096;

@@ -53,9 +53,9 @@ struct WfState {
     uint32_t* npend;  // pending estimates: lights in bits 0..3, sun in bit 4
     uint32_t* q[2];   // extension queues (ping-pong)
     uint32_t* term;   // paths that ended this bounce
-    float4* sh0;      // shadow rays: origin.xyz, tmax (entry k * n + i: light block k, queue position i)
+    float4* sh0;      // shadow rays cast this bounce, dense (count in cnt[2]): origin.xyz, tmax
     float4* sh1;      // direction.xyz, (slot << 4 | estimate) bits
-    uint32_t* shq;    // dense queue of the shadow entries cast this bounce (count in cnt[2])
+    uint32_t* shq;    // batched engine: scene index of each shadow ray
     uint32_t* cnt;    // [0..1] ext queue lengths, [2] shadow, [3] term
     float* acc;       // per-pixel radiance sums (w*h*3)
     unsigned long long* batch;  // [0] first global path index of the current batch
@@ -83,6 +83,7 @@ struct WfMulti {
     unsigned long long* acc64;   // fixed-point (2^-24) per-pixel sums, 3 per pixel
     int stream;                  // path regeneration: per-slot bounce, free-slot list
     int sh_stride;               // shadow-queue blocks per bounce: max lights over the scenes + 1 (sun)
+    int one_scene;               // >= 0: every job renders scenes[one_scene] (WF_ONE kernels); -1: per job
     unsigned long long* visits;  // WF_COUNT launches: traversal counts, MH_V_* (device)
 };
 
@@ -125,6 +126,11 @@ hipError_t wf_enqueue_end(const WfState& ws, int npix, int spp, const LaunchOut&
 // material (shadow rays walk through it) / has fog or marched water / is
 // traversed on its 4-wide tree.
 enum { WF_TRANS = 1, WF_FOG = 2, WF_WIDE = 4, WF_COUNT = 8 };
+// Launch modes of the wavefront kernels. WF_SINGLE: one scene and one job,
+// both kernel arguments. The batched engine's job table: WF_ONE when every
+// job renders the same scene (read at a launch-uniform index, so its fields
+// are scalar loads), WF_TABLE when each path slot looks its job's scene up.
+enum WfMode { WF_SINGLE = 0, WF_ONE = 1, WF_TABLE = 2 };
 // WF_COUNT: the extension and any-hit shadow kernels count their traversal
 // work into WfMulti::visits (order of mh_traversal_counts; batched engine).
 

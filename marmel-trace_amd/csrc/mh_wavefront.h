@@ -23,11 +23,13 @@
 // queue length from device memory, so the bounce sequence of a batch has
 // fixed launch parameters and is captured once into a hipGraph.
 //
-// Batched engine (MULTI = true): the same kernels over a table of jobs (a
+// Batched engine (MODE WF_ONE or WF_TABLE): the same kernels over a table of jobs (a
 // camera view of one scene each, with its own size, spp and depth). Global
 // path indices run job by job, sample-major; a batch takes the next `pool`
 // of them whatever jobs they belong to, so small jobs fill the machine
-// together. Each slot records its job; the kernels look up the job's scene.
+// together. Each slot records its job; the kernels look up the job's scene
+// (WF_TABLE), or read the one scene every job renders at a launch-uniform
+// index (WF_ONE).
 // A batch may then hold several samples of one pixel, so pixel sums are
 // 64-bit fixed-point atomics (2^-24 units): integer addition is associative,
 // and the result does not depend on the order in which paths end.
@@ -76,14 +78,18 @@ __device__ __forceinline__ void block_append(uint32_t* const cnt[Q], const bool 
 }
 
 // block_append<2> for the shade kernel's two queues, plus this thread's
-// shadow entries (bit k of `emitted`: entry k * n + i) appended to the dense
-// shadow queue `shq` (counter shc), in one pass with three barriers (every
-// thread calls it). Within the block's range the shadow entries are grouped
-// by light block, so 64 consecutive queue entries mostly hold one light's
-// rays from neighbouring paths. S <= 9.
+// shadow rays (bit k of `emitted`: light block k, staged in stage[k * bs +
+// tid] as direction + tmax, the shared origin in stage[9 * bs + tid])
+// appended to the dense shadow queue (counter shc) as records sh0 = origin +
+// tmax, sh1 = direction + (slot << 4 | estimate), shq = the scene index, in
+// one pass with three barriers (every thread calls it). Within the block's
+// range the rays are grouped by light block, so consecutive queue entries
+// mostly hold one light's rays from neighbouring paths. S <= 9; the sun is
+// block S - 1 and estimate 8.
+template <bool TABLE>
 __device__ __forceinline__ void shade_append(uint32_t* const cnt[2], const bool want[2], uint32_t at[2],
-                                             uint32_t* shc, uint32_t* shq, uint32_t emitted, uint32_t S,
-                                             uint32_t n, uint32_t i)
+                                             uint32_t* shc, const WfState& st, const float4* stage,
+                                             uint32_t emitted, uint32_t S, uint32_t slot, uint32_t scn)
 {
     __shared__ uint32_t wn[2][4];
     __shared__ uint32_t base[2];
@@ -129,11 +135,19 @@ __device__ __forceinline__ void shade_append(uint32_t* const cnt[2], const bool 
         for (int w = 0; w < wave; ++w) b += wn[q][w];
         at[q] = b + __builtin_amdgcn_mbcnt_hi((uint32_t)(m[q] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[q], 0u));
     }
+    const uint32_t bs = blockDim.x;
+    const float4 org = stage[9u * bs + threadIdx.x];
     for (uint32_t k = 0; k < S; ++k) {
         uint64_t mk_ = __ballot((emitted >> k) & 1u);
-        if ((emitted >> k) & 1u)
-            shq[off[k][wave] + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk_ >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mk_, 0u))] = k * n + i;
+        if ((emitted >> k) & 1u) {
+            const uint32_t p = off[k][wave] + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk_ >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mk_, 0u));
+            const float4 w = stage[k * bs + threadIdx.x];
+            const uint32_t est = k == S - 1u ? 8u : k;
+            st.sh0[p] = make_float4(org.x, org.y, org.z, w.w);
+            st.sh1[p] = make_float4(w.x, w.y, w.z, __uint_as_float((slot << 4) | est));
+            if (TABLE) st.shq[p] = scn;
+        }
     }
 }
 
@@ -187,22 +201,31 @@ __device__ __forceinline__ int find_job(const WfMulti& mj, long long g)
 }
 
 // The scene a path slot renders.
-template <bool MULTI>
+template <int MODE>
 __device__ __forceinline__ const SceneView<float>& slot_scene(const SceneView<float>& sc, const WfMulti& mj,
                                                               const WfState& st, uint32_t slot)
 {
-    return MULTI ? mj.scenes[mj.jobs[st.jid[slot]].scene] : sc;
+    return MODE == WF_TABLE ? mj.scenes[mj.jobs[st.jid[slot]].scene]
+                            : MODE == WF_ONE ? mj.scenes[mj.one_scene] : sc;
+}
+
+// The scene of table index `i` (a job's scene) under launch mode MODE.
+template <int MODE>
+__device__ __forceinline__ const SceneView<float>& mode_scene(const SceneView<float>& sc, const WfMulti& mj, int i)
+{
+    return MODE == WF_TABLE ? mj.scenes[i] : MODE == WF_ONE ? mj.scenes[mj.one_scene] : sc;
 }
 
 
 // Start global path g (camera ray, seed, unit throughput) in path slot `slot`.
-template <bool MULTI>
+template <int MODE>
 __device__ __forceinline__ void start_path(const SceneView<float>& sc0, const CamView<float>& cam0, const WfJob& job,
                                            const WfState& st, const WfMulti& mj, long long g, uint32_t slot)
 {
+    constexpr bool MULTI = MODE != WF_SINGLE;
     int j = MULTI ? find_job(mj, g) : 0;
     const BJob* bj = MULTI ? &mj.jobs[j] : nullptr;
-    const SceneView<float>& sc = MULTI ? mj.scenes[bj->scene] : sc0;
+    const SceneView<float>& sc = mode_scene<MODE>(sc0, mj, MULTI ? bj->scene : 0);
     const CamView<float>& cam = MULTI ? bj->cam : cam0;
     long long l = MULTI ? g - bj->begin : g;
     int npix = MULTI ? bj->npix : job.npix;
@@ -234,16 +257,17 @@ __device__ __forceinline__ void start_path(const SceneView<float>& sc0, const Ca
     if (MULTI) st.jid[slot] = (uint32_t)j;
 }
 
-template <bool MULTI>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_wf_gen(SceneView<float> sc0, CamView<float> cam0, WfJob job, WfState st,
                                                 WfMulti mj)
 {
+    constexpr bool MULTI = MODE != WF_SINGLE;
     unsigned long long g0 = st.batch[0];
     long long remain = (MULTI ? mj.total : job.total) - (long long)g0;
     int n = (int)(remain < job.pool ? remain : job.pool);
     if (blockIdx.x == 0 && threadIdx.x == 0) st.cnt[0] = (uint32_t)(n > 0 ? n : 0);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        start_path<MULTI>(sc0, cam0, job, st, mj, (long long)g0 + i, (uint32_t)i);
+        start_path<MODE>(sc0, cam0, job, st, mj, (long long)g0 + i, (uint32_t)i);
         st.q[0][i] = (uint32_t)i;
     }
 }
@@ -264,6 +288,7 @@ __device__ __forceinline__ uint32_t refill_count(const WfState& st, const WfMult
     return remain < (long long)nfree ? (uint32_t)remain : nfree;
 }
 
+template <int MODE>
 __global__ __launch_bounds__(256) void k_wf_refill(WfState st, WfMulti mj, int which)
 {
     uint32_t nfree = st.cnt[4];
@@ -279,7 +304,7 @@ __global__ __launch_bounds__(256) void k_wf_refill(WfState st, WfMulti mj, int w
         uint32_t slot = 0;
         if (active) {
             slot = st.freeq[nfree - 1u - i];
-            start_path<true>(none, cnone, jnone, st, mj, (long long)g0 + i, slot);
+            start_path<MODE>(none, cnone, jnone, st, mj, (long long)g0 + i, slot);
         }
         const bool want[1] = {active};
         uint32_t at[1];
@@ -310,7 +335,7 @@ __global__ __launch_bounds__(256) void k_wf_stream_init(WfState st, int pool)
     }
 }
 
-template <bool MULTI>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState st, int which, WfMulti mj)
 {
 #if MH_EXTEND_LDS
@@ -322,7 +347,7 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState
     const uint32_t* q = st.q[which];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t slot = q[i];
-        const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
+        const SceneView<float>& sc = slot_scene<MODE>(sc0, mj, st, slot);
         float4 a = st.p0[slot], b = st.p1[slot];
         float t;
         int prim;
@@ -361,7 +386,7 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState
 // wave-uniform position; whenever MH_EXTEND_REFILL lanes have finished and
 // entries remain, the idle lanes take the next ones. Same hits as
 // k_wf_extend.
-template <bool MULTI, bool COUNT = false>
+template <int MODE, bool COUNT = false>
 __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, WfState st, int which, WfMulti mj)
 {
     constexpr int K = MH_EXTEND_LDS;
@@ -438,7 +463,7 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
             uint32_t i = entry(pos + rank);
             if (!busy && i < n) {
                 slot = q[i];
-                const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
+                const SceneView<float>& sc = slot_scene<MODE>(sc0, mj, st, slot);
                 float4 a = st.p0[slot], b = st.p1[slot];
                 o = mk(a.x, a.y, a.z);
                 d = mk(b.x, b.y, b.z);
@@ -617,32 +642,40 @@ __device__ __forceinline__ V3<float> resolve_pending(const WfState& st, uint32_t
     return L;
 }
 
-template <bool MULTI, bool FOG>
+template <int MODE, bool FOG>
 __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState st, WfJob job, int which, int bounce,
                                                    unsigned long long* rays, WfMulti mj)
 {
+    constexpr bool MULTI = MODE != WF_SINGLE;
     // FOG: fog segments and marched water; the marches trace their sun
     // shadow rays here, with this stack
     MH_STACK_PRIV(ts)
+    // the shadow rays a thread casts, staged until their queue positions are
+    // known (shade_append): 10 * blockDim.x float4s of dynamic LDS
+    extern __shared__ float4 wf_stage[];
     uint32_t n = st.cnt[which];
     const uint32_t* qin = st.q[which];
     uint32_t* qout = st.q[which ^ 1];
     int nq = 0;
-    // Shadow rays: fixed entries, light-major (entry li * n + i for queue
-    // position i); the entries actually cast go to the dense shadow queue
-    // (shadow_append). S blocks: the lights (at most 8) and the sun, last.
+    // Shadow rays: S light blocks, the lights (at most 8) and the sun, last;
+    // each ray cast is staged in wf_stage[block * blockDim.x + thread] and
+    // appended to the dense shadow queue by shade_append.
     const uint32_t S = MULTI ? (uint32_t)mj.sh_stride : (uint32_t)sc0.n_lights + 1u;
+    const uint32_t bs = blockDim.x;
     uint32_t* const cq[2] = {&st.cnt[which ^ 1], &st.cnt[3]};
     uint32_t n_round = (n + blockDim.x - 1u) / blockDim.x * blockDim.x;  // block-uniform trip count
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += gridDim.x * blockDim.x) {
         bool active = i < n;
         uint32_t slot = active ? qin[i] : 0u;
         bool alive = false, ended = false;
-        uint32_t emitted = 0u;  // shadow entries written (bit = block)
+        uint32_t emitted = 0u;  // shadow rays staged (bit = block)
+        uint32_t scn = 0u;      // scene index of the slot (batched engine)
         V3<float> nray_o = zero3<float>(), nray_d = zero3<float>();
         if (active) {
-            const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
-            int depth = MULTI ? mj.jobs[st.jid[slot]].depth : job.depth;
+            const BJob* bj = MULTI ? &mj.jobs[st.jid[slot]] : nullptr;
+            scn = MULTI ? (uint32_t)bj->scene : 0u;
+            const SceneView<float>& sc = mode_scene<MODE>(sc0, mj, (int)scn);
+            int depth = MULTI ? bj->depth : job.depth;
             float4 a = st.p0[slot], b = st.p1[slot], c = st.p2[slot], e = st.p3[slot];
             float2 hh = st.hit[slot];
             V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
@@ -745,9 +778,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
                                 }
                             }
                             if (emit) {
-                                size_t at = (size_t)li * n + i;
-                                st.sh0[at] = make_float4(so.x, so.y, so.z, tmax);
-                                st.sh1[at] = make_float4(wi.x, wi.y, wi.z, u2f((slot << 4) | (uint32_t)li));
+                                wf_stage[(uint32_t)li * bs + threadIdx.x] = make_float4(wi.x, wi.y, wi.z, tmax);
                                 st.pend[(size_t)slot * 9 + li] = make_float4(cc.x, cc.y, cc.z, 0.f);
                                 emitted |= 1u << li;
                             }
@@ -766,14 +797,13 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
                                 }
                             }
                             if (emit) {
-                                size_t at = (size_t)(S - 1u) * n + i;
-                                st.sh0[at] = make_float4(so.x, so.y, so.z, 1e30f);
-                                st.sh1[at] = make_float4(Ls.x, Ls.y, Ls.z, u2f((slot << 4) | 8u));
+                                wf_stage[(S - 1u) * bs + threadIdx.x] = make_float4(Ls.x, Ls.y, Ls.z, 1e30f);
                                 st.pend[(size_t)slot * 9 + 8] = make_float4(cc.x, cc.y, cc.z, 0.f);
                                 emitted |= 1u << (S - 1u);
                                 npend |= 16u;
                             }
                         }
+                        if (emitted) wf_stage[9u * bs + threadIdx.x] = make_float4(so.x, so.y, so.z, 0.f);
                         st.vis[slot] = 0u;
                         st.npend[slot] = npend;
                         // ---- BSDF sample, throughput, roulette
@@ -822,7 +852,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
         }
         const bool want[2] = {alive, ended};
         uint32_t at[2];
-        shade_append(cq, want, at, &st.cnt[2], st.shq, emitted, S, n, i);
+        shade_append<MODE == WF_TABLE>(cq, want, at, &st.cnt[2], st, wf_stage, emitted, S, slot, scn);
         if (alive) qout[at[0]] = slot;
         if (ended) st.term[at[1]] = slot;
     }
@@ -834,7 +864,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
 // Beer-Lambert attenuation). Otherwise every first hit is opaque and the
 // query is a single any-hit traversal (same answer), with a smaller stack and
 // register footprint.
-template <bool MULTI, bool TRANS>
+template <int MODE, bool TRANS>
 __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState st, unsigned long long* rays,
                                                     WfMulti mj)
 {
@@ -848,16 +878,15 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 #else
     MH_STACK_PRIV(tsa)
 #endif
-    // One lane per entry of the dense shadow queue (every entry holds a ray).
+    // One lane per ray of the dense shadow queue.
     const uint32_t n = st.cnt[2];
     int nq = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-        uint32_t i = st.shq[v];
-        float4 b = st.sh1[i], a = st.sh0[i];
+        float4 b = st.sh1[v], a = st.sh0[v];
         uint32_t tag = f2u(b.w);
         uint32_t slot = tag >> 4, k = tag & 15u;
-        const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, slot);
+        const SceneView<float>& sc = mode_scene<MODE>(sc0, mj, MODE == WF_TABLE ? (int)st.shq[v] : 0);
         V3<float> o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
         bool occ;
         if (TRANS) {
@@ -883,10 +912,11 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 // per frame), binary tree, tag-scanned queue: 0 -> 278, 8 -> 259, 16 -> 242,
 // 24 -> 236, 32 -> 238, 40 -> 240, 48 -> 233, 56 -> 230; 4-wide tree, dense
 // queue with prefetched entries (a refill round is about one memory round
-// trip): 16 -> 209, 32 -> 194, 48 -> 185.6, 56 -> 186. Frequent rounds still
-// cost more than the idle lanes they fill.
+// trip): 16 -> 209, 32 -> 194, 48 -> 185.6, 56 -> 186; rays prefetched into
+// LDS by direct loads (a round reads LDS only): 16 -> 167.2, 24 -> 160.1,
+// 32 -> 158.2, 48 -> 158.3, 56 -> 170.4.
 #ifndef MH_SHADOW_REFILL
-#define MH_SHADOW_REFILL 56
+#define MH_SHADOW_REFILL 32
 #endif
 // Tree the any-hit shadow kernel walks: 0 = the binary tree, 1 = the 4-wide
 // tree where the scene has one (children in index order), 2 = the 4-wide
@@ -905,29 +935,34 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 #endif
 
 // Any-hit shadow queries (no transmissive material) with dynamic fetch. Each
-// lane owns one ray at a time and walks the tree one node (or one leaf) per
-// step. Each wave reads its own 64-entry chunks of the dense shadow queue
-// (chunk j at first + j * grid threads) and keeps the next entries, up to
-// 64, prefetched in LDS; whenever MH_SHADOW_REFILL of its lanes have
-// finished and work remains, it leaves the step loop, hands the prefetched
-// entries to its idle lanes and tops the prefetch up again. The new lanes'
-// ray loads and the prefetch's queue loads are in flight together, so a
-// refill round costs about one memory round trip. Lanes that finish early
-// therefore take new rays instead of idling until the wave's longest
-// traversal ends. Same visibility bits as k_wf_shadow<., false> (the query
-// result does not depend on the order).
-template <bool MULTI, bool COUNT = false>
+// lane owns one ray at a time and walks the tree one node (or one leaf
+// primitive) per step. Each wave reads its own chunks of MH_SHQ_CHUNK = 32
+// rays of the dense shadow queue (chunk j at queue position (j * waves +
+// wave) * 32) straight into LDS with direct global-to-LDS loads: two chunk
+// slots per wave, each [32 origin + tmax records][32 direction + tag
+// records], so a chunk is one load instruction. Whenever MH_SHADOW_REFILL of
+// its lanes have finished and rays remain, the wave leaves the step loop,
+// hands the prefetched rays to its idle lanes (LDS reads: the loads were
+// issued at least one round earlier) and refills every chunk slot whose rays
+// are all handed out; only the batched engine's scene lookup goes to
+// memory. Lanes that finish early therefore take new rays instead of idling
+// until the wave's longest traversal ends. Same visibility bits as
+// k_wf_shadow<., false> (the query result does not depend on the order).
+template <int MODE, bool COUNT = false>
 __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, WfState st, unsigned long long* rays,
                                                            WfMulti mj)
 {
+    constexpr bool TABLE = MODE == WF_TABLE;
     // per-lane stack of node refs: the top K in LDS (lane-interleaved, slot
     // k of a lane at my[k * blockDim.x]), older ones in a private overflow
     // array; held in scalars so that restarting it for a new ray is three
     // register writes
     constexpr int K = MH_SHADOW_ANY_LDS;
     static_assert(K > 0 && (K & (K - 1)) == 0 && K < MH_STACK, "LDS stack depth must be a power of two");
+    constexpr uint32_t CH = 32u;  // rays per chunk
     __shared__ int lds_ref[K * 256];
-    __shared__ uint32_t wq[4][64];
+    __shared__ float4 ring[4][2][2 * CH];    // per wave: 2 chunk slots of [origin + tmax][direction + tag]
+    __shared__ uint32_t rscn[4][2][CH];      // WF_TABLE: the rays' scene indices
     int* const my = lds_ref + threadIdx.x;
     const int bs = (int)blockDim.x;
     // capacity MH_STACK: K - 1 ring slots (push4 keeps one free) + the overflow
@@ -970,16 +1005,39 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     };
 
     const uint32_t n = st.cnt[2];
-    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
-    uint32_t* q = wq[wave];  // prefetched entries (shadow entry indices), in hand-out order
-    // queue positions in 32 bits: n <= 9 * pool <= 9 * 2^26 and the grid's
-    // thread count is far below 2^31, so first + j * stride stays below 2^32
-    // for every chunk j the sequence reaches (the first beyond n ends it)
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t first = blockIdx.x * blockDim.x + (uint32_t)wave * 64u;  // this wave's first chunk
-    uint32_t pos = 0;                // entries of the wave's chunk sequence fetched so far (wave-uniform)
-    uint32_t held = 0;               // prefetched entries in q (wave-uniform)
-    bool seq_more = first < n;       // entries of the sequence remain unfetched (wave-uniform)
+    // the wave index through readfirstlane, so that everything derived from
+    // it (chunk cursors, LDS slot addresses) is known to be wave-uniform
+    const int lane = (int)__lane_id(), wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // queue positions in 32 bits: n <= 9 * pool <= 9 * 2^26, and a chunk
+    // start is computed only while the previous one was below n, so every
+    // start stays below n + 32 * waves < 2^32
+    const uint32_t nw = gridDim.x * (blockDim.x >> 6);             // waves in the grid
+    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (uint32_t)wave;
+    uint32_t jn = 0;          // next chunk of the wave's sequence to load (wave-uniform)
+    uint32_t rd = 0, wr = 0;  // rays of the sequence handed out / loaded so far (wave-uniform)
+    bool seq_more = gw * CH < n;  // chunks of the sequence remain unloaded (wave-uniform)
+    // Load chunk jn into slot jn & 1: lanes 0..31 the origin records, lanes
+    // 32..63 the direction records, each lane's 16 bytes at slot + 16 * lane
+    // (positions past the queue end read its last ray and are never handed out).
+    auto load_chunk = [&]() {
+        const uint32_t start = (jn * nw + gw) * CH;
+        const uint32_t valid = n - start < CH ? n - start : CH;
+        uint32_t p = start + ((uint32_t)lane & (CH - 1u));
+        if (p > n - 1u) p = n - 1u;
+        auto* dst = (__attribute__((address_space(3))) void*)&ring[wave][jn & 1u][0];
+        if (lane < (int)CH) {
+            __builtin_amdgcn_global_load_lds((const void*)(st.sh0 + p), dst, 16, 0, 0);
+            if (TABLE)
+                __builtin_amdgcn_global_load_lds((const void*)(st.shq + p),
+                                                 (__attribute__((address_space(3))) void*)&rscn[wave][jn & 1u][0], 4,
+                                                 0, 0);
+        } else {
+            __builtin_amdgcn_global_load_lds((const void*)(st.sh1 + p), dst, 16, 0, 0);
+        }
+        wr += valid;
+        ++jn;
+        seq_more = valid == CH && (jn * nw + gw) * CH < n;
+    };
     int nq = 0;
     bool busy = false;
     Slab<float> sl;
@@ -998,56 +1056,40 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     for (;;) {
         const uint64_t idle = __ballot(!busy);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (nidle == 64u && held == 0u && !seq_more) break;
-        // the prefetched entries to the idle lanes, in lane order
-        const uint32_t take = held < nidle ? held : nidle;
+        const uint32_t avail = wr - rd;
+        if (nidle == 64u && avail == 0u && !seq_more) break;
+        // the loaded rays to the idle lanes, in lane order
+        const uint32_t take = avail < nidle ? avail : nidle;
         bool got = false;
-        uint32_t mine = 0u;
+        uint32_t scn = 0u;
         if (take > 0u) {
+            // the chunk loads of earlier rounds have landed in LDS: s_waitcnt
+            // vmcnt(0) (the traversal steps leave no other load in flight)
+            __builtin_amdgcn_s_waitcnt(0x0f70);
             uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
             if (!busy && rank < take) {
-                mine = q[rank];
+                const uint32_t s = rd + rank;
+                const float4 a = ring[wave][(s / CH) & 1u][s & (CH - 1u)];
+                const float4 b = ring[wave][(s / CH) & 1u][CH + (s & (CH - 1u))];
+                if (TABLE) scn = rscn[wave][(s / CH) & 1u][s & (CH - 1u)];
+                o = mk(a.x, a.y, a.z);
+                tmax = a.w;
+                d = mk(b.x, b.y, b.z);
+                tag = f2u(b.w);
                 got = true;
             }
-            __builtin_amdgcn_wave_barrier();
-            // move the rest (fewer than 64) to the front
-            uint32_t rest = held - take;
-            uint32_t v = (uint32_t)lane < rest ? q[take + (uint32_t)lane] : 0u;
-            __builtin_amdgcn_wave_barrier();
-            if ((uint32_t)lane < rest) q[lane] = v;
-            __builtin_amdgcn_wave_barrier();
-            held = rest;
+            rd += take;
         }
-        // top the prefetch up to 64 entries, its queue loads in flight with
-        // the new lanes' ray loads; the sequence's queue positions rise with
-        // the lane, so the lanes that find one form a prefix
-        uint32_t e = 0u, c = 0u;
-        bool ok = false;
-        if (held < 64u && seq_more) {
-            const uint32_t p = pos + (uint32_t)lane;
-            const uint32_t v = first + (p >> 6) * stride + (p & 63u);
-            ok = (uint32_t)lane < 64u - held && v < n;
-            if (ok) e = st.shq[v];
-            c = (uint32_t)__popcll(__ballot(ok));
-        }
+        // every LDS read of this round is done before a chunk load may
+        // overwrite its slot: s_waitcnt lgkmcnt(0)
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        // a slot is free once every ray of the chunk loaded two before into it
+        // is handed out
+        if (seq_more && (jn < 2u || rd >= (jn - 1u) * CH)) load_chunk();
+        if (seq_more && (jn < 2u || rd >= (jn - 1u) * CH)) load_chunk();
         if (got) {
-            // straight into the lane's ray registers (an idle lane's are free)
-            float4 b = st.sh1[mine], a = st.sh0[mine];
-            o = mk(a.x, a.y, a.z);
-            tmax = a.w;
-            d = mk(b.x, b.y, b.z);
-            tag = f2u(b.w);
-        }
-        if (held < 64u && seq_more) {
-            if (ok) q[held + (uint32_t)lane] = e;
-            __builtin_amdgcn_wave_barrier();
-            if (c < 64u - held) seq_more = false;
-            held += c;
-            pos += c;
-        }
-        if (got) {
-            const SceneView<float>& sc = slot_scene<MULTI>(sc0, mj, st, tag >> 4);
+            const SceneView<float>& sc = mode_scene<MODE>(sc0, mj, (int)scn);
             d = unit_dir(d);
             ++nq;
             // planes first (never in the tree), as any_hit does
@@ -1074,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
                 atomicOr(&st.vis[tag >> 4], 1u << (tag & 15u));
             }
         }
-        const bool more = held > 0u || seq_more;  // wave-uniform
+        const bool more = wr > rd || seq_more;  // wave-uniform
         // traversal steps until the wave is done, or enough lanes are idle
         // while work remains
         for (;;) {
@@ -1177,9 +1219,10 @@ __device__ __forceinline__ unsigned long long to_fixed(float v)
     return __float2ull_rn(fminf(v, 67108864.0f) * 16777216.0f);
 }
 
-template <bool MULTI>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_wf_finalize(WfState st, WfMulti mj)
 {
+    constexpr bool MULTI = MODE != WF_SINGLE;
     uint32_t n = st.cnt[3];
     // path regeneration appends the freed slots: block-uniform trip count
     uint32_t n_round = (MULTI && mj.stream) ? (n + blockDim.x - 1u) / blockDim.x * blockDim.x : n;

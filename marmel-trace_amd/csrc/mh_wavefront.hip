@@ -20,26 +20,30 @@ hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st)
 
 // The extension kernel: dynamic fetch (on the tree width each scene view
 // selects).
-template <bool MULTI> static auto extend_kernel(int feat)
+template <int MODE> static auto extend_kernel(int feat)
 {
     if (MH_EXTEND_REFILL > 0) {
-        if constexpr (MULTI)
-            if (feat & WF_COUNT) return k_wf_extend_refill<true, true>;
-        return k_wf_extend_refill<MULTI, false>;
+        if constexpr (MODE != WF_SINGLE)
+            if (feat & WF_COUNT) return k_wf_extend_refill<MODE, true>;
+        return k_wf_extend_refill<MODE, false>;
     }
-    return k_wf_extend<MULTI>;
+    return k_wf_extend<MODE>;
 }
 
 // The any-hit shadow kernel (no transmissive material in the launch).
-template <bool MULTI> static auto shadow_any(int feat)
+template <int MODE> static auto shadow_any(int feat)
 {
     if (MH_SHADOW_REFILL > 0) {
-        if constexpr (MULTI)
-            if (feat & WF_COUNT) return k_wf_shadow_refill<true, true>;
-        return k_wf_shadow_refill<MULTI, false>;
+        if constexpr (MODE != WF_SINGLE)
+            if (feat & WF_COUNT) return k_wf_shadow_refill<MODE, true>;
+        return k_wf_shadow_refill<MODE, false>;
     }
-    return k_wf_shadow<MULTI, false>;
+    return k_wf_shadow<MODE, false>;
 }
+
+// Dynamic LDS of the shade kernel: the shadow rays of each thread, staged
+// until their queue positions are known (9 light blocks + the origin).
+static size_t shade_lds(int block) { return (size_t)10 * (size_t)block * sizeof(float4); }
 
 // Launch wrapped in the optional clock.
 #define MH_TIMED(kind, launch)                                                                            \
@@ -56,22 +60,22 @@ template <bool MULTI> static auto shadow_any(int feat)
     } while (0)
 
 // The bounce sequence of one batch, shared by both engines.
-template <bool MULTI>
+template <int MODE>
 static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<float>& cam, const WfJob& job,
                                   const WfState& ws, const WfMulti& mj, unsigned long long* rays, int grid, int block,
                                   int feat, hipStream_t st, KernelClock* kc)
 {
-    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<MULTI, true> : shadow_any<MULTI>(feat);
-    auto shade = (feat & WF_FOG) ? k_wf_shade<MULTI, true> : k_wf_shade<MULTI, false>;
+    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<MODE, true> : shadow_any<MODE>(feat);
+    auto shade = (feat & WF_FOG) ? k_wf_shade<MODE, true> : k_wf_shade<MODE, false>;
     dim3 G(grid), B(block);
-    MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_gen<MULTI>, G, B, 0, st, sc, cam, job, ws, mj));
+    MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_gen<MODE>, G, B, 0, st, sc, cam, job, ws, mj));
     for (int b = 0; b <= job.depth; ++b) {
         int which = b & 1;
         MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, ws.cnt, which ^ 1));
-        MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(extend_kernel<MULTI>(feat), G, B, 0, st, sc, ws, which, mj));
-        MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, 0, st, sc, ws, job, which, b, rays, mj));
+        MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(extend_kernel<MODE>(feat), G, B, 0, st, sc, ws, which, mj));
+        MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, shade_lds(block), st, sc, ws, job, which, b, rays, mj));
         MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
-        MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MULTI>, G, B, 0, st, ws, mj));
+        MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj));
     }
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_next_batch, dim3(1), dim3(64), 0, st, ws, job.pool));
     return hipGetLastError();
@@ -82,7 +86,7 @@ hipError_t wf_enqueue_batch(const SceneView<float>& sc, const CamView<float>& ca
 {
     WfMulti none{};
     int feat = (sc.has_transmissive ? WF_TRANS : 0) | (sc.fog_kernel ? WF_FOG : 0) | (sc.bvh4 ? WF_WIDE : 0);
-    return enqueue_bounces<false>(sc, cam, job, ws, none, rays, grid, 256, feat, st, kc);
+    return enqueue_bounces<WF_SINGLE>(sc, cam, job, ws, none, rays, grid, 256, feat, st, kc);
 }
 
 hipError_t wf_multi_begin(const WfState& ws, const WfMulti& mj, hipStream_t st)
@@ -103,7 +107,8 @@ hipError_t wf_multi_batch(const WfMulti& mj, const WfState& ws, int pool, int ma
     WfJob job{};
     job.depth = max_depth;
     job.pool = pool;
-    return enqueue_bounces<true>(sc, cam, job, ws, mj, rays, grid, block, feat, st, kc);
+    if (mj.one_scene >= 0) return enqueue_bounces<WF_ONE>(sc, cam, job, ws, mj, rays, grid, block, feat, st, kc);
+    return enqueue_bounces<WF_TABLE>(sc, cam, job, ws, mj, rays, grid, block, feat, st, kc);
 }
 
 hipError_t wf_stream_init(const WfState& ws, int pool, hipStream_t st)
@@ -114,22 +119,30 @@ hipError_t wf_stream_init(const WfState& ws, int pool, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t wf_stream_step(const WfMulti& mj, const WfState& ws, int which, unsigned long long* rays, int grid,
-                          int block, int feat, hipStream_t st, KernelClock* kc)
+template <int MODE>
+static hipError_t stream_step(const WfMulti& mj, const WfState& ws, int which, unsigned long long* rays, int grid,
+                              int block, int feat, hipStream_t st, KernelClock* kc)
 {
     dim3 G(grid), B(block);
     SceneView<float> sc{};
     WfJob job{};
-    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<true, true> : shadow_any<true>(feat);
-    auto shade = (feat & WF_FOG) ? k_wf_shade<true, true> : k_wf_shade<true, false>;
-    MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_refill, G, B, 0, st, ws, mj, which));
+    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<MODE, true> : shadow_any<MODE>(feat);
+    auto shade = (feat & WF_FOG) ? k_wf_shade<MODE, true> : k_wf_shade<MODE, false>;
+    MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_refill<MODE>, G, B, 0, st, ws, mj, which));
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_refill_commit, dim3(1), dim3(64), 0, st, ws, mj));
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, ws.cnt, which ^ 1));
-    MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(extend_kernel<true>(feat), G, B, 0, st, sc, ws, which, mj));
-    MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, 0, st, sc, ws, job, which, 0, rays, mj));
+    MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(extend_kernel<MODE>(feat), G, B, 0, st, sc, ws, which, mj));
+    MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, shade_lds(block), st, sc, ws, job, which, 0, rays, mj));
     MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
-    MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<true>, G, B, 0, st, ws, mj));
+    MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj));
     return hipGetLastError();
+}
+
+hipError_t wf_stream_step(const WfMulti& mj, const WfState& ws, int which, unsigned long long* rays, int grid,
+                          int block, int feat, hipStream_t st, KernelClock* kc)
+{
+    if (mj.one_scene >= 0) return stream_step<WF_ONE>(mj, ws, which, rays, grid, block, feat, st, kc);
+    return stream_step<WF_TABLE>(mj, ws, which, rays, grid, block, feat, st, kc);
 }
 
 hipError_t batch_mega(const WfMulti& mj, const BUnit* units, int n_units, bool fog, unsigned long long* rays,
