@@ -389,8 +389,9 @@ inline int build_host_layout(const mh_scene_soa* soa, HostLayout* out)
     Bvh4Builder b4{soa, {}, 0};
     out->root4 = out->root_ref;
     if (soa->n_nodes > 0 && soa->nodes[0].count == 0) out->root4 = b4.make(0, 1);
-    // 4-wide traversal pushes up to 3 entries per level
-    bool use4 = soa->n_nodes > 0 && 3 * b4.max_depth + 1 <= MH_STACK;
+    // 4-wide traversal pushes up to 3 entries per level; the shadow walk
+    // addresses 4-wide nodes with 32-bit byte offsets (below 2^25 nodes)
+    bool use4 = soa->n_nodes > 0 && 3 * b4.max_depth + 1 <= MH_STACK && b4.nodes.size() < ((size_t)1 << 25);
     if (!use4) {
         b4.nodes.clear();
         out->root4 = out->root_ref;
@@ -445,7 +446,10 @@ inline void fill_view(SceneView<R>& v, const Layout& lay, const mh_scene_soa* so
     v.nodes = (const DNode*)lay.nodes;
     v.n_nodes = lay.n_nodes;
     v.root_ref = lay.root_ref;
-    v.nodes4 = (const DNode4*)lay.nodes4;
+    // no 4-wide tree in the view unless it is usable: walks that pick the
+    // tree by its pointer (the shadow kernels) would pair it with root4 =
+    // the binary root
+    v.nodes4 = lay.bvh4 ? (const DNode4*)lay.nodes4 : nullptr;
     v.root4 = lay.root4;
     v.bvh4 = lay.bvh4 ? 1 : 0;
     v.prims = (const typename PrimT<R>::type*)(f ? lay.p32 : lay.p64);

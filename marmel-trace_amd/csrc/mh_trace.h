@@ -643,10 +643,28 @@ template <typename R> struct Slab;
 
 template <> struct Slab<float> {
     float ix, iy, iz, ox, oy, oz;
+    // A direction component of magnitude below 1e-30 (zero included) counts
+    // as 1e-30: its inverse stays finite, so plane * i - o * i keeps the
+    // sign of plane - o (with i = inf, lo * i - o * i is inf - inf or -inf
+    // and would drop a box whose slab holds the origin).
+    __device__ __forceinline__ static float inv(float v) { return 1.0f / (fabsf(v) < 1e-30f ? 1e-30f : v); }
     __device__ __forceinline__ void init(V3<float> o, V3<float> d)
     {
-        ix = 1.0f / d.x; iy = 1.0f / d.y; iz = 1.0f / d.z;
+        ix = inv(d.x); iy = inv(d.y); iz = inv(d.z);
         ox = o.x * ix; oy = o.y * iy; oz = o.z * iz;
+    }
+    // Offset (0 or 16 bytes) of the near row of an axis in a lo / hi pair of
+    // float4 rows: the hi row when the ray runs towards -axis.
+    __device__ __forceinline__ static uint32_t near_off(float i) { return i < 0.f ? 16u : 0u; }
+    // One box from its near and far planes per axis (the lo / hi planes
+    // ordered by the direction's sign): the same decision and entry distance
+    // as one(), whose min / max of each axis pair picks the same values.
+    __device__ __forceinline__ bool nf(float nx, float fx, float ny, float fy, float nz, float fz, float tmin,
+                                       float tmax) const
+    {
+        float en = fmaxf(fmaxf(fmaf(nx, ix, -ox), fmaf(ny, iy, -oy)), fmaxf(fmaf(nz, iz, -oz), tmin));
+        float ex = fminf(fminf(fmaf(fx, ix, -ox), fmaf(fy, iy, -oy)), fminf(fmaf(fz, iz, -oz), tmax));
+        return en <= ex * 1.0000004f;
     }
     __device__ __forceinline__ void test(const DNode& n, float tmin, float tmax, bool* hl, bool* hr, float* tl,
                                          float* tr) const

@@ -209,6 +209,39 @@ __device__ __forceinline__ const SceneView<float>& slot_scene(const SceneView<fl
                             : MODE == WF_ONE ? mj.scenes[mj.one_scene] : sc;
 }
 
+// The fields a traversal kernel reads from the launch's one scene (WF_SINGLE,
+// WF_ONE), held in scalar registers: read once at kernel start, so the walk
+// neither re-reads the scene table (which the compiler cannot prove
+// unwritten) nor keeps the bases in vector registers.
+struct UScene {
+    const DNode* nodes;
+    const DNode4* nodes4;
+    const P32* prims;
+    const P32* planes;
+    int n_planes, root_ref, root4;
+};
+
+__device__ __forceinline__ const void* uniform_ptr(const void* p)
+{
+    const uint64_t v = (uint64_t)(uintptr_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return (const void*)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ UScene uniform_scene(const SceneView<float>& s)
+{
+    UScene u;
+    u.nodes = (const DNode*)uniform_ptr(s.nodes);
+    u.nodes4 = (const DNode4*)uniform_ptr(s.nodes4);
+    u.prims = (const P32*)uniform_ptr(s.prims);
+    u.planes = (const P32*)uniform_ptr(s.planes);
+    u.n_planes = __builtin_amdgcn_readfirstlane(s.n_planes);
+    u.root_ref = __builtin_amdgcn_readfirstlane(s.root_ref);
+    u.root4 = __builtin_amdgcn_readfirstlane(s.root4);
+    return u;
+}
+
 // The scene of table index `i` (a job's scene) under launch mode MODE.
 template <int MODE>
 __device__ __forceinline__ const SceneView<float>& mode_scene(const SceneView<float>& sc, const WfMulti& mj, int i)
@@ -914,9 +947,10 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 // queue with prefetched entries (a refill round is about one memory round
 // trip): 16 -> 209, 32 -> 194, 48 -> 185.6, 56 -> 186; rays prefetched into
 // LDS by direct loads (a round reads LDS only): 16 -> 167.2, 24 -> 160.1,
-// 32 -> 158.2, 48 -> 158.3, 56 -> 170.4.
+// 32 -> 158.2, 48 -> 158.3, 56 -> 170.4; with the near / far node test and
+// the linear stack: 24 -> 144.2, 32 -> 146.1, 48 -> 149.3.
 #ifndef MH_SHADOW_REFILL
-#define MH_SHADOW_REFILL 32
+#define MH_SHADOW_REFILL 24
 #endif
 // Tree the any-hit shadow kernel walks: 0 = the binary tree, 1 = the 4-wide
 // tree where the scene has one (children in index order), 2 = the 4-wide
@@ -930,6 +964,13 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 // of the wave waits for one load per lane; 0: the whole leaf in one step
 // (a load per primitive, one after another). 1080p terrain frame, shadow ms
 // per frame: 0 -> 185.6, 1 -> 163.6 (both at 80 VGPRs, 6 waves per SIMD).
+// 1: the any-hit kernel's LDS stack is linear (K slots at a 256-lane
+// stride, the older half moved to the private overflow when full); 0: a
+// ring of K slots whose oldest entry moves out when full. Shadow ms per
+// 1080p frame: 1 -> 146.06, 0 -> 146.05.
+#ifndef MH_SHADOW_LINEAR
+#define MH_SHADOW_LINEAR 1
+#endif
 #ifndef MH_SHADOW_PRIM_STEP
 #define MH_SHADOW_PRIM_STEP 1
 #endif
@@ -953,6 +994,10 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
                                                            WfMulti mj)
 {
     constexpr bool TABLE = MODE == WF_TABLE;
+    // one scene for the whole launch (WF_SINGLE, WF_ONE): its tree and
+    // primitive arrays are read at launch-uniform bases, and 4-wide nodes
+    // load only each child's near and far planes (Slab::nf)
+    constexpr bool UNI = !TABLE;
     // per-lane stack of node refs: the top K in LDS (lane-interleaved, slot
     // k of a lane at my[k * blockDim.x]), older ones in a private overflow
     // array; held in scalars so that restarting it for a new ray is three
@@ -964,6 +1009,48 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     __shared__ float4 ring[4][2][2 * CH];    // per wave: 2 chunk slots of [origin + tmax][direction + tag]
     __shared__ uint32_t rscn[4][2][CH];      // WF_TABLE: the rays' scene indices
     int* const my = lds_ref + threadIdx.x;
+#if MH_SHADOW_LINEAR
+    // Linear stack: LDS slot k of this lane at my[k * 256] (a constant
+    // stride: one address op per push or pop), sp entries in LDS above
+    // psp older ones in the private overflow array; when the LDS part is
+    // full its older half moves to the overflow (rare: the walk rarely
+    // holds more than a dozen entries).
+    constexpr int NO = MH_STACK;
+    int ovf[NO];
+    int sp = 0, psp = 0;
+    auto spill = [&]() {
+        constexpr int H = K / 2;
+        for (int j = 0; j < H; ++j) ovf[psp + j < NO ? psp + j : NO - 1] = my[j * 256];
+        for (int j = H; j < sp; ++j) my[(j - H) * 256] = my[j * 256];
+        psp += H;
+        sp -= H;
+    };
+    auto push = [&](int r) {
+        if (sp == K) spill();
+        my[sp * 256] = r;
+        ++sp;
+    };
+    // room for the four pushes of a 4-wide node
+    auto room4 = [&]() {
+        if (sp > K - 4) spill();
+    };
+    // each child's ref goes to the next free slot, which it keeps only if hit
+    auto put = [&](bool h, int r) {
+        my[sp * 256] = r;
+        sp += (int)h;
+    };
+    auto pop = [&]() -> int {
+        // the LDS read is unconditional (a slot of this lane either way)
+        int v = lds_read(&my[((sp - 1) & (K - 1)) * 256]);
+        if (sp > 0) {
+            --sp;
+        } else {
+            --psp;
+            v = ovf[psp < NO - 1 ? psp : NO - 1];
+        }
+        return v;
+    };
+#else
     const int bs = (int)blockDim.x;
     // capacity MH_STACK: K - 1 ring slots (push4 keeps one free) + the overflow
     constexpr int NO = MH_STACK - K + 1;
@@ -1003,6 +1090,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
         }
         return v;
     };
+#endif
 
     const uint32_t n = st.cnt[2];
     // the wave index through readfirstlane, so that everything derived from
@@ -1044,10 +1132,14 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     V3<float> o = zero3<float>(), d = zero3<float>();
     float tmax = 0.f;
     uint32_t tag = 0;  // slot << 4 | estimate of the lane's ray
-    // the lane's tree: the 4-wide one when MH_SHADOW_W4 and the scene has
-    // one (flagged in bit 0: node arrays are 256-byte aligned), else the binary one
+    // the lane's tree (WF_TABLE): the 4-wide one when MH_SHADOW_W4 and the
+    // scene has one (flagged in bit 0: node arrays are 256-byte aligned),
+    // else the binary one
     uintptr_t tree = 0;
     const P32* prims = nullptr;
+    // UNI: the launch's scene, and the lane's near-row offsets (Slab::near_off)
+    const UScene scu = uniform_scene(mode_scene<MODE>(sc0, mj, 0));
+    uint32_t nrx = 0, nry = 0, nrz = 0;
     // MH_SHADOW_PRIM_STEP: the current leaf's untested primitives, the next
     // one's index in bits 0..27 (tree primitives number at most 2^27) and
     // how many remain in bits 28..31
@@ -1090,21 +1182,36 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
         if (seq_more && (jn < 2u || rd >= (jn - 1u) * CH)) load_chunk();
         if (got) {
             const SceneView<float>& sc = mode_scene<MODE>(sc0, mj, (int)scn);
+            const int n_planes = UNI ? scu.n_planes : sc.n_planes;
+            const P32* planes = UNI ? scu.planes : sc.planes;
+            const int root_ref = UNI ? scu.root_ref : sc.root_ref;
             d = unit_dir(d);
             ++nq;
             // planes first (never in the tree), as any_hit does
             bool occ = false;
-            for (int pl = 0; pl < sc.n_planes && !occ; ++pl) {
-                occ = occ_prim(load_prim(&sc.planes[pl]), o, d, 1e-3f, tmax);
+            for (int pl = 0; pl < n_planes && !occ; ++pl) {
+                occ = occ_prim(load_prim(&planes[pl]), o, d, 1e-3f, tmax);
                 if (COUNT) ++cnt_o;
             }
             if (COUNT) ++cnt_q;
-            if (!occ && sc.root_ref != INT_MIN) {
-                prims = sc.prims;
+            if (!occ && root_ref != INT_MIN) {
                 sl.init(o, d);
+#if MH_SHADOW_LINEAR
+                sp = psp = 0;
+#else
                 sp = base = psp = 0;
+#endif
                 leaf = 0;
-                if (MH_SHADOW_W4 && sc.nodes4) {
+                if (UNI) {
+                    nrx = Slab<float>::near_off(sl.ix);
+                    nry = Slab<float>::near_off(sl.iy);
+                    nrz = Slab<float>::near_off(sl.iz);
+                } else {
+                    prims = sc.prims;
+                }
+                if (UNI) {
+                    push(MH_SHADOW_W4 && scu.nodes4 ? scu.root4 : scu.root_ref);
+                } else if (MH_SHADOW_W4 && sc.nodes4) {
                     tree = (uintptr_t)sc.nodes4 | 1u;
                     push(sc.root4);
                 } else {
@@ -1128,10 +1235,29 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
                     leaf += 1u - (1u << 28);
                 } else {
                 int ref = pop();
-                if (MH_SHADOW_W4 && ref >= 0 && (tree & 1u)) {
+                const bool wide = UNI ? (MH_SHADOW_W4 && scu.nodes4 != nullptr) : (tree & 1u) != 0u;
+                if (MH_SHADOW_W4 && ref >= 0 && wide) {
                     if (COUNT) ++cnt_n;
-                    if (MH_SHADOW_W4 == 2) {
-                        DNode4 nd = ldg(&((const DNode4*)(tree & ~(uintptr_t)1))[ref]);
+                    if (UNI && MH_SHADOW_W4 == 1) {
+                        // each child's near and far planes: rows picked by
+                        // address (node arrays stay below 4 GiB: 32-bit offsets)
+                        const char* nb = (const char*)scu.nodes4;
+                        const uint32_t off = (uint32_t)ref << 7;
+                        const uint32_t vx = off | nrx, vy = off | nry, vz = off | nrz;
+                        const float4 nx4 = ldg((const float4*)(nb + (size_t)vx));
+                        const float4 fx4 = ldg((const float4*)(nb + (size_t)(vx ^ 16u)));
+                        const float4 ny4 = ldg((const float4*)(nb + (size_t)vy + 32));
+                        const float4 fy4 = ldg((const float4*)(nb + (size_t)(vy ^ 16u) + 32));
+                        const float4 nz4 = ldg((const float4*)(nb + (size_t)vz + 64));
+                        const float4 fz4 = ldg((const float4*)(nb + (size_t)(vz ^ 16u) + 64));
+                        const int4 rf = ldg((const int4*)(nb + (size_t)off + 96));
+                        room4();
+                        put(sl.nf(nx4.w, fx4.w, ny4.w, fy4.w, nz4.w, fz4.w, 1e-3f, tmax) && rf.w != INT_MIN, rf.w);
+                        put(sl.nf(nx4.z, fx4.z, ny4.z, fy4.z, nz4.z, fz4.z, 1e-3f, tmax) && rf.z != INT_MIN, rf.z);
+                        put(sl.nf(nx4.y, fx4.y, ny4.y, fy4.y, nz4.y, fz4.y, 1e-3f, tmax), rf.y);
+                        put(sl.nf(nx4.x, fx4.x, ny4.x, fy4.x, nz4.x, fz4.x, 1e-3f, tmax), rf.x);
+                    } else if (MH_SHADOW_W4 == 2) {
+                        DNode4 nd = ldg(&(UNI ? scu.nodes4 : (const DNode4*)(tree & ~(uintptr_t)1))[ref]);
                         int r[4], nh;
                         float t[4];
                         children4(sl, nd, 1e-3f, tmax, r, t, &nh);
@@ -1139,7 +1265,8 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
                             if (i < nh) push(r[i]);
                     } else {
                         // boxes and refs only: a slot past the child count has ref INT_MIN
-                        DNode4Box nd = ldg((const DNode4Box*)&((const DNode4*)(tree & ~(uintptr_t)1))[ref]);
+                        DNode4Box nd = ldg(
+                            (const DNode4Box*)&(UNI ? scu.nodes4 : (const DNode4*)(tree & ~(uintptr_t)1))[ref]);
                         bool h;
                         float t;
                         room4();
@@ -1153,7 +1280,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
                         put(h, nd.ref.x);
                     }
                 } else if (ref >= 0) {
-                    DNode nd = ldg(&((const DNode*)tree)[ref]);
+                    DNode nd = ldg(&(UNI ? scu.nodes : (const DNode*)tree)[ref]);
                     if (COUNT) ++cnt_n;
                     bool hl, hr;
                     float tl, tr;
@@ -1175,7 +1302,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
                         leaf = (uint32_t)(first + 1) | ((uint32_t)(cnt - 1) << 28);
                     } else {
                         for (int j = 0; j < cnt && !occ; ++j) {
-                            PD<float> pr = load_prim(&prims[first + j]);
+                            PD<float> pr = load_prim(&(UNI ? scu.prims : prims)[first + j]);
                             occ = occ_prim(pr, o, d, 1e-3f, tmax);
                             if (COUNT) {
                                 if (pr.kind == K_TRIANGLE) ++cnt_t;
@@ -1186,7 +1313,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
                 }
                 }
                 if (MH_SHADOW_PRIM_STEP && pj >= 0) {
-                    PD<float> pr = load_prim(&prims[pj]);
+                    PD<float> pr = load_prim(&(UNI ? scu.prims : prims)[pj]);
                     occ = occ_prim(pr, o, d, 1e-3f, tmax);
                     if (COUNT) {
                         if (pr.kind == K_TRIANGLE) ++cnt_t;
