@@ -660,10 +660,11 @@ template <> struct Slab<float> {
     // ordered by the direction's sign): the same decision and entry distance
     // as one(), whose min / max of each axis pair picks the same values.
     __device__ __forceinline__ bool nf(float nx, float fx, float ny, float fy, float nz, float fz, float tmin,
-                                       float tmax) const
+                                       float tmax, float* te = nullptr) const
     {
         float en = fmaxf(fmaxf(fmaf(nx, ix, -ox), fmaf(ny, iy, -oy)), fmaxf(fmaf(nz, iz, -oz), tmin));
         float ex = fminf(fminf(fmaf(fx, ix, -ox), fmaf(fy, iy, -oy)), fminf(fmaf(fz, iz, -oz), tmax));
+        if (te) *te = en;
         return en <= ex * 1.0000004f;
     }
     __device__ __forceinline__ void test(const DNode& n, float tmin, float tmax, bool* hl, bool* hr, float* tl,

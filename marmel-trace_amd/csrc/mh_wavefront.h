@@ -218,7 +218,8 @@ struct UScene {
     const DNode4* nodes4;
     const P32* prims;
     const P32* planes;
-    int n_planes, root_ref, root4;
+    const int* plane_index;
+    int n_planes, root_ref, root4, bvh4;
 };
 
 __device__ __forceinline__ const void* uniform_ptr(const void* p)
@@ -236,10 +237,47 @@ __device__ __forceinline__ UScene uniform_scene(const SceneView<float>& s)
     u.nodes4 = (const DNode4*)uniform_ptr(s.nodes4);
     u.prims = (const P32*)uniform_ptr(s.prims);
     u.planes = (const P32*)uniform_ptr(s.planes);
+    u.plane_index = (const int*)uniform_ptr(s.plane_index);
     u.n_planes = __builtin_amdgcn_readfirstlane(s.n_planes);
     u.root_ref = __builtin_amdgcn_readfirstlane(s.root_ref);
     u.root4 = __builtin_amdgcn_readfirstlane(s.root4);
+    u.bvh4 = __builtin_amdgcn_readfirstlane(s.bvh4);
     return u;
+}
+
+// The four children of 4-wide node `ref` of a launch-uniform node array,
+// tested from their near and far planes (Slab::nf; near-row offsets nrx,
+// nry, nrz from Slab::near_off): hit flags and entry distances, and the
+// child refs. Slots past the child count have ref INT_MIN and never hit.
+struct Kids4 {
+    bool h[4];
+    float t[4];
+    int r[4];
+};
+__device__ __forceinline__ Kids4 kids4_nf(const Slab<float>& sl, const DNode4* nodes4, int ref, uint32_t nrx,
+                                          uint32_t nry, uint32_t nrz, float tmin, float tmax)
+{
+    const char* nb = (const char*)nodes4;
+    // node arrays stay below 4 GiB: 32-bit byte offsets
+    const uint32_t off = (uint32_t)ref << 7;
+    const uint32_t vx = off | nrx, vy = off | nry, vz = off | nrz;
+    const float4 nx4 = ldg((const float4*)(nb + (size_t)vx));
+    const float4 fx4 = ldg((const float4*)(nb + (size_t)(vx ^ 16u)));
+    const float4 ny4 = ldg((const float4*)(nb + (size_t)vy + 32));
+    const float4 fy4 = ldg((const float4*)(nb + (size_t)(vy ^ 16u) + 32));
+    const float4 nz4 = ldg((const float4*)(nb + (size_t)vz + 64));
+    const float4 fz4 = ldg((const float4*)(nb + (size_t)(vz ^ 16u) + 64));
+    const int4 rf = ldg((const int4*)(nb + (size_t)off + 96));
+    Kids4 k;
+    k.r[0] = rf.x;
+    k.r[1] = rf.y;
+    k.r[2] = rf.z;
+    k.r[3] = rf.w;
+    k.h[0] = sl.nf(nx4.x, fx4.x, ny4.x, fy4.x, nz4.x, fz4.x, tmin, tmax, &k.t[0]);
+    k.h[1] = sl.nf(nx4.y, fx4.y, ny4.y, fy4.y, nz4.y, fz4.y, tmin, tmax, &k.t[1]);
+    k.h[2] = sl.nf(nx4.z, fx4.z, ny4.z, fy4.z, nz4.z, fz4.z, tmin, tmax, &k.t[2]) && rf.z != INT_MIN;
+    k.h[3] = sl.nf(nx4.w, fx4.w, ny4.w, fy4.w, nz4.w, fz4.w, tmin, tmax, &k.t[3]) && rf.w != INT_MIN;
+    return k;
 }
 
 // The scene of table index `i` (a job's scene) under launch mode MODE.
@@ -406,6 +444,11 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState
 // 1: a leaf's primitives are tested one per step, like nodes (one load per
 // lane per step); 0: the whole leaf in one step. Extend ms per frame:
 // 0 -> 57.1, 1 -> 55.5.
+// 1: the extend kernel's LDS stack is linear (K slots at a 256-lane stride,
+// the older half moved to the private overflow when full); 0: a ring.
+#ifndef MH_EXTEND_LINEAR
+#define MH_EXTEND_LINEAR 1
+#endif
 #ifndef MH_EXTEND_PRIM_STEP
 #define MH_EXTEND_PRIM_STEP 1
 #endif
@@ -428,6 +471,52 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
     __shared__ float lds_t[K * 256];
     int* const myr = lds_ref + threadIdx.x;
     float* const myt = lds_t + threadIdx.x;
+    constexpr bool UNI = MODE != WF_TABLE;
+#if MH_EXTEND_LINEAR
+    // Linear stack of (ref, entry distance): LDS slot k at a 256-lane
+    // stride, sp entries in LDS above psp older ones in the private
+    // overflow; a full LDS part moves its older half to the overflow.
+    constexpr int NO = MH_STACK;
+    int ovr[NO];
+    float ovt[NO];
+    int sp = 0, psp = 0;
+    auto spill = [&]() {
+        constexpr int H = K / 2;
+        for (int j = 0; j < H; ++j) {
+            const int o = psp + j < NO ? psp + j : NO - 1;
+            ovr[o] = myr[j * 256];
+            ovt[o] = myt[j * 256];
+        }
+        for (int j = H; j < sp; ++j) {
+            myr[(j - H) * 256] = myr[j * 256];
+            myt[(j - H) * 256] = myt[j * 256];
+        }
+        psp += H;
+        sp -= H;
+    };
+    auto push = [&](int r, float t) {
+        if (sp == K) spill();
+        myr[sp * 256] = r;
+        myt[sp * 256] = t;
+        ++sp;
+    };
+    auto pop = [&](float* t) -> int {
+        // unconditional LDS reads (a slot of this lane either way)
+        const int kk = ((sp - 1) & (K - 1)) * 256;
+        float tv = lds_read(&myt[kk]);
+        int v = lds_read(&myr[kk]);
+        if (sp > 0) {
+            --sp;
+        } else {
+            --psp;
+            const int j = psp < NO - 1 ? psp : NO - 1;
+            tv = ovt[j];
+            v = ovr[j];
+        }
+        *t = tv;
+        return v;
+    };
+#else
     const int bs = (int)blockDim.x;
     int ovr[MH_STACK - K];
     float ovt[MH_STACK - K];
@@ -462,11 +551,13 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
         *t = tv;
         return v;
     };
+#endif
 
     const uint32_t n = st.cnt[which];
     const uint32_t* q = st.q[which];
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t wg = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // wave-uniform through readfirstlane (queue positions stay scalar)
+    const uint32_t wg = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const int lane = (int)__lane_id();
     // queue index of the wave's p-th entry: chunks wg, wg + nwaves, ...
     auto entry = [&](uint32_t p) -> uint32_t { return (wg + (p >> 6) * nwaves) * 64u + (p & 63u); };
@@ -478,9 +569,13 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
     float closest = 0.f;
     int best = INT_MIN, bprim = 0x7fffffff, ref = 0;
     uint32_t slot = 0;
+    // the lane's scene arrays (WF_TABLE); UNI: the launch's scene in scalar
+    // registers and the lane's near-row offsets (Slab::near_off)
     const DNode* nodes = nullptr;
     const DNode4* nodes4 = nullptr;  // the 4-wide tree when the lane's scene view selects it
     const P32* prims = nullptr;
+    const UScene scu = uniform_scene(mode_scene<MODE>(sc0, mj, 0));
+    uint32_t nrx = 0, nry = 0, nrz = 0;
     // MH_EXTEND_PRIM_STEP: the current leaf's untested primitives, the next
     // one's index in bits 0..27 (tree primitives number at most 2^27) and
     // how many remain in bits 28..31
@@ -497,6 +592,9 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
             if (!busy && i < n) {
                 slot = q[i];
                 const SceneView<float>& sc = slot_scene<MODE>(sc0, mj, st, slot);
+                const int n_planes = UNI ? scu.n_planes : sc.n_planes;
+                const P32* planes = UNI ? scu.planes : sc.planes;
+                const int* plane_index = UNI ? scu.plane_index : sc.plane_index;
                 float4 a = st.p0[slot], b = st.p1[slot];
                 o = mk(a.x, a.y, a.z);
                 d = mk(b.x, b.y, b.z);
@@ -505,13 +603,13 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                 bprim = 0x7fffffff;
                 if (COUNT) {
                     ++cnt_q;
-                    cnt_o += (uint32_t)sc.n_planes;
+                    cnt_o += (uint32_t)n_planes;
                 }
-                for (int k = 0; k < sc.n_planes; ++k) {
-                    PD<float> p = load_prim(&sc.planes[k]);
+                for (int k = 0; k < n_planes; ++k) {
+                    PD<float> p = load_prim(&planes[k]);
                     float t;
                     if (hit_plane<float, false>(p, o, d, tmin, closest, &t, nullptr)) {
-                        int pi = sc.plane_index[k];
+                        int pi = plane_index[k];
                         if (best == INT_MIN || t < closest || (t == closest && pi < bprim)) {
                             closest = t;
                             bprim = pi;
@@ -519,14 +617,25 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                         }
                     }
                 }
-                if (sc.root_ref != INT_MIN) {
-                    nodes = sc.nodes;
-                    nodes4 = sc.bvh4 ? sc.nodes4 : nullptr;
-                    prims = sc.prims;
+                if ((UNI ? scu.root_ref : sc.root_ref) != INT_MIN) {
                     sl.init(o, d);
+#if MH_EXTEND_LINEAR
+                    sp = psp = 0;
+#else
                     sp = base = psp = 0;
+#endif
                     leaf = 0;
-                    ref = nodes4 ? sc.root4 : sc.root_ref;
+                    if (UNI) {
+                        nrx = Slab<float>::near_off(sl.ix);
+                        nry = Slab<float>::near_off(sl.iy);
+                        nrz = Slab<float>::near_off(sl.iz);
+                        ref = scu.bvh4 ? scu.root4 : scu.root_ref;
+                    } else {
+                        nodes = sc.nodes;
+                        nodes4 = sc.bvh4 ? sc.nodes4 : nullptr;
+                        prims = sc.prims;
+                        ref = nodes4 ? sc.root4 : sc.root_ref;
+                    }
                     held = true;
                     busy = true;
                 } else {
@@ -551,8 +660,36 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                 }
                 held = false;
                 if (work) {
-                    if (ref >= 0 && nodes4) {
-                        DNode4 nd = ldg(&nodes4[ref]);
+                    if (UNI && MH_EXTEND_W4_SORT && ref >= 0 && scu.bvh4) {
+                        // children from their near and far planes, nearest
+                        // first (the sorting network of children4)
+                        if (COUNT) ++cnt_n;
+                        const Kids4 kd = kids4_nf(sl, scu.nodes4, ref, nrx, nry, nrz, tmin, closest);
+                        const float inf = __builtin_inff();
+                        float kk[4] = {kd.h[0] ? kd.t[0] : inf, kd.h[1] ? kd.t[1] : inf, kd.h[2] ? kd.t[2] : inf,
+                                       kd.h[3] ? kd.t[3] : inf};
+                        int qq[4] = {kd.r[0], kd.r[1], kd.r[2], kd.r[3]};
+#define MH_KSWAP(a_, b_)                                                                                  \
+    if (kk[b_] < kk[a_]) {                                                                                \
+        float tk_ = kk[a_];                                                                               \
+        kk[a_] = kk[b_];                                                                                  \
+        kk[b_] = tk_;                                                                                     \
+        int tq_ = qq[a_];                                                                                 \
+        qq[a_] = qq[b_];                                                                                  \
+        qq[b_] = tq_;                                                                                     \
+    }
+                        MH_KSWAP(0, 1)
+                        MH_KSWAP(2, 3)
+                        MH_KSWAP(0, 2)
+                        MH_KSWAP(1, 3)
+                        MH_KSWAP(1, 2)
+#undef MH_KSWAP
+                        for (int i = 3; i >= 1; --i)
+                            if (kk[i] < inf) push(qq[i], kk[i]);
+                        ref = qq[0];
+                        held = kk[0] < inf;
+                    } else if (ref >= 0 && (UNI ? scu.bvh4 != 0 : nodes4 != nullptr)) {
+                        DNode4 nd = ldg(&(UNI ? scu.nodes4 : nodes4)[ref]);
                         if (COUNT) ++cnt_n;
                         if (MH_EXTEND_W4_SORT) {
                             int r[4], nh;
@@ -587,7 +724,7 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                             held = k >= 0;
                         }
                     } else if (ref >= 0) {
-                        DNode nd = ldg(&nodes[ref]);
+                        DNode nd = ldg(&(UNI ? scu.nodes : nodes)[ref]);
                         if (COUNT) ++cnt_n;
                         bool hl, hr;
                         float tl, tr;
@@ -604,7 +741,7 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                             leaf = (uint32_t)(first + 1) | ((uint32_t)(cnt - 1) << 28);
                         } else {
                             for (int j = 0; j < cnt; ++j) {
-                                PD<float> p = load_prim(&prims[first + j]);
+                                PD<float> p = load_prim(&(UNI ? scu.prims : prims)[first + j]);
                                 if (COUNT) {
                                     if (p.kind == K_TRIANGLE) ++cnt_t;
                                     else ++cnt_o;
@@ -623,7 +760,7 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                 }
                 }
                 if (MH_EXTEND_PRIM_STEP && pj >= 0) {
-                    PD<float> p = load_prim(&prims[pj]);
+                    PD<float> p = load_prim(&(UNI ? scu.prims : prims)[pj]);
                     if (COUNT) {
                         if (p.kind == K_TRIANGLE) ++cnt_t;
                         else ++cnt_o;
