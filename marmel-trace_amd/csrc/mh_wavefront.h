@@ -1101,6 +1101,11 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 // of the wave waits for one load per lane; 0: the whole leaf in one step
 // (a load per primitive, one after another). 1080p terrain frame, shadow ms
 // per frame: 0 -> 185.6, 1 -> 163.6 (both at 80 VGPRs, 6 waves per SIMD).
+// LDS entries per lane of the dynamic-fetch any-hit kernel's stack (the
+// linear stack takes any depth from 8).
+#ifndef MH_SHADOW_REFILL_LDS
+#define MH_SHADOW_REFILL_LDS MH_SHADOW_ANY_LDS
+#endif
 // 1: the any-hit kernel's LDS stack is linear (K slots at a 256-lane
 // stride, the older half moved to the private overflow when full); 0: a
 // ring of K slots whose oldest entry moves out when full. Shadow ms per
@@ -1139,8 +1144,9 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     // k of a lane at my[k * blockDim.x]), older ones in a private overflow
     // array; held in scalars so that restarting it for a new ray is three
     // register writes
-    constexpr int K = MH_SHADOW_ANY_LDS;
-    static_assert(K > 0 && (K & (K - 1)) == 0 && K < MH_STACK, "LDS stack depth must be a power of two");
+    constexpr int K = MH_SHADOW_REFILL_LDS;
+    static_assert(K >= 8 && K < MH_STACK && (MH_SHADOW_LINEAR || (K & (K - 1)) == 0),
+                  "LDS stack depth: at least 8, a power of two for the ring");
     constexpr uint32_t CH = 32u;  // rays per chunk
     __shared__ int lds_ref[K * 256];
     __shared__ float4 ring[4][2][2 * CH];    // per wave: 2 chunk slots of [origin + tmax][direction + tag]
@@ -1178,7 +1184,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     };
     auto pop = [&]() -> int {
         // the LDS read is unconditional (a slot of this lane either way)
-        int v = lds_read(&my[((sp - 1) & (K - 1)) * 256]);
+        int v = lds_read(&my[(sp > 0 ? sp - 1 : 0) * 256]);
         if (sp > 0) {
             --sp;
         } else {
@@ -1504,7 +1510,11 @@ __global__ __launch_bounds__(256) void k_wf_finalize(WfState st, WfMulti mj)
             L.y = L.y < 0.f ? 0.f : L.y;
             L.z = L.z < 0.f ? 0.f : L.z;
             uint32_t pix = f2u(c.w);
-            if (MULTI) {
+            if (MULTI && !mj.stream) {
+                // fixed batches: the slot keeps the path's radiance until
+                // k_wf_batch_sum adds the batch's samples of each pixel
+                st.p3[slot] = make_float4(L.x, L.y, L.z, e.w);
+            } else if (MULTI) {
                 size_t p = ((size_t)mj.jobs[st.jid[slot]].acc_off + pix) * 3;
                 atomicAdd(&mj.acc64[p], to_fixed(L.x));
                 atomicAdd(&mj.acc64[p + 1], to_fixed(L.y));
@@ -1522,6 +1532,43 @@ __global__ __launch_bounds__(256) void k_wf_finalize(WfState st, WfMulti mj)
             uint32_t at[1];
             block_append<1>(cq, want, at);
             if (active) st.freeq[at[0]] = slot;
+        }
+    }
+}
+
+// Fixed batches of the batched engine: slot i holds global path g0 + i for
+// the whole batch, and k_wf_finalize leaves each path's final radiance in its
+// slot (every path has ended after the last bounce). One thread per pixel of
+// each job the batch covers adds the pixel's samples of the batch, each in
+// 2^-24 fixed point as k_wf_finalize's atomics would, to its sums: the same
+// integers, with no atomics (a pixel's samples in one batch go to one thread).
+__global__ __launch_bounds__(256) void k_wf_batch_sum(WfState st, WfMulti mj, int pool)
+{
+    const long long g0 = (long long)st.batch[0];
+    const long long g1 = g0 + pool < mj.total ? g0 + pool : mj.total;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (int j = 0; j < mj.n_jobs; ++j) {
+        const BJob& bj = mj.jobs[j];
+        const long long npix = bj.npix;
+        const long long jb = bj.begin, je = bj.begin + npix * bj.spp;
+        // the job's paths in this batch: job-local indices [lo, hi)
+        const long long lo = (g0 > jb ? g0 : jb) - jb, hi = (g1 < je ? g1 : je) - jb;
+        if (lo >= hi) continue;
+        for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < npix; t += stride) {
+            // samples s of tile-order pixel t with lo <= s * npix + t < hi
+            long long l = lo > t ? t + (lo - t + npix - 1) / npix * npix : t;
+            if (l >= hi) continue;
+            unsigned long long f0 = 0ull, f1 = 0ull, f2 = 0ull;
+            for (; l < hi; l += npix) {
+                const float4 e = st.p3[(size_t)(jb + l - g0)];
+                f0 += to_fixed(e.x);
+                f1 += to_fixed(e.y);
+                f2 += to_fixed(e.z);
+            }
+            const size_t p = ((size_t)bj.acc_off + tile_pixel((uint32_t)t, (uint32_t)bj.w, (uint32_t)bj.h)) * 3;
+            mj.acc64[p] += f0;
+            mj.acc64[p + 1] += f1;
+            mj.acc64[p + 2] += f2;
         }
     }
 }

@@ -77,6 +77,10 @@ static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<floa
         MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
         MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj));
     }
+    // batched engine, fixed batches: the batch's pixel sums (k_wf_finalize
+    // left each path's radiance in its slot)
+    if (MODE != WF_SINGLE)
+        MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_batch_sum, G, B, 0, st, ws, mj, job.pool));
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_next_batch, dim3(1), dim3(64), 0, st, ws, job.pool));
     return hipGetLastError();
 }
