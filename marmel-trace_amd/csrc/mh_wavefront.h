@@ -209,6 +209,31 @@ __device__ __forceinline__ const SceneView<float>& slot_scene(const SceneView<fl
                             : MODE == WF_ONE ? mj.scenes[mj.one_scene] : sc;
 }
 
+// Whether the dynamic-fetch kernels number their waves XCD-major (1):
+// workgroup i runs on XCD i mod 8, so the usual numbering (workgroup-major,
+// 0) spreads consecutive queue chunks over all eight XCDs; numbered
+// XCD-major, the waves of one XCD take consecutive chunks (the rays of
+// neighbouring pixels). Measured on the 1080p frame (ms per frame): extend
+// 39.3 -> 47.9, shadow 144.8 -> 148.6 with it, so it is off.
+#ifndef MH_XCD_ORDER
+#define MH_XCD_ORDER 0
+#endif
+
+// This wave's index among the grid's waves, wave-uniform: XCD-major when
+// MH_XCD_ORDER (a bijection onto [0, waves) for any grid size), else
+// workgroup-major.
+__device__ __forceinline__ uint32_t grid_wave()
+{
+    const uint32_t W = blockDim.x >> 6;
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (!MH_XCD_ORDER) return blockIdx.x * W + wave;
+    const uint32_t G = gridDim.x, q = G >> 3, r = G & 7u;
+    const uint32_t x = blockIdx.x & 7u, bx = blockIdx.x >> 3;
+    // workgroups on XCDs before x: q each, one more on the first r
+    const uint32_t before = x * q + (x < r ? x : r);
+    return (before + bx) * W + wave;
+}
+
 // The fields a traversal kernel reads from the launch's one scene (WF_SINGLE,
 // WF_ONE), held in scalar registers: read once at kernel start, so the walk
 // neither re-reads the scene table (which the compiler cannot prove
@@ -556,8 +581,7 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
     const uint32_t n = st.cnt[which];
     const uint32_t* q = st.q[which];
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    // wave-uniform through readfirstlane (queue positions stay scalar)
-    const uint32_t wg = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const uint32_t wg = grid_wave();
     const int lane = (int)__lane_id();
     // queue index of the wave's p-th entry: chunks wg, wg + nwaves, ...
     auto entry = [&](uint32_t p) -> uint32_t { return (wg + (p >> 6) * nwaves) * 64u + (p & 63u); };
@@ -1243,7 +1267,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
     // start is computed only while the previous one was below n, so every
     // start stays below n + 32 * waves < 2^32
     const uint32_t nw = gridDim.x * (blockDim.x >> 6);             // waves in the grid
-    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (uint32_t)wave;
+    const uint32_t gw = grid_wave();
     uint32_t jn = 0;          // next chunk of the wave's sequence to load (wave-uniform)
     uint32_t rd = 0, wr = 0;  // rays of the sequence handed out / loaded so far (wave-uniform)
     bool seq_more = gw * CH < n;  // chunks of the sequence remain unloaded (wave-uniform)
