@@ -40,19 +40,21 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "marmel-trace_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Measured HBM bytes per k_wf_shadow_refill<true> launch of the batched
-# engine (the default main engine) on this workload: rocprofv3 --pmc
-# FETCH_SIZE (6,799,617.6 KiB, x 2.0) + WRITE_SIZE (414,309.3 KiB, x 1.0) per
-# dispatch, averaged over the clocked frame's 18 dispatches (pool 2^26 paths,
-# 16384 workgroups), the factors being what the same counters read for a
-# 1 GiB copy (bench.py --hbm-calib 1024). Re-measure when the shadow kernel or
-# its launch changes.
-SHADOW_TRAFFIC_PER_LAUNCH = 14349.9e6
-# What the same kernel's counters show limits it (8.76 ms per launch):
-# VALUBusy 72.9 %, VALUUtilization 43.6 %, occupancy 70.2 % (6 of 8 waves per
-# SIMD at 80 VGPRs), L2 hit 62.1 %, HBM 1.64 TB/s.
-SHADOW_LIMITER = ("VALU issue (73 % busy) at 44 % lane utilisation: divergent traversal steps; "
-                  "HBM at 1.6 TB/s, 20 % of peak")
+# Measured HBM bytes per k_wf_shadow_refill launch of the batched engine
+# (the default main engine) on this workload: rocprofv3 --pmc FETCH_SIZE
+# (6,077,929.4 KiB, x 2.0) + WRITE_SIZE (425,185.2 KiB, x 1.0) per dispatch,
+# averaged over the clocked frame's 18 dispatches (pool 2^26 paths, 16384
+# workgroups), the factors being what the same counters read for a 1 GiB
+# copy (bench.py --hbm-calib 1024). Re-measure when the shadow kernel or its
+# launch changes.
+SHADOW_TRAFFIC_PER_LAUNCH = 12883.0e6
+# What the same kernel's counters show limits it (8.01 ms per launch): VALU
+# instructions x 4 cycles fill 63 % of the SIMDs' issue cycles at 2.35 GHz
+# (SQ_INSTS_VALU 3.03e9, SQ_BUSY_CYCLES) with 56 % of the lanes active
+# (VALUUtilization); 23 of 32 wave slots per CU occupied (SQ_WAVE_CYCLES,
+# 78 VGPRs, 24.5 KiB LDS: 6 waves per SIMD); L2 hit 69 %; HBM 1.61 TB/s.
+SHADOW_LIMITER = ("dependent node fetches and VALU issue (63 % of SIMD cycles, 56 % of lanes active: "
+                  "divergent traversal steps) at 6 of 8 waves per SIMD; L2 hit 69 %; HBM at 1.6 TB/s, 20 % of peak")
 W, H, SPP, DEPTH = 1920, 1080, 64, 8
 GRID = 1024
 
