@@ -25,8 +25,10 @@
 // Built without FMA contraction so that both passes round identically and
 // take identical decisions.
 // box hits at o + t d in fp32 here (see mh_trace.h): the adjoint kernels sit at
-// the register ceiling, where the snap halved the two-pass kernel's occupancy
+// the register ceiling, where the snap halved the two-pass kernel's occupancy;
+// for the same reason the fp32 slab setup runs without its tiny-component guard
 #define MH_SNAP_BOX 0
+#define MH_SLAB_GUARD 0
 #include "mh_kernels.h"
 #include "mh_render.h"
 

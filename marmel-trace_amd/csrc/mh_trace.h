@@ -639,15 +639,30 @@ template <int K> struct LdsStack {
 
 // Both child slabs of one node. fp32: origin-scaled planes and a widened exit
 // (robust traversal); fp64: the reference's slab arithmetic in double.
+// 1: the fp32 slab setup guards direction components of magnitude below
+// 1e-30 (see Slab<float>::inv). The path-replay adjoint's translation unit
+// sets 0: its kernel sits at 251 of 256 registers, and the guard's compare
+// and select took it to occupancy 1 (two-pass gradient 55 -> 82 ms); there
+// an exact zero needs a uniform sample of exactly 0 (2^-24 per sample) in an
+// axis-aligned frame, and the adjoint is compared with the oracle within a
+// tolerance only.
+#ifndef MH_SLAB_GUARD
+#define MH_SLAB_GUARD 1
+#endif
+
 template <typename R> struct Slab;
 
 template <> struct Slab<float> {
     float ix, iy, iz, ox, oy, oz;
-    // A direction component of magnitude below 1e-30 (zero included) counts
-    // as 1e-30: its inverse stays finite, so plane * i - o * i keeps the
-    // sign of plane - o (with i = inf, lo * i - o * i is inf - inf or -inf
-    // and would drop a box whose slab holds the origin).
-    __device__ __forceinline__ static float inv(float v) { return 1.0f / (fabsf(v) < 1e-30f ? 1e-30f : v); }
+    // MH_SLAB_GUARD: a direction component of magnitude below 1e-30 (zero
+    // included) counts as 1e-30, so its inverse stays finite and
+    // plane * i - o * i keeps the sign of plane - o (with i = inf,
+    // lo * i - o * i is inf - inf or -inf and would drop a box whose slab
+    // holds the origin).
+    __device__ __forceinline__ static float inv(float v)
+    {
+        return MH_SLAB_GUARD ? 1.0f / (fabsf(v) < 1e-30f ? 1e-30f : v) : 1.0f / v;
+    }
     __device__ __forceinline__ void init(V3<float> o, V3<float> d)
     {
         ix = inv(d.x); iy = inv(d.y); iz = inv(d.z);
