@@ -991,16 +991,21 @@ template <typename R>
 __device__ __forceinline__ void finish_hit(const SceneView<R>& sc, int rec, V3<R> o, V3<R> d, R tmin, R tmax, DHit<R>* h)
 {
     R t;
+    bool ok;
     if (rec < 0) {
         int slot = -1 - rec;
         PD<R> p = load_prim(&sc.planes[slot]);
-        hit_plane<R, true>(p, o, d, tmin, tmax, &t, h);
+        ok = hit_plane<R, true>(p, o, d, tmin, tmax, &t, h);
         h->prim = sc.plane_index[slot];
     } else {
         PD<R> p = load_prim(&sc.prims[rec]);
-        hit_prim<R, true>(p, sc.attrs ? &sc.attrs[rec] : nullptr, o, d, tmin, tmax, &t, h);
+        ok = hit_prim<R, true>(p, sc.attrs ? &sc.attrs[rec] : nullptr, o, d, tmin, tmax, &t, h);
         h->prim = p.prim;
     }
+    // the winner's full test repeats the query's, so it passes; were it ever
+    // to fail, the record holds no material (callers treat it as no surface)
+    // instead of whatever the registers held
+    if (!ok) store_hit(h, tmax, o + d * tmax, -d, d, -1, R(0), R(0));
 }
 
 template <typename R, typename TS>
