@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -1188,9 +1189,15 @@ int mh_trace_any(mh_scene* s, const mh_ray* rays, int n, double tmin, double tma
 
 // Explicit Whitted work stacks of a persistent grid of `threads` lanes
 // (at most 256 Ki, at most one per pixel): 17 pending rays per level suffice.
+// MARMEL_WHITTED_STACK_CAP (a test hook) lowers the capacity, so that the
+// kernels' own bound check (every push tests it first) reports an overflow.
 static int ensure_wstack(mh_scene* s, int w, int h, int depth, bool f64, int* cap_out, int* threads_out)
 {
     int cap = 17 * (depth + 2) + 4;
+    if (const char* e = getenv("MARMEL_WHITTED_STACK_CAP")) {
+        int c = atoi(e);
+        if (c >= 1 && c < cap) cap = c;
+    }
     int threads = 256 * 1024;
     size_t px = (size_t)w * h;
     if ((size_t)threads > px) threads = (int)((px + 255) / 256 * 256);

@@ -143,6 +143,22 @@ def test_whitted_drop_in_entry(default_scene):
     assert within(rgb, g["whitted_320x180_s4_d4"]) >= 0.999
 
 
+def test_whitted_work_stack_overflow_is_reported(default_scene, monkeypatch):
+    """Code 6 (the Whitted work stack overflowed) comes back from the drop-in
+    entry when the stack is too small for the scene's reflected and refracted
+    rays: MARMEL_WHITTED_STACK_CAP lowers the capacity, and every push checks
+    it before writing. With the capacity restored the same call renders."""
+    g, hs, gs, o = default_scene
+    cam = cam_of(g["cam_320x180"])
+    monkeypatch.setenv("MARMEL_WHITTED_STACK_CAP", "1")
+    rc, _ = gs.render_image(cam, 320, 180, 4, 4)
+    assert rc == 6
+    monkeypatch.delenv("MARMEL_WHITTED_STACK_CAP")
+    rc, rgb = gs.render_image(cam, 320, 180, 4, 4)
+    assert rc == 0
+    assert within(rgb, g["whitted_320x180_s4_d4"]) >= 0.999
+
+
 def test_pathtrace_fp64_matches_reference_image(default_scene):
     g, hs, gs, o = default_scene
     for (w, h, spp, depth) in ((320, 180, 4, 4), (160, 90, 16, 6)):
