@@ -462,7 +462,8 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState
 // On the 4-wide tree (scenes whose view selects it; the autotuner picks it
 // for the 1080p terrain frame): the nearest hit child is visited next from a
 // register; the others are stacked 0 = in index order, 1 = nearest last
-// (sorted). Extend ms per frame: binary 60.5, 0 -> 59.0, 1 -> 57.1.
+// (sorted). Extend ms per frame: binary 60.5, 0 -> 59.0, 1 -> 57.1; with
+// the near / far child tests: 0 -> 41.2, 1 -> 39.05.
 #ifndef MH_EXTEND_W4_SORT
 #define MH_EXTEND_W4_SORT 1
 #endif
