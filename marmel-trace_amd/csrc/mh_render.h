@@ -12,6 +12,13 @@
 
 namespace mh {
 
+// 1: the double-precision Whitted renderer tests its 16 sun-disk shadow rays
+// of a hit as one bundle (any_hit_bundle: one walk, the same occlusion bits);
+// 0: sixteen any_hit walks.
+#ifndef MH_WHITTED_BUNDLE
+#define MH_WHITTED_BUNDLE 1
+#endif
+
 // ---- path tracer --------------------------------------------------------------
 
 template <typename R> struct Lobe {
@@ -752,13 +759,29 @@ __device__ bool whitted_hit(const SceneView<R>& sc, TS ts, const WItem<R>& it, c
     V3<R> so = P + N * R(1e-3);
     if (sc.sky.sun_radius > R(1e-9)) {
         R lit = R(0);
-        for (int i = 0; i < 16; ++i) {
-            uint32_t k = seed * 0x9e3779b9u + (uint32_t)i * 2u;
-            R r1 = rand01<R>(k, (uint32_t)i, 0x5a17u);
-            R r2 = rand01<R>(k, (uint32_t)i, 0x7c3du);
-            V3<R> Ld = sun_disk_dir(Ls, sc.sky.sun_radius, r1, r2);
-            ++*nq;
-            if (!any_hit(sc, ts, so, unit_dir(Ld), R(1e-3), R(1e30))) lit += R(1);
+        if constexpr (sizeof(R) == 8 && MH_WHITTED_BUNDLE) {
+            // the 16 sun-disk rays leave so within the sun's angular radius of
+            // its direction: one bundle walk, the same per-ray occlusion
+            V3<R> dirs[16];
+            for (int i = 0; i < 16; ++i) {
+                uint32_t k = seed * 0x9e3779b9u + (uint32_t)i * 2u;
+                R r1 = rand01<R>(k, (uint32_t)i, 0x5a17u);
+                R r2 = rand01<R>(k, (uint32_t)i, 0x7c3du);
+                dirs[i] = unit_dir(sun_disk_dir(Ls, sc.sky.sun_radius, r1, r2));
+                ++*nq;
+            }
+            const uint32_t occ = any_hit_bundle(sc, ts, so, dirs, 16, normalize(Ls),
+                                                sc.sky.sun_radius * (Consts<R>::pi / R(180)), R(1e-3), R(1e30));
+            lit = R(16 - __builtin_popcount(occ));
+        } else {
+            for (int i = 0; i < 16; ++i) {
+                uint32_t k = seed * 0x9e3779b9u + (uint32_t)i * 2u;
+                R r1 = rand01<R>(k, (uint32_t)i, 0x5a17u);
+                R r2 = rand01<R>(k, (uint32_t)i, 0x7c3du);
+                V3<R> Ld = sun_disk_dir(Ls, sc.sky.sun_radius, r1, r2);
+                ++*nq;
+                if (!any_hit(sc, ts, so, unit_dir(Ld), R(1e-3), R(1e30))) lit += R(1);
+            }
         }
         if (lit > R(0)) color = color + shade(m, alb, N, Ls, V, sc.sky.sun_color) * (lit / R(16));
     } else {

@@ -1101,6 +1101,88 @@ __device__ bool any_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin,
     return false;
 }
 
+// Occlusion of n <= 32 unit-direction rays d[0..n) that leave o within the
+// angle `spread` (radians) of the unit axis a, over [tmin, tmax]: bit k of the
+// result is set when ray k is blocked, as n separate any_hit queries would
+// set it. One walk of the tree visits every node that the cone around a of
+// half-angle `spread` can meet, and each ray still open is tested exactly
+// against the primitives of the leaves reached (stopping once all are
+// blocked). The cone test grows each box by e = D * spread, D an upper bound
+// (L1) of the distance from o to the box: a ray k that meets the box at
+// t <= D passes within t * |d[k] - a| <= D * spread of the axis point
+// o + t a, which therefore lies in the grown box. Double precision only
+// (the growth must not be lost to rounding of the box bounds).
+template <typename TS>
+__device__ uint32_t any_hit_bundle(const SceneView<double>& sc, TS ts, V3<double> o, const V3<double>* d, int n,
+                                   V3<double> a, double spread, double tmin, double tmax)
+{
+    const uint32_t all = n >= 32 ? 0xffffffffu : (1u << n) - 1u;
+    uint32_t occ = 0;
+    for (int i = 0; i < sc.n_planes; ++i) {
+        PD<double> p = load_prim(&sc.planes[i]);
+        for (int k = 0; k < n; ++k)
+            if (!((occ >> k) & 1u) && occ_prim(p, o, d[k], tmin, tmax)) occ |= 1u << k;
+    }
+    if (occ == all || sc.root_ref == INT_MIN) return occ;
+    // growth per unit distance, with a margin for the rounding of the
+    // directions and of the bound itself
+    const double g = spread * 1.000001 + 1e-9;
+    Slab<double> sl;
+    sl.init(o, a);
+    auto meets = [&](double x0, double x1, double y0, double y1, double z0, double z1) -> bool {
+        const double D = mfabs((x0 + x1) * 0.5 - o.x) + mfabs((y0 + y1) * 0.5 - o.y) +
+                         mfabs((z0 + z1) * 0.5 - o.z) + (x1 - x0) * 0.5 + (y1 - y0) * 0.5 + (z1 - z0) * 0.5;
+        const double e = D * g;
+        bool hit;
+        float te;
+        sl.one(x0 - e, x1 + e, y0 - e, y1 + e, z0 - e, z1 + e, tmin, tmax, &hit, &te);
+        return hit;
+    };
+    auto leaf = [&](int ref) -> bool {
+        const int code = ~ref;
+        const int first = code >> 3, cnt = (code & 7) + 1;
+        for (int i = 0; i < cnt; ++i) {
+            PD<double> p = load_prim(&sc.prims[first + i]);
+            for (int k = 0; k < n; ++k)
+                if (!((occ >> k) & 1u) && occ_prim(p, o, d[k], tmin, tmax)) occ |= 1u << k;
+            if (occ == all) return true;
+        }
+        return false;
+    };
+    typename TS::template Lane<false> st(ts);
+    if (sc.bvh4) {
+        st.push(sc.root4, 0.f);
+        while (!st.empty()) {
+            float te;
+            const int ref = st.pop(&te);
+            if (ref >= 0) {
+                const DNode4 nd = ldg(&sc.nodes4[ref]);
+                const int cnt = nd.meta.x;
+                if (cnt > 3 && meets(nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w)) st.push(nd.ref.w, 0.f);
+                if (cnt > 2 && meets(nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z)) st.push(nd.ref.z, 0.f);
+                if (meets(nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y)) st.push(nd.ref.y, 0.f);
+                if (meets(nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x)) st.push(nd.ref.x, 0.f);
+            } else if (leaf(ref)) {
+                return occ;
+            }
+        }
+        return occ;
+    }
+    st.push(sc.root_ref, 0.f);
+    while (!st.empty()) {
+        float te;
+        const int ref = st.pop(&te);
+        if (ref >= 0) {
+            const DNode nd = ldg(&sc.nodes[ref]);
+            if (meets(nd.c1.z, nd.c1.w, nd.c2.x, nd.c2.y, nd.c2.z, nd.c2.w)) st.push(nd.c3.y, 0.f);
+            if (meets(nd.c0.x, nd.c0.y, nd.c0.z, nd.c0.w, nd.c1.x, nd.c1.y)) st.push(nd.c3.x, 0.f);
+        } else if (leaf(ref)) {
+            return occ;
+        }
+    }
+    return occ;
+}
+
 // Normalise a direction unless it is already unit within 1e-12 in |d|^2,
 // as the reference's bvh_intersect / primitive_intersect do on entry.
 template <typename R> __device__ __forceinline__ V3<R> unit_dir(V3<R> d)
