@@ -343,7 +343,7 @@ def stress_leg(tune, seconds, chunk=64, target=1024):
     kw = {}
     if tune:
         kw = dict(pool=tune["pool"], grid_blocks=tune["grid_blocks"], block=tune["block"],
-                  bvh_width=2, fixed_batches=bool(tune["fixed_batches"]))
+                  bvh_width=tune["bvh_width"], fixed_batches=bool(tune["fixed_batches"]))
     b = Batch()
     acc = np.zeros((h, w, 3), np.float64)
     secs = []
