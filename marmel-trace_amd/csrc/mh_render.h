@@ -12,9 +12,11 @@
 
 namespace mh {
 
-// 1: the double-precision Whitted renderer tests its 16 sun-disk shadow rays
+// 1: the double-precision Whitted renderer tests the 16 sun-disk shadow rays
 // of a hit as one bundle (any_hit_bundle: one walk, the same occlusion bits);
-// 0: sixteen any_hit walks.
+// 0: sixteen any_hit walks. The 16 rays to an emissive light stay separate
+// walks: their cone soon grows wider than small primitives, and a bundle walk
+// then reaches many more leaves, each tested against all 16 rays.
 #ifndef MH_WHITTED_BUNDLE
 #define MH_WHITTED_BUNDLE 1
 #endif

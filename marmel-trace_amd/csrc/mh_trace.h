@@ -1101,17 +1101,25 @@ __device__ bool any_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin,
     return false;
 }
 
+// 1: the bundle walk below takes children nearest first along the axis
+// (forest scene, fp64 Whitted 1280x720x4 depth 5: 52.8 ms vs 53.9 in index
+// order, the same bytes)
+#ifndef MH_BUNDLE_SORT
+#define MH_BUNDLE_SORT 1
+#endif
+
 // Occlusion of n <= 32 unit-direction rays d[0..n) that leave o within the
 // angle `spread` (radians) of the unit axis a, over [tmin, tmax]: bit k of the
 // result is set when ray k is blocked, as n separate any_hit queries would
-// set it. One walk of the tree visits every node that the cone around a of
-// half-angle `spread` can meet, and each ray still open is tested exactly
-// against the primitives of the leaves reached (stopping once all are
-// blocked). The cone test grows each box by e = D * spread, D an upper bound
-// (L1) of the distance from o to the box: a ray k that meets the box at
-// t <= D passes within t * |d[k] - a| <= D * spread of the axis point
-// o + t a, which therefore lies in the grown box. Double precision only
-// (the growth must not be lost to rounding of the box bounds).
+// set it. One walk of the tree, nearest child first along the axis, visits
+// every node that the cone around a of half-angle `spread` can meet, and
+// each ray still open is tested exactly against the primitives of the leaves
+// reached (stopping once all are blocked). The cone test grows each box by
+// e = D * spread, D an upper bound (L1) of the distance from o to the box: a
+// ray k that meets the box at t <= D passes within t * |d[k] - a| <=
+// D * spread of the axis point o + t a, which therefore lies in the grown
+// box. Double precision only (the growth must not be lost to rounding of the
+// box bounds).
 template <typename TS>
 __device__ uint32_t any_hit_bundle(const SceneView<double>& sc, TS ts, V3<double> o, const V3<double>* d, int n,
                                    V3<double> a, double spread, double tmin, double tmax)
@@ -1129,13 +1137,13 @@ __device__ uint32_t any_hit_bundle(const SceneView<double>& sc, TS ts, V3<double
     const double g = spread * 1.000001 + 1e-9;
     Slab<double> sl;
     sl.init(o, a);
-    auto meets = [&](double x0, double x1, double y0, double y1, double z0, double z1) -> bool {
+    // the grown box's entry distance along the axis in *te (near-first order)
+    auto meets = [&](double x0, double x1, double y0, double y1, double z0, double z1, float* te) -> bool {
         const double D = mfabs((x0 + x1) * 0.5 - o.x) + mfabs((y0 + y1) * 0.5 - o.y) +
                          mfabs((z0 + z1) * 0.5 - o.z) + (x1 - x0) * 0.5 + (y1 - y0) * 0.5 + (z1 - z0) * 0.5;
         const double e = D * g;
         bool hit;
-        float te;
-        sl.one(x0 - e, x1 + e, y0 - e, y1 + e, z0 - e, z1 + e, tmin, tmax, &hit, &te);
+        sl.one(x0 - e, x1 + e, y0 - e, y1 + e, z0 - e, z1 + e, tmin, tmax, &hit, te);
         return hit;
     };
     auto leaf = [&](int ref) -> bool {
@@ -1158,10 +1166,33 @@ __device__ uint32_t any_hit_bundle(const SceneView<double>& sc, TS ts, V3<double
             if (ref >= 0) {
                 const DNode4 nd = ldg(&sc.nodes4[ref]);
                 const int cnt = nd.meta.x;
-                if (cnt > 3 && meets(nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w)) st.push(nd.ref.w, 0.f);
-                if (cnt > 2 && meets(nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z)) st.push(nd.ref.z, 0.f);
-                if (meets(nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y)) st.push(nd.ref.y, 0.f);
-                if (meets(nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x)) st.push(nd.ref.x, 0.f);
+                const float inf = __builtin_inff();
+                float kk[4];
+                int qq[4] = {nd.ref.x, nd.ref.y, nd.ref.z, nd.ref.w};
+                if (!meets(nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, &kk[0])) kk[0] = inf;
+                if (!meets(nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, &kk[1])) kk[1] = inf;
+                if (!(cnt > 2 && meets(nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, &kk[2]))) kk[2] = inf;
+                if (!(cnt > 3 && meets(nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, &kk[3]))) kk[3] = inf;
+                if (MH_BUNDLE_SORT) {
+                    // nearest first along the axis: blocked rays end the walk sooner
+                    auto cs = [&](int a_, int b_) {
+                        if (kk[b_] < kk[a_]) {
+                            const float tk_ = kk[a_];
+                            kk[a_] = kk[b_];
+                            kk[b_] = tk_;
+                            const int tq_ = qq[a_];
+                            qq[a_] = qq[b_];
+                            qq[b_] = tq_;
+                        }
+                    };
+                    cs(0, 1);
+                    cs(2, 3);
+                    cs(0, 2);
+                    cs(1, 3);
+                    cs(1, 2);
+                }
+                for (int i = 3; i >= 0; --i)
+                    if (kk[i] < inf) st.push(qq[i], 0.f);
             } else if (leaf(ref)) {
                 return occ;
             }
@@ -1174,8 +1205,16 @@ __device__ uint32_t any_hit_bundle(const SceneView<double>& sc, TS ts, V3<double
         const int ref = st.pop(&te);
         if (ref >= 0) {
             const DNode nd = ldg(&sc.nodes[ref]);
-            if (meets(nd.c1.z, nd.c1.w, nd.c2.x, nd.c2.y, nd.c2.z, nd.c2.w)) st.push(nd.c3.y, 0.f);
-            if (meets(nd.c0.x, nd.c0.y, nd.c0.z, nd.c0.w, nd.c1.x, nd.c1.y)) st.push(nd.c3.x, 0.f);
+            float tl, tr;
+            const bool hr = meets(nd.c1.z, nd.c1.w, nd.c2.x, nd.c2.y, nd.c2.z, nd.c2.w, &tr);
+            const bool hl = meets(nd.c0.x, nd.c0.y, nd.c0.z, nd.c0.w, nd.c1.x, nd.c1.y, &tl);
+            if (hl && hr && MH_BUNDLE_SORT && tr < tl) {
+                st.push(nd.c3.x, 0.f);
+                st.push(nd.c3.y, 0.f);
+            } else {
+                if (hr) st.push(nd.c3.y, 0.f);
+                if (hl) st.push(nd.c3.x, 0.f);
+            }
         } else if (leaf(ref)) {
             return occ;
         }

@@ -13,7 +13,7 @@ from marmel_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 VARIANTS = ("hd_render", "hd_render_asan", "hd_render_pattern", "hd_render_zero",
-            "hd_render_anysort0", "hd_render_heldfp64")
+            "hd_render_anysort0", "hd_render_heldfp64", "hd_render_nobundle")
 
 
 class Request(C.Structure):
