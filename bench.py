@@ -58,6 +58,8 @@ SHADOW_LIMITER = ("dependent node fetches and VALU issue (63 % of SIMD cycles, 5
 W, H, SPP, DEPTH = 1920, 1080, 64, 8
 GRID = 1024
 STRESS_MAX_LEAF, STRESS_BINS = 8, 32  # device tree build options of the stress scene
+# the autotuner's choice for the terrain frame, which the stress leg reuses
+STRESS_TUNE = dict(pool=1 << 26, grid_blocks=16384, block=256, bvh_width=4, fixed_batches=1)
 
 
 def log(msg):
@@ -455,6 +457,9 @@ def main():
                     help="skip the adaptive Whitted + flare leg (steps after the framebuffer)")
     ap.add_argument("--only-post", action="store_true",
                     help="run only the adaptive Whitted + flare leg (for profiling its kernels)")
+    ap.add_argument("--only-stress", action="store_true",
+                    help="run only the stress leg, with the launch settings the terrain frame is tuned to "
+                         "(for profiling its kernels; --stress-seconds bounds it)")
     ap.add_argument("--hbm-calib", type=int, default=0, metavar="MIB",
                     help="only run the copy / multiply kernels of MIB MiB that calibrate the HBM counters")
     args = ap.parse_args()
@@ -468,6 +473,9 @@ def main():
 
     from marmel_amd import Batch, GpuScene, abi, device_info
 
+    if args.only_stress:
+        log(f"bench: stress {stress_leg(STRESS_TUNE, args.stress_seconds)}")
+        return
     t0 = time.time()
     hs, info, st = build_scene(args.grid)
     log(f"bench: scene {info.n_prims} prims ({info.n_triangles} triangles, {info.n_spheres} spheres), "

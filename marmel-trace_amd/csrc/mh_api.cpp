@@ -184,10 +184,12 @@ struct mh_scene {
     hipGraph_t graph_def = nullptr;
     std::vector<unsigned char> graph_key;
     EventClock clock;
-    // material tables and the spheres that may be emissive lights (for updates)
+    // material tables, the spheres that may be emissive lights and each
+    // material's primitive bounds (for updates)
     void* d_m32 = nullptr;
     void* d_m64 = nullptr;
     std::vector<LightSphere> spheres;
+    std::vector<double> mat_box;
     // adjoint buffers
     float* d_dimg = nullptr;
     size_t dimg_n = 0;
@@ -348,7 +350,8 @@ int alloc_wf(int pool, size_t npix, bool jid, void** mem, size_t* bytes, WfState
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t sz[] = {al(P * 16), al(P * 16), al(P * 16), al(P * 16), al(P * 8), al(P * 9 * 16), al(P * 4),
                    al(P * 4),  al(P * 4),  al(P * 4),  al(P * 4),  al(P * 9 * 16), al(P * 9 * 16), al(64),
-                   al(npix * 3 * 4), al(64), jid ? al(P * 4) : 0, jid ? al(P * 4) : 0, al(P * 9 * 4)};
+                   al(npix * 3 * 4), al(64), jid ? al(P * 4) : 0, jid ? al(P * 4) : 0, al(P * 9 * 4),
+                   al(P * 9 * 4)};
     const int nbuf = (int)(sizeof sz / sizeof sz[0]);
     size_t total = 0;
     for (size_t v : sz) total += v;
@@ -379,6 +382,7 @@ int alloc_wf(int pool, size_t npix, bool jid, void** mem, size_t* bytes, WfState
     w->jid = (uint32_t*)ptrs[16];
     w->freeq = (uint32_t*)ptrs[17];
     w->shq = (uint32_t*)ptrs[18];
+    w->slowq = (uint32_t*)ptrs[19];
     return 0;
 }
 
@@ -758,8 +762,12 @@ int finish_scene(mh_scene* s, const mh_scene_soa* soa, Layout lay, const std::ve
     int has_transmissive = any_transmissive(soa->materials, soa->n_materials);
     s->fog_host = soa->fog;
     s->has_water = any_water(soa->materials, soa->n_materials);
-    fill_view(s->v32, lay, soa, s->has_water, l32, nl, has_transmissive);
-    fill_view(s->v64, lay, soa, s->has_water, l64, nl, has_transmissive);
+    s->mat_box = material_boxes(soa);
+    float tlo[3], thi[3];
+    transmissive_box(s->mat_box, soa->materials, soa->n_materials, tlo, thi);
+    const int split = trans_split_pays(s->mat_box, tlo, thi);
+    fill_view(s->v32, lay, soa, s->has_water, l32, nl, has_transmissive, tlo, thi, split);
+    fill_view(s->v64, lay, soa, s->has_water, l64, nl, has_transmissive, tlo, thi, split);
     return 0;
 }
 
@@ -995,6 +1003,13 @@ int mh_scene_set_materials(mh_scene* s, const mh_material* mats, int n)
         s->v64.lights[i] = l64[i];
     }
     s->v32.has_transmissive = s->v64.has_transmissive = tr;
+    float tlo[3], thi[3];
+    transmissive_box(s->mat_box, mats, n, tlo, thi);
+    s->v32.tlo = V3<float>{tlo[0], tlo[1], tlo[2]};
+    s->v32.thi = V3<float>{thi[0], thi[1], thi[2]};
+    s->v64.tlo = V3<double>{tlo[0], tlo[1], tlo[2]};
+    s->v64.thi = V3<double>{thi[0], thi[1], thi[2]};
+    s->v32.trans_split = s->v64.trans_split = trans_split_pays(s->mat_box, tlo, thi);
     s->has_water = any_water(mats, n);
     s->v32.fog_kernel = s->v64.fog_kernel = needs_fog_kernel(s->fog_host, s->has_water);
     drop_scene_graph(s);  // captured launches hold the scene view by value
@@ -1786,7 +1801,15 @@ int mh_render_batch(mh_batch* b, mh_scene* const* scenes, int n_scenes, const mh
     std::vector<SceneView<float>> views((size_t)n_scenes);
     int feat = b->count_on ? WF_COUNT : 0;
     for (int i = 0; i < n_scenes; ++i) {
-        if (scenes[i]->v32.has_transmissive) feat |= WF_TRANS;
+        if (scenes[i]->v32.has_transmissive) {
+            feat |= WF_TRANS;
+            // the queue is split only when every transmissive scene's box
+            // is small (SceneView::trans_split); test hook:
+            // MARMEL_SHADOW_SPLIT=0 / 1 never / always splits
+            const char* e = std::getenv("MARMEL_SHADOW_SPLIT");
+            const bool force = e && (e[0] == '0' || e[0] == '1') && e[1] == 0;
+            if (force ? e[0] == '0' : !scenes[i]->v32.trans_split) feat |= WF_NOSPLIT;
+        }
         if (scenes[i]->v32.fog_kernel) feat |= WF_FOG;
     }
     // width 4: every scene whose 4-wide tree was built; width 2: binary

@@ -56,11 +56,12 @@ struct WfState {
     float4* sh0;      // shadow rays cast this bounce, dense (count in cnt[2]): origin.xyz, tmax
     float4* sh1;      // direction.xyz, (slot << 4 | estimate) bits
     uint32_t* shq;    // batched engine: scene index of each shadow ray
-    uint32_t* cnt;    // [0..1] ext queue lengths, [2] shadow, [3] term
+    uint32_t* cnt;    // [0..1] ext queue lengths, [2] shadow, [3] term, [4] free slots, [5] slow list
     float* acc;       // per-pixel radiance sums (w*h*3)
     unsigned long long* batch;  // [0] first global path index of the current batch
     uint32_t* jid;    // batched engine: job of each path slot
     uint32_t* freeq;  // batched engine: free path slots (count in cnt[4])
+    uint32_t* slowq;  // shadow-queue positions of rays that can meet a transmissive surface (count in cnt[5])
 };
 
 // One render job of the batched engine: a camera view of one scene.
@@ -125,7 +126,11 @@ hipError_t wf_enqueue_end(const WfState& ws, int npix, int spp, const LaunchOut&
 // Kernel variants a batch needs (bit set): some scene has a transmissive
 // material (shadow rays walk through it) / has fog or marched water / is
 // traversed on its 4-wide tree.
-enum { WF_TRANS = 1, WF_FOG = 2, WF_WIDE = 4, WF_COUNT = 8 };
+// WF_NOSPLIT: a transmissive launch sends every shadow ray through the
+// closest-hit steps (k_wf_shadow<., true>) instead of splitting the queue by
+// the transmissive box (some scene's box is large, SceneView::trans_split;
+// the same image either way).
+enum { WF_TRANS = 1, WF_FOG = 2, WF_WIDE = 4, WF_COUNT = 8, WF_NOSPLIT = 16 };
 // Launch modes of the wavefront kernels. WF_SINGLE: one scene and one job,
 // both kernel arguments. The batched engine's job table: WF_ONE when every
 // job renders the same scene (read at a launch-uniform index, so its fields

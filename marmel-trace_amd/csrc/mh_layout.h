@@ -15,6 +15,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 namespace mh {
@@ -323,11 +324,127 @@ inline int collect_lights(const std::vector<LightSphere>& spheres, const mh_mate
     return nl;
 }
 
+inline bool transmissive(const mh_material& m) { return m.is_water || m.transparency > 0.8; }
+
 inline int any_transmissive(const mh_material* mats, int n)
 {
     for (int i = 0; i < n; ++i)
-        if (mats[i].is_water || mats[i].transparency > 0.8) return 1;
+        if (transmissive(mats[i])) return 1;
     return 0;
+}
+
+// Bounding box of each material's primitives: box[6 * m .. 6 * m + 6) = lo
+// xyz, hi xyz (lo > hi when no primitive uses material m; infinite when a
+// plane, or a primitive of unknown kind, does). Primitives whose material
+// index is out of range are left out: a shadow walk treats them as opaque.
+inline std::vector<double> material_boxes(const mh_scene_soa* soa)
+{
+    const int nm = soa->n_materials > 0 ? soa->n_materials : 0;
+    const double inf = std::numeric_limits<double>::infinity();
+    std::vector<double> box((size_t)nm * 6);
+    for (int m = 0; m < nm; ++m)
+        for (int a = 0; a < 3; ++a) {
+            box[6 * m + a] = inf;
+            box[6 * m + 3 + a] = -inf;
+        }
+    for (int i = 0; i < soa->n_prims; ++i) {
+        const mh_prim& p = soa->prims[i];
+        if (p.material < 0 || p.material >= nm) continue;
+        double lo[3], hi[3];
+        const double c0[3] = {p.p0.x, p.p0.y, p.p0.z}, c1[3] = {p.p1.x, p.p1.y, p.p1.z},
+                     c2[3] = {p.p2.x, p.p2.y, p.p2.z};
+        const double rmax = p.r0 > p.r1 ? p.r0 : p.r1;
+        for (int a = 0; a < 3; ++a) {
+            switch (p.kind) {
+            case MH_PRIM_SPHERE:
+                lo[a] = c0[a] - std::fabs(p.r0);
+                hi[a] = c0[a] + std::fabs(p.r0);
+                break;
+            case MH_PRIM_BOX:
+                lo[a] = c0[a] - std::fabs(c1[a]);
+                hi[a] = c0[a] + std::fabs(c1[a]);
+                break;
+            case MH_PRIM_TRIANGLE:
+                lo[a] = std::min(c0[a], std::min(c1[a], c2[a]));
+                hi[a] = std::max(c0[a], std::max(c1[a], c2[a]));
+                break;
+            case MH_PRIM_CYLINDER:
+                lo[a] = std::min(c0[a], c1[a]) - std::fabs(rmax);
+                hi[a] = std::max(c0[a], c1[a]) + std::fabs(rmax);
+                break;
+            default:  // planes: no finite box
+                lo[a] = -inf;
+                hi[a] = inf;
+            }
+            // NaN coordinates: no finite box either
+            if (!(lo[a] <= hi[a])) {
+                lo[a] = -inf;
+                hi[a] = inf;
+            }
+            double* b = &box[6 * (size_t)p.material];
+            b[a] = std::min(b[a], lo[a]);
+            b[3 + a] = std::max(b[3 + a], hi[a]);
+        }
+    }
+    return box;
+}
+
+// Whether the shadow kernels should split their queue by the transmissive
+// box (SceneView::trans_split): the box's surface area, the chance that a
+// segment through the scene meets it, is under a quarter of the area of the
+// box around every finite primitive, or no primitive is transmissive. On
+// scenes with transmissive primitives all over (glass spheres scattered
+// through the scene) nearly every ray would be listed for the closest-hit
+// steps, and the extra any-hit pass over the queue would cost more than it
+// saves.
+inline int trans_split_pays(const std::vector<double>& box, const float tlo[3], const float thi[3])
+{
+    if (!(tlo[0] <= thi[0] && tlo[1] <= thi[1] && tlo[2] <= thi[2])) return 1;
+    const float big = 1e30f;
+    for (int a = 0; a < 3; ++a)
+        if (!(tlo[a] > -big && thi[a] < big)) return 0;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t m = 0; 6 * m + 5 < box.size(); ++m) {
+        bool finite = true;
+        for (int a = 0; a < 3; ++a)
+            finite = finite && std::fabs(box[6 * m + a]) < 1e30 && std::fabs(box[6 * m + 3 + a]) < 1e30 &&
+                     box[6 * m + a] <= box[6 * m + 3 + a];
+        if (!finite) continue;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], box[6 * m + a]);
+            hi[a] = std::max(hi[a], box[6 * m + 3 + a]);
+        }
+    }
+    auto area = [](const double* l, const double* h) {
+        const double dx = h[0] - l[0], dy = h[1] - l[1], dz = h[2] - l[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    const double tl[3] = {tlo[0], tlo[1], tlo[2]}, th[3] = {thi[0], thi[1], thi[2]};
+    return area(tl, th) < 0.25 * area(lo, hi) ? 1 : 0;
+}
+
+// The box around every primitive of a transmissive material (see
+// SceneView::tlo), rounded outwards to fp32.
+inline void transmissive_box(const std::vector<double>& box, const mh_material* mats, int n, float lo[3],
+                             float hi[3])
+{
+    const float inf = std::numeric_limits<float>::infinity();
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = inf;
+        hi[a] = -inf;
+    }
+    for (int m = 0; m < n && 6 * (size_t)m + 5 < box.size(); ++m) {
+        if (!transmissive(mats[m])) continue;
+        for (int a = 0; a < 3; ++a) {
+            const double l = box[6 * (size_t)m + a], h = box[6 * (size_t)m + 3 + a];
+            if (!(l <= h)) continue;
+            float fl = (float)l, fh = (float)h;
+            if ((double)fl > l) fl = std::nextafter(fl, -inf);
+            if ((double)fh < h) fh = std::nextafter(fh, inf);
+            lo[a] = std::min(lo[a], fl);
+            hi[a] = std::max(hi[a], fh);
+        }
+    }
 }
 
 // Pointers of a scene's arrays (device memory, or host memory in a host
@@ -440,7 +557,8 @@ inline int build_host_layout(const mh_scene_soa* soa, HostLayout* out)
 // The view of one precision: arrays from `lay`, tables from the scene.
 template <typename R>
 inline void fill_view(SceneView<R>& v, const Layout& lay, const mh_scene_soa* soa, int has_water,
-                      const DLight<R>* lights, int nl, int has_transmissive)
+                      const DLight<R>* lights, int nl, int has_transmissive, const float tlo[3],
+                      const float thi[3], int trans_split)
 {
     const bool f = sizeof(R) == 4;
     v.nodes = (const DNode*)lay.nodes;
@@ -466,6 +584,9 @@ inline void fill_view(SceneView<R>& v, const Layout& lay, const mh_scene_soa* so
     v.water_level = (R)soa->water_level;
     v.water_material = soa->water_material;
     v.has_transmissive = has_transmissive;
+    v.tlo = V3<R>{(R)tlo[0], (R)tlo[1], (R)tlo[2]};
+    v.thi = V3<R>{(R)thi[0], (R)thi[1], (R)thi[2]};
+    v.trans_split = trans_split;
     v.n_lights = nl;
     for (int i = 0; i < 8; ++i) v.lights[i] = lights[i];
 }

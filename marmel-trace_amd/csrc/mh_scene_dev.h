@@ -153,6 +153,14 @@ template <typename R> struct SceneView {
     // 0 when no material is water or has transparency > 0.8: a shadow
     // segment is then blocked exactly when anything lies on it (any-hit)
     int has_transmissive;
+    // box around every primitive of such a material (tlo > thi: none;
+    // unbounded when a plane has one): a shadow segment that misses it meets
+    // opaque surfaces only, and any-hit answers it too
+    V3<R> tlo, thi;
+    // 1 when the wavefront shadow kernels split their queue by that box
+    // (it is small against the scene); 0: every shadow ray of a transmissive
+    // scene takes the closest-hit steps
+    int trans_split;
     int n_lights;
     DLight<R> lights[8];
 };

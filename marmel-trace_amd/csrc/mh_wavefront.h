@@ -180,11 +180,13 @@ template <int N> __device__ __forceinline__ void add_visits(unsigned long long* 
 
 __global__ __launch_bounds__(64) void k_wf_reset(uint32_t* cnt, int which)
 {
-    // zero the output queue of this bounce plus the shadow / term counters
+    // zero the output queue of this bounce plus the shadow / term / slow-list
+    // counters
     if (threadIdx.x == 0) {
         cnt[which] = 0;
         cnt[2] = 0;
         cnt[3] = 0;
+        cnt[5] = 0;
     }
 }
 
@@ -244,7 +246,8 @@ struct UScene {
     const P32* prims;
     const P32* planes;
     const int* plane_index;
-    int n_planes, root_ref, root4, bvh4;
+    int n_planes, root_ref, root4, bvh4, has_transmissive;
+    V3<float> tlo, thi;
 };
 
 __device__ __forceinline__ const void* uniform_ptr(const void* p)
@@ -267,7 +270,39 @@ __device__ __forceinline__ UScene uniform_scene(const SceneView<float>& s)
     u.root_ref = __builtin_amdgcn_readfirstlane(s.root_ref);
     u.root4 = __builtin_amdgcn_readfirstlane(s.root4);
     u.bvh4 = __builtin_amdgcn_readfirstlane(s.bvh4);
+    u.has_transmissive = __builtin_amdgcn_readfirstlane(s.has_transmissive);
+    auto uf = [](float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); };
+    u.tlo = mk(uf(s.tlo.x), uf(s.tlo.y), uf(s.tlo.z));
+    u.thi = mk(uf(s.thi.x), uf(s.thi.y), uf(s.thi.z));
     return u;
+}
+
+// Whether the segment o + t d (d unit), t in [0, tmax], can meet the box
+// [lo, hi] grown by 1e-3 + 1e-5 |coordinate| on every side: an fp32 slab
+// test made conservative by the growth and a relative slack on the
+// distances. An empty box (lo > hi) is never met, an unbounded one always.
+__device__ __forceinline__ bool seg_meets_box(V3<float> o, V3<float> d, float tmax, V3<float> lo, V3<float> hi)
+{
+    if (!(lo.x <= hi.x && lo.y <= hi.y && lo.z <= hi.z)) return false;
+    const float big = 1e30f;
+    if (!(lo.x > -big && lo.y > -big && lo.z > -big && hi.x < big && hi.y < big && hi.z < big)) return true;
+    float t0 = 0.f, t1 = fminf(tmax, big) * 1.00001f + 2e-3f;
+    const float os[3] = {o.x, o.y, o.z}, ds[3] = {d.x, d.y, d.z};
+    const float ls[3] = {lo.x, lo.y, lo.z}, hs[3] = {hi.x, hi.y, hi.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float m = 1e-3f + 1e-5f * fmaxf(fabsf(ls[a]), fabsf(hs[a]));
+        const float l = ls[a] - m, h = hs[a] + m;
+        if (fabsf(ds[a]) < 1e-20f) {
+            if (os[a] < l || os[a] > h) return false;
+            continue;
+        }
+        const float inv = 1.f / ds[a];
+        const float ta = (l - os[a]) * inv, tb = (h - os[a]) * inv;
+        t0 = fmaxf(t0, fminf(ta, tb));
+        t1 = fminf(t1, fmaxf(ta, tb));
+    }
+    return t0 <= t1 * 1.00001f + 1e-4f;
 }
 
 // The four children of 4-wide node `ref` of a launch-uniform node array,
@@ -1059,7 +1094,9 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneView<float> sc0, WfState 
 // Beer-Lambert attenuation). Otherwise every first hit is opaque and the
 // query is a single any-hit traversal (same answer), with a smaller stack and
 // register footprint.
-template <int MODE, bool TRANS>
+// LIST: only the rays k_wf_shadow_refill<., ., true> listed in st.slowq
+// (count in cnt[5]): those whose segment can meet a transmissive surface.
+template <int MODE, bool TRANS, bool LIST = false>
 __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState st, unsigned long long* rays,
                                                     WfMulti mj)
 {
@@ -1073,11 +1110,12 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 #else
     MH_STACK_PRIV(tsa)
 #endif
-    // One lane per ray of the dense shadow queue.
-    const uint32_t n = st.cnt[2];
+    // One lane per ray of the dense shadow queue (LIST: of the slow list).
+    const uint32_t n = LIST ? st.cnt[5] : st.cnt[2];
     int nq = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t v = LIST ? st.slowq[i] : i;
         float4 b = st.sh1[v], a = st.sh0[v];
         uint32_t tag = f2u(b.w);
         uint32_t slot = tag >> 4, k = tag & 15u;
@@ -1156,7 +1194,13 @@ __global__ __launch_bounds__(256) void k_wf_shadow(SceneView<float> sc0, WfState
 // memory. Lanes that finish early therefore take new rays instead of idling
 // until the wave's longest traversal ends. Same visibility bits as
 // k_wf_shadow<., false> (the query result does not depend on the order).
-template <int MODE, bool COUNT = false>
+// TRANS (some scene of the launch has a transmissive material): a ray whose
+// segment meets the scene's transmissive box (seg_meets_box) is not walked
+// here but listed in st.slowq for k_wf_shadow<., true, true>, whose
+// closest-hit steps pass through those surfaces; every other ray meets
+// opaque surfaces only, so the any-hit walk gives its answer (and an
+// attenuation of 1).
+template <int MODE, bool COUNT = false, bool TRANS = false>
 __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, WfState st, unsigned long long* rays,
                                                            WfMulti mj)
 {
@@ -1320,8 +1364,8 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
         if (nidle == 64u && avail == 0u && !seq_more) break;
         // the loaded rays to the idle lanes, in lane order
         const uint32_t take = avail < nidle ? avail : nidle;
-        bool got = false;
-        uint32_t scn = 0u;
+        bool got = false, slow = false;
+        uint32_t scn = 0u, qpos = 0u;
         if (take > 0u) {
             // the chunk loads of earlier rounds have landed in LDS: s_waitcnt
             // vmcnt(0) (the traversal steps leave no other load in flight)
@@ -1338,6 +1382,8 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
                 d = mk(b.x, b.y, b.z);
                 tag = f2u(b.w);
                 got = true;
+                // the ray's queue position (its chunk's start + its place)
+                qpos = ((s / CH) * nw + gw) * CH + (s & (CH - 1u));
             }
             rd += take;
         }
@@ -1350,10 +1396,29 @@ __global__ __launch_bounds__(256) void k_wf_shadow_refill(SceneView<float> sc0, 
         if (seq_more && (jn < 2u || rd >= (jn - 1u) * CH)) load_chunk();
         if (got) {
             const SceneView<float>& sc = mode_scene<MODE>(sc0, mj, (int)scn);
+            d = unit_dir(d);
+            if (TRANS)
+                slow = (UNI ? scu.has_transmissive : sc.has_transmissive) &&
+                       seg_meets_box(o, d, tmax, UNI ? scu.tlo : sc.tlo, UNI ? scu.thi : sc.thi);
+        }
+        if (TRANS) {
+            // the slow rays to the list: one atomic per wave
+            const uint64_t sm = __ballot(slow);
+            if (sm) {
+                const int first = __builtin_ctzll(sm);
+                uint32_t at = 0u;
+                if (lane == first) at = atomicAdd(&st.cnt[5], (uint32_t)__popcll(sm));
+                at = (uint32_t)__shfl((int)at, first);
+                if (slow)
+                    st.slowq[at + __builtin_amdgcn_mbcnt_hi((uint32_t)(sm >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)sm, 0u))] = qpos;
+            }
+        }
+        if (got && !slow) {
+            const SceneView<float>& sc = mode_scene<MODE>(sc0, mj, (int)scn);
             const int n_planes = UNI ? scu.n_planes : sc.n_planes;
             const P32* planes = UNI ? scu.planes : sc.planes;
             const int root_ref = UNI ? scu.root_ref : sc.root_ref;
-            d = unit_dir(d);
             ++nq;
             // planes first (never in the tree), as any_hit does
             bool occ = false;

@@ -6,8 +6,9 @@ namespace mh {
 size_t wf_bytes_per_path()
 {
     // p0..p3 + hit + 9 pending + vis + npend + 2 queues + term + 9 shadow rays + 9 shadow queue entries
+    // + 9 slow-list entries
     return 4 * sizeof(float4) + sizeof(float2) + 9 * sizeof(float4) + 2 * sizeof(uint32_t) +
-           3 * sizeof(uint32_t) + 9 * 2 * sizeof(float4) + 9 * sizeof(uint32_t);
+           3 * sizeof(uint32_t) + 9 * 2 * sizeof(float4) + 9 * sizeof(uint32_t) + 9 * sizeof(uint32_t);
 }
 
 hipError_t wf_enqueue_begin(const WfState& ws, int npix, hipStream_t st)
@@ -41,6 +42,24 @@ template <int MODE> static auto shadow_any(int feat)
     return k_wf_shadow<MODE, false>;
 }
 
+// Transmissive launches: 1 = the dynamic-fetch any-hit kernel takes every
+// shadow ray whose segment misses the transmissive surfaces' box and lists
+// the others for the closest-hit steps of k_wf_shadow<., true, true>; 0 =
+// k_wf_shadow<., true> takes every ray through the closest-hit steps.
+#ifndef MH_SHADOW_SPLIT
+#define MH_SHADOW_SPLIT 1
+#endif
+static bool split_trans(int feat) { return (feat & WF_TRANS) && !(feat & WF_NOSPLIT) && MH_SHADOW_SPLIT && MH_SHADOW_REFILL > 0; }
+template <int MODE> static auto shadow_trans(int feat)
+{
+    if (split_trans(feat)) {
+        if constexpr (MODE != WF_SINGLE)
+            if (feat & WF_COUNT) return k_wf_shadow_refill<MODE, true, true>;
+        return k_wf_shadow_refill<MODE, false, true>;
+    }
+    return k_wf_shadow<MODE, true>;
+}
+
 // Dynamic LDS of the shade kernel: the shadow rays of each thread, staged
 // until their queue positions are known (9 light blocks + the origin).
 static size_t shade_lds(int block) { return (size_t)10 * (size_t)block * sizeof(float4); }
@@ -65,7 +84,8 @@ static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<floa
                                   const WfState& ws, const WfMulti& mj, unsigned long long* rays, int grid, int block,
                                   int feat, hipStream_t st, KernelClock* kc)
 {
-    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<MODE, true> : shadow_any<MODE>(feat);
+    const bool split = split_trans(feat);
+    auto shadow = (feat & WF_TRANS) ? shadow_trans<MODE>(feat) : shadow_any<MODE>(feat);
     auto shade = (feat & WF_FOG) ? k_wf_shade<MODE, true> : k_wf_shade<MODE, false>;
     dim3 G(grid), B(block);
     MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_gen<MODE>, G, B, 0, st, sc, cam, job, ws, mj));
@@ -75,6 +95,8 @@ static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<floa
         MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(extend_kernel<MODE>(feat), G, B, 0, st, sc, ws, which, mj));
         MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, shade_lds(block), st, sc, ws, job, which, b, rays, mj));
         MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
+        if (split)
+            MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL((k_wf_shadow<MODE, true, true>), G, B, 0, st, sc, ws, rays, mj));
         MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj));
     }
     // batched engine, fixed batches: the batch's pixel sums (k_wf_finalize
@@ -89,7 +111,8 @@ hipError_t wf_enqueue_batch(const SceneView<float>& sc, const CamView<float>& ca
                             const WfState& ws, unsigned long long* rays, int grid, hipStream_t st, KernelClock* kc)
 {
     WfMulti none{};
-    int feat = (sc.has_transmissive ? WF_TRANS : 0) | (sc.fog_kernel ? WF_FOG : 0) | (sc.bvh4 ? WF_WIDE : 0);
+    int feat = (sc.has_transmissive ? WF_TRANS : 0) | (sc.has_transmissive && !sc.trans_split ? WF_NOSPLIT : 0) |
+               (sc.fog_kernel ? WF_FOG : 0) | (sc.bvh4 ? WF_WIDE : 0);
     return enqueue_bounces<WF_SINGLE>(sc, cam, job, ws, none, rays, grid, 256, feat, st, kc);
 }
 
@@ -130,7 +153,8 @@ static hipError_t stream_step(const WfMulti& mj, const WfState& ws, int which, u
     dim3 G(grid), B(block);
     SceneView<float> sc{};
     WfJob job{};
-    auto shadow = (feat & WF_TRANS) ? k_wf_shadow<MODE, true> : shadow_any<MODE>(feat);
+    const bool split = split_trans(feat);
+    auto shadow = (feat & WF_TRANS) ? shadow_trans<MODE>(feat) : shadow_any<MODE>(feat);
     auto shade = (feat & WF_FOG) ? k_wf_shade<MODE, true> : k_wf_shade<MODE, false>;
     MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_refill<MODE>, G, B, 0, st, ws, mj, which));
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_refill_commit, dim3(1), dim3(64), 0, st, ws, mj));
@@ -138,6 +162,8 @@ static hipError_t stream_step(const WfMulti& mj, const WfState& ws, int which, u
     MH_TIMED(MH_K_EXTEND, hipLaunchKernelGGL(extend_kernel<MODE>(feat), G, B, 0, st, sc, ws, which, mj));
     MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, shade_lds(block), st, sc, ws, job, which, 0, rays, mj));
     MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
+    if (split)
+        MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL((k_wf_shadow<MODE, true, true>), G, B, 0, st, sc, ws, rays, mj));
     MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj));
     return hipGetLastError();
 }

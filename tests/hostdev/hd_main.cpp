@@ -79,8 +79,12 @@ void render(const mh_scene_soa& soa, const Request& q, std::vector<float>* lin, 
     SceneView<R> v{};
     int has_water = any_water(soa.materials, soa.n_materials);
     int tr = any_transmissive(soa.materials, soa.n_materials);
-    if constexpr (sizeof(R) == 4) fill_view(v, lay, &soa, has_water, l32, nl, tr);
-    else fill_view(v, lay, &soa, has_water, l64, nl, tr);
+    float tlo[3], thi[3];
+    const std::vector<double> mb = material_boxes(&soa);
+    transmissive_box(mb, soa.materials, soa.n_materials, tlo, thi);
+    const int split = trans_split_pays(mb, tlo, thi);
+    if constexpr (sizeof(R) == 4) fill_view(v, lay, &soa, has_water, l32, nl, tr, tlo, thi, split);
+    else fill_view(v, lay, &soa, has_water, l64, nl, tr, tlo, thi, split);
     if (q.width == 2) v.bvh4 = 0;
     CamView<R> cam = conv_cam<R>(q.cam);
     const int w = q.w, h = q.h;

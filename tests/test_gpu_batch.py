@@ -188,7 +188,7 @@ def test_autotune_stops_at_the_largest_pool_that_fits(world):
     jobs = [dict(scene=0, cam=hs.camera(16 / 9), width=1920, height=1080, spp=32, depth=2)]
     b = Batch()
     free, _ = torch.cuda.mem_get_info()
-    keep = 24 << 30  # room for pools up to 2^25 paths (~17.5 GB), not 2^26 (~35 GB)
+    keep = 24 << 30  # room for pools up to 2^25 paths (~20 GB), not 2^26 (~40 GB)
     blocker = torch.empty(max(free - keep, 0), dtype=torch.uint8, device="cuda") if free > keep else None
     try:
         res = b.render([gs], jobs, autotune=True)
@@ -200,4 +200,49 @@ def test_autotune_stops_at_the_largest_pool_that_fits(world):
     assert t["pool"] <= 1 << 25
     ref = b.render([gs], jobs, engine=t["engine"])
     assert np.array_equal(ref[0][1], res[0][1])
+    b.close()
+
+
+def _water_plane_scene():
+    import scenes
+    hs = HostScene()
+    hs.generate(native.GEN_SPHERES, seed=3)
+    wm = hs.add_materials(np.array([scenes.material(albedo=(0.02, 0.06, 0.08), transparency=1.0, ior=1.333,
+                                                    is_water=1)], dtype=abi.MATERIAL_DTYPE))
+    hs.add_prims(np.array([scenes.prim(abi.PRIM_PLANE, wm, p0=(0, 0.3, 0), p1=(0, 1, 0))], dtype=abi.PRIM_DTYPE))
+    hs.build_bvh()
+    return hs, hs.camera(1.0)
+
+
+def test_transmissive_split_keeps_the_image(monkeypatch):
+    """Shadow queries of scenes with water / near-clear materials: a ray
+    whose segment misses the box around those primitives takes the any-hit
+    walk, the others the closest-hit steps through them (src/pathtrace.c:
+    504-532). The image with the queue split (MARMEL_SHADOW_SPLIT=1) equals
+    the one with every ray through the closest-hit steps (=0), bit for bit:
+    on the forest with its water box, on the ocean world (water triangles)
+    and with a water plane (no finite box: every ray takes the steps)."""
+    from test_ingest import ocean_world
+    hf = HostScene()
+    hf.generate(native.GEN_FOREST, seed=5, grid=6, subdiv=3)
+    hf.build_bvh()
+    ho = ocean_world()
+    ho.build_bvh()
+    hw, cw = _water_plane_scene()
+    cases = [("forest", hf, hf.camera(16 / 9), 320, 180),
+             ("ocean", ho, ho.camera(1.0, eye=[0.0, 0.25, -9.0], at=[0.0, -0.2, 4.0], up=[0, 1, 0], vfov=55), 160, 160),
+             ("water plane", hw, cw, 128, 128)]
+    b = Batch()
+    for name, hs, cam, w, h in cases:
+        gs = GpuScene(hs.soa())
+        job = [dict(scene=0, cam=cam, width=w, height=h, spp=16, depth=8)]
+        for fixed in (True, False):
+            monkeypatch.setenv("MARMEL_SHADOW_SPLIT", "1")
+            (_, split), = b.render([gs], job, pool=1 << 16, fixed_batches=fixed)
+            monkeypatch.setenv("MARMEL_SHADOW_SPLIT", "0")
+            (_, steps), = b.render([gs], job, pool=1 << 16, fixed_batches=fixed)
+            print(name, "fixed batches" if fixed else "path regeneration", "image mean", split.mean())
+            assert split.mean() > 0
+            assert np.array_equal(split, steps), (name, fixed)
+        gs.close()
     b.close()
