@@ -490,9 +490,12 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState
 // Idle lanes of a wave at which the closest-hit extend kernel hands them new
 // rays (dynamic fetch); 0 selects k_wf_extend. Measured on the 1080p terrain
 // frame (extend ms per frame, binary tree): 0 -> 77.2, 32 -> 61.7,
-// 48 -> 59.9, 56 -> 61.0.
+// 48 -> 59.9, 56 -> 61.0; 4-wide tree, near / far tests, linear stack:
+// 16 -> 40.7, 24 -> 39.7, 32 -> 40.0, 48 -> 39.2. The 4K stress scene's
+// deeper, less even walks (one 64-spp chunk, s): 16 -> 0.938, 24 -> 0.938,
+// 32 -> 0.946, 40 -> 0.964, 48 -> 0.992.
 #ifndef MH_EXTEND_REFILL
-#define MH_EXTEND_REFILL 48
+#define MH_EXTEND_REFILL 24
 #endif
 // On the 4-wide tree (scenes whose view selects it; the autotuner picks it
 // for the 1080p terrain frame): the nearest hit child is visited next from a

@@ -60,6 +60,11 @@ template <int MODE> static auto shadow_trans(int feat)
     return k_wf_shadow<MODE, true>;
 }
 
+// Workgroups of the slow-list kernel: the queue is split only where few rays
+// meet the transmissive box, so the list is short, and a full persistent
+// grid of workgroups that find nothing to do costs ~0.1 ms per launch.
+static dim3 list_grid(int grid) { return dim3(grid < 1024 ? grid : 1024); }
+
 // Dynamic LDS of the shade kernel: the shadow rays of each thread, staged
 // until their queue positions are known (9 light blocks + the origin).
 static size_t shade_lds(int block) { return (size_t)10 * (size_t)block * sizeof(float4); }
@@ -96,7 +101,8 @@ static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<floa
         MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, shade_lds(block), st, sc, ws, job, which, b, rays, mj));
         MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
         if (split)
-            MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL((k_wf_shadow<MODE, true, true>), G, B, 0, st, sc, ws, rays, mj));
+            MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL((k_wf_shadow<MODE, true, true>), list_grid(grid), B, 0, st, sc, ws,
+                                                     rays, mj));
         MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj));
     }
     // batched engine, fixed batches: the batch's pixel sums (k_wf_finalize
@@ -163,7 +169,8 @@ static hipError_t stream_step(const WfMulti& mj, const WfState& ws, int which, u
     MH_TIMED(MH_K_SHADE, hipLaunchKernelGGL(shade, G, B, shade_lds(block), st, sc, ws, job, which, 0, rays, mj));
     MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL(shadow, G, B, 0, st, sc, ws, rays, mj));
     if (split)
-        MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL((k_wf_shadow<MODE, true, true>), G, B, 0, st, sc, ws, rays, mj));
+        MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL((k_wf_shadow<MODE, true, true>), list_grid(grid), B, 0, st, sc, ws,
+                                                 rays, mj));
     MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj));
     return hipGetLastError();
 }
