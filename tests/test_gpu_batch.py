@@ -221,7 +221,11 @@ def test_transmissive_split_keeps_the_image(monkeypatch):
     504-532). The image with the queue split (MARMEL_SHADOW_SPLIT=1) equals
     the one with every ray through the closest-hit steps (=0), bit for bit:
     on the forest with its water box, on the ocean world (water triangles)
-    and with a water plane (no finite box: every ray takes the steps)."""
+    and with a water plane (no finite box: every ray takes the steps); and on
+    the spheres scene after mh_scene_set_materials turns it from all opaque
+    to near-clear spheres over an opaque ground (the box must follow the
+    materials: a stale empty box would take every sphere for an opaque
+    occluder)."""
     from test_ingest import ocean_world
     hf = HostScene()
     hf.generate(native.GEN_FOREST, seed=5, grid=6, subdiv=3)
@@ -245,4 +249,30 @@ def test_transmissive_split_keeps_the_image(monkeypatch):
             assert split.mean() > 0
             assert np.array_equal(split, steps), (name, fixed)
         gs.close()
+    # materials changed after upload
+    hs = HostScene()
+    hs.generate(native.GEN_SPHERES, seed=3)
+    hs.build_bvh()
+    prims, _, mats, _, _, _ = hs.arrays()
+    ground = int(prims["material"][prims["kind"] == abi.PRIM_PLANE][0])
+    opaque = mats.copy()
+    opaque["transparency"] = 0.0
+    opaque["is_water"] = 0
+    clear = opaque.copy()
+    clear["transparency"] = 0.9
+    clear["transparency"][ground] = 0.0
+    gs = GpuScene(hs.soa())
+    job = [dict(scene=0, cam=hs.camera(1.0), width=128, height=128, spp=16, depth=6)]
+    means = []
+    for m in (opaque, clear):
+        gs.set_materials(m)
+        monkeypatch.setenv("MARMEL_SHADOW_SPLIT", "1")
+        (_, split), = b.render([gs], job, pool=1 << 16, fixed_batches=True)
+        monkeypatch.setenv("MARMEL_SHADOW_SPLIT", "0")
+        (_, steps), = b.render([gs], job, pool=1 << 16, fixed_batches=True)
+        assert np.array_equal(split, steps)
+        means.append(float(split.mean()))
+    print("spheres: opaque image mean", means[0], "clear spheres", means[1])
+    assert abs(means[1] - means[0]) > 0.01 * means[0]
+    gs.close()
     b.close()
