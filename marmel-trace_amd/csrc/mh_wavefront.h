@@ -513,6 +513,12 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneView<float> sc0, WfState
 #ifndef MH_EXTEND_LINEAR
 #define MH_EXTEND_LINEAR 1
 #endif
+// 1: the 4-wide closest-hit walk tests each child from its near and far
+// planes (kids4_nf) in every launch mode; 0: the per-scene table mode loads
+// whole nodes and tests every plane (children4)
+#ifndef MH_EXTEND_NF
+#define MH_EXTEND_NF 1
+#endif
 #ifndef MH_EXTEND_PRIM_STEP
 #define MH_EXTEND_PRIM_STEP 1
 #endif
@@ -688,10 +694,10 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                     sp = base = psp = 0;
 #endif
                     leaf = 0;
+                    nrx = Slab<float>::near_off(sl.ix);
+                    nry = Slab<float>::near_off(sl.iy);
+                    nrz = Slab<float>::near_off(sl.iz);
                     if (UNI) {
-                        nrx = Slab<float>::near_off(sl.ix);
-                        nry = Slab<float>::near_off(sl.iy);
-                        nrz = Slab<float>::near_off(sl.iz);
                         ref = scu.bvh4 ? scu.root4 : scu.root_ref;
                     } else {
                         nodes = sc.nodes;
@@ -723,11 +729,12 @@ __global__ __launch_bounds__(256) void k_wf_extend_refill(SceneView<float> sc0, 
                 }
                 held = false;
                 if (work) {
-                    if (UNI && MH_EXTEND_W4_SORT && ref >= 0 && scu.bvh4) {
+                    if (MH_EXTEND_W4_SORT && MH_EXTEND_NF && ref >= 0 && (UNI ? scu.bvh4 != 0 : nodes4 != nullptr)) {
                         // children from their near and far planes, nearest
-                        // first (the sorting network of children4)
+                        // first (the sorting network of children4); the
+                        // table mode's node base is per lane
                         if (COUNT) ++cnt_n;
-                        const Kids4 kd = kids4_nf(sl, scu.nodes4, ref, nrx, nry, nrz, tmin, closest);
+                        const Kids4 kd = kids4_nf(sl, UNI ? scu.nodes4 : nodes4, ref, nrx, nry, nrz, tmin, closest);
                         const float inf = __builtin_inff();
                         float kk[4] = {kd.h[0] ? kd.t[0] : inf, kd.h[1] ? kd.t[1] : inf, kd.h[2] ? kd.t[2] : inf,
                                        kd.h[3] ? kd.t[3] : inf};
