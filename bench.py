@@ -457,6 +457,8 @@ def main():
                     help="skip the adaptive Whitted + flare leg (steps after the framebuffer)")
     ap.add_argument("--only-post", action="store_true",
                     help="run only the adaptive Whitted + flare leg (for profiling its kernels)")
+    ap.add_argument("--only-batch32", action="store_true",
+                    help="run only the 32-scene batched leg (for profiling its kernels)")
     ap.add_argument("--only-stress", action="store_true",
                     help="run only the stress leg, with the launch settings the terrain frame is tuned to "
                          "(for profiling its kernels; --stress-seconds bounds it)")
@@ -475,6 +477,9 @@ def main():
 
     if args.only_stress:
         log(f"bench: stress {stress_leg(STRESS_TUNE, args.stress_seconds)}")
+        return
+    if args.only_batch32:
+        log(f"bench: batch32 {batch_leg(args.batch_reps)}")
         return
     t0 = time.time()
     hs, info, st = build_scene(args.grid)
