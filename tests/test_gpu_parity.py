@@ -1,7 +1,7 @@
 """GPU parity tests: libmarmel_hip through its C ABI against the oracle and the
 reference-generated golden fixtures.
 
-Bars (BASELINE north star):
+Bars:
   * nearest-hit prim ids equal to the CPU BVH except where hit distances tie
     within 1e-5 relative (fp64: also equal t),
   * fixed-spp Whitted (--no-pathtrace --no-adaptive) within 1/255 per 8-bit
