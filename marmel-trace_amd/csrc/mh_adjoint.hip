@@ -27,8 +27,11 @@
 // box hits at o + t d in fp32 here (see mh_trace.h): the adjoint kernels sit at
 // the register ceiling, where the snap halved the two-pass kernel's occupancy;
 // for the same reason the fp32 slab setup runs without its tiny-component guard
+// and the closest-hit walk loads whole 4-wide nodes (the near / far variant
+// takes the recorded adjoint from 2 waves per SIMD to 1)
 #define MH_SNAP_BOX 0
 #define MH_SLAB_GUARD 0
+#define MH_TRACE_NF 0
 #include "mh_kernels.h"
 #include "mh_render.h"
 

@@ -801,6 +801,65 @@ __device__ __forceinline__ void children4(const Slab<R>& sl, const DNode4& n, R 
     *nh = c;
 }
 
+// 1: the fp32 closest-hit walk tests each child of a 4-wide node from its
+// near and far planes only (children4_nf); 0: whole nodes (children4).
+#ifndef MH_TRACE_NF
+#define MH_TRACE_NF 1
+#endif
+
+// children4 from the near and far planes of node `ref` (the rows picked by
+// address from the ray's direction signs, near-row offsets from
+// Slab::near_off): 112 of the node's 128 bytes, two planes per axis.
+__device__ __forceinline__ void children4_nf(const Slab<float>& sl, const DNode4* nodes4, int ref, uint32_t nrx,
+                                             uint32_t nry, uint32_t nrz, float tmin, float tmax, int r[4],
+                                             float t[4], int* nh)
+{
+    const char* nb = (const char*)nodes4;
+    // node arrays stay below 4 GiB: 32-bit byte offsets
+    const uint32_t off = (uint32_t)ref << 7;
+    const uint32_t vx = off | nrx, vy = off | nry, vz = off | nrz;
+    const float4 nx4 = ldg((const float4*)(nb + (size_t)vx));
+    const float4 fx4 = ldg((const float4*)(nb + (size_t)(vx ^ 16u)));
+    const float4 ny4 = ldg((const float4*)(nb + (size_t)vy + 32));
+    const float4 fy4 = ldg((const float4*)(nb + (size_t)(vy ^ 16u) + 32));
+    const float4 nz4 = ldg((const float4*)(nb + (size_t)vz + 64));
+    const float4 fz4 = ldg((const float4*)(nb + (size_t)(vz ^ 16u) + 64));
+    const int4 rf = ldg((const int4*)(nb + (size_t)off + 96));
+    const float inf = __builtin_inff();
+    float k[4];
+    const bool h0 = sl.nf(nx4.x, fx4.x, ny4.x, fy4.x, nz4.x, fz4.x, tmin, tmax, &k[0]);
+    const bool h1 = sl.nf(nx4.y, fx4.y, ny4.y, fy4.y, nz4.y, fz4.y, tmin, tmax, &k[1]);
+    const bool h2 = sl.nf(nx4.z, fx4.z, ny4.z, fy4.z, nz4.z, fz4.z, tmin, tmax, &k[2]) && rf.z != INT_MIN;
+    const bool h3 = sl.nf(nx4.w, fx4.w, ny4.w, fy4.w, nz4.w, fz4.w, tmin, tmax, &k[3]) && rf.w != INT_MIN;
+    if (!h0) k[0] = inf;
+    if (!h1) k[1] = inf;
+    if (!h2) k[2] = inf;
+    if (!h3) k[3] = inf;
+    int q[4] = {rf.x, rf.y, rf.z, rf.w};
+#define MH_CSWAP(a, b)                                                                                    \
+    if (k[b] < k[a]) {                                                                                    \
+        float tk = k[a];                                                                                  \
+        k[a] = k[b];                                                                                      \
+        k[b] = tk;                                                                                        \
+        int tq = q[a];                                                                                    \
+        q[a] = q[b];                                                                                      \
+        q[b] = tq;                                                                                        \
+    }
+    MH_CSWAP(0, 1)
+    MH_CSWAP(2, 3)
+    MH_CSWAP(0, 2)
+    MH_CSWAP(1, 3)
+    MH_CSWAP(1, 2)
+#undef MH_CSWAP
+    int c = 0;
+    for (int i = 0; i < 4; ++i) {
+        r[i] = q[i];
+        t[i] = k[i];
+        c += k[i] < inf ? 1 : 0;
+    }
+    *nh = c;
+}
+
 // Closest hit. Returns the leaf-record index of the winner (or -1 - plane
 // slot for a plane, or INT_MIN for a miss) and its t / prim index.
 // fp32: the nearest hit child is visited next straight from a register, the
@@ -837,6 +896,12 @@ __device__ int closest_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tm
             st.push(sc.root4, (float)tmin);
             int ref = 0;
             bool held = false;  // the nearest hit child, visited next without a stack round trip
+            uint32_t nrx = 0, nry = 0, nrz = 0;
+            if constexpr (sizeof(R) == 4 && MH_TRACE_NF) {
+                nrx = Slab<float>::near_off(sl.ix);
+                nry = Slab<float>::near_off(sl.iy);
+                nrz = Slab<float>::near_off(sl.iz);
+            }
             for (;;) {
                 if (!held) {
                     if (st.empty()) break;
@@ -846,10 +911,14 @@ __device__ int closest_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tm
                 }
                 held = false;
                 if (ref >= 0) {
-                    DNode4 n = ldg(&sc.nodes4[ref]);
                     int r[4], nh;
                     float t[4];
-                    children4(sl, n, tmin, closest, r, t, &nh);
+                    if constexpr (sizeof(R) == 4 && MH_TRACE_NF) {
+                        children4_nf(sl, sc.nodes4, ref, nrx, nry, nrz, tmin, closest, r, t, &nh);
+                    } else {
+                        DNode4 n = ldg(&sc.nodes4[ref]);
+                        children4(sl, n, tmin, closest, r, t, &nh);
+                    }
                     // push the farthest first; the nearest stays in a register
                     for (int i = 3; i >= 1; --i)
                         if (i < nh) st.push(r[i], t[i]);
