@@ -801,8 +801,9 @@ __device__ __forceinline__ void children4(const Slab<R>& sl, const DNode4& n, R 
     *nh = c;
 }
 
-// 1: the fp32 closest-hit walk tests each child of a 4-wide node from its
-// near and far planes only (children4_nf); 0: whole nodes (children4).
+// 1: the fp32 closest-hit and any-hit walks test each child of a 4-wide node
+// from its near and far planes only (children4_nf); 0: whole nodes
+// (children4).
 #ifndef MH_TRACE_NF
 #define MH_TRACE_NF 1
 #endif
@@ -1103,19 +1104,30 @@ __device__ bool any_hit(const SceneView<R>& sc, TS ts, V3<R> o, V3<R> d, R tmin,
     sl.init(o, d);
     typename TS::template Lane<false> st(ts);
     if (W != 2 && sc.bvh4) {
+        uint32_t nrx = 0, nry = 0, nrz = 0;
+        if constexpr (sizeof(R) == 4 && MH_TRACE_NF && MH_ANY_SORT) {
+            nrx = Slab<float>::near_off(sl.ix);
+            nry = Slab<float>::near_off(sl.iy);
+            nrz = Slab<float>::near_off(sl.iz);
+        }
         st.push(sc.root4, 0.f);
         while (!st.empty()) {
             float te;
             int ref = st.pop(&te);
             if (ref >= 0) {
-                DNode4 n = ldg(&sc.nodes4[ref]);
 #if MH_ANY_SORT
                 int r[4], nh;
                 float t[4];
-                children4(sl, n, tmin, tmax, r, t, &nh);
+                if constexpr (sizeof(R) == 4 && MH_TRACE_NF) {
+                    children4_nf(sl, sc.nodes4, ref, nrx, nry, nrz, tmin, tmax, r, t, &nh);
+                } else {
+                    DNode4 n = ldg(&sc.nodes4[ref]);
+                    children4(sl, n, tmin, tmax, r, t, &nh);
+                }
                 for (int i = 3; i >= 0; --i)
                     if (i < nh) st.push(r[i], 0.f);
 #else
+                DNode4 n = ldg(&sc.nodes4[ref]);
                 // any hit ends the query: visit order only affects speed
                 bool h0, h1, h2, h3;
                 float t0, t1, t2, t3;
