@@ -42,12 +42,12 @@ sys.path.insert(0, os.path.join(REPO, "marmel-trace_amd"))
 HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per k_wf_shadow_refill launch of the batched engine
 # (the default main engine) on this workload: rocprofv3 --pmc FETCH_SIZE
-# (6,077,929.4 KiB, x 2.0) + WRITE_SIZE (425,185.2 KiB, x 1.0) per dispatch,
+# (6,086,904.7 KiB, x 2.0) + WRITE_SIZE (425,176.0 KiB, x 1.0) per dispatch,
 # averaged over the clocked frame's 18 dispatches (pool 2^26 paths, 16384
 # workgroups), the factors being what the same counters read for a 1 GiB
 # copy (bench.py --hbm-calib 1024). Re-measure when the shadow kernel or its
 # launch changes.
-SHADOW_TRAFFIC_PER_LAUNCH = 12883.0e6
+SHADOW_TRAFFIC_PER_LAUNCH = 12901.4e6
 # What the same kernel's counters show limits it (8.01 ms per launch): VALU
 # instructions x 4 cycles fill 63 % of the SIMDs' issue cycles at 2.35 GHz
 # (SQ_INSTS_VALU 3.03e9, SQ_BUSY_CYCLES) with 56 % of the lanes active
