@@ -1251,8 +1251,14 @@ int mh_render(mh_scene* s, const mh_camera* cam, const mh_render_params* p, unsi
     if (p->mode == MH_MODE_PATHTRACE && p->engine == MH_ENGINE_WAVEFRONT) {
         if (f64) return fail(5, "the wavefront engine computes in fp32 only");
         int npix = p->width * p->height;
-        int pool = p->pool > 0 ? p->pool : (1 << 21);
-        if (pool > npix) pool = npix; // one sample per pixel per batch
+        // a batch holds at most one sample of each pixel, or whole samples
+        // (pool a multiple of npix, summed in sample order at the batch end:
+        // the same sums); the default keeps a scene's path state near 5 GB
+        long long want = p->pool > 0 ? p->pool : std::min<long long>(1LL << 23, 4LL * npix);
+        const long long all = (long long)npix * p->spp;
+        if (want > all) want = all;
+        if (want > (1LL << 26)) want = 1LL << 26;  // the shadow tags address 2^26 slots
+        int pool = want >= 2LL * npix ? (int)(want / npix * npix) : (int)(want < npix ? want : npix);
         rc = ensure_wf(s, pool, (size_t)npix);
         if (rc) return rc;
         int grid = p->grid_blocks > 0 ? p->grid_blocks : 4096;  // 1080p terrain: 2048 -> 9 % slower, 8192 -> 0.5 %

@@ -102,7 +102,8 @@ struct WfJob {
     int w, h, spp, s0, depth;
     int npix;
     long long total;  // npix * spp
-    int pool;         // batch size (<= npix)
+    int pool;         // batch size: at most npix (one sample of each pixel), or a multiple of npix
+                      // (whole samples, added to the pixel sums in sample order at the batch end)
 };
 
 // Bytes of device state per path slot (state + pending + queues + shadow rays).

@@ -1589,8 +1589,10 @@ __device__ __forceinline__ unsigned long long to_fixed(float v)
     return __float2ull_rn(fminf(v, 67108864.0f) * 16777216.0f);
 }
 
+// defer (single-scene engine, batches of whole samples): each path's
+// radiance stays in its slot for k_wf_single_sum.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_wf_finalize(WfState st, WfMulti mj)
+__global__ __launch_bounds__(256) void k_wf_finalize(WfState st, WfMulti mj, int defer)
 {
     constexpr bool MULTI = MODE != WF_SINGLE;
     uint32_t n = st.cnt[3];
@@ -1610,9 +1612,10 @@ __global__ __launch_bounds__(256) void k_wf_finalize(WfState st, WfMulti mj)
             L.y = L.y < 0.f ? 0.f : L.y;
             L.z = L.z < 0.f ? 0.f : L.z;
             uint32_t pix = f2u(c.w);
-            if (MULTI && !mj.stream) {
+            if ((MULTI && !mj.stream) || (!MULTI && defer)) {
                 // fixed batches: the slot keeps the path's radiance until
-                // k_wf_batch_sum adds the batch's samples of each pixel
+                // k_wf_batch_sum / k_wf_single_sum adds the batch's samples
+                // of each pixel
                 st.p3[slot] = make_float4(L.x, L.y, L.z, e.w);
             } else if (MULTI) {
                 size_t p = ((size_t)mj.jobs[st.jid[slot]].acc_off + pix) * 3;
@@ -1670,6 +1673,32 @@ __global__ __launch_bounds__(256) void k_wf_batch_sum(WfState st, WfMulti mj, in
             mj.acc64[p + 1] += f1;
             mj.acc64[p + 2] += f2;
         }
+    }
+}
+
+// Single-scene engine, a batch of k whole samples (pool = k * npix): each
+// pixel adds its k samples' radiance to its fp32 sum in sample order (slot
+// j * npix + t holds sample j of tile-order pixel t), the same additions in
+// the same order as k batches of one sample each.
+__global__ __launch_bounds__(256) void k_wf_single_sum(WfState st, WfJob job)
+{
+    const long long g0 = (long long)st.batch[0];
+    const long long remain = job.total - g0;
+    const long long n = remain < (long long)job.pool ? remain : (long long)job.pool;
+    const int k = (int)(n / job.npix);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < job.npix; t += gridDim.x * blockDim.x) {
+        if (k <= 0) break;
+        const uint32_t pix = f2u(st.p2[t].w);
+        float sx = st.acc[(size_t)pix * 3], sy = st.acc[(size_t)pix * 3 + 1], sz = st.acc[(size_t)pix * 3 + 2];
+        for (int j = 0; j < k; ++j) {
+            const float4 e = st.p3[(size_t)j * (size_t)job.npix + (size_t)t];
+            sx += e.x;
+            sy += e.y;
+            sz += e.z;
+        }
+        st.acc[(size_t)pix * 3] = sx;
+        st.acc[(size_t)pix * 3 + 1] = sy;
+        st.acc[(size_t)pix * 3 + 2] = sz;
     }
 }
 

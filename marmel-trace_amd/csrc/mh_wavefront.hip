@@ -92,6 +92,8 @@ static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<floa
     const bool split = split_trans(feat);
     auto shadow = (feat & WF_TRANS) ? shadow_trans<MODE>(feat) : shadow_any<MODE>(feat);
     auto shade = (feat & WF_FOG) ? k_wf_shade<MODE, true> : k_wf_shade<MODE, false>;
+    // single-scene batches of several whole samples sum at the batch end
+    const int defer = MODE == WF_SINGLE && job.pool > job.npix ? 1 : 0;
     dim3 G(grid), B(block);
     MH_TIMED(MH_K_GEN, hipLaunchKernelGGL(k_wf_gen<MODE>, G, B, 0, st, sc, cam, job, ws, mj));
     for (int b = 0; b <= job.depth; ++b) {
@@ -103,12 +105,15 @@ static hipError_t enqueue_bounces(const SceneView<float>& sc, const CamView<floa
         if (split)
             MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL((k_wf_shadow<MODE, true, true>), list_grid(grid), B, 0, st, sc, ws,
                                                      rays, mj));
-        MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj));
+        MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj, defer));
     }
-    // batched engine, fixed batches: the batch's pixel sums (k_wf_finalize
-    // left each path's radiance in its slot)
+    // batched engine, fixed batches / single-scene batches of whole samples:
+    // the batch's pixel sums (k_wf_finalize left each path's radiance in its
+    // slot)
     if (MODE != WF_SINGLE)
         MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_batch_sum, G, B, 0, st, ws, mj, job.pool));
+    else if (defer)
+        MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_single_sum, G, B, 0, st, ws, job));
     MH_TIMED(MH_K_OTHER, hipLaunchKernelGGL(k_wf_next_batch, dim3(1), dim3(64), 0, st, ws, job.pool));
     return hipGetLastError();
 }
@@ -171,7 +176,7 @@ static hipError_t stream_step(const WfMulti& mj, const WfState& ws, int which, u
     if (split)
         MH_TIMED(MH_K_SHADOW, hipLaunchKernelGGL((k_wf_shadow<MODE, true, true>), list_grid(grid), B, 0, st, sc, ws,
                                                  rays, mj));
-    MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj));
+    MH_TIMED(MH_K_FINALIZE, hipLaunchKernelGGL(k_wf_finalize<MODE>, G, B, 0, st, ws, mj, 0));
     return hipGetLastError();
 }
 

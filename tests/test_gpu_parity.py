@@ -296,6 +296,27 @@ def test_wavefront_matches_megakernel(name):
     gs.close()
 
 
+def test_wavefront_sums_do_not_depend_on_the_batch_size():
+    """The single-scene wavefront engine sums each pixel's samples in fp32 in
+    sample order whether a batch holds part of one sample of every pixel,
+    exactly one, or several whole samples (their radiance added in sample
+    order at the batch end): the same bits for every pool, with and without
+    graph replay, also when the last batch holds fewer samples."""
+    hs, gs, o = host_scene("terrain")
+    w, h, spp = 96, 64, 11
+    cam = hs.camera(w / h)
+    npix = w * h
+    ref = None
+    for pool, graph in ((1000, False), (npix, False), (2 * npix, False), (4 * npix, True), (5 * npix + 7, False),
+                        (0, True)):
+        _, lin = gs.render(cam, w, h, spp, 8, precision=FP32, engine=WF, pool=pool, grid_blocks=64,
+                           use_graph=graph)
+        if ref is None:
+            ref = lin
+        assert np.array_equal(lin, ref), (pool, graph)
+    gs.close()
+
+
 def test_wavefront_matches_oracle_statistics():
     hs, gs, o = host_scene("terrain")
     w, h = 48, 36
