@@ -272,7 +272,7 @@ typedef struct mh_render_params {
     int32_t precision;      /* MH_PREC_*  */
     int32_t sample_offset;  /* first sample index (progressive chunks) */
     int32_t engine;         /* MH_ENGINE_* (path tracing only)          */
-    int32_t pool;           /* wavefront: paths per batch (0 = min(2^23, 4*w*h)); at most
+    int32_t pool;           /* wavefront: paths per batch (0 = min(2^24, 8*w*h)); at most
                                w*h*spp and 2^26; from 2*w*h up, rounded down to whole
                                samples of every pixel, else capped at w*h            */
     int32_t grid_blocks;    /* wavefront: persistent blocks of 256 (0 = default)       */
